@@ -1,0 +1,185 @@
+// capi.cpp -- extern "C" entry points of include/gvdb_amd.h over gvdb::Volume.
+#include "volume.h"
+#include "../../include/gvdb_amd.h"
+#include <new>
+#include <cstring>
+
+struct gvdb_volume {
+    gvdb::Volume v;
+    explicit gvdb_volume(int d) : v(d) {}
+};
+
+static_assert(sizeof(gvdb_node) == sizeof(gvdb::Node), "gvdb_node mirrors the 64-byte node");
+
+#define H_OR_FAIL(h)                     \
+    if (!(h)) return GVDB_ERR_INVALID;   \
+    (void)hipSetDevice((h)->v.device)
+
+extern "C" {
+
+int gvdb_create(gvdb_handle* out, int device) {
+    if (!out) return GVDB_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return GVDB_ERR_HIP;
+    gvdb_volume* h = new (std::nothrow) gvdb_volume(device);
+    if (!h) return GVDB_ERR_OOM;
+    int rc = h->v.init();
+    if (rc) { delete h; return rc; }
+    *out = h;
+    return GVDB_OK;
+}
+
+int gvdb_destroy(gvdb_handle h) {
+    if (!h) return GVDB_ERR_INVALID;
+    delete h;
+    return GVDB_OK;
+}
+
+const char* gvdb_last_error(gvdb_handle h) { return h ? h->v.err.c_str() : "null handle"; }
+
+int gvdb_configure(gvdb_handle h, int levels, const int* log2dims, const int* initcnt) {
+    H_OR_FAIL(h);
+    if (!log2dims) return GVDB_ERR_INVALID;
+    return h->v.configure(levels, log2dims, initcnt);
+}
+int gvdb_set_channel_default(gvdb_handle h, int cx, int cy, int cz) { H_OR_FAIL(h); return h->v.set_channel_default(cx, cy, cz); }
+int gvdb_add_channel(gvdb_handle h, int chan, int dtype, int apron, int filter, int border, int ax, int ay, int az) {
+    H_OR_FAIL(h);
+    return h->v.add_channel(chan, dtype, apron, filter, border, ax, ay, az);
+}
+int gvdb_set_epsilon(gvdb_handle h, float eps, int maxiter) { H_OR_FAIL(h); return h->v.set_epsilon(eps, maxiter); }
+int gvdb_set_points_gpu(gvdb_handle h, const void* dptr, int n, int offset, int stride) {
+    H_OR_FAIL(h);
+    return h->v.set_points_gpu(dptr, n, offset, stride);
+}
+int gvdb_upload_points(gvdb_handle h, const float* host_xyz, int n) { H_OR_FAIL(h); return h->v.upload_points(host_xyz, n); }
+int gvdb_request_full_rebuild(gvdb_handle h, int on) { H_OR_FAIL(h); return h->v.request_full_rebuild(on != 0); }
+int gvdb_rebuild_topology(gvdb_handle h, int n, float radius, const float orig[3]) {
+    H_OR_FAIL(h);
+    const float z[3] = {0, 0, 0};
+    return h->v.rebuild_topology(n, radius, orig ? orig : z);
+}
+int gvdb_rebuild_topology_cpu(gvdb_handle h, int n, const float orig[3], const float* host_xyz) {
+    H_OR_FAIL(h);
+    const float z[3] = {0, 0, 0};
+    return h->v.rebuild_topology_cpu(n, orig ? orig : z, host_xyz);
+}
+int gvdb_activate_space(gvdb_handle h, const float pos[3]) {
+    H_OR_FAIL(h);
+    if (!pos) return GVDB_ERR_INVALID;
+    return h->v.activate_space(pos);
+}
+int gvdb_clear(gvdb_handle h) { H_OR_FAIL(h); return h->v.clear(); }
+int gvdb_finish_topology(gvdb_handle h, int commit_pool, int compute_bounds) {
+    H_OR_FAIL(h);
+    return h->v.finish_topology(commit_pool != 0, compute_bounds != 0);
+}
+int gvdb_update_atlas(gvdb_handle h) { H_OR_FAIL(h); return h->v.update_atlas(); }
+int gvdb_clear_channel(gvdb_handle h, int chan) { H_OR_FAIL(h); return h->v.clear_channel(chan); }
+int gvdb_insert_points_subcell(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int* out_len) {
+    H_OR_FAIL(h);
+    if (!out_len) return GVDB_ERR_INVALID;
+    const float z[3] = {0, 0, 0};
+    return h->v.insert_points_subcell(subcell_size, n, radius, trans ? trans : z, out_len);
+}
+int gvdb_gather_density(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len,
+                        int chan_density, int chan_clr, int accumulate) {
+    H_OR_FAIL(h);
+    const float z[3] = {0, 0, 0};
+    return h->v.gather_density(subcell_size, n, radius, trans ? trans : z, sc_len, chan_density, chan_clr, accumulate != 0);
+}
+int gvdb_update_apron(gvdb_handle h, int chan, float boundval) { H_OR_FAIL(h); return h->v.update_apron(chan, boundval); }
+
+int gvdb_set_camera_orbit(gvdb_handle h, const float angs[3], const float to[3], float dist, float dolly, float fov,
+                          float znear, float zfar) {
+    H_OR_FAIL(h);
+    if (!angs || !to || !(znear > 0) || !(zfar > znear) || !(fov > 0 && fov < 180)) return GVDB_ERR_INVALID;
+    for (int a = 0; a < 3; a++) { h->v.cam.angs[a] = angs[a]; h->v.cam.to[a] = to[a]; }
+    h->v.cam.dist = dist; h->v.cam.dolly = dolly; h->v.cam.fov = fov; h->v.cam.znear = znear; h->v.cam.zfar = zfar;
+    return GVDB_OK;
+}
+int gvdb_set_light(gvdb_handle h, const float pos[3]) {
+    H_OR_FAIL(h);
+    if (!pos) return GVDB_ERR_INVALID;
+    for (int a = 0; a < 3; a++) h->v.light_[a] = pos[a];
+    return GVDB_OK;
+}
+int gvdb_set_background(gvdb_handle h, const float rgba[4]) {
+    H_OR_FAIL(h);
+    if (!rgba) return GVDB_ERR_INVALID;
+    for (int a = 0; a < 4; a++) h->v.back_[a] = rgba[a];
+    return GVDB_OK;
+}
+static int set3(float* d, float a, float b, float c) { d[0] = a; d[1] = b; d[2] = c; return GVDB_OK; }
+int gvdb_set_extinct(gvdb_handle h, float a, float b, float c) { H_OR_FAIL(h); return set3(h->v.extinct_, a, b, c); }
+int gvdb_set_steps(gvdb_handle h, float a, float b, float c) {
+    H_OR_FAIL(h);
+    if (!(a > 0)) return GVDB_ERR_INVALID;
+    return set3(h->v.steps_, a, b, c);
+}
+int gvdb_set_cutoff(gvdb_handle h, float a, float b, float c) { H_OR_FAIL(h); return set3(h->v.cutoff_, a, b, c); }
+int gvdb_set_volume_range(gvdb_handle h, float a, float b, float c) { H_OR_FAIL(h); return set3(h->v.vrange_, a, b, c); }
+int gvdb_set_shadow_params(gvdb_handle h, float a, float b, float c) { H_OR_FAIL(h); return set3(h->v.shadow_, a, b, c); }
+int gvdb_linear_transfer_func(gvdb_handle h, float t0, float t1, const float a[4], const float b[4]) {
+    H_OR_FAIL(h);
+    if (!a || !b || !(t1 > t0)) return GVDB_ERR_INVALID;
+    return h->v.linear_tf(t0, t1, a, b);
+}
+int gvdb_commit_transfer_func(gvdb_handle h) { H_OR_FAIL(h); return h->v.commit_tf(); }
+int gvdb_add_render_buf(gvdb_handle h, int buf, int w, int hgt, int bpp) { H_OR_FAIL(h); return h->v.add_render_buf(buf, w, hgt, bpp); }
+int gvdb_render(gvdb_handle h, int shading, int chan, int buf) { H_OR_FAIL(h); return h->v.render(shading, chan, buf); }
+int gvdb_read_render_buf(gvdb_handle h, int buf, void* dst) { H_OR_FAIL(h); return h->v.read_render_buf(buf, dst); }
+int gvdb_get_scene_block(gvdb_handle h, int w, int hgt, float* out68) {
+    H_OR_FAIL(h);
+    if (!out68 || w <= 0 || hgt <= 0) return GVDB_ERR_INVALID;
+    gvdb::ScnBlock s;
+    h->v.scene_block(w, hgt, s);
+    memcpy(out68, &s, sizeof(s));
+    return GVDB_OK;
+}
+int gvdb_get_pool(gvdb_handle h, int grp, int lev, void* dst, uint64_t* count, uint64_t* stride) {
+    H_OR_FAIL(h);
+    gvdb::u64 c = 0, s = 0;
+    int rc = h->v.get_pool(grp, lev, dst, &c, &s);
+    if (count) *count = c;
+    if (stride) *stride = s;
+    return rc;
+}
+int gvdb_get_info(gvdb_handle h, int* out2, float* outf12) { H_OR_FAIL(h); return h->v.get_info(out2, outf12); }
+int gvdb_get_atlas_info(gvdb_handle h, int chan, int* out7) {
+    H_OR_FAIL(h);
+    if (!out7) return GVDB_ERR_INVALID;
+    return h->v.get_atlas_info(chan, out7);
+}
+int gvdb_read_atlas(gvdb_handle h, int chan, void* dst) { H_OR_FAIL(h); return h->v.read_atlas(chan, dst); }
+int gvdb_read_atlas_map(gvdb_handle h, void* dst) { H_OR_FAIL(h); return h->v.read_atlas_map(dst); }
+int gvdb_read_subcells(gvdb_handle h, int which, void* dst, uint64_t* count) {
+    H_OR_FAIL(h);
+    gvdb::u64 c = 0;
+    int rc = h->v.read_subcells(which, dst, &c);
+    if (count) *count = c;
+    return rc;
+}
+int gvdb_get_stream(gvdb_handle h, void** stream) {
+    H_OR_FAIL(h);
+    if (!stream) return GVDB_ERR_INVALID;
+    *stream = h->v.stream;
+    return GVDB_OK;
+}
+int gvdb_synchronize(gvdb_handle h) {
+    H_OR_FAIL(h);
+    return hipStreamSynchronize(h->v.stream) == hipSuccess ? GVDB_OK : GVDB_ERR_HIP;
+}
+int gvdb_stage_ms(gvdb_handle h, float* out8) {
+    H_OR_FAIL(h);
+    if (!out8) return GVDB_ERR_INVALID;
+    return h->v.stage_ms(out8);
+}
+int gvdb_kernel_ms(gvdb_handle h, float* out4) {
+    H_OR_FAIL(h);
+    if (!out4) return GVDB_ERR_INVALID;
+    return h->v.kernel_ms(out4);
+}
+
+}  // extern "C"

@@ -1,0 +1,330 @@
+// points.hip -- atlas slot assignment, subcell binning and the density gather.
+#include "kernels.h"
+#include "tree_device.h"
+#include <hip/hip_fp16.h>
+
+namespace gvdb {
+
+__device__ inline const float* point_at(const PointSrc& p, int i) {
+    return reinterpret_cast<const float*>(p.base + u64(i) * p.stride + p.off);
+}
+
+// ------------------------------------------------------------ UpdateAtlas
+// Slot per flagged leaf in pool order, mValue = slot corner + apron, and the
+// atlas map entry (src/gvdb_volume_gvdb.cpp:2630-2704, getAtlasPos
+// src/gvdb_allocator.cpp:705-715).  The map is cleared to -1 beforehand.
+__global__ __launch_bounds__(256) void k_update_atlas(Tree t, const uint32_t* slot_scan, int nleaves) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nleaves) return;
+    Node* n = t.nodes[0] + i;
+    if (!n->mFlags) return;
+    u64 id = slot_scan[i];
+    u64 a2 = u64(t.atlas_cnt[0]) * t.atlas_cnt[1];
+    int pz = int(id / a2);
+    id -= u64(pz) * a2;
+    int py = int(id / t.atlas_cnt[0]);
+    id -= u64(py) * t.atlas_cnt[0];
+    int px = int(id);
+    int br = t.brick_res;
+    n->mValue[0] = px * br + t.apron;
+    n->mValue[1] = py * br + t.apron;
+    n->mValue[2] = pz * br + t.apron;
+    u64 slot = (u64(pz) * t.atlas_cnt[1] + py) * t.atlas_cnt[0] + px;
+    AtlasNode an;
+    an.mPos[0] = n->mPos[0]; an.mPos[1] = n->mPos[1]; an.mPos[2] = n->mPos[2];
+    an.mLeafNode = i;
+    t.amap[slot] = an;
+}
+
+hipError_t launch_update_atlas(Tree t, const uint32_t* slot_scan, int nleaves, hipStream_t s) {
+    if (nleaves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_update_atlas, dim3((nleaves + 255) / 256), dim3(256), 0, s, t, slot_scan, nleaves);
+    return hipGetLastError();
+}
+
+// flags[i] = 1 for leaves in use: mode 0 tests mFlags (atlas assignment),
+// mode 1 tests mParent != ID_UNDEF64 (gvdbSetFlagSubcell,
+// kernels/cuda_gvdb_particles.cuh:109-118).
+__global__ __launch_bounds__(256) void k_leaf_flags(Tree t, uint32_t* flags, int nleaves, int mode) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nleaves) return;
+    const Node* n = t.nodes[0] + i;
+    flags[i] = mode == 0 ? (n->mFlags ? 1u : 0u) : (n->mParent == ID_UNDEF64 ? 0u : 1u);
+}
+
+hipError_t launch_leaf_flags(Tree t, uint32_t* flags, int nleaves, hipStream_t s) {
+    if (nleaves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_leaf_flags, dim3((nleaves + 255) / 256), dim3(256), 0, s, t, flags, nleaves, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_leaf_used(Tree t, uint32_t* flags, int nleaves, hipStream_t s) {
+    if (nleaves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_leaf_flags, dim3((nleaves + 255) / 256), dim3(256), 0, s, t, flags, nleaves, 0);
+    return hipGetLastError();
+}
+
+// -------------------------------------------------------- subcell binning
+// Each point is binned into every subcell overlapping
+// [int(p-r)/sc*sc, int(p+1+r)/sc*sc] per axis whose brick exists
+// (gvdbCountSubcell / gvdbInsertSubcell, kernels/cuda_gvdb_particles.cuh:218-315).
+// The last leaf found is reused while the subcell corner stays inside it.
+template <bool INSERT>
+__global__ __launch_bounds__(256) void k_bin_subcell(Tree t, PointSrc pts, int scs, int spb, float tx, float ty, float tz,
+                                                     float radius, const uint32_t* mapping, uint32_t* sc_cnt,
+                                                     const uint32_t* sc_off, float* sc_pnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pts.n) return;
+    const float* p = point_at(pts, i);
+    float p0 = p[0], p1 = p[1], p2 = p[2];
+    if (p2 == NOHIT) return;
+    float w[3] = {p0 + tx, p1 + ty, p2 + tz};
+    if (!isfinite(w[0]) || !isfinite(w[1]) || !isfinite(w[2])) return;
+    if (w[0] < 0 || w[1] < 0 || w[2] < 0) return;
+    // The reference's insert pass tests p < 0 before adding trans while its
+    // count pass tests p + trans < 0; both tests apply here so that the insert
+    // pass never places more entries in a subcell than the count pass counted.
+    if (INSERT && (p0 < 0 || p1 < 0 || p2 < 0)) return;
+    int scmin[3], scmax[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        scmin[a] = (int(w[a] - radius) / scs) * scs;
+        scmax[a] = (int(w[a] + 1 + radius) / scs) * scs;
+    }
+    const int res0 = t.res[0];
+    const int scdim = res0 / scs;
+    int cache_leaf = -1, cx = 0, cy = 0, cz = 0;
+    for (int sx = scmin[0]; sx <= scmax[0]; sx += scs)
+        for (int sy = scmin[1]; sy <= scmax[1]; sy += scs)
+            for (int sz = scmin[2]; sz <= scmax[2]; sz += scs) {
+                int leaf;
+                if (cache_leaf >= 0 && sx >= cx && sy >= cy && sz >= cz && sx < cx + res0 && sy < cy + res0 && sz < cz + res0) {
+                    leaf = cache_leaf;
+                } else {
+                    leaf = pos_leaf(t, sx, sy, sz);
+                    if (leaf < 0) continue;
+                    const Node* ln = t.nodes[0] + leaf;
+                    cache_leaf = leaf;
+                    cx = ln->mPos[0]; cy = ln->mPos[1]; cz = ln->mPos[2];
+                }
+                int lx = (sx - cx) / scs, ly = (sy - cy) / scs, lz = (sz - cz) / scs;
+                u64 idx = u64(mapping[leaf]) * spb + u64((lz * scdim + ly) * scdim + lx);
+                uint32_t slot = atomicAdd(&sc_cnt[idx], 1u);
+                if (INSERT) {
+                    u64 e = u64(sc_off[idx]) + slot;
+                    sc_pnt[3 * e] = w[0];
+                    sc_pnt[3 * e + 1] = w[1];
+                    sc_pnt[3 * e + 2] = w[2];
+                }
+            }
+}
+
+hipError_t launch_count_subcell(Tree t, PointSrc pts, int scs, int spb, const float trans[3], float radius,
+                                const uint32_t* mapping, uint32_t* sc_cnt, hipStream_t s) {
+    if (pts.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bin_subcell<false>, dim3((pts.n + 255) / 256), dim3(256), 0, s, t, pts, scs, spb, trans[0],
+                       trans[1], trans[2], radius, mapping, sc_cnt, (const uint32_t*)nullptr, (float*)nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_insert_subcell(Tree t, PointSrc pts, int scs, int spb, const float trans[3], float radius,
+                                 const uint32_t* mapping, uint32_t* sc_cnt, const uint32_t* sc_off, float* sc_pnt,
+                                 hipStream_t s) {
+    if (pts.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bin_subcell<true>, dim3((pts.n + 255) / 256), dim3(256), 0, s, t, pts, scs, spb, trans[0],
+                       trans[1], trans[2], radius, mapping, sc_cnt, sc_off, sc_pnt);
+    return hipGetLastError();
+}
+
+// gvdbCalcSubcellPos: kernels/cuda_gvdb_particles.cuh:677-694
+__global__ __launch_bounds__(256) void k_subcell_pos(Tree t, int scs, int spb, const uint32_t* mapping, int* sc_pos,
+                                                     int* sc_nid, int nleaves) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nleaves) return;
+    const Node* n = t.nodes[0] + i;
+    if (n->mParent == ID_UNDEF64) return;
+    int scdim = t.res[0] / scs;
+    for (int k = 0; k < spb; k++) {
+        u64 idx = u64(mapping[i]) * spb + k;
+        sc_pos[3 * idx] = n->mPos[0] + k % scdim * scs;
+        sc_pos[3 * idx + 1] = n->mPos[1] + (k / scdim) % scdim * scs;
+        sc_pos[3 * idx + 2] = n->mPos[2] + (k / scdim / scdim) % scdim * scs;
+        sc_nid[idx] = i;
+    }
+}
+
+hipError_t launch_subcell_pos(Tree t, int scs, int spb, const uint32_t* mapping, int* sc_pos, int* sc_nid, int nleaves,
+                              hipStream_t s) {
+    if (nleaves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_subcell_pos, dim3((nleaves + 255) / 256), dim3(256), 0, s, t, scs, spb, mapping, sc_pos, sc_nid,
+                       nleaves);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------- GatherDensity
+// radius - |p - v| evaluated with an error-free d^2 (FMA products and TwoSum)
+// and r^2 - d^2 divided by r + |p - v|, so the result keeps its relative
+// accuracy down to the edge of the falloff.  p - v is exact in float for the
+// positions of interest (|p - v| < 2r, p >= 0).
+__device__ __noinline__ float falloff_accurate(float px, float py, float pz, int vx, int vy, int vz, float r) {
+#pragma clang fp contract(off)
+    float dx = px - float(vx), dy = py - float(vy), dz = pz - float(vz);
+    float hx = dx * dx, hy = dy * dy, hz = dz * dz;
+    float lx = __builtin_fmaf(dx, dx, -hx), ly = __builtin_fmaf(dy, dy, -hy), lz = __builtin_fmaf(dz, dz, -hz);
+    float s1 = hx + hy;
+    float b1 = s1 - hx;
+    float e1 = (hx - (s1 - b1)) + (hy - b1);
+    float s2 = s1 + hz;
+    float b2 = s2 - s1;
+    float e2 = (s1 - (s2 - b2)) + (hz - b2);
+    float tail = (e1 + e2) + (lx + ly + lz);
+    float hi = s2 + tail;
+    float lo = tail - (hi - s2);
+    float r2h = r * r;
+    float r2l = __builtin_fmaf(r, r, -r2h);
+    float num = (r2h - hi) + (r2l - lo);
+    float c = __builtin_sqrtf(hi);
+    return num / (r + c);
+}
+
+// One workgroup per brick, one thread per interior voxel; the brick's
+// subcell entries are staged in LDS and every voxel keeps the nearest entry
+// of its subcell (gvdbGatherDensity, kernels/cuda_gvdb_particles.cuh:696-733:
+// val = max(val, r - |p_j - v|), v the integer voxel coordinate).
+static const int GATHER_CHUNK = 1024;
+
+template <typename AT>
+__device__ inline void store_atlas(AT* a, u64 off, float v);
+template <>
+__device__ inline void store_atlas<float>(float* a, u64 off, float v) { a[off] = v; }
+template <>
+__device__ inline void store_atlas<__half>(__half* a, u64 off, float v) { a[off] = __float2half(v); }
+template <typename AT>
+__device__ inline float load_atlas(const AT* a, u64 off);
+template <>
+__device__ inline float load_atlas<float>(const float* a, u64 off) { return a[off]; }
+template <>
+__device__ inline float load_atlas<__half>(const __half* a, u64 off) { return __half2float(a[off]); }
+
+template <typename AT>
+__global__ __launch_bounds__(512) void k_gather(Tree t, int scs, float radius, const int* sc_nid, const uint32_t* sc_cnt,
+                                                const uint32_t* sc_off, const int* sc_pos, const float* sc_pnt, AT* atlas,
+                                                int accumulate) {
+    __shared__ float lx[GATHER_CHUNK], ly[GATHER_CHUNK], lz[GATHER_CHUNK];
+    const int res0 = t.res[0];
+    const int scs3 = scs * scs * scs;
+    const int spb = (res0 / scs) * (res0 / scs) * (res0 / scs);
+    const int nthreads = res0 * res0 * res0;
+    const u64 brick = blockIdx.x;
+    const int tid = threadIdx.x;
+    const bool active = tid < nthreads;
+    const int k = active ? tid / scs3 : 0;      // subcell within the brick
+    const int q = tid - k * scs3;               // voxel within the subcell
+    const u64 sc = brick * spb + k;
+    int vx = 0, vy = 0, vz = 0;
+    uint32_t beg = 0, cnt = 0;
+    if (active) {
+        vx = sc_pos[3 * sc] + q % scs;
+        vy = sc_pos[3 * sc + 1] + (q / scs) % scs;
+        vz = sc_pos[3 * sc + 2] + q / (scs * scs);
+        beg = sc_off[sc];
+        cnt = sc_cnt[sc];
+    }
+    const u64 sc0 = brick * spb;
+    const uint32_t bbeg = sc_off[sc0];
+    const uint32_t bend = sc_off[sc0 + spb - 1] + sc_cnt[sc0 + spb - 1];
+    const float fx = float(vx), fy = float(vy), fz = float(vz);
+    float best = INFINITY;
+    uint32_t arg = 0xFFFFFFFFu;
+    for (uint32_t c0 = bbeg; c0 < bend; c0 += GATHER_CHUNK) {
+        uint32_t c1 = min(bend, c0 + GATHER_CHUNK);
+        __syncthreads();
+        for (uint32_t e = c0 + tid; e < c1; e += blockDim.x) {
+            lx[e - c0] = sc_pnt[3 * u64(e)];
+            ly[e - c0] = sc_pnt[3 * u64(e) + 1];
+            lz[e - c0] = sc_pnt[3 * u64(e) + 2];
+        }
+        __syncthreads();
+        uint32_t j0 = max(beg, c0), j1 = min(beg + cnt, c1);
+        for (uint32_t j = j0; j < j1; j++) {
+            float dx = lx[j - c0] - fx, dy = ly[j - c0] - fy, dz = lz[j - c0] - fz;
+            float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+            if (d2 < best) { best = d2; arg = j; }
+        }
+    }
+    if (!active) return;
+    const Node* nd = t.nodes[0] + sc_nid[sc];
+    int ax = nd->mValue[0] + (vx - nd->mPos[0]);
+    int ay = nd->mValue[1] + (vy - nd->mPos[1]);
+    int az = nd->mValue[2] + (vz - nd->mPos[2]);
+    u64 off = atlas_offset(t, ax, ay, az);
+    float val = accumulate ? load_atlas<AT>(atlas, off) : 0.0f;
+    if (arg != 0xFFFFFFFFu) {
+        const float* p = sc_pnt + 3 * u64(arg);
+        float f = falloff_accurate(p[0], p[1], p[2], vx, vy, vz, radius);
+        val = fmaxf(val, f);
+    }
+    store_atlas<AT>(atlas, off, val);
+}
+
+hipError_t launch_gather_density(Tree t, int scs, int nbricks_sc, float radius, const int* sc_nid,
+                                 const uint32_t* sc_cnt, const uint32_t* sc_off, const int* sc_pos,
+                                 const float* sc_pnt, void* atlas, int atlas_type, int accumulate, hipStream_t s) {
+    if (nbricks_sc <= 0) return hipSuccess;
+    int threads = t.res[0] * t.res[0] * t.res[0];
+    if (threads > 512) return hipErrorInvalidValue;
+    threads = (threads + 63) / 64 * 64;
+    if (atlas_type == T_HALF)
+        hipLaunchKernelGGL(k_gather<__half>, dim3(nbricks_sc), dim3(threads), 0, s, t, scs, radius, sc_nid, sc_cnt,
+                           sc_off, sc_pos, sc_pnt, (__half*)atlas, accumulate);
+    else
+        hipLaunchKernelGGL(k_gather<float>, dim3(nbricks_sc), dim3(threads), 0, s, t, scs, radius, sc_nid, sc_cnt,
+                           sc_off, sc_pos, sc_pnt, (float*)atlas, accumulate);
+    return hipGetLastError();
+}
+
+}  // namespace gvdb
+
+namespace gvdb {
+
+// ComputeBounds (src/gvdb_volume_gvdb.cpp:1792-1816): min corner and max
+// corner + range of the flagged leaves.  out6 starts as {INT_MAX x3, INT_MIN x3}.
+__global__ __launch_bounds__(256) void k_leaf_bounds(Tree t, int nleaves, int* out6) {
+    int mn[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, mx[3] = {(int)0x80000000, (int)0x80000000, (int)0x80000000};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nleaves; i += gridDim.x * blockDim.x) {
+        const Node* n = t.nodes[0] + i;
+        if (!n->mFlags) continue;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            mn[a] = min(mn[a], n->mPos[a]);
+            mx[a] = max(mx[a], n->mPos[a] + t.range[0]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            mn[a] = min(mn[a], __shfl_xor(mn[a], d, 64));
+            mx[a] = max(mx[a], __shfl_xor(mx[a], d, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int a = 0; a < 3; a++) {
+            atomicMin(&out6[a], mn[a]);
+            atomicMax(&out6[3 + a], mx[a]);
+        }
+}
+
+hipError_t launch_leaf_bounds(Tree t, int nleaves, int* out6, hipStream_t s) {
+    hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(out6), 0x7FFFFFFFu, 3, s);
+    if (e != hipSuccess) return e;
+    e = launch_fill_u32(reinterpret_cast<uint32_t*>(out6) + 3, 0x80000000u, 3, s);
+    if (e != hipSuccess) return e;
+    if (nleaves <= 0) return hipSuccess;
+    int blocks = (nleaves + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_leaf_bounds, dim3(blocks), dim3(256), 0, s, t, nleaves, out6);
+    return hipGetLastError();
+}
+
+}  // namespace gvdb

@@ -1,0 +1,310 @@
+// render.hip -- hierarchical DDA ray march over the sparse tree.
+//
+// Follows rayCast (kernels/cuda_gvdb_raycast.cuh:543-611 of the reference)
+// with the HDDA stepping of kernels/cuda_gvdb_dda.cuh:38-91, the
+// emission-absorption brick march rayDeepBrick (:485-533) and the trilinear
+// isosurface march raySurfaceTrilinearBrick (:281-300) with getGradient
+// (:132-141) and Phong + shadow ray (kernels/cuda_gvdb_module.cu:38-57).
+// Samples are exact software trilinear reads of the brick-major atlas; the
+// reference's texture unit is replaced because its filter weights are
+// quantised.  One 64-thread workgroup renders an 8x8 pixel tile.
+#include "kernels.h"
+#include "tree_device.h"
+#include <hip/hip_fp16.h>
+
+namespace gvdb {
+
+struct f3 { float x, y, z; };
+__device__ inline f3 mk3(float a, float b, float c) { return f3{a, b, c}; }
+__device__ inline f3 operator+(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline f3 operator-(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline f3 operator*(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__device__ inline float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline f3 norm3(f3 a) { float il = 1.0f / sqrtf(dot3(a, a)); return a * il; }
+
+template <typename AT>
+__device__ inline float ldv(const AT* a, u64 o);
+template <>
+__device__ inline float ldv<float>(const float* a, u64 o) { return a[o]; }
+template <>
+__device__ inline float ldv<__half>(const __half* a, u64 o) { return __half2float(a[o]); }
+
+// Trilinear sample at brick-local position p (interior coordinates) from the
+// padded brick starting at `b`; equals an unnormalised linear texture fetch
+// at atlas coordinate p + mValue.
+template <typename AT>
+__device__ inline float tri(const AT* b, int br, float px, float py, float pz) {
+    float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
+    float fx0 = floorf(qx), fy0 = floorf(qy), fz0 = floorf(qz);
+    int ix = min(max(int(fx0), 0), br - 2), iy = min(max(int(fy0), 0), br - 2), iz = min(max(int(fz0), 0), br - 2);
+    float fx = qx - fx0, fy = qy - fy0, fz = qz - fz0;
+    u64 o = u64((iz * br + iy) * br + ix);
+    u64 sy = u64(br), sz = u64(br) * br;
+    float c000 = ldv<AT>(b, o), c100 = ldv<AT>(b, o + 1);
+    float c010 = ldv<AT>(b, o + sy), c110 = ldv<AT>(b, o + sy + 1);
+    float c001 = ldv<AT>(b, o + sz), c101 = ldv<AT>(b, o + sz + 1);
+    float c011 = ldv<AT>(b, o + sz + sy), c111 = ldv<AT>(b, o + sz + sy + 1);
+    float c00 = c000 + fx * (c100 - c000), c10 = c010 + fx * (c110 - c010);
+    float c01 = c001 + fx * (c101 - c001), c11 = c011 + fx * (c111 - c011);
+    float c0 = c00 + fy * (c10 - c00), c1 = c01 + fy * (c11 - c01);
+    return c0 + fz * (c1 - c0);
+}
+
+struct RenderArgs {
+    Tree t;
+    ScnBlock s;
+    const float4* tf;
+    const void* atlas;
+    void* out;
+    int out_float;
+};
+
+struct Hdda {
+    f3 pos, dir, tDel, t, tSide;
+    int ps[3], p[3], m[3];
+    __device__ void prepare(f3 vmin, float vdel) {
+        tDel = mk3(fabsf(vdel / dir.x), fabsf(vdel / dir.y), fabsf(vdel / dir.z));
+        float pf[3] = {(pos.x + t.x * dir.x - vmin.x) / vdel, (pos.y + t.x * dir.y - vmin.y) / vdel,
+                       (pos.z + t.x * dir.z - vmin.z) / vdel};
+        float td[3] = {tDel.x, tDel.y, tDel.z};
+        float ts[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float fl = floorf(pf[a]);
+            ts[a] = ((fl - pf[a] + 0.5f) * float(ps[a]) + 0.5f) * td[a] + t.x;
+            p[a] = int(fl);
+        }
+        tSide = mk3(ts[0], ts[1], ts[2]);
+    }
+    __device__ void next() {
+        m[0] = int((tSide.x < tSide.y) & (tSide.x <= tSide.z));
+        m[1] = int((tSide.y < tSide.z) & (tSide.y <= tSide.x));
+        m[2] = int((tSide.z < tSide.x) & (tSide.z <= tSide.y));
+        t.y = m[0] ? tSide.x : (m[1] ? tSide.y : tSide.z);
+    }
+    __device__ void step() {
+        t.x = t.y;
+        tSide = tSide + mk3(float(m[0]) * tDel.x, float(m[1]) * tDel.y, float(m[2]) * tDel.z);
+#pragma unroll
+        for (int a = 0; a < 3; a++) p[a] += m[a] * ps[a];
+    }
+};
+
+template <typename AT>
+__device__ inline const AT* brick_base(const Tree& t, const AT* atlas, const Node* n) {
+    return atlas + slot_of(t, n->mValue) * u64(t.brick_vox);
+}
+
+// rayDeepBrick: emission-absorption march through one brick.
+template <typename AT>
+__device__ void deep_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 tt, f3 pos, f3 dir, float clr[4]) {
+    const Tree& t = A.t;
+    const Node* n = t.nodes[0] + nodeid;
+    const AT* b = brick_base(t, atlas, n);
+    const float dstep = A.s.steps[0];
+    f3 vmin = mk3(float(n->mPos[0]), float(n->mPos[1]), float(n->mPos[2]));
+    float tx = dstep * ceilf(tt.x / dstep);
+    f3 p = pos + dir * tx - vmin;
+    f3 wpt = dir * dstep;
+    float r0 = float(t.res[0]);
+    const float extinct = A.s.extinct[0], albedo = A.s.extinct[1];
+    const float minval = A.s.cutoff[0], alphacut = A.s.cutoff[1];
+    const float th = A.s.thresh[0], vr = A.s.thresh[2] - A.s.thresh[1];
+    for (int it = 0; clr[3] > alphacut && it < MAX_ITER && p.x >= 0 && p.y >= 0 && p.z >= 0 && p.x < r0 && p.y < r0 &&
+                     p.z < r0; it++) {
+        float raw = tri<AT>(b, t.brick_res, p.x, p.y, p.z);
+        if (raw >= minval) {
+            float x = fminf(1.0f, fmaxf(0.0f, (raw - th) / vr));
+            float4 val = A.tf[int(x * 16300.0f)];
+            float w = expf(extinct * val.w * dstep);
+            float k = clr[3] * (1.0f - w) * albedo;
+            clr[0] += val.x * k;
+            clr[1] += val.y * k;
+            clr[2] += val.z * k;
+            clr[3] *= w;
+        }
+        p = p + wpt;
+    }
+    clr[0] = fminf(clr[0], 1.0f); clr[1] = fminf(clr[1], 1.0f); clr[2] = fminf(clr[2], 1.0f);
+    clr[3] = fmaxf(clr[3], 0.0f);
+}
+
+// raySurfaceTrilinearBrick: first sample >= threshold, gradient normal.
+template <typename AT>
+__device__ void surf_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 tt, f3 pos, f3 dir, f3& hit, f3& nrm) {
+    const Tree& t = A.t;
+    const Node* n = t.nodes[0] + nodeid;
+    const AT* b = brick_base(t, atlas, n);
+    const float dstep = A.s.steps[0];
+    f3 vmin = mk3(float(n->mPos[0]), float(n->mPos[1]), float(n->mPos[2]));
+    float tx = dstep * ceilf(tt.x / dstep);
+    f3 p = pos + dir * tx - vmin;
+    float r0 = float(t.res[0]);
+    const float th = A.s.thresh[0];
+    const int br = t.brick_res;
+    for (int it = 0; it < MAX_ITER && p.x >= 0 && p.y >= 0 && p.z >= 0 && p.x < r0 && p.y < r0 && p.z < r0; it++) {
+        if (tri<AT>(b, br, p.x, p.y, p.z) >= th) {
+            hit = p + vmin;
+            f3 g;
+            g.x = tri<AT>(b, br, p.x - 0.5f, p.y, p.z) - tri<AT>(b, br, p.x + 0.5f, p.y, p.z);
+            g.y = tri<AT>(b, br, p.x, p.y - 0.5f, p.z) - tri<AT>(b, br, p.x, p.y + 0.5f, p.z);
+            g.z = tri<AT>(b, br, p.x, p.y, p.z - 0.5f) - tri<AT>(b, br, p.x, p.y, p.z + 0.5f);
+            nrm = norm3(g);
+            return;
+        }
+        p = p + dir * dstep;
+    }
+}
+
+// rayCast.  MODE 0: deep (EA), MODE 1: trilinear surface.
+template <int MODE, typename AT>
+__device__ void ray_cast(const RenderArgs& A, const AT* atlas, f3 pos, f3 dir, f3& hit, f3& nrm, float clr[4]) {
+    const Tree& t = A.t;
+    int nodeid[MAXLEV];
+    float tMax[MAXLEV];
+    int lev = t.top_lev;
+    nodeid[lev] = 0;
+    float h0 = (t.bmin[0] - pos.x) / dir.x, h1 = (t.bmax[0] - pos.x) / dir.x;
+    float h2 = (t.bmin[1] - pos.y) / dir.y, h3 = (t.bmax[1] - pos.y) / dir.y;
+    float h4 = (t.bmin[2] - pos.z) / dir.z, h5 = (t.bmax[2] - pos.z) / dir.z;
+    float t0 = fmaxf(fmaxf(fminf(h0, h1), fminf(h2, h3)), fminf(h4, h5));
+    float t1 = fminf(fminf(fmaxf(h0, h1), fmaxf(h2, h3)), fmaxf(h4, h5));
+    t0 = (t0 < 0) ? 0.0f : t0;
+    if (t1 < t0 || t1 < 0) return;
+    const Node* node = t.nodes[lev];
+    Hdda d;
+    d.pos = pos;
+    d.dir = dir;
+    d.ps[0] = dir.x > 0 ? 1 : -1; d.ps[1] = dir.y > 0 ? 1 : -1; d.ps[2] = dir.z > 0 ? 1 : -1;
+    d.t = mk3(t0 + t.epsilon, t1, 0.0f);
+    tMax[lev] = t1 - t.epsilon;
+    d.prepare(mk3(float(node->mPos[0]), float(node->mPos[1]), float(node->mPos[2])), float(t.range[lev] / t.res[lev]));
+    for (int it = 0; it < MAX_ITER && lev > 0 && lev <= t.top_lev && d.p[0] >= 0 && d.p[1] >= 0 && d.p[2] >= 0 &&
+                     d.p[0] <= t.res[lev] && d.p[1] <= t.res[lev] && d.p[2] <= t.res[lev]; it++) {
+        d.next();
+        int res = t.res[lev];
+        int child = -1;
+        if (d.p[0] < res && d.p[1] < res && d.p[2] < res) {
+            int b = (((d.p[2] << t.logdim[lev]) + d.p[1]) << t.logdim[lev]) + d.p[0];
+            child = child_of(t, node, b);
+        }
+        if (child >= 0) {
+            if (lev == 1) {
+                f3 tt = d.t;
+                tt.x += t.epsilon;
+                if (MODE == 0) {
+                    deep_brick<AT>(A, atlas, child, tt, pos, dir, clr);
+                    if (clr[3] <= 0) { clr[3] = 0; return; }
+                } else {
+                    surf_brick<AT>(A, atlas, child, tt, pos, dir, hit, nrm);
+                    if (hit.z != NOHIT) return;
+                }
+                d.step();
+            } else {
+                lev--;
+                nodeid[lev] = child;
+                node = t.nodes[lev] + child;
+                d.t.x += t.epsilon;
+                tMax[lev] = d.t.y - t.epsilon;
+                d.prepare(mk3(float(node->mPos[0]), float(node->mPos[1]), float(node->mPos[2])),
+                          float(t.range[lev] / t.res[lev]));
+            }
+        } else {
+            d.step();
+        }
+        while (d.t.x > tMax[lev] && lev <= t.top_lev) {
+            lev++;
+            if (lev <= t.top_lev) {
+                node = t.nodes[lev] + nodeid[lev];
+                d.prepare(mk3(float(node->mPos[0]), float(node->mPos[1]), float(node->mPos[2])),
+                          float(t.range[lev] / t.res[lev]));
+            }
+        }
+    }
+}
+
+__device__ inline f3 mmult(const float* m, f3 v) {
+    return mk3(v.x * m[0] + v.y * m[4] + v.z * m[8] + m[12], v.x * m[1] + v.y * m[5] + v.z * m[9] + m[13],
+               v.x * m[2] + v.y * m[6] + v.z * m[10] + m[14]);
+}
+
+template <int MODE, typename AT>
+__global__ __launch_bounds__(64) void k_render(RenderArgs A) {
+    const int tiles_x = (A.s.width + 7) / 8;
+    const int tile = blockIdx.x;
+    const int x = (tile % tiles_x) * 8 + (threadIdx.x & 7);
+    const int y = (tile / tiles_x) * 8 + (threadIdx.x >> 3);
+    if (x >= A.s.width || y >= A.s.height) return;
+    const AT* atlas = reinterpret_cast<const AT*>(A.atlas);
+    f3 rpos = mmult(A.s.invxform, mk3(A.s.campos[0], A.s.campos[1], A.s.campos[2]));
+    float fx = float(x + 0.5f) / float(A.s.width), fy = float(y + 0.5f) / float(A.s.height);
+    f3 v = mk3(fx * A.s.camu[0] + fy * A.s.camv[0] + A.s.cams[0], fx * A.s.camu[1] + fy * A.s.camv[1] + A.s.cams[1],
+               fx * A.s.camu[2] + fy * A.s.camv[2] + A.s.cams[2]);
+    f3 rdir = norm3(mmult(A.s.invxrot, v));
+    float res[4];
+    if (MODE == 0) {
+        float clr[4] = {0, 0, 0, 1};
+        f3 hit = mk3(0, 0, NOHIT), nrm = mk3(0, 0, 0);
+        ray_cast<0, AT>(A, atlas, rpos, rdir, hit, nrm, clr);
+        float a = 1.0f - clr[3];
+        for (int k = 0; k < 3; k++) res[k] = A.s.backclr[k] + a * (clr[k] - A.s.backclr[k]);
+        res[3] = a;
+    } else {
+        float clr[4] = {1, 1, 1, 1};
+        f3 hit = mk3(NOHIT, NOHIT, NOHIT), nrm = mk3(0, 0, 0);
+        ray_cast<1, AT>(A, atlas, rpos, rdir, hit, nrm, clr);
+        if (hit.z == NOHIT) {
+            for (int k = 0; k < 4; k++) res[k] = A.s.backclr[k];
+        } else {
+            f3 lp = mk3(A.s.light_pos[0], A.s.light_pos[1], A.s.light_pos[2]);
+            f3 ld = norm3(lp - hit);
+            float diff = 0.9f * fmaxf(0.0f, dot3(nrm, ld));
+            const float amb = 0.1f;
+            if (A.s.shadow[0] > 0) {
+                f3 h2 = mk3(0, 0, NOHIT), n2 = mk3(0, 0, 0);
+                float c2[4] = {0, 0, 0, 1};
+                ray_cast<1, AT>(A, atlas, hit + nrm * A.s.shadow[1], ld, h2, n2, c2);
+                diff = (h2.z == NOHIT) ? diff : diff * (1.0f - A.s.shadow[0]);
+            }
+            for (int k = 0; k < 3; k++) res[k] = clr[k] * (diff + amb);
+            res[3] = 1.0f;
+        }
+    }
+    u64 pix = u64(y) * A.s.width + x;
+    if (A.out_float) {
+        reinterpret_cast<float4*>(A.out)[pix] = make_float4(res[0], res[1], res[2], res[3]);
+    } else {
+        uchar4 c;
+        c.x = (unsigned char)(res[0] * 255.0f);
+        c.y = (unsigned char)(res[1] * 255.0f);
+        c.z = (unsigned char)(res[2] * 255.0f);
+        c.w = (unsigned char)(res[3] * 255.0f);
+        reinterpret_cast<uchar4*>(A.out)[pix] = c;
+    }
+}
+
+hipError_t launch_render(Tree t, const ScnBlock& scn, const float4* tf, const void* atlas, int atlas_type, int shading,
+                         void* out, int out_float, hipStream_t s) {
+    RenderArgs A;
+    A.t = t;
+    A.s = scn;
+    A.tf = tf;
+    A.atlas = atlas;
+    A.out = out;
+    A.out_float = out_float;
+    unsigned tiles = unsigned(((scn.width + 7) / 8) * ((scn.height + 7) / 8));
+    if (tiles == 0) return hipSuccess;
+    bool half = atlas_type == T_HALF;
+    if (shading == SHADE_VOLUME) {
+        if (half) hipLaunchKernelGGL((k_render<0, __half>), dim3(tiles), dim3(64), 0, s, A);
+        else hipLaunchKernelGGL((k_render<0, float>), dim3(tiles), dim3(64), 0, s, A);
+    } else if (shading == SHADE_TRILINEAR) {
+        if (half) hipLaunchKernelGGL((k_render<1, __half>), dim3(tiles), dim3(64), 0, s, A);
+        else hipLaunchKernelGGL((k_render<1, float>), dim3(tiles), dim3(64), 0, s, A);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace gvdb

@@ -1,0 +1,73 @@
+// tree_device.h -- GVDB tree walks on the device.
+//
+// These follow the reference's walks exactly where its behaviour is defined
+// (kernels/cuda_gvdb_nodes.cuh:115-124 getChild, :199-253 getNode /
+// getNodeAtPoint, :255-288 getPosLeafParent).  Where the reference would
+// index a child list outside [0, res) (positions outside the start node), the
+// walks here report "no node" instead of reading neighbouring memory.
+#pragma once
+#include "gvdb_device.h"
+
+namespace gvdb {
+
+__device__ inline Node* node_at(const Tree& t, int lev, u64 idx) { return t.nodes[lev] + idx; }
+
+// Returns the child index, -1 for an empty slot or a node without a list.
+__device__ inline int child_of(const Tree& t, const Node* n, int b) {
+    u64 listid = n->mChildList;
+    if (listid == ID_UNDEFL) return -1;
+    int clev = int((listid >> 8) & 0xFF);
+    u64 cndx = listid >> 16;
+    u64 r = u64(t.res[clev]);
+    u64 v = t.clists[clev][cndx * r * r * r + u64(b)];
+    return int(v >> 16);
+}
+
+// Leaf index of the brick containing integer position `pos`, or -1.
+__device__ inline int pos_leaf(const Tree& t, int px, int py, int pz) {
+    const Node* nd = node_at(t, t.top_lev, 0);
+    int l = nd->mLev;
+    if (l == 0) return 0;
+    while (true) {
+        int res = t.res[l];
+        long long rg = t.range[l];
+        int a = int((long long)(px - nd->mPos[0]) * res / rg);
+        int b = int((long long)(py - nd->mPos[1]) * res / rg);
+        int c = int((long long)(pz - nd->mPos[2]) * res / rg);
+        if (a < 0 || b < 0 || c < 0 || a >= res || b >= res || c >= res) return -1;
+        int child = child_of(t, nd, (c * res + b) * res + a);
+        if (child < 0) return -1;
+        l--;
+        if (l == 0) return child;
+        nd = node_at(t, l, u64(child));
+    }
+}
+
+// getNodeAtPoint: float walk with per-level bounds, returns the leaf or -1.
+__device__ inline int point_leaf(const Tree& t, float px, float py, float pz) {
+    int lev = t.top_lev;
+    const Node* nd = node_at(t, lev, 0);
+    int id = 0;
+    while (lev > 0) {
+        float vx = float(nd->mPos[0]), vy = float(nd->mPos[1]), vz = float(nd->mPos[2]);
+        float rg = float(t.range[lev]);
+        if (px < vx || py < vy || pz < vz || px >= vx + rg || py >= vy + rg || pz >= vz + rg) return -1;
+        float vdel = float(t.range[lev] / t.res[lev]);
+        int a = int((px - vx) / vdel), b = int((py - vy) / vdel), c = int((pz - vz) / vdel);
+        int bit = (((c << t.logdim[lev]) + b) << t.logdim[lev]) + a;
+        lev--;
+        int child = child_of(t, nd, bit);
+        if (child < 0) return -1;
+        id = child;
+        nd = node_at(t, lev, u64(child));
+    }
+    return id;
+}
+
+// Atlas slot of a node whose interior corner is mValue.
+__device__ inline u64 slot_of(const Tree& t, const int mv[3]) {
+    int br = t.brick_res;
+    return (u64(mv[2] / br) * t.atlas_cnt[1] + u64(mv[1] / br)) * t.atlas_cnt[0] + u64(mv[0] / br);
+}
+
+}  // namespace gvdb

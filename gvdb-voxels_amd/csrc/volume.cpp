@@ -1,0 +1,1021 @@
+// volume.cpp -- host runtime of one grid and the C ABI (include/gvdb_amd.h).
+//
+// Follows the host side of VolumeGVDB (source/gvdb_library/src/
+// gvdb_volume_gvdb.cpp of the reference) for the point-to-volume path:
+// Configure, pools with doubling growth, GPU RebuildTopology, the host
+// ActivateSpace build, FinishTopology, UpdateAtlas, InsertPointsSubcell,
+// GatherDensity, UpdateApron and Render.  All device work runs on the grid's
+// own stream; the only host round trips are the ones whose results the API
+// returns to the caller (bounding box/depth, node counts, subcell entries).
+#include "volume.h"
+#include "../../include/gvdb_amd.h"
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+
+namespace gvdb {
+
+// ------------------------------------------------------------- buffers ----
+hipError_t DevBuf::ensure(size_t n, bool preserve, hipStream_t s) {
+    if (n <= bytes) return hipSuccess;
+    size_t want = std::max(n, bytes + bytes / 4);
+    want = (want + 255) & ~size_t(255);
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        want = (n + 255) & ~size_t(255);
+        e = hipMalloc(&q, want);
+        if (e != hipSuccess) return e;
+    }
+    if (preserve && p && bytes) {
+        e = hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { (void)hipFree(q); return e; }
+    }
+    if (p) (void)hipFree(p);
+    p = q;
+    bytes = want;
+    return hipSuccess;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+#define HIPCHK(x)                                          \
+    do {                                                   \
+        hipError_t e_ = (x);                               \
+        if (e_ != hipSuccess) return hip_fail(e_, #x);     \
+    } while (0)
+
+int Volume::hip_fail(hipError_t e, const char* what) {
+    err = std::string("HIP error ") + hipGetErrorString(e) + " in " + what;
+    return e == hipErrorOutOfMemory ? GVDB_ERR_OOM : GVDB_ERR_HIP;
+}
+
+Volume::Volume(int dev) : device(dev) {}
+
+Volume::~Volume() {
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (int g = 0; g < 2; g++)
+        for (int l = 0; l < MAXLEV; l++) pool_[g][l].dev.release();
+    for (auto& c : chan_) c.data.release();
+    for (auto& r : rbuf_) r.data.release();
+    DevBuf* bufs[] = {&amap_, &nbr_, &pts_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_,
+                      &sc_flags_, &small_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev};
+    for (DevBuf* b : bufs) b->release();
+    for (int s = 0; s < ST_COUNT; s++)
+        for (int k = 0; k < 2; k++)
+            if (ev_[s][k]) (void)hipEventDestroy(ev_[s][k]);
+    for (int s = 0; s < 4; s++)
+        for (int k = 0; k < 2; k++)
+            if (kev_[s][k]) (void)hipEventDestroy(kev_[s][k]);
+    if (pinned_) (void)hipHostFree(pinned_);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Volume::init() {
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (int s = 0; s < ST_COUNT; s++)
+        for (int k = 0; k < 2; k++) HIPCHK(hipEventCreate(&ev_[s][k]));
+    for (int s = 0; s < 4; s++)
+        for (int k = 0; k < 2; k++) HIPCHK(hipEventCreate(&kev_[s][k]));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 256 * sizeof(uint32_t), hipHostMallocDefault));
+    HIPCHK(small_.ensure(4096));
+    // Initialize(): default transfer function and range (gvdb_volume_gvdb.cpp:2341-2348)
+    tf_host.assign(size_t(TF_SIZE) * 4, 0.f);
+    const float a[4] = {0, 0, 0, 0}, b[4] = {1, 1, 1, 0.1f};
+    linear_tf(0.f, 1.f, a, b);
+    return GVDB_OK;
+}
+
+void Volume::stage_begin(Stage s) {
+    (void)hipEventRecord(ev_[s][0], stream);
+}
+void Volume::stage_end(Stage s) {
+    (void)hipEventRecord(ev_[s][1], stream);
+    ev_used_[s] = true;
+}
+
+int Volume::stage_ms(float* out) {
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int s = 0; s < ST_COUNT; s++) {
+        out[s] = 0.f;
+        if (ev_used_[s]) HIPCHK(hipEventElapsedTime(&out[s], ev_[s][0], ev_[s][1]));
+    }
+    out[ST_COUNT] = 0.f;
+    return GVDB_OK;
+}
+
+int Volume::kernel_ms(float* out) {
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int s = 0; s < 4; s++) {
+        out[s] = 0.f;
+        if (kev_used_[s]) HIPCHK(hipEventElapsedTime(&out[s], kev_[s][0], kev_[s][1]));
+    }
+    return GVDB_OK;
+}
+
+// ------------------------------------------------------- configuration ----
+int Volume::configure(int levels, const int* log2dims, const int* initcnt) {
+    if (levels < 2 || levels > MAXLEV) return fail(GVDB_ERR_INVALID, "Configure: levels must be 2..10");
+    HIPCHK(hipStreamSynchronize(stream));
+    levels_ = levels;
+    int defcnt[MAXLEV];
+    {
+        // Configure(q4..q0) pool sizes (gvdb_volume_gvdb.cpp:2369-2374)
+        int c = 4;
+        defcnt[0] = 4;
+        for (int l = 1; l < levels; l++) { defcnt[l] = c; c >>= 1; }
+    }
+    for (int n = 0; n < levels; n++) {
+        int r = log2dims[n];
+        if (r < 0 || r > 10) return fail(GVDB_ERR_INVALID, "Configure: log2 dims must be 0..10");
+        logdim_[n] = (r == 0) ? 1 : r;
+        res_[n] = 1 << logdim_[n];
+    }
+    for (int l = 0; l < levels; l++) {
+        long long rr = res_[0];
+        for (int k = 1; k <= l; k++) rr *= res_[k];
+        range_[l] = rr;
+    }
+    if (range_[levels - 1] > (1ll << 30)) return fail(GVDB_ERR_INVALID, "Configure: tree range exceeds 2^30 voxels");
+    for (int g = 0; g < 2; g++)
+        for (int l = 0; l < MAXLEV; l++) {
+            pool_[g][l].dev.release();
+            pool_[g][l] = Pool();
+        }
+    for (int n = 0; n < levels; n++) {
+        u64 cnt = u64(std::max(1, initcnt ? initcnt[n] : defcnt[n]));
+        Pool& p0 = pool_[0][n];
+        p0.exists = true;
+        p0.stride = sizeof(Node);
+        p0.max = cnt;
+        if (n > 0) {
+            Pool& p1 = pool_[1][n];
+            p1.exists = true;
+            p1.stride = sizeof(u64) * voxcnt(n);
+            p1.max = cnt;
+        }
+    }
+    brickwid_ = res_[0];
+    host_valid_ = true;
+    dev_stale_ = false;
+    return clear();
+}
+
+int Volume::set_channel_default(int x, int y, int z) {
+    if (x <= 0 || y <= 0 || z <= 0) return fail(GVDB_ERR_INVALID, "SetChannelDefault: counts must be positive");
+    default_axiscnt_[0] = x; default_axiscnt_[1] = y; default_axiscnt_[2] = z;
+    return GVDB_OK;
+}
+
+int Volume::add_channel(int c, int dtype, int apron, int filter, int border, int ax, int ay, int az) {
+    // AddChannel + AtlasCreate (gvdb_volume_gvdb.cpp:2437-2453, gvdb_allocator.cpp:528-560)
+    if (levels_ == 0) return fail(GVDB_ERR_STATE, "AddChannel before Configure");
+    if (c < 0 || c >= 8) return fail(GVDB_ERR_INVALID, "AddChannel: channel must be 0..7");
+    if (dtype != T_FLOAT && dtype != T_HALF) return fail(GVDB_ERR_UNSUPPORTED, "AddChannel: only float and half channels");
+    if (apron < 0 || apron > 1) return fail(GVDB_ERR_UNSUPPORTED, "AddChannel: apron must be 0 or 1");
+    if (c > 0 && !chan_[0].exists) return fail(GVDB_ERR_STATE, "AddChannel: channel 0 first");
+    if (ax == 0 && ay == 0 && az == 0) {
+        if (c == 0) { ax = default_axiscnt_[0]; ay = default_axiscnt_[1]; az = default_axiscnt_[2]; }
+        else { ax = atlas_cnt_[0]; ay = atlas_cnt_[1]; az = atlas_cnt_[2]; }
+    }
+    if (c == 0) {
+        atlas_cnt_[0] = ax; atlas_cnt_[1] = ay; atlas_cnt_[2] = az;
+        atlas_max_ = u64(ax) * ay * az;
+        apron_ = apron;
+    } else if (ax != atlas_cnt_[0] || ay != atlas_cnt_[1] || az != atlas_cnt_[2] || apron != apron_) {
+        return fail(GVDB_ERR_UNSUPPORTED, "AddChannel: all channels share channel 0's atlas layout");
+    }
+    Channel& ch = chan_[c];
+    ch.exists = true;
+    ch.type = dtype;
+    ch.filter = filter;
+    ch.border = border;
+    int br = brickwid_ + 2 * apron_;
+    size_t bytes = size_t(atlas_max_) * br * br * br * ch.elem();
+    HIPCHK(ch.data.ensure(bytes));
+    HIPCHK(launch_fill_u32(ch.data.as<uint32_t>(), 0u, ch.data.bytes / 4, stream));
+    return GVDB_OK;
+}
+
+int Volume::set_epsilon(float eps, int maxiter) {
+    epsilon_ = eps;
+    maxiter_ = maxiter;
+    return GVDB_OK;
+}
+
+// ---------------------------------------------------------------- points ----
+int Volume::set_points_gpu(const void* p, int n, int off, int stride) {
+    if (n < 0 || (n > 0 && !p)) return fail(GVDB_ERR_INVALID, "SetPoints: null positions");
+    if (stride < 12 || off < 0) return fail(GVDB_ERR_INVALID, "SetPoints: stride must hold a float3");
+    pts_ = PointSrc{reinterpret_cast<const char*>(p), n, off, stride};
+    return GVDB_OK;
+}
+
+int Volume::upload_points(const float* host, int n) {
+    if (n < 0 || (n > 0 && !host)) return fail(GVDB_ERR_INVALID, "upload_points: null positions");
+    HIPCHK(pts_owned_.ensure(size_t(std::max(n, 1)) * 12));
+    if (n) {
+        HIPCHK(hipMemcpyAsync(pts_owned_.p, host, size_t(n) * 12, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    pts_ = PointSrc{reinterpret_cast<const char*>(pts_owned_.p), n, 0, 12};
+    return GVDB_OK;
+}
+
+// ----------------------------------------------------------------- pools ----
+u64 Volume::pool_alloc(int grp, int lev) {
+    // PoolAlloc (gvdb_allocator.cpp:163-192) on the host mirror
+    if (lev >= levels_) return ID_UNDEFL;
+    Pool& p = pool_[grp][lev];
+    if (!p.exists) return ID_UNDEFL;
+    if (p.lastEle >= p.max) p.max *= 2;
+    if (p.host.size() < p.max * p.stride) p.host.resize(p.max * p.stride, 0);
+    p.lastEle++;
+    p.usedNum++;
+    return Elem(grp, lev, p.lastEle - 1);
+}
+
+int Volume::ensure_pool_capacity(int grp, int lev, u64 need) {
+    Pool& p = pool_[grp][lev];
+    while (p.max < need) p.max *= 2;
+    HIPCHK(p.dev.ensure(size_t(p.max * p.stride)));
+    return GVDB_OK;
+}
+
+Node* Volume::hnode(u64 ref) {
+    Pool& p = pool_[0][(ref >> 8) & 0xFF];
+    return reinterpret_cast<Node*>(&p.host[(ref >> 16) * p.stride]);
+}
+u64* Volume::hclist(u64 ref) {
+    Pool& p = pool_[ref & 0xFF][(ref >> 8) & 0xFF];
+    return reinterpret_cast<u64*>(&p.host[(ref >> 16) * p.stride]);
+}
+
+int Volume::sync_host() {
+    // PoolFetchAll (gvdb_allocator.cpp:119-131)
+    if (host_valid_) return GVDB_OK;
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int g = 0; g < 2; g++)
+        for (int l = 0; l < levels_; l++) {
+            Pool& p = pool_[g][l];
+            if (!p.exists) continue;
+            if (p.host.size() < p.max * p.stride) p.host.resize(p.max * p.stride, 0);
+            if (p.lastEle) HIPCHK(hipMemcpy(p.host.data(), p.dev.p, p.lastEle * p.stride, hipMemcpyDeviceToHost));
+        }
+    host_valid_ = true;
+    return GVDB_OK;
+}
+
+int Volume::commit_pools() {
+    // PoolCommitAll (gvdb_allocator.cpp:93-116)
+    if (!dev_stale_) return GVDB_OK;
+    for (int g = 0; g < 2; g++)
+        for (int l = 0; l < levels_; l++) {
+            Pool& p = pool_[g][l];
+            if (!p.exists) continue;
+            int rc = ensure_pool_capacity(g, l, std::max<u64>(p.lastEle, 1));
+            if (rc) return rc;
+            if (p.lastEle)
+                HIPCHK(hipMemcpyAsync(p.dev.p, p.host.data(), p.lastEle * p.stride, hipMemcpyHostToDevice, stream));
+        }
+    HIPCHK(hipStreamSynchronize(stream));
+    dev_stale_ = false;
+    return GVDB_OK;
+}
+
+void Volume::setup_node(u64 id, int lev, const int pos[3]) {
+    // SetupNode (gvdb_volume_gvdb.cpp:2512-2524)
+    Node* n = hnode(id);
+    memset(n, 0, sizeof(Node));
+    n->mLev = uint8_t(lev);
+    n->mPos[0] = pos[0]; n->mPos[1] = pos[1]; n->mPos[2] = pos[2];
+    n->mChildList = ID_UNDEFL;
+    n->mParent = ID_UNDEFL;
+    n->mValue[0] = n->mValue[1] = n->mValue[2] = -1;
+    n->mFlags = 1;
+}
+
+void Volume::compute_top_lev() {
+    // PrepareVDB's top level: the lowest level holding exactly one node (gvdb_volume_gvdb.cpp:3955-3979)
+    int t = 1;
+    for (int n = levels_ - 1; n >= 0; n--)
+        if (pool_[0][n].lastEle == 1) t = n;
+    top_lev_ = t;
+}
+
+int Volume::clear() {
+    // Clear (gvdb_volume_gvdb.cpp:2487-2503)
+    for (int g = 0; g < 2; g++)
+        for (int l = 0; l < levels_; l++) { pool_[g][l].lastEle = 0; pool_[g][l].usedNum = 0; }
+    root_ = ID_UNDEFL;
+    atlas_used_ = 0;
+    rebuild_topo_ = true;
+    host_valid_ = true;
+    dev_stale_ = false;
+    topo_gen_++;
+    return GVDB_OK;
+}
+
+void Volume::covering_node_host(int lev, const int pos[3], int out[3]) const {
+    // GetCoveringNode (gvdb_volume_gvdb.cpp:2915-2931)
+    long long rg = range_[lev];
+    for (int a = 0; a < 3; a++) {
+        long long np = (pos[a] / rg) * rg;
+        if (pos[a] < np) np -= rg;
+        out[a] = int(np);
+    }
+}
+
+int Volume::determine_depth(int root_pos[3]) {
+    // DetermineDepth (gvdb_volume_gvdb.cpp:821-837)
+    int pmin[3] = {int(posMin_[0]), int(posMin_[1]), int(posMin_[2])};
+    int pmax[3] = {int(posMax_[0]), int(posMax_[1]), int(posMax_[2])};
+    for (int lev = 0; lev < levels_; lev++) {
+        int a[3], b[3];
+        covering_node_host(lev, pmin, a);
+        covering_node_host(lev, pmax, b);
+        if (a[0] == b[0] && a[1] == b[1] && a[2] == b[2]) {
+            root_pos[0] = a[0]; root_pos[1] = a[1]; root_pos[2] = a[2];
+            return lev;
+        }
+    }
+    return -1;
+}
+
+static inline float key_float(uint32_t k) {
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+// --------------------------------------------------------- GPU topology ----
+int Volume::rebuild_topology(int n, float radius, const float orig[3]) {
+    if (levels_ == 0) return fail(GVDB_ERR_STATE, "RebuildTopology before Configure");
+    if (n <= 0 || !pts_.base || n > pts_.n) return fail(GVDB_ERR_INVALID, "RebuildTopology: point count exceeds SetPoints");
+    if (!(radius >= 0.0f) || radius > 32.0f) return fail(GVDB_ERR_UNSUPPORTED, "RebuildTopology: radius must be in [0, 32]");
+    stage_begin(ST_TOPO);
+    PointSrc P = pts_;
+    P.n = n;
+    uint32_t* small = small_.as<uint32_t>();
+    HIPCHK(launch_bbox(P, orig, small, stream));
+    HIPCHK(hipMemcpyAsync(pinned_, small, 6 * 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    float mn[3], mx[3];
+    for (int a = 0; a < 3; a++) {
+        mn[a] = key_float(pinned_[a]);
+        mx[a] = key_float(pinned_[3 + a]);
+        // GetBoundingBox padding (gvdb_volume_gvdb.cpp:5452-5460)
+        posMin_[a] = mn[a] - 1;
+        posMax_[a] = mx[a] + 2;
+        posMin_[a] = 0 > posMin_[a] ? 0 : posMin_[a];
+        posMax_[a] = 0 > posMax_[a] ? 0 : posMax_[a];
+    }
+    int rootPos[3];
+    int depth = determine_depth(rootPos);
+    if (depth == 0) return fail(GVDB_ERR_DEPTH, "Points extents smaller than 1 brick. Not yet supported.");
+    if (depth < 0 || depth > 5 || depth >= levels_) return fail(GVDB_ERR_DEPTH, "Point extents exceed 5 levels.");
+
+    // The reference rebuilds fully on the first call, after RequestFullRebuild
+    // and on depth changes, and walks an incremental path otherwise
+    // (gvdb_volume_gvdb.cpp:1195-1217); every call here is a full rebuild.
+    clear();
+    curr_depth_ = depth;
+    const int D = depth;
+    const bool dilate = radius >= 1.0f;
+
+    // node-coordinate boxes per level, clipped to the root
+    long long root_end = rootPos[0] + range_[D] - 1;
+    int rr = dilate ? int(ceilf(radius)) : 0;
+    long long lo_v[3], hi_v[3];
+    for (int a = 0; a < 3; a++) {
+        long long re = (long long)rootPos[a] + range_[D] - 1;
+        (void)root_end;
+        long long l = std::isfinite(mn[a]) ? (long long)floorf(std::max(mn[a], 0.0f)) - rr - 1 : 0;
+        long long h = std::isfinite(mx[a]) ? (long long)floorf(std::max(mx[a], 0.0f)) + 2 * rr + 1 : 0;
+        lo_v[a] = std::max<long long>(rootPos[a], l);
+        hi_v[a] = std::min<long long>(re, std::max<long long>(h, lo_v[a]));
+    }
+    if (hi_v[0] / range_[0] > 65535 || hi_v[1] / range_[0] > 65535 || hi_v[2] / range_[0] > 65535)
+        return fail(GVDB_ERR_RANGE, "RebuildTopology: brick coordinates exceed the 16-bit keys of the reference");
+    BitDomain dom[MAXLEV];
+    u64 bm_off[MAXLEV], rk_off[MAXLEV], bm_words = 0, rk_words = 0, tmp_words = 0;
+    for (int L = 0; L < D; L++) {
+        int shift = 0;
+        for (int k = 0; k <= L; k++) shift += logdim_[k];
+        BitDomain& d = dom[L];
+        d.nbits = 1;
+        for (int a = 0; a < 3; a++) {
+            d.lo[a] = int(lo_v[a] >> shift);
+            d.dim[a] = int((hi_v[a] >> shift) - d.lo[a] + 1);
+            d.nbits *= u64(d.dim[a]);
+        }
+        d.nwords = (d.nbits + 31) / 32;
+        bm_off[L] = bm_words;
+        bm_words += d.nwords * (dilate ? 2 : 1);
+        rk_off[L] = rk_words;
+        rk_words += d.nwords * (dilate ? 2 : 1);
+        tmp_words = std::max(tmp_words, scan_tmp_words(d.nwords));
+    }
+    if (dom[0].nbits > (u64(1) << 34))
+        return fail(GVDB_ERR_RANGE, "RebuildTopology: brick box of the points exceeds 2^34 bricks");
+    HIPCHK(bitmaps_.ensure(bm_words * 4 + 256));
+    HIPCHK(ranks_.ensure(rk_words * 4 + 256));
+    HIPCHK(scan_tmp_.ensure(tmp_words * 4 + 256));
+    uint32_t* bm = bitmaps_.as<uint32_t>();
+    uint32_t* rk = ranks_.as<uint32_t>();
+    uint32_t* counts = small + 16;
+    HIPCHK(launch_fill_u32(bm, 0u, bm_words, stream));
+    auto A = [&](int L) { return bm + bm_off[L]; };
+    auto B = [&](int L) { return dilate ? bm + bm_off[L] + dom[L].nwords : nullptr; };
+    HIPCHK(launch_mark_points(P, orig, radius, dilate ? 1 : 0, int(range_[0]), dom[0], A(0), B(0), stream));
+    for (int L = 1; L < D; L++)
+        HIPCHK(launch_mark_parents(dom[L - 1], A(L - 1), B(L - 1), logdim_[L], dom[L], A(L), B(L), stream));
+    for (int L = 0; L < D; L++) {
+        HIPCHK(launch_scan_u32(A(L), nullptr, 1, rk + rk_off[L], dom[L].nwords, counts + 2 * L, scan_tmp_.as<uint32_t>(),
+                               stream));
+        if (dilate)
+            HIPCHK(launch_scan_u32(A(L), B(L), 2, rk + rk_off[L] + dom[L].nwords, dom[L].nwords, counts + 2 * L + 1,
+                                   scan_tmp_.as<uint32_t>(), stream));
+    }
+    HIPCHK(hipMemcpyAsync(pinned_ + 16, counts, 2 * D * 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+
+    LevelBuild LB[MAXLEV];
+    for (int L = 0; L < D; L++) {
+        LB[L].dom = dom[L];
+        LB[L].bmA = A(L);
+        LB[L].bmB = dilate ? B(L) : A(L);
+        LB[L].rankA = rk + rk_off[L];
+        LB[L].rankE = dilate ? rk + rk_off[L] + dom[L].nwords : nullptr;
+        LB[L].countA = pinned_[16 + 2 * L];
+        LB[L].countE = dilate ? pinned_[16 + 2 * L + 1] : 0u;
+        LB[L].lev = L;
+        LB[L].range = int(range_[L]);
+        u64 cnt = u64(LB[L].countA) + LB[L].countE;
+        pool_[0][L].lastEle = pool_[0][L].usedNum = cnt;
+        int rc = ensure_pool_capacity(0, L, std::max<u64>(cnt, 1));
+        if (rc) return rc;
+        if (L > 0) {
+            pool_[1][L].lastEle = pool_[1][L].usedNum = cnt;
+            rc = ensure_pool_capacity(1, L, std::max<u64>(cnt, 1));
+            if (rc) return rc;
+        }
+    }
+    pool_[0][D].lastEle = pool_[0][D].usedNum = 1;
+    pool_[1][D].lastEle = pool_[1][D].usedNum = 1;
+    int rc = ensure_pool_capacity(0, D, 1);
+    if (rc) return rc;
+    rc = ensure_pool_capacity(1, D, 1);
+    if (rc) return rc;
+    compute_top_lev();
+
+    // nodes and child lists
+    HIPCHK(launch_write_root(pool_[0][D].dev.as<Node>(), rootPos, D, stream));
+    HIPCHK(launch_fill_u64(pool_[1][D].dev.as<u64>(), ID_UNDEF64, voxcnt(D), stream));
+    for (int L = 0; L < D; L++) {
+        HIPCHK(launch_emit_nodes(LB[L], pool_[0][L].dev.as<Node>(), L > 0 ? 1 : 0, stream));
+        if (L > 0) HIPCHK(launch_fill_u64(pool_[1][L].dev.as<u64>(), ID_UNDEF64, pool_[1][L].lastEle * voxcnt(L), stream));
+    }
+    // gvdbLinkBricks walks from the top node; levels above it stay unlinked
+    for (int L = 0; L < D; L++) {
+        if (L + 1 > top_lev_) continue;
+        bool toRoot = (L + 1 == D);
+        HIPCHK(launch_link_level(LB[L], toRoot ? nullptr : &LB[L + 1], toRoot ? 1 : 0, logdim_[L + 1],
+                                 pool_[0][L].dev.as<Node>(), pool_[1][L + 1].dev.as<u64>(), voxcnt(L + 1), stream));
+    }
+    root_ = Elem(0, D, 0);
+    rebuild_topo_ = false;
+    host_valid_ = false;
+    dev_stale_ = false;
+    topo_gen_++;
+    stage_end(ST_TOPO);
+    return GVDB_OK;
+}
+
+// -------------------------------------------------------- host topology ----
+bool Volume::pos_in_node(u64 nodeid, const int pos[3], uint32_t& bit) {
+    // getPosInNode (gvdb_volume_gvdb.cpp:2858-2875)
+    Node* c = hnode(nodeid);
+    int lev = c->mLev;
+    long long rg = range_[lev];
+    int res = res_[lev];
+    long long p[3] = {(long long)pos[0] - c->mPos[0], (long long)pos[1] - c->mPos[1], (long long)pos[2] - c->mPos[2]};
+    if (p[0] >= 0 && p[1] >= 0 && p[2] >= 0 && p[0] < rg && p[1] < rg && p[2] < rg) {
+        for (int a = 0; a < 3; a++) p[a] = (p[a] * res) / rg;
+        bit = uint32_t((p[2] * res + p[1]) * res + p[0]);
+        return true;
+    }
+    bit = 0;
+    return false;
+}
+
+u64 Volume::child_node_host(u64 nodeid, uint32_t b) {
+    Node* c = hnode(nodeid);
+    if (c->mChildList == ID_UNDEFL) return ID_UNDEF64;
+    return hclist(c->mChildList)[b];
+}
+
+u64 Volume::insert_child(u64 nodeid, u64 childid, uint32_t i) {
+    // InsertChild, dense child list (gvdb_volume_gvdb.cpp:2980-3027)
+    hnode(childid)->mParent = nodeid;
+    if (hnode(nodeid)->mChildList == ID_UNDEFL) {
+        int lev = hnode(nodeid)->mLev;
+        u64 cl = pool_alloc(1, lev);
+        hnode(nodeid)->mChildList = cl;
+        u64* l = hclist(cl);
+        for (u64 k = 0; k < voxcnt(lev); k++) l[k] = ID_UNDEF64;
+    }
+    hclist(hnode(nodeid)->mChildList)[i] = childid;
+    return childid;
+}
+
+u64 Volume::reparent(int lev, u64 prevroot, const int pos[3], bool& bnew) {
+    // Reparent (gvdb_volume_gvdb.cpp:2726-2763)
+    int pp[3] = {hnode(prevroot)->mPos[0], hnode(prevroot)->mPos[1], hnode(prevroot)->mPos[2]};
+    int p1[3] = {0, 0, 0}, p2[3];
+    bool cover = false;
+    while (!cover) {
+        lev++;
+        if (lev >= levels_) return ID_UNDEFL;
+        covering_node_host(lev, pos, p1);
+        covering_node_host(lev, pp, p2);
+        cover = (p1[0] == p2[0] && p1[1] == p2[1] && p1[2] == p2[2]);
+    }
+    u64 nr = pool_alloc(0, lev);
+    if (nr == ID_UNDEFL) return ID_UNDEFL;
+    setup_node(nr, lev, p1);
+    bool bn = false;
+    activate_space_r(nr, pp, bn, prevroot, 0);
+    u64 leaf = activate_space_r(nr, pos, bnew, ID_UNDEFL, 0);
+    root_ = nr;
+    if (leaf == ID_UNDEFL) return ID_UNDEFL;
+    return hnode(leaf)->mParent;
+}
+
+u64 Volume::activate_space_r(u64 nodeid, const int pos[3], bool& bnew, u64 stopnode, int stoplev) {
+    // ActivateSpace (gvdb_volume_gvdb.cpp:2804-2855)
+    if (root_ == ID_UNDEFL && nodeid == root_) {
+        int p[3];
+        covering_node_host(stoplev, pos, p);
+        root_ = pool_alloc(0, stoplev);
+        setup_node(root_, stoplev, p);
+        nodeid = root_;
+    }
+    uint32_t b;
+    if (pos_in_node(nodeid, pos, b)) {
+        Node* cur = hnode(nodeid);
+        if (stopnode != ID_UNDEFL) {
+            Node* sn = hnode(stopnode);
+            if (pos[0] == sn->mPos[0] && pos[1] == sn->mPos[1] && pos[2] == sn->mPos[2] &&
+                child_node_host(nodeid, b) == ID_UNDEF64 && cur->mLev == sn->mLev + 1)
+                return insert_child(nodeid, stopnode, b);
+        }
+        if (cur->mLev == stoplev) return nodeid;
+        u64 child;
+        if (child_node_host(nodeid, b) == ID_UNDEF64) {
+            int lev = cur->mLev;
+            int ppos[3] = {cur->mPos[0], cur->mPos[1], cur->mPos[2]};
+            // AddChildNode (:2707-2723) with getPosFromBit
+            child = pool_alloc(0, lev - 1);
+            long long rg = range_[lev - 1];
+            int lg = logdim_[lev];
+            uint32_t m = (1u << lg) - 1;
+            int lp[3] = {int(b & m), int((b >> lg) & m), int((b >> (2 * lg)) & m)};
+            int cp[3] = {int(lp[0] * rg + ppos[0]), int(lp[1] * rg + ppos[1]), int(lp[2] * rg + ppos[2])};
+            setup_node(child, lev - 1, cp);
+            insert_child(nodeid, child, b);
+            if (lev == 1) bnew = true;
+        } else {
+            child = child_node_host(nodeid, b);
+        }
+        if (((child >> 8) & 0xFF) == 0) return child;
+        return activate_space_r(child, pos, bnew, stopnode, stoplev);
+    }
+    u64 parent = hnode(nodeid)->mParent;
+    if (parent == ID_UNDEFL) {
+        parent = reparent(hnode(nodeid)->mLev, nodeid, pos, bnew);
+        if (parent == ID_UNDEFL) return ID_UNDEFL;
+    }
+    return activate_space_r(parent, pos, bnew, stopnode, stoplev);
+}
+
+int Volume::rebuild_topology_cpu(int n, const float orig[3], const float* host) {
+    if (levels_ == 0) return fail(GVDB_ERR_STATE, "RebuildTopologyCPU before Configure");
+    if (n < 0 || (n > 0 && !host)) return fail(GVDB_ERR_INVALID, "RebuildTopologyCPU: null positions");
+    int rc = sync_host();
+    if (rc) return rc;
+    for (int i = 0; i < n; i++) {
+        float p[3] = {host[3 * i] + orig[0], host[3 * i + 1] + orig[1], host[3 * i + 2] + orig[2]};
+        int ip[3] = {int(p[0]), int(p[1]), int(p[2])};
+        bool bnew = false;
+        activate_space_r(root_, ip, bnew, ID_UNDEFL, 0);
+    }
+    dev_stale_ = true;
+    topo_gen_++;
+    compute_top_lev();
+    return GVDB_OK;
+}
+
+int Volume::activate_space(const float pos[3]) {
+    if (levels_ == 0) return fail(GVDB_ERR_STATE, "ActivateSpace before Configure");
+    int rc = sync_host();
+    if (rc) return rc;
+    int ip[3] = {int(pos[0]), int(pos[1]), int(pos[2])};
+    bool bnew = false;
+    activate_space_r(root_, ip, bnew, ID_UNDEFL, 0);
+    dev_stale_ = true;
+    topo_gen_++;
+    compute_top_lev();
+    return GVDB_OK;
+}
+
+// ----------------------------------------------------- finish and atlas ----
+int Volume::finish_topology(bool commit, bool bounds) {
+    // FinishTopology (gvdb_volume_gvdb.cpp:1579-1593) with ComputeBounds (:1792-1816)
+    if (levels_ == 0) return fail(GVDB_ERR_STATE, "FinishTopology before Configure");
+    if (bounds) {
+        Pool& p = pool_[0][0];
+        float mnv[3] = {0, 0, 0}, mxv[3] = {0, 0, 0};
+        if (p.lastEle && dev_stale_) {
+            // host-built pools: bounds from the host mirror
+            Node* c = hnode(Elem(0, 0, 0));
+            for (int a = 0; a < 3; a++) mnv[a] = mxv[a] = float(c->mPos[a]);
+            for (u64 i = 0; i < p.lastEle; i++) {
+                c = hnode(Elem(0, 0, i));
+                if (!c->mFlags) continue;
+                for (int a = 0; a < 3; a++) {
+                    if (c->mPos[a] < mnv[a]) mnv[a] = float(c->mPos[a]);
+                    if (c->mPos[a] + range_[0] > mxv[a]) mxv[a] = float(c->mPos[a] + range_[0]);
+                }
+            }
+        } else if (p.lastEle) {
+            int* out6 = reinterpret_cast<int*>(small_.as<uint32_t>() + 40);
+            HIPCHK(launch_leaf_bounds(make_tree(), int(p.lastEle), out6, stream));
+            HIPCHK(hipMemcpyAsync(pinned_ + 40, out6, 24, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            const int* r = reinterpret_cast<const int*>(pinned_ + 40);
+            for (int a = 0; a < 3; a++) { mnv[a] = float(r[a]); mxv[a] = float(r[3 + a]); }
+        }
+        for (int a = 0; a < 3; a++) { objMin_[a] = mnv[a]; objMax_[a] = mxv[a]; }
+    }
+    if (commit) {
+        int rc = commit_pools();
+        if (rc) return rc;
+    }
+    compute_top_lev();
+    return GVDB_OK;
+}
+
+int Volume::resize_atlas(int cz) {
+    // AtlasResize: Z grows, content kept (gvdb_allocator.cpp:622-650)
+    if (cz == atlas_cnt_[2]) return GVDB_OK;
+    int br = brickwid_ + 2 * apron_;
+    u64 slots = u64(atlas_cnt_[0]) * atlas_cnt_[1] * cz;
+    for (auto& ch : chan_) {
+        if (!ch.exists) continue;
+        size_t old = size_t(atlas_max_) * br * br * br * ch.elem();
+        size_t need = size_t(slots) * br * br * br * ch.elem();
+        if (need > ch.data.bytes) {
+            DevBuf nb;
+            HIPCHK(nb.ensure(need));
+            HIPCHK(launch_fill_u32(nb.as<uint32_t>(), 0u, nb.bytes / 4, stream));
+            if (old) HIPCHK(hipMemcpyAsync(nb.p, ch.data.p, std::min(old, need), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            ch.data.release();
+            ch.data = nb;
+        }
+    }
+    atlas_cnt_[2] = cz;
+    atlas_max_ = slots;
+    return GVDB_OK;
+}
+
+Tree Volume::make_tree() {
+    Tree t;
+    memset(&t, 0, sizeof(t));
+    t.levels = levels_;
+    t.top_lev = top_lev_;
+    for (int l = 0; l < levels_; l++) {
+        t.logdim[l] = logdim_[l];
+        t.res[l] = res_[l];
+        t.range[l] = int(range_[l]);
+        t.nodecnt[l] = int(pool_[0][l].lastEle);
+        t.nodes[l] = pool_[0][l].dev.as<Node>();
+        t.clists[l] = l > 0 ? pool_[1][l].dev.as<u64>() : nullptr;
+    }
+    t.amap = amap_.as<AtlasNode>();
+    for (int a = 0; a < 3; a++) t.atlas_cnt[a] = atlas_cnt_[a];
+    t.apron = apron_;
+    t.brickwid = brickwid_;
+    t.brick_res = brickwid_ + 2 * apron_;
+    t.brick_vox = t.brick_res * t.brick_res * t.brick_res;
+    for (int a = 0; a < 3; a++) { t.bmin[a] = objMin_[a]; t.bmax[a] = objMax_[a]; }
+    t.epsilon = epsilon_;
+    return t;
+}
+
+int Volume::update_atlas() {
+    // UpdateAtlas (gvdb_volume_gvdb.cpp:2630-2704)
+    if (!chan_[0].exists) return fail(GVDB_ERR_STATE, "UpdateAtlas: no atlas created");
+    int rc = commit_pools();
+    if (rc) return rc;
+    stage_begin(ST_ATLAS);
+    u64 total = pool_[0][0].lastEle, used = pool_[0][0].usedNum;
+    atlas_used_ = 0;
+    if (used > atlas_max_) {
+        int cz = int(ceil(used / float(atlas_cnt_[0] * atlas_cnt_[1])));
+        rc = resize_atlas(cz);
+        if (rc) return rc;
+    }
+    amap_max_ = u64(atlas_cnt_[0]) * atlas_cnt_[1] * atlas_cnt_[2];
+    HIPCHK(amap_.ensure(size_t(amap_max_) * sizeof(AtlasNode)));
+    HIPCHK(launch_fill_u32(amap_.as<uint32_t>(), 0xFFFFFFFFu, amap_max_ * 4, stream));
+    if (total) {
+        HIPCHK(flags_.ensure(size_t(total) * 4));
+        HIPCHK(slot_scan_.ensure(size_t(total) * 4));
+        HIPCHK(scan_tmp_.ensure(scan_tmp_words(total) * 4 + 256));
+        Tree t = make_tree();
+        HIPCHK(launch_leaf_used(t, flags_.as<uint32_t>(), int(total), stream));
+        HIPCHK(launch_scan_u32(flags_.as<uint32_t>(), nullptr, 0, slot_scan_.as<uint32_t>(), total, nullptr,
+                               scan_tmp_.as<uint32_t>(), stream));
+        HIPCHK(launch_update_atlas(t, slot_scan_.as<uint32_t>(), int(total), stream));
+    }
+    atlas_used_ = total;
+    host_valid_ = false;
+    compute_top_lev();
+    atlas_gen_ = topo_gen_;
+    stage_end(ST_ATLAS);
+    return GVDB_OK;
+}
+
+int Volume::clear_channel(int c) {
+    // ClearChannel -> AtlasFill (gvdb_allocator.cpp:813-822)
+    if (c < 0 || c >= 8 || !chan_[c].exists) return fail(GVDB_ERR_INVALID, "ClearChannel: no such channel");
+    stage_begin(ST_CLEAR);
+    HIPCHK(launch_fill_u32(chan_[c].data.as<uint32_t>(), 0u, chan_[c].data.bytes / 4, stream));
+    stage_end(ST_CLEAR);
+    return GVDB_OK;
+}
+
+// ------------------------------------------------------ points -> voxels ----
+int Volume::insert_points_subcell(int scs, int n, float radius, const float trans[3], int* len) {
+    // InsertPointsSubcell (gvdb_volume_gvdb.cpp:4971-5092)
+    u64 numBrick = pool_[0][0].usedNum;
+    if (numBrick == 0) return fail(GVDB_ERR_NOBRICKS, "Warning: InsertPointsSubcell yet no bricks exist.");
+    if (scs <= 0 || res_[0] % scs != 0) return fail(GVDB_ERR_INVALID, "InsertPointsSubcell: subcell size must divide the brick");
+    if (n < 0 || n > pts_.n || (n > 0 && !pts_.base)) return fail(GVDB_ERR_INVALID, "InsertPointsSubcell: point count exceeds SetPoints");
+    if (!(radius >= 0.0f) || radius > 32.0f) return fail(GVDB_ERR_UNSUPPORTED, "InsertPointsSubcell: radius must be in [0, 32]");
+    if (atlas_gen_ != topo_gen_) return fail(GVDB_ERR_STATE, "InsertPointsSubcell: call UpdateAtlas after the topology changes");
+    int rc = commit_pools();
+    if (rc) return rc;
+    stage_begin(ST_INSERT);
+    int sd = res_[0] / scs;
+    int spb = sd * sd * sd;
+    u64 numSC = u64(spb) * numBrick;
+    u64 numMap = pool_[0][0].lastEle;
+    if (numSC > 0xFFFFFFFFull) return fail(GVDB_ERR_RANGE, "InsertPointsSubcell: too many subcells");
+    HIPCHK(sc_flags_.ensure(size_t(numMap) * 4));
+    HIPCHK(sc_map_.ensure(size_t(numMap) * 4));
+    HIPCHK(sc_cnt_.ensure(size_t(numSC) * 4));
+    HIPCHK(sc_off_.ensure(size_t(numSC) * 4));
+    HIPCHK(scan_tmp_.ensure(scan_tmp_words(std::max(numSC, numMap)) * 4 + 256));
+    Tree t = make_tree();
+    uint32_t* small = small_.as<uint32_t>();
+    PointSrc P = pts_;
+    P.n = n;
+    HIPCHK(launch_leaf_flags(t, sc_flags_.as<uint32_t>(), int(numMap), stream));
+    HIPCHK(launch_scan_u32(sc_flags_.as<uint32_t>(), nullptr, 0, sc_map_.as<uint32_t>(), numMap, small + 8,
+                           scan_tmp_.as<uint32_t>(), stream));
+    HIPCHK(launch_fill_u32(sc_cnt_.as<uint32_t>(), 0u, numSC, stream));
+    HIPCHK(launch_count_subcell(t, P, scs, spb, trans, radius, sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(), stream));
+    HIPCHK(launch_scan_u32(sc_cnt_.as<uint32_t>(), nullptr, 0, sc_off_.as<uint32_t>(), numSC, small + 9,
+                           scan_tmp_.as<uint32_t>(), stream));
+    HIPCHK(hipMemcpyAsync(pinned_ + 8, small + 8, 8, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    uint32_t linked = pinned_[8];
+    uint32_t total = pinned_[9];
+    if (total > 0x7FFFFFFFu) return fail(GVDB_ERR_RANGE, "InsertPointsSubcell: more than 2^31 subcell entries");
+    *len = int(total);
+    sc_len_ = int(total);
+    num_sc_ = int(numSC);
+    sc_bricks_ = int(linked);
+    sc_map_n_ = int(numMap);
+    sc_gen_ = topo_gen_;
+    if (total == 0) { stage_end(ST_INSERT); return GVDB_OK; }
+    HIPCHK(sc_pnt_.ensure(size_t(total) * 12));
+    HIPCHK(sc_pos_.ensure(size_t(numSC) * 12));
+    HIPCHK(sc_nid_.ensure(size_t(numSC) * 4));
+    HIPCHK(launch_fill_u32(sc_cnt_.as<uint32_t>(), 0u, numSC, stream));
+    HIPCHK(launch_insert_subcell(t, P, scs, spb, trans, radius, sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
+                                 sc_off_.as<uint32_t>(), sc_pnt_.as<float>(), stream));
+    HIPCHK(launch_subcell_pos(t, scs, spb, sc_map_.as<uint32_t>(), sc_pos_.as<int>(), sc_nid_.as<int>(), int(numMap),
+                              stream));
+    stage_end(ST_INSERT);
+    return GVDB_OK;
+}
+
+int Volume::gather_density(int scs, int n, float radius, const float trans[3], int len, int chanD, int chanC, bool accum) {
+    // GatherDensity (gvdb_volume_gvdb.cpp:5318-5344)
+    (void)n; (void)trans; (void)chanC;
+    if (len == 0) return GVDB_OK;
+    if (pool_[0][0].usedNum == 0) return GVDB_OK;
+    if (chanD < 0 || chanD >= 8 || !chan_[chanD].exists) return fail(GVDB_ERR_INVALID, "GatherDensity: no such channel");
+    if (len != sc_len_ || scs <= 0 || res_[0] % scs) return fail(GVDB_ERR_STATE, "GatherDensity: call InsertPointsSubcell first");
+    if (atlas_gen_ != topo_gen_ || sc_gen_ != topo_gen_) return fail(GVDB_ERR_STATE, "GatherDensity: topology changed since UpdateAtlas/InsertPointsSubcell");
+    int rc = commit_pools();
+    if (rc) return rc;
+    stage_begin(ST_GATHER);
+    Tree t = make_tree();
+    (void)hipEventRecord(kev_[0][0], stream);
+    HIPCHK(launch_gather_density(t, scs, sc_bricks_, radius, sc_nid_.as<int>(), sc_cnt_.as<uint32_t>(),
+                                 sc_off_.as<uint32_t>(), sc_pos_.as<int>(), sc_pnt_.as<float>(), chan_[chanD].data.p,
+                                 chan_[chanD].type, accum ? 1 : 0, stream));
+    (void)hipEventRecord(kev_[0][1], stream);
+    kev_used_[0] = true;
+    stage_end(ST_GATHER);
+    return GVDB_OK;
+}
+
+int Volume::update_apron(int c, float boundval) {
+    // UpdateApron (gvdb_volume_gvdb.cpp:4418-4457)
+    if (apron_ == 0) return GVDB_OK;
+    if (c < 0 || c >= 8 || !chan_[c].exists) return fail(GVDB_ERR_INVALID, "UpdateApron: no such channel");
+    int rc = commit_pools();
+    if (rc) return rc;
+    int bricks = int(pool_[0][0].lastEle);
+    if (bricks == 0) return GVDB_OK;
+    if (atlas_gen_ != topo_gen_) return fail(GVDB_ERR_STATE, "UpdateApron: call UpdateAtlas after the topology changes");
+    stage_begin(ST_APRON);
+    Tree t = make_tree();
+    if (nbr_gen_ != topo_gen_) {
+        HIPCHK(nbr_.ensure(size_t(bricks) * 27 * 4));
+        HIPCHK(launch_neighbors(t, nbr_.as<int>(), bricks, stream));
+        nbr_gen_ = topo_gen_;
+    }
+    (void)hipEventRecord(kev_[1][0], stream);
+    HIPCHK(launch_update_apron(t, nbr_.as<int>(), bricks, chan_[c].data.p, chan_[c].type, boundval, stream));
+    (void)hipEventRecord(kev_[1][1], stream);
+    kev_used_[1] = true;
+    stage_end(ST_APRON);
+    return GVDB_OK;
+}
+
+// ---------------------------------------------------------------- render ----
+int Volume::linear_tf(float t0, float t1, const float a[4], const float b[4]) {
+    // Scene::LinearTransferFunc (gvdb_scene.cpp:66-88)
+    int n0 = int(t0 * float(TF_SIZE)), n1 = int(t1 * float(TF_SIZE));
+    n0 = std::max(0, n0);
+    n1 = std::min(TF_SIZE, n1);
+    for (int k = n0; k < n1; k++) {
+        float x = float(k - n0) / float(n1 - n0);
+        for (int c = 0; c < 4; c++) tf_host[4 * k + c] = a[c] + x * (b[c] - a[c]);
+    }
+    tf_committed = false;
+    return GVDB_OK;
+}
+
+int Volume::commit_tf() {
+    HIPCHK(tf_dev.ensure(size_t(TF_SIZE) * 16));
+    HIPCHK(hipMemcpyAsync(tf_dev.p, tf_host.data(), size_t(TF_SIZE) * 16, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    tf_committed = true;
+    return GVDB_OK;
+}
+
+int Volume::add_render_buf(int buf, int w, int h, int bpp) {
+    if (buf < 0 || buf >= 4 || w <= 0 || h <= 0 || (bpp != 4 && bpp != 16))
+        return fail(GVDB_ERR_INVALID, "AddRenderBuf: buffer 0..3, positive size, 4 or 16 bytes per pixel");
+    RenderBuf& r = rbuf_[buf];
+    r.w = w; r.h = h; r.bpp = bpp;
+    HIPCHK(r.data.ensure(size_t(w) * h * bpp));
+    return GVDB_OK;
+}
+
+int Volume::render(int shading, int c, int buf) {
+    // Render (gvdb_volume_gvdb.cpp:4336-4381)
+    if (buf < 0 || buf >= 4 || rbuf_[buf].w == 0) return fail(GVDB_ERR_INVALID, "Render: no such render buffer");
+    if (c < 0 || c >= 8 || !chan_[c].exists) return fail(GVDB_ERR_INVALID, "Render: no such channel");
+    if (shading != SHADE_VOLUME && shading != SHADE_TRILINEAR) return fail(GVDB_ERR_UNSUPPORTED, "Render: shading must be SHADE_VOLUME or SHADE_TRILINEAR");
+    if (pool_[0][0].lastEle == 0 || amap_max_ == 0) return fail(GVDB_ERR_STATE, "Render: no topology or atlas");
+    if (atlas_gen_ != topo_gen_) return fail(GVDB_ERR_STATE, "Render: call UpdateAtlas after the topology changes");
+    int rc = commit_pools();
+    if (rc) return rc;
+    if (!tf_committed) {
+        rc = commit_tf();
+        if (rc) return rc;
+    }
+    RenderBuf& r = rbuf_[buf];
+    ScnBlock s;
+    scene_block(r.w, r.h, s);
+    stage_begin(ST_RENDER);
+    Tree t = make_tree();
+    (void)hipEventRecord(kev_[2][0], stream);
+    HIPCHK(launch_render(t, s, tf_dev.as<float4>(), chan_[c].data.p, chan_[c].type, shading, r.data.p, r.bpp == 16,
+                         stream));
+    (void)hipEventRecord(kev_[2][1], stream);
+    kev_used_[2] = true;
+    stage_end(ST_RENDER);
+    return GVDB_OK;
+}
+
+int Volume::read_render_buf(int buf, void* dst) {
+    if (buf < 0 || buf >= 4 || rbuf_[buf].w == 0 || !dst) return fail(GVDB_ERR_INVALID, "ReadRenderBuf: no such buffer");
+    RenderBuf& r = rbuf_[buf];
+    HIPCHK(hipMemcpyAsync(dst, r.data.p, size_t(r.w) * r.h * r.bpp, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return GVDB_OK;
+}
+
+// ---------------------------------------------------------------- access ----
+int Volume::get_pool(int grp, int lev, void* dst, u64* count, u64* stride) {
+    if (grp < 0 || grp > 1 || lev < 0 || lev >= levels_) return fail(GVDB_ERR_INVALID, "get_pool: bad group or level");
+    Pool& p = pool_[grp][lev];
+    if (count) *count = p.exists ? p.lastEle : 0;
+    if (stride) *stride = p.stride;
+    if (!dst || !p.exists || !p.lastEle) return GVDB_OK;
+    int rc = sync_host();
+    if (rc) return rc;
+    memcpy(dst, p.host.data(), p.lastEle * p.stride);
+    return GVDB_OK;
+}
+
+int Volume::get_info(int* out2, float* outf) {
+    if (out2) { out2[0] = top_lev_; out2[1] = curr_depth_; }
+    if (outf)
+        for (int a = 0; a < 3; a++) {
+            outf[a] = objMin_[a]; outf[3 + a] = objMax_[a]; outf[6 + a] = posMin_[a]; outf[9 + a] = posMax_[a];
+        }
+    return GVDB_OK;
+}
+
+int Volume::get_atlas_info(int c, int* out) {
+    if (c < 0 || c >= 8 || !chan_[c].exists) return fail(GVDB_ERR_INVALID, "atlas info: no such channel");
+    int br = brickwid_ + 2 * apron_;
+    for (int a = 0; a < 3; a++) { out[a] = atlas_cnt_[a]; out[3 + a] = atlas_cnt_[a] * br; }
+    out[6] = int(atlas_used_);
+    return GVDB_OK;
+}
+
+int Volume::read_atlas(int c, void* dst) {
+    // Device brick-major storage -> GVDB's 3D atlas layout (x fastest)
+    if (c < 0 || c >= 8 || !chan_[c].exists || !dst) return fail(GVDB_ERR_INVALID, "read_atlas: no such channel");
+    Channel& ch = chan_[c];
+    int br = brickwid_ + 2 * apron_;
+    u64 slots = atlas_max_;
+    size_t bv = size_t(br) * br * br;
+    std::vector<uint8_t> tmp(size_t(slots) * bv * ch.elem());
+    HIPCHK(hipMemcpyAsync(tmp.data(), ch.data.p, tmp.size(), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    int rx = atlas_cnt_[0] * br, ry = atlas_cnt_[1] * br;
+    int e = ch.elem();
+    uint8_t* out = reinterpret_cast<uint8_t*>(dst);
+    for (u64 s = 0; s < slots; s++) {
+        int sx = int(s % atlas_cnt_[0]), sy = int((s / atlas_cnt_[0]) % atlas_cnt_[1]), sz = int(s / (u64(atlas_cnt_[0]) * atlas_cnt_[1]));
+        for (int z = 0; z < br; z++)
+            for (int y = 0; y < br; y++) {
+                size_t src = (s * bv + size_t(z * br + y) * br) * e;
+                size_t d = ((size_t(sz * br + z) * ry + (sy * br + y)) * rx + sx * br) * e;
+                memcpy(out + d, tmp.data() + src, size_t(br) * e);
+            }
+    }
+    return GVDB_OK;
+}
+
+int Volume::read_atlas_map(void* dst) {
+    if (!dst) return fail(GVDB_ERR_INVALID, "read_atlas_map: null destination");
+    if (!amap_max_) return GVDB_OK;
+    HIPCHK(hipMemcpyAsync(dst, amap_.p, size_t(amap_max_) * sizeof(AtlasNode), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return GVDB_OK;
+}
+
+int Volume::read_subcells(int which, void* dst, u64* count) {
+    size_t n = 0, esz = 4;
+    DevBuf* b = nullptr;
+    switch (which) {
+        case 0: b = &sc_cnt_; n = num_sc_; break;
+        case 1: b = &sc_off_; n = num_sc_; break;
+        case 2: b = &sc_pos_; n = num_sc_; esz = 12; break;
+        case 3: b = &sc_nid_; n = num_sc_; break;
+        case 4: b = &sc_pnt_; n = sc_len_; esz = 12; break;
+        case 5: b = &sc_map_; n = sc_map_n_; break;
+        default: return fail(GVDB_ERR_INVALID, "read_subcells: which must be 0..5");
+    }
+    if (count) *count = n;
+    if (!dst || !n) return GVDB_OK;
+    HIPCHK(hipMemcpyAsync(dst, b->p, n * esz, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return GVDB_OK;
+}
+
+}  // namespace gvdb

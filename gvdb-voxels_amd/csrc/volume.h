@@ -1,0 +1,191 @@
+// volume.h -- host runtime of one grid: pools, atlas, aux buffers, scene.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "gvdb_device.h"
+#include "kernels.h"
+
+namespace gvdb {
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n, bool preserve = false, hipStream_t s = nullptr);
+    void release();
+    template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+struct Pool {
+    bool exists = false;
+    u64 stride = 0;      // bytes per element
+    u64 max = 0;         // capacity in elements (doubling growth, as the reference's PoolAlloc)
+    u64 lastEle = 0;
+    u64 usedNum = 0;
+    std::vector<uint8_t> host;   // host mirror (host-built topology, readback)
+    DevBuf dev;
+};
+
+struct Channel {
+    bool exists = false;
+    int type = T_FLOAT;
+    int filter = 0, border = 0;
+    DevBuf data;
+    int elem() const { return type == T_HALF ? 2 : 4; }
+};
+
+struct RenderBuf {
+    int w = 0, h = 0, bpp = 4;
+    DevBuf data;
+};
+
+struct CameraParams {
+    float angs[3] = {0, 45, 0};
+    float to[3] = {0, 0, 0};
+    float dist = 120.f, dolly = 1.f, fov = 40.f, znear = 0.1f, zfar = 5000.f;
+};
+
+enum Stage { ST_TOPO = 0, ST_ATLAS, ST_INSERT, ST_GATHER, ST_APRON, ST_RENDER, ST_CLEAR, ST_COUNT };
+
+class Volume {
+public:
+    explicit Volume(int device);
+    ~Volume();
+    int init();
+
+    std::string err;
+    int fail(int code, const std::string& msg) { err = msg; return code; }
+
+    // configuration
+    int configure(int levels, const int* log2dims, const int* initcnt);
+    int set_channel_default(int x, int y, int z);
+    int add_channel(int chan, int dtype, int apron, int filter, int border, int ax, int ay, int az);
+    int set_epsilon(float eps, int maxiter);
+
+    // points
+    int set_points_gpu(const void* p, int n, int off, int stride);
+    int upload_points(const float* host, int n);
+
+    // topology
+    int request_full_rebuild(bool on) { if (on) rebuild_topo_ = true; return 0; }
+    int rebuild_topology(int n, float radius, const float orig[3]);
+    int rebuild_topology_cpu(int n, const float orig[3], const float* host);
+    int activate_space(const float pos[3]);
+    int clear();
+    int finish_topology(bool commit, bool bounds);
+    int update_atlas();
+    int clear_channel(int chan);
+
+    // points -> voxels
+    int insert_points_subcell(int scs, int n, float radius, const float trans[3], int* len);
+    int gather_density(int scs, int n, float radius, const float trans[3], int len, int chanD, int chanC, bool accum);
+    int update_apron(int chan, float boundval);
+
+    // render
+    CameraParams cam;
+    float light_[3] = {0, 84.85f, 84.85f};
+    float back_[4] = {0, 0, 0, 1};
+    float extinct_[3] = {-1.1f, 1.5f, 0.f}, steps_[3] = {1.f, 16.f, 0.1f}, cutoff_[3] = {0.005f, 0.01f, 0.f};
+    float vrange_[3] = {0.1f, 0.f, 1.f}, shadow_[3] = {0.8f, 1.f, 0.f};
+    std::vector<float> tf_host;
+    DevBuf tf_dev;
+    bool tf_committed = false;
+    int linear_tf(float t0, float t1, const float a[4], const float b[4]);
+    int commit_tf();
+    int add_render_buf(int buf, int w, int h, int bpp);
+    int render(int shading, int chan, int buf);
+    int read_render_buf(int buf, void* dst);
+    void scene_block(int w, int h, ScnBlock& s) const;
+
+    // access
+    int get_pool(int grp, int lev, void* dst, u64* count, u64* stride);
+    int get_info(int* out2, float* outf);
+    int get_atlas_info(int chan, int* out7);
+    int read_atlas(int chan, void* dst);
+    int read_atlas_map(void* dst);
+    int read_subcells(int which, void* dst, u64* count);
+    int stage_ms(float* out);
+    int kernel_ms(float* out);
+
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+private:
+    // pools
+    int levels_ = 0;
+    int logdim_[MAXLEV] = {0}, res_[MAXLEV] = {0};
+    long long range_[MAXLEV] = {0};
+    Pool pool_[2][MAXLEV];
+    u64 root_ = ID_UNDEFL;
+    bool rebuild_topo_ = true;
+    int curr_depth_ = -1;
+    int top_lev_ = 1;
+    bool host_valid_ = true;   // host mirrors hold the current pools
+    bool dev_stale_ = false;   // device pools are older than the host mirrors
+    unsigned topo_gen_ = 0;    // bumps on every topology change
+    unsigned atlas_gen_ = ~0u; // topology generation UpdateAtlas last ran on
+    unsigned sc_gen_ = ~0u;    // topology generation InsertPointsSubcell last ran on
+    float posMin_[3] = {0, 0, 0}, posMax_[3] = {0, 0, 0};
+    float objMin_[3] = {0, 0, 0}, objMax_[3] = {0, 0, 0};
+    float epsilon_ = 0.001f;
+    int maxiter_ = 256;
+
+    // atlas
+    int default_axiscnt_[3] = {16, 16, 1};
+    int atlas_cnt_[3] = {0, 0, 0};
+    int apron_ = 0;
+    int brickwid_ = 0;
+    u64 atlas_max_ = 0, atlas_used_ = 0;
+    Channel chan_[8];
+    DevBuf amap_;
+    u64 amap_max_ = 0;
+    DevBuf nbr_;
+    unsigned nbr_gen_ = ~0u;
+
+    // points
+    PointSrc pts_{nullptr, 0, 0, 12};
+    DevBuf pts_owned_;
+
+    // subcells
+    DevBuf sc_cnt_, sc_off_, sc_pos_, sc_nid_, sc_map_, sc_pnt_, sc_flags_;
+    int sc_len_ = 0, num_sc_ = 0, sc_bricks_ = 0, sc_map_n_ = 0;
+
+    // topology scratch
+    DevBuf small_;       // bbox keys, counts
+    DevBuf bitmaps_, ranks_, scan_tmp_, flags_, slot_scan_;
+    uint32_t* pinned_ = nullptr;   // host staging (64 words)
+
+    // render
+    RenderBuf rbuf_[4];
+
+    // timing
+    hipEvent_t ev_[ST_COUNT][2] = {};
+    bool ev_used_[ST_COUNT] = {};
+    hipEvent_t kev_[4][2] = {};
+    bool kev_used_[4] = {};
+
+    int hip_fail(hipError_t e, const char* what);
+    void stage_begin(Stage s);
+    void stage_end(Stage s);
+    u64 pool_alloc(int grp, int lev);
+    int ensure_pool_capacity(int grp, int lev, u64 need);
+    int sync_host();
+    int commit_pools();
+    Tree make_tree();
+    void setup_node(u64 id, int lev, const int pos[3]);
+    void compute_top_lev();
+    int determine_depth(int root_pos[3]);
+    void covering_node_host(int lev, const int pos[3], int out[3]) const;
+    u64 activate_space_r(u64 nodeid, const int pos[3], bool& bnew, u64 stopnode, int stoplev);
+    u64 reparent(int lev, u64 prevroot, const int pos[3], bool& bnew);
+    u64 insert_child(u64 nodeid, u64 childid, uint32_t i);
+    bool pos_in_node(u64 nodeid, const int pos[3], uint32_t& bit);
+    u64 child_node_host(u64 nodeid, uint32_t b);
+    Node* hnode(u64 ref);
+    u64* hclist(u64 ref);
+    int resize_atlas(int cz);
+    u64 voxcnt(int lev) const { u64 r = u64(1) << logdim_[lev]; return r * r * r; }
+};
+
+}  // namespace gvdb

@@ -1,0 +1,280 @@
+"""Python binding of libgvdb_amd (ctypes over the C ABI in include/gvdb_amd.h).
+
+`VolumeGVDB` mirrors the reference's nvdb::VolumeGVDB methods on the point
+path (source/gvdb_library/src/gvdb_volume_gvdb.h:319-901 of NVIDIA GVDB
+Voxels 1.1.1) with the same names and argument meaning.  Methods raise
+GVDBError on a negative status; the library has no CPU fallback, so a missing
+shared library or device is an error.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libgvdb_amd.so")
+
+OK = 0
+ERR_INVALID, ERR_HIP, ERR_STATE, ERR_DEPTH, ERR_NOBRICKS, ERR_RANGE, ERR_UNSUPPORTED, ERR_OOM = -1, -2, -3, -4, -5, -6, -7, -8
+T_FLOAT, T_HALF = 3, 100
+SHADE_TRILINEAR, SHADE_VOLUME = 4, 7
+SCN_FLOATS = 68
+
+NODE_DTYPE = np.dtype([
+    ("mLev", "u1"), ("mFlags", "u1"), ("mPriority", "u1"), ("pad", "u1"),
+    ("mPos", "<i4", 3), ("mValue", "<i4", 3), ("mVRange", "<f4", 3),
+    ("mParent", "<u8"), ("mChildList", "<u8"), ("mMask", "<u8")])
+ATLASNODE_DTYPE = np.dtype([("mPos", "<i4", 3), ("mLeafNode", "<i4")])
+
+# (name, restype, argtypes) of every C entry point
+_vp, _i, _f, _u64p = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_uint64)
+_SIGS = [
+    ("gvdb_create", _i, [ctypes.POINTER(_vp), _i]), ("gvdb_destroy", _i, [_vp]),
+    ("gvdb_last_error", ctypes.c_char_p, [_vp]),
+    ("gvdb_configure", _i, [_vp, _i, _vp, _vp]), ("gvdb_set_channel_default", _i, [_vp, _i, _i, _i]),
+    ("gvdb_add_channel", _i, [_vp, _i, _i, _i, _i, _i, _i, _i, _i]), ("gvdb_set_epsilon", _i, [_vp, _f, _i]),
+    ("gvdb_set_points_gpu", _i, [_vp, _vp, _i, _i, _i]), ("gvdb_upload_points", _i, [_vp, _vp, _i]),
+    ("gvdb_request_full_rebuild", _i, [_vp, _i]), ("gvdb_rebuild_topology", _i, [_vp, _i, _f, _vp]),
+    ("gvdb_rebuild_topology_cpu", _i, [_vp, _i, _vp, _vp]), ("gvdb_activate_space", _i, [_vp, _vp]),
+    ("gvdb_clear", _i, [_vp]), ("gvdb_finish_topology", _i, [_vp, _i, _i]), ("gvdb_update_atlas", _i, [_vp]),
+    ("gvdb_clear_channel", _i, [_vp, _i]),
+    ("gvdb_insert_points_subcell", _i, [_vp, _i, _i, _f, _vp, ctypes.POINTER(_i)]),
+    ("gvdb_gather_density", _i, [_vp, _i, _i, _f, _vp, _i, _i, _i, _i]), ("gvdb_update_apron", _i, [_vp, _i, _f]),
+    ("gvdb_set_camera_orbit", _i, [_vp, _vp, _vp, _f, _f, _f, _f, _f]), ("gvdb_set_light", _i, [_vp, _vp]),
+    ("gvdb_set_background", _i, [_vp, _vp]), ("gvdb_set_extinct", _i, [_vp, _f, _f, _f]),
+    ("gvdb_set_steps", _i, [_vp, _f, _f, _f]), ("gvdb_set_cutoff", _i, [_vp, _f, _f, _f]),
+    ("gvdb_set_volume_range", _i, [_vp, _f, _f, _f]), ("gvdb_set_shadow_params", _i, [_vp, _f, _f, _f]),
+    ("gvdb_linear_transfer_func", _i, [_vp, _f, _f, _vp, _vp]), ("gvdb_commit_transfer_func", _i, [_vp]),
+    ("gvdb_add_render_buf", _i, [_vp, _i, _i, _i, _i]), ("gvdb_render", _i, [_vp, _i, _i, _i]),
+    ("gvdb_read_render_buf", _i, [_vp, _i, _vp]), ("gvdb_get_scene_block", _i, [_vp, _i, _i, _vp]),
+    ("gvdb_get_pool", _i, [_vp, _i, _i, _vp, _u64p, _u64p]), ("gvdb_get_info", _i, [_vp, _vp, _vp]),
+    ("gvdb_get_atlas_info", _i, [_vp, _i, _vp]), ("gvdb_read_atlas", _i, [_vp, _i, _vp]),
+    ("gvdb_read_atlas_map", _i, [_vp, _vp]), ("gvdb_read_subcells", _i, [_vp, _i, _vp, _u64p]),
+    ("gvdb_get_stream", _i, [_vp, ctypes.POINTER(_vp)]), ("gvdb_synchronize", _i, [_vp]),
+    ("gvdb_stage_ms", _i, [_vp, _vp]), ("gvdb_kernel_ms", _i, [_vp, _vp]),
+]
+
+_lib = None
+
+
+class GVDBError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (status %d)" % (msg, code))
+        self.code = code
+
+
+def load():
+    """Loads libgvdb_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError("libgvdb_amd.so not built: run gvdb-voxels_amd/build.py (%s)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_names():
+    return [s[0] for s in _SIGS]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f3(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(3))
+
+
+class VolumeGVDB:
+    """One grid on one GPU (VolumeGVDB + Initialize)."""
+
+    def __init__(self, device=0):
+        L = load()
+        h = ctypes.c_void_p()
+        rc = L.gvdb_create(ctypes.byref(h), int(device))
+        if rc != OK:
+            raise GVDBError(rc, "gvdb_create failed (no HIP device %d?)" % device)
+        self.h = h
+        self._keep = []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().gvdb_destroy(self.h)
+            self.h = None
+
+    def close(self):
+        self.__del__()
+
+    def _chk(self, rc, allow=()):
+        if rc != OK and rc not in allow:
+            raise GVDBError(rc, load().gvdb_last_error(self.h).decode())
+        return rc
+
+    # setup -----------------------------------------------------------------
+    def Configure(self, q4, q3, q2, q1, q0):
+        dims = np.asarray([q0, q1, q2, q3, q4], np.int32)
+        return self._chk(load().gvdb_configure(self.h, 5, _p(dims), None))
+
+    def ConfigureLevels(self, log2dims, initcnt=None):
+        dims = np.asarray(log2dims, np.int32)
+        cnt = None if initcnt is None else _p(np.asarray(initcnt, np.int32))
+        return self._chk(load().gvdb_configure(self.h, len(dims), _p(dims), cnt))
+
+    def SetChannelDefault(self, x, y, z):
+        return self._chk(load().gvdb_set_channel_default(self.h, x, y, z))
+
+    def AddChannel(self, chan, dtype=T_FLOAT, apron=1, filter=0, border=0, axiscnt=(0, 0, 0)):
+        return self._chk(load().gvdb_add_channel(self.h, chan, dtype, apron, filter, border, *axiscnt))
+
+    def SetEpsilon(self, eps, maxiter=256):
+        return self._chk(load().gvdb_set_epsilon(self.h, float(eps), int(maxiter)))
+
+    # points ----------------------------------------------------------------
+    def SetPoints(self, pts):
+        """Uploads host float3 positions (AllocData + CommitData + SetPoints)."""
+        pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+        return self._chk(load().gvdb_upload_points(self.h, _p(pts), len(pts)))
+
+    def SetDataGPU(self, dev_ptr, n, offset=0, stride=12):
+        return self._chk(load().gvdb_set_points_gpu(self.h, ctypes.c_void_p(dev_ptr), n, offset, stride))
+
+    # topology --------------------------------------------------------------
+    def RequestFullRebuild(self, on=True):
+        return self._chk(load().gvdb_request_full_rebuild(self.h, int(on)))
+
+    def RebuildTopology(self, n, radius, orig=(0, 0, 0)):
+        return self._chk(load().gvdb_rebuild_topology(self.h, int(n), float(radius), _p(_f3(orig))))
+
+    def RebuildTopologyCPU(self, pts, orig=(0, 0, 0)):
+        pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+        return self._chk(load().gvdb_rebuild_topology_cpu(self.h, len(pts), _p(_f3(orig)), _p(pts)))
+
+    def ActivateSpace(self, pos):
+        return self._chk(load().gvdb_activate_space(self.h, _p(_f3(pos))))
+
+    def Clear(self):
+        return self._chk(load().gvdb_clear(self.h))
+
+    def FinishTopology(self, commit=True, bounds=True):
+        return self._chk(load().gvdb_finish_topology(self.h, int(commit), int(bounds)))
+
+    def UpdateAtlas(self):
+        return self._chk(load().gvdb_update_atlas(self.h))
+
+    def ClearChannel(self, chan):
+        return self._chk(load().gvdb_clear_channel(self.h, chan))
+
+    # points -> voxels ------------------------------------------------------
+    def InsertPointsSubcell(self, subcell_size, n, radius, trans=(0, 0, 0)):
+        ln = ctypes.c_int(0)
+        self._chk(load().gvdb_insert_points_subcell(self.h, subcell_size, int(n), float(radius), _p(_f3(trans)),
+                                                    ctypes.byref(ln)))
+        return ln.value
+
+    def GatherDensity(self, subcell_size, n, radius, trans, sc_len, chan_density=0, chan_clr=0, accumulate=False):
+        return self._chk(load().gvdb_gather_density(self.h, subcell_size, int(n), float(radius), _p(_f3(trans)),
+                                                    int(sc_len), chan_density, chan_clr, int(accumulate)))
+
+    def UpdateApron(self, chan=0, boundval=0.0):
+        return self._chk(load().gvdb_update_apron(self.h, chan, float(boundval)))
+
+    # scene / render --------------------------------------------------------
+    def apply_scene(self, sc):
+        L = load()
+        c = sc["cam"]
+        self._chk(L.gvdb_set_camera_orbit(self.h, _p(_f3(c["angs"])), _p(_f3(c["to"])), float(c["dist"]),
+                                          float(c["dolly"]), float(c["fov"]), float(c["near"]), float(c["far"])))
+        self._chk(L.gvdb_set_light(self.h, _p(_f3(sc["light"]))))
+        self._chk(L.gvdb_set_background(self.h, _p(np.asarray(sc["backclr"], np.float32))))
+        self._chk(L.gvdb_set_extinct(self.h, *map(float, sc["extinct"])))
+        self._chk(L.gvdb_set_steps(self.h, *map(float, sc["steps"])))
+        self._chk(L.gvdb_set_cutoff(self.h, *map(float, sc["cutoff"])))
+        self._chk(L.gvdb_set_volume_range(self.h, *map(float, sc["vrange"])))
+        self._chk(L.gvdb_set_shadow_params(self.h, *map(float, sc["shadow"])))
+        t0, t1, a, b = sc["tf"]
+        self._chk(L.gvdb_linear_transfer_func(self.h, float(t0), float(t1), _p(np.asarray(a, np.float32)),
+                                              _p(np.asarray(b, np.float32))))
+        return self._chk(L.gvdb_commit_transfer_func(self.h))
+
+    def AddRenderBuf(self, buf, w, h, bytes_per_pixel=4):
+        return self._chk(load().gvdb_add_render_buf(self.h, buf, w, h, bytes_per_pixel))
+
+    def Render(self, shading, chan=0, buf=0):
+        return self._chk(load().gvdb_render(self.h, shading, chan, buf))
+
+    def ReadRenderBuf(self, buf, w, h, bytes_per_pixel=4):
+        out = np.zeros((h, w, 4), np.float32 if bytes_per_pixel == 16 else np.uint8)
+        self._chk(load().gvdb_read_render_buf(self.h, buf, _p(out)))
+        return out
+
+    def scene_block(self, w, h):
+        s = np.zeros(SCN_FLOATS, np.float32)
+        self._chk(load().gvdb_get_scene_block(self.h, w, h, _p(s)))
+        return s
+
+    # access ----------------------------------------------------------------
+    def pool(self, grp, lev):
+        L = load()
+        cnt, stride = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        self._chk(L.gvdb_get_pool(self.h, grp, lev, None, ctypes.byref(cnt), ctypes.byref(stride)))
+        buf = np.zeros(int(cnt.value * stride.value), np.uint8)
+        if cnt.value:
+            self._chk(L.gvdb_get_pool(self.h, grp, lev, _p(buf), ctypes.byref(cnt), ctypes.byref(stride)))
+        if grp == 0:
+            return buf.view(NODE_DTYPE)
+        return buf.view(np.uint64).reshape(int(cnt.value), -1) if cnt.value else np.zeros((0, 0), np.uint64)
+
+    def info(self):
+        i2 = np.zeros(2, np.int32)
+        f = np.zeros(12, np.float32)
+        self._chk(load().gvdb_get_info(self.h, _p(i2), _p(f)))
+        return dict(top_lev=int(i2[0]), depth=int(i2[1]), obj_min=f[0:3], obj_max=f[3:6], pos_min=f[6:9],
+                    pos_max=f[9:12])
+
+    def atlas_info(self, chan=0):
+        o = np.zeros(7, np.int32)
+        self._chk(load().gvdb_get_atlas_info(self.h, chan, _p(o)))
+        return dict(cnt=tuple(int(x) for x in o[0:3]), res=tuple(int(x) for x in o[3:6]), used=int(o[6]))
+
+    def atlas(self, chan=0, dtype=np.float32):
+        rx, ry, rz = self.atlas_info(chan)["res"]
+        a = np.zeros((rz, ry, rx), dtype)
+        self._chk(load().gvdb_read_atlas(self.h, chan, _p(a)))
+        return a
+
+    def atlas_map(self):
+        n = int(np.prod(self.atlas_info(0)["cnt"]))
+        a = np.zeros(n, ATLASNODE_DTYPE)
+        self._chk(load().gvdb_read_atlas_map(self.h, _p(a)))
+        return a
+
+    def subcells(self):
+        L = load()
+        out = {}
+        for k, name, cols, dt in ((0, "cnt", 1, np.uint32), (1, "off", 1, np.uint32), (2, "pos", 3, np.int32),
+                                  (3, "nid", 1, np.int32), (4, "pnt", 3, np.float32), (5, "mapping", 1, np.uint32)):
+            c = ctypes.c_uint64(0)
+            self._chk(L.gvdb_read_subcells(self.h, k, None, ctypes.byref(c)))
+            a = np.zeros((int(c.value), cols) if cols > 1 else int(c.value), dt)
+            if c.value:
+                self._chk(L.gvdb_read_subcells(self.h, k, _p(a), ctypes.byref(c)))
+            out[name] = a
+        return out
+
+    def synchronize(self):
+        return self._chk(load().gvdb_synchronize(self.h))
+
+    def stage_ms(self):
+        o = np.zeros(8, np.float32)
+        self._chk(load().gvdb_stage_ms(self.h, _p(o)))
+        return dict(zip(("topology", "atlas", "insert", "gather", "apron", "render", "clear"), map(float, o[:7])))
+
+    def kernel_ms(self):
+        o = np.zeros(4, np.float32)
+        self._chk(load().gvdb_kernel_ms(self.h, _p(o)))
+        return dict(gather=float(o[0]), apron=float(o[1]), render=float(o[2]))

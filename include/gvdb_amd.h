@@ -1,0 +1,163 @@
+/* gvdb_amd.h -- C ABI of the MI355X point-to-sparse-volume library.
+ *
+ * Each entry point replaces one VolumeGVDB method of NVIDIA GVDB Voxels 1.1.1
+ * (class nvdb::VolumeGVDB, source/gvdb_library/src/gvdb_volume_gvdb.h:319-901
+ * of the reference); the comment above each names the method it replaces as
+ * file:line of the reference.  Arguments keep the reference's meaning.
+ *
+ * Conventions
+ *   - One handle per grid; a handle owns one HIP stream on one device.  Calls
+ *     on one handle are not thread-safe (as the reference).  Work is enqueued
+ *     on the handle's stream; calls that return sizes to the host (bounding
+ *     box, node counts, subcell entry count) synchronise that stream, as the
+ *     reference's blocking copies do.
+ *   - Every function returns a status: GVDB_OK (0) or a negative code; the
+ *     message of the last failure is available from gvdb_last_error().  The
+ *     reference prints and returns (soft errors) or exits (hard errors).
+ *   - Device pointers are plain `const void*` / `void*` (HIP device memory);
+ *     the library never frees memory it did not allocate.
+ *   - Data layout kept from the reference: 64-byte Node records in one pool
+ *     per level (group 0), dense child lists of res^3 uint64 references per
+ *     interior node (group 1), references packed grp | lev<<8 | idx<<16,
+ *     atlas slots placed x-fastest in a cnt.x*cnt.y*cnt.z grid of padded
+ *     bricks with mValue = slot corner + apron.
+ */
+#ifndef GVDB_AMD_H
+#define GVDB_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GVDB_OK 0
+#define GVDB_ERR_INVALID (-1)     /* bad argument */
+#define GVDB_ERR_HIP (-2)         /* HIP runtime failure */
+#define GVDB_ERR_STATE (-3)       /* call order (e.g. Configure before Initialize) */
+#define GVDB_ERR_DEPTH (-4)       /* point extents below one brick or above the tree */
+#define GVDB_ERR_NOBRICKS (-5)    /* InsertPointsSubcell with no bricks (reference: warning) */
+#define GVDB_ERR_RANGE (-6)       /* coordinates outside what the topology build supports */
+#define GVDB_ERR_UNSUPPORTED (-7) /* feature of the reference not provided */
+#define GVDB_ERR_OOM (-8)         /* device allocation failed */
+
+/* channel types (src/gvdb_types.h of the reference); T_HALF is an addition */
+#define GVDB_T_FLOAT 3
+#define GVDB_T_HALF 100
+
+/* shading modes (kernels/cuda_gvdb_scene.cuh:25-32 of the reference) */
+#define GVDB_SHADE_TRILINEAR 4
+#define GVDB_SHADE_VOLUME 7
+
+/* pool group / level node record (src/gvdb_node.h:27-40 of the reference) */
+typedef struct gvdb_node {
+    uint8_t lev, flags, priority, pad;
+    int32_t pos[3];
+    int32_t value[3];
+    float vrange[3];
+    uint64_t parent;
+    uint64_t childlist;
+    uint64_t mask;
+} gvdb_node;
+
+typedef struct gvdb_volume* gvdb_handle;
+
+/* VolumeGVDB() + SetCudaDevice + Initialize: gvdb_volume_gvdb.cpp:60-172, :240-334, :2320-2361 */
+int gvdb_create(gvdb_handle* out, int device);
+int gvdb_destroy(gvdb_handle h);
+const char* gvdb_last_error(gvdb_handle h);
+
+/* Configure(levs, r, numcnt): gvdb_volume_gvdb.cpp:2380-2434.  log2dims leaf
+ * first; initcnt may be NULL for the Configure(q4..q0) defaults (:2364-2377). */
+int gvdb_configure(gvdb_handle h, int levels, const int* log2dims, const int* initcnt);
+/* SetChannelDefault: gvdb_volume_gvdb.h (Initialize sets 16,16,1) */
+int gvdb_set_channel_default(gvdb_handle h, int cx, int cy, int cz);
+/* AddChannel(chan, dt, apron, filter, border, axiscnt): gvdb_volume_gvdb.cpp:2437-2453 */
+int gvdb_add_channel(gvdb_handle h, int chan, int dtype, int apron, int filter, int border, int ax, int ay, int az);
+/* SetEpsilon(eps, maxiter): gvdb_volume_gvdb.h:334 */
+int gvdb_set_epsilon(gvdb_handle h, float eps, int maxiter);
+
+/* SetDataGPU + SetPoints(pos, vel, clr): gvdb_volume_gvdb.cpp:4872-4880, :4900-4908.
+ * Non-owning float3 positions at dptr + i*stride + offset. */
+int gvdb_set_points_gpu(gvdb_handle h, const void* dptr, int n, int offset, int stride);
+/* AllocData + CommitData: gvdb_volume_gvdb.cpp:4821-4862 (owning copy of host float3 positions) */
+int gvdb_upload_points(gvdb_handle h, const float* host_xyz, int n);
+
+/* RequestFullRebuild: gvdb_volume_gvdb.h:432 */
+int gvdb_request_full_rebuild(gvdb_handle h, int on);
+/* RebuildTopology(num_pnts, radius, orig): gvdb_volume_gvdb.cpp:1181-1218 */
+int gvdb_rebuild_topology(gvdb_handle h, int n, float radius, const float orig[3]);
+/* RebuildTopologyCPU(num_pnts, orig, pos): gvdb_volume_gvdb.cpp:1220-1227 (host build) */
+int gvdb_rebuild_topology_cpu(gvdb_handle h, int n, const float orig[3], const float* host_xyz);
+/* ActivateSpace(pos): gvdb_volume_gvdb.cpp:2766-2774 (host build) */
+int gvdb_activate_space(gvdb_handle h, const float pos[3]);
+/* Clear: gvdb_volume_gvdb.cpp:2487-2503 */
+int gvdb_clear(gvdb_handle h);
+/* FinishTopology(commit, bounds): gvdb_volume_gvdb.cpp:1579-1593 */
+int gvdb_finish_topology(gvdb_handle h, int commit_pool, int compute_bounds);
+/* UpdateAtlas: gvdb_volume_gvdb.cpp:2630-2704 */
+int gvdb_update_atlas(gvdb_handle h);
+/* ClearChannel: gvdb_volume_gvdb.cpp:1596-1606 */
+int gvdb_clear_channel(gvdb_handle h, int chan);
+
+/* InsertPointsSubcell(subcell_size, num_pnts, radius, trans, &len): gvdb_volume_gvdb.cpp:4971-5092 */
+int gvdb_insert_points_subcell(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int* out_len);
+/* GatherDensity(subcell_size, num_pnts, radius, trans, len, chanDensity, chanClr, bAccum): gvdb_volume_gvdb.cpp:5318-5344 */
+int gvdb_gather_density(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len,
+                        int chan_density, int chan_clr, int accumulate);
+/* UpdateApron(chan, boundval): gvdb_volume_gvdb.cpp:4418-4457 */
+int gvdb_update_apron(gvdb_handle h, int chan, float boundval);
+
+/* Scene: Camera3D::setOrbit + fov/near/far (gvdb_camera.cpp:90-105, :175-228),
+ * Light position, Scene setters SetBackgroundClr/SetExtinct/SetSteps/SetCutoff/
+ * SetVolumeRange/SetShadowParams (gvdb_scene.h:142-153, gvdb_scene.cpp:492-495). */
+int gvdb_set_camera_orbit(gvdb_handle h, const float angs[3], const float to[3], float dist, float dolly, float fov,
+                          float znear, float zfar);
+int gvdb_set_light(gvdb_handle h, const float pos[3]);
+int gvdb_set_background(gvdb_handle h, const float rgba[4]);
+int gvdb_set_extinct(gvdb_handle h, float a, float b, float c);
+int gvdb_set_steps(gvdb_handle h, float direct, float shadow, float fine);
+int gvdb_set_cutoff(gvdb_handle h, float a, float b, float c);
+int gvdb_set_volume_range(gvdb_handle h, float thresh, float vmin, float vmax);
+int gvdb_set_shadow_params(gvdb_handle h, float amount, float bias, float c);
+/* LinearTransferFunc(t0, t1, a, b) + CommitTransferFunc: gvdb_scene.cpp:66-88, gvdb_volume_gvdb.cpp:4892-4898 */
+int gvdb_linear_transfer_func(gvdb_handle h, float t0, float t1, const float a[4], const float b[4]);
+int gvdb_commit_transfer_func(gvdb_handle h);
+/* AddRenderBuf(buf, w, h, bytes_per_pixel): gvdb_volume_gvdb.h; bytes 4 = uchar4 (reference), 16 = float4 */
+int gvdb_add_render_buf(gvdb_handle h, int buf, int w, int hgt, int bytes_per_pixel);
+/* Render(shading, chan, buf): gvdb_volume_gvdb.cpp:4336-4381 */
+int gvdb_render(gvdb_handle h, int shading, int chan, int buf);
+/* ReadRenderBuf(buf, dst): gvdb_volume_gvdb.cpp:4241-4251 */
+int gvdb_read_render_buf(gvdb_handle h, int buf, void* host_dst);
+/* The 68-word scene block PrepareRender sends to the kernels (gvdb_volume_gvdb.cpp:4254-4306) */
+int gvdb_get_scene_block(gvdb_handle h, int w, int hgt, float* out68);
+
+/* Node and pool access: getNumTotalNodes/getNodeAtLevel/PoolFetch (gvdb_volume_gvdb.cpp:5799, gvdb_allocator.cpp:127-131).
+ * host_dst may be NULL to query count and stride (bytes per element). */
+int gvdb_get_pool(gvdb_handle h, int grp, int lev, void* host_dst, uint64_t* count, uint64_t* stride);
+/* Tree facts: top level (PrepareVDB :3955-3979), bounds (ComputeBounds :1792-1816), bbox (GetBoundingBox :5404-5467).
+ * out: [0] top_lev, [1] depth; outf: objmin[3], objmax[3], posmin[3], posmax[3]. */
+int gvdb_get_info(gvdb_handle h, int* out2, float* outf12);
+/* Atlas geometry: cnt[3], res[3] (voxels), used slots. */
+int gvdb_get_atlas_info(gvdb_handle h, int chan, int* out7);
+/* Atlas contents in GVDB's 3D layout (x fastest, res.x*res.y*res.z values of the channel type) */
+int gvdb_read_atlas(gvdb_handle h, int chan, void* host_dst);
+/* Atlas map (AtlasNode {int3 pos; int leaf}) per slot */
+int gvdb_read_atlas_map(gvdb_handle h, void* host_dst);
+/* InsertPointsSubcell outputs: which = 0 cnt, 1 off, 2 pos (int3), 3 nid, 4 positions (float3), 5 mapping */
+int gvdb_read_subcells(gvdb_handle h, int which, void* host_dst, uint64_t* count);
+
+/* Device-side handles for callers that stay on the GPU */
+int gvdb_get_stream(gvdb_handle h, void** stream);
+int gvdb_synchronize(gvdb_handle h);
+/* Milliseconds of the last call of each stage, measured with HIP events on
+ * the handle's stream: 0 topology, 1 atlas, 2 insert, 3 gather, 4 apron,
+ * 5 render, 6 clear.  Synchronises the stream. */
+int gvdb_stage_ms(gvdb_handle h, float* out8);
+/* Kernel-only time of the most recent gather / apron / render kernel (ms). */
+int gvdb_kernel_ms(gvdb_handle h, float* out4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GVDB_AMD_H */
