@@ -1,0 +1,179 @@
+"""Benchmark of the point-to-sparse-volume path on one MI355X.
+
+Workload (plain method, medium, on GPU): 4,194,304 points uniform in the
+400x200x400-voxel slab [56,456]x[20,220]x[56,456] plus 200,000 points uniform
+on the plane y = 10; tree log2 dims {3,3,3,4,5} (8^3 bricks, apron 1), one
+FP32 density channel, gather radius 2, subcell 4; isosurface march at density
+0.5 with trilinear sampling, 1024x1024, step 0.25, Phong + shadow ray.
+
+One step = RebuildTopology + FinishTopology + UpdateAtlas + ClearChannel +
+InsertPointsSubcell + GatherDensity + UpdateApron (the "splat" part) and one
+Render.  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line.  `value` = points splatted per second (points / splat
+time); `mrays_per_s` = primary rays per second of the render.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gvdb-voxels_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "points splatted/s + Mrays/s on 1 MI355X; batched p50/p99 latency vs plain form"
+
+
+def algorithmic_bytes(v, n_points):
+    """Per-launch algorithmic HBM bytes of the gather and apron kernels.
+    gather: E*12 (entries) + S*8 (counts, offsets) + B*512*4 (interior writes)
+    apron:  B*488*4 read + B*488*4 written."""
+    sc = v.subcells()
+    E = int(sc["cnt"].astype(np.int64).sum())
+    S = len(sc["cnt"])
+    B = len(v.pool(0, 0))
+    return dict(E=E, S=S, B=B, gather=E * 12 + S * 8 + B * 512 * 4, apron=B * 488 * 8,
+                insert=n_points * 12 + E * 12 + S * 8)
+
+
+def run_gpu(args):
+    import gvdb_amd as G
+    from gvdb_amd import workloads as W
+    pts = W.slab_plane()
+    n = len(pts)
+    sc = W.iso_scene(args.width, args.width)
+    v = G.VolumeGVDB(0)
+    v.Configure(5, 4, 3, 3, 3)
+    v.SetChannelDefault(16, 16, 1)
+    v.AddChannel(0, G.T_FLOAT, 1)
+    v.SetPoints(pts)
+    v.apply_scene(sc)
+    v.AddRenderBuf(0, sc["width"], sc["height"], 4)
+    r = 2.0
+    z = (0.0, 0.0, 0.0)
+
+    def splat():
+        v.RequestFullRebuild(True)
+        v.RebuildTopology(n, r, z)
+        v.FinishTopology(False, True)
+        v.UpdateAtlas()
+        v.ClearChannel(0)
+        ln = v.InsertPointsSubcell(4, n, r, z)
+        v.GatherDensity(4, n, r, z, ln, 0, 0, False)
+        v.UpdateApron(0, 0.0)
+        return ln
+
+    def render():
+        v.Render(G.SHADE_TRILINEAR, 0, 0)
+
+    for _ in range(args.warmup):
+        splat()
+        render()
+    v.synchronize()
+    t_splat, t_render, kg, ka, kr, st = [], [], [], [], [], []
+    t_all0 = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        splat()
+        v.synchronize()
+        t1 = time.perf_counter()
+        render()
+        v.synchronize()
+        t2 = time.perf_counter()
+        t_splat.append(t1 - t0)
+        t_render.append(t2 - t1)
+        k = v.kernel_ms()
+        kg.append(k["gather"]); ka.append(k["apron"]); kr.append(k["render"])
+        st.append(v.stage_ms())
+    v.synchronize()
+    t_all = time.perf_counter() - t_all0
+    ab = algorithmic_bytes(v, n)
+    gather_ms = float(np.mean(kg))
+    apron_ms = float(np.mean(ka))
+    render_ms = float(np.mean(kr))
+    stages = {k: float(np.mean([s[k] for s in st])) for k in st[0]}
+    out = dict(n=n, t_splat=float(np.mean(t_splat)), t_render=float(np.mean(t_render)), t_all=t_all, ab=ab,
+               gather_ms=gather_ms, apron_ms=apron_ms, render_ms=render_ms, stages=stages,
+               rays=sc["width"] * sc["height"], topo=[len(v.pool(0, l)) for l in range(5)])
+    v.close()
+    return out
+
+
+def run_cpu_baseline(sample_stride=4, render_w=512):
+    """The oracle's host path (host ActivateSpace topology, UpdateAtlas, subcell
+    insert, FP32 gather, apron; FP32 march) on a bounded sample: every
+    `sample_stride`-th point of the same workload, and the same isosurface
+    scene at render_w x render_w."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from gvdb_amd import workloads as W
+    pts = np.ascontiguousarray(W.slab_plane()[::sample_stride])
+    O.set_threads(os.cpu_count() or 1)
+    o = O.Oracle()
+    t0 = time.perf_counter()
+    o.rebuild_topology_cpu(pts)
+    o.finish_topology()
+    o.update_atlas()
+    o.clear_channel()
+    rc, ln = o.insert_points_subcell(pts, 2.0)
+    o.gather_density(2.0, precision=32)
+    o.update_apron(0.0, 32)
+    t1 = time.perf_counter()
+    sc = W.iso_scene(render_w, render_w)
+    blk = O.scene_block(sc)
+    tf = O.linear_tf(*sc["tf"])
+    t2 = time.perf_counter()
+    o.render(blk, tf, 4, 32)
+    t3 = time.perf_counter()
+    return dict(points=len(pts), t_splat=t1 - t0, t_render=t3 - t2, rays=render_w * render_w,
+                threads=O.max_threads(), sample="every %dth point of the workload (%d points), %dx%d render" % (
+                    sample_stride, len(pts), render_w, render_w))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the point-to-volume path runs one grid per GPU and this build has "
+              "no multi-GPU form (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    g = run_gpu(args)
+    ab = g["ab"]
+    gather_gbs = ab["gather"] / (g["gather_ms"] * 1e-3) / 1e9
+    apron_gbs = ab["apron"] / (g["apron_ms"] * 1e-3) / 1e9
+    value = g["n"] / g["t_splat"]
+    line = dict(
+        metric=METRIC, value=value, unit="points/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=1e3 * g["t_all"] / args.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f32", data="synthetic",
+        config=dict(workload="plain medium: 4,194,304 slab + 200,000 plane points, tree {3,3,3,4,5}, radius 2, "
+                             "subcell 4, isosurface 0.5 trilinear %dx%d step 0.25" % (args.width, args.width),
+                    points=g["n"], subcell_entries=ab["E"], bricks=ab["B"], nodes_per_level=g["topo"]),
+        mrays_per_s=g["rays"] / g["t_render"] / 1e6,
+        splat_ms=1e3 * g["t_splat"], render_ms=1e3 * g["t_render"], stage_ms=g["stages"],
+        kernel_ms=dict(gather=g["gather_ms"], apron=g["apron_ms"], render=g["render_ms"]),
+        roofline=dict(bound="hbm", kernel="gather", achieved=gather_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=gather_gbs / HBM_PEAK_GBS, traffic=None, algorithmic_bytes=ab["gather"]),
+        roofline_apron=dict(bound="hbm", achieved=apron_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
+                            frac=apron_gbs / HBM_PEAK_GBS, algorithmic_bytes=ab["apron"]),
+        batched=None,
+    )
+    if not args.no_cpu:
+        c = run_cpu_baseline()
+        line["cpu_baseline"] = dict(value=c["points"] / c["t_splat"], unit="points/s", cores=c["threads"], kind="port",
+                                    sample=c["sample"], mrays_per_s=c["rays"] / c["t_render"] / 1e6,
+                                    splat_s=c["t_splat"], render_s=c["t_render"])
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
