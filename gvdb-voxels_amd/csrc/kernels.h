@@ -75,6 +75,19 @@ hipError_t launch_gather_density(Tree t, int scs, int nbricks_sc, float radius, 
                                  const uint32_t* sc_cnt, const uint32_t* sc_off, const int* sc_pos,
                                  const float* sc_pnt, void* atlas, int atlas_type, int accumulate, hipStream_t s);
 
+// pull-based binning (points.hip): bucket points by their own subcell, then
+// every brick pulls from the subcells around it
+hipError_t launch_bucket_points(Tree t, PointSrc pts, int scs, int spb, const float trans[3], uint32_t* key,
+                                uint32_t* slot, uint32_t* bcnt, uint32_t* orphans, hipStream_t s);
+hipError_t launch_bucket_scatter(PointSrc pts, const float trans[3], const uint32_t* key, const uint32_t* slot,
+                                 const uint32_t* boff, float* sorted, hipStream_t s);
+int pull_cells(int sd, int R);
+int pull_max_cells();
+int pull_max_spb();
+hipError_t launch_pull(int write, Tree t, int nleaves, int scs, int R, float radius, const int* nbr27,
+                       const uint32_t* bcnt, const uint32_t* boff, const float* sorted, const uint32_t* mapping,
+                       uint32_t* sc_cnt, const uint32_t* sc_off, float* sc_pnt, hipStream_t s);
+
 // ---- apron (apron.hip)
 // nbr27[b*27 + d]: leaf holding the neighbour of brick b in direction
 // d = (dz+1)*9 + (dy+1)*3 + (dx+1), or -1 (d = 13 is b itself)

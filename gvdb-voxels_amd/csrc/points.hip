@@ -328,3 +328,235 @@ hipError_t launch_leaf_bounds(Tree t, int nleaves, int* out6, hipStream_t s) {
 }
 
 }  // namespace gvdb
+
+// ===================================================== pull-based binning
+// The reference bins by pushing every (point, subcell) pair through a global
+// atomic counter, twice (count, then insert).  Here each point is bucketed
+// once by the subcell that contains it, and every brick then pulls the points
+// of the subcells around it: a point can only be binned into subcells at most
+// R = ceil((r+1)/sc) subcells away from its own, so a brick reads the
+// (spd+2R)^3 subcells around its spd^3 own ones and applies the reference's
+// per-axis test  int(p-r)/sc*sc <= s <= int(p+1+r)/sc*sc  to each.  Counts
+// and placement are per workgroup (LDS and wave ballots), without global
+// atomics.  Used when every valid point lies in an existing brick and the
+// neighbourhood fits in the 27 bricks around; otherwise the push kernels run.
+namespace gvdb {
+
+__global__ __launch_bounds__(256) void k_bucket_points(Tree t, PointSrc pts, int scs, int spb, float tx, float ty,
+                                                       float tz, uint32_t* key, uint32_t* slot, uint32_t* bcnt,
+                                                       uint32_t* orphans) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pts.n) return;
+    const float* p = point_at(pts, i);
+    float p0 = p[0], p1 = p[1], p2 = p[2];
+    uint32_t k = 0xFFFFFFFFu;
+    float w0 = p0 + tx, w1 = p1 + ty, w2 = p2 + tz;
+    bool valid = p2 != NOHIT && isfinite(w0) && isfinite(w1) && isfinite(w2) && w0 >= 0 && w1 >= 0 && w2 >= 0 &&
+                 p0 >= 0 && p1 >= 0 && p2 >= 0;
+    if (valid) {
+        int ox = int(w0) / scs * scs, oy = int(w1) / scs * scs, oz = int(w2) / scs * scs;
+        int leaf = pos_leaf(t, ox, oy, oz);
+        if (leaf < 0) {
+            atomicAdd(orphans, 1u);
+        } else {
+            const Node* n = t.nodes[0] + leaf;
+            int sd = t.res[0] / scs;
+            int lx = (ox - n->mPos[0]) / scs, ly = (oy - n->mPos[1]) / scs, lz = (oz - n->mPos[2]) / scs;
+            k = uint32_t(leaf) * uint32_t(spb) + uint32_t((lz * sd + ly) * sd + lx);
+            slot[i] = atomicAdd(&bcnt[k], 1u);
+        }
+    }
+    key[i] = k;
+}
+
+__global__ __launch_bounds__(256) void k_bucket_scatter(PointSrc pts, float tx, float ty, float tz, const uint32_t* key,
+                                                        const uint32_t* slot, const uint32_t* boff, float* sorted) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pts.n) return;
+    uint32_t k = key[i];
+    if (k == 0xFFFFFFFFu) return;
+    const float* p = point_at(pts, i);
+    u64 e = u64(boff[k]) + slot[i];
+    sorted[3 * e] = p[0] + tx;
+    sorted[3 * e + 1] = p[1] + ty;
+    sorted[3 * e + 2] = p[2] + tz;
+}
+
+static const int PULL_THREADS = 256;
+static const int PULL_MAX_CELLS = 512;   // (spd + 2R)^3 upper bound handled
+static const int PULL_MAX_SPB = 32;
+
+// Per-brick setup shared by the count and write passes: the neighbourhood
+// cells' bucket ranges and their exclusive prefix.  Returns the number of
+// points to test (0 if the brick is unreachable or has no subcells).
+struct PullShared {
+    uint32_t cell_off[PULL_MAX_CELLS];
+    uint32_t cell_start[PULL_MAX_CELLS + 1];
+    int nbr[27];
+    int ok;
+    uint32_t wave_cnt[PULL_THREADS / 64][PULL_MAX_SPB];
+    uint32_t run[PULL_MAX_SPB];
+};
+
+__device__ inline uint32_t pull_setup(const Tree& t, int brick, int scs, int R, const int* nbr27,
+                                      const uint32_t* bcnt, const uint32_t* boff, PullShared& S) {
+    const int sd = t.res[0] / scs;
+    const int span = sd + 2 * R;
+    const int ncell = span * span * span;
+    const int spb = sd * sd * sd;
+    if (threadIdx.x == 0) {
+        const Node* n = t.nodes[0] + brick;
+        // only bricks reachable from the top node receive entries (getPosLeafParent)
+        S.ok = (n->mParent != ID_UNDEF64) && pos_leaf(t, n->mPos[0], n->mPos[1], n->mPos[2]) == brick;
+    }
+    if (threadIdx.x < 27) S.nbr[threadIdx.x] = nbr27[u64(brick) * 27 + threadIdx.x];
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        int cx = c % span - R, cy = (c / span) % span - R, cz = c / (span * span) - R;
+        int dx = cx < 0 ? -1 : (cx >= sd ? 1 : 0), dy = cy < 0 ? -1 : (cy >= sd ? 1 : 0), dz = cz < 0 ? -1 : (cz >= sd ? 1 : 0);
+        int nb = S.nbr[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+        uint32_t cnt = 0, off = 0;
+        if (S.ok && nb >= 0) {
+            int lx = cx - dx * sd, ly = cy - dy * sd, lz = cz - dz * sd;
+            u64 b = u64(nb) * spb + u64((lz * sd + ly) * sd + lx);
+            cnt = bcnt[b];
+            off = boff[b];
+        }
+        S.cell_off[c] = off;
+        S.cell_start[c + 1] = cnt;   // counts, prefix-summed below
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        S.cell_start[0] = 0;
+        for (int c = 0; c < ncell; c++) { acc += S.cell_start[c + 1]; S.cell_start[c + 1] = acc; }
+    }
+    __syncthreads();
+    return S.cell_start[ncell];
+}
+
+// Target mask of one pulled point over the brick's spb subcells.
+__device__ inline uint32_t pull_mask(const Node* brick, int scs, int sd, float r, float w0, float w1, float w2) {
+    int lo0 = (int(w0 - r) / scs) * scs, hi0 = (int(w0 + 1 + r) / scs) * scs;
+    int lo1 = (int(w1 - r) / scs) * scs, hi1 = (int(w1 + 1 + r) / scs) * scs;
+    int lo2 = (int(w2 - r) / scs) * scs, hi2 = (int(w2 + 1 + r) / scs) * scs;
+    uint32_t mx = 0, my = 0, mz = 0;
+    for (int a = 0; a < sd; a++) {
+        int sx = brick->mPos[0] + a * scs, sy = brick->mPos[1] + a * scs, sz = brick->mPos[2] + a * scs;
+        mx |= (sx >= lo0 && sx <= hi0) ? (1u << a) : 0u;
+        my |= (sy >= lo1 && sy <= hi1) ? (1u << a) : 0u;
+        mz |= (sz >= lo2 && sz <= hi2) ? (1u << a) : 0u;
+    }
+    uint32_t m = 0;
+    for (int z = 0; z < sd; z++)
+        if (mz & (1u << z))
+            for (int y = 0; y < sd; y++)
+                if (my & (1u << y))
+                    for (int x = 0; x < sd; x++)
+                        if (mx & (1u << x)) m |= 1u << ((z * sd + y) * sd + x);
+    return m;
+}
+
+__device__ inline int find_cell(const PullShared& S, int ncell, uint32_t f) {
+    int lo = 0, hi = ncell - 1;   // largest c with cell_start[c] <= f
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (S.cell_start[mid] <= f) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(PULL_THREADS) void k_pull(Tree t, int scs, int R, float radius, const int* nbr27,
+                                                       const uint32_t* bcnt, const uint32_t* boff, const float* sorted,
+                                                       const uint32_t* mapping, uint32_t* sc_cnt, const uint32_t* sc_off,
+                                                       float* sc_pnt) {
+    __shared__ PullShared S;
+    const int brick = blockIdx.x;
+    const Node* bn = t.nodes[0] + brick;
+    const int sd = t.res[0] / scs;
+    const int spb = sd * sd * sd;
+    const int span = sd + 2 * R;
+    const int ncell = span * span * span;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < PULL_MAX_SPB) S.run[threadIdx.x] = 0;
+    uint32_t T = pull_setup(t, brick, scs, R, nbr27, bcnt, boff, S);
+    const bool linked = bn->mParent != ID_UNDEF64;
+    const u64 sc0 = linked ? u64(mapping[brick]) * spb : 0;
+    for (uint32_t base = 0; base < T; base += PULL_THREADS) {
+        uint32_t f = base + threadIdx.x;
+        uint32_t m = 0;
+        float w0 = 0, w1 = 0, w2 = 0;
+        if (f < T) {
+            int c = find_cell(S, ncell, f);
+            u64 e = u64(S.cell_off[c]) + (f - S.cell_start[c]);
+            w0 = sorted[3 * e]; w1 = sorted[3 * e + 1]; w2 = sorted[3 * e + 2];
+            m = pull_mask(bn, scs, sd, radius, w0, w1, w2);
+        }
+        for (int k = 0; k < spb; k++) {
+            u64 bal = __ballot((m >> k) & 1u);
+            if (lane == 0) S.wave_cnt[wave][k] = uint32_t(__popcll(bal));
+        }
+        __syncthreads();
+        if (WRITE) {
+            // per target: exclusive rank = entries of earlier chunks + earlier waves + lower lanes
+            for (int k = 0; k < spb; k++) {
+                bool on = (m >> k) & 1u;
+                u64 bal = __ballot(on);
+                if (on) {
+                    uint32_t before = 0;
+                    for (int w = 0; w < wave; w++) before += S.wave_cnt[w][k];
+                    u64 lower = lane ? (bal & ((~0ull) >> (64 - lane))) : 0ull;
+                    uint32_t rank = S.run[k] + before + uint32_t(__popcll(lower));
+                    u64 e = u64(sc_off[sc0 + k]) + rank;
+                    sc_pnt[3 * e] = w0;
+                    sc_pnt[3 * e + 1] = w1;
+                    sc_pnt[3 * e + 2] = w2;
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < spb) {
+            uint32_t s = 0;
+            for (int w = 0; w < PULL_THREADS / 64; w++) s += S.wave_cnt[w][threadIdx.x];
+            S.run[threadIdx.x] += s;
+        }
+        __syncthreads();
+    }
+    if (!WRITE && linked && threadIdx.x < spb) sc_cnt[sc0 + threadIdx.x] = S.run[threadIdx.x];
+}
+
+hipError_t launch_bucket_points(Tree t, PointSrc pts, int scs, int spb, const float trans[3], uint32_t* key,
+                                uint32_t* slot, uint32_t* bcnt, uint32_t* orphans, hipStream_t s) {
+    if (pts.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bucket_points, dim3((pts.n + 255) / 256), dim3(256), 0, s, t, pts, scs, spb, trans[0], trans[1],
+                       trans[2], key, slot, bcnt, orphans);
+    return hipGetLastError();
+}
+
+hipError_t launch_bucket_scatter(PointSrc pts, const float trans[3], const uint32_t* key, const uint32_t* slot,
+                                 const uint32_t* boff, float* sorted, hipStream_t s) {
+    if (pts.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bucket_scatter, dim3((pts.n + 255) / 256), dim3(256), 0, s, pts, trans[0], trans[1], trans[2],
+                       key, slot, boff, sorted);
+    return hipGetLastError();
+}
+
+int pull_cells(int sd, int R) { int span = sd + 2 * R; return span * span * span; }
+int pull_max_cells() { return PULL_MAX_CELLS; }
+int pull_max_spb() { return PULL_MAX_SPB; }
+
+hipError_t launch_pull(int write, Tree t, int nleaves, int scs, int R, float radius, const int* nbr27,
+                       const uint32_t* bcnt, const uint32_t* boff, const float* sorted, const uint32_t* mapping,
+                       uint32_t* sc_cnt, const uint32_t* sc_off, float* sc_pnt, hipStream_t s) {
+    if (nleaves <= 0) return hipSuccess;
+    if (write)
+        hipLaunchKernelGGL(k_pull<true>, dim3(nleaves), dim3(PULL_THREADS), 0, s, t, scs, R, radius, nbr27, bcnt, boff,
+                           sorted, mapping, sc_cnt, sc_off, sc_pnt);
+    else
+        hipLaunchKernelGGL(k_pull<false>, dim3(nleaves), dim3(PULL_THREADS), 0, s, t, scs, R, radius, nbr27, bcnt, boff,
+                           sorted, mapping, sc_cnt, sc_off, sc_pnt);
+    return hipGetLastError();
+}
+
+}  // namespace gvdb

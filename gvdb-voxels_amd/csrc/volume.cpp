@@ -67,7 +67,7 @@ Volume::~Volume() {
     for (auto& c : chan_) c.data.release();
     for (auto& r : rbuf_) r.data.release();
     DevBuf* bufs[] = {&amap_, &nbr_, &pts_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_,
-                      &sc_flags_, &small_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev};
+                      &bkt_key_, &bkt_slot_, &bkt_cnt_, &bkt_off_, &sorted_, &sc_flags_, &small_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev};
     for (DevBuf* b : bufs) b->release();
     for (int s = 0; s < ST_COUNT; s++)
         for (int k = 0; k < 2; k++)
@@ -796,14 +796,56 @@ int Volume::insert_points_subcell(int scs, int n, float radius, const float tran
     HIPCHK(launch_leaf_flags(t, sc_flags_.as<uint32_t>(), int(numMap), stream));
     HIPCHK(launch_scan_u32(sc_flags_.as<uint32_t>(), nullptr, 0, sc_map_.as<uint32_t>(), numMap, small + 8,
                            scan_tmp_.as<uint32_t>(), stream));
-    HIPCHK(launch_fill_u32(sc_cnt_.as<uint32_t>(), 0u, numSC, stream));
-    HIPCHK(launch_count_subcell(t, P, scs, spb, trans, radius, sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(), stream));
-    HIPCHK(launch_scan_u32(sc_cnt_.as<uint32_t>(), nullptr, 0, sc_off_.as<uint32_t>(), numSC, small + 9,
-                           scan_tmp_.as<uint32_t>(), stream));
-    HIPCHK(hipMemcpyAsync(pinned_ + 8, small + 8, 8, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    uint32_t linked = pinned_[8];
-    uint32_t total = pinned_[9];
+    // pull path when the neighbourhood of a brick stays inside its 27 neighbours
+    const int R = int(ceilf((radius + 1.0f) / float(scs)));
+    const bool pull = R <= sd && pull_cells(sd, R) <= pull_max_cells() && spb <= pull_max_spb();
+    uint32_t linked = 0, total = 0, orphans = 0;
+    if (pull) {
+        if (nbr_gen_ != topo_gen_) {
+            HIPCHK(nbr_.ensure(size_t(numMap) * 27 * 4));
+            HIPCHK(launch_neighbors(t, nbr_.as<int>(), int(numMap), stream));
+            nbr_gen_ = topo_gen_;
+        }
+        u64 nb = numMap * u64(spb);
+        HIPCHK(bkt_cnt_.ensure(size_t(nb) * 4));
+        HIPCHK(bkt_off_.ensure(size_t(nb) * 4));
+        HIPCHK(bkt_key_.ensure(size_t(std::max(n, 1)) * 4));
+        HIPCHK(bkt_slot_.ensure(size_t(std::max(n, 1)) * 4));
+        HIPCHK(sorted_.ensure(size_t(std::max(n, 1)) * 12));
+        HIPCHK(scan_tmp_.ensure(scan_tmp_words(std::max(nb, numSC)) * 4 + 256));
+        HIPCHK(launch_fill_u32(bkt_cnt_.as<uint32_t>(), 0u, nb, stream));
+        HIPCHK(launch_fill_u32(small + 10, 0u, 1, stream));
+        HIPCHK(launch_bucket_points(t, P, scs, spb, trans, bkt_key_.as<uint32_t>(), bkt_slot_.as<uint32_t>(),
+                                    bkt_cnt_.as<uint32_t>(), small + 10, stream));
+        HIPCHK(launch_scan_u32(bkt_cnt_.as<uint32_t>(), nullptr, 0, bkt_off_.as<uint32_t>(), nb, small + 11,
+                               scan_tmp_.as<uint32_t>(), stream));
+        HIPCHK(launch_bucket_scatter(P, trans, bkt_key_.as<uint32_t>(), bkt_slot_.as<uint32_t>(),
+                                     bkt_off_.as<uint32_t>(), sorted_.as<float>(), stream));
+        HIPCHK(launch_fill_u32(sc_cnt_.as<uint32_t>(), 0u, numSC, stream));
+        HIPCHK(launch_pull(0, t, int(numMap), scs, R, radius, nbr_.as<int>(), bkt_cnt_.as<uint32_t>(),
+                           bkt_off_.as<uint32_t>(), sorted_.as<float>(), sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
+                           sc_off_.as<uint32_t>(), nullptr, stream));
+        HIPCHK(launch_scan_u32(sc_cnt_.as<uint32_t>(), nullptr, 0, sc_off_.as<uint32_t>(), numSC, small + 9,
+                               scan_tmp_.as<uint32_t>(), stream));
+        HIPCHK(hipMemcpyAsync(pinned_ + 8, small + 8, 16, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        linked = pinned_[8];
+        total = pinned_[9];
+        orphans = pinned_[10];
+    }
+    const bool use_pull = pull && orphans == 0;
+    if (!use_pull) {
+        // push path: the reference's count pass with global atomics
+        HIPCHK(launch_fill_u32(sc_cnt_.as<uint32_t>(), 0u, numSC, stream));
+        HIPCHK(launch_count_subcell(t, P, scs, spb, trans, radius, sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
+                                    stream));
+        HIPCHK(launch_scan_u32(sc_cnt_.as<uint32_t>(), nullptr, 0, sc_off_.as<uint32_t>(), numSC, small + 9,
+                               scan_tmp_.as<uint32_t>(), stream));
+        HIPCHK(hipMemcpyAsync(pinned_ + 8, small + 8, 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        linked = pinned_[8];
+        total = pinned_[9];
+    }
     if (total > 0x7FFFFFFFu) return fail(GVDB_ERR_RANGE, "InsertPointsSubcell: more than 2^31 subcell entries");
     *len = int(total);
     sc_len_ = int(total);
@@ -811,13 +853,20 @@ int Volume::insert_points_subcell(int scs, int n, float radius, const float tran
     sc_bricks_ = int(linked);
     sc_map_n_ = int(numMap);
     sc_gen_ = topo_gen_;
+    last_insert_pull_ = use_pull;
     if (total == 0) { stage_end(ST_INSERT); return GVDB_OK; }
     HIPCHK(sc_pnt_.ensure(size_t(total) * 12));
     HIPCHK(sc_pos_.ensure(size_t(numSC) * 12));
     HIPCHK(sc_nid_.ensure(size_t(numSC) * 4));
-    HIPCHK(launch_fill_u32(sc_cnt_.as<uint32_t>(), 0u, numSC, stream));
-    HIPCHK(launch_insert_subcell(t, P, scs, spb, trans, radius, sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
-                                 sc_off_.as<uint32_t>(), sc_pnt_.as<float>(), stream));
+    if (use_pull) {
+        HIPCHK(launch_pull(1, t, int(numMap), scs, R, radius, nbr_.as<int>(), bkt_cnt_.as<uint32_t>(),
+                           bkt_off_.as<uint32_t>(), sorted_.as<float>(), sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
+                           sc_off_.as<uint32_t>(), sc_pnt_.as<float>(), stream));
+    } else {
+        HIPCHK(launch_fill_u32(sc_cnt_.as<uint32_t>(), 0u, numSC, stream));
+        HIPCHK(launch_insert_subcell(t, P, scs, spb, trans, radius, sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
+                                     sc_off_.as<uint32_t>(), sc_pnt_.as<float>(), stream));
+    }
     HIPCHK(launch_subcell_pos(t, scs, spb, sc_map_.as<uint32_t>(), sc_pos_.as<int>(), sc_nid_.as<int>(), int(numMap),
                               stream));
     stage_end(ST_INSERT);

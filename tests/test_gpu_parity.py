@@ -118,3 +118,53 @@ def test_host_topology_path(gpu):
     mask = H.used_brick_mask(b.shape, o.atlas_map(), o.atlas_info()["cnt"])
     _, rel, zeros = H.density_errors(a[mask], b[mask])
     assert zeros == 0 and rel <= REL_TOL
+
+
+@pytest.mark.parametrize("variant", ["large_radius", "orphans", "subcell2", "radius3"])
+def test_insert_paths(gpu, variant):
+    """Both binning paths against the oracle: the pull path (own-subcell
+    buckets) and the push path it falls back to for points outside every brick
+    (topology built from a subset), neighbourhoods wider than one brick, and
+    subcell sizes with more than 32 subcells per brick."""
+    import oracle as O
+    pts = _sphere(6000, 31)
+    n_topo, r_topo, r_ins, scs = len(pts), 2.0, 2.0, 4
+    if variant == "large_radius":
+        r_topo = r_ins = 8.0
+    elif variant == "orphans":
+        n_topo = 3000
+    elif variant == "subcell2":
+        scs = 2
+    elif variant == "radius3":
+        r_topo = r_ins = 3.0
+    v = gpu.VolumeGVDB(0)
+    v.Configure(*H.TREE)
+    v.SetChannelDefault(16, 16, 1)
+    v.AddChannel(0, gpu.T_FLOAT, 1)
+    v.SetPoints(pts)
+    v.RebuildTopology(n_topo, r_topo, H.ZERO3)
+    v.FinishTopology(False, True)
+    v.UpdateAtlas()
+    v.ClearChannel(0)
+    n = v.InsertPointsSubcell(scs, len(pts), r_ins, H.ZERO3)
+    v.GatherDensity(scs, len(pts), r_ins, H.ZERO3, n, 0, 0, False)
+    v.UpdateApron(0, 0.0)
+    o = O.Oracle()
+    assert o.rebuild_topology(pts[:n_topo], r_topo) == 0
+    o.finish_topology()
+    o.update_atlas()
+    o.clear_channel()
+    rc, m = o.insert_points_subcell(pts, r_ins, scs)
+    assert rc == 0 and n == m
+    o.gather_density(r_ins, scs, 64)
+    o.update_apron(0.0, 64)
+    gs, os_ = v.subcells(), o.subcells(m)
+    assert np.array_equal(gs["cnt"].astype(np.int64), os_["cnt"].astype(np.int64))
+    assert np.array_equal(gs["off"].astype(np.int64), os_["off"].astype(np.int64))
+    la = H.subcell_lists(gs["cnt"], gs["off"], gs["pnt"])
+    lb = H.subcell_lists(os_["cnt"], os_["off"], os_["pnt"])
+    assert all(np.array_equal(x, y) for x, y in zip(la, lb))
+    a, b = v.atlas(), o.atlas(64)
+    mask = H.used_brick_mask(b.shape, o.atlas_map(), o.atlas_info()["cnt"])
+    _, rel, zeros = H.density_errors(a[mask], b[mask])
+    assert zeros == 0 and rel <= REL_TOL
