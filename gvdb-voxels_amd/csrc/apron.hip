@@ -25,8 +25,12 @@ __global__ __launch_bounds__(256) void k_neighbors(Tree t, int* nbr, int nbricks
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nbricks * 27) return;
     int b = g / 27, d = g - b * 27;
-    if (d == 13) { nbr[g] = b; return; }
     const Node* n = t.nodes[0] + b;
+    if (d == 13) {
+        // the brick itself, or -1 when the walk from the top node does not reach it
+        nbr[g] = pos_leaf(t, n->mPos[0], n->mPos[1], n->mPos[2]) == b ? b : -1;
+        return;
+    }
     int dx, dy, dz;
     dir_of(d, dx, dy, dz);
     int w = t.brickwid;

@@ -37,8 +37,9 @@ hipError_t launch_scan_u32(const uint32_t* in0, const uint32_t* in1, int mode, u
 
 // ---- topology (topology.hip)
 hipError_t launch_bbox(PointSrc pts, const float orig[3], uint32_t* keys6, hipStream_t s);
+// masks: optional zeroed u32 per level-0 domain brick (nbits words)
 hipError_t launch_mark_points(PointSrc pts, const float orig[3], float radius, int dilate, int range0,
-                              BitDomain d0, uint32_t* bmA, uint32_t* bmB, hipStream_t s);
+                              BitDomain d0, uint32_t* bmA, uint32_t* bmB, uint32_t* masks, hipStream_t s);
 hipError_t launch_mark_parents(BitDomain dc, const uint32_t* bmA_c, const uint32_t* bmB_c, int shift, BitDomain dp,
                                uint32_t* bmA_p, uint32_t* bmB_p, hipStream_t s);
 struct LevelBuild {
@@ -71,9 +72,12 @@ hipError_t launch_insert_subcell(Tree t, PointSrc pts, int scs, int spb, const f
                                  float* sc_pnt, hipStream_t s);
 hipError_t launch_subcell_pos(Tree t, int scs, int spb, const uint32_t* mapping, int* sc_pos, int* sc_nid,
                               int nleaves, hipStream_t s);
+// sc_use[s]: entries at the head of subcell s that can raise a voxel (all of
+// them when the push path binned the points); accumulate mode reads sc_cnt.
 hipError_t launch_gather_density(Tree t, int scs, int nbricks_sc, float radius, const int* sc_nid,
-                                 const uint32_t* sc_cnt, const uint32_t* sc_off, const int* sc_pos,
-                                 const float* sc_pnt, void* atlas, int atlas_type, int accumulate, hipStream_t s);
+                                 const uint32_t* sc_cnt, const uint32_t* sc_use, const uint32_t* sc_off,
+                                 const int* sc_pos, const float* sc_pnt, void* atlas, int atlas_type, int accumulate,
+                                 hipStream_t s);
 
 // pull-based binning (points.hip): bucket points by their own subcell, then
 // every brick pulls from the subcells around it
@@ -86,11 +90,12 @@ int pull_max_cells();
 int pull_max_spb();
 hipError_t launch_pull(int write, Tree t, int nleaves, int scs, int R, float radius, const int* nbr27,
                        const uint32_t* bcnt, const uint32_t* boff, const float* sorted, const uint32_t* mapping,
-                       uint32_t* sc_cnt, const uint32_t* sc_off, float* sc_pnt, hipStream_t s);
+                       uint32_t* sc_cnt, uint32_t* sc_use, const uint32_t* sc_off, float* sc_pnt, hipStream_t s);
 
 // ---- apron (apron.hip)
 // nbr27[b*27 + d]: leaf holding the neighbour of brick b in direction
-// d = (dz+1)*9 + (dy+1)*3 + (dx+1), or -1 (d = 13 is b itself)
+// d = (dz+1)*9 + (dy+1)*3 + (dx+1), or -1; d = 13 is b itself, or -1 when the
+// walk from the top node does not reach b
 hipError_t launch_neighbors(Tree t, int* nbr27, int nbricks, hipStream_t s);
 hipError_t launch_update_apron(Tree t, const int* nbr27, int nbricks, void* atlas, int atlas_type, float boundval,
                                hipStream_t s);

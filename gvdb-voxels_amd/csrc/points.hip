@@ -208,8 +208,8 @@ __device__ inline float load_atlas<__half>(const __half* a, u64 off) { return __
 
 template <typename AT>
 __global__ __launch_bounds__(512) void k_gather(Tree t, int scs, float radius, const int* sc_nid, const uint32_t* sc_cnt,
-                                                const uint32_t* sc_off, const int* sc_pos, const float* sc_pnt, AT* atlas,
-                                                int accumulate) {
+                                                const uint32_t* sc_use, const uint32_t* sc_off, const int* sc_pos,
+                                                const float* sc_pnt, AT* atlas, int accumulate) {
     __shared__ float lx[GATHER_CHUNK], ly[GATHER_CHUNK], lz[GATHER_CHUNK];
     const int res0 = t.res[0];
     const int scs3 = scs * scs * scs;
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(512) void k_gather(Tree t, int scs, float radius, c
         vy = sc_pos[3 * sc + 1] + (q / scs) % scs;
         vz = sc_pos[3 * sc + 2] + q / (scs * scs);
         beg = sc_off[sc];
-        cnt = sc_cnt[sc];
+        cnt = accumulate ? sc_cnt[sc] : sc_use[sc];
     }
     const u64 sc0 = brick * spb;
     const uint32_t bbeg = sc_off[sc0];
@@ -267,19 +267,100 @@ __global__ __launch_bounds__(512) void k_gather(Tree t, int scs, float radius, c
     store_atlas<AT>(atlas, off, val);
 }
 
+// Row-parallel gather for 8^3 bricks with 4^3 subcells: one wave per subcell,
+// lane l owns the 4-voxel x-row (y, z) = (l & 3, (l >> 2) & 3) of the subcell
+// for the entries j = l >> 4 (mod 4), so a wave step evaluates four entries on
+// all 64 voxels; the four partial argmins per voxel are merged with shuffles
+// (smallest d^2, ties to the earlier entry, as one sequential pass).
+template <typename AT>
+__global__ __launch_bounds__(512) void k_gather_rows(Tree t, float radius, const int* sc_nid, const uint32_t* sc_use,
+                                                     const uint32_t* sc_off, const int* sc_pos, const float* sc_pnt,
+                                                     AT* atlas) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const u64 sc = u64(blockIdx.x) * 8 + wave;
+    const int row = lane & 15, g = lane >> 4;
+    const int ry = row & 3, rz = row >> 2;
+    const int sx = sc_pos[3 * sc], sy = sc_pos[3 * sc + 1], sz = sc_pos[3 * sc + 2];
+    const uint32_t beg = sc_off[sc], use = sc_use[sc];
+    const float fy = float(sy + ry), fz = float(sz + rz);
+    const float fx0 = float(sx), fx1 = float(sx + 1), fx2 = float(sx + 2), fx3 = float(sx + 3);
+    float b0 = INFINITY, b1 = INFINITY, b2 = INFINITY, b3 = INFINITY;
+    uint32_t a0 = 0xFFFFFFFFu, a1 = 0xFFFFFFFFu, a2 = 0xFFFFFFFFu, a3 = 0xFFFFFFFFu;
+    const float* base = sc_pnt + 3 * u64(beg);
+    // 64 entries per chunk: one coalesced load per lane, broadcast by shuffles
+    for (uint32_t c0 = 0; c0 < use; c0 += 64) {
+        const uint32_t jl = c0 + lane;
+        float qx = 0.f, qy = 0.f, qz = 0.f;
+        if (jl < use) {
+            const float* q = base + 3 * u64(jl);
+            qx = q[0]; qy = q[1]; qz = q[2];
+        }
+        const uint32_t nn = min(64u, use - c0);
+        for (uint32_t k = 0; k < nn; k += 4) {
+            const int src = int(k) + g;
+            const float px = __shfl(qx, src, 64), py = __shfl(qy, src, 64), pz = __shfl(qz, src, 64);
+            if (uint32_t(src) < nn) {
+                const uint32_t j = c0 + uint32_t(src);
+                const float dy = py - fy, dz = pz - fz;
+                const float dyz = __builtin_fmaf(dy, dy, dz * dz);
+                float dx, d2;
+                dx = px - fx0; d2 = __builtin_fmaf(dx, dx, dyz); if (d2 < b0) { b0 = d2; a0 = j; }
+                dx = px - fx1; d2 = __builtin_fmaf(dx, dx, dyz); if (d2 < b1) { b1 = d2; a1 = j; }
+                dx = px - fx2; d2 = __builtin_fmaf(dx, dx, dyz); if (d2 < b2) { b2 = d2; a2 = j; }
+                dx = px - fx3; d2 = __builtin_fmaf(dx, dx, dyz); if (d2 < b3) { b3 = d2; a3 = j; }
+            }
+        }
+    }
+    float bb[4] = {b0, b1, b2, b3};
+    uint32_t aa[4] = {a0, a1, a2, a3};
+#pragma unroll
+    for (int sh = 16; sh <= 32; sh <<= 1) {
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+            float ob = __shfl_xor(bb[x], sh, 64);
+            uint32_t oa = __shfl_xor(aa[x], sh, 64);
+            if (ob < bb[x] || (ob == bb[x] && oa < aa[x])) { bb[x] = ob; aa[x] = oa; }
+        }
+    }
+    if (g != 0) return;
+    const Node* nd = t.nodes[0] + sc_nid[sc];
+    const int vy = sy + ry, vz = sz + rz;
+    const u64 off = atlas_offset(t, nd->mValue[0] + (sx - nd->mPos[0]), nd->mValue[1] + (vy - nd->mPos[1]),
+                                 nd->mValue[2] + (vz - nd->mPos[2]));
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+        float val = 0.0f;
+        if (aa[x] != 0xFFFFFFFFu) {
+            const float* p = base + 3 * u64(aa[x]);
+            val = fmaxf(0.0f, falloff_accurate(p[0], p[1], p[2], sx + x, vy, vz, radius));
+        }
+        store_atlas<AT>(atlas, off + x, val);
+    }
+}
+
 hipError_t launch_gather_density(Tree t, int scs, int nbricks_sc, float radius, const int* sc_nid,
-                                 const uint32_t* sc_cnt, const uint32_t* sc_off, const int* sc_pos,
-                                 const float* sc_pnt, void* atlas, int atlas_type, int accumulate, hipStream_t s) {
+                                 const uint32_t* sc_cnt, const uint32_t* sc_use, const uint32_t* sc_off,
+                                 const int* sc_pos, const float* sc_pnt, void* atlas, int atlas_type, int accumulate,
+                                 hipStream_t s) {
     if (nbricks_sc <= 0) return hipSuccess;
     int threads = t.res[0] * t.res[0] * t.res[0];
     if (threads > 512) return hipErrorInvalidValue;
     threads = (threads + 63) / 64 * 64;
-    if (atlas_type == T_HALF)
+    const bool rows = !accumulate && t.res[0] == 8 && scs == 4;
+    if (rows) {
+        if (atlas_type == T_HALF)
+            hipLaunchKernelGGL(k_gather_rows<__half>, dim3(nbricks_sc), dim3(512), 0, s, t, radius, sc_nid, sc_use,
+                               sc_off, sc_pos, sc_pnt, (__half*)atlas);
+        else
+            hipLaunchKernelGGL(k_gather_rows<float>, dim3(nbricks_sc), dim3(512), 0, s, t, radius, sc_nid, sc_use,
+                               sc_off, sc_pos, sc_pnt, (float*)atlas);
+    } else if (atlas_type == T_HALF) {
         hipLaunchKernelGGL(k_gather<__half>, dim3(nbricks_sc), dim3(threads), 0, s, t, scs, radius, sc_nid, sc_cnt,
-                           sc_off, sc_pos, sc_pnt, (__half*)atlas, accumulate);
-    else
+                           sc_use, sc_off, sc_pos, sc_pnt, (__half*)atlas, accumulate);
+    } else {
         hipLaunchKernelGGL(k_gather<float>, dim3(nbricks_sc), dim3(threads), 0, s, t, scs, radius, sc_nid, sc_cnt,
-                           sc_off, sc_pos, sc_pnt, (float*)atlas, accumulate);
+                           sc_use, sc_off, sc_pos, sc_pnt, (float*)atlas, accumulate);
+    }
     return hipGetLastError();
 }
 
@@ -308,11 +389,17 @@ __global__ __launch_bounds__(256) void k_leaf_bounds(Tree t, int nleaves, int* o
             mx[a] = max(mx[a], __shfl_xor(mx[a], d, 64));
         }
     }
+    __shared__ int smn[4][3], smx[4][3];
+    const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
-        for (int a = 0; a < 3; a++) {
-            atomicMin(&out6[a], mn[a]);
-            atomicMax(&out6[3 + a], mx[a]);
-        }
+        for (int a = 0; a < 3; a++) { smn[wave][a] = mn[a]; smx[wave][a] = mx[a]; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int a = threadIdx.x, lo = smn[0][a], hi = smx[0][a];
+        for (int w = 1; w < 4; w++) { lo = min(lo, smn[w][a]); hi = max(hi, smx[w][a]); }
+        atomicMin(&out6[a], lo);
+        atomicMax(&out6[3 + a], hi);
+    }
 }
 
 hipError_t launch_leaf_bounds(Tree t, int nleaves, int* out6, hipStream_t s) {
@@ -322,7 +409,7 @@ hipError_t launch_leaf_bounds(Tree t, int nleaves, int* out6, hipStream_t s) {
     if (e != hipSuccess) return e;
     if (nleaves <= 0) return hipSuccess;
     int blocks = (nleaves + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
+    if (blocks > 256) blocks = 256;
     hipLaunchKernelGGL(k_leaf_bounds, dim3(blocks), dim3(256), 0, s, t, nleaves, out6);
     return hipGetLastError();
 }
@@ -385,92 +472,137 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(PointSrc pts, float tx, 
 static const int PULL_THREADS = 256;
 static const int PULL_MAX_CELLS = 512;   // (spd + 2R)^3 upper bound handled
 static const int PULL_MAX_SPB = 32;
+static const int PULL_CAP = 4096;        // pulled points per LDS window
 
-// Per-brick setup shared by the count and write passes: the neighbourhood
-// cells' bucket ranges and their exclusive prefix.  Returns the number of
-// points to test (0 if the brick is unreachable or has no subcells).
 struct PullShared {
     uint32_t cell_off[PULL_MAX_CELLS];
-    uint32_t cell_start[PULL_MAX_CELLS + 1];
+    uint32_t cell_start[PULL_MAX_CELLS];
+    uint16_t owner[PULL_CAP];
     int nbr[27];
-    int ok;
-    uint32_t wave_cnt[PULL_THREADS / 64][PULL_MAX_SPB];
+    uint32_t wsum[PULL_THREADS / 64];
+    uint32_t wave_cnt[PULL_THREADS / 64][PULL_MAX_SPB];   // binned entries that can raise a voxel
+    uint32_t wave_zro[PULL_THREADS / 64][PULL_MAX_SPB];   // binned entries that cannot
     uint32_t run[PULL_MAX_SPB];
+    uint32_t runz[PULL_MAX_SPB];
+    uint32_t use[PULL_MAX_SPB];
 };
 
+// Per-brick setup shared by the count and write passes: the bucket ranges of
+// the (sd+2R)^3 subcells around the brick and their exclusive prefix
+// (workgroup scan).  nbr27[13] < 0 marks a brick the top node cannot reach,
+// which receives no entries.  Returns the number of points to test.
 __device__ inline uint32_t pull_setup(const Tree& t, int brick, int scs, int R, const int* nbr27,
                                       const uint32_t* bcnt, const uint32_t* boff, PullShared& S) {
     const int sd = t.res[0] / scs;
     const int span = sd + 2 * R;
     const int ncell = span * span * span;
     const int spb = sd * sd * sd;
-    if (threadIdx.x == 0) {
-        const Node* n = t.nodes[0] + brick;
-        // only bricks reachable from the top node receive entries (getPosLeafParent)
-        S.ok = (n->mParent != ID_UNDEF64) && pos_leaf(t, n->mPos[0], n->mPos[1], n->mPos[2]) == brick;
-    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (threadIdx.x < 27) S.nbr[threadIdx.x] = nbr27[u64(brick) * 27 + threadIdx.x];
     __syncthreads();
-    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        int cx = c % span - R, cy = (c / span) % span - R, cz = c / (span * span) - R;
-        int dx = cx < 0 ? -1 : (cx >= sd ? 1 : 0), dy = cy < 0 ? -1 : (cy >= sd ? 1 : 0), dz = cz < 0 ? -1 : (cz >= sd ? 1 : 0);
-        int nb = S.nbr[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+    const bool ok = S.nbr[13] >= 0;
+    // each thread owns two consecutive cells for the scan
+    uint32_t cnt2[2];
+    for (int k = 0; k < 2; k++) {
+        int c = 2 * threadIdx.x + k;
         uint32_t cnt = 0, off = 0;
-        if (S.ok && nb >= 0) {
-            int lx = cx - dx * sd, ly = cy - dy * sd, lz = cz - dz * sd;
-            u64 b = u64(nb) * spb + u64((lz * sd + ly) * sd + lx);
-            cnt = bcnt[b];
-            off = boff[b];
+        if (c < ncell && ok) {
+            int cx = c % span - R, cy = (c / span) % span - R, cz = c / (span * span) - R;
+            int dx = cx < 0 ? -1 : (cx >= sd ? 1 : 0), dy = cy < 0 ? -1 : (cy >= sd ? 1 : 0), dz = cz < 0 ? -1 : (cz >= sd ? 1 : 0);
+            int nb = S.nbr[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+            if (nb >= 0) {
+                int lx = cx - dx * sd, ly = cy - dy * sd, lz = cz - dz * sd;
+                u64 b = u64(nb) * spb + u64((lz * sd + ly) * sd + lx);
+                cnt = bcnt[b];
+                off = boff[b];
+            }
         }
-        S.cell_off[c] = off;
-        S.cell_start[c + 1] = cnt;   // counts, prefix-summed below
+        if (c < ncell) S.cell_off[c] = off;
+        cnt2[k] = cnt;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        S.cell_start[0] = 0;
-        for (int c = 0; c < ncell; c++) { acc += S.cell_start[c + 1]; S.cell_start[c + 1] = acc; }
+    uint32_t mine = cnt2[0] + cnt2[1];
+    uint32_t x = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
     }
+    if (lane == 63) S.wsum[wave] = x;
     __syncthreads();
-    return S.cell_start[ncell];
+    uint32_t pre = x - mine, total = 0;
+    for (int w = 0; w < PULL_THREADS / 64; w++) {
+        if (w < wave) pre += S.wsum[w];
+        total += S.wsum[w];
+    }
+    int c0 = 2 * threadIdx.x;
+    if (c0 < ncell) S.cell_start[c0] = pre;
+    if (c0 + 1 < ncell) S.cell_start[c0 + 1] = pre + cnt2[0];
+    return total;
 }
 
-// Target mask of one pulled point over the brick's spb subcells.
-__device__ inline uint32_t pull_mask(const Node* brick, int scs, int sd, float r, float w0, float w1, float w2) {
-    int lo0 = (int(w0 - r) / scs) * scs, hi0 = (int(w0 + 1 + r) / scs) * scs;
-    int lo1 = (int(w1 - r) / scs) * scs, hi1 = (int(w1 + 1 + r) / scs) * scs;
-    int lo2 = (int(w2 - r) / scs) * scs, hi2 = (int(w2 + 1 + r) / scs) * scs;
-    uint32_t mx = 0, my = 0, mz = 0;
-    for (int a = 0; a < sd; a++) {
-        int sx = brick->mPos[0] + a * scs, sy = brick->mPos[1] + a * scs, sz = brick->mPos[2] + a * scs;
-        mx |= (sx >= lo0 && sx <= hi0) ? (1u << a) : 0u;
-        my |= (sy >= lo1 && sy <= hi1) ? (1u << a) : 0u;
-        mz |= (sz >= lo2 && sz <= hi2) ? (1u << a) : 0u;
+// Cell of every pulled point in the window [w0, w0 + PULL_CAP), written by
+// the cell owners (replaces a search per point).
+__device__ inline void pull_owners(PullShared& S, int ncell, uint32_t w0, uint32_t T) {
+    for (int c = threadIdx.x; c < ncell; c += PULL_THREADS) {
+        uint32_t beg = S.cell_start[c];
+        uint32_t end = (c + 1 < ncell) ? S.cell_start[c + 1] : T;
+        uint32_t lo = max(beg, w0);
+        for (uint32_t f = lo; f < end && f < w0 + PULL_CAP; f++) S.owner[f - w0] = uint16_t(c);
     }
-    uint32_t m = 0;
+}
+
+// Integer interval [a, b] of v with |p - v| < r.  p - v is exact in float for
+// these magnitudes, so the end tests are exact.
+__device__ inline void hit_interval(float p, float r, int& a, int& b) {
+    a = int(floorf(p - r));
+    if (p - float(a) >= r) a++;
+    if (p - float(a - 1) < r) a--;
+    b = int(ceilf(p + r));
+    if (float(b) - p >= r) b--;
+    if (float(b + 1) - p < r) b++;
+}
+
+// Masks of one pulled point over the brick's spb subcells: `m` the subcells
+// it is binned into (the reference's per-axis test), `u` those where it can
+// raise some voxel (|p - v| < r on every axis for a voxel v of the subcell).
+__device__ inline void pull_mask(const Node* brick, int scs, int sd, float r, float w0, float w1, float w2,
+                                 uint32_t& m, uint32_t& u) {
+    const float w[3] = {w0, w1, w2};
+    uint32_t mm[3], uu[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        int lo = (int(w[ax] - r) / scs) * scs, hi = (int(w[ax] + 1 + r) / scs) * scs;
+        int a, b;
+        hit_interval(w[ax], r, a, b);
+        uint32_t mb = 0, ub = 0;
+        for (int i = 0; i < sd; i++) {
+            int s = brick->mPos[ax] + i * scs;
+            mb |= (s >= lo && s <= hi) ? (1u << i) : 0u;
+            ub |= (max(a, s) <= min(b, s + scs - 1)) ? (1u << i) : 0u;
+        }
+        mm[ax] = mb;
+        uu[ax] = ub;
+    }
+    m = 0;
+    u = 0;
     for (int z = 0; z < sd; z++)
-        if (mz & (1u << z))
-            for (int y = 0; y < sd; y++)
-                if (my & (1u << y))
-                    for (int x = 0; x < sd; x++)
-                        if (mx & (1u << x)) m |= 1u << ((z * sd + y) * sd + x);
-    return m;
+        for (int y = 0; y < sd; y++)
+            for (int x = 0; x < sd; x++) {
+                uint32_t bit = 1u << ((z * sd + y) * sd + x);
+                if ((mm[2] >> z) & (mm[1] >> y) & (mm[0] >> x) & 1u) m |= bit;
+                if ((uu[2] >> z) & (uu[1] >> y) & (uu[0] >> x) & 1u) u |= bit;
+            }
+    u &= m;
 }
 
-__device__ inline int find_cell(const PullShared& S, int ncell, uint32_t f) {
-    int lo = 0, hi = ncell - 1;   // largest c with cell_start[c] <= f
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (S.cell_start[mid] <= f) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
+// Entries inside a subcell are placed with the ones that can raise a voxel
+// first (the reference leaves the order inside a subcell unspecified); their
+// number per subcell goes to sc_use, so the gather can stop there.
 template <bool WRITE>
 __global__ __launch_bounds__(PULL_THREADS) void k_pull(Tree t, int scs, int R, float radius, const int* nbr27,
                                                        const uint32_t* bcnt, const uint32_t* boff, const float* sorted,
-                                                       const uint32_t* mapping, uint32_t* sc_cnt, const uint32_t* sc_off,
-                                                       float* sc_pnt) {
+                                                       const uint32_t* mapping, uint32_t* sc_cnt, uint32_t* sc_use,
+                                                       const uint32_t* sc_off, float* sc_pnt) {
     __shared__ PullShared S;
     const int brick = blockIdx.x;
     const Node* bn = t.nodes[0] + brick;
@@ -479,51 +611,72 @@ __global__ __launch_bounds__(PULL_THREADS) void k_pull(Tree t, int scs, int R, f
     const int span = sd + 2 * R;
     const int ncell = span * span * span;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (threadIdx.x < PULL_MAX_SPB) S.run[threadIdx.x] = 0;
-    uint32_t T = pull_setup(t, brick, scs, R, nbr27, bcnt, boff, S);
     const bool linked = bn->mParent != ID_UNDEF64;
     const u64 sc0 = linked ? u64(mapping[brick]) * spb : 0;
-    for (uint32_t base = 0; base < T; base += PULL_THREADS) {
-        uint32_t f = base + threadIdx.x;
-        uint32_t m = 0;
-        float w0 = 0, w1 = 0, w2 = 0;
-        if (f < T) {
-            int c = find_cell(S, ncell, f);
-            u64 e = u64(S.cell_off[c]) + (f - S.cell_start[c]);
-            w0 = sorted[3 * e]; w1 = sorted[3 * e + 1]; w2 = sorted[3 * e + 2];
-            m = pull_mask(bn, scs, sd, radius, w0, w1, w2);
-        }
-        for (int k = 0; k < spb; k++) {
-            u64 bal = __ballot((m >> k) & 1u);
-            if (lane == 0) S.wave_cnt[wave][k] = uint32_t(__popcll(bal));
-        }
+    if (threadIdx.x < PULL_MAX_SPB) {
+        S.run[threadIdx.x] = 0;
+        S.runz[threadIdx.x] = 0;
+        S.use[threadIdx.x] = (WRITE && linked && threadIdx.x < spb) ? sc_use[sc0 + threadIdx.x] : 0u;
+    }
+    const uint32_t T = pull_setup(t, brick, scs, R, nbr27, bcnt, boff, S);
+    __syncthreads();
+    for (uint32_t w0 = 0; w0 < T; w0 += PULL_CAP) {
+        pull_owners(S, ncell, w0, T);
         __syncthreads();
-        if (WRITE) {
-            // per target: exclusive rank = entries of earlier chunks + earlier waves + lower lanes
+        const uint32_t wend = min(T, w0 + PULL_CAP);
+        for (uint32_t base = w0; base < wend; base += PULL_THREADS) {
+            const uint32_t f = base + threadIdx.x;
+            uint32_t m = 0, u = 0;
+            float w0f = 0, w1f = 0, w2f = 0;
+            if (f < wend) {
+                const int c = S.owner[f - w0];
+                const u64 e = u64(S.cell_off[c]) + (f - S.cell_start[c]);
+                w0f = sorted[3 * e]; w1f = sorted[3 * e + 1]; w2f = sorted[3 * e + 2];
+                pull_mask(bn, scs, sd, radius, w0f, w1f, w2f, m, u);
+            }
+            const uint32_t z = m & ~u;
             for (int k = 0; k < spb; k++) {
-                bool on = (m >> k) & 1u;
-                u64 bal = __ballot(on);
-                if (on) {
-                    uint32_t before = 0;
-                    for (int w = 0; w < wave; w++) before += S.wave_cnt[w][k];
-                    u64 lower = lane ? (bal & ((~0ull) >> (64 - lane))) : 0ull;
-                    uint32_t rank = S.run[k] + before + uint32_t(__popcll(lower));
-                    u64 e = u64(sc_off[sc0 + k]) + rank;
-                    sc_pnt[3 * e] = w0;
-                    sc_pnt[3 * e + 1] = w1;
-                    sc_pnt[3 * e + 2] = w2;
+                u64 bu = __ballot((u >> k) & 1u);
+                u64 bz = __ballot((z >> k) & 1u);
+                if (lane == 0) {
+                    S.wave_cnt[wave][k] = uint32_t(__popcll(bu));
+                    S.wave_zro[wave][k] = uint32_t(__popcll(bz));
                 }
             }
+            __syncthreads();
+            if (WRITE) {
+                // exclusive rank = entries of earlier chunks + earlier waves + lower lanes
+                const u64 lower = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+                for (int k = 0; k < spb; k++) {
+                    bool onu = (u >> k) & 1u, onz = (z >> k) & 1u;
+                    u64 bu = __ballot(onu);
+                    u64 bz = __ballot(onz);
+                    if (onu || onz) {
+                        uint32_t before = 0;
+                        for (int w = 0; w < wave; w++) before += onu ? S.wave_cnt[w][k] : S.wave_zro[w][k];
+                        uint32_t rank = onu ? S.run[k] + before + uint32_t(__popcll(bu & lower))
+                                            : S.use[k] + S.runz[k] + before + uint32_t(__popcll(bz & lower));
+                        u64 e = u64(sc_off[sc0 + k]) + rank;
+                        sc_pnt[3 * e] = w0f;
+                        sc_pnt[3 * e + 1] = w1f;
+                        sc_pnt[3 * e + 2] = w2f;
+                    }
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x < spb) {
+                uint32_t su = 0, sz = 0;
+                for (int w = 0; w < PULL_THREADS / 64; w++) { su += S.wave_cnt[w][threadIdx.x]; sz += S.wave_zro[w][threadIdx.x]; }
+                S.run[threadIdx.x] += su;
+                S.runz[threadIdx.x] += sz;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if (threadIdx.x < spb) {
-            uint32_t s = 0;
-            for (int w = 0; w < PULL_THREADS / 64; w++) s += S.wave_cnt[w][threadIdx.x];
-            S.run[threadIdx.x] += s;
-        }
-        __syncthreads();
     }
-    if (!WRITE && linked && threadIdx.x < spb) sc_cnt[sc0 + threadIdx.x] = S.run[threadIdx.x];
+    if (!WRITE && linked && threadIdx.x < spb) {
+        sc_cnt[sc0 + threadIdx.x] = S.run[threadIdx.x] + S.runz[threadIdx.x];
+        sc_use[sc0 + threadIdx.x] = S.run[threadIdx.x];
+    }
 }
 
 hipError_t launch_bucket_points(Tree t, PointSrc pts, int scs, int spb, const float trans[3], uint32_t* key,
@@ -548,15 +701,17 @@ int pull_max_spb() { return PULL_MAX_SPB; }
 
 hipError_t launch_pull(int write, Tree t, int nleaves, int scs, int R, float radius, const int* nbr27,
                        const uint32_t* bcnt, const uint32_t* boff, const float* sorted, const uint32_t* mapping,
-                       uint32_t* sc_cnt, const uint32_t* sc_off, float* sc_pnt, hipStream_t s) {
+                       uint32_t* sc_cnt, uint32_t* sc_use, const uint32_t* sc_off, float* sc_pnt, hipStream_t s) {
     if (nleaves <= 0) return hipSuccess;
     if (write)
         hipLaunchKernelGGL(k_pull<true>, dim3(nleaves), dim3(PULL_THREADS), 0, s, t, scs, R, radius, nbr27, bcnt, boff,
-                           sorted, mapping, sc_cnt, sc_off, sc_pnt);
+                           sorted, mapping, sc_cnt, sc_use, sc_off, sc_pnt);
     else
         hipLaunchKernelGGL(k_pull<false>, dim3(nleaves), dim3(PULL_THREADS), 0, s, t, scs, R, radius, nbr27, bcnt, boff,
-                           sorted, mapping, sc_cnt, sc_off, sc_pnt);
+                           sorted, mapping, sc_cnt, sc_use, sc_off, sc_pnt);
     return hipGetLastError();
 }
 
 }  // namespace gvdb
+
+

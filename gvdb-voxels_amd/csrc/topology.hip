@@ -97,10 +97,16 @@ __device__ inline void set_bit(uint32_t* bm, u64 idx) {
 }
 
 // gvdbCalcBrickId (kernels/cuda_gvdb_particles.cuh:352-382) for level 0 and
-// gvdbCalcExtraBrickId (:467-512) for the dilation, as bitmap marks.
+// gvdbCalcExtraBrickId (:467-512) for the dilation.  Each point contributes
+// its brick and, for the dilation, the bricks covering [int(p)-r, int(p)+2r]
+// on every axis.  When those lie within one brick of the point's own (r up to
+// one brick), the point ORs a 27-bit direction mask into its own brick's word
+// (`masks`, bit 13 = the brick itself), which k_expand_masks turns into bitmap
+// bits; this replaces up to 8 bitmap atomics per point by one rarely taken
+// atomic.  Points with wider reach (or no mask array) set the bits directly.
 __global__ __launch_bounds__(256) void k_mark_points(PointSrc pts, float ox, float oy, float oz, float radius,
                                                      int dilate, int range0, float inv_range0, BitDomain d0,
-                                                     uint32_t* bmA, uint32_t* bmB) {
+                                                     uint32_t* bmA, uint32_t* bmB, uint32_t* masks) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < pts.n; i += gridDim.x * blockDim.x) {
         const float* p = point_at(pts, i);
         float w[3] = {p[0] + ox, p[1] + oy, p[2] + oz};
@@ -110,9 +116,9 @@ __global__ __launch_bounds__(256) void k_mark_points(PointSrc pts, float ox, flo
 #pragma unroll
         for (int a = 0; a < 3; a++) c[a] = int(floorf(w[a] * inv_range0));
         u64 idx;
-        if (dom_index(d0, c[0], c[1], c[2], idx)) set_bit(bmA, idx);
+        const bool inside = dom_index(d0, c[0], c[1], c[2], idx);
+        int lo[3] = {c[0], c[1], c[2]}, hi[3] = {c[0], c[1], c[2]};
         if (dilate) {
-            int lo[3], hi[3];
 #pragma unroll
             for (int a = 0; a < 3; a++) {
                 int iw = int(w[a]);
@@ -121,6 +127,23 @@ __global__ __launch_bounds__(256) void k_mark_points(PointSrc pts, float ox, flo
                 lo[a] = bmin / range0;   // C truncation, as the reference's int division
                 hi[a] = bmax / range0;
             }
+        }
+        const bool near = lo[0] >= c[0] - 1 && lo[1] >= c[1] - 1 && lo[2] >= c[2] - 1 && hi[0] <= c[0] + 1 &&
+                          hi[1] <= c[1] + 1 && hi[2] <= c[2] + 1;
+        if (masks && inside && near) {
+            uint32_t m = 0;
+            for (int x = lo[0]; x <= hi[0]; x++)
+                for (int y = lo[1]; y <= hi[1]; y++)
+                    for (int z = lo[2]; z <= hi[2]; z++)
+                        m |= 1u << ((z - c[2] + 1) * 9 + (y - c[1] + 1) * 3 + (x - c[0] + 1));
+            m |= 1u << 13;
+            uint32_t* wd = masks + idx;
+            if ((__hip_atomic_load(wd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) != m)
+                __hip_atomic_fetch_or(wd, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            continue;
+        }
+        if (inside) set_bit(bmA, idx);
+        if (dilate) {
             for (int x = lo[0]; x <= hi[0]; x++)
                 for (int y = lo[1]; y <= hi[1]; y++)
                     for (int z = lo[2]; z <= hi[2]; z++)
@@ -129,13 +152,38 @@ __global__ __launch_bounds__(256) void k_mark_points(PointSrc pts, float ox, flo
     }
 }
 
+// Direction masks -> level-0 bitmaps: bit 13 marks the brick itself in A (and
+// in B), the other bits mark the neighbours in B.
+__global__ __launch_bounds__(256) void k_expand_masks(BitDomain d0, const uint32_t* masks, uint32_t* bmA, uint32_t* bmB) {
+    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < d0.nbits; i += u64(gridDim.x) * blockDim.x) {
+        uint32_t m = masks[i];
+        if (!m) continue;
+        int x, y, z;
+        dom_coords(d0, i, x, y, z);
+        if (m & (1u << 13)) set_bit(bmA, i);
+        if (!bmB) continue;
+        while (m) {
+            int d = __ffs(m) - 1;
+            m &= m - 1;
+            u64 j;
+            if (dom_index(d0, x + d % 3 - 1, y + (d / 3) % 3 - 1, z + d / 9 - 1, j)) set_bit(bmB, j);
+        }
+    }
+}
+
 hipError_t launch_mark_points(PointSrc pts, const float orig[3], float radius, int dilate, int range0, BitDomain d0,
-                              uint32_t* bmA, uint32_t* bmB, hipStream_t s) {
+                              uint32_t* bmA, uint32_t* bmB, uint32_t* masks, hipStream_t s) {
     int blocks = (pts.n + 255) / 256;
     if (blocks > 16384) blocks = 16384;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_mark_points, dim3(blocks), dim3(256), 0, s, pts, orig[0], orig[1], orig[2], radius, dilate,
-                       range0, 1.0f / float(range0), d0, bmA, bmB);
+                       range0, 1.0f / float(range0), d0, bmA, bmB, masks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !masks) return e;
+    u64 b2 = (d0.nbits + 255) / 256;
+    if (b2 > 16384) b2 = 16384;
+    if (b2 < 1) b2 = 1;
+    hipLaunchKernelGGL(k_expand_masks, dim3(unsigned(b2)), dim3(256), 0, s, d0, masks, bmA, dilate ? bmB : nullptr);
     return hipGetLastError();
 }
 

@@ -66,8 +66,8 @@ Volume::~Volume() {
         for (int l = 0; l < MAXLEV; l++) pool_[g][l].dev.release();
     for (auto& c : chan_) c.data.release();
     for (auto& r : rbuf_) r.data.release();
-    DevBuf* bufs[] = {&amap_, &nbr_, &pts_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_,
-                      &bkt_key_, &bkt_slot_, &bkt_cnt_, &bkt_off_, &sorted_, &sc_flags_, &small_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev};
+    DevBuf* bufs[] = {&amap_, &nbr_, &pts_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_, &sc_use_,
+                      &bkt_key_, &bkt_slot_, &bkt_cnt_, &bkt_off_, &sorted_, &sc_flags_, &small_, &masks_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev};
     for (DevBuf* b : bufs) b->release();
     for (int s = 0; s < ST_COUNT; s++)
         for (int k = 0; k < 2; k++)
@@ -437,7 +437,15 @@ int Volume::rebuild_topology(int n, float radius, const float orig[3]) {
     HIPCHK(launch_fill_u32(bm, 0u, bm_words, stream));
     auto A = [&](int L) { return bm + bm_off[L]; };
     auto B = [&](int L) { return dilate ? bm + bm_off[L] + dom[L].nwords : nullptr; };
-    HIPCHK(launch_mark_points(P, orig, radius, dilate ? 1 : 0, int(range_[0]), dom[0], A(0), B(0), stream));
+    // direction masks cost 4 bytes per domain brick (filled, then scanned
+    // once): used while that stays comparable to the point stream itself
+    uint32_t* masks = nullptr;
+    if (dom[0].nbits <= std::max(u64(1) << 22, u64(P.n) * 4)) {
+        HIPCHK(masks_.ensure(size_t(dom[0].nbits) * 4));
+        masks = masks_.as<uint32_t>();
+        HIPCHK(launch_fill_u32(masks, 0u, dom[0].nbits, stream));
+    }
+    HIPCHK(launch_mark_points(P, orig, radius, dilate ? 1 : 0, int(range_[0]), dom[0], A(0), B(0), masks, stream));
     for (int L = 1; L < D; L++)
         HIPCHK(launch_mark_parents(dom[L - 1], A(L - 1), B(L - 1), logdim_[L], dom[L], A(L), B(L), stream));
     for (int L = 0; L < D; L++) {
@@ -787,6 +795,7 @@ int Volume::insert_points_subcell(int scs, int n, float radius, const float tran
     HIPCHK(sc_flags_.ensure(size_t(numMap) * 4));
     HIPCHK(sc_map_.ensure(size_t(numMap) * 4));
     HIPCHK(sc_cnt_.ensure(size_t(numSC) * 4));
+    HIPCHK(sc_use_.ensure(size_t(numSC) * 4));
     HIPCHK(sc_off_.ensure(size_t(numSC) * 4));
     HIPCHK(scan_tmp_.ensure(scan_tmp_words(std::max(numSC, numMap)) * 4 + 256));
     Tree t = make_tree();
@@ -824,7 +833,7 @@ int Volume::insert_points_subcell(int scs, int n, float radius, const float tran
         HIPCHK(launch_fill_u32(sc_cnt_.as<uint32_t>(), 0u, numSC, stream));
         HIPCHK(launch_pull(0, t, int(numMap), scs, R, radius, nbr_.as<int>(), bkt_cnt_.as<uint32_t>(),
                            bkt_off_.as<uint32_t>(), sorted_.as<float>(), sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
-                           sc_off_.as<uint32_t>(), nullptr, stream));
+                           sc_use_.as<uint32_t>(), sc_off_.as<uint32_t>(), nullptr, stream));
         HIPCHK(launch_scan_u32(sc_cnt_.as<uint32_t>(), nullptr, 0, sc_off_.as<uint32_t>(), numSC, small + 9,
                                scan_tmp_.as<uint32_t>(), stream));
         HIPCHK(hipMemcpyAsync(pinned_ + 8, small + 8, 16, hipMemcpyDeviceToHost, stream));
@@ -861,11 +870,13 @@ int Volume::insert_points_subcell(int scs, int n, float radius, const float tran
     if (use_pull) {
         HIPCHK(launch_pull(1, t, int(numMap), scs, R, radius, nbr_.as<int>(), bkt_cnt_.as<uint32_t>(),
                            bkt_off_.as<uint32_t>(), sorted_.as<float>(), sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
-                           sc_off_.as<uint32_t>(), sc_pnt_.as<float>(), stream));
+                           sc_use_.as<uint32_t>(), sc_off_.as<uint32_t>(), sc_pnt_.as<float>(), stream));
     } else {
         HIPCHK(launch_fill_u32(sc_cnt_.as<uint32_t>(), 0u, numSC, stream));
         HIPCHK(launch_insert_subcell(t, P, scs, spb, trans, radius, sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
                                      sc_off_.as<uint32_t>(), sc_pnt_.as<float>(), stream));
+        // unordered entries: the gather considers all of them
+        HIPCHK(hipMemcpyAsync(sc_use_.p, sc_cnt_.p, size_t(numSC) * 4, hipMemcpyDeviceToDevice, stream));
     }
     HIPCHK(launch_subcell_pos(t, scs, spb, sc_map_.as<uint32_t>(), sc_pos_.as<int>(), sc_nid_.as<int>(), int(numMap),
                               stream));
@@ -887,8 +898,8 @@ int Volume::gather_density(int scs, int n, float radius, const float trans[3], i
     Tree t = make_tree();
     (void)hipEventRecord(kev_[0][0], stream);
     HIPCHK(launch_gather_density(t, scs, sc_bricks_, radius, sc_nid_.as<int>(), sc_cnt_.as<uint32_t>(),
-                                 sc_off_.as<uint32_t>(), sc_pos_.as<int>(), sc_pnt_.as<float>(), chan_[chanD].data.p,
-                                 chan_[chanD].type, accum ? 1 : 0, stream));
+                                 sc_use_.as<uint32_t>(), sc_off_.as<uint32_t>(), sc_pos_.as<int>(), sc_pnt_.as<float>(),
+                                 chan_[chanD].data.p, chan_[chanD].type, accum ? 1 : 0, stream));
     (void)hipEventRecord(kev_[0][1], stream);
     kev_used_[0] = true;
     stage_end(ST_GATHER);

@@ -148,14 +148,14 @@ private:
     DevBuf pts_owned_;
 
     // subcells
-    DevBuf sc_cnt_, sc_off_, sc_pos_, sc_nid_, sc_map_, sc_pnt_, sc_flags_;
+    DevBuf sc_cnt_, sc_use_, sc_off_, sc_pos_, sc_nid_, sc_map_, sc_pnt_, sc_flags_;
     DevBuf bkt_key_, bkt_slot_, bkt_cnt_, bkt_off_, sorted_;
     int sc_len_ = 0, num_sc_ = 0, sc_bricks_ = 0, sc_map_n_ = 0;
     bool last_insert_pull_ = false;
 
     // topology scratch
     DevBuf small_;       // bbox keys, counts
-    DevBuf bitmaps_, ranks_, scan_tmp_, flags_, slot_scan_;
+    DevBuf bitmaps_, ranks_, masks_, scan_tmp_, flags_, slot_scan_;
     uint32_t* pinned_ = nullptr;   // host staging (64 words)
 
     // render

@@ -1340,6 +1340,14 @@ void orc_clear_channel(void* h) {
     std::fill(v.atlasD.begin(), v.atlasD.end(), 0.0);
 }
 
+// Every atlas voxel (interior and apron) set to `value`: a uniform field for
+// closed-form render checks (no reference counterpart; FillChannel-like).
+void orc_fill_channel(void* h, float value) {
+    Vol& v = *static_cast<Vol*>(h);
+    std::fill(v.atlasF.begin(), v.atlasF.end(), value);
+    std::fill(v.atlasD.begin(), v.atlasD.end(), double(value));
+}
+
 unsigned long long orc_pool_count(void* h, int grp, int lev) { return static_cast<Vol*>(h)->pool[grp][lev].lastEle; }
 unsigned long long orc_pool_stride(void* h, int grp, int lev) { return static_cast<Vol*>(h)->pool[grp][lev].stride; }
 void orc_pool_copy(void* h, int grp, int lev, void* dst) {

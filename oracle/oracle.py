@@ -41,7 +41,7 @@ def lib():
             "orc_set_epsilon": (None, [vp, f]), "orc_undefined_events": (i, [vp]),
             "orc_rebuild_topology": (i, [vp, fp, i, f, fp]), "orc_rebuild_topology_cpu": (i, [vp, fp, i, fp]),
             "orc_clear": (None, [vp]), "orc_finish_topology": (None, [vp]), "orc_update_atlas": (i, [vp]),
-            "orc_clear_channel": (None, [vp]),
+            "orc_clear_channel": (None, [vp]), "orc_fill_channel": (None, [vp, f]),
             "orc_pool_count": (ctypes.c_ulonglong, [vp, i, i]), "orc_pool_stride": (ctypes.c_ulonglong, [vp, i, i]),
             "orc_pool_copy": (None, [vp, i, i, vp]), "orc_top_lev": (i, [vp]), "orc_bounds": (None, [vp, fp]),
             "orc_atlas_info": (None, [vp, ip]), "orc_atlas_copy": (None, [vp, i, vp]),
@@ -107,6 +107,10 @@ class Oracle:
 
     def clear_channel(self):
         lib().orc_clear_channel(self.h)
+
+    def fill_channel(self, value):
+        """Sets every atlas voxel to `value` (uniform field for closed-form checks)."""
+        lib().orc_fill_channel(self.h, float(value))
 
     def undefined_events(self):
         return lib().orc_undefined_events(self.h)

@@ -1,0 +1,375 @@
+"""The CPU oracle pinned against brute-force models and closed forms (no GPU).
+
+The reference ships no tests, golden vectors or fixtures for this path, so the
+oracle is pinned by what the mathematics fixes: set-based topology models on
+small inputs, the closed-form atlas layout, per-point subcell binning, an FP64
+brute-force density field, brute-force apron lookups, and closed-form renders
+(missed rays, a uniform-density emission-absorption slab, an isosurface on an
+axis through one point, the camera's projection).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import model as M
+import oracle as O
+
+TREE = (3, 3, 3, 4, 5)
+
+
+def _cloud(n, seed, lo=70.0, hi=440.0):
+    g = np.random.default_rng(seed)
+    return (lo + (hi - lo) * g.random((n, 3))).astype(np.float32)
+
+
+def _ball(n, seed, centre=(200.0, 180.0, 220.0), radius=30.0):
+    g = np.random.default_rng(seed)
+    d = g.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = radius + g.standard_normal(n)
+    return (np.asarray(centre) + d * r[:, None]).astype(np.float32)
+
+
+def _pools_match(o, expected, levels=5):
+    for lev in range(levels):
+        a = o.pool(0, lev)
+        e = expected[lev]
+        if e is None:
+            assert len(a) == 0, "level %d should be empty" % lev
+            continue
+        assert len(a) == len(e["mPos"]), "level %d: %d nodes, model %d" % (lev, len(a), len(e["mPos"]))
+        for f in ("mLev", "mFlags", "mPos", "mValue", "mParent", "mChildList"):
+            assert np.array_equal(a[f], e[f]), "level %d field %s" % (lev, f)
+        if lev > 0:
+            assert np.array_equal(o.pool(1, lev), e["clists"]), "level %d child lists" % lev
+
+
+# ------------------------------------------------------------------ topology
+@pytest.mark.parametrize("n,seed,radius", [(1, 1, 0.5), (2, 2, 0.5), (40, 3, 0.5), (300, 4, 0.5), (40, 5, 1.0),
+                                           (300, 6, 2.0), (120, 7, 3.0), (200, 8, 1.5)])
+def test_topology_matches_set_model(n, seed, radius):
+    pts = _cloud(n, seed) if n > 2 else (np.array([[100.3, 200.7, 300.1], [350.5, 90.2, 140.9]], np.float32)[:n] + 0)
+    m = M.topology_model(pts, radius)
+    o = O.Oracle()
+    rc = o.rebuild_topology(pts, radius)
+    assert rc == 0
+    if any(len(m["levels"][l]) == 1 for l in range(m["depth"])):
+        # a level below the root with a single node becomes the walk's top
+        # node; linking above it is outside the reference's defined behaviour
+        assert o.top_lev() < m["depth"]
+        return
+    assert o.undefined_events() == 0
+    assert o.top_lev() == m["depth"]
+    _pools_match(o, M.expected_pools(m))
+
+
+def test_topology_sphere_model_and_invariants():
+    pts = _ball(2500, 9)
+    m = M.topology_model(pts, 2.0)
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, 2.0) == 0
+    assert o.undefined_events() == 0
+    _pools_match(o, M.expected_pools(m))
+    # every point lies in a level-0 node
+    leaves = {tuple(p) for p in o.pool(0, 0)["mPos"].tolist()}
+    for p in np.floor(pts / 8).astype(np.int64) * 8:
+        assert tuple(p.tolist()) in leaves
+
+
+def test_topology_translation_and_negative_points():
+    """Points with a negative coordinate after translation get no key."""
+    pts = _cloud(200, 10)
+    orig = (-120.0, 5.0, 0.0)
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, 0.5, orig) == 0
+    w = pts + np.asarray(orig, np.float32)
+    m = M.topology_model(w, 0.5)
+    if all(len(m["levels"][l]) > 1 for l in range(m["depth"])):
+        _pools_match(o, M.expected_pools(m))
+    leaves = {tuple(p) for p in o.pool(0, 0)["mPos"].tolist()}
+    want = {tuple(p) for p in (np.floor(w[np.all(w >= 0, 1)] / 8).astype(np.int64) * 8).tolist()}
+    assert leaves == want
+
+
+def test_bounding_box_and_depth():
+    pts = _ball(500, 11)
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, 2.0) == 0
+    b = o.bounds()
+    assert np.array_equal(b["pos_min"], np.maximum(0, pts.min(0) - np.float32(1)))
+    assert np.array_equal(b["pos_max"], np.maximum(0, pts.max(0) + np.float32(2)))
+    o.finish_topology()
+    leaves = o.pool(0, 0)["mPos"]
+    b = o.bounds()
+    assert np.array_equal(b["obj_min"], leaves.min(0).astype(np.float32))
+    assert np.array_equal(b["obj_max"], (leaves.max(0) + 8).astype(np.float32))
+
+
+def test_host_activate_space_same_node_set_as_rebuild():
+    """RebuildTopologyCPU (host ActivateSpace, insertion order) builds the same
+    node set as the GPU rebuild without dilation; parent and child references
+    agree with each other."""
+    pts = _ball(3000, 12)
+    a = O.Oracle()
+    assert a.rebuild_topology_cpu(pts) == 0
+    b = O.Oracle()
+    assert b.rebuild_topology(pts, 0.5) == 0
+    for lev in range(4):
+        sa = {tuple(p) for p in a.pool(0, lev)["mPos"].tolist()}
+        sb = {tuple(p) for p in b.pool(0, lev)["mPos"].tolist()}
+        assert sa == sb, "level %d" % lev
+    # reference consistency of the host-built pools
+    for lev in range(1, 5):
+        nodes = a.pool(0, lev)
+        if not len(nodes):
+            continue
+        cl = a.pool(1, lev)
+        kids = a.pool(0, lev - 1)
+        for i, nd in enumerate(nodes):
+            lst = cl[int(nd["mChildList"]) >> 16]
+            for slot in np.nonzero(lst != M.ID_UNDEF64)[0]:
+                ci = int(lst[slot]) >> 16
+                assert int(kids[ci]["mParent"]) == M.elem(0, lev, i)
+
+
+# --------------------------------------------------------------------- atlas
+def test_update_atlas_closed_form():
+    pts = _ball(3000, 13)
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, 2.0) == 0
+    o.finish_topology()
+    assert o.update_atlas() == 0
+    leaves = o.pool(0, 0)
+    B = len(leaves)
+    assert B > 256          # forces Z growth
+    info = o.atlas_info()
+    Z = max(1, math.ceil(B / 256))
+    assert info["cnt"] == (16, 16, Z) and info["res"] == (160, 160, 10 * Z) and info["used"] == B
+    assert np.array_equal(leaves["mValue"], M.atlas_slots(B))
+    amap = o.atlas_map()
+    assert len(amap) == 256 * Z
+    assert np.array_equal(amap["mLeafNode"][:B], np.arange(B))
+    assert np.array_equal(amap["mPos"][:B], leaves["mPos"])
+    assert np.all(amap["mLeafNode"][B:] == -1) and np.all(amap["mPos"][B:] == -1)
+
+
+# ------------------------------------------------------------------ subcells
+@pytest.mark.parametrize("radius,scs", [(2.0, 4), (0.5, 4), (3.0, 4), (2.0, 2), (1.5, 8)])
+def test_subcells_match_per_point_binning(radius, scs):
+    pts = _ball(600, 14)
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, radius) == 0
+    o.finish_topology()
+    o.update_atlas()
+    rc, n = o.insert_points_subcell(pts, radius, scs)
+    assert rc == 0
+    leaves = [tuple(p) for p in o.pool(0, 0)["mPos"].tolist()]
+    m = M.subcell_model(pts, radius, leaves, scs)
+    s = o.subcells(n)
+    assert n == int(m["cnt"].sum())
+    assert np.array_equal(s["cnt"].astype(np.int64), m["cnt"])
+    assert np.array_equal(s["off"].astype(np.int64), m["off"])
+    assert np.array_equal(s["pos"].astype(np.int64), m["pos"])
+    assert np.array_equal(s["nid"].astype(np.int64), m["nid"])
+    assert np.array_equal(s["pnt"], m["pnt"])      # canonical (point) order inside each subcell
+
+
+def test_subcells_skip_nohit_and_negative_points():
+    pts = _ball(300, 15)
+    extra = np.array([[100.0, 100.0, 1.0e10], [-3.0, 180.0, 200.0]], np.float32)
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, 2.0) == 0
+    o.finish_topology()
+    o.update_atlas()
+    _, n0 = o.insert_points_subcell(pts, 2.0)
+    _, n1 = o.insert_points_subcell(np.concatenate([pts, extra]), 2.0)
+    assert n0 == n1
+
+
+# ------------------------------------------------------------ density, apron
+def _gathered(pts, radius, boundval=0.0, precision=64, scs=4):
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, radius) == 0
+    o.finish_topology()
+    o.update_atlas()
+    o.clear_channel()
+    rc, n = o.insert_points_subcell(pts, radius, scs)
+    assert rc == 0
+    o.gather_density(radius, scs, precision)
+    o.update_apron(boundval, precision)
+    return o
+
+
+@pytest.mark.parametrize("radius,scs", [(2.0, 4), (1.5, 4), (3.0, 2), (0.75, 4)])
+def test_gather_equals_brute_force_field(radius, scs):
+    pts = _ball(1500, 16)
+    o = _gathered(pts, radius, scs=scs)
+    leaves = o.pool(0, 0)
+    lo = leaves["mPos"].min(0)
+    hi = leaves["mPos"].max(0) + 8
+    g = M.density_grid(pts, radius, lo, hi)
+    got = M.brick_interiors(o.atlas(64), leaves["mPos"], leaves["mValue"])
+    for i, p in enumerate(leaves["mPos"]):
+        want = g[p[0] - lo[0]:p[0] - lo[0] + 8, p[1] - lo[1]:p[1] - lo[1] + 8, p[2] - lo[2]:p[2] - lo[2] + 8]
+        assert np.abs(got[i] - want).max() <= 1e-12, "brick %d at %s" % (i, p)
+
+
+def test_gather_fp32_close_to_fp64():
+    pts = _ball(1500, 17)
+    a = _gathered(pts, 2.0, precision=32).atlas(32).astype(np.float64)
+    b = _gathered(pts, 2.0, precision=64).atlas(64)
+    assert np.abs(a - b).max() < 2e-6
+    assert np.array_equal(a == 0, b == 0)
+
+
+@pytest.mark.parametrize("boundval", [0.0, -1.5])
+def test_apron_matches_neighbour_lookup(boundval):
+    pts = _ball(1200, 18)
+    o = _gathered(pts, 2.0, boundval=boundval)
+    leaves = o.pool(0, 0)
+    pos = leaves["mPos"]
+    slot = leaves["mValue"] - 1          # corner of the padded 10^3 brick
+    lo = pos.min(0) - 8
+    hi = pos.max(0) + 16
+    g = M.density_grid(pts, 2.0, lo, hi)
+    have = {tuple(p) for p in pos.tolist()}
+    a = o.atlas(64)
+    loc = np.stack(np.meshgrid(np.arange(10), np.arange(10), np.arange(10), indexing="ij"), -1).reshape(-1, 3)
+    ap = loc[np.any((loc == 0) | (loc == 9), axis=1)]
+    assert len(ap) == 488
+    for i in range(len(pos)):
+        wv = pos[i] + ap - 1
+        got = a[slot[i, 2] + ap[:, 2], slot[i, 1] + ap[:, 1], slot[i, 0] + ap[:, 0]]
+        owner = np.floor_divide(wv, 8) * 8
+        exists = np.array([tuple(o_) in have for o_ in owner.tolist()])
+        want = np.where(exists, g[wv[:, 0] - lo[0], wv[:, 1] - lo[1], wv[:, 2] - lo[2]], boundval)
+        assert np.abs(got - want).max() <= 1e-12, "brick %d" % i
+
+
+# -------------------------------------------------------------------- render
+def _scene(w, h, to, angs, dist, **kw):
+    s = dict(width=w, height=h, cam=dict(angs=angs, to=to, dist=dist, dolly=1.0, fov=40.0, near=0.1, far=5000.0),
+             light=(0.0, 0.0, 0.0), backclr=(0.1, 0.2, 0.3, 1.0), extinct=(-0.1, 1.0, 0.0), steps=(0.5, 16.0, 0.1),
+             cutoff=(0.005, 0.01, 0.0), vrange=(0.0, 0.0, 2.0), shadow=(0.0, 1.0, 0.0))
+    s.update(kw)
+    return s
+
+
+def test_camera_projection_closed_form():
+    """campos = to + dist*(cos b sin a, sin b, cos b cos a); the image corners
+    sit at tan(fov/2)/2 horizontally and tan(fov/2)/(2*aspect) vertically off
+    the view axis (Camera3D::updateMatricies, src/gvdb_camera.cpp:175-228:
+    proj(0,0) = 2n/sx with sx = tan(fov/2) n)."""
+    for (w, h), angs, fov in (((64, 48), (30.0, 20.0, 0.0), 40.0), ((100, 100), (-75.0, 50.0, 0.0), 65.0)):
+        sc = _scene(w, h, (256.0, 120.0, 256.0), angs, 600.0)
+        sc["cam"]["fov"] = fov
+        blk = O.scene_block(sc)
+        d2r = 3.141592 / 180.0
+        a, b = angs[0] * d2r, angs[1] * d2r
+        want = np.array([256.0, 120.0, 256.0]) + 600.0 * np.array([math.cos(b) * math.sin(a), math.sin(b),
+                                                                    math.cos(b) * math.cos(a)])
+        assert np.allclose(blk[2:5], want, atol=2e-3)
+        cams, camu, camv = blk[5:8].astype(np.float64), blk[8:11].astype(np.float64), blk[11:14].astype(np.float64)
+        fwd = (np.array([256.0, 120.0, 256.0]) - want)
+        fwd /= np.linalg.norm(fwd)
+        side = np.cross(fwd, [0, 1, 0])
+        side /= np.linalg.norm(side)
+        up = np.cross(side, fwd)
+        t = math.tan(fov * d2r / 2)
+        tl = cams
+        assert np.isclose(tl @ side / (tl @ fwd), -t / 2, rtol=1e-4)
+        assert np.isclose(tl @ up / (tl @ fwd), t / (2 * w / h), rtol=1e-4)
+        centre = cams + 0.5 * camu + 0.5 * camv
+        assert np.allclose(centre / np.linalg.norm(centre), fwd, atol=1e-5)
+
+
+def test_linear_transfer_function_closed_form():
+    lut = O.linear_tf(0.25, 0.75, (0.0, 0.2, 0.4, 0.0), (1.0, 0.6, 0.4, 1.0))
+    n0, n1 = int(0.25 * 16384), int(0.75 * 16384)
+    assert np.all(lut[:n0] == 0) and np.all(lut[n1:] == 0)
+    x = (np.arange(n0, n1) - n0) / (n1 - n0)
+    assert np.allclose(lut[n0:n1, 0], x, atol=1e-6)
+    assert np.allclose(lut[n0:n1, 1], 0.2 + 0.4 * x, atol=1e-6)
+    assert np.allclose(lut[n0:n1, 2], 0.4, atol=1e-6)
+
+
+@pytest.mark.parametrize("shading", [7, 4])
+def test_render_rays_that_miss_show_background(shading):
+    pts = _ball(800, 19)
+    o = _gathered(pts, 2.0)
+    o.finish_topology()
+    sc = _scene(24, 16, (200.0, 180.0, -3000.0), (0.0, 0.0, 0.0), 1000.0)   # looking down -z, volume behind
+    img = o.render(O.scene_block(sc), O.linear_tf(0.0, 1.0, (1, 1, 1, 0), (1, 1, 1, 1)), shading, 64)
+    back = np.array(sc["backclr"], np.float32)
+    if shading == 7:
+        back = np.array([0.1, 0.2, 0.3, 0.0], np.float32)     # alpha = 1 - transmittance = 0
+    assert np.array_equal(img, np.broadcast_to(back, img.shape))
+
+
+def _block_points():
+    """One point per voxel centre of [56,88) x [64,96) x [64,128): exactly a
+    4 x 4 x 8 block of bricks with radius below 1 (no dilation), spread over
+    two level-1 nodes."""
+    g = np.stack(np.meshgrid(np.arange(56, 88), np.arange(64, 96), np.arange(64, 128), indexing="ij"), -1)
+    return (g.reshape(-1, 3) + 0.5).astype(np.float32)
+
+
+def test_emission_absorption_uniform_slab():
+    """Uniform value c in a 64-voxel-deep block: after n samples the
+    transmittance is e^n with e = exp(extinct * alpha * step) and the colour
+    is col * albedo * (1 - e^n) (rayDeepBrick, kernels/cuda_gvdb_raycast.cuh:
+    485-533); the pixel composites over the background with alpha 1 - e^n
+    (gvdbRayDeep, kernels/cuda_gvdb_module.cu:60-78).  n is 64 / step up to
+    one sample at either end of the lattice.  The ray is slightly oblique: on
+    an exactly axis-aligned ray the reference's DDA multiplies an infinite
+    tDel by a zero mask (NaN) and stops advancing after the second brick."""
+    pts = _block_points()
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, 0.5) == 0
+    o.finish_topology()
+    o.update_atlas()
+    assert len(o.pool(0, 0)) == 128
+    c = 0.8
+    o.fill_channel(c)
+    step = 0.5
+    sc = _scene(1, 1, (72.3, 79.6, 96.0), (0.01, 0.01, 0.0), 300.17, extinct=(-0.05, 0.9, 0.0),
+                steps=(step, 16.0, 0.1), vrange=(0.0, 0.0, 2.0))
+    col = np.array([1.0, 0.5, 0.25, 1.0], np.float32)
+    tf = O.linear_tf(0.0, 1.0, (0, 0, 0, 0), col)
+    img = o.render(O.scene_block(sc), tf, 7, 64)[0, 0].astype(np.float64)
+    i = int(np.float64(c / 2.0) * 16300)
+    tcol = tf[i].astype(np.float64)
+    e = math.exp(-0.05 * tcol[3] * step)
+    back = np.array(sc["backclr"][:3])
+    ok = []
+    for n in range(int(64 / step) - 1, int(64 / step) + 2):
+        T = e ** n
+        clr = np.minimum(tcol[:3] * 0.9 * (1 - T), 1.0)
+        want = np.concatenate([back + (1 - T) * (clr - back), [1 - T]])
+        ok.append(np.abs(img - want).max())
+    assert min(ok) < 1e-6, "pixel %s, closest closed form off by %.3g" % (img, min(ok))   # float RGBA output
+
+
+def test_isosurface_on_axis_through_one_point():
+    """One point at the integer position P with radius 2: on the line through
+    the voxel centres x = P.x + 0.5, y = P.y + 0.5 the trilinear field is
+    2 - |z - 0.5 - P.z|, so a ray down -z first reaches 0.5 within one step
+    of z = P.z + 2; the normal there is +z and the Phong term is
+    0.9 * (l . n) + 0.1 with no shadow (gvdbRaySurfaceTrilinear,
+    kernels/cuda_gvdb_module.cu:38-57, :103-120)."""
+    P = np.array([200.0, 180.0, 220.0], np.float32)
+    pts = np.array([P, [60.0, 60.0, 60.0]], np.float32)     # the second keeps two level-1 nodes
+    o = _gathered(pts, 2.0)
+    o.finish_topology()
+    light = (200.5 + 500.0, 180.5, 222.0 + 500.0)
+    sc = _scene(1, 1, (200.5, 180.5, 220.0), (0.0, 0.0, 0.0), 150.1, light=light, steps=(0.25, 16.0, 0.1),
+                vrange=(0.5, 0.0, 1.0), shadow=(0.8, 1.0, 0.0))
+    img = o.render(O.scene_block(sc), O.linear_tf(0.0, 1.0, (0, 0, 0, 0), (1, 1, 1, 1)), 4, 64)[0, 0]
+    vals = []
+    for z in np.arange(221.75, 222.0001, 0.0625):
+        l = np.array(light) - np.array([200.5, 180.5, z])
+        l /= np.linalg.norm(l)
+        vals.append(0.9 * l[2] + 0.1)
+    assert min(vals) - 1e-4 <= img[0] <= max(vals) + 1e-4
+    assert np.allclose(img[:3], img[0], atol=1e-12) and img[3] == 1.0
