@@ -134,6 +134,25 @@ def run_cpu_baseline(sample_stride=4, render_w=512):
                     sample_stride, len(pts), render_w, render_w))
 
 
+def copy_calibration(mbytes):
+    """One device-to-device hipMemcpy of `mbytes` MiB (a blit kernel moving a
+    known number of bytes), for scaling the FETCH_SIZE / WRITE_SIZE counters
+    of a profiled run."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = int(mbytes) << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0     # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps(dict(copy_calibration_bytes=n, copies=3)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,7 +160,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--width", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the point-to-volume path runs one grid per GPU and this build has "
               "no multi-GPU form (--gpus %d requested)" % args.gpus, file=sys.stderr)

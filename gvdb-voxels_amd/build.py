@@ -41,6 +41,15 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    # C++ facade sample linked against the library (runs on a GPU host)
+    demo = os.path.join(LIBDIR, "facade_demo")
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "examples", "facade_demo.cpp"), "-o", demo + ".tmp", "-L" + LIBDIR, "-lgvdb_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("facade_demo build failed: %s\n%s" % (r.stdout, r.stderr))
+    os.replace(demo + ".tmp", demo)
     if verbose:
         print("built", LIB)
     return LIB

@@ -82,9 +82,9 @@ __global__ __launch_bounds__(256) void k_bin_subcell(Tree t, PointSrc pts, int s
     if (!isfinite(w[0]) || !isfinite(w[1]) || !isfinite(w[2])) return;
     if (w[0] < 0 || w[1] < 0 || w[2] < 0) return;
     // The reference's insert pass tests p < 0 before adding trans while its
-    // count pass tests p + trans < 0; both tests apply here so that the insert
-    // pass never places more entries in a subcell than the count pass counted.
-    if (INSERT && (p0 < 0 || p1 < 0 || p2 < 0)) return;
+    // count pass tests p + trans < 0 (it writes past the counted entries when
+    // they disagree); both tests apply to both passes here.
+    if (p0 < 0 || p1 < 0 || p2 < 0) return;
     int scmin[3], scmax[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {

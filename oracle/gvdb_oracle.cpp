@@ -737,12 +737,23 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
                     fn(v.sc_mapping[pid] * spb + lo);
                 }
     };
-    // count pass (position test after adding trans)
+    // The reference's count pass tests the position after adding trans, its
+    // insert pass before; when the two disagree it writes past the counted
+    // entries (undefined).  Such points are recorded and skipped in both.
+    auto binned = [&](const float* p, float w[3]) {
+        if (p[2] == NOHIT) return false;
+        for (int a = 0; a < 3; a++) w[a] = p[a] + trans[a];
+        bool wneg = w[0] < 0 || w[1] < 0 || w[2] < 0, pneg = p[0] < 0 || p[1] < 0 || p[2] < 0;
+        return !wneg && !pneg;
+    };
     for (int i = 0; i < n; i++) {
         const float* p = pts + 3 * i;
+        float w[3];
         if (p[2] == NOHIT) continue;
-        float w[3] = {p[0] + trans[0], p[1] + trans[1], p[2] + trans[2]};
-        if (w[0] < 0 || w[1] < 0 || w[2] < 0) continue;
+        bool wneg = p[0] + trans[0] < 0 || p[1] + trans[1] < 0 || p[2] + trans[2] < 0;
+        bool pneg = p[0] < 0 || p[1] < 0 || p[2] < 0;
+        if (wneg != pneg) v.undefined_events++;
+        if (!binned(p, w)) continue;
         forEachSubcell(w, [&](int idx) { v.sc_cnt[idx]++; });
     }
     v.sc_off.assign(numSC, 0);
@@ -754,14 +765,13 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
     *outLen = len;
     v.sc_len = len;
     if (len == 0) return 0;
-    // insert pass (position test before adding trans, as the reference does)
+    // insert pass
     std::vector<int> cnt2(numSC, 0);
     v.sc_pnt.assign(size_t(len) * 3, 0.f);
     for (int i = 0; i < n; i++) {
         const float* p = pts + 3 * i;
-        if (p[2] == NOHIT) continue;
-        if (p[0] < 0 || p[1] < 0 || p[2] < 0) continue;
-        float w[3] = {p[0] + trans[0], p[1] + trans[1], p[2] + trans[2]};
+        float w[3];
+        if (!binned(p, w)) continue;
         forEachSubcell(w, [&](int idx) {
             int slot = cnt2[idx]++;
             size_t e = size_t(v.sc_off[idx]) + slot;

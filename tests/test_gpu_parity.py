@@ -168,3 +168,47 @@ def test_insert_paths(gpu, variant):
     mask = H.used_brick_mask(b.shape, o.atlas_map(), o.atlas_info()["cnt"])
     _, rel, zeros = H.density_errors(a[mask], b[mask])
     assert zeros == 0 and rel <= REL_TOL
+
+
+@pytest.mark.parametrize("trans", [(-60.0, 5.0, 0.0), (20.0, 0.0, 0.0)])
+def test_translated_points(gpu, trans):
+    """RebuildTopology(orig) and InsertPointsSubcell/GatherDensity(trans) with
+    a translation that moves some points below zero (dropped), or moves some
+    negative points above zero (dropped in both binning passes: the reference
+    tests them inconsistently there)."""
+    import oracle as O
+    pts = _sphere(6000, 51)
+    extra = np.random.default_rng(52).random((300, 3)).astype(np.float32) * 30
+    extra[:, 0] -= 15.0          # x in [-15, 15)
+    pts = np.concatenate([pts, extra + np.float32(240.0) * np.array([0, 1, 1], np.float32)])
+    v = gpu.VolumeGVDB(0)
+    v.Configure(*H.TREE)
+    v.SetChannelDefault(16, 16, 1)
+    v.AddChannel(0, gpu.T_FLOAT, 1)
+    v.SetPoints(pts)
+    v.RebuildTopology(len(pts), 2.0, trans)
+    v.FinishTopology(False, True)
+    v.UpdateAtlas()
+    v.ClearChannel(0)
+    n = v.InsertPointsSubcell(4, len(pts), 2.0, trans)
+    v.GatherDensity(4, len(pts), 2.0, trans, n, 0, 0, False)
+    v.UpdateApron(0, 0.0)
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, 2.0, trans) == 0
+    o.finish_topology()
+    o.update_atlas()
+    o.clear_channel()
+    rc, m = o.insert_points_subcell(pts, 2.0, 4, trans)
+    assert rc == 0 and n == m
+    o.gather_density(2.0, 4, 64)
+    o.update_apron(0.0, 64)
+    H.assert_pools_equal(v, o)
+    gs, os_ = v.subcells(), o.subcells(m)
+    assert np.array_equal(gs["cnt"].astype(np.int64), os_["cnt"].astype(np.int64))
+    la = H.subcell_lists(gs["cnt"], gs["off"], gs["pnt"])
+    lb = H.subcell_lists(os_["cnt"], os_["off"], os_["pnt"])
+    assert all(np.array_equal(x, y) for x, y in zip(la, lb))
+    a, b = v.atlas(), o.atlas(64)
+    mask = H.used_brick_mask(b.shape, o.atlas_map(), o.atlas_info()["cnt"])
+    _, rel, zeros = H.density_errors(a[mask], b[mask])
+    assert zeros == 0 and rel <= REL_TOL
