@@ -21,9 +21,11 @@ __device__ inline void dir_of(int d, int& dx, int& dy, int& dz) {
     dz = d / 9 - 1;
 }
 
-__global__ __launch_bounds__(256) void k_neighbors(Tree t, int* nbr, int nbricks) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nbricks * 27) return;
+__device__ inline void k_neighbors_body(const NbrArgs& a, uint32_t bid, uint32_t) {
+    const Tree& t = a.t;
+    int* nbr = a.nbr;
+    int g = bid * 256 + threadIdx.x;
+    if (g >= a.n * 27) return;
     int b = g / 27, d = g - b * 27;
     const Node* n = t.nodes[0] + b;
     if (d == 13) {
@@ -41,10 +43,15 @@ __global__ __launch_bounds__(256) void k_neighbors(Tree t, int* nbr, int nbricks
     nbr[g] = point_leaf(t, px, py, pz);
 }
 
-hipError_t launch_neighbors(Tree t, int* nbr27, int nbricks, hipStream_t s) {
-    if (nbricks <= 0) return hipSuccess;
-    u64 n = u64(nbricks) * 27;
-    hipLaunchKernelGGL(k_neighbors, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, t, nbr27, nbricks);
+GVDB_MULTI(k_neighbors, NbrArgs, 256)
+
+hipError_t launch_neighbors_multi(const NbrArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> nb(G);
+    for (int g = 0; g < G; g++) nb[g] = a[g].n > 0 ? uint32_t((u64(a[g].n) * 27 + 255) / 256) : 0;
+    MultiLaunch L;
+    hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
+    if (e != hipSuccess || L.blocks == 0) return e;
+    hipLaunchKernelGGL(k_neighbors, dim3(L.blocks), dim3(256), 0, s, a[0], L.map);
     return hipGetLastError();
 }
 
@@ -65,8 +72,12 @@ __device__ inline void st<__half>(__half* a, u64 o, float v) { a[o] = __float2ha
 // of the padded 10^3 brick (apron 1, 8^3 interior).  Enumeration: the z = 0
 // and z = 9 faces (100 each), then the 36-voxel rings of z = 1..8.
 template <typename AT>
-__global__ __launch_bounds__(512) void k_apron(Tree t, const int* nbr, AT* atlas, float boundval) {
-    const int b = blockIdx.x;
+__device__ inline void k_apron_body(const ApronArgs& ap, uint32_t bid, uint32_t) {
+    const Tree& t = ap.t;
+    const int* nbr = ap.nbr;
+    AT* atlas = reinterpret_cast<AT*>(ap.atlas);
+    const float boundval = ap.boundval;
+    const int b = int(bid);
     const int a = threadIdx.x;
     const int br = t.brick_res;        // 10
     const int w = t.brickwid;          // 8
@@ -106,15 +117,29 @@ __global__ __launch_bounds__(512) void k_apron(Tree t, const int* nbr, AT* atlas
     st<AT>(atlas, own, v);
 }
 
-hipError_t launch_update_apron(Tree t, const int* nbr27, int nbricks, void* atlas, int atlas_type, float boundval,
-                               hipStream_t s) {
-    if (nbricks <= 0) return hipSuccess;
-    if (t.apron != 1) return hipErrorInvalidValue;
-    if (atlas_type == T_HALF)
-        hipLaunchKernelGGL(k_apron<__half>, dim3(nbricks), dim3(512), 0, s, t, nbr27, (__half*)atlas, boundval);
-    else
-        hipLaunchKernelGGL(k_apron<float>, dim3(nbricks), dim3(512), 0, s, t, nbr27, (float*)atlas, boundval);
-    return hipGetLastError();
+GVDB_MULTI_T(k_apron, typename AT, AT, ApronArgs, 512)
+
+hipError_t launch_apron_multi(const ApronArgs* a, int G, Arena* ar, hipStream_t s) {
+    for (int half = 0; half < 2; half++) {
+        std::vector<ApronArgs> sel;
+        std::vector<uint32_t> nb;
+        for (int g = 0; g < G; g++) {
+            if (a[g].t.apron != 1) return hipErrorInvalidValue;
+            if ((a[g].atlas_type == T_HALF) != bool(half) || a[g].n <= 0) continue;
+            sel.push_back(a[g]);
+            nb.push_back(uint32_t(a[g].n));
+        }
+        if (sel.empty()) continue;
+        MultiLaunch L;
+        hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
+        if (e != hipSuccess) return e;
+        if (L.blocks == 0) continue;
+        if (half) hipLaunchKernelGGL(k_apron<__half>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
+        else hipLaunchKernelGGL(k_apron<float>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace gvdb

@@ -1,12 +1,19 @@
 // capi.cpp -- extern "C" entry points of include/gvdb_amd.h over gvdb::Volume.
 #include "volume.h"
+#include "batch.h"
 #include "../../include/gvdb_amd.h"
 #include <new>
 #include <cstring>
+#include <vector>
 
 struct gvdb_volume {
     gvdb::Volume v;
     explicit gvdb_volume(int d) : v(d) {}
+};
+
+struct gvdb_batch_s {
+    gvdb::Batch b;
+    gvdb_batch_s(int d, int m) : b(d, m) {}
 };
 
 static_assert(sizeof(gvdb_node) == sizeof(gvdb::Node), "gvdb_node mirrors the 64-byte node");
@@ -180,6 +187,92 @@ int gvdb_kernel_ms(gvdb_handle h, float* out4) {
     H_OR_FAIL(h);
     if (!out4) return GVDB_ERR_INVALID;
     return h->v.kernel_ms(out4);
+}
+
+// ------------------------------------------------------------- batches
+int gvdb_batch_create(gvdb_batch* out, int device, int max_grids) {
+    if (!out) return GVDB_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return GVDB_ERR_HIP;
+    gvdb_batch_s* b = new (std::nothrow) gvdb_batch_s(device, max_grids);
+    if (!b) return GVDB_ERR_OOM;
+    int rc = b->b.init();
+    if (rc) { delete b; return rc; }
+    *out = b;
+    return GVDB_OK;
+}
+int gvdb_batch_destroy(gvdb_batch b) {
+    if (!b) return GVDB_ERR_INVALID;
+    delete b;
+    return GVDB_OK;
+}
+const char* gvdb_batch_last_error(gvdb_batch b) { return b ? b->b.err.c_str() : "null batch"; }
+
+// handles -> volumes (null handles are rejected)
+#define VOLS(b, count, hs)                                                       \
+    if (!(b) || !(hs) || (count) < 1) return GVDB_ERR_INVALID;                   \
+    std::vector<gvdb::Volume*> vs_(static_cast<size_t>(count));                             \
+    for (int i_ = 0; i_ < (count); i_++) {                                       \
+        if (!(hs)[i_]) return GVDB_ERR_INVALID;                                  \
+        vs_[i_] = &(hs)[i_]->v;                                                  \
+    }
+
+int gvdb_batch_rebuild_topology(gvdb_batch b, int count, const gvdb_handle* hs, const int* n, const float* radius,
+                                const float* orig3, int* failed) {
+    VOLS(b, count, hs);
+    if (!n || !radius) return GVDB_ERR_INVALID;
+    std::vector<float> z(size_t(count) * 3, 0.f);
+    return b->b.rebuild_topology(vs_.data(), count, n, radius, orig3 ? orig3 : z.data(), failed);
+}
+int gvdb_batch_finish_topology(gvdb_batch b, int count, const gvdb_handle* hs, int commit_pool, int compute_bounds,
+                               int* failed) {
+    VOLS(b, count, hs);
+    return b->b.finish_topology(vs_.data(), count, commit_pool != 0, compute_bounds != 0, failed);
+}
+int gvdb_batch_update_atlas(gvdb_batch b, int count, const gvdb_handle* hs, int* failed) {
+    VOLS(b, count, hs);
+    return b->b.update_atlas(vs_.data(), count, failed);
+}
+int gvdb_batch_clear_channel(gvdb_batch b, int count, const gvdb_handle* hs, int chan, int* failed) {
+    VOLS(b, count, hs);
+    return b->b.clear_channel(vs_.data(), count, chan, failed);
+}
+int gvdb_batch_insert_points_subcell(gvdb_batch b, int count, const gvdb_handle* hs, int subcell_size, const int* n,
+                                     const float* radius, const float* trans3, int* out_len, int* failed) {
+    VOLS(b, count, hs);
+    if (!n || !radius || !out_len) return GVDB_ERR_INVALID;
+    std::vector<float> z(size_t(count) * 3, 0.f);
+    return b->b.insert_points_subcell(vs_.data(), count, subcell_size, n, radius, trans3 ? trans3 : z.data(), out_len,
+                                      failed);
+}
+int gvdb_batch_gather_density(gvdb_batch b, int count, const gvdb_handle* hs, int subcell_size, const int* sc_len,
+                              const float* radius, int chan_density, int accumulate, int* failed) {
+    VOLS(b, count, hs);
+    if (!sc_len || !radius) return GVDB_ERR_INVALID;
+    return b->b.gather_density(vs_.data(), count, subcell_size, sc_len, radius, chan_density, accumulate != 0, failed);
+}
+int gvdb_batch_update_apron(gvdb_batch b, int count, const gvdb_handle* hs, int chan, float boundval, int* failed) {
+    VOLS(b, count, hs);
+    return b->b.update_apron(vs_.data(), count, chan, boundval, failed);
+}
+int gvdb_batch_render(gvdb_batch b, int count, const gvdb_handle* hs, const int* shading, const int* chan,
+                      const int* buf, int* failed) {
+    VOLS(b, count, hs);
+    if (!shading || !chan || !buf) return GVDB_ERR_INVALID;
+    return b->b.render(vs_.data(), count, shading, chan, buf, failed);
+}
+int gvdb_batch_synchronize(gvdb_batch b) {
+    if (!b) return GVDB_ERR_INVALID;
+    return b->b.synchronize();
+}
+int gvdb_batch_elapsed_ms(gvdb_batch b, float* ms) {
+    if (!b || !ms) return GVDB_ERR_INVALID;
+    return b->b.elapsed_ms(ms);
+}
+int gvdb_release_stream(gvdb_handle h) {
+    H_OR_FAIL(h);
+    return h->v.use_stream(nullptr);
 }
 
 }  // extern "C"

@@ -1,0 +1,689 @@
+// pipeline.cpp -- the stages of the point path over G grids at once.
+//
+// Each stage validates every grid, does the per-grid host work (pool sizes,
+// buffer growth), then issues one launch per kernel over all grids
+// (kernels.h, multi.h).  Host round trips happen once per stage for the
+// whole set, not once per grid.  A single VolumeGVDB call is the same code
+// with G = 1.  Grids of one call share the stream and staging in `Exec`.
+#include "volume.h"
+#include "../../include/gvdb_amd.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace gvdb {
+
+#define XCHK(v, x)                                                   \
+    do {                                                             \
+        hipError_t e_ = (x);                                         \
+        if (e_ != hipSuccess) return (v)->hip_fail(e_, #x);          \
+    } while (0)
+
+static inline float key_float(uint32_t k) {
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+// small per-grid result words (device dsmall / pinned hsmall, SMALL_WORDS each)
+enum { SM_BBOX = 0, SM_LINKED = 8, SM_TOTAL = 9, SM_ORPHANS = 10, SM_BUCKETS = 11, SM_COUNTS = 16, SM_BOUNDS = 40 };
+
+static int drain(Volume* v, const Exec& ex, int G) {
+    XCHK(v, hipMemcpyAsync(ex.hsmall, ex.dsmall, size_t(G) * SMALL_WORDS * 4, hipMemcpyDeviceToHost, ex.s));
+    XCHK(v, hipStreamSynchronize(ex.s));
+    if (ex.ar) ex.ar->reset();
+    return GVDB_OK;
+}
+
+#define FAILG(g, code)               \
+    do {                             \
+        if (failed) *failed = (g);   \
+        return (code);               \
+    } while (0)
+
+// ------------------------------------------------------- RebuildTopology ----
+int Volume::multi_rebuild_topology(Volume* const* vs, int G, const int* n, const float* radius, const float* orig3,
+                                   const Exec& ex, int* failed) {
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        if (v->levels_ == 0) FAILG(g, v->fail(GVDB_ERR_STATE, "RebuildTopology before Configure"));
+        if (n[g] <= 0 || !v->pts_.base || n[g] > v->pts_.n)
+            FAILG(g, v->fail(GVDB_ERR_INVALID, "RebuildTopology: point count exceeds SetPoints"));
+        if (!(radius[g] >= 0.0f) || radius[g] > 32.0f)
+            FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, "RebuildTopology: radius must be in [0, 32]"));
+    }
+    Volume* v0 = vs[0];
+    if (G == 1) v0->stage_begin(ST_TOPO);
+    std::vector<BboxArgs> ba(G);
+    for (int g = 0; g < G; g++) {
+        PointSrc P = vs[g]->pts_;
+        P.n = n[g];
+        ba[g] = BboxArgs{P, {orig3[3 * g], orig3[3 * g + 1], orig3[3 * g + 2]}, ex.dsmall + g * SMALL_WORDS + SM_BBOX};
+    }
+    XCHK(v0, launch_bbox_multi(ba.data(), G, ex.ar, ex.s));
+    int rc = drain(v0, ex, G);
+    if (rc) return rc;
+
+    // per grid: depth, root, domains, scratch
+    struct Plan {
+        int D = 0;
+        bool dilate = false;
+        int rootPos[3] = {0, 0, 0};
+        BitDomain dom[MAXLEV];
+        u64 bm_off[MAXLEV], rk_off[MAXLEV], tmp_off[2 * MAXLEV];
+        uint32_t* masks = nullptr;
+    };
+    std::vector<Plan> plan(G);
+    std::vector<FillArgs> fills;
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        Plan& pl = plan[g];
+        const uint32_t* hk = ex.hsmall + g * SMALL_WORDS + SM_BBOX;
+        float mn[3], mx[3];
+        for (int a = 0; a < 3; a++) {
+            mn[a] = key_float(hk[a]);
+            mx[a] = key_float(hk[3 + a]);
+            // GetBoundingBox padding (gvdb_volume_gvdb.cpp:5452-5460)
+            v->posMin_[a] = mn[a] - 1;
+            v->posMax_[a] = mx[a] + 2;
+            v->posMin_[a] = 0 > v->posMin_[a] ? 0 : v->posMin_[a];
+            v->posMax_[a] = 0 > v->posMax_[a] ? 0 : v->posMax_[a];
+        }
+        int depth = v->determine_depth(pl.rootPos);
+        if (depth == 0) FAILG(g, v->fail(GVDB_ERR_DEPTH, "Points extents smaller than 1 brick. Not yet supported."));
+        if (depth < 0 || depth > 5 || depth >= v->levels_) FAILG(g, v->fail(GVDB_ERR_DEPTH, "Point extents exceed 5 levels."));
+        // The reference rebuilds fully on the first call, after
+        // RequestFullRebuild and on depth changes, and walks an incremental
+        // path otherwise (gvdb_volume_gvdb.cpp:1195-1217); every call here is
+        // a full rebuild.
+        v->clear();
+        v->curr_depth_ = depth;
+        pl.D = depth;
+        pl.dilate = radius[g] >= 1.0f;
+        const int D = depth;
+        int rr = pl.dilate ? int(ceilf(radius[g])) : 0;
+        long long lo_v[3], hi_v[3];
+        for (int a = 0; a < 3; a++) {
+            long long re = (long long)pl.rootPos[a] + v->range_[D] - 1;
+            long long l = std::isfinite(mn[a]) ? (long long)floorf(std::max(mn[a], 0.0f)) - rr - 1 : 0;
+            long long h = std::isfinite(mx[a]) ? (long long)floorf(std::max(mx[a], 0.0f)) + 2 * rr + 1 : 0;
+            lo_v[a] = std::max<long long>(pl.rootPos[a], l);
+            hi_v[a] = std::min<long long>(re, std::max<long long>(h, lo_v[a]));
+        }
+        if (hi_v[0] / v->range_[0] > 65535 || hi_v[1] / v->range_[0] > 65535 || hi_v[2] / v->range_[0] > 65535)
+            FAILG(g, v->fail(GVDB_ERR_RANGE, "RebuildTopology: brick coordinates exceed the 16-bit keys of the reference"));
+        u64 bm_words = 0, rk_words = 0, tmp_words = 0;
+        for (int L = 0; L < D; L++) {
+            int shift = 0;
+            for (int k = 0; k <= L; k++) shift += v->logdim_[k];
+            BitDomain& d = pl.dom[L];
+            d.nbits = 1;
+            for (int a = 0; a < 3; a++) {
+                d.lo[a] = int(lo_v[a] >> shift);
+                d.dim[a] = int((hi_v[a] >> shift) - d.lo[a] + 1);
+                d.nbits *= u64(d.dim[a]);
+            }
+            d.nwords = (d.nbits + 31) / 32;
+            pl.bm_off[L] = bm_words;
+            bm_words += d.nwords * (pl.dilate ? 2 : 1);
+            pl.rk_off[L] = rk_words;
+            rk_words += d.nwords * (pl.dilate ? 2 : 1);
+            for (int k = 0; k < (pl.dilate ? 2 : 1); k++) {
+                pl.tmp_off[2 * L + k] = tmp_words;
+                tmp_words += scan_tmp_words(d.nwords);
+            }
+        }
+        if (pl.dom[0].nbits > (u64(1) << 34))
+            FAILG(g, v->fail(GVDB_ERR_RANGE, "RebuildTopology: brick box of the points exceeds 2^34 bricks"));
+        XCHK(v, v->bitmaps_.ensure(bm_words * 4 + 256));
+        XCHK(v, v->ranks_.ensure(rk_words * 4 + 256));
+        XCHK(v, v->scan_tmp_.ensure(tmp_words * 4 + 256));
+        fills.push_back(FillArgs{v->bitmaps_.p, bm_words, 0, 0});
+        // direction masks cost 4 bytes per domain brick (filled, then scanned
+        // once): used while that stays comparable to the point stream itself
+        if (pl.dom[0].nbits <= std::max(u64(1) << 22, u64(n[g]) * 4)) {
+            XCHK(v, v->masks_.ensure(size_t(pl.dom[0].nbits) * 4));
+            pl.masks = v->masks_.as<uint32_t>();
+            fills.push_back(FillArgs{pl.masks, pl.dom[0].nbits, 0, 0});
+        }
+    }
+    XCHK(v0, launch_fill_multi(fills.data(), int(fills.size()), ex.ar, ex.s));
+    auto A = [&](int g, int L) { return vs[g]->bitmaps_.as<uint32_t>() + plan[g].bm_off[L]; };
+    auto B = [&](int g, int L) {
+        return plan[g].dilate ? vs[g]->bitmaps_.as<uint32_t>() + plan[g].bm_off[L] + plan[g].dom[L].nwords : nullptr;
+    };
+    std::vector<MarkArgs> ma(G);
+    int maxD = 0;
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        PointSrc P = v->pts_;
+        P.n = n[g];
+        ma[g] = MarkArgs{P, {orig3[3 * g], orig3[3 * g + 1], orig3[3 * g + 2]}, radius[g], plan[g].dilate ? 1 : 0,
+                         int(v->range_[0]), 1.0f / float(v->range_[0]), plan[g].dom[0], A(g, 0), B(g, 0), plan[g].masks};
+        maxD = std::max(maxD, plan[g].D);
+    }
+    XCHK(v0, launch_mark_points_multi(ma.data(), G, ex.ar, ex.s));
+    for (int L = 1; L < maxD; L++) {
+        std::vector<ParentArgs> pa;
+        for (int g = 0; g < G; g++)
+            if (L < plan[g].D)
+                pa.push_back(ParentArgs{plan[g].dom[L - 1], A(g, L - 1), B(g, L - 1), vs[g]->logdim_[L], plan[g].dom[L],
+                                        A(g, L), B(g, L)});
+        XCHK(v0, launch_mark_parents_multi(pa.data(), int(pa.size()), ex.ar, ex.s));
+    }
+    std::vector<ScanArgs> sa;
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        uint32_t* rk = v->ranks_.as<uint32_t>();
+        uint32_t* tmp = v->scan_tmp_.as<uint32_t>();
+        uint32_t* counts = ex.dsmall + g * SMALL_WORDS + SM_COUNTS;
+        for (int L = 0; L < plan[g].D; L++) {
+            const BitDomain& d = plan[g].dom[L];
+            sa.push_back(ScanArgs{A(g, L), nullptr, 1, rk + plan[g].rk_off[L], d.nwords, tmp + plan[g].tmp_off[2 * L],
+                                  counts + 2 * L});
+            if (plan[g].dilate)
+                sa.push_back(ScanArgs{A(g, L), B(g, L), 2, rk + plan[g].rk_off[L] + d.nwords, d.nwords,
+                                      tmp + plan[g].tmp_off[2 * L + 1], counts + 2 * L + 1});
+        }
+    }
+    XCHK(v0, launch_scan_multi(sa.data(), int(sa.size()), ex.ar, ex.s));
+    rc = drain(v0, ex, G);
+    if (rc) return rc;
+
+    // per grid: pools sized from the counts, then nodes and links in one launch per kernel
+    std::vector<RootArgs> ra;
+    std::vector<FillArgs> cf;
+    std::vector<EmitArgs> ea;
+    std::vector<LinkArgs> la;
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        Plan& pl = plan[g];
+        const int D = pl.D;
+        const uint32_t* hc = ex.hsmall + g * SMALL_WORDS + SM_COUNTS;
+        uint32_t* rk = v->ranks_.as<uint32_t>();
+        LevelBuild LB[MAXLEV];
+        for (int L = 0; L < D; L++) {
+            LB[L].dom = pl.dom[L];
+            LB[L].bmA = A(g, L);
+            LB[L].bmB = pl.dilate ? B(g, L) : A(g, L);
+            LB[L].rankA = rk + pl.rk_off[L];
+            LB[L].rankE = pl.dilate ? rk + pl.rk_off[L] + pl.dom[L].nwords : nullptr;
+            LB[L].countA = hc[2 * L];
+            LB[L].countE = pl.dilate ? hc[2 * L + 1] : 0u;
+            LB[L].lev = L;
+            LB[L].range = int(v->range_[L]);
+            u64 cnt = u64(LB[L].countA) + LB[L].countE;
+            v->pool_[0][L].lastEle = v->pool_[0][L].usedNum = cnt;
+            rc = v->ensure_pool_capacity(0, L, std::max<u64>(cnt, 1));
+            if (rc) FAILG(g, rc);
+            if (L > 0) {
+                v->pool_[1][L].lastEle = v->pool_[1][L].usedNum = cnt;
+                rc = v->ensure_pool_capacity(1, L, std::max<u64>(cnt, 1));
+                if (rc) FAILG(g, rc);
+            }
+        }
+        v->pool_[0][D].lastEle = v->pool_[0][D].usedNum = 1;
+        v->pool_[1][D].lastEle = v->pool_[1][D].usedNum = 1;
+        rc = v->ensure_pool_capacity(0, D, 1);
+        if (rc) FAILG(g, rc);
+        rc = v->ensure_pool_capacity(1, D, 1);
+        if (rc) FAILG(g, rc);
+        v->compute_top_lev();
+        RootArgs r;
+        r.root = v->pool_[0][D].dev.as<Node>();
+        for (int a = 0; a < 3; a++) r.pos[a] = pl.rootPos[a];
+        r.lev = D;
+        ra.push_back(r);
+        cf.push_back(FillArgs{v->pool_[1][D].dev.p, v->voxcnt(D), ID_UNDEF64, 1});
+        for (int L = 0; L < D; L++) {
+            ea.push_back(EmitArgs{LB[L], v->pool_[0][L].dev.as<Node>(), L > 0 ? 1 : 0});
+            if (L > 0) cf.push_back(FillArgs{v->pool_[1][L].dev.p, v->pool_[1][L].lastEle * v->voxcnt(L), ID_UNDEF64, 1});
+        }
+        // gvdbLinkBricks walks from the top node; levels above it stay unlinked
+        for (int L = 0; L < D; L++) {
+            if (L + 1 > v->top_lev_) continue;
+            const bool toRoot = (L + 1 == D);
+            la.push_back(LinkArgs{LB[L], toRoot ? LB[L] : LB[L + 1], toRoot ? 1 : 0, v->logdim_[L + 1],
+                                  v->pool_[0][L].dev.as<Node>(), v->pool_[1][L + 1].dev.as<u64>(), v->voxcnt(L + 1)});
+        }
+    }
+    XCHK(v0, launch_write_root_multi(ra.data(), int(ra.size()), ex.ar, ex.s));
+    XCHK(v0, launch_fill_multi(cf.data(), int(cf.size()), ex.ar, ex.s));
+    XCHK(v0, launch_emit_nodes_multi(ea.data(), int(ea.size()), ex.ar, ex.s));
+    XCHK(v0, launch_link_level_multi(la.data(), int(la.size()), ex.ar, ex.s));
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        v->root_ = Elem(0, plan[g].D, 0);
+        v->rebuild_topo_ = false;
+        v->host_valid_ = false;
+        v->dev_stale_ = false;
+        v->topo_gen_++;
+    }
+    if (G == 1) v0->stage_end(ST_TOPO);
+    return GVDB_OK;
+}
+
+// -------------------------------------------------------- FinishTopology ----
+int Volume::multi_finish_topology(Volume* const* vs, int G, bool commit, bool bounds, const Exec& ex, int* failed) {
+    for (int g = 0; g < G; g++)
+        if (vs[g]->levels_ == 0) FAILG(g, vs[g]->fail(GVDB_ERR_STATE, "FinishTopology before Configure"));
+    if (bounds) {
+        // ComputeBounds (gvdb_volume_gvdb.cpp:1792-1816)
+        std::vector<BoundsArgs> bd;
+        std::vector<int> who;
+        for (int g = 0; g < G; g++) {
+            Volume* v = vs[g];
+            Pool& p = v->pool_[0][0];
+            float mnv[3] = {0, 0, 0}, mxv[3] = {0, 0, 0};
+            if (p.lastEle && v->dev_stale_) {
+                // host-built pools: bounds from the host mirror
+                Node* c = v->hnode(Elem(0, 0, 0));
+                for (int a = 0; a < 3; a++) mnv[a] = mxv[a] = float(c->mPos[a]);
+                for (u64 i = 0; i < p.lastEle; i++) {
+                    c = v->hnode(Elem(0, 0, i));
+                    if (!c->mFlags) continue;
+                    for (int a = 0; a < 3; a++) {
+                        if (c->mPos[a] < mnv[a]) mnv[a] = float(c->mPos[a]);
+                        if (c->mPos[a] + v->range_[0] > mxv[a]) mxv[a] = float(c->mPos[a] + v->range_[0]);
+                    }
+                }
+            } else if (p.lastEle) {
+                bd.push_back(BoundsArgs{v->make_tree(), int(p.lastEle),
+                                        reinterpret_cast<int*>(ex.dsmall + g * SMALL_WORDS + SM_BOUNDS)});
+                who.push_back(g);
+            }
+            for (int a = 0; a < 3; a++) { v->objMin_[a] = mnv[a]; v->objMax_[a] = mxv[a]; }
+        }
+        if (!bd.empty()) {
+            XCHK(vs[0], launch_leaf_bounds_multi(bd.data(), int(bd.size()), ex.ar, ex.s));
+            int rc = drain(vs[0], ex, G);
+            if (rc) return rc;
+            for (int g : who) {
+                const int* r = reinterpret_cast<const int*>(ex.hsmall + g * SMALL_WORDS + SM_BOUNDS);
+                for (int a = 0; a < 3; a++) { vs[g]->objMin_[a] = float(r[a]); vs[g]->objMax_[a] = float(r[3 + a]); }
+            }
+        }
+    }
+    for (int g = 0; g < G; g++) {
+        if (commit) {
+            int rc = vs[g]->commit_pools();
+            if (rc) FAILG(g, rc);
+        }
+        vs[g]->compute_top_lev();
+    }
+    return GVDB_OK;
+}
+
+// ----------------------------------------------------------- UpdateAtlas ----
+int Volume::multi_update_atlas(Volume* const* vs, int G, const Exec& ex, int* failed) {
+    std::vector<FillArgs> fa;
+    std::vector<FlagArgs> fl;
+    std::vector<ScanArgs> sa;
+    std::vector<AtlasArgs> aa;
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        if (!v->chan_[0].exists) FAILG(g, v->fail(GVDB_ERR_STATE, "UpdateAtlas: no atlas created"));
+        int rc = v->commit_pools();
+        if (rc) FAILG(g, rc);
+        if (G == 1) v->stage_begin(ST_ATLAS);
+        // UpdateAtlas (gvdb_volume_gvdb.cpp:2630-2704)
+        u64 total = v->pool_[0][0].lastEle, used = v->pool_[0][0].usedNum;
+        v->atlas_used_ = 0;
+        if (used > v->atlas_max_) {
+            int cz = int(ceil(used / float(v->atlas_cnt_[0] * v->atlas_cnt_[1])));
+            rc = v->resize_atlas(cz);
+            if (rc) FAILG(g, rc);
+        }
+        v->amap_max_ = u64(v->atlas_cnt_[0]) * v->atlas_cnt_[1] * v->atlas_cnt_[2];
+        XCHK(v, v->amap_.ensure(size_t(v->amap_max_) * sizeof(AtlasNode)));
+        fa.push_back(FillArgs{v->amap_.p, v->amap_max_ * 4, 0xFFFFFFFFull, 0});
+        if (total) {
+            XCHK(v, v->flags_.ensure(size_t(total) * 4));
+            XCHK(v, v->slot_scan_.ensure(size_t(total) * 4));
+            XCHK(v, v->scan_tmp_.ensure(scan_tmp_words(total) * 4 + 256));
+            Tree t = v->make_tree();
+            fl.push_back(FlagArgs{t, v->flags_.as<uint32_t>(), int(total), 0});
+            sa.push_back(ScanArgs{v->flags_.as<uint32_t>(), nullptr, 0, v->slot_scan_.as<uint32_t>(), total,
+                                  v->scan_tmp_.as<uint32_t>(), nullptr});
+            aa.push_back(AtlasArgs{t, v->slot_scan_.as<uint32_t>(), int(total)});
+        }
+    }
+    Volume* v0 = vs[0];
+    XCHK(v0, launch_fill_multi(fa.data(), int(fa.size()), ex.ar, ex.s));
+    XCHK(v0, launch_leaf_flags_multi(fl.data(), int(fl.size()), ex.ar, ex.s));
+    XCHK(v0, launch_scan_multi(sa.data(), int(sa.size()), ex.ar, ex.s));
+    XCHK(v0, launch_update_atlas_multi(aa.data(), int(aa.size()), ex.ar, ex.s));
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        v->atlas_used_ = v->pool_[0][0].lastEle;
+        v->host_valid_ = false;
+        v->compute_top_lev();
+        v->atlas_gen_ = v->topo_gen_;
+    }
+    if (G == 1) v0->stage_end(ST_ATLAS);
+    return GVDB_OK;
+}
+
+int Volume::multi_clear_channel(Volume* const* vs, int G, int c, const Exec& ex, int* failed) {
+    // ClearChannel -> AtlasFill (gvdb_allocator.cpp:813-822)
+    std::vector<FillArgs> fa;
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        if (c < 0 || c >= 8 || !v->chan_[c].exists) FAILG(g, v->fail(GVDB_ERR_INVALID, "ClearChannel: no such channel"));
+        fa.push_back(FillArgs{v->chan_[c].data.p, v->chan_[c].data.bytes / 4, 0, 0});
+    }
+    if (G == 1) vs[0]->stage_begin(ST_CLEAR);
+    XCHK(vs[0], launch_fill_multi(fa.data(), G, ex.ar, ex.s));
+    if (G == 1) vs[0]->stage_end(ST_CLEAR);
+    return GVDB_OK;
+}
+
+// --------------------------------------------------- InsertPointsSubcell ----
+int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const float* radius, const float* trans3,
+                         int* len, const Exec& ex, int* failed) {
+    // InsertPointsSubcell (gvdb_volume_gvdb.cpp:4971-5092)
+    struct Plan {
+        int sd, spb, R;
+        u64 numSC, numMap;
+        bool pull;
+        uint32_t linked = 0, total = 0, orphans = 0;
+        PointSrc P;
+        Tree t;
+    };
+    std::vector<Plan> plan(G);
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        Plan& pl = plan[g];
+        u64 numBrick = v->pool_[0][0].usedNum;
+        if (numBrick == 0) FAILG(g, v->fail(GVDB_ERR_NOBRICKS, "Warning: InsertPointsSubcell yet no bricks exist."));
+        if (scs <= 0 || v->res_[0] % scs != 0)
+            FAILG(g, v->fail(GVDB_ERR_INVALID, "InsertPointsSubcell: subcell size must divide the brick"));
+        if (n[g] < 0 || n[g] > v->pts_.n || (n[g] > 0 && !v->pts_.base))
+            FAILG(g, v->fail(GVDB_ERR_INVALID, "InsertPointsSubcell: point count exceeds SetPoints"));
+        if (!(radius[g] >= 0.0f) || radius[g] > 32.0f)
+            FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, "InsertPointsSubcell: radius must be in [0, 32]"));
+        if (v->atlas_gen_ != v->topo_gen_)
+            FAILG(g, v->fail(GVDB_ERR_STATE, "InsertPointsSubcell: call UpdateAtlas after the topology changes"));
+        int rc = v->commit_pools();
+        if (rc) FAILG(g, rc);
+        pl.sd = v->res_[0] / scs;
+        pl.spb = pl.sd * pl.sd * pl.sd;
+        pl.numSC = u64(pl.spb) * numBrick;
+        pl.numMap = v->pool_[0][0].lastEle;
+        if (pl.numSC > 0xFFFFFFFFull) FAILG(g, v->fail(GVDB_ERR_RANGE, "InsertPointsSubcell: too many subcells"));
+        // pull path when the neighbourhood of a brick stays inside its 27 neighbours
+        pl.R = int(ceilf((radius[g] + 1.0f) / float(scs)));
+        pl.pull = pl.R <= pl.sd && pull_cells(pl.sd, pl.R) <= pull_max_cells() && pl.spb <= pull_max_spb();
+        XCHK(v, v->sc_flags_.ensure(size_t(pl.numMap) * 4));
+        XCHK(v, v->sc_map_.ensure(size_t(pl.numMap) * 4));
+        XCHK(v, v->sc_cnt_.ensure(size_t(pl.numSC) * 4));
+        XCHK(v, v->sc_use_.ensure(size_t(pl.numSC) * 4));
+        XCHK(v, v->sc_off_.ensure(size_t(pl.numSC) * 4));
+        u64 nb = pl.numMap * u64(pl.spb);
+        // scan scratch: mapping, then buckets (pull) or counts, in separate regions
+        XCHK(v, v->scan_tmp_.ensure((scan_tmp_words(pl.numMap) + scan_tmp_words(std::max(nb, pl.numSC))) * 4 + 512));
+        pl.P = v->pts_;
+        pl.P.n = n[g];
+        if (pl.pull) {
+            XCHK(v, v->nbr_.ensure(size_t(pl.numMap) * 27 * 4));
+            XCHK(v, v->bkt_cnt_.ensure(size_t(nb) * 4));
+            XCHK(v, v->bkt_off_.ensure(size_t(nb) * 4));
+            XCHK(v, v->bkt_key_.ensure(size_t(std::max(n[g], 1)) * 4));
+            XCHK(v, v->bkt_slot_.ensure(size_t(std::max(n[g], 1)) * 4));
+            XCHK(v, v->sorted_.ensure(size_t(std::max(n[g], 1)) * 12));
+        }
+        pl.t = v->make_tree();
+    }
+    Volume* v0 = vs[0];
+    if (G == 1) v0->stage_begin(ST_INSERT);
+    auto small = [&](int g, int k) { return ex.dsmall + g * SMALL_WORDS + k; };
+    auto tmp2 = [&](int g) { return vs[g]->scan_tmp_.as<uint32_t>() + scan_tmp_words(plan[g].numMap) + 64; };
+    {
+        std::vector<FlagArgs> fl;
+        std::vector<ScanArgs> sa;
+        std::vector<NbrArgs> na;
+        std::vector<FillArgs> fa;
+        for (int g = 0; g < G; g++) {
+            Volume* v = vs[g];
+            Plan& pl = plan[g];
+            fl.push_back(FlagArgs{pl.t, v->sc_flags_.as<uint32_t>(), int(pl.numMap), 1});
+            sa.push_back(ScanArgs{v->sc_flags_.as<uint32_t>(), nullptr, 0, v->sc_map_.as<uint32_t>(), pl.numMap,
+                                  v->scan_tmp_.as<uint32_t>(), small(g, SM_LINKED)});
+            fa.push_back(FillArgs{v->sc_cnt_.p, pl.numSC, 0, 0});
+            if (pl.pull) {
+                if (v->nbr_gen_ != v->topo_gen_) {
+                    na.push_back(NbrArgs{pl.t, v->nbr_.as<int>(), int(pl.numMap)});
+                    v->nbr_gen_ = v->topo_gen_;
+                }
+                fa.push_back(FillArgs{v->bkt_cnt_.p, pl.numMap * u64(pl.spb), 0, 0});
+                fa.push_back(FillArgs{small(g, SM_ORPHANS), 1, 0, 0});
+            }
+        }
+        XCHK(v0, launch_leaf_flags_multi(fl.data(), int(fl.size()), ex.ar, ex.s));
+        XCHK(v0, launch_scan_multi(sa.data(), int(sa.size()), ex.ar, ex.s));
+        XCHK(v0, launch_neighbors_multi(na.data(), int(na.size()), ex.ar, ex.s));
+        XCHK(v0, launch_fill_multi(fa.data(), int(fa.size()), ex.ar, ex.s));
+    }
+    {
+        std::vector<BucketArgs> ba;
+        std::vector<ScanArgs> sb, sc;
+        std::vector<ScatterArgs> sca;
+        std::vector<PullArgs> pa;
+        for (int g = 0; g < G; g++) {
+            Volume* v = vs[g];
+            Plan& pl = plan[g];
+            const float* tr = trans3 + 3 * g;
+            if (!pl.pull) {
+                // push path: the reference's count pass with global atomics
+                XCHK(v, launch_count_subcell(pl.t, pl.P, scs, pl.spb, tr, radius[g], v->sc_map_.as<uint32_t>(),
+                                             v->sc_cnt_.as<uint32_t>(), ex.s));
+                sc.push_back(ScanArgs{v->sc_cnt_.as<uint32_t>(), nullptr, 0, v->sc_off_.as<uint32_t>(), pl.numSC,
+                                      tmp2(g), small(g, SM_TOTAL)});
+                continue;
+            }
+            u64 nb = pl.numMap * u64(pl.spb);
+            ba.push_back(BucketArgs{pl.t, pl.P, scs, pl.spb, {tr[0], tr[1], tr[2]}, v->bkt_key_.as<uint32_t>(),
+                                    v->bkt_slot_.as<uint32_t>(), v->bkt_cnt_.as<uint32_t>(), small(g, SM_ORPHANS)});
+            sb.push_back(ScanArgs{v->bkt_cnt_.as<uint32_t>(), nullptr, 0, v->bkt_off_.as<uint32_t>(), nb, tmp2(g),
+                                  small(g, SM_BUCKETS)});
+            sca.push_back(ScatterArgs{pl.P, {tr[0], tr[1], tr[2]}, v->bkt_key_.as<uint32_t>(),
+                                      v->bkt_slot_.as<uint32_t>(), v->bkt_off_.as<uint32_t>(), v->sorted_.as<float>()});
+            pa.push_back(PullArgs{pl.t, int(pl.numMap), scs, pl.R, radius[g], v->nbr_.as<int>(),
+                                  v->bkt_cnt_.as<uint32_t>(), v->bkt_off_.as<uint32_t>(), v->sorted_.as<float>(),
+                                  v->sc_map_.as<uint32_t>(), v->sc_cnt_.as<uint32_t>(), v->sc_use_.as<uint32_t>(),
+                                  v->sc_off_.as<uint32_t>(), nullptr});
+            sc.push_back(ScanArgs{v->sc_cnt_.as<uint32_t>(), nullptr, 0, v->sc_off_.as<uint32_t>(), pl.numSC, tmp2(g),
+                                  small(g, SM_TOTAL)});
+        }
+        XCHK(v0, launch_bucket_points_multi(ba.data(), int(ba.size()), ex.ar, ex.s));
+        XCHK(v0, launch_scan_multi(sb.data(), int(sb.size()), ex.ar, ex.s));
+        XCHK(v0, launch_bucket_scatter_multi(sca.data(), int(sca.size()), ex.ar, ex.s));
+        XCHK(v0, launch_pull_multi(0, pa.data(), int(pa.size()), ex.ar, ex.s));
+        XCHK(v0, launch_scan_multi(sc.data(), int(sc.size()), ex.ar, ex.s));
+    }
+    int rc = drain(v0, ex, G);
+    if (rc) return rc;
+    // grids with points outside every brick fall back to the push path
+    bool redo = false;
+    for (int g = 0; g < G; g++) {
+        Plan& pl = plan[g];
+        pl.linked = ex.hsmall[g * SMALL_WORDS + SM_LINKED];
+        pl.total = ex.hsmall[g * SMALL_WORDS + SM_TOTAL];
+        pl.orphans = pl.pull ? ex.hsmall[g * SMALL_WORDS + SM_ORPHANS] : 0;
+        if (pl.pull && pl.orphans) {
+            Volume* v = vs[g];
+            pl.pull = false;
+            redo = true;
+            XCHK(v, launch_fill_u32(v->sc_cnt_.as<uint32_t>(), 0u, pl.numSC, ex.s));
+            XCHK(v, launch_count_subcell(pl.t, pl.P, scs, pl.spb, trans3 + 3 * g, radius[g], v->sc_map_.as<uint32_t>(),
+                                         v->sc_cnt_.as<uint32_t>(), ex.s));
+            XCHK(v, launch_scan_u32(v->sc_cnt_.as<uint32_t>(), nullptr, 0, v->sc_off_.as<uint32_t>(), pl.numSC,
+                                    small(g, SM_TOTAL), tmp2(g), ex.s));
+        }
+    }
+    if (redo) {
+        rc = drain(v0, ex, G);
+        if (rc) return rc;
+        for (int g = 0; g < G; g++) plan[g].total = ex.hsmall[g * SMALL_WORDS + SM_TOTAL];
+    }
+    std::vector<PullArgs> pw;
+    std::vector<SubPosArgs> sp;
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        Plan& pl = plan[g];
+        if (pl.total > 0x7FFFFFFFu) FAILG(g, v->fail(GVDB_ERR_RANGE, "InsertPointsSubcell: more than 2^31 subcell entries"));
+        len[g] = int(pl.total);
+        v->sc_len_ = int(pl.total);
+        v->num_sc_ = int(pl.numSC);
+        v->sc_bricks_ = int(pl.linked);
+        v->sc_map_n_ = int(pl.numMap);
+        v->sc_gen_ = v->topo_gen_;
+        v->last_insert_pull_ = pl.pull;
+        if (pl.total == 0) continue;
+        XCHK(v, v->sc_pnt_.ensure(size_t(pl.total) * 12));
+        XCHK(v, v->sc_pos_.ensure(size_t(pl.numSC) * 12));
+        XCHK(v, v->sc_nid_.ensure(size_t(pl.numSC) * 4));
+        if (pl.pull) {
+            pw.push_back(PullArgs{pl.t, int(pl.numMap), scs, pl.R, radius[g], v->nbr_.as<int>(),
+                                  v->bkt_cnt_.as<uint32_t>(), v->bkt_off_.as<uint32_t>(), v->sorted_.as<float>(),
+                                  v->sc_map_.as<uint32_t>(), v->sc_cnt_.as<uint32_t>(), v->sc_use_.as<uint32_t>(),
+                                  v->sc_off_.as<uint32_t>(), v->sc_pnt_.as<float>()});
+        } else {
+            XCHK(v, launch_fill_u32(v->sc_cnt_.as<uint32_t>(), 0u, pl.numSC, ex.s));
+            XCHK(v, launch_insert_subcell(pl.t, pl.P, scs, pl.spb, trans3 + 3 * g, radius[g], v->sc_map_.as<uint32_t>(),
+                                          v->sc_cnt_.as<uint32_t>(), v->sc_off_.as<uint32_t>(), v->sc_pnt_.as<float>(),
+                                          ex.s));
+            // unordered entries: the gather considers all of them
+            XCHK(v, hipMemcpyAsync(v->sc_use_.p, v->sc_cnt_.p, size_t(pl.numSC) * 4, hipMemcpyDeviceToDevice, ex.s));
+        }
+        sp.push_back(SubPosArgs{pl.t, scs, pl.spb, v->sc_map_.as<uint32_t>(), v->sc_pos_.as<int>(), v->sc_nid_.as<int>(),
+                                int(pl.numMap)});
+    }
+    XCHK(v0, launch_pull_multi(1, pw.data(), int(pw.size()), ex.ar, ex.s));
+    XCHK(v0, launch_subcell_pos_multi(sp.data(), int(sp.size()), ex.ar, ex.s));
+    if (G == 1) v0->stage_end(ST_INSERT);
+    return GVDB_OK;
+}
+
+// --------------------------------------------------------- GatherDensity ----
+int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, const float* radius, int chanD, bool accum,
+                         const Exec& ex, int* failed) {
+    // GatherDensity (gvdb_volume_gvdb.cpp:5318-5344)
+    std::vector<GatherArgs> ga;
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        if (len[g] == 0 || v->pool_[0][0].usedNum == 0) continue;
+        if (chanD < 0 || chanD >= 8 || !v->chan_[chanD].exists)
+            FAILG(g, v->fail(GVDB_ERR_INVALID, "GatherDensity: no such channel"));
+        if (len[g] != v->sc_len_ || scs <= 0 || v->res_[0] % scs)
+            FAILG(g, v->fail(GVDB_ERR_STATE, "GatherDensity: call InsertPointsSubcell first"));
+        if (v->atlas_gen_ != v->topo_gen_ || v->sc_gen_ != v->topo_gen_)
+            FAILG(g, v->fail(GVDB_ERR_STATE, "GatherDensity: topology changed since UpdateAtlas/InsertPointsSubcell"));
+        int rc = v->commit_pools();
+        if (rc) FAILG(g, rc);
+        ga.push_back(GatherArgs{v->make_tree(), scs, radius[g], v->sc_nid_.as<int>(), v->sc_cnt_.as<uint32_t>(),
+                                v->sc_use_.as<uint32_t>(), v->sc_off_.as<uint32_t>(), v->sc_pos_.as<int>(),
+                                v->sc_pnt_.as<float>(), v->chan_[chanD].data.p, v->chan_[chanD].type, accum ? 1 : 0,
+                                v->sc_bricks_});
+    }
+    if (ga.empty()) return GVDB_OK;
+    Volume* v0 = vs[0];
+    if (G == 1) {
+        v0->stage_begin(ST_GATHER);
+        (void)hipEventRecord(v0->kev_[0][0], ex.s);
+    }
+    XCHK(v0, launch_gather_multi(ga.data(), int(ga.size()), ex.ar, ex.s));
+    if (G == 1) {
+        (void)hipEventRecord(v0->kev_[0][1], ex.s);
+        v0->kev_used_[0] = true;
+        v0->stage_end(ST_GATHER);
+    }
+    return GVDB_OK;
+}
+
+// ----------------------------------------------------------- UpdateApron ----
+int Volume::multi_apron(Volume* const* vs, int G, int c, float boundval, const Exec& ex, int* failed) {
+    // UpdateApron (gvdb_volume_gvdb.cpp:4418-4457)
+    std::vector<NbrArgs> na;
+    std::vector<ApronArgs> aa;
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        if (v->apron_ == 0) continue;
+        if (c < 0 || c >= 8 || !v->chan_[c].exists) FAILG(g, v->fail(GVDB_ERR_INVALID, "UpdateApron: no such channel"));
+        int rc = v->commit_pools();
+        if (rc) FAILG(g, rc);
+        int bricks = int(v->pool_[0][0].lastEle);
+        if (bricks == 0) continue;
+        if (v->atlas_gen_ != v->topo_gen_)
+            FAILG(g, v->fail(GVDB_ERR_STATE, "UpdateApron: call UpdateAtlas after the topology changes"));
+        Tree t = v->make_tree();
+        if (v->nbr_gen_ != v->topo_gen_) {
+            XCHK(v, v->nbr_.ensure(size_t(bricks) * 27 * 4));
+            na.push_back(NbrArgs{t, v->nbr_.as<int>(), bricks});
+            v->nbr_gen_ = v->topo_gen_;
+        }
+        aa.push_back(ApronArgs{t, v->nbr_.as<int>(), v->chan_[c].data.p, v->chan_[c].type, boundval, bricks});
+    }
+    if (aa.empty()) return GVDB_OK;
+    Volume* v0 = vs[0];
+    if (G == 1) v0->stage_begin(ST_APRON);
+    XCHK(v0, launch_neighbors_multi(na.data(), int(na.size()), ex.ar, ex.s));
+    if (G == 1) (void)hipEventRecord(v0->kev_[1][0], ex.s);
+    XCHK(v0, launch_apron_multi(aa.data(), int(aa.size()), ex.ar, ex.s));
+    if (G == 1) {
+        (void)hipEventRecord(v0->kev_[1][1], ex.s);
+        v0->kev_used_[1] = true;
+        v0->stage_end(ST_APRON);
+    }
+    return GVDB_OK;
+}
+
+// ---------------------------------------------------------------- Render ----
+int Volume::multi_render(Volume* const* vs, int G, const int* shading, const int* chan, const int* buf, const Exec& ex,
+                         int* failed) {
+    // Render (gvdb_volume_gvdb.cpp:4336-4381)
+    std::vector<RenderArgs> ra;
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        const int b = buf[g], c = chan[g];
+        if (b < 0 || b >= 4 || v->rbuf_[b].w == 0) FAILG(g, v->fail(GVDB_ERR_INVALID, "Render: no such render buffer"));
+        if (c < 0 || c >= 8 || !v->chan_[c].exists) FAILG(g, v->fail(GVDB_ERR_INVALID, "Render: no such channel"));
+        if (shading[g] != SHADE_VOLUME && shading[g] != SHADE_TRILINEAR)
+            FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, "Render: shading must be SHADE_VOLUME or SHADE_TRILINEAR"));
+        if (v->pool_[0][0].lastEle == 0 || v->amap_max_ == 0) FAILG(g, v->fail(GVDB_ERR_STATE, "Render: no topology or atlas"));
+        if (v->atlas_gen_ != v->topo_gen_)
+            FAILG(g, v->fail(GVDB_ERR_STATE, "Render: call UpdateAtlas after the topology changes"));
+        int rc = v->commit_pools();
+        if (rc) FAILG(g, rc);
+        if (!v->tf_committed) {
+            rc = v->commit_tf();
+            if (rc) FAILG(g, rc);
+        }
+        RenderBuf& r = v->rbuf_[b];
+        RenderArgs a;
+        a.t = v->make_tree();
+        v->scene_block(r.w, r.h, a.s);
+        a.tf = v->tf_dev.as<float4>();
+        a.atlas = v->chan_[c].data.p;
+        a.out = r.data.p;
+        a.out_float = r.bpp == 16;
+        a.shading = shading[g];
+        a.atlas_type = v->chan_[c].type;
+        ra.push_back(a);
+    }
+    Volume* v0 = vs[0];
+    if (G == 1) {
+        v0->stage_begin(ST_RENDER);
+        (void)hipEventRecord(v0->kev_[2][0], ex.s);
+    }
+    XCHK(v0, launch_render_multi(ra.data(), int(ra.size()), ex.ar, ex.s));
+    if (G == 1) {
+        (void)hipEventRecord(v0->kev_[2][1], ex.s);
+        v0->kev_used_[2] = true;
+        v0->stage_end(ST_RENDER);
+    }
+    return GVDB_OK;
+}
+
+}  // namespace gvdb

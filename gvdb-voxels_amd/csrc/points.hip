@@ -13,9 +13,11 @@ __device__ inline const float* point_at(const PointSrc& p, int i) {
 // Slot per flagged leaf in pool order, mValue = slot corner + apron, and the
 // atlas map entry (src/gvdb_volume_gvdb.cpp:2630-2704, getAtlasPos
 // src/gvdb_allocator.cpp:705-715).  The map is cleared to -1 beforehand.
-__global__ __launch_bounds__(256) void k_update_atlas(Tree t, const uint32_t* slot_scan, int nleaves) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nleaves) return;
+__device__ inline void k_update_atlas_body(const AtlasArgs& a, uint32_t bid, uint32_t) {
+    const Tree& t = a.t;
+    const uint32_t* slot_scan = a.slot_scan;
+    int i = bid * 256 + threadIdx.x;
+    if (i >= a.n) return;
     Node* n = t.nodes[0] + i;
     if (!n->mFlags) return;
     u64 id = slot_scan[i];
@@ -36,32 +38,40 @@ __global__ __launch_bounds__(256) void k_update_atlas(Tree t, const uint32_t* sl
     t.amap[slot] = an;
 }
 
-hipError_t launch_update_atlas(Tree t, const uint32_t* slot_scan, int nleaves, hipStream_t s) {
-    if (nleaves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_update_atlas, dim3((nleaves + 255) / 256), dim3(256), 0, s, t, slot_scan, nleaves);
+GVDB_MULTI(k_update_atlas, AtlasArgs, 256)
+
+template <typename A, void (*K)(A, MultiMap)>
+static hipError_t launch_per_item(const A* a, const uint32_t* items, int G, Arena* ar, hipStream_t s, int threads) {
+    std::vector<uint32_t> nb(G);
+    for (int g = 0; g < G; g++) nb[g] = (items[g] + threads - 1) / threads;
+    MultiLaunch L;
+    hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
+    if (e != hipSuccess || L.blocks == 0) return e;
+    hipLaunchKernelGGL(K, dim3(L.blocks), dim3(threads), 0, s, a[0], L.map);
     return hipGetLastError();
+}
+
+hipError_t launch_update_atlas_multi(const AtlasArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> items(G);
+    for (int g = 0; g < G; g++) items[g] = uint32_t(a[g].n);
+    return launch_per_item<AtlasArgs, k_update_atlas>(a, items.data(), G, ar, s, 256);
 }
 
 // flags[i] = 1 for leaves in use: mode 0 tests mFlags (atlas assignment),
 // mode 1 tests mParent != ID_UNDEF64 (gvdbSetFlagSubcell,
 // kernels/cuda_gvdb_particles.cuh:109-118).
-__global__ __launch_bounds__(256) void k_leaf_flags(Tree t, uint32_t* flags, int nleaves, int mode) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nleaves) return;
-    const Node* n = t.nodes[0] + i;
-    flags[i] = mode == 0 ? (n->mFlags ? 1u : 0u) : (n->mParent == ID_UNDEF64 ? 0u : 1u);
+__device__ inline void k_leaf_flags_body(const FlagArgs& a, uint32_t bid, uint32_t) {
+    int i = bid * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const Node* n = a.t.nodes[0] + i;
+    a.flags[i] = a.mode == 0 ? (n->mFlags ? 1u : 0u) : (n->mParent == ID_UNDEF64 ? 0u : 1u);
 }
+GVDB_MULTI(k_leaf_flags, FlagArgs, 256)
 
-hipError_t launch_leaf_flags(Tree t, uint32_t* flags, int nleaves, hipStream_t s) {
-    if (nleaves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_leaf_flags, dim3((nleaves + 255) / 256), dim3(256), 0, s, t, flags, nleaves, 1);
-    return hipGetLastError();
-}
-
-hipError_t launch_leaf_used(Tree t, uint32_t* flags, int nleaves, hipStream_t s) {
-    if (nleaves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_leaf_flags, dim3((nleaves + 255) / 256), dim3(256), 0, s, t, flags, nleaves, 0);
-    return hipGetLastError();
+hipError_t launch_leaf_flags_multi(const FlagArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> items(G);
+    for (int g = 0; g < G; g++) items[g] = uint32_t(a[g].n);
+    return launch_per_item<FlagArgs, k_leaf_flags>(a, items.data(), G, ar, s, 256);
 }
 
 // -------------------------------------------------------- subcell binning
@@ -119,7 +129,7 @@ __global__ __launch_bounds__(256) void k_bin_subcell(Tree t, PointSrc pts, int s
             }
 }
 
-hipError_t launch_count_subcell(Tree t, PointSrc pts, int scs, int spb, const float trans[3], float radius,
+hipError_t launch_count_subcell(const Tree& t, PointSrc pts, int scs, int spb, const float trans[3], float radius,
                                 const uint32_t* mapping, uint32_t* sc_cnt, hipStream_t s) {
     if (pts.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_bin_subcell<false>, dim3((pts.n + 255) / 256), dim3(256), 0, s, t, pts, scs, spb, trans[0],
@@ -127,7 +137,7 @@ hipError_t launch_count_subcell(Tree t, PointSrc pts, int scs, int spb, const fl
     return hipGetLastError();
 }
 
-hipError_t launch_insert_subcell(Tree t, PointSrc pts, int scs, int spb, const float trans[3], float radius,
+hipError_t launch_insert_subcell(const Tree& t, PointSrc pts, int scs, int spb, const float trans[3], float radius,
                                  const uint32_t* mapping, uint32_t* sc_cnt, const uint32_t* sc_off, float* sc_pnt,
                                  hipStream_t s) {
     if (pts.n <= 0) return hipSuccess;
@@ -137,10 +147,14 @@ hipError_t launch_insert_subcell(Tree t, PointSrc pts, int scs, int spb, const f
 }
 
 // gvdbCalcSubcellPos: kernels/cuda_gvdb_particles.cuh:677-694
-__global__ __launch_bounds__(256) void k_subcell_pos(Tree t, int scs, int spb, const uint32_t* mapping, int* sc_pos,
-                                                     int* sc_nid, int nleaves) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nleaves) return;
+__device__ inline void k_subcell_pos_body(const SubPosArgs& a, uint32_t bid, uint32_t) {
+    const Tree& t = a.t;
+    const int scs = a.scs, spb = a.spb;
+    const uint32_t* mapping = a.mapping;
+    int* sc_pos = a.pos;
+    int* sc_nid = a.nid;
+    int i = bid * 256 + threadIdx.x;
+    if (i >= a.n) return;
     const Node* n = t.nodes[0] + i;
     if (n->mParent == ID_UNDEF64) return;
     int scdim = t.res[0] / scs;
@@ -153,12 +167,12 @@ __global__ __launch_bounds__(256) void k_subcell_pos(Tree t, int scs, int spb, c
     }
 }
 
-hipError_t launch_subcell_pos(Tree t, int scs, int spb, const uint32_t* mapping, int* sc_pos, int* sc_nid, int nleaves,
-                              hipStream_t s) {
-    if (nleaves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_subcell_pos, dim3((nleaves + 255) / 256), dim3(256), 0, s, t, scs, spb, mapping, sc_pos, sc_nid,
-                       nleaves);
-    return hipGetLastError();
+GVDB_MULTI(k_subcell_pos, SubPosArgs, 256)
+
+hipError_t launch_subcell_pos_multi(const SubPosArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> items(G);
+    for (int g = 0; g < G; g++) items[g] = uint32_t(a[g].n);
+    return launch_per_item<SubPosArgs, k_subcell_pos>(a, items.data(), G, ar, s, 256);
 }
 
 // ---------------------------------------------------------- GatherDensity
@@ -207,15 +221,22 @@ template <>
 __device__ inline float load_atlas<__half>(const __half* a, u64 off) { return __half2float(a[off]); }
 
 template <typename AT>
-__global__ __launch_bounds__(512) void k_gather(Tree t, int scs, float radius, const int* sc_nid, const uint32_t* sc_cnt,
-                                                const uint32_t* sc_use, const uint32_t* sc_off, const int* sc_pos,
-                                                const float* sc_pnt, AT* atlas, int accumulate) {
+__device__ inline void k_gather_body(const GatherArgs& a, uint32_t bid, uint32_t) {
+    const Tree& t = a.t;
+    const int scs = a.scs;
+    const float radius = a.radius;
+    const int* sc_nid = a.sc_nid;
+    const uint32_t *sc_cnt = a.sc_cnt, *sc_use = a.sc_use, *sc_off = a.sc_off;
+    const int* sc_pos = a.sc_pos;
+    const float* sc_pnt = a.sc_pnt;
+    AT* atlas = reinterpret_cast<AT*>(a.atlas);
+    const int accumulate = a.accumulate;
     __shared__ float lx[GATHER_CHUNK], ly[GATHER_CHUNK], lz[GATHER_CHUNK];
     const int res0 = t.res[0];
     const int scs3 = scs * scs * scs;
     const int spb = (res0 / scs) * (res0 / scs) * (res0 / scs);
     const int nthreads = res0 * res0 * res0;
-    const u64 brick = blockIdx.x;
+    const u64 brick = bid;
     const int tid = threadIdx.x;
     const bool active = tid < nthreads;
     const int k = active ? tid / scs3 : 0;      // subcell within the brick
@@ -273,11 +294,16 @@ __global__ __launch_bounds__(512) void k_gather(Tree t, int scs, float radius, c
 // all 64 voxels; the four partial argmins per voxel are merged with shuffles
 // (smallest d^2, ties to the earlier entry, as one sequential pass).
 template <typename AT>
-__global__ __launch_bounds__(512) void k_gather_rows(Tree t, float radius, const int* sc_nid, const uint32_t* sc_use,
-                                                     const uint32_t* sc_off, const int* sc_pos, const float* sc_pnt,
-                                                     AT* atlas) {
+__device__ inline void k_gather_rows_body(const GatherArgs& a, uint32_t bid, uint32_t) {
+    const Tree& t = a.t;
+    const float radius = a.radius;
+    const int* sc_nid = a.sc_nid;
+    const uint32_t *sc_use = a.sc_use, *sc_off = a.sc_off;
+    const int* sc_pos = a.sc_pos;
+    const float* sc_pnt = a.sc_pnt;
+    AT* atlas = reinterpret_cast<AT*>(a.atlas);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const u64 sc = u64(blockIdx.x) * 8 + wave;
+    const u64 sc = u64(bid) * 8 + wave;
     const int row = lane & 15, g = lane >> 4;
     const int ry = row & 3, rz = row >> 2;
     const int sx = sc_pos[3 * sc], sy = sc_pos[3 * sc + 1], sz = sc_pos[3 * sc + 2];
@@ -338,41 +364,48 @@ __global__ __launch_bounds__(512) void k_gather_rows(Tree t, float radius, const
     }
 }
 
-hipError_t launch_gather_density(Tree t, int scs, int nbricks_sc, float radius, const int* sc_nid,
-                                 const uint32_t* sc_cnt, const uint32_t* sc_use, const uint32_t* sc_off,
-                                 const int* sc_pos, const float* sc_pnt, void* atlas, int atlas_type, int accumulate,
-                                 hipStream_t s) {
-    if (nbricks_sc <= 0) return hipSuccess;
-    int threads = t.res[0] * t.res[0] * t.res[0];
-    if (threads > 512) return hipErrorInvalidValue;
-    threads = (threads + 63) / 64 * 64;
-    const bool rows = !accumulate && t.res[0] == 8 && scs == 4;
-    if (rows) {
-        if (atlas_type == T_HALF)
-            hipLaunchKernelGGL(k_gather_rows<__half>, dim3(nbricks_sc), dim3(512), 0, s, t, radius, sc_nid, sc_use,
-                               sc_off, sc_pos, sc_pnt, (__half*)atlas);
-        else
-            hipLaunchKernelGGL(k_gather_rows<float>, dim3(nbricks_sc), dim3(512), 0, s, t, radius, sc_nid, sc_use,
-                               sc_off, sc_pos, sc_pnt, (float*)atlas);
-    } else if (atlas_type == T_HALF) {
-        hipLaunchKernelGGL(k_gather<__half>, dim3(nbricks_sc), dim3(threads), 0, s, t, scs, radius, sc_nid, sc_cnt,
-                           sc_use, sc_off, sc_pos, sc_pnt, (__half*)atlas, accumulate);
-    } else {
-        hipLaunchKernelGGL(k_gather<float>, dim3(nbricks_sc), dim3(threads), 0, s, t, scs, radius, sc_nid, sc_cnt,
-                           sc_use, sc_off, sc_pos, sc_pnt, (float*)atlas, accumulate);
-    }
-    return hipGetLastError();
-}
+GVDB_MULTI_T(k_gather, typename AT, AT, GatherArgs, 512)
+GVDB_MULTI_T(k_gather_rows, typename AT, AT, GatherArgs, 512)
 
+// One launch per (kernel, channel type) group: 8^3 bricks with 4^3 subcells
+// and no accumulation take the row-parallel kernel.
+hipError_t launch_gather_multi(const GatherArgs* a, int G, Arena* ar, hipStream_t s) {
+    for (int variant = 0; variant < 4; variant++) {
+        const bool rows = variant & 1, half = variant >> 1;
+        std::vector<GatherArgs> sel;
+        std::vector<uint32_t> nb;
+        for (int g = 0; g < G; g++) {
+            const bool r = !a[g].accumulate && a[g].t.res[0] == 8 && a[g].scs == 4;
+            if (r != rows || (a[g].atlas_type == T_HALF) != half || a[g].nbricks <= 0) continue;
+            if (a[g].t.res[0] * a[g].t.res[0] * a[g].t.res[0] > 512) return hipErrorInvalidValue;
+            sel.push_back(a[g]);
+            nb.push_back(uint32_t(a[g].nbricks));
+        }
+        if (sel.empty()) continue;
+        MultiLaunch L;
+        hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
+        if (e != hipSuccess) return e;
+        if (L.blocks == 0) continue;
+        if (rows && half) hipLaunchKernelGGL(k_gather_rows<__half>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
+        else if (rows) hipLaunchKernelGGL(k_gather_rows<float>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
+        else if (half) hipLaunchKernelGGL(k_gather<__half>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
+        else hipLaunchKernelGGL(k_gather<float>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
 }  // namespace gvdb
 
 namespace gvdb {
 
 // ComputeBounds (src/gvdb_volume_gvdb.cpp:1792-1816): min corner and max
 // corner + range of the flagged leaves.  out6 starts as {INT_MAX x3, INT_MIN x3}.
-__global__ __launch_bounds__(256) void k_leaf_bounds(Tree t, int nleaves, int* out6) {
+__device__ inline void k_leaf_bounds_body(const BoundsArgs& ba, uint32_t bid, uint32_t nblk) {
+    const Tree& t = ba.t;
+    int* out6 = ba.out6;
     int mn[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, mx[3] = {(int)0x80000000, (int)0x80000000, (int)0x80000000};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nleaves; i += gridDim.x * blockDim.x) {
+    for (int i = bid * 256 + threadIdx.x; i < ba.n; i += nblk * 256) {
         const Node* n = t.nodes[0] + i;
         if (!n->mFlags) continue;
 #pragma unroll
@@ -402,15 +435,23 @@ __global__ __launch_bounds__(256) void k_leaf_bounds(Tree t, int nleaves, int* o
     }
 }
 
-hipError_t launch_leaf_bounds(Tree t, int nleaves, int* out6, hipStream_t s) {
-    hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(out6), 0x7FFFFFFFu, 3, s);
+GVDB_MULTI(k_leaf_bounds, BoundsArgs, 256)
+
+hipError_t launch_leaf_bounds_multi(const BoundsArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<FillArgs> f;
+    std::vector<uint32_t> nb(G);
+    for (int g = 0; g < G; g++) {
+        f.push_back(FillArgs{a[g].out6, 3, 0x7FFFFFFFull, 0});
+        f.push_back(FillArgs{a[g].out6 + 3, 3, 0x80000000ull, 0});
+        uint32_t b = uint32_t((a[g].n + 255) / 256);
+        nb[g] = b > 256 ? 256 : b;
+    }
+    hipError_t e = launch_fill_multi(f.data(), int(f.size()), ar, s);
     if (e != hipSuccess) return e;
-    e = launch_fill_u32(reinterpret_cast<uint32_t*>(out6) + 3, 0x80000000u, 3, s);
-    if (e != hipSuccess) return e;
-    if (nleaves <= 0) return hipSuccess;
-    int blocks = (nleaves + 255) / 256;
-    if (blocks > 256) blocks = 256;
-    hipLaunchKernelGGL(k_leaf_bounds, dim3(blocks), dim3(256), 0, s, t, nleaves, out6);
+    MultiLaunch L;
+    e = multi_setup(ar, a, nb.data(), G, s, L);
+    if (e != hipSuccess || L.blocks == 0) return e;
+    hipLaunchKernelGGL(k_leaf_bounds, dim3(L.blocks), dim3(256), 0, s, a[0], L.map);
     return hipGetLastError();
 }
 
@@ -429,10 +470,13 @@ hipError_t launch_leaf_bounds(Tree t, int nleaves, int* out6, hipStream_t s) {
 // neighbourhood fits in the 27 bricks around; otherwise the push kernels run.
 namespace gvdb {
 
-__global__ __launch_bounds__(256) void k_bucket_points(Tree t, PointSrc pts, int scs, int spb, float tx, float ty,
-                                                       float tz, uint32_t* key, uint32_t* slot, uint32_t* bcnt,
-                                                       uint32_t* orphans) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void k_bucket_points_body(const BucketArgs& a, uint32_t bid, uint32_t) {
+    const Tree& t = a.t;
+    const PointSrc& pts = a.pts;
+    const int scs = a.scs, spb = a.spb;
+    const float tx = a.tr[0], ty = a.tr[1], tz = a.tr[2];
+    uint32_t *key = a.key, *slot = a.slot, *bcnt = a.bcnt, *orphans = a.orphans;
+    int i = bid * 256 + threadIdx.x;
     if (i >= pts.n) return;
     const float* p = point_at(pts, i);
     float p0 = p[0], p1 = p[1], p2 = p[2];
@@ -456,9 +500,12 @@ __global__ __launch_bounds__(256) void k_bucket_points(Tree t, PointSrc pts, int
     key[i] = k;
 }
 
-__global__ __launch_bounds__(256) void k_bucket_scatter(PointSrc pts, float tx, float ty, float tz, const uint32_t* key,
-                                                        const uint32_t* slot, const uint32_t* boff, float* sorted) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void k_bucket_scatter_body(const ScatterArgs& a, uint32_t bid, uint32_t) {
+    const PointSrc& pts = a.pts;
+    const float tx = a.tr[0], ty = a.tr[1], tz = a.tr[2];
+    const uint32_t *key = a.key, *slot = a.slot, *boff = a.boff;
+    float* sorted = a.sorted;
+    int i = bid * 256 + threadIdx.x;
     if (i >= pts.n) return;
     uint32_t k = key[i];
     if (k == 0xFFFFFFFFu) return;
@@ -599,12 +646,17 @@ __device__ inline void pull_mask(const Node* brick, int scs, int sd, float r, fl
 // first (the reference leaves the order inside a subcell unspecified); their
 // number per subcell goes to sc_use, so the gather can stop there.
 template <bool WRITE>
-__global__ __launch_bounds__(PULL_THREADS) void k_pull(Tree t, int scs, int R, float radius, const int* nbr27,
-                                                       const uint32_t* bcnt, const uint32_t* boff, const float* sorted,
-                                                       const uint32_t* mapping, uint32_t* sc_cnt, uint32_t* sc_use,
-                                                       const uint32_t* sc_off, float* sc_pnt) {
+__device__ inline void k_pull_body(const PullArgs& a, uint32_t bid, uint32_t) {
+    const Tree& t = a.t;
+    const int scs = a.scs, R = a.R;
+    const float radius = a.radius;
+    const int* nbr27 = a.nbr;
+    const uint32_t *bcnt = a.bcnt, *boff = a.boff, *mapping = a.mapping, *sc_off = a.sc_off;
+    const float* sorted = a.sorted;
+    uint32_t *sc_cnt = a.sc_cnt, *sc_use = a.sc_use;
+    float* sc_pnt = a.sc_pnt;
     __shared__ PullShared S;
-    const int brick = blockIdx.x;
+    const int brick = int(bid);
     const Node* bn = t.nodes[0] + brick;
     const int sd = t.res[0] / scs;
     const int spb = sd * sd * sd;
@@ -679,36 +731,34 @@ __global__ __launch_bounds__(PULL_THREADS) void k_pull(Tree t, int scs, int R, f
     }
 }
 
-hipError_t launch_bucket_points(Tree t, PointSrc pts, int scs, int spb, const float trans[3], uint32_t* key,
-                                uint32_t* slot, uint32_t* bcnt, uint32_t* orphans, hipStream_t s) {
-    if (pts.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bucket_points, dim3((pts.n + 255) / 256), dim3(256), 0, s, t, pts, scs, spb, trans[0], trans[1],
-                       trans[2], key, slot, bcnt, orphans);
-    return hipGetLastError();
+GVDB_MULTI(k_bucket_points, BucketArgs, 256)
+GVDB_MULTI(k_bucket_scatter, ScatterArgs, 256)
+GVDB_MULTI_T(k_pull, bool WRITE, WRITE, PullArgs, PULL_THREADS)
+
+hipError_t launch_bucket_points_multi(const BucketArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> items(G);
+    for (int g = 0; g < G; g++) items[g] = uint32_t(a[g].pts.n > 0 ? a[g].pts.n : 0);
+    return launch_per_item<BucketArgs, k_bucket_points>(a, items.data(), G, ar, s, 256);
 }
 
-hipError_t launch_bucket_scatter(PointSrc pts, const float trans[3], const uint32_t* key, const uint32_t* slot,
-                                 const uint32_t* boff, float* sorted, hipStream_t s) {
-    if (pts.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bucket_scatter, dim3((pts.n + 255) / 256), dim3(256), 0, s, pts, trans[0], trans[1], trans[2],
-                       key, slot, boff, sorted);
-    return hipGetLastError();
+hipError_t launch_bucket_scatter_multi(const ScatterArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> items(G);
+    for (int g = 0; g < G; g++) items[g] = uint32_t(a[g].pts.n > 0 ? a[g].pts.n : 0);
+    return launch_per_item<ScatterArgs, k_bucket_scatter>(a, items.data(), G, ar, s, 256);
 }
 
 int pull_cells(int sd, int R) { int span = sd + 2 * R; return span * span * span; }
 int pull_max_cells() { return PULL_MAX_CELLS; }
 int pull_max_spb() { return PULL_MAX_SPB; }
 
-hipError_t launch_pull(int write, Tree t, int nleaves, int scs, int R, float radius, const int* nbr27,
-                       const uint32_t* bcnt, const uint32_t* boff, const float* sorted, const uint32_t* mapping,
-                       uint32_t* sc_cnt, uint32_t* sc_use, const uint32_t* sc_off, float* sc_pnt, hipStream_t s) {
-    if (nleaves <= 0) return hipSuccess;
-    if (write)
-        hipLaunchKernelGGL(k_pull<true>, dim3(nleaves), dim3(PULL_THREADS), 0, s, t, scs, R, radius, nbr27, bcnt, boff,
-                           sorted, mapping, sc_cnt, sc_use, sc_off, sc_pnt);
-    else
-        hipLaunchKernelGGL(k_pull<false>, dim3(nleaves), dim3(PULL_THREADS), 0, s, t, scs, R, radius, nbr27, bcnt, boff,
-                           sorted, mapping, sc_cnt, sc_use, sc_off, sc_pnt);
+hipError_t launch_pull_multi(int write, const PullArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> nb(G);
+    for (int g = 0; g < G; g++) nb[g] = uint32_t(a[g].nleaves > 0 ? a[g].nleaves : 0);
+    MultiLaunch L;
+    hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
+    if (e != hipSuccess || L.blocks == 0) return e;
+    if (write) hipLaunchKernelGGL(k_pull<true>, dim3(L.blocks), dim3(PULL_THREADS), 0, s, a[0], L.map);
+    else hipLaunchKernelGGL(k_pull<false>, dim3(L.blocks), dim3(PULL_THREADS), 0, s, a[0], L.map);
     return hipGetLastError();
 }
 

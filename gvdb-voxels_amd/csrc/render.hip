@@ -50,14 +50,6 @@ __device__ inline float tri(const AT* b, int br, float px, float py, float pz) {
     return c0 + fz * (c1 - c0);
 }
 
-struct RenderArgs {
-    Tree t;
-    ScnBlock s;
-    const float4* tf;
-    const void* atlas;
-    void* out;
-    int out_float;
-};
 
 struct Hdda {
     f3 pos, dir, tDel, t, tSide;
@@ -229,9 +221,9 @@ __device__ inline f3 mmult(const float* m, f3 v) {
 }
 
 template <int MODE, typename AT>
-__global__ __launch_bounds__(64) void k_render(RenderArgs A) {
+__device__ inline void k_render_body(const RenderArgs& A, uint32_t bid) {
     const int tiles_x = (A.s.width + 7) / 8;
-    const int tile = blockIdx.x;
+    const int tile = int(bid);
     const int x = (tile % tiles_x) * 8 + (threadIdx.x & 7);
     const int y = (tile / tiles_x) * 8 + (threadIdx.x >> 3);
     if (x >= A.s.width || y >= A.s.height) return;
@@ -283,28 +275,43 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs A) {
     }
 }
 
-hipError_t launch_render(Tree t, const ScnBlock& scn, const float4* tf, const void* atlas, int atlas_type, int shading,
-                         void* out, int out_float, hipStream_t s) {
-    RenderArgs A;
-    A.t = t;
-    A.s = scn;
-    A.tf = tf;
-    A.atlas = atlas;
-    A.out = out;
-    A.out_float = out_float;
-    unsigned tiles = unsigned(((scn.width + 7) / 8) * ((scn.height + 7) / 8));
-    if (tiles == 0) return hipSuccess;
-    bool half = atlas_type == T_HALF;
-    if (shading == SHADE_VOLUME) {
-        if (half) hipLaunchKernelGGL((k_render<0, __half>), dim3(tiles), dim3(64), 0, s, A);
-        else hipLaunchKernelGGL((k_render<0, float>), dim3(tiles), dim3(64), 0, s, A);
-    } else if (shading == SHADE_TRILINEAR) {
-        if (half) hipLaunchKernelGGL((k_render<1, __half>), dim3(tiles), dim3(64), 0, s, A);
-        else hipLaunchKernelGGL((k_render<1, float>), dim3(tiles), dim3(64), 0, s, A);
-    } else {
-        return hipErrorInvalidValue;
+// One 64-thread workgroup per 8x8 tile; several grids' images in one launch.
+template <int MODE, typename AT>
+__global__ __launch_bounds__(64) void k_render(RenderArgs a0, MultiMap m) {
+    if (m.G <= 1) {
+        k_render_body<MODE, AT>(a0, blockIdx.x);
+        return;
     }
-    return hipGetLastError();
+    const int g = multi_grid(m, blockIdx.x);
+    k_render_body<MODE, AT>(reinterpret_cast<const RenderArgs*>(m.table)[g], blockIdx.x - m.first[g]);
+}
+
+hipError_t launch_render_multi(const RenderArgs* a, int G, Arena* ar, hipStream_t s) {
+    for (int variant = 0; variant < 4; variant++) {
+        const int mode = variant & 1;
+        const bool half = variant >> 1;
+        std::vector<RenderArgs> sel;
+        std::vector<uint32_t> nb;
+        for (int g = 0; g < G; g++) {
+            if (a[g].shading != SHADE_VOLUME && a[g].shading != SHADE_TRILINEAR) return hipErrorInvalidValue;
+            const int am = a[g].shading == SHADE_VOLUME ? 0 : 1;
+            if (am != mode || (a[g].atlas_type == T_HALF) != half) continue;
+            sel.push_back(a[g]);
+            nb.push_back(uint32_t(((a[g].s.width + 7) / 8) * ((a[g].s.height + 7) / 8)));
+        }
+        if (sel.empty()) continue;
+        MultiLaunch L;
+        hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
+        if (e != hipSuccess) return e;
+        if (L.blocks == 0) continue;
+        if (mode == 0 && half) hipLaunchKernelGGL((k_render<0, __half>), dim3(L.blocks), dim3(64), 0, s, sel[0], L.map);
+        else if (mode == 0) hipLaunchKernelGGL((k_render<0, float>), dim3(L.blocks), dim3(64), 0, s, sel[0], L.map);
+        else if (half) hipLaunchKernelGGL((k_render<1, __half>), dim3(L.blocks), dim3(64), 0, s, sel[0], L.map);
+        else hipLaunchKernelGGL((k_render<1, float>), dim3(L.blocks), dim3(64), 0, s, sel[0], L.map);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace gvdb

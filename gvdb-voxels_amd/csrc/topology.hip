@@ -29,46 +29,60 @@ __device__ inline const float* point_at(const PointSrc& p, int i) {
     return reinterpret_cast<const float*>(p.base + u64(i) * p.stride + p.off);
 }
 
+static uint32_t grid_for(u64 items, uint32_t cap) {
+    u64 b = (items + 255) / 256;
+    if (b > cap) b = cap;
+    if (b < 1) b = 1;
+    return uint32_t(b);
+}
+
 // GetBoundingBox's split + Thrust min/max (src/gvdb_volume_gvdb.cpp:5404-5467).
-__global__ __launch_bounds__(256) void k_bbox(PointSrc pts, float ox, float oy, float oz, uint32_t* keys) {
+// keys6 must hold {~0 x3, 0 x3} before the launch.
+__device__ inline void k_bbox_body(const BboxArgs& a, uint32_t bid, uint32_t nblk) {
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < pts.n; i += gridDim.x * blockDim.x) {
-        const float* p = point_at(pts, i);
-        float w[3] = {p[0] + ox, p[1] + oy, p[2] + oz};
+    for (int i = bid * 256 + threadIdx.x; i < a.pts.n; i += nblk * 256) {
+        const float* p = point_at(a.pts, i);
+        float w[3] = {p[0] + a.o[0], p[1] + a.o[1], p[2] + a.o[2]};
 #pragma unroll
-        for (int a = 0; a < 3; a++) { mn[a] = fminf(mn[a], w[a]); mx[a] = fmaxf(mx[a], w[a]); }
+        for (int k = 0; k < 3; k++) { mn[k] = fminf(mn[k], w[k]); mx[k] = fmaxf(mx[k], w[k]); }
     }
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
+    for (int k = 0; k < 3; k++) {
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
-            mn[a] = fminf(mn[a], __shfl_xor(mn[a], d, 64));
-            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], d, 64));
+            mn[k] = fminf(mn[k], __shfl_xor(mn[k], d, 64));
+            mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], d, 64));
         }
     }
     __shared__ float smn[4][3], smx[4][3];
     int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
-        for (int a = 0; a < 3; a++) { smn[wave][a] = mn[a]; smx[wave][a] = mx[a]; }
+        for (int k = 0; k < 3; k++) { smn[wave][k] = mn[k]; smx[wave][k] = mx[k]; }
     __syncthreads();
     if (threadIdx.x < 3) {
-        int a = threadIdx.x;
-        float m0 = smn[0][a], m1 = smx[0][a];
-        for (int w = 1; w < 4; w++) { m0 = fminf(m0, smn[w][a]); m1 = fmaxf(m1, smx[w][a]); }
-        atomicMin(&keys[a], float_key(m0));
-        atomicMax(&keys[3 + a], float_key(m1));
+        int k = threadIdx.x;
+        float m0 = smn[0][k], m1 = smx[0][k];
+        for (int w = 1; w < 4; w++) { m0 = fminf(m0, smn[w][k]); m1 = fmaxf(m1, smx[w][k]); }
+        atomicMin(&a.keys6[k], float_key(m0));
+        atomicMax(&a.keys6[3 + k], float_key(m1));
     }
 }
+GVDB_MULTI(k_bbox, BboxArgs, 256)
 
-hipError_t launch_bbox(PointSrc pts, const float orig[3], uint32_t* keys6, hipStream_t s) {
-    hipError_t e = launch_fill_u32(keys6, 0xFFFFFFFFu, 3, s);
+hipError_t launch_bbox_multi(const BboxArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<FillArgs> f;
+    std::vector<uint32_t> nb(G);
+    for (int g = 0; g < G; g++) {
+        f.push_back(FillArgs{a[g].keys6, 3, 0xFFFFFFFFull, 0});
+        f.push_back(FillArgs{a[g].keys6 + 3, 3, 0ull, 0});
+        nb[g] = grid_for(u64(a[g].pts.n), 2048);
+    }
+    hipError_t e = launch_fill_multi(f.data(), int(f.size()), ar, s);
     if (e != hipSuccess) return e;
-    e = launch_fill_u32(keys6 + 3, 0u, 3, s);
-    if (e != hipSuccess) return e;
-    int blocks = (pts.n + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_bbox, dim3(blocks), dim3(256), 0, s, pts, orig[0], orig[1], orig[2], keys6);
+    MultiLaunch L;
+    e = multi_setup(ar, a, nb.data(), G, s, L);
+    if (e != hipSuccess || L.blocks == 0) return e;
+    hipLaunchKernelGGL(k_bbox, dim3(L.blocks), dim3(256), 0, s, a[0], L.map);
     return hipGetLastError();
 }
 
@@ -104,63 +118,66 @@ __device__ inline void set_bit(uint32_t* bm, u64 idx) {
 // (`masks`, bit 13 = the brick itself), which k_expand_masks turns into bitmap
 // bits; this replaces up to 8 bitmap atomics per point by one rarely taken
 // atomic.  Points with wider reach (or no mask array) set the bits directly.
-__global__ __launch_bounds__(256) void k_mark_points(PointSrc pts, float ox, float oy, float oz, float radius,
-                                                     int dilate, int range0, float inv_range0, BitDomain d0,
-                                                     uint32_t* bmA, uint32_t* bmB, uint32_t* masks) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < pts.n; i += gridDim.x * blockDim.x) {
-        const float* p = point_at(pts, i);
-        float w[3] = {p[0] + ox, p[1] + oy, p[2] + oz};
+__device__ inline void k_mark_points_body(const MarkArgs& a, uint32_t bid, uint32_t nblk) {
+    const BitDomain d0 = a.d0;
+    const int range0 = a.range0;
+    for (int i = bid * 256 + threadIdx.x; i < a.pts.n; i += nblk * 256) {
+        const float* p = point_at(a.pts, i);
+        float w[3] = {p[0] + a.o[0], p[1] + a.o[1], p[2] + a.o[2]};
         if (!isfinite(w[0]) || !isfinite(w[1]) || !isfinite(w[2])) continue;
         if (w[0] < 0 || w[1] < 0 || w[2] < 0) continue;
         int c[3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) c[a] = int(floorf(w[a] * inv_range0));
+        for (int k = 0; k < 3; k++) c[k] = int(floorf(w[k] * a.inv_range0));
         u64 idx;
         const bool inside = dom_index(d0, c[0], c[1], c[2], idx);
         int lo[3] = {c[0], c[1], c[2]}, hi[3] = {c[0], c[1], c[2]};
-        if (dilate) {
+        if (a.dilate) {
 #pragma unroll
-            for (int a = 0; a < 3; a++) {
-                int iw = int(w[a]);
-                int bmin = int(float(iw) - radius);
-                int bmax = int(float(iw) + 2.0f * radius);
-                lo[a] = bmin / range0;   // C truncation, as the reference's int division
-                hi[a] = bmax / range0;
+            for (int k = 0; k < 3; k++) {
+                int iw = int(w[k]);
+                int bmin = int(float(iw) - a.radius);
+                int bmax = int(float(iw) + 2.0f * a.radius);
+                lo[k] = bmin / range0;   // C truncation, as the reference's int division
+                hi[k] = bmax / range0;
             }
         }
         const bool near = lo[0] >= c[0] - 1 && lo[1] >= c[1] - 1 && lo[2] >= c[2] - 1 && hi[0] <= c[0] + 1 &&
                           hi[1] <= c[1] + 1 && hi[2] <= c[2] + 1;
-        if (masks && inside && near) {
+        if (a.masks && inside && near) {
             uint32_t m = 0;
             for (int x = lo[0]; x <= hi[0]; x++)
                 for (int y = lo[1]; y <= hi[1]; y++)
                     for (int z = lo[2]; z <= hi[2]; z++)
                         m |= 1u << ((z - c[2] + 1) * 9 + (y - c[1] + 1) * 3 + (x - c[0] + 1));
             m |= 1u << 13;
-            uint32_t* wd = masks + idx;
+            uint32_t* wd = a.masks + idx;
             if ((__hip_atomic_load(wd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) != m)
                 __hip_atomic_fetch_or(wd, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             continue;
         }
-        if (inside) set_bit(bmA, idx);
-        if (dilate) {
+        if (inside) set_bit(a.bmA, idx);
+        if (a.dilate) {
             for (int x = lo[0]; x <= hi[0]; x++)
                 for (int y = lo[1]; y <= hi[1]; y++)
                     for (int z = lo[2]; z <= hi[2]; z++)
-                        if (dom_index(d0, x, y, z, idx)) set_bit(bmB, idx);
+                        if (dom_index(d0, x, y, z, idx)) set_bit(a.bmB, idx);
         }
     }
 }
+GVDB_MULTI(k_mark_points, MarkArgs, 256)
 
 // Direction masks -> level-0 bitmaps: bit 13 marks the brick itself in A (and
 // in B), the other bits mark the neighbours in B.
-__global__ __launch_bounds__(256) void k_expand_masks(BitDomain d0, const uint32_t* masks, uint32_t* bmA, uint32_t* bmB) {
-    for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < d0.nbits; i += u64(gridDim.x) * blockDim.x) {
-        uint32_t m = masks[i];
+__device__ inline void k_expand_masks_body(const MarkArgs& a, uint32_t bid, uint32_t nblk) {
+    const BitDomain d0 = a.d0;
+    uint32_t* bmB = a.dilate ? a.bmB : nullptr;
+    for (u64 i = u64(bid) * 256 + threadIdx.x; i < d0.nbits; i += u64(nblk) * 256) {
+        uint32_t m = a.masks[i];
         if (!m) continue;
         int x, y, z;
         dom_coords(d0, i, x, y, z);
-        if (m & (1u << 13)) set_bit(bmA, i);
+        if (m & (1u << 13)) set_bit(a.bmA, i);
         if (!bmB) continue;
         while (m) {
             int d = __ffs(m) - 1;
@@ -170,38 +187,46 @@ __global__ __launch_bounds__(256) void k_expand_masks(BitDomain d0, const uint32
         }
     }
 }
+GVDB_MULTI(k_expand_masks, MarkArgs, 256)
 
-hipError_t launch_mark_points(PointSrc pts, const float orig[3], float radius, int dilate, int range0, BitDomain d0,
-                              uint32_t* bmA, uint32_t* bmB, uint32_t* masks, hipStream_t s) {
-    int blocks = (pts.n + 255) / 256;
-    if (blocks > 16384) blocks = 16384;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_mark_points, dim3(blocks), dim3(256), 0, s, pts, orig[0], orig[1], orig[2], radius, dilate,
-                       range0, 1.0f / float(range0), d0, bmA, bmB, masks);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !masks) return e;
-    u64 b2 = (d0.nbits + 255) / 256;
-    if (b2 > 16384) b2 = 16384;
-    if (b2 < 1) b2 = 1;
-    hipLaunchKernelGGL(k_expand_masks, dim3(unsigned(b2)), dim3(256), 0, s, d0, masks, bmA, dilate ? bmB : nullptr);
+hipError_t launch_mark_points_multi(const MarkArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> nb(G), nb2(G);
+    bool any_mask = false;
+    for (int g = 0; g < G; g++) {
+        nb[g] = a[g].pts.n > 0 ? grid_for(u64(a[g].pts.n), 16384) : 0;
+        nb2[g] = a[g].masks ? grid_for(a[g].d0.nbits, 16384) : 0;
+        any_mask |= a[g].masks != nullptr;
+    }
+    MultiLaunch L;
+    hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
+    if (e != hipSuccess) return e;
+    if (L.blocks) {
+        hipLaunchKernelGGL(k_mark_points, dim3(L.blocks), dim3(256), 0, s, a[0], L.map);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (!any_mask) return hipSuccess;
+    MultiLaunch L2;
+    e = multi_setup(ar, a, nb2.data(), G, s, L2);
+    if (e != hipSuccess || L2.blocks == 0) return e;
+    hipLaunchKernelGGL(k_expand_masks, dim3(L2.blocks), dim3(256), 0, s, a[0], L2.map);
     return hipGetLastError();
 }
 
 // Mark level L+1 from the set bits of level L (parent = child >> shift).
-__global__ __launch_bounds__(256) void k_mark_parents(BitDomain dc, const uint32_t* bmA_c, const uint32_t* bmB_c,
-                                                      int shift, BitDomain dp, uint32_t* bmA_p, uint32_t* bmB_p) {
-    for (u64 w = u64(blockIdx.x) * blockDim.x + threadIdx.x; w < dc.nwords; w += u64(gridDim.x) * blockDim.x) {
+__device__ inline void k_mark_parents_body(const ParentArgs& a, uint32_t bid, uint32_t nblk) {
+    for (u64 w = u64(bid) * 256 + threadIdx.x; w < a.dc.nwords; w += u64(nblk) * 256) {
         for (int pass = 0; pass < 2; pass++) {
-            uint32_t bits = pass == 0 ? bmA_c[w] : (bmB_c ? bmB_c[w] : 0u);
-            uint32_t* out = pass == 0 ? bmA_p : bmB_p;
+            uint32_t bits = pass == 0 ? a.Ac[w] : (a.Bc ? a.Bc[w] : 0u);
+            uint32_t* out = pass == 0 ? a.Ap : a.Bp;
             u64 last = ~u64(0);
             while (bits) {
                 int b = __ffs(bits) - 1;
                 bits &= bits - 1;
                 int x, y, z;
-                dom_coords(dc, w * 32 + b, x, y, z);
+                dom_coords(a.dc, w * 32 + b, x, y, z);
                 u64 idx;
-                if (dom_index(dp, x >> shift, y >> shift, z >> shift, idx) && idx != last) {
+                if (dom_index(a.dp, x >> a.shift, y >> a.shift, z >> a.shift, idx) && idx != last) {
                     set_bit(out, idx);
                     last = idx;
                 }
@@ -209,14 +234,15 @@ __global__ __launch_bounds__(256) void k_mark_parents(BitDomain dc, const uint32
         }
     }
 }
+GVDB_MULTI(k_mark_parents, ParentArgs, 256)
 
-hipError_t launch_mark_parents(BitDomain dc, const uint32_t* bmA_c, const uint32_t* bmB_c, int shift, BitDomain dp,
-                               uint32_t* bmA_p, uint32_t* bmB_p, hipStream_t s) {
-    u64 blocks = (dc.nwords + 255) / 256;
-    if (blocks > 16384) blocks = 16384;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_mark_parents, dim3(unsigned(blocks)), dim3(256), 0, s, dc, bmA_c, bmB_c, shift, dp, bmA_p,
-                       bmB_p);
+hipError_t launch_mark_parents_multi(const ParentArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> nb(G);
+    for (int g = 0; g < G; g++) nb[g] = grid_for(a[g].dc.nwords, 16384);
+    MultiLaunch L;
+    hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
+    if (e != hipSuccess || L.blocks == 0) return e;
+    hipLaunchKernelGGL(k_mark_parents, dim3(L.blocks), dim3(256), 0, s, a[0], L.map);
     return hipGetLastError();
 }
 
@@ -237,39 +263,43 @@ __device__ inline void write_node(Node* n, int lev, int x, int y, int z, int ran
 
 // Node creation in sorted key order (src/gvdb_volume_gvdb.cpp:1308-1331 for the
 // first pass, :965-988 for the appended dilation nodes).
-__global__ __launch_bounds__(256) void k_emit_nodes(LevelBuild L, Node* nodes, int has_clist) {
-    for (u64 w = u64(blockIdx.x) * blockDim.x + threadIdx.x; w < L.dom.nwords; w += u64(gridDim.x) * blockDim.x) {
-        uint32_t a = L.bmA[w];
+__device__ inline void k_emit_nodes_body(const EmitArgs& a, uint32_t bid, uint32_t nblk) {
+    const LevelBuild& L = a.L;
+    for (u64 w = u64(bid) * 256 + threadIdx.x; w < L.dom.nwords; w += u64(nblk) * 256) {
+        uint32_t av = L.bmA[w];
         uint32_t idx = L.rankA[w];
-        uint32_t bits = a;
+        uint32_t bits = av;
         while (bits) {
             int b = __ffs(bits) - 1;
             bits &= bits - 1;
             int x, y, z;
             dom_coords(L.dom, w * 32 + b, x, y, z);
-            write_node(nodes + idx, L.lev, x, y, z, L.range, has_clist ? Elem(1, L.lev, idx) : ID_UNDEFL);
+            write_node(a.nodes + idx, L.lev, x, y, z, L.range, a.has_clist ? Elem(1, L.lev, idx) : ID_UNDEFL);
             idx++;
         }
         if (L.bmB != L.bmA && L.bmB) {
-            uint32_t e = L.bmB[w] & ~a;
+            uint32_t e = L.bmB[w] & ~av;
             uint32_t j = L.countA + L.rankE[w];
             while (e) {
                 int b = __ffs(e) - 1;
                 e &= e - 1;
                 int x, y, z;
                 dom_coords(L.dom, w * 32 + b, x, y, z);
-                write_node(nodes + j, L.lev, x, y, z, L.range, has_clist ? Elem(1, L.lev, j) : ID_UNDEFL);
+                write_node(a.nodes + j, L.lev, x, y, z, L.range, a.has_clist ? Elem(1, L.lev, j) : ID_UNDEFL);
                 j++;
             }
         }
     }
 }
+GVDB_MULTI(k_emit_nodes, EmitArgs, 256)
 
-hipError_t launch_emit_nodes(const LevelBuild& L, Node* nodes, int has_clist, hipStream_t s) {
-    u64 blocks = (L.dom.nwords + 255) / 256;
-    if (blocks > 16384) blocks = 16384;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_emit_nodes, dim3(unsigned(blocks)), dim3(256), 0, s, L, nodes, has_clist);
+hipError_t launch_emit_nodes_multi(const EmitArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> nb(G);
+    for (int g = 0; g < G; g++) nb[g] = grid_for(a[g].L.dom.nwords, 16384);
+    MultiLaunch L;
+    hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
+    if (e != hipSuccess || L.blocks == 0) return e;
+    hipLaunchKernelGGL(k_emit_nodes, dim3(L.blocks), dim3(256), 0, s, a[0], L.map);
     return hipGetLastError();
 }
 
@@ -285,56 +315,66 @@ __device__ inline uint32_t rank_of(const LevelBuild& P, u64 idx) {
 
 // gvdbLinkBricks (kernels/cuda_gvdb_nodes.cuh:470-505): parent pointer and
 // the child-list slot (z*res + y)*res + x of the node inside its parent.
-__global__ __launch_bounds__(256) void k_link_level(LevelBuild C, LevelBuild P, int parent_is_root, int logdim_p,
-                                                    Node* cnodes, u64* pclists, u64 clist_len) {
-    uint32_t total = C.countA + C.countE;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        Node* n = cnodes + i;
+__device__ inline void k_link_level_body(const LinkArgs& a, uint32_t bid, uint32_t nblk) {
+    const LevelBuild& C = a.C;
+    const uint32_t total = C.countA + C.countE;
+    const int lp = a.logdim_p;
+    for (uint32_t i = bid * 256 + threadIdx.x; i < total; i += nblk * 256) {
+        Node* n = a.cnodes + i;
         int c[3] = {n->mPos[0] / C.range, n->mPos[1] / C.range, n->mPos[2] / C.range};
-        int pc[3] = {c[0] >> logdim_p, c[1] >> logdim_p, c[2] >> logdim_p};
+        int pc[3] = {c[0] >> lp, c[1] >> lp, c[2] >> lp};
         uint32_t pidx = 0;
-        if (!parent_is_root) {
+        if (!a.parent_is_root) {
             u64 idx;
-            if (!dom_index(P.dom, pc[0], pc[1], pc[2], idx)) continue;
-            pidx = rank_of(P, idx);
+            if (!dom_index(a.P.dom, pc[0], pc[1], pc[2], idx)) continue;
+            pidx = rank_of(a.P, idx);
         }
-        int res = 1 << logdim_p;
-        int l[3] = {c[0] - (pc[0] << logdim_p), c[1] - (pc[1] << logdim_p), c[2] - (pc[2] << logdim_p)};
+        int res = 1 << lp;
+        int l[3] = {c[0] - (pc[0] << lp), c[1] - (pc[1] << lp), c[2] - (pc[2] << lp)};
         int bit = (l[2] * res + l[1]) * res + l[0];
         n->mParent = Elem(0, C.lev + 1, pidx);
-        pclists[u64(pidx) * clist_len + bit] = Elem(0, C.lev, i);
+        a.pclists[u64(pidx) * a.clist_len + bit] = Elem(0, C.lev, i);
     }
 }
+GVDB_MULTI(k_link_level, LinkArgs, 256)
 
-hipError_t launch_link_level(const LevelBuild& child, const LevelBuild* parent, int parent_is_root, int logdim_parent,
-                             Node* cnodes, u64* parent_clists, u64 clist_len, hipStream_t s) {
-    uint32_t total = child.countA + child.countE;
-    if (total == 0) return hipSuccess;
-    unsigned blocks = (total + 255) / 256;
-    if (blocks > 16384) blocks = 16384;
-    LevelBuild P = parent ? *parent : child;
-    hipLaunchKernelGGL(k_link_level, dim3(blocks), dim3(256), 0, s, child, P, parent_is_root, logdim_parent, cnodes,
-                       parent_clists, clist_len);
+hipError_t launch_link_level_multi(const LinkArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> nb(G);
+    for (int g = 0; g < G; g++) {
+        u64 t = u64(a[g].C.countA) + a[g].C.countE;
+        nb[g] = t ? grid_for(t, 16384) : 0;
+    }
+    MultiLaunch L;
+    hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
+    if (e != hipSuccess || L.blocks == 0) return e;
+    hipLaunchKernelGGL(k_link_level, dim3(L.blocks), dim3(256), 0, s, a[0], L.map);
     return hipGetLastError();
 }
 
-__global__ void k_write_root(Node* root, int x, int y, int z, int lev) {
+// Root node (SetupNode at the top level, src/gvdb_volume_gvdb.cpp:1289-1300).
+__device__ inline void k_write_root_body(const RootArgs& a, uint32_t, uint32_t) {
+    if (threadIdx.x != 0) return;
     Node v;
-    v.mLev = uint8_t(lev);
+    v.mLev = uint8_t(a.lev);
     v.mFlags = 1;
     v.mPriority = 0;
     v.pad = 0;
-    v.mPos[0] = x; v.mPos[1] = y; v.mPos[2] = z;
+    v.mPos[0] = a.pos[0]; v.mPos[1] = a.pos[1]; v.mPos[2] = a.pos[2];
     v.mValue[0] = v.mValue[1] = v.mValue[2] = -1;
     v.mVRange[0] = v.mVRange[1] = v.mVRange[2] = 0.f;
     v.mParent = ID_UNDEFL;
-    v.mChildList = Elem(1, lev, 0);
+    v.mChildList = Elem(1, a.lev, 0);
     v.mMask = 0;
-    *root = v;
+    *a.root = v;
 }
+GVDB_MULTI(k_write_root, RootArgs, 64)
 
-hipError_t launch_write_root(Node* root, const int pos[3], int lev, hipStream_t s) {
-    hipLaunchKernelGGL(k_write_root, dim3(1), dim3(1), 0, s, root, pos[0], pos[1], pos[2], lev);
+hipError_t launch_write_root_multi(const RootArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> nb(G, 1u);
+    MultiLaunch L;
+    hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
+    if (e != hipSuccess || L.blocks == 0) return e;
+    hipLaunchKernelGGL(k_write_root, dim3(L.blocks), dim3(64), 0, s, a[0], L.map);
     return hipGetLastError();
 }
 

@@ -52,6 +52,17 @@ void DevBuf::release() {
         if (e_ != hipSuccess) return hip_fail(e_, #x);     \
     } while (0)
 
+Exec Volume::self_exec() {
+    return Exec{stream, &arena_, small_.as<uint32_t>(), pinned_};
+}
+
+int Volume::use_stream(hipStream_t s) {
+    // joins (or leaves, s = null) a shared stream; work queued so far drains first
+    HIPCHK(hipStreamSynchronize(stream));
+    stream = s ? s : own_stream_;
+    return GVDB_OK;
+}
+
 int Volume::hip_fail(hipError_t e, const char* what) {
     err = std::string("HIP error ") + hipGetErrorString(e) + " in " + what;
     return e == hipErrorOutOfMemory ? GVDB_ERR_OOM : GVDB_ERR_HIP;
@@ -62,6 +73,7 @@ Volume::Volume(int dev) : device(dev) {}
 Volume::~Volume() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
+    if (own_stream_ && own_stream_ != stream) (void)hipStreamSynchronize(own_stream_);
     for (int g = 0; g < 2; g++)
         for (int l = 0; l < MAXLEV; l++) pool_[g][l].dev.release();
     for (auto& c : chan_) c.data.release();
@@ -76,18 +88,21 @@ Volume::~Volume() {
         for (int k = 0; k < 2; k++)
             if (kev_[s][k]) (void)hipEventDestroy(kev_[s][k]);
     if (pinned_) (void)hipHostFree(pinned_);
-    if (stream) (void)hipStreamDestroy(stream);
+    arena_.release();
+    if (own_stream_) (void)hipStreamDestroy(own_stream_);
 }
 
 int Volume::init() {
     HIPCHK(hipSetDevice(device));
-    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&own_stream_, hipStreamNonBlocking));
+    stream = own_stream_;
     for (int s = 0; s < ST_COUNT; s++)
         for (int k = 0; k < 2; k++) HIPCHK(hipEventCreate(&ev_[s][k]));
     for (int s = 0; s < 4; s++)
         for (int k = 0; k < 2; k++) HIPCHK(hipEventCreate(&kev_[s][k]));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 256 * sizeof(uint32_t), hipHostMallocDefault));
-    HIPCHK(small_.ensure(4096));
+    HIPCHK(small_.ensure(SMALL_WORDS * 4 * 4));
+    HIPCHK(arena_.init(1 << 16));
     // Initialize(): default transfer function and range (gvdb_volume_gvdb.cpp:2341-2348)
     tf_host.assign(size_t(TF_SIZE) * 4, 0.f);
     const float a[4] = {0, 0, 0, 0}, b[4] = {1, 1, 1, 0.1f};
@@ -351,163 +366,10 @@ int Volume::determine_depth(int root_pos[3]) {
     return -1;
 }
 
-static inline float key_float(uint32_t k) {
-    uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
-    float f;
-    memcpy(&f, &b, 4);
-    return f;
-}
-
 // --------------------------------------------------------- GPU topology ----
 int Volume::rebuild_topology(int n, float radius, const float orig[3]) {
-    if (levels_ == 0) return fail(GVDB_ERR_STATE, "RebuildTopology before Configure");
-    if (n <= 0 || !pts_.base || n > pts_.n) return fail(GVDB_ERR_INVALID, "RebuildTopology: point count exceeds SetPoints");
-    if (!(radius >= 0.0f) || radius > 32.0f) return fail(GVDB_ERR_UNSUPPORTED, "RebuildTopology: radius must be in [0, 32]");
-    stage_begin(ST_TOPO);
-    PointSrc P = pts_;
-    P.n = n;
-    uint32_t* small = small_.as<uint32_t>();
-    HIPCHK(launch_bbox(P, orig, small, stream));
-    HIPCHK(hipMemcpyAsync(pinned_, small, 6 * 4, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    float mn[3], mx[3];
-    for (int a = 0; a < 3; a++) {
-        mn[a] = key_float(pinned_[a]);
-        mx[a] = key_float(pinned_[3 + a]);
-        // GetBoundingBox padding (gvdb_volume_gvdb.cpp:5452-5460)
-        posMin_[a] = mn[a] - 1;
-        posMax_[a] = mx[a] + 2;
-        posMin_[a] = 0 > posMin_[a] ? 0 : posMin_[a];
-        posMax_[a] = 0 > posMax_[a] ? 0 : posMax_[a];
-    }
-    int rootPos[3];
-    int depth = determine_depth(rootPos);
-    if (depth == 0) return fail(GVDB_ERR_DEPTH, "Points extents smaller than 1 brick. Not yet supported.");
-    if (depth < 0 || depth > 5 || depth >= levels_) return fail(GVDB_ERR_DEPTH, "Point extents exceed 5 levels.");
-
-    // The reference rebuilds fully on the first call, after RequestFullRebuild
-    // and on depth changes, and walks an incremental path otherwise
-    // (gvdb_volume_gvdb.cpp:1195-1217); every call here is a full rebuild.
-    clear();
-    curr_depth_ = depth;
-    const int D = depth;
-    const bool dilate = radius >= 1.0f;
-
-    // node-coordinate boxes per level, clipped to the root
-    long long root_end = rootPos[0] + range_[D] - 1;
-    int rr = dilate ? int(ceilf(radius)) : 0;
-    long long lo_v[3], hi_v[3];
-    for (int a = 0; a < 3; a++) {
-        long long re = (long long)rootPos[a] + range_[D] - 1;
-        (void)root_end;
-        long long l = std::isfinite(mn[a]) ? (long long)floorf(std::max(mn[a], 0.0f)) - rr - 1 : 0;
-        long long h = std::isfinite(mx[a]) ? (long long)floorf(std::max(mx[a], 0.0f)) + 2 * rr + 1 : 0;
-        lo_v[a] = std::max<long long>(rootPos[a], l);
-        hi_v[a] = std::min<long long>(re, std::max<long long>(h, lo_v[a]));
-    }
-    if (hi_v[0] / range_[0] > 65535 || hi_v[1] / range_[0] > 65535 || hi_v[2] / range_[0] > 65535)
-        return fail(GVDB_ERR_RANGE, "RebuildTopology: brick coordinates exceed the 16-bit keys of the reference");
-    BitDomain dom[MAXLEV];
-    u64 bm_off[MAXLEV], rk_off[MAXLEV], bm_words = 0, rk_words = 0, tmp_words = 0;
-    for (int L = 0; L < D; L++) {
-        int shift = 0;
-        for (int k = 0; k <= L; k++) shift += logdim_[k];
-        BitDomain& d = dom[L];
-        d.nbits = 1;
-        for (int a = 0; a < 3; a++) {
-            d.lo[a] = int(lo_v[a] >> shift);
-            d.dim[a] = int((hi_v[a] >> shift) - d.lo[a] + 1);
-            d.nbits *= u64(d.dim[a]);
-        }
-        d.nwords = (d.nbits + 31) / 32;
-        bm_off[L] = bm_words;
-        bm_words += d.nwords * (dilate ? 2 : 1);
-        rk_off[L] = rk_words;
-        rk_words += d.nwords * (dilate ? 2 : 1);
-        tmp_words = std::max(tmp_words, scan_tmp_words(d.nwords));
-    }
-    if (dom[0].nbits > (u64(1) << 34))
-        return fail(GVDB_ERR_RANGE, "RebuildTopology: brick box of the points exceeds 2^34 bricks");
-    HIPCHK(bitmaps_.ensure(bm_words * 4 + 256));
-    HIPCHK(ranks_.ensure(rk_words * 4 + 256));
-    HIPCHK(scan_tmp_.ensure(tmp_words * 4 + 256));
-    uint32_t* bm = bitmaps_.as<uint32_t>();
-    uint32_t* rk = ranks_.as<uint32_t>();
-    uint32_t* counts = small + 16;
-    HIPCHK(launch_fill_u32(bm, 0u, bm_words, stream));
-    auto A = [&](int L) { return bm + bm_off[L]; };
-    auto B = [&](int L) { return dilate ? bm + bm_off[L] + dom[L].nwords : nullptr; };
-    // direction masks cost 4 bytes per domain brick (filled, then scanned
-    // once): used while that stays comparable to the point stream itself
-    uint32_t* masks = nullptr;
-    if (dom[0].nbits <= std::max(u64(1) << 22, u64(P.n) * 4)) {
-        HIPCHK(masks_.ensure(size_t(dom[0].nbits) * 4));
-        masks = masks_.as<uint32_t>();
-        HIPCHK(launch_fill_u32(masks, 0u, dom[0].nbits, stream));
-    }
-    HIPCHK(launch_mark_points(P, orig, radius, dilate ? 1 : 0, int(range_[0]), dom[0], A(0), B(0), masks, stream));
-    for (int L = 1; L < D; L++)
-        HIPCHK(launch_mark_parents(dom[L - 1], A(L - 1), B(L - 1), logdim_[L], dom[L], A(L), B(L), stream));
-    for (int L = 0; L < D; L++) {
-        HIPCHK(launch_scan_u32(A(L), nullptr, 1, rk + rk_off[L], dom[L].nwords, counts + 2 * L, scan_tmp_.as<uint32_t>(),
-                               stream));
-        if (dilate)
-            HIPCHK(launch_scan_u32(A(L), B(L), 2, rk + rk_off[L] + dom[L].nwords, dom[L].nwords, counts + 2 * L + 1,
-                                   scan_tmp_.as<uint32_t>(), stream));
-    }
-    HIPCHK(hipMemcpyAsync(pinned_ + 16, counts, 2 * D * 4, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-
-    LevelBuild LB[MAXLEV];
-    for (int L = 0; L < D; L++) {
-        LB[L].dom = dom[L];
-        LB[L].bmA = A(L);
-        LB[L].bmB = dilate ? B(L) : A(L);
-        LB[L].rankA = rk + rk_off[L];
-        LB[L].rankE = dilate ? rk + rk_off[L] + dom[L].nwords : nullptr;
-        LB[L].countA = pinned_[16 + 2 * L];
-        LB[L].countE = dilate ? pinned_[16 + 2 * L + 1] : 0u;
-        LB[L].lev = L;
-        LB[L].range = int(range_[L]);
-        u64 cnt = u64(LB[L].countA) + LB[L].countE;
-        pool_[0][L].lastEle = pool_[0][L].usedNum = cnt;
-        int rc = ensure_pool_capacity(0, L, std::max<u64>(cnt, 1));
-        if (rc) return rc;
-        if (L > 0) {
-            pool_[1][L].lastEle = pool_[1][L].usedNum = cnt;
-            rc = ensure_pool_capacity(1, L, std::max<u64>(cnt, 1));
-            if (rc) return rc;
-        }
-    }
-    pool_[0][D].lastEle = pool_[0][D].usedNum = 1;
-    pool_[1][D].lastEle = pool_[1][D].usedNum = 1;
-    int rc = ensure_pool_capacity(0, D, 1);
-    if (rc) return rc;
-    rc = ensure_pool_capacity(1, D, 1);
-    if (rc) return rc;
-    compute_top_lev();
-
-    // nodes and child lists
-    HIPCHK(launch_write_root(pool_[0][D].dev.as<Node>(), rootPos, D, stream));
-    HIPCHK(launch_fill_u64(pool_[1][D].dev.as<u64>(), ID_UNDEF64, voxcnt(D), stream));
-    for (int L = 0; L < D; L++) {
-        HIPCHK(launch_emit_nodes(LB[L], pool_[0][L].dev.as<Node>(), L > 0 ? 1 : 0, stream));
-        if (L > 0) HIPCHK(launch_fill_u64(pool_[1][L].dev.as<u64>(), ID_UNDEF64, pool_[1][L].lastEle * voxcnt(L), stream));
-    }
-    // gvdbLinkBricks walks from the top node; levels above it stay unlinked
-    for (int L = 0; L < D; L++) {
-        if (L + 1 > top_lev_) continue;
-        bool toRoot = (L + 1 == D);
-        HIPCHK(launch_link_level(LB[L], toRoot ? nullptr : &LB[L + 1], toRoot ? 1 : 0, logdim_[L + 1],
-                                 pool_[0][L].dev.as<Node>(), pool_[1][L + 1].dev.as<u64>(), voxcnt(L + 1), stream));
-    }
-    root_ = Elem(0, D, 0);
-    rebuild_topo_ = false;
-    host_valid_ = false;
-    dev_stale_ = false;
-    topo_gen_++;
-    stage_end(ST_TOPO);
-    return GVDB_OK;
+    Volume* v = this;
+    return multi_rebuild_topology(&v, 1, &n, &radius, orig, self_exec(), nullptr);
 }
 
 // -------------------------------------------------------- host topology ----
@@ -649,39 +511,8 @@ int Volume::activate_space(const float pos[3]) {
 
 // ----------------------------------------------------- finish and atlas ----
 int Volume::finish_topology(bool commit, bool bounds) {
-    // FinishTopology (gvdb_volume_gvdb.cpp:1579-1593) with ComputeBounds (:1792-1816)
-    if (levels_ == 0) return fail(GVDB_ERR_STATE, "FinishTopology before Configure");
-    if (bounds) {
-        Pool& p = pool_[0][0];
-        float mnv[3] = {0, 0, 0}, mxv[3] = {0, 0, 0};
-        if (p.lastEle && dev_stale_) {
-            // host-built pools: bounds from the host mirror
-            Node* c = hnode(Elem(0, 0, 0));
-            for (int a = 0; a < 3; a++) mnv[a] = mxv[a] = float(c->mPos[a]);
-            for (u64 i = 0; i < p.lastEle; i++) {
-                c = hnode(Elem(0, 0, i));
-                if (!c->mFlags) continue;
-                for (int a = 0; a < 3; a++) {
-                    if (c->mPos[a] < mnv[a]) mnv[a] = float(c->mPos[a]);
-                    if (c->mPos[a] + range_[0] > mxv[a]) mxv[a] = float(c->mPos[a] + range_[0]);
-                }
-            }
-        } else if (p.lastEle) {
-            int* out6 = reinterpret_cast<int*>(small_.as<uint32_t>() + 40);
-            HIPCHK(launch_leaf_bounds(make_tree(), int(p.lastEle), out6, stream));
-            HIPCHK(hipMemcpyAsync(pinned_ + 40, out6, 24, hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipStreamSynchronize(stream));
-            const int* r = reinterpret_cast<const int*>(pinned_ + 40);
-            for (int a = 0; a < 3; a++) { mnv[a] = float(r[a]); mxv[a] = float(r[3 + a]); }
-        }
-        for (int a = 0; a < 3; a++) { objMin_[a] = mnv[a]; objMax_[a] = mxv[a]; }
-    }
-    if (commit) {
-        int rc = commit_pools();
-        if (rc) return rc;
-    }
-    compute_top_lev();
-    return GVDB_OK;
+    Volume* v = this;
+    return multi_finish_topology(&v, 1, commit, bounds, self_exec(), nullptr);
 }
 
 int Volume::resize_atlas(int cz) {
@@ -733,201 +564,30 @@ Tree Volume::make_tree() {
 }
 
 int Volume::update_atlas() {
-    // UpdateAtlas (gvdb_volume_gvdb.cpp:2630-2704)
-    if (!chan_[0].exists) return fail(GVDB_ERR_STATE, "UpdateAtlas: no atlas created");
-    int rc = commit_pools();
-    if (rc) return rc;
-    stage_begin(ST_ATLAS);
-    u64 total = pool_[0][0].lastEle, used = pool_[0][0].usedNum;
-    atlas_used_ = 0;
-    if (used > atlas_max_) {
-        int cz = int(ceil(used / float(atlas_cnt_[0] * atlas_cnt_[1])));
-        rc = resize_atlas(cz);
-        if (rc) return rc;
-    }
-    amap_max_ = u64(atlas_cnt_[0]) * atlas_cnt_[1] * atlas_cnt_[2];
-    HIPCHK(amap_.ensure(size_t(amap_max_) * sizeof(AtlasNode)));
-    HIPCHK(launch_fill_u32(amap_.as<uint32_t>(), 0xFFFFFFFFu, amap_max_ * 4, stream));
-    if (total) {
-        HIPCHK(flags_.ensure(size_t(total) * 4));
-        HIPCHK(slot_scan_.ensure(size_t(total) * 4));
-        HIPCHK(scan_tmp_.ensure(scan_tmp_words(total) * 4 + 256));
-        Tree t = make_tree();
-        HIPCHK(launch_leaf_used(t, flags_.as<uint32_t>(), int(total), stream));
-        HIPCHK(launch_scan_u32(flags_.as<uint32_t>(), nullptr, 0, slot_scan_.as<uint32_t>(), total, nullptr,
-                               scan_tmp_.as<uint32_t>(), stream));
-        HIPCHK(launch_update_atlas(t, slot_scan_.as<uint32_t>(), int(total), stream));
-    }
-    atlas_used_ = total;
-    host_valid_ = false;
-    compute_top_lev();
-    atlas_gen_ = topo_gen_;
-    stage_end(ST_ATLAS);
-    return GVDB_OK;
+    Volume* v = this;
+    return multi_update_atlas(&v, 1, self_exec(), nullptr);
 }
 
 int Volume::clear_channel(int c) {
-    // ClearChannel -> AtlasFill (gvdb_allocator.cpp:813-822)
-    if (c < 0 || c >= 8 || !chan_[c].exists) return fail(GVDB_ERR_INVALID, "ClearChannel: no such channel");
-    stage_begin(ST_CLEAR);
-    HIPCHK(launch_fill_u32(chan_[c].data.as<uint32_t>(), 0u, chan_[c].data.bytes / 4, stream));
-    stage_end(ST_CLEAR);
-    return GVDB_OK;
+    Volume* v = this;
+    return multi_clear_channel(&v, 1, c, self_exec(), nullptr);
 }
 
 // ------------------------------------------------------ points -> voxels ----
 int Volume::insert_points_subcell(int scs, int n, float radius, const float trans[3], int* len) {
-    // InsertPointsSubcell (gvdb_volume_gvdb.cpp:4971-5092)
-    u64 numBrick = pool_[0][0].usedNum;
-    if (numBrick == 0) return fail(GVDB_ERR_NOBRICKS, "Warning: InsertPointsSubcell yet no bricks exist.");
-    if (scs <= 0 || res_[0] % scs != 0) return fail(GVDB_ERR_INVALID, "InsertPointsSubcell: subcell size must divide the brick");
-    if (n < 0 || n > pts_.n || (n > 0 && !pts_.base)) return fail(GVDB_ERR_INVALID, "InsertPointsSubcell: point count exceeds SetPoints");
-    if (!(radius >= 0.0f) || radius > 32.0f) return fail(GVDB_ERR_UNSUPPORTED, "InsertPointsSubcell: radius must be in [0, 32]");
-    if (atlas_gen_ != topo_gen_) return fail(GVDB_ERR_STATE, "InsertPointsSubcell: call UpdateAtlas after the topology changes");
-    int rc = commit_pools();
-    if (rc) return rc;
-    stage_begin(ST_INSERT);
-    int sd = res_[0] / scs;
-    int spb = sd * sd * sd;
-    u64 numSC = u64(spb) * numBrick;
-    u64 numMap = pool_[0][0].lastEle;
-    if (numSC > 0xFFFFFFFFull) return fail(GVDB_ERR_RANGE, "InsertPointsSubcell: too many subcells");
-    HIPCHK(sc_flags_.ensure(size_t(numMap) * 4));
-    HIPCHK(sc_map_.ensure(size_t(numMap) * 4));
-    HIPCHK(sc_cnt_.ensure(size_t(numSC) * 4));
-    HIPCHK(sc_use_.ensure(size_t(numSC) * 4));
-    HIPCHK(sc_off_.ensure(size_t(numSC) * 4));
-    HIPCHK(scan_tmp_.ensure(scan_tmp_words(std::max(numSC, numMap)) * 4 + 256));
-    Tree t = make_tree();
-    uint32_t* small = small_.as<uint32_t>();
-    PointSrc P = pts_;
-    P.n = n;
-    HIPCHK(launch_leaf_flags(t, sc_flags_.as<uint32_t>(), int(numMap), stream));
-    HIPCHK(launch_scan_u32(sc_flags_.as<uint32_t>(), nullptr, 0, sc_map_.as<uint32_t>(), numMap, small + 8,
-                           scan_tmp_.as<uint32_t>(), stream));
-    // pull path when the neighbourhood of a brick stays inside its 27 neighbours
-    const int R = int(ceilf((radius + 1.0f) / float(scs)));
-    const bool pull = R <= sd && pull_cells(sd, R) <= pull_max_cells() && spb <= pull_max_spb();
-    uint32_t linked = 0, total = 0, orphans = 0;
-    if (pull) {
-        if (nbr_gen_ != topo_gen_) {
-            HIPCHK(nbr_.ensure(size_t(numMap) * 27 * 4));
-            HIPCHK(launch_neighbors(t, nbr_.as<int>(), int(numMap), stream));
-            nbr_gen_ = topo_gen_;
-        }
-        u64 nb = numMap * u64(spb);
-        HIPCHK(bkt_cnt_.ensure(size_t(nb) * 4));
-        HIPCHK(bkt_off_.ensure(size_t(nb) * 4));
-        HIPCHK(bkt_key_.ensure(size_t(std::max(n, 1)) * 4));
-        HIPCHK(bkt_slot_.ensure(size_t(std::max(n, 1)) * 4));
-        HIPCHK(sorted_.ensure(size_t(std::max(n, 1)) * 12));
-        HIPCHK(scan_tmp_.ensure(scan_tmp_words(std::max(nb, numSC)) * 4 + 256));
-        HIPCHK(launch_fill_u32(bkt_cnt_.as<uint32_t>(), 0u, nb, stream));
-        HIPCHK(launch_fill_u32(small + 10, 0u, 1, stream));
-        HIPCHK(launch_bucket_points(t, P, scs, spb, trans, bkt_key_.as<uint32_t>(), bkt_slot_.as<uint32_t>(),
-                                    bkt_cnt_.as<uint32_t>(), small + 10, stream));
-        HIPCHK(launch_scan_u32(bkt_cnt_.as<uint32_t>(), nullptr, 0, bkt_off_.as<uint32_t>(), nb, small + 11,
-                               scan_tmp_.as<uint32_t>(), stream));
-        HIPCHK(launch_bucket_scatter(P, trans, bkt_key_.as<uint32_t>(), bkt_slot_.as<uint32_t>(),
-                                     bkt_off_.as<uint32_t>(), sorted_.as<float>(), stream));
-        HIPCHK(launch_fill_u32(sc_cnt_.as<uint32_t>(), 0u, numSC, stream));
-        HIPCHK(launch_pull(0, t, int(numMap), scs, R, radius, nbr_.as<int>(), bkt_cnt_.as<uint32_t>(),
-                           bkt_off_.as<uint32_t>(), sorted_.as<float>(), sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
-                           sc_use_.as<uint32_t>(), sc_off_.as<uint32_t>(), nullptr, stream));
-        HIPCHK(launch_scan_u32(sc_cnt_.as<uint32_t>(), nullptr, 0, sc_off_.as<uint32_t>(), numSC, small + 9,
-                               scan_tmp_.as<uint32_t>(), stream));
-        HIPCHK(hipMemcpyAsync(pinned_ + 8, small + 8, 16, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        linked = pinned_[8];
-        total = pinned_[9];
-        orphans = pinned_[10];
-    }
-    const bool use_pull = pull && orphans == 0;
-    if (!use_pull) {
-        // push path: the reference's count pass with global atomics
-        HIPCHK(launch_fill_u32(sc_cnt_.as<uint32_t>(), 0u, numSC, stream));
-        HIPCHK(launch_count_subcell(t, P, scs, spb, trans, radius, sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
-                                    stream));
-        HIPCHK(launch_scan_u32(sc_cnt_.as<uint32_t>(), nullptr, 0, sc_off_.as<uint32_t>(), numSC, small + 9,
-                               scan_tmp_.as<uint32_t>(), stream));
-        HIPCHK(hipMemcpyAsync(pinned_ + 8, small + 8, 8, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        linked = pinned_[8];
-        total = pinned_[9];
-    }
-    if (total > 0x7FFFFFFFu) return fail(GVDB_ERR_RANGE, "InsertPointsSubcell: more than 2^31 subcell entries");
-    *len = int(total);
-    sc_len_ = int(total);
-    num_sc_ = int(numSC);
-    sc_bricks_ = int(linked);
-    sc_map_n_ = int(numMap);
-    sc_gen_ = topo_gen_;
-    last_insert_pull_ = use_pull;
-    if (total == 0) { stage_end(ST_INSERT); return GVDB_OK; }
-    HIPCHK(sc_pnt_.ensure(size_t(total) * 12));
-    HIPCHK(sc_pos_.ensure(size_t(numSC) * 12));
-    HIPCHK(sc_nid_.ensure(size_t(numSC) * 4));
-    if (use_pull) {
-        HIPCHK(launch_pull(1, t, int(numMap), scs, R, radius, nbr_.as<int>(), bkt_cnt_.as<uint32_t>(),
-                           bkt_off_.as<uint32_t>(), sorted_.as<float>(), sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
-                           sc_use_.as<uint32_t>(), sc_off_.as<uint32_t>(), sc_pnt_.as<float>(), stream));
-    } else {
-        HIPCHK(launch_fill_u32(sc_cnt_.as<uint32_t>(), 0u, numSC, stream));
-        HIPCHK(launch_insert_subcell(t, P, scs, spb, trans, radius, sc_map_.as<uint32_t>(), sc_cnt_.as<uint32_t>(),
-                                     sc_off_.as<uint32_t>(), sc_pnt_.as<float>(), stream));
-        // unordered entries: the gather considers all of them
-        HIPCHK(hipMemcpyAsync(sc_use_.p, sc_cnt_.p, size_t(numSC) * 4, hipMemcpyDeviceToDevice, stream));
-    }
-    HIPCHK(launch_subcell_pos(t, scs, spb, sc_map_.as<uint32_t>(), sc_pos_.as<int>(), sc_nid_.as<int>(), int(numMap),
-                              stream));
-    stage_end(ST_INSERT);
-    return GVDB_OK;
+    Volume* v = this;
+    return multi_insert(&v, 1, scs, &n, &radius, trans, len, self_exec(), nullptr);
 }
 
 int Volume::gather_density(int scs, int n, float radius, const float trans[3], int len, int chanD, int chanC, bool accum) {
-    // GatherDensity (gvdb_volume_gvdb.cpp:5318-5344)
     (void)n; (void)trans; (void)chanC;
-    if (len == 0) return GVDB_OK;
-    if (pool_[0][0].usedNum == 0) return GVDB_OK;
-    if (chanD < 0 || chanD >= 8 || !chan_[chanD].exists) return fail(GVDB_ERR_INVALID, "GatherDensity: no such channel");
-    if (len != sc_len_ || scs <= 0 || res_[0] % scs) return fail(GVDB_ERR_STATE, "GatherDensity: call InsertPointsSubcell first");
-    if (atlas_gen_ != topo_gen_ || sc_gen_ != topo_gen_) return fail(GVDB_ERR_STATE, "GatherDensity: topology changed since UpdateAtlas/InsertPointsSubcell");
-    int rc = commit_pools();
-    if (rc) return rc;
-    stage_begin(ST_GATHER);
-    Tree t = make_tree();
-    (void)hipEventRecord(kev_[0][0], stream);
-    HIPCHK(launch_gather_density(t, scs, sc_bricks_, radius, sc_nid_.as<int>(), sc_cnt_.as<uint32_t>(),
-                                 sc_use_.as<uint32_t>(), sc_off_.as<uint32_t>(), sc_pos_.as<int>(), sc_pnt_.as<float>(),
-                                 chan_[chanD].data.p, chan_[chanD].type, accum ? 1 : 0, stream));
-    (void)hipEventRecord(kev_[0][1], stream);
-    kev_used_[0] = true;
-    stage_end(ST_GATHER);
-    return GVDB_OK;
+    Volume* v = this;
+    return multi_gather(&v, 1, scs, &len, &radius, chanD, accum, self_exec(), nullptr);
 }
 
 int Volume::update_apron(int c, float boundval) {
-    // UpdateApron (gvdb_volume_gvdb.cpp:4418-4457)
-    if (apron_ == 0) return GVDB_OK;
-    if (c < 0 || c >= 8 || !chan_[c].exists) return fail(GVDB_ERR_INVALID, "UpdateApron: no such channel");
-    int rc = commit_pools();
-    if (rc) return rc;
-    int bricks = int(pool_[0][0].lastEle);
-    if (bricks == 0) return GVDB_OK;
-    if (atlas_gen_ != topo_gen_) return fail(GVDB_ERR_STATE, "UpdateApron: call UpdateAtlas after the topology changes");
-    stage_begin(ST_APRON);
-    Tree t = make_tree();
-    if (nbr_gen_ != topo_gen_) {
-        HIPCHK(nbr_.ensure(size_t(bricks) * 27 * 4));
-        HIPCHK(launch_neighbors(t, nbr_.as<int>(), bricks, stream));
-        nbr_gen_ = topo_gen_;
-    }
-    (void)hipEventRecord(kev_[1][0], stream);
-    HIPCHK(launch_update_apron(t, nbr_.as<int>(), bricks, chan_[c].data.p, chan_[c].type, boundval, stream));
-    (void)hipEventRecord(kev_[1][1], stream);
-    kev_used_[1] = true;
-    stage_end(ST_APRON);
-    return GVDB_OK;
+    Volume* v = this;
+    return multi_apron(&v, 1, c, boundval, self_exec(), nullptr);
 }
 
 // ---------------------------------------------------------------- render ----
@@ -962,30 +622,8 @@ int Volume::add_render_buf(int buf, int w, int h, int bpp) {
 }
 
 int Volume::render(int shading, int c, int buf) {
-    // Render (gvdb_volume_gvdb.cpp:4336-4381)
-    if (buf < 0 || buf >= 4 || rbuf_[buf].w == 0) return fail(GVDB_ERR_INVALID, "Render: no such render buffer");
-    if (c < 0 || c >= 8 || !chan_[c].exists) return fail(GVDB_ERR_INVALID, "Render: no such channel");
-    if (shading != SHADE_VOLUME && shading != SHADE_TRILINEAR) return fail(GVDB_ERR_UNSUPPORTED, "Render: shading must be SHADE_VOLUME or SHADE_TRILINEAR");
-    if (pool_[0][0].lastEle == 0 || amap_max_ == 0) return fail(GVDB_ERR_STATE, "Render: no topology or atlas");
-    if (atlas_gen_ != topo_gen_) return fail(GVDB_ERR_STATE, "Render: call UpdateAtlas after the topology changes");
-    int rc = commit_pools();
-    if (rc) return rc;
-    if (!tf_committed) {
-        rc = commit_tf();
-        if (rc) return rc;
-    }
-    RenderBuf& r = rbuf_[buf];
-    ScnBlock s;
-    scene_block(r.w, r.h, s);
-    stage_begin(ST_RENDER);
-    Tree t = make_tree();
-    (void)hipEventRecord(kev_[2][0], stream);
-    HIPCHK(launch_render(t, s, tf_dev.as<float4>(), chan_[c].data.p, chan_[c].type, shading, r.data.p, r.bpp == 16,
-                         stream));
-    (void)hipEventRecord(kev_[2][1], stream);
-    kev_used_[2] = true;
-    stage_end(ST_RENDER);
-    return GVDB_OK;
+    Volume* v = this;
+    return multi_render(&v, 1, &shading, &c, &buf, self_exec(), nullptr);
 }
 
 int Volume::read_render_buf(int buf, void* dst) {

@@ -48,6 +48,17 @@ struct CameraParams {
 
 enum Stage { ST_TOPO = 0, ST_ATLAS, ST_INSERT, ST_GATHER, ST_APRON, ST_RENDER, ST_CLEAR, ST_COUNT };
 
+// Where the stages of one call run: the stream, argument staging, and a
+// SMALL_WORDS-word slot per grid (device and pinned host) for the few values
+// a stage reads back (bounding box, node counts, subcell totals).
+static const int SMALL_WORDS = 64;
+struct Exec {
+    hipStream_t s;
+    Arena* ar;
+    uint32_t* dsmall;
+    uint32_t* hsmall;
+};
+
 class Volume {
 public:
     explicit Volume(int device);
@@ -110,6 +121,24 @@ public:
 
     hipStream_t stream = nullptr;
     int device = 0;
+    Exec self_exec();
+    int use_stream(hipStream_t s);
+
+    // Stages over G grids at once (pipeline.cpp); the single-grid methods
+    // above call these with G = 1.  On failure `failed` names the grid.
+    static int multi_rebuild_topology(Volume* const* v, int G, const int* n, const float* radius, const float* orig3,
+                                      const Exec& ex, int* failed);
+    static int multi_finish_topology(Volume* const* v, int G, bool commit, bool bounds, const Exec& ex, int* failed);
+    static int multi_update_atlas(Volume* const* v, int G, const Exec& ex, int* failed);
+    static int multi_clear_channel(Volume* const* v, int G, int chan, const Exec& ex, int* failed);
+    static int multi_insert(Volume* const* v, int G, int scs, const int* n, const float* radius, const float* trans3,
+                            int* len, const Exec& ex, int* failed);
+    static int multi_gather(Volume* const* v, int G, int scs, const int* len, const float* radius, int chanD, bool accum,
+                            const Exec& ex, int* failed);
+    static int multi_apron(Volume* const* v, int G, int chan, float boundval, const Exec& ex, int* failed);
+    static int multi_render(Volume* const* v, int G, const int* shading, const int* chan, const int* buf, const Exec& ex,
+                            int* failed);
+    int hip_fail(hipError_t e, const char* what);
 
 private:
     // pools
@@ -156,7 +185,9 @@ private:
     // topology scratch
     DevBuf small_;       // bbox keys, counts
     DevBuf bitmaps_, ranks_, masks_, scan_tmp_, flags_, slot_scan_;
-    uint32_t* pinned_ = nullptr;   // host staging (64 words)
+    uint32_t* pinned_ = nullptr;   // host staging (256 words)
+    Arena arena_;
+    hipStream_t own_stream_ = nullptr;
 
     // render
     RenderBuf rbuf_[4];
@@ -167,7 +198,6 @@ private:
     hipEvent_t kev_[4][2] = {};
     bool kev_used_[4] = {};
 
-    int hip_fail(hipError_t e, const char* what);
     void stage_begin(Stage s);
     void stage_end(Stage s);
     u64 pool_alloc(int grp, int lev);

@@ -52,6 +52,17 @@ _SIGS = [
     ("gvdb_read_atlas_map", _i, [_vp, _vp]), ("gvdb_read_subcells", _i, [_vp, _i, _vp, _u64p]),
     ("gvdb_get_stream", _i, [_vp, ctypes.POINTER(_vp)]), ("gvdb_synchronize", _i, [_vp]),
     ("gvdb_stage_ms", _i, [_vp, _vp]), ("gvdb_kernel_ms", _i, [_vp, _vp]),
+    ("gvdb_batch_create", _i, [ctypes.POINTER(_vp), _i, _i]), ("gvdb_batch_destroy", _i, [_vp]),
+    ("gvdb_batch_last_error", ctypes.c_char_p, [_vp]),
+    ("gvdb_batch_rebuild_topology", _i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    ("gvdb_batch_finish_topology", _i, [_vp, _i, _vp, _i, _i, _vp]),
+    ("gvdb_batch_update_atlas", _i, [_vp, _i, _vp, _vp]), ("gvdb_batch_clear_channel", _i, [_vp, _i, _vp, _i, _vp]),
+    ("gvdb_batch_insert_points_subcell", _i, [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    ("gvdb_batch_gather_density", _i, [_vp, _i, _vp, _i, _vp, _vp, _i, _i, _vp]),
+    ("gvdb_batch_update_apron", _i, [_vp, _i, _vp, _i, _f, _vp]),
+    ("gvdb_batch_render", _i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    ("gvdb_batch_synchronize", _i, [_vp]), ("gvdb_batch_elapsed_ms", _i, [_vp, _vp]),
+    ("gvdb_release_stream", _i, [_vp]),
 ]
 
 _lib = None
@@ -278,3 +289,109 @@ class VolumeGVDB:
         o = np.zeros(4, np.float32)
         self._chk(load().gvdb_kernel_ms(self.h, _p(o)))
         return dict(gather=float(o[0]), apron=float(o[1]), render=float(o[2]))
+
+
+class GridBatch:
+    """Many grids per launch: each method runs one stage of the point path
+    over a list of VolumeGVDB objects (any sizes) with one kernel launch per
+    kernel of the stage.  Per-grid arguments are sequences, or one value for
+    all."""
+
+    def __init__(self, device=0, max_grids=256):
+        L = load()
+        h = ctypes.c_void_p()
+        rc = L.gvdb_batch_create(ctypes.byref(h), int(device), int(max_grids))
+        if rc != OK:
+            raise GVDBError(rc, "gvdb_batch_create failed (no HIP device %d?)" % device)
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().gvdb_batch_destroy(self.h)
+            self.h = None
+
+    def close(self):
+        self.__del__()
+
+    def _chk(self, rc, vols=None, failed=None):
+        if rc != OK:
+            who = int(failed.value) if failed is not None else -1
+            raise GVDBError(rc, "batch (grid %d): %s" % (who, load().gvdb_batch_last_error(self.h).decode()))
+        return rc
+
+    @staticmethod
+    def _hs(vols):
+        return (ctypes.c_void_p * len(vols))(*[v.h.value for v in vols])
+
+    @staticmethod
+    def _arr(x, G, dt):
+        a = np.asarray(x, dt)
+        if a.ndim == 0:
+            a = np.full(G, a, dt)
+        return np.ascontiguousarray(a.reshape(G))
+
+    @staticmethod
+    def _arr3(x, G):
+        if x is None:
+            return np.zeros((G, 3), np.float32)
+        a = np.asarray(x, np.float32)
+        if a.shape == (3,):
+            a = np.tile(a, (G, 1))
+        return np.ascontiguousarray(a.reshape(G, 3))
+
+    def RebuildTopology(self, vols, n, radius, orig=None):
+        G = len(vols)
+        n, r, o = self._arr(n, G, np.int32), self._arr(radius, G, np.float32), self._arr3(orig, G)
+        f = ctypes.c_int(-1)
+        return self._chk(load().gvdb_batch_rebuild_topology(self.h, G, self._hs(vols), _p(n), _p(r), _p(o),
+                                                            ctypes.byref(f)), vols, f)
+
+    def FinishTopology(self, vols, commit=False, bounds=True):
+        f = ctypes.c_int(-1)
+        return self._chk(load().gvdb_batch_finish_topology(self.h, len(vols), self._hs(vols), int(commit), int(bounds),
+                                                           ctypes.byref(f)), vols, f)
+
+    def UpdateAtlas(self, vols):
+        f = ctypes.c_int(-1)
+        return self._chk(load().gvdb_batch_update_atlas(self.h, len(vols), self._hs(vols), ctypes.byref(f)), vols, f)
+
+    def ClearChannel(self, vols, chan=0):
+        f = ctypes.c_int(-1)
+        return self._chk(load().gvdb_batch_clear_channel(self.h, len(vols), self._hs(vols), chan, ctypes.byref(f)),
+                         vols, f)
+
+    def InsertPointsSubcell(self, vols, subcell_size, n, radius, trans=None):
+        G = len(vols)
+        n, r, t = self._arr(n, G, np.int32), self._arr(radius, G, np.float32), self._arr3(trans, G)
+        out = np.zeros(G, np.int32)
+        f = ctypes.c_int(-1)
+        self._chk(load().gvdb_batch_insert_points_subcell(self.h, G, self._hs(vols), subcell_size, _p(n), _p(r), _p(t),
+                                                          _p(out), ctypes.byref(f)), vols, f)
+        return [int(x) for x in out]
+
+    def GatherDensity(self, vols, subcell_size, sc_len, radius, chan_density=0, accumulate=False):
+        G = len(vols)
+        ln, r = self._arr(sc_len, G, np.int32), self._arr(radius, G, np.float32)
+        f = ctypes.c_int(-1)
+        return self._chk(load().gvdb_batch_gather_density(self.h, G, self._hs(vols), subcell_size, _p(ln), _p(r),
+                                                          chan_density, int(accumulate), ctypes.byref(f)), vols, f)
+
+    def UpdateApron(self, vols, chan=0, boundval=0.0):
+        f = ctypes.c_int(-1)
+        return self._chk(load().gvdb_batch_update_apron(self.h, len(vols), self._hs(vols), chan, float(boundval),
+                                                        ctypes.byref(f)), vols, f)
+
+    def Render(self, vols, shading, chan=0, buf=0):
+        G = len(vols)
+        sh, c, b = self._arr(shading, G, np.int32), self._arr(chan, G, np.int32), self._arr(buf, G, np.int32)
+        f = ctypes.c_int(-1)
+        return self._chk(load().gvdb_batch_render(self.h, G, self._hs(vols), _p(sh), _p(c), _p(b), ctypes.byref(f)),
+                         vols, f)
+
+    def synchronize(self):
+        return self._chk(load().gvdb_batch_synchronize(self.h))
+
+    def elapsed_ms(self):
+        ms = np.zeros(1, np.float32)
+        self._chk(load().gvdb_batch_elapsed_ms(self.h, _p(ms)))
+        return float(ms[0])

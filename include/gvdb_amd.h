@@ -157,6 +157,46 @@ int gvdb_stage_ms(gvdb_handle h, float* out8);
 /* Kernel-only time of the most recent gather / apron / render kernel (ms). */
 int gvdb_kernel_ms(gvdb_handle h, float* out4);
 
+/* ---------------------------------------------------------------- batches
+ * The batched form (no counterpart in the reference, which runs one grid per
+ * call): each call runs one stage of the point path over `count` volumes --
+ * grids of any size, each keeping its own pools and atlas -- with one kernel
+ * launch per kernel of the stage and one host round trip per stage for the
+ * whole set.  Per-grid arguments are arrays of `count` entries (orig3/trans3:
+ * 3 floats per grid, may be NULL for zero).  On failure *failed (may be NULL)
+ * names the grid at fault; its gvdb_last_error and gvdb_batch_last_error
+ * hold the message.  Volumes run on the batch's stream from their first
+ * batched call on; gvdb_release_stream returns one to its own stream. */
+typedef struct gvdb_batch_s* gvdb_batch;
+int gvdb_batch_create(gvdb_batch* out, int device, int max_grids);
+int gvdb_batch_destroy(gvdb_batch b);
+const char* gvdb_batch_last_error(gvdb_batch b);
+/* RebuildTopology over many grids (gvdb_volume_gvdb.cpp:1181-1218 per grid) */
+int gvdb_batch_rebuild_topology(gvdb_batch b, int count, const gvdb_handle* hs, const int* n, const float* radius,
+                                const float* orig3, int* failed);
+/* FinishTopology (gvdb_volume_gvdb.cpp:1579-1593 per grid) */
+int gvdb_batch_finish_topology(gvdb_batch b, int count, const gvdb_handle* hs, int commit_pool, int compute_bounds,
+                               int* failed);
+/* UpdateAtlas (gvdb_volume_gvdb.cpp:2630-2704 per grid) */
+int gvdb_batch_update_atlas(gvdb_batch b, int count, const gvdb_handle* hs, int* failed);
+/* ClearChannel (gvdb_volume_gvdb.cpp:1596-1606 per grid) */
+int gvdb_batch_clear_channel(gvdb_batch b, int count, const gvdb_handle* hs, int chan, int* failed);
+/* InsertPointsSubcell (gvdb_volume_gvdb.cpp:4971-5092 per grid); out_len: count entries */
+int gvdb_batch_insert_points_subcell(gvdb_batch b, int count, const gvdb_handle* hs, int subcell_size, const int* n,
+                                     const float* radius, const float* trans3, int* out_len, int* failed);
+/* GatherDensity (gvdb_volume_gvdb.cpp:5318-5344 per grid) */
+int gvdb_batch_gather_density(gvdb_batch b, int count, const gvdb_handle* hs, int subcell_size, const int* sc_len,
+                              const float* radius, int chan_density, int accumulate, int* failed);
+/* UpdateApron (gvdb_volume_gvdb.cpp:4418-4457 per grid) */
+int gvdb_batch_update_apron(gvdb_batch b, int count, const gvdb_handle* hs, int chan, float boundval, int* failed);
+/* Render (gvdb_volume_gvdb.cpp:4336-4381 per grid), each into its own buffer */
+int gvdb_batch_render(gvdb_batch b, int count, const gvdb_handle* hs, const int* shading, const int* chan,
+                      const int* buf, int* failed);
+int gvdb_batch_synchronize(gvdb_batch b);
+/* Device time from the first stage call after the last synchronize/elapsed to the last one (ms); synchronises. */
+int gvdb_batch_elapsed_ms(gvdb_batch b, float* ms);
+int gvdb_release_stream(gvdb_handle h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,151 @@
+"""The batched form: one launch per kernel over many grids of different sizes
+and shapes must give every grid exactly what the one-grid calls give it
+(pools, subcell tables, atlas and images bit for bit), and each grid must
+still match the oracle."""
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _grids():
+    from gvdb_amd import workloads as W
+    g = np.random.default_rng(77)
+    return [
+        ("shell", W.sphere_shell(6000, seed=61), 2.0, 0),
+        ("slab", W.slab_plane(15000, 1000, seed=62, lo=(40.0, 30.0, 40.0), hi=(160.0, 90.0, 150.0), plane_y=20.0), 2.0, 0),
+        ("cloud", W.noise_cloud(5000, seed=63, box=256.0), 1.5, 0),
+        ("tiny", (g.random((150, 3)) * 30 + 80).astype(np.float32), 2.0, 0),
+        ("thin", W.sphere_shell(3000, seed=64, radius=60.0), 0.5, 0),
+        ("orphans", W.sphere_shell(4000, seed=65), 2.0, 2000),   # topology from the first 2000 points
+    ]
+
+
+def _scene(k, w, h):
+    from gvdb_amd import workloads as W
+    return W.ea_scene(w, h) if k % 2 == 0 else W.iso_scene(w, h, centre=(256.0, 256.0, 256.0), dist=600.0)
+
+
+def _single(gpu, pts, r, n_topo, k, size):
+    v = gpu.VolumeGVDB(0)
+    v.Configure(*H.TREE)
+    v.SetChannelDefault(16, 16, 1)
+    v.AddChannel(0, gpu.T_FLOAT, 1)
+    v.SetPoints(pts)
+    v.RequestFullRebuild(True)
+    v.RebuildTopology(n_topo or len(pts), r, H.ZERO3)
+    v.FinishTopology(False, True)
+    v.UpdateAtlas()
+    v.ClearChannel(0)
+    n = v.InsertPointsSubcell(4, len(pts), r, H.ZERO3)
+    v.GatherDensity(4, len(pts), r, H.ZERO3, n, 0, 0, False)
+    v.UpdateApron(0, 0.0)
+    sc = _scene(k, *size)
+    v.apply_scene(sc)
+    v.AddRenderBuf(0, size[0], size[1], 16)
+    v.Render(sc["shading"], 0, 0)
+    img = v.ReadRenderBuf(0, size[0], size[1], 16)
+    return v, n, img
+
+
+SIZES = [(64, 48), (40, 40), (96, 32), (16, 16), (72, 72), (48, 64)]
+
+
+def test_batch_matches_single_grid_calls(gpu):
+    grids = _grids()
+    B = gpu.GridBatch(0, 16)
+    vols = []
+    for name, pts, r, n_topo in grids:
+        v = gpu.VolumeGVDB(0)
+        v.Configure(*H.TREE)
+        v.SetChannelDefault(16, 16, 1)
+        v.AddChannel(0, gpu.T_FLOAT, 1)
+        v.SetPoints(pts)
+        vols.append(v)
+    n_all = [len(p) for _, p, _, _ in grids]
+    n_topo = [nt or len(p) for _, p, _, nt in grids]
+    radii = [r for _, _, r, _ in grids]
+    B.RebuildTopology(vols, n_topo, radii)
+    B.FinishTopology(vols, False, True)
+    B.UpdateAtlas(vols)
+    B.ClearChannel(vols, 0)
+    lens = B.InsertPointsSubcell(vols, 4, n_all, radii)
+    B.GatherDensity(vols, 4, lens, radii, 0)
+    B.UpdateApron(vols, 0, 0.0)
+    shading = []
+    for k, v in enumerate(vols):
+        sc = _scene(k, *SIZES[k])
+        v.apply_scene(sc)
+        v.AddRenderBuf(0, SIZES[k][0], SIZES[k][1], 16)
+        shading.append(sc["shading"])
+    B.Render(vols, shading, 0, 0)
+    B.synchronize()
+    assert B.elapsed_ms() > 0
+    for k, (name, pts, r, nt) in enumerate(grids):
+        s, n, img = _single(gpu, pts, r, nt, k, SIZES[k])
+        v = vols[k]
+        assert lens[k] == n, name
+        H.assert_pools_equal(v, s)
+        gs, ss = v.subcells(), s.subcells()
+        for key in ("cnt", "off", "pos", "nid"):
+            assert np.array_equal(gs[key], ss[key]), (name, key)
+        la = H.subcell_lists(gs["cnt"], gs["off"], gs["pnt"])
+        lb = H.subcell_lists(ss["cnt"], ss["off"], ss["pnt"])
+        assert all(np.array_equal(x, y) for x, y in zip(la, lb)), name
+        bimg = v.ReadRenderBuf(0, SIZES[k][0], SIZES[k][1], 16)
+        if name == "orphans":
+            # push-path entries arrive in atomic order: equal only up to ties
+            _, rel, zeros = H.density_errors(v.atlas(), s.atlas().astype(np.float64))
+            assert zeros == 0 and rel <= 1e-6, name
+            assert H.rmse(bimg, img) <= 1e-6, name
+        else:
+            assert np.array_equal(v.atlas(), s.atlas()), name
+            assert np.array_equal(bimg, img), name
+        s.close()
+
+
+def test_batch_grids_match_oracle(gpu):
+    import oracle as O
+    grids = _grids()[:4]
+    B = gpu.GridBatch(0, 8)
+    vols = []
+    for name, pts, r, _ in grids:
+        v = gpu.VolumeGVDB(0)
+        v.Configure(*H.TREE)
+        v.SetChannelDefault(16, 16, 1)
+        v.AddChannel(0, gpu.T_FLOAT, 1)
+        v.SetPoints(pts)
+        vols.append(v)
+    n_all = [len(p) for _, p, _, _ in grids]
+    radii = [r for _, _, r, _ in grids]
+    B.RebuildTopology(vols, n_all, radii)
+    B.FinishTopology(vols)
+    B.UpdateAtlas(vols)
+    B.ClearChannel(vols)
+    lens = B.InsertPointsSubcell(vols, 4, n_all, radii)
+    B.GatherDensity(vols, 4, lens, radii)
+    B.UpdateApron(vols)
+    B.synchronize()
+    for k, (name, pts, r, _) in enumerate(grids):
+        o, m = H.oracle_volume(O, pts, r, precision=64)
+        assert lens[k] == m, name
+        H.assert_pools_equal(vols[k], o)
+        a, b = vols[k].atlas(), o.atlas(64)
+        mask = H.used_brick_mask(b.shape, o.atlas_map(), o.atlas_info()["cnt"])
+        _, rel, zeros = H.density_errors(a[mask], b[mask])
+        assert zeros == 0 and rel <= 1e-5, (name, rel)
+
+
+def test_batch_reports_failing_grid(gpu):
+    """A stage error names the grid at fault and runs nothing."""
+    B = gpu.GridBatch(0, 4)
+    good = gpu.VolumeGVDB(0)
+    good.Configure(*H.TREE)
+    good.AddChannel(0, gpu.T_FLOAT, 1)
+    good.SetPoints(np.random.default_rng(3).random((500, 3)).astype(np.float32) * 50 + 100)
+    bad = gpu.VolumeGVDB(0)          # not configured
+    with pytest.raises(gpu.GVDBError) as e:
+        B.RebuildTopology([good, bad], [500, 10], 2.0)
+    assert "grid 1" in str(e.value)
