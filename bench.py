@@ -103,6 +103,101 @@ def run_gpu(args):
     return out
 
 
+def _grid_scene(pts, w):
+    """Emission-absorption view of one grid: camera on the diagonal of its
+    bounding box at 2.2x the box size."""
+    from gvdb_amd import workloads as W
+    lo, hi = pts.min(0).astype(np.float64), pts.max(0).astype(np.float64)
+    c = tuple((lo + hi) / 2)
+    return W.ea_scene(w, w, centre=c, dist=float(2.2 * np.max(hi - lo)))
+
+
+def run_batched(args):
+    """The batched workload: 64 grids (point counts log-uniform in [10k, 2M],
+    shell / slab / value-noise cloud), each splatted (topology, atlas, insert,
+    gather, apron) and rendered at 512x512 (emission-absorption).  One request
+    = one grid's splat + render.  Plain form: the grids one after another
+    through the one-grid API.  Batched form: every stage once for all 64 grids
+    (one launch per kernel).  Latency = completion time of a request, all 64
+    submitted at once."""
+    import gvdb_amd as G
+    from gvdb_amd import workloads as W
+    grids = W.batch_grids()
+    w = args.batch_width
+    vols = []
+    for kind, pts in grids:
+        v = G.VolumeGVDB(0)
+        v.Configure(5, 4, 3, 3, 3)
+        v.SetChannelDefault(16, 16, 1)
+        v.AddChannel(0, G.T_FLOAT, 1)
+        v.SetPoints(pts)
+        v.apply_scene(_grid_scene(pts, w))
+        v.AddRenderBuf(0, w, w, 4)
+        vols.append(v)
+    ns = [len(p) for _, p in grids]
+    r, z = 2.0, (0.0, 0.0, 0.0)
+
+    def plain():
+        t0 = time.perf_counter()
+        done = []
+        for v, n in zip(vols, ns):
+            v.RequestFullRebuild(True)
+            v.RebuildTopology(n, r, z)
+            v.FinishTopology(False, True)
+            v.UpdateAtlas()
+            v.ClearChannel(0)
+            ln = v.InsertPointsSubcell(4, n, r, z)
+            v.GatherDensity(4, n, r, z, ln, 0, 0, False)
+            v.UpdateApron(0, 0.0)
+            v.Render(G.SHADE_VOLUME, 0, 0)
+            v.synchronize()
+            done.append(time.perf_counter() - t0)
+        return done
+
+    B = G.GridBatch(0, len(vols))
+
+    def batched():
+        t0 = time.perf_counter()
+        B.RebuildTopology(vols, ns, r)
+        B.FinishTopology(vols, False, True)
+        B.UpdateAtlas(vols)
+        B.ClearChannel(vols, 0)
+        lens = B.InsertPointsSubcell(vols, 4, ns, r)
+        B.GatherDensity(vols, 4, lens, r, 0)
+        B.UpdateApron(vols, 0, 0.0)
+        B.Render(vols, G.SHADE_VOLUME, 0, 0)
+        B.synchronize()
+        t = time.perf_counter() - t0
+        return [t] * len(vols)
+
+    for _ in range(max(1, args.warmup)):
+        plain()
+        batched()
+    reps = max(1, min(args.steps, 5))
+    lp, lb = [], []
+    for _ in range(reps):
+        lp.append(plain())
+        lb.append(batched())
+    # the same pixels either way
+    img_b = [v.ReadRenderBuf(0, w, w, 4) for v in vols[:4]]
+    plain()
+    same = all(np.array_equal(v.ReadRenderBuf(0, w, w, 4), ib) for v, ib in zip(vols[:4], img_b))
+    tp = float(np.median([max(x) for x in lp]))
+    tb = float(np.median([max(x) for x in lb]))
+    allp, allb = np.concatenate(lp), np.concatenate(lb)
+    for v in vols:
+        v.close()
+    B.close()
+    return dict(workload="64 grids, points log-uniform in [10k, 2M] (%d total), shell/slab/noise cloud, radius 2, "
+                         "subcell 4, %dx%d emission-absorption render each" % (sum(ns), w, w),
+                grids=len(ns), points=int(sum(ns)), reps=reps,
+                plain=dict(ms=1e3 * tp, requests_per_s=len(ns) / tp, p50_ms=1e3 * float(np.percentile(allp, 50)),
+                           p99_ms=1e3 * float(np.percentile(allp, 99))),
+                batched=dict(ms=1e3 * tb, requests_per_s=len(ns) / tb, p50_ms=1e3 * float(np.percentile(allb, 50)),
+                             p99_ms=1e3 * float(np.percentile(allb, 99))),
+                speedup=tp / tb, same_images=bool(same))
+
+
 def run_cpu_baseline(sample_stride=4, render_w=512):
     """The oracle's host path (host ActivateSpace topology, UpdateAtlas, subcell
     insert, FP32 gather, apron; FP32 march) on a bounded sample: every
@@ -160,6 +255,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--width", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-grid batched comparison")
+    ap.add_argument("--batch-width", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -191,6 +288,8 @@ def main():
                             frac=apron_gbs / HBM_PEAK_GBS, algorithmic_bytes=ab["apron"]),
         batched=None,
     )
+    if not args.no_batched:
+        line["batched"] = run_batched(args)
     if not args.no_cpu:
         c = run_cpu_baseline()
         line["cpu_baseline"] = dict(value=c["points"] / c["t_splat"], unit="points/s", cores=c["threads"], kind="port",

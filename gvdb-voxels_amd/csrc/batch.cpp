@@ -14,13 +14,16 @@ Batch::~Batch() {
     if (hsmall_) (void)hipHostFree(hsmall_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
-    if (stream) (void)hipStreamDestroy(stream);
+    shared.reset();   // the stream lives on while grids still run on it
 }
 
 int Batch::init() {
     if (max_grids < 1) { err = "batch: max_grids must be positive"; return GVDB_ERR_INVALID; }
     hipError_t e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        shared = SharedStream::make(device, e);
+        stream = shared->s;
+    }
     if (e == hipSuccess) e = arena_.init(size_t(4) << 20);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&dsmall_), size_t(max_grids) * SMALL_WORDS * 4);
     if (e == hipSuccess)
@@ -54,7 +57,7 @@ int Batch::enter(Volume* const* v, int G, int* failed) {
                 return GVDB_ERR_INVALID;
             }
         if (v[g]->stream != stream) {
-            int rc = v[g]->use_stream(stream);
+            int rc = v[g]->use_stream(shared);
             if (rc) {
                 if (failed) *failed = g;
                 err = v[g]->err;

@@ -21,6 +21,7 @@ public:
     int device;
     int max_grids;
     hipStream_t stream = nullptr;
+    std::shared_ptr<SharedStream> shared;
 
     int rebuild_topology(Volume* const* v, int G, const int* n, const float* radius, const float* orig3, int* failed);
     int finish_topology(Volume* const* v, int G, bool commit, bool bounds, int* failed);

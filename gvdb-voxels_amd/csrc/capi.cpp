@@ -272,7 +272,7 @@ int gvdb_batch_elapsed_ms(gvdb_batch b, float* ms) {
 }
 int gvdb_release_stream(gvdb_handle h) {
     H_OR_FAIL(h);
-    return h->v.use_stream(nullptr);
+    return h->v.use_stream(std::shared_ptr<gvdb::SharedStream>());
 }
 
 }  // extern "C"

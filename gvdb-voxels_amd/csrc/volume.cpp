@@ -56,10 +56,28 @@ Exec Volume::self_exec() {
     return Exec{stream, &arena_, small_.as<uint32_t>(), pinned_};
 }
 
-int Volume::use_stream(hipStream_t s) {
-    // joins (or leaves, s = null) a shared stream; work queued so far drains first
-    HIPCHK(hipStreamSynchronize(stream));
-    stream = s ? s : own_stream_;
+SharedStream::~SharedStream() {
+    if (!s) return;
+    (void)hipSetDevice(device);
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+}
+
+std::shared_ptr<SharedStream> SharedStream::make(int device, hipError_t& e) {
+    auto p = std::make_shared<SharedStream>();
+    p->device = device;
+    e = hipStreamCreateWithFlags(&p->s, hipStreamNonBlocking);
+    if (e != hipSuccess) p->s = nullptr;
+    return p;
+}
+
+int Volume::use_stream(const std::shared_ptr<SharedStream>& s) {
+    const std::shared_ptr<SharedStream>& to = s ? s : own_stream_;
+    if (to->s == stream) return GVDB_OK;
+    HIPCHK(hipEventRecord(handoff_, stream));
+    HIPCHK(hipStreamWaitEvent(to->s, handoff_, 0));
+    cur_stream_ = to;
+    stream = to->s;
     return GVDB_OK;
 }
 
@@ -73,7 +91,6 @@ Volume::Volume(int dev) : device(dev) {}
 Volume::~Volume() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
-    if (own_stream_ && own_stream_ != stream) (void)hipStreamSynchronize(own_stream_);
     for (int g = 0; g < 2; g++)
         for (int l = 0; l < MAXLEV; l++) pool_[g][l].dev.release();
     for (auto& c : chan_) c.data.release();
@@ -89,13 +106,20 @@ Volume::~Volume() {
             if (kev_[s][k]) (void)hipEventDestroy(kev_[s][k]);
     if (pinned_) (void)hipHostFree(pinned_);
     arena_.release();
-    if (own_stream_) (void)hipStreamDestroy(own_stream_);
+    if (handoff_) (void)hipEventDestroy(handoff_);
+    cur_stream_.reset();
+    own_stream_.reset();
 }
 
 int Volume::init() {
     HIPCHK(hipSetDevice(device));
-    HIPCHK(hipStreamCreateWithFlags(&own_stream_, hipStreamNonBlocking));
-    stream = own_stream_;
+    {
+        hipError_t e = hipSuccess;
+        own_stream_ = SharedStream::make(device, e);
+        HIPCHK(e);
+    }
+    cur_stream_ = own_stream_;
+    stream = own_stream_->s;
     for (int s = 0; s < ST_COUNT; s++)
         for (int k = 0; k < 2; k++) HIPCHK(hipEventCreate(&ev_[s][k]));
     for (int s = 0; s < 4; s++)
@@ -103,6 +127,7 @@ int Volume::init() {
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 256 * sizeof(uint32_t), hipHostMallocDefault));
     HIPCHK(small_.ensure(SMALL_WORDS * 4 * 4));
     HIPCHK(arena_.init(1 << 16));
+    HIPCHK(hipEventCreateWithFlags(&handoff_, hipEventDisableTiming));
     // Initialize(): default transfer function and range (gvdb_volume_gvdb.cpp:2341-2348)
     tf_host.assign(size_t(TF_SIZE) * 4, 0.f);
     const float a[4] = {0, 0, 0, 0}, b[4] = {1, 1, 1, 0.1f};

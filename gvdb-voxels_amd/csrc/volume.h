@@ -1,6 +1,7 @@
 // volume.h -- host runtime of one grid: pools, atlas, aux buffers, scene.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <memory>
 #include <string>
 #include <vector>
 #include "gvdb_device.h"
@@ -47,6 +48,15 @@ struct CameraParams {
 };
 
 enum Stage { ST_TOPO = 0, ST_ATLAS, ST_INSERT, ST_GATHER, ST_APRON, ST_RENDER, ST_CLEAR, ST_COUNT };
+
+// A stream shared by the grids that run on it; destroyed (after draining)
+// when its last user lets go, so a batch may go away before its grids.
+struct SharedStream {
+    hipStream_t s = nullptr;
+    int device = 0;
+    ~SharedStream();
+    static std::shared_ptr<SharedStream> make(int device, hipError_t& e);
+};
 
 // Where the stages of one call run: the stream, argument staging, and a
 // SMALL_WORDS-word slot per grid (device and pinned host) for the few values
@@ -122,7 +132,9 @@ public:
     hipStream_t stream = nullptr;
     int device = 0;
     Exec self_exec();
-    int use_stream(hipStream_t s);
+    // moves this grid's work to another stream (null: its own), ordered
+    // after what is already queued, without waiting on the host
+    int use_stream(const std::shared_ptr<SharedStream>& s);
 
     // Stages over G grids at once (pipeline.cpp); the single-grid methods
     // above call these with G = 1.  On failure `failed` names the grid.
@@ -187,7 +199,8 @@ private:
     DevBuf bitmaps_, ranks_, masks_, scan_tmp_, flags_, slot_scan_;
     uint32_t* pinned_ = nullptr;   // host staging (256 words)
     Arena arena_;
-    hipStream_t own_stream_ = nullptr;
+    std::shared_ptr<SharedStream> own_stream_, cur_stream_;
+    hipEvent_t handoff_ = nullptr;
 
     // render
     RenderBuf rbuf_[4];

@@ -8,8 +8,8 @@ Workloads (tree log2 dims {3,3,3,4,5} leaf first, 8^3 bricks, apron 1):
   sphere_shell   262,144 points, radius 120 + N(0, 1.5), centre (256,256,256)
   slab_plane     4,194,304 points uniform in [56,456]x[20,220]x[56,456] plus
                  200,000 points uniform on the plane y = 10, x,z in [56,456]
-  noise_cloud    points rejection-sampled where 3-octave value noise exceeds a
-                 threshold inside a 512^3 box
+  noise_cloud    points uniform in the cells (8 voxels wide in a 512^3 box)
+                 where 3-octave value noise exceeds a threshold
   clusters       spherical Gaussian clusters (sigma 20-80) with centres in a
                  2048^3 index space; points outside [0, 2048) are dropped
 """
@@ -68,17 +68,19 @@ def _value_noise(p, g, octaves=3, base_cells=8):
     return total / amp_sum
 
 
-def noise_cloud(n, seed=3, box=512.0, thresh=0.55, margin=8.0):
+def noise_cloud(n, seed=3, box=512.0, thresh=0.55, margin=8.0, cells=64):
+    """Points uniform inside the cells of a cells^3 grid over the box whose
+    3-octave value noise (evaluated at the cell centres) exceeds `thresh`."""
     g = _rng(seed)
-    out = []
-    got = 0
-    while got < n:
-        m = max(4 * (n - got), 1024)
-        p = g.random((m, 3))
-        keep = p[_value_noise(p, _rng(seed + 1000)) > thresh]
-        out.append(keep)
-        got += len(keep)
-    p = np.concatenate(out)[:n]
+    c = (np.arange(cells) + 0.5) / cells
+    cx, cy, cz = np.meshgrid(c, c, c, indexing="ij")
+    centres = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], 1)
+    keep = np.nonzero(_value_noise(centres, _rng(seed + 1000)) > thresh)[0]
+    if len(keep) == 0:
+        keep = np.arange(len(centres))
+    pick = keep[g.integers(0, len(keep), n)]
+    ijk = np.stack(np.unravel_index(pick, (cells, cells, cells)), 1).astype(np.float64)
+    p = (ijk + g.random((n, 3))) / cells
     return (margin + p * (box - 2 * margin)).astype(np.float32)
 
 
