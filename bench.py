@@ -112,7 +112,82 @@ def _grid_scene(pts, w):
     return W.ea_scene(w, w, centre=c, dist=float(2.2 * np.max(hi - lo)))
 
 
-def run_batched(args):
+def run_serving(args, grids):
+    """Serving: the 64 grids of the batched workload stay resident; a stream
+    of requests (70% renders of 256-1024 pixels a side from a random camera
+    on a sphere of 1.5x the grid's extent, 30% inserts of 20k-1M points
+    near the grid's existing points, each followed by the grid's rebuild)
+    is served plainly (one request at a time, one-grid calls) and with
+    continuous batching.  Saturation throughput comes from all requests
+    arriving at once; latency from Poisson arrivals at 80% of the plain
+    form's throughput."""
+    import gvdb_amd as G
+    from gvdb_amd import workloads as W
+    g = np.random.default_rng(5)
+    n_req = args.serve_requests
+    S = G.Server(0, 200 << 30, 64)
+    vols = []
+    for kind, pts in grids:
+        v = G.VolumeGVDB(0)
+        v.Configure(5, 4, 3, 3, 3)
+        v.SetChannelDefault(16, 16, 1)
+        v.AddChannel(0, G.T_FLOAT, 1)
+        v.apply_scene(W.ea_scene(256, 256))
+        S.add_grid(v, pts, 2.0)
+        vols.append(v)
+    plan = []
+    for i in range(n_req):
+        k = int(g.integers(0, len(grids)))
+        if g.random() < 0.7:
+            side = int(g.integers(32, 129)) * 8
+            plan.append(("render", k, side, (float(g.uniform(-180, 180)), float(g.uniform(-80, 80)), 0.0)))
+        else:
+            m = int(round(np.exp(np.log(2e4) + (np.log(1e6) - np.log(2e4)) * g.random())))
+            base = grids[k][1][g.integers(0, len(grids[k][1]), m)]
+            pts = np.maximum(base + g.normal(0, 2.0, base.shape), 0.5).astype(np.float32)
+            plan.append(("insert", k, pts, None))
+
+    def submit(arrivals):
+        S.clear_requests()
+        for (kind, k, a, b), t in zip(plan, arrivals):
+            if kind == "render":
+                S.submit_render(k, t, a, a, G.SHADE_VOLUME, b, 0.0)
+            else:
+                S.submit_insert(k, t, a)
+
+    def serve(mode, arrivals):
+        S.reset()
+        submit(arrivals)
+        lat, st, wall = S.run(mode)
+        assert np.all(st == 0), "requests failed: %s" % np.unique(st)
+        return lat, wall
+
+    zero = np.zeros(n_req)
+    serve(S.CONTINUOUS, zero)          # warm-up
+    lp0, wp0 = serve(S.PLAIN, zero)
+    lc0, wc0 = serve(S.CONTINUOUS, zero)
+    rate = 0.8 * n_req / wp0
+    arrivals = np.cumsum(g.exponential(1.0 / rate, n_req))
+    arrivals -= arrivals[0]
+    lp, wp = serve(S.PLAIN, arrivals)
+    lc, wc = serve(S.CONTINUOUS, arrivals)
+    out = dict(workload="64 resident grids (batched workload), %d requests: 70%% EA renders 256-1024 px a side, "
+                        "30%% inserts of 20k-1M points + rebuild" % n_req,
+               requests=n_req, renders=sum(1 for p in plan if p[0] == "render"),
+               saturation=dict(plain_requests_per_s=n_req / wp0, continuous_requests_per_s=n_req / wc0,
+                               speedup=wp0 / wc0),
+               poisson=dict(rate_per_s=rate,
+                            plain=dict(requests_per_s=n_req / wp, p50_ms=1e3 * float(np.percentile(lp, 50)),
+                                       p99_ms=1e3 * float(np.percentile(lp, 99))),
+                            continuous=dict(requests_per_s=n_req / wc, p50_ms=1e3 * float(np.percentile(lc, 50)),
+                                            p99_ms=1e3 * float(np.percentile(lc, 99)))))
+    S.close()
+    for v in vols:
+        v.close()
+    return out
+
+
+def run_batched(args, grids):
     """The batched workload: 64 grids (point counts log-uniform in [10k, 2M],
     shell / slab / value-noise cloud), each splatted (topology, atlas, insert,
     gather, apron) and rendered at 512x512 (emission-absorption).  One request
@@ -122,7 +197,6 @@ def run_batched(args):
     submitted at once."""
     import gvdb_amd as G
     from gvdb_amd import workloads as W
-    grids = W.batch_grids()
     w = args.batch_width
     vols = []
     for kind, pts in grids:
@@ -257,6 +331,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-grid batched comparison")
     ap.add_argument("--batch-width", type=int, default=512)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving comparison")
+    ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -288,8 +364,13 @@ def main():
                             frac=apron_gbs / HBM_PEAK_GBS, algorithmic_bytes=ab["apron"]),
         batched=None,
     )
-    if not args.no_batched:
-        line["batched"] = run_batched(args)
+    if not (args.no_batched and args.no_serve):
+        from gvdb_amd import workloads as W
+        grids = W.batch_grids()
+        if not args.no_batched:
+            line["batched"] = run_batched(args, grids)
+        if not args.no_serve:
+            line["serving"] = run_serving(args, grids)
     if not args.no_cpu:
         c = run_cpu_baseline()
         line["cpu_baseline"] = dict(value=c["points"] / c["t_splat"], unit="points/s", cores=c["threads"], kind="port",

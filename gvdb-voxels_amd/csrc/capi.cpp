@@ -1,6 +1,7 @@
 // capi.cpp -- extern "C" entry points of include/gvdb_amd.h over gvdb::Volume.
 #include "volume.h"
 #include "batch.h"
+#include "serve.h"
 #include "../../include/gvdb_amd.h"
 #include <new>
 #include <cstring>
@@ -14,6 +15,11 @@ struct gvdb_volume {
 struct gvdb_batch_s {
     gvdb::Batch b;
     gvdb_batch_s(int d, int m) : b(d, m) {}
+};
+
+struct gvdb_server_s {
+    gvdb::Server s;
+    gvdb_server_s(int d, unsigned long long b, int m) : s(d, b, m) {}
 };
 
 static_assert(sizeof(gvdb_node) == sizeof(gvdb::Node), "gvdb_node mirrors the 64-byte node");
@@ -274,5 +280,52 @@ int gvdb_release_stream(gvdb_handle h) {
     H_OR_FAIL(h);
     return h->v.use_stream(std::shared_ptr<gvdb::SharedStream>());
 }
+
+// ------------------------------------------------------------- serving
+int gvdb_server_create(gvdb_server* out, int device, unsigned long long hbm_budget, int max_batch) {
+    if (!out) return GVDB_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return GVDB_ERR_HIP;
+    gvdb_server_s* s = new (std::nothrow) gvdb_server_s(device, hbm_budget, max_batch);
+    if (!s) return GVDB_ERR_OOM;
+    int rc = s->s.init();
+    if (rc) { delete s; return rc; }
+    *out = s;
+    return GVDB_OK;
+}
+int gvdb_server_destroy(gvdb_server s) {
+    if (!s) return GVDB_ERR_INVALID;
+    delete s;
+    return GVDB_OK;
+}
+const char* gvdb_server_last_error(gvdb_server s) { return s ? s->s.err.c_str() : "null server"; }
+int gvdb_server_add_grid(gvdb_server s, gvdb_handle h, const float* host_xyz, int n, float radius) {
+    if (!s || !h) return GVDB_ERR_INVALID;
+    return s->s.add_grid(&h->v, host_xyz, n, radius);
+}
+int gvdb_server_reset(gvdb_server s) {
+    if (!s) return GVDB_ERR_INVALID;
+    return s->s.reset();
+}
+int gvdb_server_submit_render(gvdb_server s, int grid, double arrival, int w, int hgt, int shading, const float angs[3],
+                              float dist, unsigned char* out_rgba) {
+    if (!s) return GVDB_ERR_INVALID;
+    return s->s.submit_render(grid, arrival, w, hgt, shading, angs, dist, out_rgba);
+}
+int gvdb_server_submit_insert(gvdb_server s, int grid, double arrival, const float* host_xyz, int n) {
+    if (!s) return GVDB_ERR_INVALID;
+    return s->s.submit_insert(grid, arrival, host_xyz, n);
+}
+int gvdb_server_clear_requests(gvdb_server s) {
+    if (!s) return GVDB_ERR_INVALID;
+    s->s.clear_requests();
+    return GVDB_OK;
+}
+int gvdb_server_run(gvdb_server s, int mode, double* latency, int* status, double* wall) {
+    if (!s || (mode != 0 && mode != 1)) return GVDB_ERR_INVALID;
+    return s->s.run(mode, latency, status, wall);
+}
+unsigned long long gvdb_server_device_bytes(gvdb_server s) { return s ? s->s.device_bytes() : 0ull; }
 
 }  // extern "C"

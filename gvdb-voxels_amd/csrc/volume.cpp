@@ -659,6 +659,19 @@ int Volume::read_render_buf(int buf, void* dst) {
     return GVDB_OK;
 }
 
+unsigned long long Volume::device_bytes() const {
+    unsigned long long b = 0;
+    for (int g = 0; g < 2; g++)
+        for (int l = 0; l < MAXLEV; l++) b += pool_[g][l].dev.bytes;
+    for (const auto& c : chan_) b += c.data.bytes;
+    for (const auto& r : rbuf_) b += r.data.bytes;
+    const DevBuf* bufs[] = {&amap_, &nbr_, &pts_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_,
+                            &sc_use_, &bkt_key_, &bkt_slot_, &bkt_cnt_, &bkt_off_, &sorted_, &sc_flags_, &small_,
+                            &masks_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev};
+    for (const DevBuf* d : bufs) b += d->bytes;
+    return b;
+}
+
 // ---------------------------------------------------------------- access ----
 int Volume::get_pool(int grp, int lev, void* dst, u64* count, u64* stride) {
     if (grp < 0 || grp > 1 || lev < 0 || lev >= levels_) return fail(GVDB_ERR_INVALID, "get_pool: bad group or level");

@@ -117,6 +117,8 @@ public:
     int add_render_buf(int buf, int w, int h, int bpp);
     int render(int shading, int chan, int buf);
     int read_render_buf(int buf, void* dst);
+    void* render_buf_ptr(int buf) const { return (buf >= 0 && buf < 4) ? rbuf_[buf].data.p : nullptr; }
+    unsigned long long device_bytes() const;
     void scene_block(int w, int h, ScnBlock& s) const;
 
     // access

@@ -63,6 +63,13 @@ _SIGS = [
     ("gvdb_batch_render", _i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
     ("gvdb_batch_synchronize", _i, [_vp]), ("gvdb_batch_elapsed_ms", _i, [_vp, _vp]),
     ("gvdb_release_stream", _i, [_vp]),
+    ("gvdb_server_create", _i, [ctypes.POINTER(_vp), _i, ctypes.c_ulonglong, _i]),
+    ("gvdb_server_destroy", _i, [_vp]), ("gvdb_server_last_error", ctypes.c_char_p, [_vp]),
+    ("gvdb_server_add_grid", _i, [_vp, _vp, _vp, _i, _f]), ("gvdb_server_reset", _i, [_vp]),
+    ("gvdb_server_submit_render", _i, [_vp, _i, ctypes.c_double, _i, _i, _i, _vp, _f, _vp]),
+    ("gvdb_server_submit_insert", _i, [_vp, _i, ctypes.c_double, _vp, _i]),
+    ("gvdb_server_clear_requests", _i, [_vp]), ("gvdb_server_run", _i, [_vp, _i, _vp, _vp, _vp]),
+    ("gvdb_server_device_bytes", ctypes.c_ulonglong, [_vp]),
 ]
 
 _lib = None
@@ -395,3 +402,73 @@ class GridBatch:
         ms = np.zeros(1, np.float32)
         self._chk(load().gvdb_batch_elapsed_ms(self.h, _p(ms)))
         return float(ms[0])
+
+
+class Server:
+    """Serving loop over resident grids (see include/gvdb_amd.h): add grids,
+    submit timed render / insert requests, run them plainly (mode 0) or with
+    continuous batching (mode 1)."""
+
+    PLAIN, CONTINUOUS = 0, 1
+
+    def __init__(self, device=0, hbm_budget=200 << 30, max_batch=64):
+        L = load()
+        h = ctypes.c_void_p()
+        rc = L.gvdb_server_create(ctypes.byref(h), int(device), int(hbm_budget), int(max_batch))
+        if rc != OK:
+            raise GVDBError(rc, "gvdb_server_create failed")
+        self.h = h
+        self._keep = []
+        self.requests = []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().gvdb_server_destroy(self.h)
+            self.h = None
+
+    def close(self):
+        self.__del__()
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise GVDBError(rc, load().gvdb_server_last_error(self.h).decode())
+        return rc
+
+    def add_grid(self, vol, pts, radius=2.0):
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        self._keep.append(vol)
+        return self._chk(load().gvdb_server_add_grid(self.h, vol.h, _p(pts), len(pts), float(radius)))
+
+    def reset(self):
+        return self._chk(load().gvdb_server_reset(self.h))
+
+    def submit_render(self, grid, arrival, w, h, shading=SHADE_VOLUME, angs=(30.0, 20.0, 0.0), dist=0.0):
+        out = np.zeros((h, w, 4), np.uint8)
+        a = np.asarray(angs, np.float32)
+        i = self._chk(load().gvdb_server_submit_render(self.h, int(grid), float(arrival), int(w), int(h), int(shading),
+                                                       _p(a), float(dist), _p(out)))
+        self._keep.append(out)
+        self.requests.append(("render", grid, out))
+        return i
+
+    def submit_insert(self, grid, arrival, pts):
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        self._keep.append(pts)
+        i = self._chk(load().gvdb_server_submit_insert(self.h, int(grid), float(arrival), _p(pts), len(pts)))
+        self.requests.append(("insert", grid, pts))
+        return i
+
+    def clear_requests(self):
+        self.requests = []
+        return self._chk(load().gvdb_server_clear_requests(self.h))
+
+    def run(self, mode):
+        n = len(self.requests)
+        lat = np.zeros(max(n, 1), np.float64)
+        st = np.zeros(max(n, 1), np.int32)
+        wall = np.zeros(1, np.float64)
+        self._chk(load().gvdb_server_run(self.h, int(mode), _p(lat), _p(st), _p(wall)))
+        return lat[:n], st[:n], float(wall[0])
+
+    def device_bytes(self):
+        return int(load().gvdb_server_device_bytes(self.h))

@@ -197,6 +197,39 @@ int gvdb_batch_synchronize(gvdb_batch b);
 int gvdb_batch_elapsed_ms(gvdb_batch b, float* ms);
 int gvdb_release_stream(gvdb_handle h);
 
+/* ---------------------------------------------------------------- serving
+ * A serving loop over resident grids (no counterpart in the reference).
+ * Grids are added with their initial points and splat radius; the server
+ * owns a growing copy of each grid's points.  Requests carry an arrival time
+ * in seconds from the start of gvdb_server_run:
+ *   render: w x hgt RGBA8 (<= 1024 per side) into out_rgba, camera orbiting
+ *           the grid's centre at `angs` (degrees) and `dist` (<= 0: 1.5x the
+ *           grid's extent), with the volume's other scene settings;
+ *   insert: appends n points (host_xyz must stay valid until the run ends)
+ *           and rebuilds the grid (topology, atlas, insert, gather, apron).
+ * gvdb_server_run mode 0 serves one request at a time in arrival order
+ * through the one-grid calls; mode 1 batches continuously (see serve.h):
+ * each pass admits arrived requests whose grid is idle, within max_batch and
+ * the HBM budget (an insert that can never fit is refused with
+ * GVDB_ERR_OOM in its status), renders as one launch overlapping the
+ * inserts' batched pipeline.  latency/status get one entry per request in
+ * submission order; wall = first arrival to last completion (s).
+ * Returns the request index from the submit calls, the grid index from
+ * gvdb_server_add_grid. */
+typedef struct gvdb_server_s* gvdb_server;
+int gvdb_server_create(gvdb_server* out, int device, unsigned long long hbm_budget, int max_batch);
+int gvdb_server_destroy(gvdb_server s);
+const char* gvdb_server_last_error(gvdb_server s);
+int gvdb_server_add_grid(gvdb_server s, gvdb_handle h, const float* host_xyz, int n, float radius);
+/* (re)builds every grid from its initial points (batched) */
+int gvdb_server_reset(gvdb_server s);
+int gvdb_server_submit_render(gvdb_server s, int grid, double arrival, int w, int hgt, int shading, const float angs[3],
+                              float dist, unsigned char* out_rgba);
+int gvdb_server_submit_insert(gvdb_server s, int grid, double arrival, const float* host_xyz, int n);
+int gvdb_server_clear_requests(gvdb_server s);
+int gvdb_server_run(gvdb_server s, int mode, double* latency, int* status, double* wall);
+unsigned long long gvdb_server_device_bytes(gvdb_server s);
+
 #ifdef __cplusplus
 }
 #endif

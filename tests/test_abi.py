@@ -19,7 +19,7 @@ HEADER = os.path.join(ROOT, "include", "gvdb_amd.h")
 
 def _declared():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(gvdb_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^(?:int|const char\*|unsigned long long)\s+(gvdb_\w+)\s*\(", txt, re.M)))
 
 
 def _defines():
