@@ -28,19 +28,22 @@ __device__ inline void k_neighbors_body(const NbrArgs& a, uint32_t bid, uint32_t
     if (g >= a.n * 27) return;
     int b = g / 27, d = g - b * 27;
     const Node* n = t.nodes[0] + b;
+    int v;
     if (d == 13) {
         // the brick itself, or -1 when the walk from the top node does not reach it
-        nbr[g] = pos_leaf(t, n->mPos[0], n->mPos[1], n->mPos[2]) == b ? b : -1;
-        return;
+        v = pos_leaf(t, n->mPos[0], n->mPos[1], n->mPos[2]) == b ? b : -1;
+    } else {
+        int dx, dy, dz;
+        dir_of(d, dx, dy, dz);
+        int w = t.brickwid;
+        // a voxel centre inside the neighbour box: local coordinate -1, 0 or w
+        float px = float(n->mPos[0]) + float(dx < 0 ? -1 : (dx > 0 ? w : 0)) + 0.5f;
+        float py = float(n->mPos[1]) + float(dy < 0 ? -1 : (dy > 0 ? w : 0)) + 0.5f;
+        float pz = float(n->mPos[2]) + float(dz < 0 ? -1 : (dz > 0 ? w : 0)) + 0.5f;
+        v = point_leaf(t, px, py, pz);
     }
-    int dx, dy, dz;
-    dir_of(d, dx, dy, dz);
-    int w = t.brickwid;
-    // a voxel centre inside the neighbour box: local coordinate -1, 0 or w
-    float px = float(n->mPos[0]) + float(dx < 0 ? -1 : (dx > 0 ? w : 0)) + 0.5f;
-    float py = float(n->mPos[1]) + float(dy < 0 ? -1 : (dy > 0 ? w : 0)) + 0.5f;
-    float pz = float(n->mPos[2]) + float(dz < 0 ? -1 : (dz > 0 ? w : 0)) + 0.5f;
-    nbr[g] = point_leaf(t, px, py, pz);
+    nbr[g] = v;
+    if (a.mask && v >= 0) atomicOr(&a.mask[b], 1u << d);
 }
 
 GVDB_MULTI(k_neighbors, NbrArgs, 256)

@@ -199,7 +199,13 @@ struct BucketArgs {
     uint32_t* slot;
     uint32_t* bcnt;
     uint32_t* orphans;
+    // optional: adds the number of (point, subcell) entries InsertSubcell
+    // would make (u64 counter); needs the neighbour masks and R <= spd
+    const uint32_t* nmask;
+    float radius;
+    u64* len;             // 64 partial counters, summed by launch_sum_parts_multi
 };
+hipError_t launch_sum_parts_multi(const u64* const* parts, u64* const* out, int G, Arena* ar, hipStream_t s);
 hipError_t launch_bucket_points_multi(const BucketArgs* a, int G, Arena* ar, hipStream_t s);
 
 struct ScatterArgs {
@@ -208,7 +214,7 @@ struct ScatterArgs {
     const uint32_t* key;
     const uint32_t* slot;
     const uint32_t* boff;
-    float* sorted;
+    float4* sorted;          // bucketed positions (+trans), 16 bytes each
 };
 hipError_t launch_bucket_scatter_multi(const ScatterArgs* a, int G, Arena* ar, hipStream_t s);
 
@@ -223,7 +229,7 @@ struct PullArgs {
     const int* nbr;
     const uint32_t* bcnt;
     const uint32_t* boff;
-    const float* sorted;
+    const float4* sorted;
     const uint32_t* mapping;
     uint32_t* sc_cnt;
     uint32_t* sc_use;
@@ -231,6 +237,24 @@ struct PullArgs {
     float* sc_pnt;
 };
 hipError_t launch_pull_multi(int write, const PullArgs* a, int G, Arena* ar, hipStream_t s);
+
+// GatherDensity straight from the buckets: one 512-thread workgroup per
+// linked 8^3 brick with 4^3 subcells, reading the points of the subcells
+// around it (the same candidates the pull binning reads).
+struct BucketGatherArgs {
+    Tree t;
+    int nleaves;
+    int R;
+    float radius;
+    const int* nbr;
+    const uint32_t* bcnt;
+    const uint32_t* boff;
+    const float4* sorted;
+    void* atlas;
+    int atlas_type;
+    int accumulate;
+};
+hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* ar, hipStream_t s);
 
 // ---- apron (apron.hip)
 // nbr[b*27 + d]: leaf holding the neighbour of brick b in direction
@@ -240,6 +264,7 @@ struct NbrArgs {
     Tree t;
     int* nbr;
     int n;
+    uint32_t* mask;       // optional, zeroed: bit d set when nbr[b*27 + d] >= 0
 };
 hipError_t launch_neighbors_multi(const NbrArgs* a, int G, Arena* ar, hipStream_t s);
 

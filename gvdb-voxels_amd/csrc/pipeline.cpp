@@ -27,7 +27,7 @@ static inline float key_float(uint32_t k) {
 }
 
 // small per-grid result words (device dsmall / pinned hsmall, SMALL_WORDS each)
-enum { SM_BBOX = 0, SM_LINKED = 8, SM_TOTAL = 9, SM_ORPHANS = 10, SM_BUCKETS = 11, SM_COUNTS = 16, SM_BOUNDS = 40 };
+enum { SM_BBOX = 0, SM_LINKED = 8, SM_TOTAL = 9, SM_ORPHANS = 10, SM_BUCKETS = 11, SM_LEN = 12, SM_COUNTS = 16, SM_BOUNDS = 40 };
 
 static int drain(Volume* v, const Exec& ex, int G) {
     XCHK(v, hipMemcpyAsync(ex.hsmall, ex.dsmall, size_t(G) * SMALL_WORDS * 4, hipMemcpyDeviceToHost, ex.s));
@@ -431,7 +431,7 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
             XCHK(v, v->bkt_off_.ensure(size_t(nb) * 4));
             XCHK(v, v->bkt_key_.ensure(size_t(std::max(n[g], 1)) * 4));
             XCHK(v, v->bkt_slot_.ensure(size_t(std::max(n[g], 1)) * 4));
-            XCHK(v, v->sorted_.ensure(size_t(std::max(n[g], 1)) * 12));
+            XCHK(v, v->sorted_.ensure(size_t(std::max(n[g], 1)) * 16));
         }
         pl.t = v->make_tree();
     }
@@ -443,7 +443,7 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         std::vector<FlagArgs> fl;
         std::vector<ScanArgs> sa;
         std::vector<NbrArgs> na;
-        std::vector<FillArgs> fa;
+        std::vector<FillArgs> fa, nf;
         for (int g = 0; g < G; g++) {
             Volume* v = vs[g];
             Plan& pl = plan[g];
@@ -453,15 +453,20 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
             fa.push_back(FillArgs{v->sc_cnt_.p, pl.numSC, 0, 0});
             if (pl.pull) {
                 if (v->nbr_gen_ != v->topo_gen_) {
-                    na.push_back(NbrArgs{pl.t, v->nbr_.as<int>(), int(pl.numMap)});
+                    XCHK(v, v->nmask_.ensure(size_t(pl.numMap) * 4));
+                    nf.push_back(FillArgs{v->nmask_.p, pl.numMap, 0, 0});
+                    na.push_back(NbrArgs{pl.t, v->nbr_.as<int>(), int(pl.numMap), v->nmask_.as<uint32_t>()});
                     v->nbr_gen_ = v->topo_gen_;
                 }
                 fa.push_back(FillArgs{v->bkt_cnt_.p, pl.numMap * u64(pl.spb), 0, 0});
                 fa.push_back(FillArgs{small(g, SM_ORPHANS), 1, 0, 0});
+                XCHK(v, v->lenparts_.ensure(64 * 8));
+                fa.push_back(FillArgs{v->lenparts_.p, 64, 0, 1});
             }
         }
         XCHK(v0, launch_leaf_flags_multi(fl.data(), int(fl.size()), ex.ar, ex.s));
         XCHK(v0, launch_scan_multi(sa.data(), int(sa.size()), ex.ar, ex.s));
+        XCHK(v0, launch_fill_multi(nf.data(), int(nf.size()), ex.ar, ex.s));
         XCHK(v0, launch_neighbors_multi(na.data(), int(na.size()), ex.ar, ex.s));
         XCHK(v0, launch_fill_multi(fa.data(), int(fa.size()), ex.ar, ex.s));
     }
@@ -469,7 +474,8 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         std::vector<BucketArgs> ba;
         std::vector<ScanArgs> sb, sc;
         std::vector<ScatterArgs> sca;
-        std::vector<PullArgs> pa;
+        std::vector<const u64*> lp;
+        std::vector<u64*> lo;
         for (int g = 0; g < G; g++) {
             Volume* v = vs[g];
             Plan& pl = plan[g];
@@ -484,22 +490,19 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
             }
             u64 nb = pl.numMap * u64(pl.spb);
             ba.push_back(BucketArgs{pl.t, pl.P, scs, pl.spb, {tr[0], tr[1], tr[2]}, v->bkt_key_.as<uint32_t>(),
-                                    v->bkt_slot_.as<uint32_t>(), v->bkt_cnt_.as<uint32_t>(), small(g, SM_ORPHANS)});
+                                    v->bkt_slot_.as<uint32_t>(), v->bkt_cnt_.as<uint32_t>(), small(g, SM_ORPHANS),
+                                    v->nmask_.as<uint32_t>(), radius[g], v->lenparts_.as<u64>()});
+            lp.push_back(v->lenparts_.as<u64>());
+            lo.push_back(reinterpret_cast<u64*>(small(g, SM_LEN)));
             sb.push_back(ScanArgs{v->bkt_cnt_.as<uint32_t>(), nullptr, 0, v->bkt_off_.as<uint32_t>(), nb, tmp2(g),
                                   small(g, SM_BUCKETS)});
             sca.push_back(ScatterArgs{pl.P, {tr[0], tr[1], tr[2]}, v->bkt_key_.as<uint32_t>(),
-                                      v->bkt_slot_.as<uint32_t>(), v->bkt_off_.as<uint32_t>(), v->sorted_.as<float>()});
-            pa.push_back(PullArgs{pl.t, int(pl.numMap), scs, pl.R, radius[g], v->nbr_.as<int>(),
-                                  v->bkt_cnt_.as<uint32_t>(), v->bkt_off_.as<uint32_t>(), v->sorted_.as<float>(),
-                                  v->sc_map_.as<uint32_t>(), v->sc_cnt_.as<uint32_t>(), v->sc_use_.as<uint32_t>(),
-                                  v->sc_off_.as<uint32_t>(), nullptr});
-            sc.push_back(ScanArgs{v->sc_cnt_.as<uint32_t>(), nullptr, 0, v->sc_off_.as<uint32_t>(), pl.numSC, tmp2(g),
-                                  small(g, SM_TOTAL)});
+                                      v->bkt_slot_.as<uint32_t>(), v->bkt_off_.as<uint32_t>(), v->sorted_.as<float4>()});
         }
         XCHK(v0, launch_bucket_points_multi(ba.data(), int(ba.size()), ex.ar, ex.s));
+        XCHK(v0, launch_sum_parts_multi(lp.data(), lo.data(), int(lp.size()), ex.ar, ex.s));
         XCHK(v0, launch_scan_multi(sb.data(), int(sb.size()), ex.ar, ex.s));
         XCHK(v0, launch_bucket_scatter_multi(sca.data(), int(sca.size()), ex.ar, ex.s));
-        XCHK(v0, launch_pull_multi(0, pa.data(), int(pa.size()), ex.ar, ex.s));
         XCHK(v0, launch_scan_multi(sc.data(), int(sc.size()), ex.ar, ex.s));
     }
     int rc = drain(v0, ex, G);
@@ -509,8 +512,14 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
     for (int g = 0; g < G; g++) {
         Plan& pl = plan[g];
         pl.linked = ex.hsmall[g * SMALL_WORDS + SM_LINKED];
-        pl.total = ex.hsmall[g * SMALL_WORDS + SM_TOTAL];
         pl.orphans = pl.pull ? ex.hsmall[g * SMALL_WORDS + SM_ORPHANS] : 0;
+        if (pl.pull) {
+            u64 l = 0;
+            memcpy(&l, ex.hsmall + g * SMALL_WORDS + SM_LEN, 8);
+            pl.total = l > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(l);
+        } else {
+            pl.total = ex.hsmall[g * SMALL_WORDS + SM_TOTAL];
+        }
         if (pl.pull && pl.orphans) {
             Volume* v = vs[g];
             pl.pull = false;
@@ -525,9 +534,9 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
     if (redo) {
         rc = drain(v0, ex, G);
         if (rc) return rc;
-        for (int g = 0; g < G; g++) plan[g].total = ex.hsmall[g * SMALL_WORDS + SM_TOTAL];
+        for (int g = 0; g < G; g++)
+            if (!plan[g].pull) plan[g].total = ex.hsmall[g * SMALL_WORDS + SM_TOTAL];
     }
-    std::vector<PullArgs> pw;
     std::vector<SubPosArgs> sp;
     for (int g = 0; g < G; g++) {
         Volume* v = vs[g];
@@ -540,16 +549,18 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         v->sc_map_n_ = int(pl.numMap);
         v->sc_gen_ = v->topo_gen_;
         v->last_insert_pull_ = pl.pull;
-        if (pl.total == 0) continue;
+        // the pull path leaves the subcell tables to be built on demand
+        // (materialize_entries); the gather reads the buckets directly
+        v->entries_stale_ = pl.pull && pl.total > 0;
+        v->bkt_scs_ = scs;
+        v->bkt_R_ = pl.R;
+        v->bkt_radius_ = radius[g];
+        v->bkt_gen_ = pl.pull ? v->topo_gen_ : ~0u;
+        if (pl.total == 0 || pl.pull) continue;
         XCHK(v, v->sc_pnt_.ensure(size_t(pl.total) * 12));
         XCHK(v, v->sc_pos_.ensure(size_t(pl.numSC) * 12));
         XCHK(v, v->sc_nid_.ensure(size_t(pl.numSC) * 4));
-        if (pl.pull) {
-            pw.push_back(PullArgs{pl.t, int(pl.numMap), scs, pl.R, radius[g], v->nbr_.as<int>(),
-                                  v->bkt_cnt_.as<uint32_t>(), v->bkt_off_.as<uint32_t>(), v->sorted_.as<float>(),
-                                  v->sc_map_.as<uint32_t>(), v->sc_cnt_.as<uint32_t>(), v->sc_use_.as<uint32_t>(),
-                                  v->sc_off_.as<uint32_t>(), v->sc_pnt_.as<float>()});
-        } else {
+        {
             XCHK(v, launch_fill_u32(v->sc_cnt_.as<uint32_t>(), 0u, pl.numSC, ex.s));
             XCHK(v, launch_insert_subcell(pl.t, pl.P, scs, pl.spb, trans3 + 3 * g, radius[g], v->sc_map_.as<uint32_t>(),
                                           v->sc_cnt_.as<uint32_t>(), v->sc_off_.as<uint32_t>(), v->sc_pnt_.as<float>(),
@@ -560,9 +571,51 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         sp.push_back(SubPosArgs{pl.t, scs, pl.spb, v->sc_map_.as<uint32_t>(), v->sc_pos_.as<int>(), v->sc_nid_.as<int>(),
                                 int(pl.numMap)});
     }
-    XCHK(v0, launch_pull_multi(1, pw.data(), int(pw.size()), ex.ar, ex.s));
     XCHK(v0, launch_subcell_pos_multi(sp.data(), int(sp.size()), ex.ar, ex.s));
     if (G == 1) v0->stage_end(ST_INSERT);
+    return GVDB_OK;
+}
+
+// Subcell tables of pull-path inserts, built when something reads them: the
+// per-brick count pass, the offsets, the entries (useful ones first) and the
+// subcell corners.
+int Volume::multi_materialize(Volume* const* vs, int G, const Exec& ex, int* failed) {
+    std::vector<FillArgs> fa;
+    std::vector<PullArgs> pc, pw;
+    std::vector<ScanArgs> sc;
+    std::vector<SubPosArgs> sp;
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        if (!v->entries_stale_) continue;
+        if (v->bkt_gen_ != v->topo_gen_) FAILG(g, v->fail(GVDB_ERR_STATE, "subcell tables: topology changed since InsertPointsSubcell"));
+        const int scs = v->bkt_scs_;
+        const int sd = v->res_[0] / scs, spb = sd * sd * sd;
+        const u64 numSC = u64(v->num_sc_), numMap = u64(v->sc_map_n_);
+        XCHK(v, v->sc_pnt_.ensure(size_t(v->sc_len_) * 12));
+        XCHK(v, v->sc_pos_.ensure(size_t(numSC) * 12));
+        XCHK(v, v->sc_nid_.ensure(size_t(numSC) * 4));
+        XCHK(v, v->scan_tmp_.ensure(scan_tmp_words(numSC) * 4 + 256));
+        Tree t = v->make_tree();
+        fa.push_back(FillArgs{v->sc_cnt_.p, numSC, 0, 0});
+        PullArgs a{t, int(numMap), scs, v->bkt_R_, v->bkt_radius_, v->nbr_.as<int>(), v->bkt_cnt_.as<uint32_t>(),
+                   v->bkt_off_.as<uint32_t>(), v->sorted_.as<float4>(), v->sc_map_.as<uint32_t>(),
+                   v->sc_cnt_.as<uint32_t>(), v->sc_use_.as<uint32_t>(), v->sc_off_.as<uint32_t>(), nullptr};
+        pc.push_back(a);
+        a.sc_pnt = v->sc_pnt_.as<float>();
+        pw.push_back(a);
+        sc.push_back(ScanArgs{v->sc_cnt_.as<uint32_t>(), nullptr, 0, v->sc_off_.as<uint32_t>(), numSC,
+                              v->scan_tmp_.as<uint32_t>(), nullptr});
+        sp.push_back(SubPosArgs{t, scs, spb, v->sc_map_.as<uint32_t>(), v->sc_pos_.as<int>(), v->sc_nid_.as<int>(),
+                                int(numMap)});
+        v->entries_stale_ = false;
+    }
+    if (pc.empty()) return GVDB_OK;
+    Volume* v0 = vs[0];
+    XCHK(v0, launch_fill_multi(fa.data(), int(fa.size()), ex.ar, ex.s));
+    XCHK(v0, launch_pull_multi(0, pc.data(), int(pc.size()), ex.ar, ex.s));
+    XCHK(v0, launch_scan_multi(sc.data(), int(sc.size()), ex.ar, ex.s));
+    XCHK(v0, launch_pull_multi(1, pw.data(), int(pw.size()), ex.ar, ex.s));
+    XCHK(v0, launch_subcell_pos_multi(sp.data(), int(sp.size()), ex.ar, ex.s));
     return GVDB_OK;
 }
 
@@ -571,6 +624,8 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
                          const Exec& ex, int* failed) {
     // GatherDensity (gvdb_volume_gvdb.cpp:5318-5344)
     std::vector<GatherArgs> ga;
+    std::vector<BucketGatherArgs> bg;
+    std::vector<Volume*> need;
     for (int g = 0; g < G; g++) {
         Volume* v = vs[g];
         if (len[g] == 0 || v->pool_[0][0].usedNum == 0) continue;
@@ -582,18 +637,33 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
             FAILG(g, v->fail(GVDB_ERR_STATE, "GatherDensity: topology changed since UpdateAtlas/InsertPointsSubcell"));
         int rc = v->commit_pools();
         if (rc) FAILG(g, rc);
+        const bool buckets = v->bkt_gen_ == v->topo_gen_ && scs == v->bkt_scs_ && radius[g] == v->bkt_radius_ &&
+                             v->res_[0] == 8 && scs == 4 && v->bkt_R_ >= 1 && v->bkt_R_ <= 2;
+        if (buckets) {
+            bg.push_back(BucketGatherArgs{v->make_tree(), int(v->pool_[0][0].lastEle), v->bkt_R_, radius[g],
+                                          v->nbr_.as<int>(), v->bkt_cnt_.as<uint32_t>(), v->bkt_off_.as<uint32_t>(),
+                                          v->sorted_.as<float4>(), v->chan_[chanD].data.p, v->chan_[chanD].type,
+                                          accum ? 1 : 0});
+            continue;
+        }
+        if (v->entries_stale_) need.push_back(v);
         ga.push_back(GatherArgs{v->make_tree(), scs, radius[g], v->sc_nid_.as<int>(), v->sc_cnt_.as<uint32_t>(),
                                 v->sc_use_.as<uint32_t>(), v->sc_off_.as<uint32_t>(), v->sc_pos_.as<int>(),
                                 v->sc_pnt_.as<float>(), v->chan_[chanD].data.p, v->chan_[chanD].type, accum ? 1 : 0,
                                 v->sc_bricks_});
     }
-    if (ga.empty()) return GVDB_OK;
+    if (ga.empty() && bg.empty()) return GVDB_OK;
     Volume* v0 = vs[0];
+    if (!need.empty()) {
+        int rc = multi_materialize(need.data(), int(need.size()), ex, failed);
+        if (rc) return rc;
+    }
     if (G == 1) {
         v0->stage_begin(ST_GATHER);
         (void)hipEventRecord(v0->kev_[0][0], ex.s);
     }
     XCHK(v0, launch_gather_multi(ga.data(), int(ga.size()), ex.ar, ex.s));
+    XCHK(v0, launch_bucket_gather_multi(bg.data(), int(bg.size()), ex.ar, ex.s));
     if (G == 1) {
         (void)hipEventRecord(v0->kev_[0][1], ex.s);
         v0->kev_used_[0] = true;
@@ -606,6 +676,7 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
 int Volume::multi_apron(Volume* const* vs, int G, int c, float boundval, const Exec& ex, int* failed) {
     // UpdateApron (gvdb_volume_gvdb.cpp:4418-4457)
     std::vector<NbrArgs> na;
+    std::vector<FillArgs> nf;
     std::vector<ApronArgs> aa;
     for (int g = 0; g < G; g++) {
         Volume* v = vs[g];
@@ -620,7 +691,9 @@ int Volume::multi_apron(Volume* const* vs, int G, int c, float boundval, const E
         Tree t = v->make_tree();
         if (v->nbr_gen_ != v->topo_gen_) {
             XCHK(v, v->nbr_.ensure(size_t(bricks) * 27 * 4));
-            na.push_back(NbrArgs{t, v->nbr_.as<int>(), bricks});
+            XCHK(v, v->nmask_.ensure(size_t(bricks) * 4));
+            nf.push_back(FillArgs{v->nmask_.p, u64(bricks), 0, 0});
+            na.push_back(NbrArgs{t, v->nbr_.as<int>(), bricks, v->nmask_.as<uint32_t>()});
             v->nbr_gen_ = v->topo_gen_;
         }
         aa.push_back(ApronArgs{t, v->nbr_.as<int>(), v->chan_[c].data.p, v->chan_[c].type, boundval, bricks});
@@ -628,6 +701,7 @@ int Volume::multi_apron(Volume* const* vs, int G, int c, float boundval, const E
     if (aa.empty()) return GVDB_OK;
     Volume* v0 = vs[0];
     if (G == 1) v0->stage_begin(ST_APRON);
+    XCHK(v0, launch_fill_multi(nf.data(), int(nf.size()), ex.ar, ex.s));
     XCHK(v0, launch_neighbors_multi(na.data(), int(na.size()), ex.ar, ex.s));
     if (G == 1) (void)hipEventRecord(v0->kev_[1][0], ex.s);
     XCHK(v0, launch_apron_multi(aa.data(), int(aa.size()), ex.ar, ex.s));

@@ -470,50 +470,104 @@ hipError_t launch_leaf_bounds_multi(const BoundsArgs* a, int G, Arena* ar, hipSt
 // neighbourhood fits in the 27 bricks around; otherwise the push kernels run.
 namespace gvdb {
 
+static const int LEN_PARTS = 64;   // spread counters of the entry count
+
 __device__ inline void k_bucket_points_body(const BucketArgs& a, uint32_t bid, uint32_t) {
     const Tree& t = a.t;
     const PointSrc& pts = a.pts;
     const int scs = a.scs, spb = a.spb;
     const float tx = a.tr[0], ty = a.tr[1], tz = a.tr[2];
     uint32_t *key = a.key, *slot = a.slot, *bcnt = a.bcnt, *orphans = a.orphans;
-    int i = bid * 256 + threadIdx.x;
-    if (i >= pts.n) return;
-    const float* p = point_at(pts, i);
-    float p0 = p[0], p1 = p[1], p2 = p[2];
-    uint32_t k = 0xFFFFFFFFu;
-    float w0 = p0 + tx, w1 = p1 + ty, w2 = p2 + tz;
-    bool valid = p2 != NOHIT && isfinite(w0) && isfinite(w1) && isfinite(w2) && w0 >= 0 && w1 >= 0 && w2 >= 0 &&
-                 p0 >= 0 && p1 >= 0 && p2 >= 0;
-    if (valid) {
-        int ox = int(w0) / scs * scs, oy = int(w1) / scs * scs, oz = int(w2) / scs * scs;
-        int leaf = pos_leaf(t, ox, oy, oz);
-        if (leaf < 0) {
-            atomicAdd(orphans, 1u);
-        } else {
-            const Node* n = t.nodes[0] + leaf;
-            int sd = t.res[0] / scs;
-            int lx = (ox - n->mPos[0]) / scs, ly = (oy - n->mPos[1]) / scs, lz = (oz - n->mPos[2]) / scs;
-            k = uint32_t(leaf) * uint32_t(spb) + uint32_t((lz * sd + ly) * sd + lx);
-            slot[i] = atomicAdd(&bcnt[k], 1u);
+    const int i = bid * 256 + threadIdx.x;
+    uint32_t k = 0xFFFFFFFFu, binned = 0;
+    if (i < pts.n) {
+        const float* p = point_at(pts, i);
+        float p0 = p[0], p1 = p[1], p2 = p[2];
+        float w[3] = {p0 + tx, p1 + ty, p2 + tz};
+        bool valid = p2 != NOHIT && isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && w[0] >= 0 && w[1] >= 0 &&
+                     w[2] >= 0 && p0 >= 0 && p1 >= 0 && p2 >= 0;
+        if (valid) {
+            int ox = int(w[0]) / scs * scs, oy = int(w[1]) / scs * scs, oz = int(w[2]) / scs * scs;
+            int leaf = pos_leaf(t, ox, oy, oz);
+            if (leaf < 0) {
+                atomicAdd(orphans, 1u);
+            } else {
+                const Node* n = t.nodes[0] + leaf;
+                const int res0 = t.res[0];
+                int sd = res0 / scs;
+                int lx = (ox - n->mPos[0]) / scs, ly = (oy - n->mPos[1]) / scs, lz = (oz - n->mPos[2]) / scs;
+                k = uint32_t(leaf) * uint32_t(spb) + uint32_t((lz * sd + ly) * sd + lx);
+                slot[i] = atomicAdd(&bcnt[k], 1u);
+                if (a.len) {
+                    // subcells the point is binned into (CountSubcell's per-axis
+                    // range), split per axis into those below, inside and above
+                    // this brick; all lie in it or a face/edge/corner neighbour
+                    const int org[3] = {n->mPos[0], n->mPos[1], n->mPos[2]};
+                    uint32_t c[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                    for (int ax = 0; ax < 3; ax++) {
+                        const int lo = int(w[ax] - a.radius) / scs * scs, hi = int(w[ax] + 1 + a.radius) / scs * scs;
+                        for (int sv = lo; sv <= hi; sv += scs) c[ax][sv < org[ax] ? 0 : (sv >= org[ax] + res0 ? 2 : 1)]++;
+                    }
+                    const uint32_t m = a.nmask[leaf];
+#pragma unroll
+                    for (int d = 0; d < 27; d++)
+                        if ((m >> d) & 1u) binned += c[0][d % 3] * c[1][(d / 3) % 3] * c[2][d / 9];
+                }
+            }
+        }
+        key[i] = k;
+    }
+    if (a.len) {
+        // block total, added to one of LEN_PARTS counters (one address would
+        // serialise every block's atomic)
+        __shared__ uint32_t part[4];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) binned += __shfl_xor(binned, d, 64);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = binned;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long sum = u64(part[0]) + part[1] + part[2] + part[3];
+            if (sum) atomicAdd(reinterpret_cast<unsigned long long*>(a.len) + (bid % LEN_PARTS), sum);
         }
     }
-    key[i] = k;
+}
+
+// Sum of a grid's LEN_PARTS entry counters into its result word.
+struct SumArgs {
+    const u64* parts;
+    u64* out;
+};
+__device__ inline void k_sum_parts_body(const SumArgs& a, uint32_t, uint32_t) {
+    u64 v = threadIdx.x < LEN_PARTS ? a.parts[threadIdx.x] : 0ull;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (threadIdx.x == 0) *a.out = v;
+}
+GVDB_MULTI(k_sum_parts, SumArgs, 64)
+
+hipError_t launch_sum_parts_multi(const u64* const* parts, u64* const* out, int G, Arena* ar, hipStream_t s) {
+    std::vector<SumArgs> a(G);
+    std::vector<uint32_t> nb(G, 1u);
+    for (int g = 0; g < G; g++) a[g] = SumArgs{parts[g], out[g]};
+    MultiLaunch L;
+    hipError_t e = multi_setup(ar, a.data(), nb.data(), G, s, L);
+    if (e != hipSuccess || L.blocks == 0) return e;
+    hipLaunchKernelGGL(k_sum_parts, dim3(L.blocks), dim3(64), 0, s, a[0], L.map);
+    return hipGetLastError();
 }
 
 __device__ inline void k_bucket_scatter_body(const ScatterArgs& a, uint32_t bid, uint32_t) {
     const PointSrc& pts = a.pts;
     const float tx = a.tr[0], ty = a.tr[1], tz = a.tr[2];
     const uint32_t *key = a.key, *slot = a.slot, *boff = a.boff;
-    float* sorted = a.sorted;
+    float4* sorted = a.sorted;
     int i = bid * 256 + threadIdx.x;
     if (i >= pts.n) return;
     uint32_t k = key[i];
     if (k == 0xFFFFFFFFu) return;
     const float* p = point_at(pts, i);
     u64 e = u64(boff[k]) + slot[i];
-    sorted[3 * e] = p[0] + tx;
-    sorted[3 * e + 1] = p[1] + ty;
-    sorted[3 * e + 2] = p[2] + tz;
+    sorted[e] = make_float4(p[0] + tx, p[1] + ty, p[2] + tz, 0.0f);
 }
 
 static const int PULL_THREADS = 256;
@@ -652,7 +706,7 @@ __device__ inline void k_pull_body(const PullArgs& a, uint32_t bid, uint32_t) {
     const float radius = a.radius;
     const int* nbr27 = a.nbr;
     const uint32_t *bcnt = a.bcnt, *boff = a.boff, *mapping = a.mapping, *sc_off = a.sc_off;
-    const float* sorted = a.sorted;
+    const float4* sorted = a.sorted;
     uint32_t *sc_cnt = a.sc_cnt, *sc_use = a.sc_use;
     float* sc_pnt = a.sc_pnt;
     __shared__ PullShared S;
@@ -683,7 +737,8 @@ __device__ inline void k_pull_body(const PullArgs& a, uint32_t bid, uint32_t) {
             if (f < wend) {
                 const int c = S.owner[f - w0];
                 const u64 e = u64(S.cell_off[c]) + (f - S.cell_start[c]);
-                w0f = sorted[3 * e]; w1f = sorted[3 * e + 1]; w2f = sorted[3 * e + 2];
+                const float4 q = sorted[e];
+                w0f = q.x; w1f = q.y; w2f = q.z;
                 pull_mask(bn, scs, sd, radius, w0f, w1f, w2f, m, u);
             }
             const uint32_t z = m & ~u;
@@ -729,6 +784,274 @@ __device__ inline void k_pull_body(const PullArgs& a, uint32_t bid, uint32_t) {
         sc_cnt[sc0 + threadIdx.x] = S.run[threadIdx.x] + S.runz[threadIdx.x];
         sc_use[sc0 + threadIdx.x] = S.run[threadIdx.x];
     }
+}
+
+// ---------------------------------------------------- gather from buckets
+// GatherDensity without materialised subcell lists.  A voxel's value is the
+// falloff of the nearest point among those binned into its subcell; every
+// point that can raise the voxel (|p - v| < r) is binned into it, and any
+// other point gives r - |p - v| <= 0, so the nearest among all points around
+// the subcell gives the same value.  One wave per 4^3 subcell, eight per
+// 8^3 brick, with no workgroup barriers: the wave walks the buckets of the
+// (2R+1)^3 subcells around its own, keeps the points whose box reaches it
+// (staged in its own LDS list), and evaluates them row-parallel (lane =
+// 4-voxel x-row x 4 candidates per step).  The nearest point is decided on
+// float d^2, and whenever two candidates come within 2^-20 relative of each
+// other on the exact d^2 (FMA products + TwoSum), so the winner is the truly
+// nearest point (ties to the lower bucket position); its falloff is exact.
+static const int GB_THREADS = 512;
+static const int GB_CHUNK = 1024;   // candidates staged per pass (two per thread)
+
+__device__ inline void exact_d2(float px, float py, float pz, float vx, float vy, float vz, float& hi, float& lo) {
+#pragma clang fp contract(off)
+    float dx = px - vx, dy = py - vy, dz = pz - vz;
+    float hx = dx * dx, hy = dy * dy, hz = dz * dz;
+    float lx = __builtin_fmaf(dx, dx, -hx), ly = __builtin_fmaf(dy, dy, -hy), lz = __builtin_fmaf(dz, dz, -hz);
+    float s1 = hx + hy;
+    float b1 = s1 - hx;
+    float e1 = (hx - (s1 - b1)) + (hy - b1);
+    float s2 = s1 + hz;
+    float b2 = s2 - s1;
+    float e2 = (s1 - (s2 - b2)) + (hz - b2);
+    float tail = (e1 + e2) + (lx + ly + lz);
+    hi = s2 + tail;
+    lo = tail - (hi - s2);
+}
+
+// Is candidate a (index ia) nearer to v than b (index ib)?  Exact d^2, ties to the lower index.
+__device__ __noinline__ bool nearer_exact(float ax, float ay, float az, uint32_t ia, float bx, float by, float bz,
+                                          uint32_t ib, float vx, float vy, float vz) {
+    float ha, la, hb, lb;
+    exact_d2(ax, ay, az, vx, vy, vz, ha, la);
+    exact_d2(bx, by, bz, vx, vy, vz, hb, lb);
+    if (ha != hb) return ha < hb;
+    if (la != lb) return la < lb;
+    return ia < ib;
+}
+
+// Nearest candidate so far for one voxel: its bucket position (which also
+// fetches its coordinates when an exact comparison or the final falloff needs
+// them) and its float d^2 raised by NEAR_ULPS ulps.  Float d^2 (two FMAs on
+// exact differences) is within 1.5 ulps of the exact value, so a candidate
+// below the raised bound may be nearer; one also below the lowered bound
+// surely is, and the few in between are settled on the exact d^2.
+static const uint32_t NEAR_ULPS = 8;
+
+struct Best {
+    float bt;       // d^2 + NEAR_ULPS ulps (INFINITY while empty)
+    uint32_t id;
+};
+
+__device__ inline void best_offer(Best& B, float d2, float px, float py, float pz, uint32_t id, float vx, float vy,
+                                  float vz, const float4* pts) {
+    if (!(d2 < B.bt)) return;
+    bool take = B.id == 0xFFFFFFFFu;
+    if (!take) {
+        const float b = __uint_as_float(__float_as_uint(B.bt) - NEAR_ULPS);
+        take = d2 < __uint_as_float(__float_as_uint(b) - NEAR_ULPS) && b > 0.0f;
+        if (!take) {
+            const float4 q = pts[B.id];
+            take = nearer_exact(px, py, pz, id, q.x, q.y, q.z, B.id, vx, vy, vz);
+        }
+    }
+    if (take) {
+        B.bt = __uint_as_float(__float_as_uint(d2) + NEAR_ULPS);
+        B.id = id;
+    }
+}
+
+// Cell ranges of the (2 + 2R)^3 subcells around a brick and their exclusive
+// prefix (one cell per thread); returns the candidate count, 0 when the
+// brick is unreachable.
+struct GBShared {
+    uint32_t cell_off[GB_THREADS];
+    uint32_t cell_start[GB_THREADS];
+    uint16_t owner[PULL_CAP];
+    int nbr[27];
+    uint32_t wsum[GB_THREADS / 64];
+    float4 pt[GB_CHUNK];      // x, y, z, bucket position (bits)
+    uint8_t um[GB_CHUNK];
+    uint16_t lst[GB_THREADS / 64][GB_CHUNK];
+};
+
+__device__ inline uint32_t gb_setup(int brick, int R, const int* nbr27, const uint32_t* bcnt, const uint32_t* boff,
+                                    GBShared& S) {
+    const int sd = 2, span = sd + 2 * R, ncell = span * span * span;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < 27) S.nbr[threadIdx.x] = nbr27[u64(brick) * 27 + threadIdx.x];
+    __syncthreads();
+    const bool ok = S.nbr[13] >= 0;
+    const int c = threadIdx.x;
+    uint32_t cnt = 0, off = 0;
+    if (c < ncell && ok) {
+        int cx = c % span - R, cy = (c / span) % span - R, cz = c / (span * span) - R;
+        int dx = cx < 0 ? -1 : (cx >= sd ? 1 : 0), dy = cy < 0 ? -1 : (cy >= sd ? 1 : 0), dz = cz < 0 ? -1 : (cz >= sd ? 1 : 0);
+        int nb = S.nbr[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+        if (nb >= 0) {
+            int lx = cx - dx * sd, ly = cy - dy * sd, lz = cz - dz * sd;
+            u64 b = u64(nb) * 8 + u64((lz * sd + ly) * sd + lx);
+            cnt = bcnt[b];
+            off = boff[b];
+        }
+    }
+    if (c < ncell) S.cell_off[c] = off;
+    uint32_t x = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) S.wsum[wave] = x;
+    __syncthreads();
+    uint32_t pre = x - cnt, total = 0;
+    for (int w = 0; w < GB_THREADS / 64; w++) {
+        if (w < wave) pre += S.wsum[w];
+        total += S.wsum[w];
+    }
+    if (c < ncell) S.cell_start[c] = pre;
+    return total;
+}
+
+template <typename AT>
+__device__ inline void k_bucket_gather_body(const BucketGatherArgs& a, uint32_t bid, uint32_t) {
+    __shared__ GBShared S;
+    const Tree& t = a.t;
+    const int brick = int(bid);
+    const Node* bn = t.nodes[0] + brick;
+    if (bn->mParent == ID_UNDEF64) return;        // unlinked leaves have no subcells
+    const float r = a.radius;
+    const int R = a.R, span = 2 + 2 * R, ncell = span * span * span;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = lane & 15, g = lane >> 4;
+    const int ry = row & 3, rz = row >> 2;
+    // this wave's subcell (x fastest, as CalcSubcellPos numbers them)
+    const int sx = bn->mPos[0] + (wave & 1) * 4, sy = bn->mPos[1] + ((wave >> 1) & 1) * 4,
+              sz = bn->mPos[2] + (wave >> 2) * 4;
+    const float fx[4] = {float(sx), float(sx + 1), float(sx + 2), float(sx + 3)};
+    const float fy = float(sy + ry), fz = float(sz + rz);
+    Best B[4];
+#pragma unroll
+    for (int x = 0; x < 4; x++) B[x] = Best{INFINITY, 0xFFFFFFFFu};
+    const uint32_t T = gb_setup(brick, R, a.nbr, a.bcnt, a.boff, S);
+    __syncthreads();
+    const int org[3] = {bn->mPos[0], bn->mPos[1], bn->mPos[2]};
+    const u64 lower = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    for (uint32_t w0 = 0; w0 < T; w0 += PULL_CAP) {
+        // cell of every candidate in the window
+        for (int c = threadIdx.x; c < ncell; c += GB_THREADS) {
+            uint32_t beg = S.cell_start[c];
+            uint32_t end = (c + 1 < ncell) ? S.cell_start[c + 1] : T;
+            for (uint32_t f = max(beg, w0); f < end && f < w0 + PULL_CAP; f++) S.owner[f - w0] = uint16_t(c);
+        }
+        __syncthreads();
+        const uint32_t wend = min(T, w0 + PULL_CAP);
+        for (uint32_t c0 = w0; c0 < wend; c0 += GB_CHUNK) {
+            // candidates of this chunk: positions and the subcells of the brick each can reach
+            for (int q = threadIdx.x; q < GB_CHUNK; q += GB_THREADS) {
+                const uint32_t f = c0 + q;
+                uint32_t um = 0;
+                if (f < wend) {
+                    const int c = S.owner[f - w0];
+                    const uint32_t pos = S.cell_off[c] + (f - S.cell_start[c]);
+                    const float4 p = a.sorted[pos];
+                    const float pp[3] = {p.x, p.y, p.z};
+                    uint32_t am[3];
+#pragma unroll
+                    for (int ax = 0; ax < 3; ax++) {
+                        // subcell i of this axis can be reached when some voxel of
+                        // [org + 4i, org + 4i + 3] lies within r (a superset test)
+                        const float lo = pp[ax] - r, hi = pp[ax] + r;
+                        am[ax] = (lo < float(org[ax] + 3) && hi > float(org[ax]) ? 1u : 0u) |
+                                 (lo < float(org[ax] + 7) && hi > float(org[ax] + 4) ? 2u : 0u);
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; k++)
+                        if ((am[0] >> (k & 1)) & (am[1] >> ((k >> 1) & 1)) & (am[2] >> (k >> 2)) & 1u) um |= 1u << k;
+                    S.pt[q] = make_float4(p.x, p.y, p.z, __uint_as_float(pos));
+                }
+                S.um[q] = uint8_t(um);
+            }
+            __syncthreads();
+            // this wave's candidates, in candidate order
+            uint32_t n = 0;
+            const int ngrp = int((min(uint32_t(GB_CHUNK), wend - c0) + 63) / 64);
+            for (int grp = 0; grp < ngrp; grp++) {
+                const bool on = (S.um[grp * 64 + lane] >> wave) & 1u;
+                const u64 bal = __ballot(on);
+                if (on) S.lst[wave][n + uint32_t(__popcll(bal & lower))] = uint16_t(grp * 64 + lane);
+                n += uint32_t(__popcll(bal));
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t k = 0; k < n; k += 4) {
+                const uint32_t j = k + uint32_t(g);
+                if (j < n) {
+                    const float4 pe = S.pt[S.lst[wave][j]];
+                    const float px = pe.x, py = pe.y, pz = pe.z;
+                    const uint32_t id = __float_as_uint(pe.w);
+                    const float dy = py - fy, dz = pz - fz;
+                    const float dyz = __builtin_fmaf(dy, dy, dz * dz);
+#pragma unroll
+                    for (int x = 0; x < 4; x++) {
+                        const float dx = px - fx[x];
+                        best_offer(B[x], __builtin_fmaf(dx, dx, dyz), px, py, pz, id, fx[x], fy, fz, a.sorted);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // merge the four candidate groups of each row
+#pragma unroll
+    for (int sh = 16; sh <= 32; sh <<= 1) {
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+            const float obt = __shfl_xor(B[x].bt, sh, 64);
+            const uint32_t oi = __shfl_xor(B[x].id, sh, 64);
+            if (oi != 0xFFFFFFFFu) {
+                const float4 q = a.sorted[oi];
+                best_offer(B[x], __uint_as_float(__float_as_uint(obt) - NEAR_ULPS), q.x, q.y, q.z, oi, fx[x], fy, fz,
+                           a.sorted);
+            }
+        }
+    }
+    if (g != 0) return;
+    AT* atlas = reinterpret_cast<AT*>(a.atlas);
+    const int vy = sy + ry, vz = sz + rz;
+    const u64 aoff = atlas_offset(t, bn->mValue[0] + (sx - bn->mPos[0]), bn->mValue[1] + (vy - bn->mPos[1]),
+                                  bn->mValue[2] + (vz - bn->mPos[2]));
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+        float val = a.accumulate ? load_atlas<AT>(atlas, aoff + x) : 0.0f;
+        if (B[x].id != 0xFFFFFFFFu) {
+            const float4 q = a.sorted[B[x].id];
+            val = fmaxf(val, falloff_accurate(q.x, q.y, q.z, sx + x, vy, vz, r));
+        }
+        store_atlas<AT>(atlas, aoff + x, val);
+    }
+}
+GVDB_MULTI_T(k_bucket_gather, typename AT, AT, BucketGatherArgs, GB_THREADS)
+
+hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* ar, hipStream_t s) {
+    for (int half = 0; half < 2; half++) {
+        std::vector<BucketGatherArgs> sel;
+        std::vector<uint32_t> nb;
+        for (int g = 0; g < G; g++) {
+            if (a[g].t.res[0] != 8 || a[g].R < 1 || a[g].R > 2) return hipErrorInvalidValue;
+            if ((a[g].atlas_type == T_HALF) != bool(half) || a[g].nleaves <= 0) continue;
+            sel.push_back(a[g]);
+            nb.push_back(uint32_t(a[g].nleaves));
+        }
+        if (sel.empty()) continue;
+        MultiLaunch L;
+        hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
+        if (e != hipSuccess) return e;
+        if (L.blocks == 0) continue;
+        if (half) hipLaunchKernelGGL(k_bucket_gather<__half>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
+        else hipLaunchKernelGGL(k_bucket_gather<float>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 GVDB_MULTI(k_bucket_points, BucketArgs, 256)

@@ -149,6 +149,7 @@ public:
                             int* len, const Exec& ex, int* failed);
     static int multi_gather(Volume* const* v, int G, int scs, const int* len, const float* radius, int chanD, bool accum,
                             const Exec& ex, int* failed);
+    static int multi_materialize(Volume* const* v, int G, const Exec& ex, int* failed);
     static int multi_apron(Volume* const* v, int G, int chan, float boundval, const Exec& ex, int* failed);
     static int multi_render(Volume* const* v, int G, const int* shading, const int* chan, const int* buf, const Exec& ex,
                             int* failed);
@@ -183,7 +184,7 @@ private:
     Channel chan_[8];
     DevBuf amap_;
     u64 amap_max_ = 0;
-    DevBuf nbr_;
+    DevBuf nbr_, nmask_, lenparts_;
     unsigned nbr_gen_ = ~0u;
 
     // points
@@ -195,6 +196,10 @@ private:
     DevBuf bkt_key_, bkt_slot_, bkt_cnt_, bkt_off_, sorted_;
     int sc_len_ = 0, num_sc_ = 0, sc_bricks_ = 0, sc_map_n_ = 0;
     bool last_insert_pull_ = false;
+    bool entries_stale_ = false;   // pull insert done, subcell tables not built yet
+    int bkt_scs_ = 0, bkt_R_ = 0;
+    float bkt_radius_ = 0.f;
+    unsigned bkt_gen_ = ~0u;       // topology generation of the current buckets
 
     // topology scratch
     DevBuf small_;       // bbox keys, counts

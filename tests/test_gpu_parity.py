@@ -212,3 +212,30 @@ def test_translated_points(gpu, trans):
     mask = H.used_brick_mask(b.shape, o.atlas_map(), o.atlas_info()["cnt"])
     _, rel, zeros = H.density_errors(a[mask], b[mask])
     assert zeros == 0 and rel <= REL_TOL
+
+
+def test_fp16_channel_vs_fp64(gpu):
+    """Optional FP16 atlas channel: the gather and apron compute in FP32 and
+    store FP16; each voxel is within FP16 rounding of the FP64 oracle
+    (|a - b| <= 2^-10 |b| + 2^-24), and the emission-absorption image from the
+    FP16 atlas (sampled in FP32) stays within 2e-3 RMSE of the FP64 march."""
+    import oracle as O
+    from gvdb_amd import workloads as W
+    pts = W.clusters(30000, k=4, seed=91, space=400.0, sig=(10.0, 30.0))
+    v, n = H.gpu_volume(gpu, pts, 2.0, atlas_type=gpu.T_HALF)
+    o, m = H.oracle_volume(O, pts, 2.0, precision=64)
+    assert n == m
+    H.assert_pools_equal(v, o)
+    a = v.atlas(0, np.float16).astype(np.float64)
+    b = o.atlas(64)
+    mask = H.used_brick_mask(b.shape, o.atlas_map(), o.atlas_info()["cnt"])
+    d = np.abs(a[mask] - b[mask])
+    assert np.all(d <= 2.0 ** -10 * np.abs(b[mask]) + 2.0 ** -24)
+    assert np.count_nonzero((b[mask] == 0) & (a[mask] != 0)) == 0
+    sc = W.ea_scene(96, 80, centre=tuple(float(x) for x in pts.mean(0)), dist=500.0)
+    v.apply_scene(sc)
+    v.AddRenderBuf(0, sc["width"], sc["height"], 16)
+    v.Render(gpu.SHADE_VOLUME, 0, 0)
+    img = v.ReadRenderBuf(0, sc["width"], sc["height"], 16)
+    ref = o.render(v.scene_block(sc["width"], sc["height"]), O.linear_tf(*sc["tf"]), 7, 64)
+    assert H.rmse(img, ref) <= 2e-3
