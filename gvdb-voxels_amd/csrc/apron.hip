@@ -44,6 +44,13 @@ __device__ inline void k_neighbors_body(const NbrArgs& a, uint32_t bid, uint32_t
     }
     nbr[g] = v;
     if (a.mask && v >= 0) atomicOr(&a.mask[b], 1u << d);
+    if (a.base) {
+        // atlas offset of the neighbour's padded brick (d = 13: the brick's own)
+        const int m = d == 13 ? b : v;
+        const Node* mn = t.nodes[0] + (m >= 0 ? m : 0);
+        const bool live = m >= 0 && (d != 13 || mn->mFlags) && mn->mValue[0] >= 0;
+        a.base[g] = live ? (long long)(slot_of(t, mn->mValue) * u64(t.brick_vox)) : -1ll;
+    }
 }
 
 GVDB_MULTI(k_neighbors, NbrArgs, 256)
@@ -76,8 +83,8 @@ __device__ inline void st<__half>(__half* a, u64 o, float v) { a[o] = __float2ha
 // and z = 9 faces (100 each), then the 36-voxel rings of z = 1..8.
 template <typename AT>
 __device__ inline void k_apron_body(const ApronArgs& ap, uint32_t bid, uint32_t) {
+    __shared__ long long base[27];
     const Tree& t = ap.t;
-    const int* nbr = ap.nbr;
     AT* atlas = reinterpret_cast<AT*>(ap.atlas);
     const float boundval = ap.boundval;
     const int b = int(bid);
@@ -86,9 +93,10 @@ __device__ inline void k_apron_body(const ApronArgs& ap, uint32_t bid, uint32_t)
     const int w = t.brickwid;          // 8
     const int face = br * br;
     const int ring = br * br - w * w;
-    if (a >= 2 * face + w * ring) return;
-    const Node* n = t.nodes[0] + b;
-    if (!n->mFlags || n->mValue[0] < 0) return;
+    // atlas offsets of the brick and its 26 neighbours (-1: none), one load
+    if (a < 27) base[a] = ap.base[u64(b) * 27 + a];
+    __syncthreads();
+    if (a >= 2 * face + w * ring || base[13] < 0) return;
     int x, y, z;
     if (a < 2 * face) {
         z = a < face ? 0 : br - 1;
@@ -106,16 +114,13 @@ __device__ inline void k_apron_body(const ApronArgs& ap, uint32_t bid, uint32_t)
     int dx = x == 0 ? -1 : (x == br - 1 ? 1 : 0);
     int dy = y == 0 ? -1 : (y == br - 1 ? 1 : 0);
     int dz = z == 0 ? -1 : (z == br - 1 ? 1 : 0);
-    int d = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-    int nb = nbr[u64(b) * 27 + d];
-    u64 own = slot_of(t, n->mValue) * u64(t.brick_vox) + u64((z * br + y) * br + x);
+    const long long nbase = base[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+    const u64 own = u64(base[13]) + u64((z * br + y) * br + x);
     float v = boundval;
-    if (nb >= 0) {
-        const Node* m = t.nodes[0] + nb;
+    if (nbase >= 0) {
         // interior coordinate inside the neighbour, shifted by one for the apron
         int nx = (x - 1) - dx * w + 1, ny = (y - 1) - dy * w + 1, nz = (z - 1) - dz * w + 1;
-        u64 src = slot_of(t, m->mValue) * u64(t.brick_vox) + u64((nz * br + ny) * br + nx);
-        v = ld<AT>(atlas, src);
+        v = ld<AT>(atlas, u64(nbase) + u64((nz * br + ny) * br + nx));
     }
     st<AT>(atlas, own, v);
 }

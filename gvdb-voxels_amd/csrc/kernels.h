@@ -265,12 +265,13 @@ struct NbrArgs {
     int* nbr;
     int n;
     uint32_t* mask;       // optional, zeroed: bit d set when nbr[b*27 + d] >= 0
+    long long* base;      // optional: atlas offset of each neighbour's brick (d = 13: own), -1 if none
 };
 hipError_t launch_neighbors_multi(const NbrArgs* a, int G, Arena* ar, hipStream_t s);
 
 struct ApronArgs {
     Tree t;
-    const int* nbr;
+    const long long* base;   // neighbour brick offsets from the neighbour pass
     void* atlas;
     int atlas_type;
     float boundval;

@@ -32,6 +32,16 @@ __device__ inline int multi_grid(const MultiMap& m, uint32_t b) {
     return lo;
 }
 
+// Workgroups are handed to the 8 XCDs of the device in turn; this maps the
+// b-th workgroup of a launch to the brick order so that each XCD (with its
+// own L2) works through one contiguous eighth of the bricks, where the
+// neighbours of a brick (nearby in pool order) share its cache.
+__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t n) {
+    const uint32_t q = n >> 3;
+    if (b >= (q << 3)) return b;
+    return (b & 7u) * q + (b >> 3);
+}
+
 #define GVDB_MULTI(NAME, A, THREADS)                                                             \
     __global__ __launch_bounds__(THREADS) void NAME(A a0, MultiMap m) {                          \
         if (m.G <= 1) {                                                                          \

@@ -455,7 +455,9 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
                 if (v->nbr_gen_ != v->topo_gen_) {
                     XCHK(v, v->nmask_.ensure(size_t(pl.numMap) * 4));
                     nf.push_back(FillArgs{v->nmask_.p, pl.numMap, 0, 0});
-                    na.push_back(NbrArgs{pl.t, v->nbr_.as<int>(), int(pl.numMap), v->nmask_.as<uint32_t>()});
+                    XCHK(v, v->nbase_.ensure(size_t(pl.numMap) * 27 * 8));
+                    na.push_back(NbrArgs{pl.t, v->nbr_.as<int>(), int(pl.numMap), v->nmask_.as<uint32_t>(),
+                                         v->nbase_.as<long long>()});
                     v->nbr_gen_ = v->topo_gen_;
                 }
                 fa.push_back(FillArgs{v->bkt_cnt_.p, pl.numMap * u64(pl.spb), 0, 0});
@@ -693,10 +695,11 @@ int Volume::multi_apron(Volume* const* vs, int G, int c, float boundval, const E
             XCHK(v, v->nbr_.ensure(size_t(bricks) * 27 * 4));
             XCHK(v, v->nmask_.ensure(size_t(bricks) * 4));
             nf.push_back(FillArgs{v->nmask_.p, u64(bricks), 0, 0});
-            na.push_back(NbrArgs{t, v->nbr_.as<int>(), bricks, v->nmask_.as<uint32_t>()});
+            XCHK(v, v->nbase_.ensure(size_t(bricks) * 27 * 8));
+            na.push_back(NbrArgs{t, v->nbr_.as<int>(), bricks, v->nmask_.as<uint32_t>(), v->nbase_.as<long long>()});
             v->nbr_gen_ = v->topo_gen_;
         }
-        aa.push_back(ApronArgs{t, v->nbr_.as<int>(), v->chan_[c].data.p, v->chan_[c].type, boundval, bricks});
+        aa.push_back(ApronArgs{t, v->nbase_.as<long long>(), v->chan_[c].data.p, v->chan_[c].type, boundval, bricks});
     }
     if (aa.empty()) return GVDB_OK;
     Volume* v0 = vs[0];

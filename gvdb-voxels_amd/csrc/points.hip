@@ -176,13 +176,24 @@ hipError_t launch_subcell_pos_multi(const SubPosArgs* a, int G, Arena* ar, hipSt
 }
 
 // ---------------------------------------------------------- GatherDensity
-// radius - |p - v| evaluated with an error-free d^2 (FMA products and TwoSum)
-// and r^2 - d^2 divided by r + |p - v|, so the result keeps its relative
-// accuracy down to the edge of the falloff.  p - v is exact in float for the
-// positions of interest (|p - v| < 2r, p >= 0).
-__device__ __noinline__ float falloff_accurate(float px, float py, float pz, int vx, int vy, int vz, float r) {
+// d^2 = |p - v|^2 as an unevaluated sum hi + lo, accurate to about 2^-44
+// relative: the differences are split into float + error (TwoSum, exact),
+// the squares into product + FMA residual, and the parts summed with TwoSum.
+// The dropped terms (error squared, rounding of the small parts) stay below
+// 2^-44 d^2.  The differences alone are exact in float only when p is not
+// much smaller than v, which fails near the origin.
+__device__ inline float two_diff(float a, float b, float& e) {
 #pragma clang fp contract(off)
-    float dx = px - float(vx), dy = py - float(vy), dz = pz - float(vz);
+    const float s = a - b;
+    const float bb = s - a;
+    e = (a - (s - bb)) - (b + bb);
+    return s;
+}
+
+__device__ inline void exact_d2(float px, float py, float pz, float vx, float vy, float vz, float& hi, float& lo) {
+#pragma clang fp contract(off)
+    float ex, ey, ez;
+    const float dx = two_diff(px, vx, ex), dy = two_diff(py, vy, ey), dz = two_diff(pz, vz, ez);
     float hx = dx * dx, hy = dy * dy, hz = dz * dz;
     float lx = __builtin_fmaf(dx, dx, -hx), ly = __builtin_fmaf(dy, dy, -hy), lz = __builtin_fmaf(dz, dz, -hz);
     float s1 = hx + hy;
@@ -191,9 +202,18 @@ __device__ __noinline__ float falloff_accurate(float px, float py, float pz, int
     float s2 = s1 + hz;
     float b2 = s2 - s1;
     float e2 = (s1 - (s2 - b2)) + (hz - b2);
-    float tail = (e1 + e2) + (lx + ly + lz);
-    float hi = s2 + tail;
-    float lo = tail - (hi - s2);
+    float cross = 2.0f * __builtin_fmaf(dx, ex, __builtin_fmaf(dy, ey, dz * ez));
+    float tail = (e1 + e2) + ((lx + ly + lz) + cross);
+    hi = s2 + tail;
+    lo = tail - (hi - s2);
+}
+
+// radius - |p - v| from the accurate d^2: r^2 - d^2 divided by r + |p - v|,
+// so the result keeps its relative accuracy down to the edge of the falloff.
+__device__ __noinline__ float falloff_accurate(float px, float py, float pz, int vx, int vy, int vz, float r) {
+#pragma clang fp contract(off)
+    float hi, lo;
+    exact_d2(px, py, pz, float(vx), float(vy), float(vz), hi, lo);
     float r2h = r * r;
     float r2l = __builtin_fmaf(r, r, -r2h);
     float num = (r2h - hi) + (r2l - lo);
@@ -802,23 +822,7 @@ __device__ inline void k_pull_body(const PullArgs& a, uint32_t bid, uint32_t) {
 static const int GB_THREADS = 512;
 static const int GB_CHUNK = 1024;   // candidates staged per pass (two per thread)
 
-__device__ inline void exact_d2(float px, float py, float pz, float vx, float vy, float vz, float& hi, float& lo) {
-#pragma clang fp contract(off)
-    float dx = px - vx, dy = py - vy, dz = pz - vz;
-    float hx = dx * dx, hy = dy * dy, hz = dz * dz;
-    float lx = __builtin_fmaf(dx, dx, -hx), ly = __builtin_fmaf(dy, dy, -hy), lz = __builtin_fmaf(dz, dz, -hz);
-    float s1 = hx + hy;
-    float b1 = s1 - hx;
-    float e1 = (hx - (s1 - b1)) + (hy - b1);
-    float s2 = s1 + hz;
-    float b2 = s2 - s1;
-    float e2 = (s1 - (s2 - b2)) + (hz - b2);
-    float tail = (e1 + e2) + (lx + ly + lz);
-    hi = s2 + tail;
-    lo = tail - (hi - s2);
-}
-
-// Is candidate a (index ia) nearer to v than b (index ib)?  Exact d^2, ties to the lower index.
+// Is candidate a (index ia) nearer to v than b (index ib)?  Accurate d^2, ties to the lower index.
 __device__ __noinline__ bool nearer_exact(float ax, float ay, float az, uint32_t ia, float bx, float by, float bz,
                                           uint32_t ib, float vx, float vy, float vz) {
     float ha, la, hb, lb;
@@ -831,8 +835,8 @@ __device__ __noinline__ bool nearer_exact(float ax, float ay, float az, uint32_t
 
 // Nearest candidate so far for one voxel: its bucket position (which also
 // fetches its coordinates when an exact comparison or the final falloff needs
-// them) and its float d^2 raised by NEAR_ULPS ulps.  Float d^2 (two FMAs on
-// exact differences) is within 1.5 ulps of the exact value, so a candidate
+// them) and its float d^2 raised by NEAR_ULPS ulps.  Float d^2 (rounded
+// differences, two FMAs) is within 3.5 ulps of the exact value, so a candidate
 // below the raised bound may be nearer; one also below the lowered bound
 // surely is, and the few in between are settled on the exact d^2.
 static const uint32_t NEAR_ULPS = 8;

@@ -184,7 +184,7 @@ private:
     Channel chan_[8];
     DevBuf amap_;
     u64 amap_max_ = 0;
-    DevBuf nbr_, nmask_, lenparts_;
+    DevBuf nbr_, nmask_, nbase_, lenparts_;
     unsigned nbr_gen_ = ~0u;
 
     // points

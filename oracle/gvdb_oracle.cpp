@@ -1147,8 +1147,10 @@ static bool childOn(Vol& v, Node* node, const int p[3], int lev, int& child) {
 
 // rayCast: kernels/cuda_gvdb_raycast.cuh:543-611 with rayBoxIntersect
 // (kernels/cuda_gvdb_geom.cuh:85-98).  mode 0 = deep (EA), 1 = trilinear surface.
+// Returns false when the walk stopped at MAX_ITER (the reference's cap, where
+// the image depends on the exact step sequence).
 template <typename T>
-static void rayCast(const RenderCtx<T>& c, V3<T> pos, V3<T> dir, V3<T>& hit, V3<T>& norm, T clr[4], int mode) {
+static bool rayCast(const RenderCtx<T>& c, V3<T> pos, V3<T> dir, V3<T>& hit, V3<T>& norm, T clr[4], int mode) {
     Vol& v = *c.v;
     int nodeid[8];
     T tMax[8];
@@ -1163,7 +1165,7 @@ static void rayCast(const RenderCtx<T>& c, V3<T> pos, V3<T> dir, V3<T>& hit, V3<
     ht[6] = (ht[6] < 0) ? T(0) : ht[6];
     V3<T> tStart = mk(ht[6], ht[7], (ht[7] < ht[6] || ht[7] < 0) ? T(NOHIT) : T(0));
     Node* node = nodeAt(v, lev, 0);
-    if (tStart.z == T(NOHIT)) return;
+    if (tStart.z == T(NOHIT)) return true;
     tStart.x += c.eps;
     tMax[lev] = tStart.y - c.eps;
     HDDA<T> dda;
@@ -1171,8 +1173,9 @@ static void rayCast(const RenderCtx<T>& c, V3<T> pos, V3<T> dir, V3<T>& hit, V3<
     auto vminOf = [&](Node* n) { return mk(T(n->mPos[0]), T(n->mPos[1]), T(n->mPos[2])); };
     auto vdelOf = [&](int l) { return T(v.range[l] / v.res[l]); };
     dda.Prepare(vminOf(node), vdelOf(lev));
-    for (int it = 0; it < MAX_ITER && lev > 0 && lev <= v.top_lev && dda.p[0] >= 0 && dda.p[1] >= 0 && dda.p[2] >= 0 &&
-                     dda.p[0] <= v.res[lev] && dda.p[1] <= v.res[lev] && dda.p[2] <= v.res[lev]; it++) {
+    int it = 0;
+    for (; it < MAX_ITER && lev > 0 && lev <= v.top_lev && dda.p[0] >= 0 && dda.p[1] >= 0 && dda.p[2] >= 0 &&
+           dda.p[0] <= v.res[lev] && dda.p[1] <= v.res[lev] && dda.p[2] <= v.res[lev]; it++) {
         dda.Next();
         int child;
         if (childOn<T>(v, node, dda.p, lev, child)) {
@@ -1180,8 +1183,8 @@ static void rayCast(const RenderCtx<T>& c, V3<T> pos, V3<T> dir, V3<T>& hit, V3<
                 dda.t.x += c.eps;
                 if (mode == 0) rayDeepBrick(c, child, dda.t, pos, dir, hit, clr);
                 else raySurfTriBrick(c, child, dda.t, pos, dir, hit, norm);
-                if (clr[3] <= 0) { clr[3] = 0; return; }
-                if (hit.z != T(NOHIT)) return;
+                if (clr[3] <= 0) { clr[3] = 0; return true; }
+                if (hit.z != T(NOHIT)) return true;
                 dda.Step();
             } else {
                 lev--;
@@ -1202,13 +1205,15 @@ static void rayCast(const RenderCtx<T>& c, V3<T> pos, V3<T> dir, V3<T>& hit, V3<
             }
         }
     }
+    return it < MAX_ITER;
 }
 
 // gvdbRayDeep / gvdbRaySurfaceTrilinear + performPhongShading:
 // kernels/cuda_gvdb_module.cu:38-78, :103-120.  Output is float RGBA before
 // the reference's uchar conversion.
 template <typename T>
-static void renderT(Vol& v, const std::vector<T>& atlas, const ScnBlock& s, const float* tf, int shading, float* out) {
+static void renderT(Vol& v, const std::vector<T>& atlas, const ScnBlock& s, const float* tf, int shading, float* out,
+                    uint8_t* capped) {
     RenderCtx<T> c;
     c.v = &v;
     c.atlas = &atlas;
@@ -1236,17 +1241,18 @@ static void renderT(Vol& v, const std::vector<T>& atlas, const ScnBlock& s, cons
                           fx * T(s.camu[2]) + fy * T(s.camv[2]) + T(s.cams[2]));
             V3<T> rdir = normalize(mm(s.invxrot, vv));
             T res[4];
+            bool whole = true;
             if (shading == 7) {
                 T clr[4] = {0, 0, 0, 1};
                 V3<T> hit = mk(T(0), T(0), T(NOHIT)), norm = mk(T(0), T(0), T(0));
-                rayCast(c, rpos, rdir, hit, norm, clr, 0);
+                whole = rayCast(c, rpos, rdir, hit, norm, clr, 0);
                 T a = 1 - clr[3];
                 for (int k = 0; k < 3; k++) res[k] = c.back[k] + a * (clr[k] - c.back[k]);
                 res[3] = a;
             } else {
                 T clr[4] = {1, 1, 1, 1};
                 V3<T> hit = mk(T(NOHIT), T(NOHIT), T(NOHIT)), norm = mk(T(0), T(0), T(0));
-                rayCast(c, rpos, rdir, hit, norm, clr, 1);
+                whole = rayCast(c, rpos, rdir, hit, norm, clr, 1);
                 if (hit.z == T(NOHIT)) {
                     for (int k = 0; k < 4; k++) res[k] = c.back[k];
                 } else {
@@ -1256,7 +1262,7 @@ static void renderT(Vol& v, const std::vector<T>& atlas, const ScnBlock& s, cons
                     if (c.shadowamt > 0) {
                         V3<T> h2 = mk(T(0), T(0), T(NOHIT)), n2 = mk(T(0), T(0), T(0));
                         T c2[4] = {0, 0, 0, 1};
-                        rayCast(c, hit + norm * c.shadowbias, ld, h2, n2, c2, 1);
+                        whole = rayCast(c, hit + norm * c.shadowbias, ld, h2, n2, c2, 1) && whole;
                         diff = (h2.z == T(NOHIT)) ? diff : diff * (1 - c.shadowamt);
                     }
                     for (int k = 0; k < 3; k++) res[k] = clr[k] * (diff + amb);
@@ -1264,6 +1270,7 @@ static void renderT(Vol& v, const std::vector<T>& atlas, const ScnBlock& s, cons
                 }
             }
             for (int k = 0; k < 4; k++) out[4 * (size_t(y) * W + x) + k] = float(res[k]);
+            if (capped) capped[size_t(y) * W + x] = whole ? 0 : 1;
         }
 }
 
@@ -1456,12 +1463,13 @@ void orc_linear_tf(float t0, float t1, const float* a, const float* b, float* lu
     }
 }
 
-int orc_render(void* h, const float* scn, const float* tf, int shading, int precision, float* out) {
+// capped (may be null): one byte per pixel, 1 where a ray stopped at MAX_ITER
+int orc_render(void* h, const float* scn, const float* tf, int shading, int precision, float* out, uint8_t* capped) {
     Vol& v = *static_cast<Vol*>(h);
     const ScnBlock& s = *reinterpret_cast<const ScnBlock*>(scn);
     auto t0 = std::chrono::steady_clock::now();
-    if (precision == 64) renderT<double>(v, v.atlasD, s, tf, shading, out);
-    else renderT<float>(v, v.atlasF, s, tf, shading, out);
+    if (precision == 64) renderT<double>(v, v.atlasD, s, tf, shading, out, capped);
+    else renderT<float>(v, v.atlasF, s, tf, shading, out, capped);
     v.stage_ms[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;
 }

@@ -51,7 +51,7 @@ def lib():
             "orc_gather_density": (i, [vp, i, f, i, i]), "orc_update_apron": (i, [vp, f, i]),
             "orc_camera_scn": (None, [fp, i, i, fp]), "orc_scn_floats": (i, []),
             "orc_linear_tf": (None, [f, f, fp, fp, fp]),
-            "orc_render": (i, [vp, fp, fp, i, i, fp]), "orc_stage_ms": (ctypes.c_double, [vp, i]),
+            "orc_render": (i, [vp, fp, fp, i, i, fp, vp]), "orc_stage_ms": (ctypes.c_double, [vp, i]),
             "orc_max_threads": (i, []), "orc_set_threads": (None, [i]),
         }
         for name, (res, args) in sig.items():
@@ -179,13 +179,16 @@ class Oracle:
         return lib().orc_update_apron(self.h, float(boundval), precision)
 
     # render ---------------------------------------------------------------
-    def render(self, scn, tf, shading, precision=64):
+    def render(self, scn, tf, shading, precision=64, capped=False):
+        """Float RGBA image; with capped=True also a (h, w) bool mask of the
+        rays that stopped at the reference's MAX_ITER walk cap."""
         scn = np.ascontiguousarray(scn, dtype=np.float32)
         w, h = int(scn[:2].view(np.int32)[0]), int(scn[:2].view(np.int32)[1])
         out = np.zeros((h, w, 4), np.float32)
+        cap = np.zeros((h, w), np.uint8)
         tf = np.ascontiguousarray(tf, dtype=np.float32)
-        lib().orc_render(self.h, _p(scn), _p(tf), int(shading), int(precision), _p(out))
-        return out
+        lib().orc_render(self.h, _p(scn), _p(tf), int(shading), int(precision), _p(out), _p(cap))
+        return (out, cap.astype(bool)) if capped else out
 
     def stage_ms(self, i):
         return lib().orc_stage_ms(self.h, i)
