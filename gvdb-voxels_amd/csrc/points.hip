@@ -210,7 +210,7 @@ __device__ inline void exact_d2(float px, float py, float pz, float vx, float vy
 
 // radius - |p - v| from the accurate d^2: r^2 - d^2 divided by r + |p - v|,
 // so the result keeps its relative accuracy down to the edge of the falloff.
-__device__ __noinline__ float falloff_accurate(float px, float py, float pz, int vx, int vy, int vz, float r) {
+__device__ inline float falloff_inl(float px, float py, float pz, int vx, int vy, int vz, float r) {
 #pragma clang fp contract(off)
     float hi, lo;
     exact_d2(px, py, pz, float(vx), float(vy), float(vz), hi, lo);
@@ -219,6 +219,10 @@ __device__ __noinline__ float falloff_accurate(float px, float py, float pz, int
     float num = (r2h - hi) + (r2l - lo);
     float c = __builtin_sqrtf(hi);
     return num / (r + c);
+}
+
+__device__ __noinline__ float falloff_accurate(float px, float py, float pz, int vx, int vy, int vz, float r) {
+    return falloff_inl(px, py, pz, vx, vy, vz, r);
 }
 
 // One workgroup per brick, one thread per interior voxel; the brick's
@@ -878,8 +882,9 @@ struct GBShared {
     uint16_t lst[GB_THREADS / 64][GB_CHUNK];
 };
 
+template <class SH>
 __device__ inline uint32_t gb_setup(int brick, int R, const int* nbr27, const uint32_t* bcnt, const uint32_t* boff,
-                                    GBShared& S) {
+                                    SH& S) {
     const int sd = 2, span = sd + 2 * R, ncell = span * span * span;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (threadIdx.x < 27) S.nbr[threadIdx.x] = nbr27[u64(brick) * 27 + threadIdx.x];
@@ -1035,13 +1040,193 @@ __device__ inline void k_bucket_gather_body(const BucketGatherArgs& a, uint32_t 
 }
 GVDB_MULTI_T(k_bucket_gather, typename AT, AT, BucketGatherArgs, GB_THREADS)
 
+// GatherDensity by scattering: every candidate point offers itself to the
+// voxels of the brick within the radius of it, in a copy of the brick held
+// in LDS, and the brick is written once.  The result is
+// max(0, max_p (r - |p - v|)) over the points p of the subcells around the
+// brick, the value the reference's entry loop
+// (kernels/cuda_gvdb_particles.cuh:724-728) computes, since a point with
+// r - |p - v| > 0 is in the entry list of v's subcell.  Work per point is its
+// box of (2r)^3 voxels instead of every voxel of the subcells it reaches.
+//
+// Exact squared distances in integers: all candidates of a brick lie at or
+// above pmin = (brick origin - window) - 1 on every axis.  With
+// pmin >= 1/32 they are multiples of 2^-s, s = 23 - floor(log2 pmin) <= 28,
+// so X = (p - v) 2^s (p - v taken exactly as a TwoSum pair) is an exact
+// integer, below 2^31 for |p - v| < 8; D = X^2 + Y^2 + Z^2 (u64, under
+// 3 * 2^62) is the exact d^2 scaled by 2^2s.  Each voxel keeps the least D
+// (ds_min_u64: order free, so deterministic; equal D are equal values), and
+// r - sqrt(D) 2^-s is evaluated once per voxel.  Bricks whose window reaches
+// below 1/32 take the float path: the accurate falloff of every pair, kept
+// as a max through the same u64 slots.
+//
+// Per chunk of GB_CHUNK candidates: every thread loads two (independent
+// loads in flight), the ones whose box meets the brick are compacted into
+// LDS, then work items (candidate, z slice of its box) spread over the
+// workgroup.  The float bounds of a box are supersets of the voxels within r
+// (rounding is monotone and integers are representable).  LDS rows of a z
+// slice are padded to 65 slots so the slices of one candidate fall in
+// different banks.
+static const int PG_ZSTRIDE = 65;
+struct PGShared {
+    uint32_t cell_off[GB_THREADS];
+    uint32_t cell_start[GB_THREADS];
+    int nbr[27];
+    uint32_t wsum[GB_THREADS / 64];
+    uint32_t ncand;
+    u64 key[8 * PG_ZSTRIDE];
+    float4 pt[GB_CHUNK];
+};
+
+// r - sqrt(D) 2^-s, accurate to a few ulps: D split into float hi + lo
+// (exact to 2^-48), r^2 - d^2 in double-float, divided by r + d
+__device__ inline float falloff_from_d2(u64 D, int s, float r) {
+#pragma clang fp contract(off)
+    const float dh = float(D);
+    const long long rem = (long long)(D - u64(dh));
+    const float hi = __builtin_ldexpf(dh, -2 * s);
+    const float lo = __builtin_ldexpf(float(rem), -2 * s);
+    const float r2h = r * r;
+    const float r2l = __builtin_fmaf(r, r, -r2h);
+    const float num = (r2h - hi) + (r2l - lo);
+    return num / (r + __builtin_sqrtf(hi));
+}
+
+template <typename AT>
+__device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bid, uint32_t) {
+    __shared__ PGShared S;
+    const Tree& t = a.t;
+    const int brick = int(bid);
+    const Node* bn = t.nodes[0] + brick;
+    if (bn->mParent == ID_UNDEF64) return;        // unlinked leaves have no subcells
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
+    const int vi = lz * PG_ZSTRIDE + ly * 8 + lx;
+    S.key[vi] = ~0ull;
+    const uint32_t T = gb_setup(brick, a.R, a.nbr, a.bcnt, a.boff, S);
+    __syncthreads();                               // cell table and brick copy ready
+    const float r = a.radius;
+    const float r2cut = r * r * (1.0f + 0x1p-20f);
+    const int span = 2 + 2 * a.R, ncell = span * span * span;
+    const int ox = bn->mPos[0], oy = bn->mPos[1], oz = bn->mPos[2];
+    const float bx0 = float(ox), by0 = float(oy), bz0 = float(oz);
+    const float bx1 = float(ox + 7), by1 = float(oy + 7), bz1 = float(oz + 7);
+    // fixed-point scale of the brick (see above); 0 selects the float path
+    const float pmin = float(min(ox, min(oy, oz)) - a.R * 4 - 1);
+    int sh = 0;
+    if (pmin >= 0x1p-5f) sh = 23 - (int((__float_as_uint(pmin) >> 23) & 0xFF) - 127);
+    const bool exact = sh > 0;
+    const float scale = __builtin_ldexpf(1.0f, sh);
+    const int step = 1 << (exact ? sh : 0);
+    const u64 dcut = exact ? u64(__builtin_ldexpf(r2cut, 2 * sh)) : 0ull;
+    // integers in [fl(p - r), fl(p + r)]: the interval exceeds 2r by two
+    // roundings at most (under 1/32 each for coordinates below 2^18)
+    const uint32_t nz = uint32_t(2.0f * r + 0.0625f) + 1u;
+    const uint32_t nzinv = uint32_t(0xFFFFFFFFu / nz) + 1u;
+    const u64 lower = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    for (uint32_t c0 = 0; c0 < T; c0 += GB_CHUNK) {
+        float4 p[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t f = c0 + uint32_t(h * GB_THREADS + tid);
+            if (f < T) {
+                int lo = 0, hi = ncell - 1;        // last cell starting at or before f
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (S.cell_start[mid] <= f) lo = mid;
+                    else hi = mid - 1;
+                }
+                p[h] = a.sorted[S.cell_off[lo] + (f - S.cell_start[lo])];
+            }
+        }
+        if (tid == 0) S.ncand = 0;
+        __syncthreads();                           // ncand cleared; previous chunk done with pt
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t f = c0 + uint32_t(h * GB_THREADS + tid);
+            bool keep = false;
+            if (f < T) {
+                const float4 q = p[h];
+                keep = q.x - r <= bx1 && q.x + r >= bx0 && q.y - r <= by1 && q.y + r >= by0 &&
+                       q.z - r <= bz1 && q.z + r >= bz0;
+            }
+            const u64 bal = __ballot(keep);
+            uint32_t base = 0;
+            if (lane == 0 && bal) base = atomicAdd(&S.ncand, uint32_t(__popcll(bal)));
+            base = __shfl(base, 0, 64);
+            if (keep) S.pt[base + uint32_t(__popcll(bal & lower))] = p[h];
+        }
+        __syncthreads();
+        const uint32_t items = S.ncand * nz;
+        for (uint32_t it = uint32_t(tid); it < items; it += GB_THREADS) {
+            uint32_t c = __umulhi(it, nzinv);
+            if (c * nz > it) c--;
+            const int zi = int(it - c * nz);
+            const float4 q = S.pt[c];
+            const int z = int(ceilf(q.z - r)) + zi;
+            if (z < oz || z > oz + 7 || float(z) > q.z + r) continue;
+            const int y0 = max(int(ceilf(q.y - r)), oy), y1 = min(int(floorf(q.y + r)), oy + 7);
+            const int x0 = max(int(ceilf(q.x - r)), ox), x1 = min(int(floorf(q.x + r)), ox + 7);
+            if (y0 > y1 || x0 > x1) continue;
+            u64* kz = S.key + (z - oz) * PG_ZSTRIDE - ox;
+            const float dz = q.z - float(z);
+            if (exact) {
+                // p - v = hi + lo exactly (TwoSum); both are multiples of 2^-s
+                float ez, ey, ex;
+                const float hz = two_diff(q.z, float(z), ez), hy = two_diff(q.y, float(y0), ey),
+                            hx = two_diff(q.x, float(x0), ex);
+                const int Z = int(hz * scale) + int(ez * scale);
+                const int X0 = int(hx * scale) + int(ex * scale);
+                int Y = int(hy * scale) + int(ey * scale);
+                const long long Z2 = (long long)Z * Z;
+                for (int y = y0; y <= y1; y++, Y -= step) {
+                    const u64 YZ = u64((long long)Y * Y + Z2);
+                    if (YZ >= dcut) continue;
+                    u64* kr = kz + (y - oy) * 8;
+                    int X = X0;
+                    for (int x = x0; x <= x1; x++, X -= step) {
+                        const u64 D = u64((long long)X * X) + YZ;
+                        if (D < dcut) atomicMin(kr + x, D);
+                    }
+                }
+            } else {
+                const float dz2 = dz * dz;
+                for (int y = y0; y <= y1; y++) {
+                    const float dy = q.y - float(y);
+                    const float dyz = __builtin_fmaf(dy, dy, dz2);
+                    if (!(dyz < r2cut)) continue;
+                    u64* kr = kz + (y - oy) * 8;
+                    for (int x = x0; x <= x1; x++) {
+                        const float dx = q.x - float(x);
+                        if (__builtin_fmaf(dx, dx, dyz) < r2cut) {
+                            const float v = falloff_inl(q.x, q.y, q.z, x, y, z, r);
+                            if (v > 0.0f) atomicMin(kr + x, u64(0x7F800000u - __float_as_uint(v)));
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    const u64 k = S.key[vi];
+    float val = 0.0f;
+    if (k != ~0ull) val = exact ? fmaxf(0.0f, falloff_from_d2(k, sh, r)) : __uint_as_float(0x7F800000u - uint32_t(k));
+    AT* atlas = reinterpret_cast<AT*>(a.atlas);
+    store_atlas<AT>(atlas, atlas_offset(t, bn->mValue[0] + lx, bn->mValue[1] + ly, bn->mValue[2] + lz), val);
+}
+GVDB_MULTI_T(k_push_gather, typename AT, AT, BucketGatherArgs, GB_THREADS)
+
 hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* ar, hipStream_t s) {
-    for (int half = 0; half < 2; half++) {
+    // accumulate mode keeps the nearest-point kernel: there the reference's
+    // max also takes the negative r - |p - v| of entries beyond the radius
+    for (int variant = 0; variant < 4; variant++) {
+        const bool half = variant & 1, push = variant < 2;
         std::vector<BucketGatherArgs> sel;
         std::vector<uint32_t> nb;
         for (int g = 0; g < G; g++) {
             if (a[g].t.res[0] != 8 || a[g].R < 1 || a[g].R > 2) return hipErrorInvalidValue;
-            if ((a[g].atlas_type == T_HALF) != bool(half) || a[g].nleaves <= 0) continue;
+            if ((a[g].atlas_type == T_HALF) != half || (a[g].accumulate == 0) != push || a[g].nleaves <= 0) continue;
             sel.push_back(a[g]);
             nb.push_back(uint32_t(a[g].nleaves));
         }
@@ -1050,7 +1235,9 @@ hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* a
         hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
         if (e != hipSuccess) return e;
         if (L.blocks == 0) continue;
-        if (half) hipLaunchKernelGGL(k_bucket_gather<__half>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
+        if (push && half) hipLaunchKernelGGL(k_push_gather<__half>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
+        else if (push) hipLaunchKernelGGL(k_push_gather<float>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
+        else if (half) hipLaunchKernelGGL(k_bucket_gather<__half>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
         else hipLaunchKernelGGL(k_bucket_gather<float>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
         e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -34,6 +34,10 @@ CASES = {
     "sphere_r1.5": (lambda: _sphere(5000, 14), 1.5),
     "slab_r2": (lambda: _slab(30000), 2.0),
     "tiny_r2": (lambda: (np.random.default_rng(5).random((200, 3)) * 40 + 70).astype(np.float32), 2.0),
+    # bricks at the origin, where coordinates are finer than any fixed-point scale
+    "origin_r2": (lambda: np.concatenate([np.random.default_rng(6).random((3000, 3)) * 24,
+                                          [[0.0, 0.0, 0.0], [0.0, 5.5, 1e-3], [3e-5, 0.25, 9.0]]]).astype(np.float32),
+                  2.0),
 }
 
 
@@ -52,7 +56,7 @@ def test_topology_bit_exact(gpu, case):
     assert np.array_equal(vi["pos_min"], ob["pos_min"]) and np.array_equal(vi["pos_max"], ob["pos_max"])
 
 
-@pytest.mark.parametrize("case", ["sphere_r2", "slab_r2", "tiny_r2", "sphere_r0.5"])
+@pytest.mark.parametrize("case", ["sphere_r2", "slab_r2", "tiny_r2", "sphere_r0.5", "origin_r2"])
 def test_atlas_and_subcells_bit_exact(gpu, case):
     import oracle as O
     pts, r = CASES[case][0](), CASES[case][1]
@@ -72,7 +76,7 @@ def test_atlas_and_subcells_bit_exact(gpu, case):
     assert all(np.array_equal(x, y) for x, y in zip(la, lb))
 
 
-@pytest.mark.parametrize("case", ["sphere_r2", "slab_r2", "tiny_r2", "sphere_r1.5"])
+@pytest.mark.parametrize("case", ["sphere_r2", "slab_r2", "tiny_r2", "sphere_r1.5", "origin_r2"])
 def test_density_and_apron_vs_fp64(gpu, case):
     import oracle as O
     pts, r = CASES[case][0](), CASES[case][1]
