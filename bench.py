@@ -30,14 +30,23 @@ METRIC = "points splatted/s + Mrays/s on 1 MI355X; batched p50/p99 latency vs pl
 
 def algorithmic_bytes(v, n_points):
     """Per-launch algorithmic HBM bytes of the gather and apron kernels.
-    gather: E*12 (entries) + S*8 (counts, offsets) + B*512*4 (interior writes)
-    apron:  B*488*4 read + B*488*4 written."""
-    sc = v.subcells()
-    E = int(sc["cnt"].astype(np.int64).sum())
-    S = len(sc["cnt"])
+    gather: N*16 (bucketed points, read once) + B*8*8 (bucket counts and
+            offsets) + B*27*4 (neighbour rows) + B*512*4 (interior writes)
+    apron:  B*27*8 (neighbour brick offsets) + B*488*4 read + B*488*4 written."""
     B = len(v.pool(0, 0))
-    return dict(E=E, S=S, B=B, gather=E * 12 + S * 8 + B * 512 * 4, apron=B * 488 * 8,
-                insert=n_points * 12 + E * 12 + S * 8)
+    return dict(B=B, gather=n_points * 16 + B * (64 + 108 + 2048), apron=B * (216 + 488 * 8))
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    FETCH_SIZE / WRITE_SIZE passes (profiles/traffic_medium_slab.json, scaled
+    by the copy calibration recorded there); None when not profiled."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic_medium_slab.json")) as f:
+            k = json.load(f)["kernels"].get("gvdb::" + kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if k is None else float(k["traffic_bytes"])
 
 
 def run_gpu(args):
@@ -75,10 +84,11 @@ def run_gpu(args):
         render()
     v.synchronize()
     t_splat, t_render, kg, ka, kr, st = [], [], [], [], [], []
+    entries = 0
     t_all0 = time.perf_counter()
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        splat()
+        entries = splat()
         v.synchronize()
         t1 = time.perf_counter()
         render()
@@ -96,7 +106,8 @@ def run_gpu(args):
     apron_ms = float(np.mean(ka))
     render_ms = float(np.mean(kr))
     stages = {k: float(np.mean([s[k] for s in st])) for k in st[0]}
-    out = dict(n=n, t_splat=float(np.mean(t_splat)), t_render=float(np.mean(t_render)), t_all=t_all, ab=ab,
+    out = dict(n=n, entries=entries, t_splat=float(np.mean(t_splat)), t_render=float(np.mean(t_render)), t_all=t_all,
+               ab=ab,
                gather_ms=gather_ms, apron_ms=apron_ms, render_ms=render_ms, stages=stages,
                rays=sc["width"] * sc["height"], topo=[len(v.pool(0, l)) for l in range(5)])
     v.close()
@@ -354,14 +365,16 @@ def main():
         dtype="f32", data="synthetic",
         config=dict(workload="plain medium: 4,194,304 slab + 200,000 plane points, tree {3,3,3,4,5}, radius 2, "
                              "subcell 4, isosurface 0.5 trilinear %dx%d step 0.25" % (args.width, args.width),
-                    points=g["n"], subcell_entries=ab["E"], bricks=ab["B"], nodes_per_level=g["topo"]),
+                    points=g["n"], subcell_entries=g["entries"], bricks=ab["B"], nodes_per_level=g["topo"]),
         mrays_per_s=g["rays"] / g["t_render"] / 1e6,
         splat_ms=1e3 * g["t_splat"], render_ms=1e3 * g["t_render"], stage_ms=g["stages"],
         kernel_ms=dict(gather=g["gather_ms"], apron=g["apron_ms"], render=g["render_ms"]),
         roofline=dict(bound="hbm", kernel="gather", achieved=gather_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=gather_gbs / HBM_PEAK_GBS, traffic=None, algorithmic_bytes=ab["gather"]),
+                      frac=gather_gbs / HBM_PEAK_GBS, traffic=profiled_traffic("k_push_gather<float>"),
+                      algorithmic_bytes=ab["gather"]),
         roofline_apron=dict(bound="hbm", achieved=apron_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
-                            frac=apron_gbs / HBM_PEAK_GBS, algorithmic_bytes=ab["apron"]),
+                            frac=apron_gbs / HBM_PEAK_GBS, traffic=profiled_traffic("k_apron<float>"),
+                            algorithmic_bytes=ab["apron"]),
         batched=None,
     )
     if not (args.no_batched and args.no_serve):
