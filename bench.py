@@ -79,6 +79,8 @@ def run_gpu(args):
     def render():
         v.Render(G.SHADE_TRILINEAR, 0, 0)
 
+    splat()
+    tuned = v.TuneRender(G.SHADE_TRILINEAR)
     for _ in range(args.warmup):
         splat()
         render()
@@ -101,12 +103,20 @@ def run_gpu(args):
         st.append(v.stage_ms())
     v.synchronize()
     t_all = time.perf_counter() - t_all0
+    # host-buffer callers: the points cross PCIe first (pageable host memory)
+    t_up = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        v.SetPoints(pts)
+        v.synchronize()
+        t_up.append(time.perf_counter() - t0)
     ab = algorithmic_bytes(v, n)
     gather_ms = float(np.mean(kg))
     apron_ms = float(np.mean(ka))
     render_ms = float(np.mean(kr))
     stages = {k: float(np.mean([s[k] for s in st])) for k in st[0]}
     out = dict(n=n, entries=entries, t_splat=float(np.mean(t_splat)), t_render=float(np.mean(t_render)), t_all=t_all,
+               t_upload=float(np.median(t_up)), render_tuning=dict(best=tuned[0], ms=tuned[1]),
                ab=ab,
                gather_ms=gather_ms, apron_ms=apron_ms, render_ms=render_ms, stages=stages,
                rays=sc["width"] * sc["height"], topo=[len(v.pool(0, l)) for l in range(5)])
@@ -255,6 +265,8 @@ def run_batched(args, grids):
         t = time.perf_counter() - t0
         return [t] * len(vols)
 
+    plain()
+    tuned = vols[int(np.argmax(ns))].TuneRender(G.SHADE_VOLUME)      # on the largest grid
     for _ in range(max(1, args.warmup)):
         plain()
         batched()
@@ -280,7 +292,7 @@ def run_batched(args, grids):
                            p99_ms=1e3 * float(np.percentile(allp, 99))),
                 batched=dict(ms=1e3 * tb, requests_per_s=len(ns) / tb, p50_ms=1e3 * float(np.percentile(allb, 50)),
                              p99_ms=1e3 * float(np.percentile(allb, 99))),
-                speedup=tp / tb, same_images=bool(same))
+                speedup=tp / tb, same_images=bool(same), render_tuning=dict(best=tuned[0], ms=tuned[1]))
 
 
 def run_cpu_baseline(sample_stride=4, render_w=512):
@@ -368,6 +380,8 @@ def main():
                     points=g["n"], subcell_entries=g["entries"], bricks=ab["B"], nodes_per_level=g["topo"]),
         mrays_per_s=g["rays"] / g["t_render"] / 1e6,
         splat_ms=1e3 * g["t_splat"], render_ms=1e3 * g["t_render"], stage_ms=g["stages"],
+        upload_ms=1e3 * g["t_upload"], pcie_inclusive_points_per_s=g["n"] / (g["t_splat"] + g["t_upload"]),
+        render_tuning=g["render_tuning"],
         kernel_ms=dict(gather=g["gather_ms"], apron=g["apron_ms"], render=g["render_ms"]),
         roofline=dict(bound="hbm", kernel="gather", achieved=gather_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=gather_gbs / HBM_PEAK_GBS, traffic=profiled_traffic("k_push_gather<float>"),

@@ -1,5 +1,6 @@
 // capi.cpp -- extern "C" entry points of include/gvdb_amd.h over gvdb::Volume.
 #include "volume.h"
+#include "kernels.h"
 #include "batch.h"
 #include "serve.h"
 #include "../../include/gvdb_amd.h"
@@ -143,6 +144,43 @@ int gvdb_commit_transfer_func(gvdb_handle h) { H_OR_FAIL(h); return h->v.commit_
 int gvdb_add_render_buf(gvdb_handle h, int buf, int w, int hgt, int bpp) { H_OR_FAIL(h); return h->v.add_render_buf(buf, w, hgt, bpp); }
 int gvdb_render(gvdb_handle h, int shading, int chan, int buf) { H_OR_FAIL(h); return h->v.render(shading, chan, buf); }
 int gvdb_read_render_buf(gvdb_handle h, int buf, void* dst) { H_OR_FAIL(h); return h->v.read_render_buf(buf, dst); }
+int gvdb_set_render_unroll(int shading, int unroll) {
+    if (shading != GVDB_SHADE_VOLUME && shading != GVDB_SHADE_TRILINEAR) return GVDB_ERR_INVALID;
+    if (unroll != 1 && unroll != 2 && unroll != 4) return GVDB_ERR_INVALID;
+    gvdb::set_render_unroll(shading == GVDB_SHADE_VOLUME ? 0 : 1, unroll);
+    return GVDB_OK;
+}
+int gvdb_tune_render(gvdb_handle h, int shading, int chan, int buf, int* best, float* ms3) {
+    H_OR_FAIL(h);
+    if (shading != GVDB_SHADE_VOLUME && shading != GVDB_SHADE_TRILINEAR) return GVDB_ERR_INVALID;
+    const int mode = shading == GVDB_SHADE_VOLUME ? 0 : 1;
+    const int keep = gvdb::render_unroll(mode);
+    const int us[3] = {1, 2, 4};
+    int pick = keep;
+    float pick_ms = 0.f;
+    for (int i = 0; i < 3; i++) {
+        gvdb::set_render_unroll(mode, us[i]);
+        float ms = 0.f;
+        for (int rep = 0; rep < 3; rep++) {          // first call warms up, the next two are timed
+            int rc = h->v.render(shading, chan, buf);
+            float k[4];
+            if (rc == GVDB_OK) rc = h->v.kernel_ms(k);
+            if (rc != GVDB_OK) {
+                gvdb::set_render_unroll(mode, keep);
+                return rc;
+            }
+            if (rep) ms += 0.5f * k[2];
+        }
+        if (ms3) ms3[i] = ms;
+        if (i == 0 || ms < pick_ms) {
+            pick = us[i];
+            pick_ms = ms;
+        }
+    }
+    gvdb::set_render_unroll(mode, pick);
+    if (best) *best = pick;
+    return GVDB_OK;
+}
 int gvdb_get_scene_block(gvdb_handle h, int w, int hgt, float* out68) {
     H_OR_FAIL(h);
     if (!out68 || w <= 0 || hgt <= 0) return GVDB_ERR_INVALID;

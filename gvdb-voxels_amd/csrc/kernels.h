@@ -291,5 +291,9 @@ struct RenderArgs {
     int atlas_type;
 };
 hipError_t launch_render_multi(const RenderArgs* a, int G, Arena* ar, hipStream_t s);
+// samples fetched ahead in each brick (1, 2 or 4) for mode 0 (emission-
+// absorption) and 1 (isosurface); process-wide
+void set_render_unroll(int mode, int u);
+int render_unroll(int mode);
 
 }  // namespace gvdb

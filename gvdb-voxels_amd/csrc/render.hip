@@ -31,23 +31,61 @@ __device__ inline float ldv<__half>(const __half* a, u64 o) { return __half2floa
 
 // Trilinear sample at brick-local position p (interior coordinates) from the
 // padded brick starting at `b`; equals an unnormalised linear texture fetch
-// at atlas coordinate p + mValue.
+// at atlas coordinate p + mValue.  Split into the fetch (four x-adjacent
+// pairs, one 8-byte load each for FP32) and the blend, so the loads of
+// several samples can be in flight together.  Indices are clamped into the
+// brick, so a fetch at any position stays inside it.
+struct Taps {
+    float c[8];   // c000 c100 c010 c110 c001 c101 c011 c111
+    float fx, fy, fz;
+};
+
 template <typename AT>
-__device__ inline float tri(const AT* b, int br, float px, float py, float pz) {
+__device__ inline void ld_pair(const AT* p, float& a, float& b);
+template <>
+__device__ inline void ld_pair<float>(const float* p, float& a, float& b) {
+    float2 v;
+    __builtin_memcpy(&v, p, sizeof(v));
+    a = v.x;
+    b = v.y;
+}
+template <>
+__device__ inline void ld_pair<__half>(const __half* p, float& a, float& b) {
+    __half2 v;
+    __builtin_memcpy(&v, p, sizeof(v));
+    a = __low2float(v);
+    b = __high2float(v);
+}
+
+template <typename AT>
+__device__ inline void tri_fetch(const AT* b, int br, float px, float py, float pz, Taps& T) {
     float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
     float fx0 = floorf(qx), fy0 = floorf(qy), fz0 = floorf(qz);
     int ix = min(max(int(fx0), 0), br - 2), iy = min(max(int(fy0), 0), br - 2), iz = min(max(int(fz0), 0), br - 2);
-    float fx = qx - fx0, fy = qy - fy0, fz = qz - fz0;
-    u64 o = u64((iz * br + iy) * br + ix);
-    u64 sy = u64(br), sz = u64(br) * br;
-    float c000 = ldv<AT>(b, o), c100 = ldv<AT>(b, o + 1);
-    float c010 = ldv<AT>(b, o + sy), c110 = ldv<AT>(b, o + sy + 1);
-    float c001 = ldv<AT>(b, o + sz), c101 = ldv<AT>(b, o + sz + 1);
-    float c011 = ldv<AT>(b, o + sz + sy), c111 = ldv<AT>(b, o + sz + sy + 1);
-    float c00 = c000 + fx * (c100 - c000), c10 = c010 + fx * (c110 - c010);
-    float c01 = c001 + fx * (c101 - c001), c11 = c011 + fx * (c111 - c011);
-    float c0 = c00 + fy * (c10 - c00), c1 = c01 + fy * (c11 - c01);
-    return c0 + fz * (c1 - c0);
+    T.fx = qx - fx0;
+    T.fy = qy - fy0;
+    T.fz = qz - fz0;
+    const AT* o = b + ((iz * br + iy) * br + ix);
+    const int sy = br, sz = br * br;
+    ld_pair<AT>(o, T.c[0], T.c[1]);
+    ld_pair<AT>(o + sy, T.c[2], T.c[3]);
+    ld_pair<AT>(o + sz, T.c[4], T.c[5]);
+    ld_pair<AT>(o + sz + sy, T.c[6], T.c[7]);
+}
+
+__device__ inline float tri_blend(const Taps& T) {
+    const float* c = T.c;
+    float c00 = c[0] + T.fx * (c[1] - c[0]), c10 = c[2] + T.fx * (c[3] - c[2]);
+    float c01 = c[4] + T.fx * (c[5] - c[4]), c11 = c[6] + T.fx * (c[7] - c[6]);
+    float c0 = c00 + T.fy * (c10 - c00), c1 = c01 + T.fy * (c11 - c01);
+    return c0 + T.fz * (c1 - c0);
+}
+
+template <typename AT>
+__device__ inline float tri(const AT* b, int br, float px, float py, float pz) {
+    Taps T;
+    tri_fetch<AT>(b, br, px, py, pz, T);
+    return tri_blend(T);
 }
 
 
@@ -87,12 +125,20 @@ __device__ inline const AT* brick_base(const Tree& t, const AT* atlas, const Nod
     return atlas + slot_of(t, n->mValue) * u64(t.brick_vox);
 }
 
-// rayDeepBrick: emission-absorption march through one brick.
-template <typename AT>
+__device__ inline bool inside(f3 p, float r0) {
+    return p.x >= 0 && p.y >= 0 && p.z >= 0 && p.x < r0 && p.y < r0 && p.z < r0;
+}
+
+// rayDeepBrick: emission-absorption march through one brick.  U samples are
+// fetched ahead (positions advanced one step at a time, as the reference
+// does), then accumulated in order under the reference's loop conditions; a
+// sample past the end is fetched but never used.
+template <typename AT, int U>
 __device__ void deep_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 tt, f3 pos, f3 dir, float clr[4]) {
     const Tree& t = A.t;
     const Node* n = t.nodes[0] + nodeid;
     const AT* b = brick_base(t, atlas, n);
+    const int br = t.brick_res;
     const float dstep = A.s.steps[0];
     f3 vmin = mk3(float(n->mPos[0]), float(n->mPos[1]), float(n->mPos[2]));
     float tx = dstep * ceilf(tt.x / dstep);
@@ -102,27 +148,47 @@ __device__ void deep_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 
     const float extinct = A.s.extinct[0], albedo = A.s.extinct[1];
     const float minval = A.s.cutoff[0], alphacut = A.s.cutoff[1];
     const float th = A.s.thresh[0], vr = A.s.thresh[2] - A.s.thresh[1];
-    for (int it = 0; clr[3] > alphacut && it < MAX_ITER && p.x >= 0 && p.y >= 0 && p.z >= 0 && p.x < r0 && p.y < r0 &&
-                     p.z < r0; it++) {
-        float raw = tri<AT>(b, t.brick_res, p.x, p.y, p.z);
-        if (raw >= minval) {
-            float x = fminf(1.0f, fmaxf(0.0f, (raw - th) / vr));
-            float4 val = A.tf[int(x * 16300.0f)];
-            float w = expf(extinct * val.w * dstep);
-            float k = clr[3] * (1.0f - w) * albedo;
-            clr[0] += val.x * k;
-            clr[1] += val.y * k;
-            clr[2] += val.z * k;
-            clr[3] *= w;
+    for (int it = 0;;) {
+        Taps T[U];
+        bool ok[U];
+        f3 q = p;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            ok[u] = it + u < MAX_ITER && inside(q, r0);
+            tri_fetch<AT>(b, br, q.x, q.y, q.z, T[u]);
+            q = q + wpt;
         }
-        p = p + wpt;
+        float raw[U];
+        float4 val[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            raw[u] = tri_blend(T[u]);
+            float x = fminf(1.0f, fmaxf(0.0f, (raw[u] - th) / vr));
+            val[u] = A.tf[int(x * 16300.0f)];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!(clr[3] > alphacut) || !ok[u]) goto done;
+            if (raw[u] >= minval) {
+                float w = expf(extinct * val[u].w * dstep);
+                float k = clr[3] * (1.0f - w) * albedo;
+                clr[0] += val[u].x * k;
+                clr[1] += val[u].y * k;
+                clr[2] += val[u].z * k;
+                clr[3] *= w;
+            }
+            p = p + wpt;
+            it++;
+        }
     }
+done:
     clr[0] = fminf(clr[0], 1.0f); clr[1] = fminf(clr[1], 1.0f); clr[2] = fminf(clr[2], 1.0f);
     clr[3] = fmaxf(clr[3], 0.0f);
 }
 
-// raySurfaceTrilinearBrick: first sample >= threshold, gradient normal.
-template <typename AT>
+// raySurfaceTrilinearBrick: first sample >= threshold, gradient normal; U
+// samples fetched ahead as in deep_brick.
+template <typename AT, int U>
 __device__ void surf_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 tt, f3 pos, f3 dir, f3& hit, f3& nrm) {
     const Tree& t = A.t;
     const Node* n = t.nodes[0] + nodeid;
@@ -134,22 +200,36 @@ __device__ void surf_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 
     float r0 = float(t.res[0]);
     const float th = A.s.thresh[0];
     const int br = t.brick_res;
-    for (int it = 0; it < MAX_ITER && p.x >= 0 && p.y >= 0 && p.z >= 0 && p.x < r0 && p.y < r0 && p.z < r0; it++) {
-        if (tri<AT>(b, br, p.x, p.y, p.z) >= th) {
-            hit = p + vmin;
-            f3 g;
-            g.x = tri<AT>(b, br, p.x - 0.5f, p.y, p.z) - tri<AT>(b, br, p.x + 0.5f, p.y, p.z);
-            g.y = tri<AT>(b, br, p.x, p.y - 0.5f, p.z) - tri<AT>(b, br, p.x, p.y + 0.5f, p.z);
-            g.z = tri<AT>(b, br, p.x, p.y, p.z - 0.5f) - tri<AT>(b, br, p.x, p.y, p.z + 0.5f);
-            nrm = norm3(g);
-            return;
+    for (int it = 0;;) {
+        Taps T[U];
+        bool ok[U];
+        f3 q = p;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            ok[u] = it + u < MAX_ITER && inside(q, r0);
+            tri_fetch<AT>(b, br, q.x, q.y, q.z, T[u]);
+            q = q + dir * dstep;
         }
-        p = p + dir * dstep;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!ok[u]) return;
+            if (tri_blend(T[u]) >= th) {
+                hit = p + vmin;
+                f3 g;
+                g.x = tri<AT>(b, br, p.x - 0.5f, p.y, p.z) - tri<AT>(b, br, p.x + 0.5f, p.y, p.z);
+                g.y = tri<AT>(b, br, p.x, p.y - 0.5f, p.z) - tri<AT>(b, br, p.x, p.y + 0.5f, p.z);
+                g.z = tri<AT>(b, br, p.x, p.y, p.z - 0.5f) - tri<AT>(b, br, p.x, p.y, p.z + 0.5f);
+                nrm = norm3(g);
+                return;
+            }
+            p = p + dir * dstep;
+            it++;
+        }
     }
 }
 
 // rayCast.  MODE 0: deep (EA), MODE 1: trilinear surface.
-template <int MODE, typename AT>
+template <int MODE, typename AT, int U>
 __device__ void ray_cast(const RenderArgs& A, const AT* atlas, f3 pos, f3 dir, f3& hit, f3& nrm, float clr[4]) {
     const Tree& t = A.t;
     int nodeid[MAXLEV];
@@ -185,10 +265,10 @@ __device__ void ray_cast(const RenderArgs& A, const AT* atlas, f3 pos, f3 dir, f
                 f3 tt = d.t;
                 tt.x += t.epsilon;
                 if (MODE == 0) {
-                    deep_brick<AT>(A, atlas, child, tt, pos, dir, clr);
+                    deep_brick<AT, U>(A, atlas, child, tt, pos, dir, clr);
                     if (clr[3] <= 0) { clr[3] = 0; return; }
                 } else {
-                    surf_brick<AT>(A, atlas, child, tt, pos, dir, hit, nrm);
+                    surf_brick<AT, U>(A, atlas, child, tt, pos, dir, hit, nrm);
                     if (hit.z != NOHIT) return;
                 }
                 d.step();
@@ -220,7 +300,7 @@ __device__ inline f3 mmult(const float* m, f3 v) {
                v.x * m[2] + v.y * m[6] + v.z * m[10] + m[14]);
 }
 
-template <int MODE, typename AT>
+template <int MODE, typename AT, int U>
 __device__ inline void k_render_body(const RenderArgs& A, uint32_t bid) {
     const int tiles_x = (A.s.width + 7) / 8;
     const int tile = int(bid);
@@ -237,14 +317,14 @@ __device__ inline void k_render_body(const RenderArgs& A, uint32_t bid) {
     if (MODE == 0) {
         float clr[4] = {0, 0, 0, 1};
         f3 hit = mk3(0, 0, NOHIT), nrm = mk3(0, 0, 0);
-        ray_cast<0, AT>(A, atlas, rpos, rdir, hit, nrm, clr);
+        ray_cast<0, AT, U>(A, atlas, rpos, rdir, hit, nrm, clr);
         float a = 1.0f - clr[3];
         for (int k = 0; k < 3; k++) res[k] = A.s.backclr[k] + a * (clr[k] - A.s.backclr[k]);
         res[3] = a;
     } else {
         float clr[4] = {1, 1, 1, 1};
         f3 hit = mk3(NOHIT, NOHIT, NOHIT), nrm = mk3(0, 0, 0);
-        ray_cast<1, AT>(A, atlas, rpos, rdir, hit, nrm, clr);
+        ray_cast<1, AT, U>(A, atlas, rpos, rdir, hit, nrm, clr);
         if (hit.z == NOHIT) {
             for (int k = 0; k < 4; k++) res[k] = A.s.backclr[k];
         } else {
@@ -255,7 +335,7 @@ __device__ inline void k_render_body(const RenderArgs& A, uint32_t bid) {
             if (A.s.shadow[0] > 0) {
                 f3 h2 = mk3(0, 0, NOHIT), n2 = mk3(0, 0, 0);
                 float c2[4] = {0, 0, 0, 1};
-                ray_cast<1, AT>(A, atlas, hit + nrm * A.s.shadow[1], ld, h2, n2, c2);
+                ray_cast<1, AT, U>(A, atlas, hit + nrm * A.s.shadow[1], ld, h2, n2, c2);
                 diff = (h2.z == NOHIT) ? diff : diff * (1.0f - A.s.shadow[0]);
             }
             for (int k = 0; k < 3; k++) res[k] = clr[k] * (diff + amb);
@@ -276,14 +356,30 @@ __device__ inline void k_render_body(const RenderArgs& A, uint32_t bid) {
 }
 
 // One 64-thread workgroup per 8x8 tile; several grids' images in one launch.
-template <int MODE, typename AT>
+template <int MODE, typename AT, int U>
 __global__ __launch_bounds__(64) void k_render(RenderArgs a0, MultiMap m) {
     if (m.G <= 1) {
-        k_render_body<MODE, AT>(a0, blockIdx.x);
+        k_render_body<MODE, AT, U>(a0, blockIdx.x);
         return;
     }
     const int g = multi_grid(m, blockIdx.x);
-    k_render_body<MODE, AT>(reinterpret_cast<const RenderArgs*>(m.table)[g], blockIdx.x - m.first[g]);
+    k_render_body<MODE, AT, U>(reinterpret_cast<const RenderArgs*>(m.table)[g], blockIdx.x - m.first[g]);
+}
+
+// Samples fetched ahead per brick step (1, 2 or 4), per march mode: 4 for
+// emission-absorption and 1 for the isosurface measured fastest on the
+// medium and batched workloads; gvdb_tune_render re-picks per scene.
+static int g_unroll[2] = {4, 1};
+void set_render_unroll(int mode, int u) {
+    if ((mode == 0 || mode == 1) && (u == 1 || u == 2 || u == 4)) g_unroll[mode] = u;
+}
+int render_unroll(int mode) { return g_unroll[mode & 1]; }
+
+template <int MODE, typename AT>
+static void launch_u(int u, dim3 grid, hipStream_t s, const RenderArgs& a0, const MultiMap& m) {
+    if (u == 4) hipLaunchKernelGGL((k_render<MODE, AT, 4>), grid, dim3(64), 0, s, a0, m);
+    else if (u == 2) hipLaunchKernelGGL((k_render<MODE, AT, 2>), grid, dim3(64), 0, s, a0, m);
+    else hipLaunchKernelGGL((k_render<MODE, AT, 1>), grid, dim3(64), 0, s, a0, m);
 }
 
 hipError_t launch_render_multi(const RenderArgs* a, int G, Arena* ar, hipStream_t s) {
@@ -304,10 +400,11 @@ hipError_t launch_render_multi(const RenderArgs* a, int G, Arena* ar, hipStream_
         hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
         if (e != hipSuccess) return e;
         if (L.blocks == 0) continue;
-        if (mode == 0 && half) hipLaunchKernelGGL((k_render<0, __half>), dim3(L.blocks), dim3(64), 0, s, sel[0], L.map);
-        else if (mode == 0) hipLaunchKernelGGL((k_render<0, float>), dim3(L.blocks), dim3(64), 0, s, sel[0], L.map);
-        else if (half) hipLaunchKernelGGL((k_render<1, __half>), dim3(L.blocks), dim3(64), 0, s, sel[0], L.map);
-        else hipLaunchKernelGGL((k_render<1, float>), dim3(L.blocks), dim3(64), 0, s, sel[0], L.map);
+        const int u = g_unroll[mode];
+        if (mode == 0 && half) launch_u<0, __half>(u, dim3(L.blocks), s, sel[0], L.map);
+        else if (mode == 0) launch_u<0, float>(u, dim3(L.blocks), s, sel[0], L.map);
+        else if (half) launch_u<1, __half>(u, dim3(L.blocks), s, sel[0], L.map);
+        else launch_u<1, float>(u, dim3(L.blocks), s, sel[0], L.map);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

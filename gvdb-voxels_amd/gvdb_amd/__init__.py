@@ -47,6 +47,7 @@ _SIGS = [
     ("gvdb_linear_transfer_func", _i, [_vp, _f, _f, _vp, _vp]), ("gvdb_commit_transfer_func", _i, [_vp]),
     ("gvdb_add_render_buf", _i, [_vp, _i, _i, _i, _i]), ("gvdb_render", _i, [_vp, _i, _i, _i]),
     ("gvdb_read_render_buf", _i, [_vp, _i, _vp]), ("gvdb_get_scene_block", _i, [_vp, _i, _i, _vp]),
+    ("gvdb_tune_render", _i, [_vp, _i, _i, _i, _vp, _vp]), ("gvdb_set_render_unroll", _i, [_i, _i]),
     ("gvdb_get_pool", _i, [_vp, _i, _i, _vp, _u64p, _u64p]), ("gvdb_get_info", _i, [_vp, _vp, _vp]),
     ("gvdb_get_atlas_info", _i, [_vp, _i, _vp]), ("gvdb_read_atlas", _i, [_vp, _i, _vp]),
     ("gvdb_read_atlas_map", _i, [_vp, _vp]), ("gvdb_read_subcells", _i, [_vp, _i, _vp, _u64p]),
@@ -224,6 +225,21 @@ class VolumeGVDB:
 
     def Render(self, shading, chan=0, buf=0):
         return self._chk(load().gvdb_render(self.h, shading, chan, buf))
+
+    @staticmethod
+    def SetRenderUnroll(shading, unroll):
+        """Samples fetched ahead per brick step (1, 2, 4) for a shading mode, process-wide."""
+        rc = load().gvdb_set_render_unroll(int(shading), int(unroll))
+        if rc != 0:
+            raise GVDBError(rc, "gvdb_set_render_unroll(%d, %d)" % (shading, unroll))
+
+    def TuneRender(self, shading, chan=0, buf=0):
+        """Autotunes the samples fetched ahead per brick step for this shading
+        mode on the current scene; returns (best, {unroll: ms})."""
+        best = ctypes.c_int(0)
+        ms = np.zeros(3, np.float32)
+        self._chk(load().gvdb_tune_render(self.h, int(shading), int(chan), int(buf), ctypes.byref(best), _p(ms)))
+        return best.value, {1: float(ms[0]), 2: float(ms[1]), 4: float(ms[2])}
 
     def ReadRenderBuf(self, buf, w, h, bytes_per_pixel=4):
         out = np.zeros((h, w, 4), np.float32 if bytes_per_pixel == 16 else np.uint8)

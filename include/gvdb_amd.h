@@ -129,6 +129,14 @@ int gvdb_add_render_buf(gvdb_handle h, int buf, int w, int hgt, int bytes_per_pi
 int gvdb_render(gvdb_handle h, int shading, int chan, int buf);
 /* ReadRenderBuf(buf, dst): gvdb_volume_gvdb.cpp:4241-4251 */
 int gvdb_read_render_buf(gvdb_handle h, int buf, void* host_dst);
+/* Render autotuner (no counterpart in the reference; for callers that render
+ * one scene size many times): renders with 1, 2 and 4 samples fetched ahead
+ * per brick step, times each on the handle's stream (ms3[i], may be NULL) and
+ * keeps the fastest for that shading mode, process-wide.  Images do not
+ * depend on the choice. */
+int gvdb_tune_render(gvdb_handle h, int shading, int chan, int buf, int* best_unroll, float* ms3);
+/* Sets that choice directly (1, 2 or 4), e.g. from an earlier tuning run. */
+int gvdb_set_render_unroll(int shading, int unroll);
 /* The 68-word scene block PrepareRender sends to the kernels (gvdb_volume_gvdb.cpp:4254-4306) */
 int gvdb_get_scene_block(gvdb_handle h, int w, int hgt, float* out68);
 

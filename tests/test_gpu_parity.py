@@ -111,6 +111,31 @@ def test_render_vs_fp64(gpu, shading):
     assert np.abs(img - ref).max() < 0.5
 
 
+@pytest.mark.parametrize("shading", [7, 4])
+def test_render_unroll_same_image(gpu, shading):
+    """Samples fetched ahead per brick step (1, 2, 4) change only the order of
+    the loads: the images are bit-identical, and the autotuner keeps one of
+    the three."""
+    from gvdb_amd import workloads as W
+    pts = _sphere(20000)
+    v, _ = H.gpu_volume(gpu, pts, 2.0)
+    sc = W.ea_scene(96, 80) if shading == 7 else W.iso_scene(96, 80, centre=(256.0, 256.0, 256.0), dist=600.0)
+    v.apply_scene(sc)
+    v.AddRenderBuf(0, sc["width"], sc["height"], 16)
+    imgs = []
+    try:
+        for u in (1, 2, 4):
+            gpu.VolumeGVDB.SetRenderUnroll(shading, u)
+            v.Render(shading, 0, 0)
+            imgs.append(v.ReadRenderBuf(0, sc["width"], sc["height"], 16))
+        best, ms = v.TuneRender(shading)
+        assert best in (1, 2, 4) and all(t > 0 for t in ms.values())
+    finally:
+        gpu.VolumeGVDB.SetRenderUnroll(shading, 4 if shading == 7 else 1)
+    assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
+    assert imgs[0][..., 3].max() > 0
+
+
 def test_host_topology_path(gpu):
     """RebuildTopologyCPU (host ActivateSpace) then the GPU stages."""
     import oracle as O
