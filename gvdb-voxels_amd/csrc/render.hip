@@ -12,6 +12,13 @@
 #include "tree_device.h"
 #include <hip/hip_fp16.h>
 
+// Contraction is off in this file: the walk performs the reference's FP32
+// operations one by one (no fused multiply-adds), so every ray takes the
+// steps of a plain IEEE single-precision evaluation of rayCast, also where a
+// step sits on a rounding edge (far from the origin the 1e-3 epsilon is a
+// few ulps).
+#pragma clang fp contract(off)
+
 namespace gvdb {
 
 struct f3 { float x, y, z; };

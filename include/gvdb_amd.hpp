@@ -162,12 +162,150 @@ public:
     }
     void Synchronize() { check(gvdb_synchronize(h_), "Synchronize"); }
 
+    /* Render autotuner and setter (no counterpart in the reference) */
+    int TuneRender(int shading, int chan = 0, int buf = 0, float* ms3 = nullptr) {
+        int best = 0;
+        check(gvdb_tune_render(h_, shading, chan, buf, &best, ms3), "TuneRender");
+        return best;
+    }
+    static int SetRenderUnroll(int shading, int unroll) { return gvdb_set_render_unroll(shading, unroll); }
+
 private:
     void check(int rc, const char* what) {
         status_ = rc;
         if (rc != GVDB_OK && throw_) throw Error(rc, std::string(what) + ": " + error());
     }
     gvdb_handle h_ = nullptr;
+    int status_ = GVDB_OK;
+    bool throw_ = true;
+};
+
+/* Many grids per launch (gvdb_batch_*; no counterpart in the reference):
+ * each method runs one stage of the point path over `vols` with one kernel
+ * launch per kernel of the stage.  Per-grid arguments are vectors with one
+ * entry per volume.  A failure throws (or records) with the failing grid's
+ * index in failed(). */
+class GridBatch {
+public:
+    explicit GridBatch(int device = 0, int max_grids = 256, bool throw_on_error = true) : throw_(throw_on_error) {
+        check(gvdb_batch_create(&b_, device, max_grids), "gvdb_batch_create");
+    }
+    ~GridBatch() {
+        if (b_) gvdb_batch_destroy(b_);
+    }
+    GridBatch(const GridBatch&) = delete;
+    GridBatch& operator=(const GridBatch&) = delete;
+    int status() const { return status_; }
+    int failed() const { return failed_; }
+
+    void RebuildTopology(const std::vector<VolumeGVDB*>& vols, const std::vector<int>& n,
+                         const std::vector<float>& radius) {
+        auto hs = handles(vols);
+        check(gvdb_batch_rebuild_topology(b_, int(hs.size()), hs.data(), n.data(), radius.data(), nullptr, &failed_),
+              "RebuildTopology");
+    }
+    void FinishTopology(const std::vector<VolumeGVDB*>& vols, bool commit = false, bool bounds = true) {
+        auto hs = handles(vols);
+        check(gvdb_batch_finish_topology(b_, int(hs.size()), hs.data(), commit, bounds, &failed_), "FinishTopology");
+    }
+    void UpdateAtlas(const std::vector<VolumeGVDB*>& vols) {
+        auto hs = handles(vols);
+        check(gvdb_batch_update_atlas(b_, int(hs.size()), hs.data(), &failed_), "UpdateAtlas");
+    }
+    void ClearChannel(const std::vector<VolumeGVDB*>& vols, int chan) {
+        auto hs = handles(vols);
+        check(gvdb_batch_clear_channel(b_, int(hs.size()), hs.data(), chan, &failed_), "ClearChannel");
+    }
+    std::vector<int> InsertPointsSubcell(const std::vector<VolumeGVDB*>& vols, int subcell_size,
+                                         const std::vector<int>& n, const std::vector<float>& radius) {
+        auto hs = handles(vols);
+        std::vector<int> len(hs.size(), 0);
+        check(gvdb_batch_insert_points_subcell(b_, int(hs.size()), hs.data(), subcell_size, n.data(), radius.data(),
+                                               nullptr, len.data(), &failed_),
+              "InsertPointsSubcell");
+        return len;
+    }
+    void GatherDensity(const std::vector<VolumeGVDB*>& vols, int subcell_size, const std::vector<int>& len,
+                       const std::vector<float>& radius, int chan_density, bool accumulate = false) {
+        auto hs = handles(vols);
+        check(gvdb_batch_gather_density(b_, int(hs.size()), hs.data(), subcell_size, len.data(), radius.data(),
+                                        chan_density, accumulate, &failed_),
+              "GatherDensity");
+    }
+    void UpdateApron(const std::vector<VolumeGVDB*>& vols, int chan, float boundval) {
+        auto hs = handles(vols);
+        check(gvdb_batch_update_apron(b_, int(hs.size()), hs.data(), chan, boundval, &failed_), "UpdateApron");
+    }
+    void Render(const std::vector<VolumeGVDB*>& vols, int shading, int chan, int buf) {
+        auto hs = handles(vols);
+        std::vector<int> sh(hs.size(), shading), ch(hs.size(), chan), bf(hs.size(), buf);
+        check(gvdb_batch_render(b_, int(hs.size()), hs.data(), sh.data(), ch.data(), bf.data(), &failed_), "Render");
+    }
+    void Synchronize() { check(gvdb_batch_synchronize(b_), "Synchronize"); }
+
+private:
+    static std::vector<gvdb_handle> handles(const std::vector<VolumeGVDB*>& vols) {
+        std::vector<gvdb_handle> hs;
+        for (auto* v : vols) hs.push_back(v->handle());
+        return hs;
+    }
+    void check(int rc, const char* what) {
+        status_ = rc;
+        if (rc != GVDB_OK && throw_)
+            throw Error(rc, std::string(what) + ": " + (b_ ? gvdb_batch_last_error(b_) : "no batch"));
+    }
+    gvdb_batch b_ = nullptr;
+    int status_ = GVDB_OK;
+    int failed_ = -1;
+    bool throw_ = true;
+};
+
+/* Serving loop over resident grids (gvdb_server_*; no counterpart in the
+ * reference); see gvdb_amd.h for the request semantics. */
+class Server {
+public:
+    enum { PLAIN = 0, CONTINUOUS = 1 };
+    Server(int device, unsigned long long hbm_budget, int max_batch, bool throw_on_error = true)
+        : throw_(throw_on_error) {
+        check(gvdb_server_create(&s_, device, hbm_budget, max_batch), "gvdb_server_create");
+    }
+    ~Server() {
+        if (s_) gvdb_server_destroy(s_);
+    }
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+    int status() const { return status_; }
+
+    int AddGrid(VolumeGVDB& v, const float* host_xyz, int n, float radius) {
+        return check(gvdb_server_add_grid(s_, v.handle(), host_xyz, n, radius), "AddGrid");
+    }
+    void Reset() { check(gvdb_server_reset(s_), "Reset"); }
+    int SubmitRender(int grid, double arrival, int w, int h, int shading, const float angs[3], float dist,
+                     unsigned char* out_rgba) {
+        return check(gvdb_server_submit_render(s_, grid, arrival, w, h, shading, angs, dist, out_rgba),
+                     "SubmitRender");
+    }
+    int SubmitInsert(int grid, double arrival, const float* host_xyz, int n) {
+        return check(gvdb_server_submit_insert(s_, grid, arrival, host_xyz, n), "SubmitInsert");
+    }
+    void ClearRequests() { check(gvdb_server_clear_requests(s_), "ClearRequests"); }
+    /* latency/status: one entry per submitted request; returns the wall time (s) */
+    double Run(int mode, std::vector<double>& latency, std::vector<int>& status, size_t requests) {
+        latency.assign(requests, 0.0);
+        status.assign(requests, 0);
+        double wall = 0.0;
+        check(gvdb_server_run(s_, mode, latency.data(), status.data(), &wall), "Run");
+        return wall;
+    }
+
+private:
+    int check(int rc, const char* what) {
+        status_ = rc < 0 ? rc : GVDB_OK;
+        if (rc < 0 && throw_)
+            throw Error(rc, std::string(what) + ": " + (s_ ? gvdb_server_last_error(s_) : "no server"));
+        return rc;
+    }
+    gvdb_server s_ = nullptr;
     int status_ = GVDB_OK;
     bool throw_ = true;
 };

@@ -120,7 +120,8 @@ def main():
               "The march is the reference's FP32 algorithm (FP32 DDA with its 1e-3 epsilon, MAX_ITER = 256 walk "
               "cap).  Run in FP64 on the same scene it is not the same walk: at coordinates near 4000 the epsilon is "
               "four float ulps, so a child node's exit time can equal the parent's limit in FP32 and the walk then "
-              "leaves the loop, where FP64 pops to the parent and goes on.  The images are therefore compared with "
+              "leaves the loop, where FP64 pops to the parent and goes on.  The GPU march performs the FP32 "
+              "operations one by one (no contraction), so it takes the FP32 walk.  The images are compared with "
               "both the oracle's FP32 march (the reference's arithmetic: the parity claim) and its FP64 march (the "
               "error against FP64 asked of each result); the FP32-vs-FP64 row of the oracle itself shows how much "
               "of the latter is the algorithm's own.",

@@ -145,6 +145,7 @@ def test_cpp_facade_matches_python_binding(gpu, tmp_path):
     tok = r.stdout.split()
     counts = [int(x) for x in tok[1:6]]
     entries, mean = int(tok[7]), float(tok[9])
+    assert tok[10] == "batched_same" and tok[11] == "1"     # GridBatch in the C++ facade
     v, n = H.gpu_volume(gpu, pts, 2.0)
     assert counts == [len(v.pool(0, l)) for l in range(5)]
     assert entries == n
