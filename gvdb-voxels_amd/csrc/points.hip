@@ -1215,7 +1215,19 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
     AT* atlas = reinterpret_cast<AT*>(a.atlas);
     store_atlas<AT>(atlas, atlas_offset(t, bn->mValue[0] + lx, bn->mValue[1] + ly, bn->mValue[2] + lz), val);
 }
-GVDB_MULTI_T(k_push_gather, typename AT, AT, BucketGatherArgs, GB_THREADS)
+// Written out (not GVDB_MULTI_T) to ask for four workgroups per CU: at most
+// 64 VGPRs, so eight waves per SIMD.
+template <typename AT>
+__global__ __launch_bounds__(GB_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_push_gather(
+    BucketGatherArgs a0, MultiMap m) {
+    if (m.G <= 1) {
+        k_push_gather_body<AT>(a0, blockIdx.x, gridDim.x);
+        return;
+    }
+    const int g = multi_grid(m, blockIdx.x);
+    const BucketGatherArgs* tab = reinterpret_cast<const BucketGatherArgs*>(m.table);
+    k_push_gather_body<AT>(tab[g], blockIdx.x - m.first[g], m.first[g + 1] - m.first[g]);
+}
 
 hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* ar, hipStream_t s) {
     // accumulate mode keeps the nearest-point kernel: there the reference's
