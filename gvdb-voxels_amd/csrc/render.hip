@@ -236,13 +236,22 @@ __device__ void surf_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 
 }
 
 // rayCast.  MODE 0: deep (EA), MODE 1: trilinear surface.
+// Per-lane level stacks live in LDS ([level][lane], 64 lanes per workgroup)
+// rather than in dynamically indexed registers.
+struct LevelStack {
+    int nodeid[MAXLEV][64];
+    float tMax[MAXLEV][64];
+};
+
 template <int MODE, typename AT, int U>
-__device__ void ray_cast(const RenderArgs& A, const AT* atlas, f3 pos, f3 dir, f3& hit, f3& nrm, float clr[4]) {
+__device__ void ray_cast(const RenderArgs& A, const AT* atlas, f3 pos, f3 dir, f3& hit, f3& nrm, float clr[4],
+                         LevelStack& S) {
     const Tree& t = A.t;
-    int nodeid[MAXLEV];
-    float tMax[MAXLEV];
+    const int lane = threadIdx.x;
+    int (&nodeid)[MAXLEV][64] = S.nodeid;
+    float (&tMax)[MAXLEV][64] = S.tMax;
     int lev = t.top_lev;
-    nodeid[lev] = 0;
+    nodeid[lev][lane] = 0;
     float h0 = (t.bmin[0] - pos.x) / dir.x, h1 = (t.bmax[0] - pos.x) / dir.x;
     float h2 = (t.bmin[1] - pos.y) / dir.y, h3 = (t.bmax[1] - pos.y) / dir.y;
     float h4 = (t.bmin[2] - pos.z) / dir.z, h5 = (t.bmax[2] - pos.z) / dir.z;
@@ -256,7 +265,7 @@ __device__ void ray_cast(const RenderArgs& A, const AT* atlas, f3 pos, f3 dir, f
     d.dir = dir;
     d.ps[0] = dir.x > 0 ? 1 : -1; d.ps[1] = dir.y > 0 ? 1 : -1; d.ps[2] = dir.z > 0 ? 1 : -1;
     d.t = mk3(t0 + t.epsilon, t1, 0.0f);
-    tMax[lev] = t1 - t.epsilon;
+    tMax[lev][lane] = t1 - t.epsilon;
     d.prepare(mk3(float(node->mPos[0]), float(node->mPos[1]), float(node->mPos[2])), float(t.range[lev] / t.res[lev]));
     for (int it = 0; it < MAX_ITER && lev > 0 && lev <= t.top_lev && d.p[0] >= 0 && d.p[1] >= 0 && d.p[2] >= 0 &&
                      d.p[0] <= t.res[lev] && d.p[1] <= t.res[lev] && d.p[2] <= t.res[lev]; it++) {
@@ -281,20 +290,20 @@ __device__ void ray_cast(const RenderArgs& A, const AT* atlas, f3 pos, f3 dir, f
                 d.step();
             } else {
                 lev--;
-                nodeid[lev] = child;
+                nodeid[lev][lane] = child;
                 node = t.nodes[lev] + child;
                 d.t.x += t.epsilon;
-                tMax[lev] = d.t.y - t.epsilon;
+                tMax[lev][lane] = d.t.y - t.epsilon;
                 d.prepare(mk3(float(node->mPos[0]), float(node->mPos[1]), float(node->mPos[2])),
                           float(t.range[lev] / t.res[lev]));
             }
         } else {
             d.step();
         }
-        while (d.t.x > tMax[lev] && lev <= t.top_lev) {
+        while (lev <= t.top_lev && d.t.x > tMax[lev][lane]) {
             lev++;
             if (lev <= t.top_lev) {
-                node = t.nodes[lev] + nodeid[lev];
+                node = t.nodes[lev] + nodeid[lev][lane];
                 d.prepare(mk3(float(node->mPos[0]), float(node->mPos[1]), float(node->mPos[2])),
                           float(t.range[lev] / t.res[lev]));
             }
@@ -308,7 +317,7 @@ __device__ inline f3 mmult(const float* m, f3 v) {
 }
 
 template <int MODE, typename AT, int U>
-__device__ inline void k_render_body(const RenderArgs& A, uint32_t bid) {
+__device__ inline void k_render_body(const RenderArgs& A, uint32_t bid, LevelStack& S) {
     const int tiles_x = (A.s.width + 7) / 8;
     const int tile = int(bid);
     const int x = (tile % tiles_x) * 8 + (threadIdx.x & 7);
@@ -324,14 +333,14 @@ __device__ inline void k_render_body(const RenderArgs& A, uint32_t bid) {
     if (MODE == 0) {
         float clr[4] = {0, 0, 0, 1};
         f3 hit = mk3(0, 0, NOHIT), nrm = mk3(0, 0, 0);
-        ray_cast<0, AT, U>(A, atlas, rpos, rdir, hit, nrm, clr);
+        ray_cast<0, AT, U>(A, atlas, rpos, rdir, hit, nrm, clr, S);
         float a = 1.0f - clr[3];
         for (int k = 0; k < 3; k++) res[k] = A.s.backclr[k] + a * (clr[k] - A.s.backclr[k]);
         res[3] = a;
     } else {
         float clr[4] = {1, 1, 1, 1};
         f3 hit = mk3(NOHIT, NOHIT, NOHIT), nrm = mk3(0, 0, 0);
-        ray_cast<1, AT, U>(A, atlas, rpos, rdir, hit, nrm, clr);
+        ray_cast<1, AT, U>(A, atlas, rpos, rdir, hit, nrm, clr, S);
         if (hit.z == NOHIT) {
             for (int k = 0; k < 4; k++) res[k] = A.s.backclr[k];
         } else {
@@ -342,7 +351,7 @@ __device__ inline void k_render_body(const RenderArgs& A, uint32_t bid) {
             if (A.s.shadow[0] > 0) {
                 f3 h2 = mk3(0, 0, NOHIT), n2 = mk3(0, 0, 0);
                 float c2[4] = {0, 0, 0, 1};
-                ray_cast<1, AT, U>(A, atlas, hit + nrm * A.s.shadow[1], ld, h2, n2, c2);
+                ray_cast<1, AT, U>(A, atlas, hit + nrm * A.s.shadow[1], ld, h2, n2, c2, S);
                 diff = (h2.z == NOHIT) ? diff : diff * (1.0f - A.s.shadow[0]);
             }
             for (int k = 0; k < 3; k++) res[k] = clr[k] * (diff + amb);
@@ -363,20 +372,25 @@ __device__ inline void k_render_body(const RenderArgs& A, uint32_t bid) {
 }
 
 // One 64-thread workgroup per 8x8 tile; several grids' images in one launch.
+// Occupancy asked of the compiler per unroll (waves per SIMD: 8, 6, 4 for
+// U = 1, 2, 4): the most each variant reaches with no register spills.
+constexpr int render_waves(int u) { return u == 1 ? 8 : (u == 2 ? 6 : 4); }
 template <int MODE, typename AT, int U>
-__global__ __launch_bounds__(64) void k_render(RenderArgs a0, MultiMap m) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(render_waves(U)))) void k_render(RenderArgs a0,
+                                                                                                     MultiMap m) {
+    __shared__ LevelStack S;
     if (m.G <= 1) {
-        k_render_body<MODE, AT, U>(a0, blockIdx.x);
+        k_render_body<MODE, AT, U>(a0, blockIdx.x, S);
         return;
     }
     const int g = multi_grid(m, blockIdx.x);
-    k_render_body<MODE, AT, U>(reinterpret_cast<const RenderArgs*>(m.table)[g], blockIdx.x - m.first[g]);
+    k_render_body<MODE, AT, U>(reinterpret_cast<const RenderArgs*>(m.table)[g], blockIdx.x - m.first[g], S);
 }
 
 // Samples fetched ahead per brick step (1, 2 or 4), per march mode: 4 for
-// emission-absorption and 1 for the isosurface measured fastest on the
+// emission-absorption and 2 for the isosurface measured fastest on the
 // medium and batched workloads; gvdb_tune_render re-picks per scene.
-static int g_unroll[2] = {4, 1};
+static int g_unroll[2] = {4, 2};
 void set_render_unroll(int mode, int u) {
     if ((mode == 0 || mode == 1) && (u == 1 || u == 2 || u == 4)) g_unroll[mode] = u;
 }

@@ -131,7 +131,7 @@ def test_render_unroll_same_image(gpu, shading):
         best, ms = v.TuneRender(shading)
         assert best in (1, 2, 4) and all(t > 0 for t in ms.values())
     finally:
-        gpu.VolumeGVDB.SetRenderUnroll(shading, 4 if shading == 7 else 1)
+        gpu.VolumeGVDB.SetRenderUnroll(shading, 4 if shading == 7 else 2)
     assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
     assert imgs[0][..., 3].max() > 0
 
