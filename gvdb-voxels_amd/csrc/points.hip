@@ -511,15 +511,18 @@ __device__ inline void k_bucket_points_body(const BucketArgs& a, uint32_t bid, u
         bool valid = p2 != NOHIT && isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && w[0] >= 0 && w[1] >= 0 &&
                      w[2] >= 0 && p0 >= 0 && p1 >= 0 && p2 >= 0;
         if (valid) {
-            int ox = int(w[0]) / scs * scs, oy = int(w[1]) / scs * scs, oz = int(w[2]) / scs * scs;
+            // the subcell size divides the (power-of-two) brick width, so it is
+            // a power of two: divisions are shifts (towards zero for negatives)
+            const int ssh = __builtin_ctz(unsigned(scs));
+            int ox = (int(w[0]) >> ssh) << ssh, oy = (int(w[1]) >> ssh) << ssh, oz = (int(w[2]) >> ssh) << ssh;
             int leaf = pos_leaf(t, ox, oy, oz);
             if (leaf < 0) {
                 atomicAdd(orphans, 1u);
             } else {
                 const Node* n = t.nodes[0] + leaf;
                 const int res0 = t.res[0];
-                int sd = res0 / scs;
-                int lx = (ox - n->mPos[0]) / scs, ly = (oy - n->mPos[1]) / scs, lz = (oz - n->mPos[2]) / scs;
+                int sd = res0 >> ssh;
+                int lx = (ox - n->mPos[0]) >> ssh, ly = (oy - n->mPos[1]) >> ssh, lz = (oz - n->mPos[2]) >> ssh;
                 k = uint32_t(leaf) * uint32_t(spb) + uint32_t((lz * sd + ly) * sd + lx);
                 slot[i] = atomicAdd(&bcnt[k], 1u);
                 if (a.len) {
@@ -529,7 +532,8 @@ __device__ inline void k_bucket_points_body(const BucketArgs& a, uint32_t bid, u
                     const int org[3] = {n->mPos[0], n->mPos[1], n->mPos[2]};
                     uint32_t c[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
                     for (int ax = 0; ax < 3; ax++) {
-                        const int lo = int(w[ax] - a.radius) / scs * scs, hi = int(w[ax] + 1 + a.radius) / scs * scs;
+                        const int lo = child_cell(int(w[ax] - a.radius), ssh) * scs,
+                                  hi = child_cell(int(w[ax] + 1 + a.radius), ssh) * scs;
                         for (int sv = lo; sv <= hi; sv += scs) c[ax][sv < org[ax] ? 0 : (sv >= org[ax] + res0 ? 2 : 1)]++;
                     }
                     const uint32_t m = a.nmask[leaf];

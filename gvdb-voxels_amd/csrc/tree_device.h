@@ -24,16 +24,20 @@ __device__ inline int child_of(const Tree& t, const Node* n, int b) {
 }
 
 // Leaf index of the brick containing integer position `pos`, or -1.
+// Node ranges are powers of two, so (d * res) / range (C division, towards
+// zero) is d shifted right by log2(range / res), with the sign applied after.
+__device__ inline int child_cell(long long d, int sh) { return d >= 0 ? int(d >> sh) : -int((-d) >> sh); }
+
 __device__ inline int pos_leaf(const Tree& t, int px, int py, int pz) {
     const Node* nd = node_at(t, t.top_lev, 0);
     int l = nd->mLev;
     if (l == 0) return 0;
     while (true) {
         int res = t.res[l];
-        long long rg = t.range[l];
-        int a = int((long long)(px - nd->mPos[0]) * res / rg);
-        int b = int((long long)(py - nd->mPos[1]) * res / rg);
-        int c = int((long long)(pz - nd->mPos[2]) * res / rg);
+        const int sh = __builtin_ctz(unsigned(t.range[l])) - t.logdim[l];
+        int a = child_cell((long long)px - nd->mPos[0], sh);
+        int b = child_cell((long long)py - nd->mPos[1], sh);
+        int c = child_cell((long long)pz - nd->mPos[2], sh);
         if (a < 0 || b < 0 || c < 0 || a >= res || b >= res || c >= res) return -1;
         int child = child_of(t, nd, (c * res + b) * res + a);
         if (child < 0) return -1;
@@ -52,8 +56,10 @@ __device__ inline int point_leaf(const Tree& t, float px, float py, float pz) {
         float vx = float(nd->mPos[0]), vy = float(nd->mPos[1]), vz = float(nd->mPos[2]);
         float rg = float(t.range[lev]);
         if (px < vx || py < vy || pz < vz || px >= vx + rg || py >= vy + rg || pz >= vz + rg) return -1;
-        float vdel = float(t.range[lev] / t.res[lev]);
-        int a = int((px - vx) / vdel), b = int((py - vy) / vdel), c = int((pz - vz) / vdel);
+        // division by the child range, a power of two: the same as a
+        // multiplication by its exact reciprocal
+        const float inv = __builtin_ldexpf(1.0f, t.logdim[lev] - __builtin_ctz(unsigned(t.range[lev])));
+        int a = int((px - vx) * inv), b = int((py - vy) * inv), c = int((pz - vz) * inv);
         int bit = (((c << t.logdim[lev]) + b) << t.logdim[lev]) + a;
         lev--;
         int child = child_of(t, nd, bit);
