@@ -103,11 +103,13 @@ __device__ inline void dom_coords(const BitDomain& d, u64 idx, int& x, int& y, i
     z = int(r - b * d.dim[2]) + d.lo[2];
 }
 
+// Bits are only ever set while a bitmap is built, so the plain (L2-cached)
+// read that filters the OR can only miss bits set meanwhile by other XCDs;
+// then the OR is done anyway.
 __device__ inline void set_bit(uint32_t* bm, u64 idx) {
     uint32_t m = 1u << (idx & 31);
     uint32_t* w = bm + (idx >> 5);
-    if (!(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m))
-        __hip_atomic_fetch_or(w, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!(*w & m)) __hip_atomic_fetch_or(w, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // gvdbCalcBrickId (kernels/cuda_gvdb_particles.cuh:352-382) for level 0 and
@@ -152,8 +154,10 @@ __device__ inline void k_mark_points_body(const MarkArgs& a, uint32_t bid, uint3
                         m |= 1u << ((z - c[2] + 1) * 9 + (y - c[1] + 1) * 3 + (x - c[0] + 1));
             m |= 1u << 13;
             uint32_t* wd = a.masks + idx;
-            if ((__hip_atomic_load(wd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) != m)
-                __hip_atomic_fetch_or(wd, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // Bits are only ever set here, so a plain (L2-cached) read can
+            // only miss bits other XCDs set since: then the OR is done anyway.
+            // Skipping the OR needs every bit of m already present.
+            if ((*wd & m) != m) __hip_atomic_fetch_or(wd, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             continue;
         }
         if (inside) set_bit(a.bmA, idx);
