@@ -21,43 +21,49 @@ __device__ inline void dir_of(int d, int& dx, int& dy, int& dz) {
     dz = d / 9 - 1;
 }
 
+// 32 lanes per brick (27 directions used), two bricks per wave: the mask of
+// present neighbours comes from one ballot, written by the brick's lane 0.
 __device__ inline void k_neighbors_body(const NbrArgs& a, uint32_t bid, uint32_t) {
     const Tree& t = a.t;
     int* nbr = a.nbr;
-    int g = bid * 256 + threadIdx.x;
-    if (g >= a.n * 27) return;
-    int b = g / 27, d = g - b * 27;
-    const Node* n = t.nodes[0] + b;
-    int v;
-    if (d == 13) {
-        // the brick itself, or -1 when the walk from the top node does not reach it
-        v = pos_leaf(t, n->mPos[0], n->mPos[1], n->mPos[2]) == b ? b : -1;
-    } else {
-        int dx, dy, dz;
-        dir_of(d, dx, dy, dz);
-        int w = t.brickwid;
-        // a voxel centre inside the neighbour box: local coordinate -1, 0 or w
-        float px = float(n->mPos[0]) + float(dx < 0 ? -1 : (dx > 0 ? w : 0)) + 0.5f;
-        float py = float(n->mPos[1]) + float(dy < 0 ? -1 : (dy > 0 ? w : 0)) + 0.5f;
-        float pz = float(n->mPos[2]) + float(dz < 0 ? -1 : (dz > 0 ? w : 0)) + 0.5f;
-        v = point_leaf(t, px, py, pz);
+    const u64 g = u64(bid) * 256 + threadIdx.x;
+    const int b = int(g >> 5), d = int(g & 31);
+    const bool valid = b < a.n && d < 27;
+    int v = -1;
+    if (valid) {
+        const Node* n = t.nodes[0] + b;
+        if (d == 13) {
+            // the brick itself, or -1 when the walk from the top node does not reach it
+            v = pos_leaf(t, n->mPos[0], n->mPos[1], n->mPos[2]) == b ? b : -1;
+        } else {
+            int dx, dy, dz;
+            dir_of(d, dx, dy, dz);
+            int w = t.brickwid;
+            // a voxel centre inside the neighbour box: local coordinate -1, 0 or w
+            float px = float(n->mPos[0]) + float(dx < 0 ? -1 : (dx > 0 ? w : 0)) + 0.5f;
+            float py = float(n->mPos[1]) + float(dy < 0 ? -1 : (dy > 0 ? w : 0)) + 0.5f;
+            float pz = float(n->mPos[2]) + float(dz < 0 ? -1 : (dz > 0 ? w : 0)) + 0.5f;
+            v = point_leaf(t, px, py, pz);
+        }
+        const u64 e = u64(b) * 27 + u64(d);
+        nbr[e] = v;
+        if (a.base) {
+            // atlas offset of the neighbour's padded brick (d = 13: the brick's own)
+            const int m = d == 13 ? b : v;
+            const Node* mn = t.nodes[0] + (m >= 0 ? m : 0);
+            const bool live = m >= 0 && (d != 13 || mn->mFlags) && mn->mValue[0] >= 0;
+            a.base[e] = live ? (long long)(slot_of(t, mn->mValue) * u64(t.brick_vox)) : -1ll;
+        }
     }
-    nbr[g] = v;
-    if (a.mask && v >= 0) atomicOr(&a.mask[b], 1u << d);
-    if (a.base) {
-        // atlas offset of the neighbour's padded brick (d = 13: the brick's own)
-        const int m = d == 13 ? b : v;
-        const Node* mn = t.nodes[0] + (m >= 0 ? m : 0);
-        const bool live = m >= 0 && (d != 13 || mn->mFlags) && mn->mValue[0] >= 0;
-        a.base[g] = live ? (long long)(slot_of(t, mn->mValue) * u64(t.brick_vox)) : -1ll;
-    }
+    const u64 bal = __ballot(valid && v >= 0);
+    if (a.mask && valid && d == 0) a.mask[b] = uint32_t(bal >> (threadIdx.x & 32)) & 0x7FFFFFFu;
 }
 
 GVDB_MULTI(k_neighbors, NbrArgs, 256)
 
 hipError_t launch_neighbors_multi(const NbrArgs* a, int G, Arena* ar, hipStream_t s) {
     std::vector<uint32_t> nb(G);
-    for (int g = 0; g < G; g++) nb[g] = a[g].n > 0 ? uint32_t((u64(a[g].n) * 27 + 255) / 256) : 0;
+    for (int g = 0; g < G; g++) nb[g] = a[g].n > 0 ? uint32_t((u64(a[g].n) * 32 + 255) / 256) : 0;
     MultiLaunch L;
     hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
     if (e != hipSuccess || L.blocks == 0) return e;

@@ -264,7 +264,7 @@ struct NbrArgs {
     Tree t;
     int* nbr;
     int n;
-    uint32_t* mask;       // optional, zeroed: bit d set when nbr[b*27 + d] >= 0
+    uint32_t* mask;       // optional: bit d set when nbr[b*27 + d] >= 0 (written whole)
     long long* base;      // optional: atlas offset of each neighbour's brick (d = 13: own), -1 if none
 };
 hipError_t launch_neighbors_multi(const NbrArgs* a, int G, Arena* ar, hipStream_t s);
