@@ -120,13 +120,22 @@ __device__ inline void k_apron_body(const ApronArgs& ap, uint32_t bid, uint32_t)
     int dx = x == 0 ? -1 : (x == br - 1 ? 1 : 0);
     int dy = y == 0 ? -1 : (y == br - 1 ? 1 : 0);
     int dz = z == 0 ? -1 : (z == br - 1 ? 1 : 0);
-    const long long nbase = base[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+    const int d = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+    const long long nbase = base[d];
     const u64 own = u64(base[13]) + u64((z * br + y) * br + x);
     float v = boundval;
     if (nbase >= 0) {
-        // interior coordinate inside the neighbour, shifted by one for the apron
-        int nx = (x - 1) - dx * w + 1, ny = (y - 1) - dy * w + 1, nz = (z - 1) - dz * w + 1;
-        v = ld<AT>(atlas, u64(nbase) + u64((nz * br + ny) * br + nx));
+        // interior coordinate inside the neighbour
+        const int sx = (x - 1) - dx * w, sy = (y - 1) - dy * w, sz = (z - 1) - dz * w;
+        if (ap.faces) {
+            // it lies on the neighbour's face towards this brick (x face first)
+            const int f = dx ? (dx < 0 ? 1 : 0) : (dy ? (dy < 0 ? 3 : 2) : (dz < 0 ? 5 : 4));
+            const int i = dx ? sz * 8 + sy : (dy ? sz * 8 + sx : sy * 8 + sx);
+            const int nb = ap.nbr[u64(b) * 27 + d];
+            v = ld<AT>(reinterpret_cast<const AT*>(ap.faces), u64(nb) * FACE_VALS + u64(f * 64 + i));
+        } else {
+            v = ld<AT>(atlas, u64(nbase) + u64(((sz + 1) * br + sy + 1) * br + sx + 1));
+        }
     }
     st<AT>(atlas, own, v);
 }

@@ -253,8 +253,14 @@ struct BucketGatherArgs {
     void* atlas;
     int atlas_type;
     int accumulate;
+    void* faces;          // optional (scatter kernel): the six 8x8 boundary faces of each brick, channel type
 };
 hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* ar, hipStream_t s);
+// Face buffer layout: brick b's faces at b * FACE_VALS; face f = x0, x7, y0,
+// y7, z0, z7 (local interior coordinate 0 or 7) at f * 64, each face row-major
+// in its other two coordinates, the higher one first (x faces: z*8 + y; y
+// faces: z*8 + x; z faces: y*8 + x).
+static const int FACE_VALS = 384;
 
 // ---- apron (apron.hip)
 // nbr[b*27 + d]: leaf holding the neighbour of brick b in direction
@@ -272,6 +278,8 @@ hipError_t launch_neighbors_multi(const NbrArgs* a, int G, Arena* ar, hipStream_
 struct ApronArgs {
     Tree t;
     const long long* base;   // neighbour brick offsets from the neighbour pass
+    const int* nbr;          // neighbour leaves (read with faces)
+    const void* faces;       // optional: the gather's face buffer, current with the atlas
     void* atlas;
     int atlas_type;
     float boundval;

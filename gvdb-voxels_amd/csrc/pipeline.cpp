@@ -360,6 +360,7 @@ int Volume::multi_update_atlas(Volume* const* vs, int G, const Exec& ex, int* fa
         v->host_valid_ = false;
         v->compute_top_lev();
         v->atlas_gen_ = v->topo_gen_;
+        v->atlas_epoch_++;
     }
     if (G == 1) v0->stage_end(ST_ATLAS);
     return GVDB_OK;
@@ -372,6 +373,7 @@ int Volume::multi_clear_channel(Volume* const* vs, int G, int c, const Exec& ex,
         Volume* v = vs[g];
         if (c < 0 || c >= 8 || !v->chan_[c].exists) FAILG(g, v->fail(GVDB_ERR_INVALID, "ClearChannel: no such channel"));
         fa.push_back(FillArgs{v->chan_[c].data.p, v->chan_[c].data.bytes / 4, 0, 0});
+        v->atlas_epoch_++;
     }
     if (G == 1) vs[0]->stage_begin(ST_CLEAR);
     XCHK(vs[0], launch_fill_multi(fa.data(), G, ex.ar, ex.s));
@@ -641,11 +643,23 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
         if (rc) FAILG(g, rc);
         const bool buckets = v->bkt_gen_ == v->topo_gen_ && scs == v->bkt_scs_ && radius[g] == v->bkt_radius_ &&
                              v->res_[0] == 8 && scs == 4 && v->bkt_R_ >= 1 && v->bkt_R_ <= 2;
+        v->atlas_epoch_++;
         if (buckets) {
+            // the scatter kernel (no accumulate) also leaves the bricks' faces for the apron
+            void* faces = nullptr;
+            if (!accum) {
+                const size_t need = size_t(v->pool_[0][0].lastEle) * FACE_VALS * v->chan_[chanD].elem();
+                if (v->faces_.ensure(need) == hipSuccess) {
+                    faces = v->faces_.p;
+                    v->faces_epoch_ = v->atlas_epoch_;
+                    v->faces_topo_ = v->topo_gen_;
+                    v->faces_chan_ = chanD;
+                }
+            }
             bg.push_back(BucketGatherArgs{v->make_tree(), int(v->pool_[0][0].lastEle), v->bkt_R_, radius[g],
                                           v->nbr_.as<int>(), v->bkt_cnt_.as<uint32_t>(), v->bkt_off_.as<uint32_t>(),
                                           v->sorted_.as<float4>(), v->chan_[chanD].data.p, v->chan_[chanD].type,
-                                          accum ? 1 : 0});
+                                          accum ? 1 : 0, faces});
             continue;
         }
         if (v->entries_stale_) need.push_back(v);
@@ -699,7 +713,9 @@ int Volume::multi_apron(Volume* const* vs, int G, int c, float boundval, const E
             na.push_back(NbrArgs{t, v->nbr_.as<int>(), bricks, v->nmask_.as<uint32_t>(), v->nbase_.as<long long>()});
             v->nbr_gen_ = v->topo_gen_;
         }
-        aa.push_back(ApronArgs{t, v->nbase_.as<long long>(), v->chan_[c].data.p, v->chan_[c].type, boundval, bricks});
+        const bool faces = v->faces_chan_ == c && v->faces_epoch_ == v->atlas_epoch_ && v->faces_topo_ == v->topo_gen_;
+        aa.push_back(ApronArgs{t, v->nbase_.as<long long>(), v->nbr_.as<int>(), faces ? v->faces_.p : nullptr,
+                               v->chan_[c].data.p, v->chan_[c].type, boundval, bricks});
     }
     if (aa.empty()) return GVDB_OK;
     Volume* v0 = vs[0];

@@ -1218,6 +1218,16 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
     if (k != ~0ull) val = exact ? fmaxf(0.0f, falloff_from_d2(k, sh, r)) : __uint_as_float(0x7F800000u - uint32_t(k));
     AT* atlas = reinterpret_cast<AT*>(a.atlas);
     store_atlas<AT>(atlas, atlas_offset(t, bn->mValue[0] + lx, bn->mValue[1] + ly, bn->mValue[2] + lz), val);
+    if (a.faces) {
+        // the brick's boundary layers, compact, for the apron pass
+        AT* F = reinterpret_cast<AT*>(a.faces) + u64(brick) * FACE_VALS;
+        if (lx == 0) store_atlas<AT>(F, 0 * 64 + lz * 8 + ly, val);
+        if (lx == 7) store_atlas<AT>(F, 1 * 64 + lz * 8 + ly, val);
+        if (ly == 0) store_atlas<AT>(F, 2 * 64 + lz * 8 + lx, val);
+        if (ly == 7) store_atlas<AT>(F, 3 * 64 + lz * 8 + lx, val);
+        if (lz == 0) store_atlas<AT>(F, 4 * 64 + ly * 8 + lx, val);
+        if (lz == 7) store_atlas<AT>(F, 5 * 64 + ly * 8 + lx, val);
+    }
 }
 // Written out (not GVDB_MULTI_T) to ask for four workgroups per CU: at most
 // 64 VGPRs, so eight waves per SIMD.

@@ -95,7 +95,7 @@ Volume::~Volume() {
         for (int l = 0; l < MAXLEV; l++) pool_[g][l].dev.release();
     for (auto& c : chan_) c.data.release();
     for (auto& r : rbuf_) r.data.release();
-    DevBuf* bufs[] = {&amap_, &nbr_, &nmask_, &nbase_, &lenparts_, &pts_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_, &sc_use_,
+    DevBuf* bufs[] = {&amap_, &nbr_, &nmask_, &nbase_, &faces_, &lenparts_, &pts_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_, &sc_use_,
                       &bkt_key_, &bkt_slot_, &bkt_cnt_, &bkt_off_, &sorted_, &sc_flags_, &small_, &masks_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev};
     for (DevBuf* b : bufs) b->release();
     for (int s = 0; s < ST_COUNT; s++)
@@ -234,6 +234,7 @@ int Volume::add_channel(int c, int dtype, int apron, int filter, int border, int
     } else if (ax != atlas_cnt_[0] || ay != atlas_cnt_[1] || az != atlas_cnt_[2] || apron != apron_) {
         return fail(GVDB_ERR_UNSUPPORTED, "AddChannel: all channels share channel 0's atlas layout");
     }
+    atlas_epoch_++;
     Channel& ch = chan_[c];
     ch.exists = true;
     ch.type = dtype;
@@ -561,6 +562,7 @@ int Volume::resize_atlas(int cz) {
     }
     atlas_cnt_[2] = cz;
     atlas_max_ = slots;
+    atlas_epoch_++;
     return GVDB_OK;
 }
 
@@ -665,7 +667,7 @@ unsigned long long Volume::device_bytes() const {
         for (int l = 0; l < MAXLEV; l++) b += pool_[g][l].dev.bytes;
     for (const auto& c : chan_) b += c.data.bytes;
     for (const auto& r : rbuf_) b += r.data.bytes;
-    const DevBuf* bufs[] = {&amap_, &nbr_, &nmask_, &nbase_, &lenparts_, &pts_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_,
+    const DevBuf* bufs[] = {&amap_, &nbr_, &nmask_, &nbase_, &faces_, &lenparts_, &pts_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_,
                             &sc_use_, &bkt_key_, &bkt_slot_, &bkt_cnt_, &bkt_off_, &sorted_, &sc_flags_, &small_,
                             &masks_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev};
     for (const DevBuf* d : bufs) b += d->bytes;

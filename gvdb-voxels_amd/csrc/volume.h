@@ -185,6 +185,11 @@ private:
     DevBuf amap_;
     u64 amap_max_ = 0;
     DevBuf nbr_, nmask_, nbase_, lenparts_;
+    // The scatter gather's copy of each brick's boundary faces, current while
+    // nothing else has written the atlas (atlas_epoch_ counts atlas writes).
+    DevBuf faces_;
+    unsigned atlas_epoch_ = 0, faces_epoch_ = ~0u, faces_topo_ = ~0u;
+    int faces_chan_ = -1;
     unsigned nbr_gen_ = ~0u;
 
     // points

@@ -160,6 +160,45 @@ def test_render_fp32_walk_at_large_coordinates(gpu):
     assert np.abs(img - ref).max() < 5e-3
 
 
+def test_apron_follows_every_atlas_write(gpu):
+    """The apron copies neighbour faces from the gather's compact face buffer
+    only while nothing else has written the atlas.  After a ClearChannel, a
+    gather into another channel, an accumulate gather or a second apron, each
+    apron must equal what the brick interiors say (the oracle's apron of the
+    same interiors)."""
+    import oracle as O
+    pts = _sphere(20000)
+    n = len(pts)
+    v, m = H.gpu_volume(gpu, pts, 2.0)
+    v.AddChannel(1, gpu.T_FLOAT, 1)
+    o, _ = H.oracle_volume(O, pts, 2.0, precision=64)
+    mask = H.used_brick_mask(o.atlas(64).shape, o.atlas_map(), o.atlas_info()["cnt"])
+    z = (0.0, 0.0, 0.0)
+
+    def check(expect_zero):
+        a, b = v.atlas(0), o.atlas(64)
+        if expect_zero:
+            assert not np.any(a[mask]), "apron kept densities after ClearChannel"
+        else:
+            _, rel, zeros = H.density_errors(a[mask], b[mask])
+            assert zeros == 0 and rel <= REL_TOL
+    check(False)                                   # gather -> apron (face buffer)
+    v.UpdateApron(0, 0.0)                          # a second apron, same result
+    check(False)
+    v.ClearChannel(0)
+    v.UpdateApron(0, 0.0)                          # faces are stale now: all zero
+    check(True)
+    v.GatherDensity(4, n, 2.0, z, m, 0, 0, False)
+    v.GatherDensity(4, n, 2.0, z, m, 1, 0, False)  # channel 1 takes the face buffer
+    v.UpdateApron(0, 0.0)
+    check(False)
+    v.ClearChannel(0)
+    v.GatherDensity(4, n, 2.0, z, m, 0, 0, True)   # accumulate over zeros: same interiors
+    v.UpdateApron(0, 0.0)
+    check(False)
+    v.close()
+
+
 def test_host_topology_path(gpu):
     """RebuildTopologyCPU (host ActivateSpace) then the GPU stages."""
     import oracle as O
