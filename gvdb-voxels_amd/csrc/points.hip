@@ -886,7 +886,7 @@ struct GBShared {
     uint16_t lst[GB_THREADS / 64][GB_CHUNK];
 };
 
-template <class SH>
+template <int NT = GB_THREADS, class SH>
 __device__ inline uint32_t gb_setup(int brick, int R, const int* nbr27, const uint32_t* bcnt, const uint32_t* boff,
                                     SH& S) {
     const int sd = 2, span = sd + 2 * R, ncell = span * span * span;
@@ -917,7 +917,7 @@ __device__ inline uint32_t gb_setup(int brick, int R, const int* nbr27, const ui
     if (lane == 63) S.wsum[wave] = x;
     __syncthreads();
     uint32_t pre = x - cnt, total = 0;
-    for (int w = 0; w < GB_THREADS / 64; w++) {
+    for (int w = 0; w < NT / 64; w++) {
         if (w < wave) pre += S.wsum[w];
         total += S.wsum[w];
     }
@@ -1072,14 +1072,16 @@ GVDB_MULTI_T(k_bucket_gather, typename AT, AT, BucketGatherArgs, GB_THREADS)
 // slice are padded to 65 slots so the slices of one candidate fall in
 // different banks.
 static const int PG_ZSTRIDE = 65;
+static const int PG_THREADS = 256;               // four waves per brick, two voxels per thread
+static const int PG_CHUNK = 2 * PG_THREADS;
 struct PGShared {
-    uint32_t cell_off[GB_THREADS];
-    uint32_t cell_start[GB_THREADS];
+    uint32_t cell_off[PG_THREADS];               // >= (2 + 2R)^3 = 216 cells for R <= 2
+    uint32_t cell_start[PG_THREADS];
     int nbr[27];
-    uint32_t wsum[GB_THREADS / 64];
+    uint32_t wsum[PG_THREADS / 64];
     uint32_t ncand;
     u64 key[8 * PG_ZSTRIDE];
-    float4 pt[GB_CHUNK];
+    float4 pt[PG_CHUNK];
 };
 
 // r - sqrt(D) 2^-s, accurate to a few ulps: D split into float hi + lo
@@ -1104,10 +1106,9 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
     const Node* bn = t.nodes[0] + brick;
     if (bn->mParent == ID_UNDEF64) return;        // unlinked leaves have no subcells
     const int tid = threadIdx.x, lane = tid & 63;
-    const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
-    const int vi = lz * PG_ZSTRIDE + ly * 8 + lx;
-    S.key[vi] = ~0ull;
-    const uint32_t T = gb_setup(brick, a.R, a.nbr, a.bcnt, a.boff, S);
+    // voxel v = tid + 256 j (j = 0, 1) of the brick: x = v & 7, y = (v >> 3) & 7, z = v >> 6
+    for (int v = tid; v < 512; v += PG_THREADS) S.key[(v >> 6) * PG_ZSTRIDE + (v & 63)] = ~0ull;
+    const uint32_t T = gb_setup<PG_THREADS>(brick, a.R, a.nbr, a.bcnt, a.boff, S);
     __syncthreads();                               // cell table and brick copy ready
     const float r = a.radius;
     const float r2cut = r * r * (1.0f + 0x1p-20f);
@@ -1128,11 +1129,11 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
     const uint32_t nz = uint32_t(2.0f * r + 0.0625f) + 1u;
     const uint32_t nzinv = uint32_t(0xFFFFFFFFu / nz) + 1u;
     const u64 lower = lane ? ((~0ull) >> (64 - lane)) : 0ull;
-    for (uint32_t c0 = 0; c0 < T; c0 += GB_CHUNK) {
+    for (uint32_t c0 = 0; c0 < T; c0 += PG_CHUNK) {
         float4 p[2];
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const uint32_t f = c0 + uint32_t(h * GB_THREADS + tid);
+            const uint32_t f = c0 + uint32_t(h * PG_THREADS + tid);
             if (f < T) {
                 int lo = 0, hi = ncell - 1;        // last cell starting at or before f
                 while (lo < hi) {
@@ -1147,7 +1148,7 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
         __syncthreads();                           // ncand cleared; previous chunk done with pt
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const uint32_t f = c0 + uint32_t(h * GB_THREADS + tid);
+            const uint32_t f = c0 + uint32_t(h * PG_THREADS + tid);
             bool keep = false;
             if (f < T) {
                 const float4 q = p[h];
@@ -1162,7 +1163,7 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
         }
         __syncthreads();
         const uint32_t items = S.ncand * nz;
-        for (uint32_t it = uint32_t(tid); it < items; it += GB_THREADS) {
+        for (uint32_t it = uint32_t(tid); it < items; it += PG_THREADS) {
             uint32_t c = __umulhi(it, nzinv);
             if (c * nz > it) c--;
             const int zi = int(it - c * nz);
@@ -1213,26 +1214,31 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
         __syncthreads();
     }
     __syncthreads();
-    const u64 k = S.key[vi];
-    float val = 0.0f;
-    if (k != ~0ull) val = exact ? fmaxf(0.0f, falloff_from_d2(k, sh, r)) : __uint_as_float(0x7F800000u - uint32_t(k));
     AT* atlas = reinterpret_cast<AT*>(a.atlas);
-    store_atlas<AT>(atlas, atlas_offset(t, bn->mValue[0] + lx, bn->mValue[1] + ly, bn->mValue[2] + lz), val);
-    if (a.faces) {
-        // the brick's boundary layers, compact, for the apron pass
-        AT* F = reinterpret_cast<AT*>(a.faces) + u64(brick) * FACE_VALS;
-        if (lx == 0) store_atlas<AT>(F, 0 * 64 + lz * 8 + ly, val);
-        if (lx == 7) store_atlas<AT>(F, 1 * 64 + lz * 8 + ly, val);
-        if (ly == 0) store_atlas<AT>(F, 2 * 64 + lz * 8 + lx, val);
-        if (ly == 7) store_atlas<AT>(F, 3 * 64 + lz * 8 + lx, val);
-        if (lz == 0) store_atlas<AT>(F, 4 * 64 + ly * 8 + lx, val);
-        if (lz == 7) store_atlas<AT>(F, 5 * 64 + ly * 8 + lx, val);
+    AT* F = a.faces ? reinterpret_cast<AT*>(a.faces) + u64(brick) * FACE_VALS : nullptr;
+    for (int v = tid; v < 512; v += PG_THREADS) {
+        const int lx = v & 7, ly = (v >> 3) & 7, lz = v >> 6;
+        const u64 k = S.key[lz * PG_ZSTRIDE + (v & 63)];
+        float val = 0.0f;
+        if (k != ~0ull)
+            val = exact ? fmaxf(0.0f, falloff_from_d2(k, sh, r)) : __uint_as_float(0x7F800000u - uint32_t(k));
+        store_atlas<AT>(atlas, atlas_offset(t, bn->mValue[0] + lx, bn->mValue[1] + ly, bn->mValue[2] + lz), val);
+        if (F) {
+            // the brick's boundary layers, compact, for the apron pass
+            if (lx == 0) store_atlas<AT>(F, 0 * 64 + lz * 8 + ly, val);
+            if (lx == 7) store_atlas<AT>(F, 1 * 64 + lz * 8 + ly, val);
+            if (ly == 0) store_atlas<AT>(F, 2 * 64 + lz * 8 + lx, val);
+            if (ly == 7) store_atlas<AT>(F, 3 * 64 + lz * 8 + lx, val);
+            if (lz == 0) store_atlas<AT>(F, 4 * 64 + ly * 8 + lx, val);
+            if (lz == 7) store_atlas<AT>(F, 5 * 64 + ly * 8 + lx, val);
+        }
     }
 }
-// Written out (not GVDB_MULTI_T) to ask for four workgroups per CU: at most
-// 64 VGPRs, so eight waves per SIMD.
+
+// Written out (not GVDB_MULTI_T) to ask for eight waves per SIMD (at most 64
+// VGPRs).
 template <typename AT>
-__global__ __launch_bounds__(GB_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_push_gather(
+__global__ __launch_bounds__(PG_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_push_gather(
     BucketGatherArgs a0, MultiMap m) {
     if (m.G <= 1) {
         k_push_gather_body<AT>(a0, blockIdx.x, gridDim.x);
@@ -1261,8 +1267,8 @@ hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* a
         hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
         if (e != hipSuccess) return e;
         if (L.blocks == 0) continue;
-        if (push && half) hipLaunchKernelGGL(k_push_gather<__half>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
-        else if (push) hipLaunchKernelGGL(k_push_gather<float>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
+        if (push && half) hipLaunchKernelGGL(k_push_gather<__half>, dim3(L.blocks), dim3(PG_THREADS), 0, s, sel[0], L.map);
+        else if (push) hipLaunchKernelGGL(k_push_gather<float>, dim3(L.blocks), dim3(PG_THREADS), 0, s, sel[0], L.map);
         else if (half) hipLaunchKernelGGL(k_bucket_gather<__half>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
         else hipLaunchKernelGGL(k_bucket_gather<float>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
         e = hipGetLastError();
