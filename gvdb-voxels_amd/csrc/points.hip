@@ -1072,16 +1072,17 @@ GVDB_MULTI_T(k_bucket_gather, typename AT, AT, BucketGatherArgs, GB_THREADS)
 // slice are padded to 65 slots so the slices of one candidate fall in
 // different banks.
 static const int PG_ZSTRIDE = 65;
-static const int PG_THREADS = 256;               // four waves per brick, two voxels per thread
-static const int PG_CHUNK = 2 * PG_THREADS;
+// NT threads per brick: 128 (two waves) when the (2 + 2R)^3 cells fit one
+// per thread (R = 1: 64), else 256 (R = 2: 216 cells).
+template <int NT>
 struct PGShared {
-    uint32_t cell_off[PG_THREADS];               // >= (2 + 2R)^3 = 216 cells for R <= 2
-    uint32_t cell_start[PG_THREADS];
+    uint32_t cell_off[NT];
+    uint32_t cell_start[NT];
     int nbr[27];
-    uint32_t wsum[PG_THREADS / 64];
+    uint32_t wsum[NT / 64];
     uint32_t ncand;
     u64 key[8 * PG_ZSTRIDE];
-    float4 pt[PG_CHUNK];
+    float4 pt[2 * NT];
 };
 
 // r - sqrt(D) 2^-s, accurate to a few ulps: D split into float hi + lo
@@ -1098,17 +1099,17 @@ __device__ inline float falloff_from_d2(u64 D, int s, float r) {
     return num / (r + __builtin_sqrtf(hi));
 }
 
-template <typename AT>
+template <int NT, typename AT>
 __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bid, uint32_t) {
-    __shared__ PGShared S;
+    __shared__ PGShared<NT> S;
     const Tree& t = a.t;
     const int brick = int(bid);
     const Node* bn = t.nodes[0] + brick;
     if (bn->mParent == ID_UNDEF64) return;        // unlinked leaves have no subcells
     const int tid = threadIdx.x, lane = tid & 63;
     // voxel v = tid + 256 j (j = 0, 1) of the brick: x = v & 7, y = (v >> 3) & 7, z = v >> 6
-    for (int v = tid; v < 512; v += PG_THREADS) S.key[(v >> 6) * PG_ZSTRIDE + (v & 63)] = ~0ull;
-    const uint32_t T = gb_setup<PG_THREADS>(brick, a.R, a.nbr, a.bcnt, a.boff, S);
+    for (int v = tid; v < 512; v += NT) S.key[(v >> 6) * PG_ZSTRIDE + (v & 63)] = ~0ull;
+    const uint32_t T = gb_setup<NT>(brick, a.R, a.nbr, a.bcnt, a.boff, S);
     __syncthreads();                               // cell table and brick copy ready
     const float r = a.radius;
     const float r2cut = r * r * (1.0f + 0x1p-20f);
@@ -1129,11 +1130,11 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
     const uint32_t nz = uint32_t(2.0f * r + 0.0625f) + 1u;
     const uint32_t nzinv = uint32_t(0xFFFFFFFFu / nz) + 1u;
     const u64 lower = lane ? ((~0ull) >> (64 - lane)) : 0ull;
-    for (uint32_t c0 = 0; c0 < T; c0 += PG_CHUNK) {
+    for (uint32_t c0 = 0; c0 < T; c0 += (2 * NT)) {
         float4 p[2];
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const uint32_t f = c0 + uint32_t(h * PG_THREADS + tid);
+            const uint32_t f = c0 + uint32_t(h * NT + tid);
             if (f < T) {
                 int lo = 0, hi = ncell - 1;        // last cell starting at or before f
                 while (lo < hi) {
@@ -1148,7 +1149,7 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
         __syncthreads();                           // ncand cleared; previous chunk done with pt
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const uint32_t f = c0 + uint32_t(h * PG_THREADS + tid);
+            const uint32_t f = c0 + uint32_t(h * NT + tid);
             bool keep = false;
             if (f < T) {
                 const float4 q = p[h];
@@ -1163,7 +1164,7 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
         }
         __syncthreads();
         const uint32_t items = S.ncand * nz;
-        for (uint32_t it = uint32_t(tid); it < items; it += PG_THREADS) {
+        for (uint32_t it = uint32_t(tid); it < items; it += NT) {
             uint32_t c = __umulhi(it, nzinv);
             if (c * nz > it) c--;
             const int zi = int(it - c * nz);
@@ -1216,7 +1217,7 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
     __syncthreads();
     AT* atlas = reinterpret_cast<AT*>(a.atlas);
     AT* F = a.faces ? reinterpret_cast<AT*>(a.faces) + u64(brick) * FACE_VALS : nullptr;
-    for (int v = tid; v < 512; v += PG_THREADS) {
+    for (int v = tid; v < 512; v += NT) {
         const int lx = v & 7, ly = (v >> 3) & 7, lz = v >> 6;
         const u64 k = S.key[lz * PG_ZSTRIDE + (v & 63)];
         float val = 0.0f;
@@ -1237,28 +1238,31 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
 
 // Written out (not GVDB_MULTI_T) to ask for eight waves per SIMD (at most 64
 // VGPRs).
-template <typename AT>
-__global__ __launch_bounds__(PG_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_push_gather(
-    BucketGatherArgs a0, MultiMap m) {
+template <int NT, typename AT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_push_gather(BucketGatherArgs a0,
+                                                                                              MultiMap m) {
     if (m.G <= 1) {
-        k_push_gather_body<AT>(a0, blockIdx.x, gridDim.x);
+        k_push_gather_body<NT, AT>(a0, blockIdx.x, gridDim.x);
         return;
     }
     const int g = multi_grid(m, blockIdx.x);
     const BucketGatherArgs* tab = reinterpret_cast<const BucketGatherArgs*>(m.table);
-    k_push_gather_body<AT>(tab[g], blockIdx.x - m.first[g], m.first[g + 1] - m.first[g]);
+    k_push_gather_body<NT, AT>(tab[g], blockIdx.x - m.first[g], m.first[g + 1] - m.first[g]);
 }
 
 hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* ar, hipStream_t s) {
-    // accumulate mode keeps the nearest-point kernel: there the reference's
-    // max also takes the negative r - |p - v| of entries beyond the radius
-    for (int variant = 0; variant < 4; variant++) {
-        const bool half = variant & 1, push = variant < 2;
+    // variants: scatter kernel (no accumulate) with 128 or 256 threads, or the
+    // nearest-entry kernel (accumulate: there the reference's max also takes
+    // the negative r - |p - v| of entries beyond the radius); FP32 or FP16
+    for (int variant = 0; variant < 6; variant++) {
+        const bool half = variant & 1;
+        const int kind = variant >> 1;             // 0: scatter, 128; 1: scatter, 256; 2: nearest entry
         std::vector<BucketGatherArgs> sel;
         std::vector<uint32_t> nb;
         for (int g = 0; g < G; g++) {
             if (a[g].t.res[0] != 8 || a[g].R < 1 || a[g].R > 2) return hipErrorInvalidValue;
-            if ((a[g].atlas_type == T_HALF) != half || (a[g].accumulate == 0) != push || a[g].nleaves <= 0) continue;
+            const int k = a[g].accumulate ? 2 : (a[g].R == 1 ? 0 : 1);
+            if ((a[g].atlas_type == T_HALF) != half || k != kind || a[g].nleaves <= 0) continue;
             sel.push_back(a[g]);
             nb.push_back(uint32_t(a[g].nleaves));
         }
@@ -1267,10 +1271,13 @@ hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* a
         hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
         if (e != hipSuccess) return e;
         if (L.blocks == 0) continue;
-        if (push && half) hipLaunchKernelGGL(k_push_gather<__half>, dim3(L.blocks), dim3(PG_THREADS), 0, s, sel[0], L.map);
-        else if (push) hipLaunchKernelGGL(k_push_gather<float>, dim3(L.blocks), dim3(PG_THREADS), 0, s, sel[0], L.map);
-        else if (half) hipLaunchKernelGGL(k_bucket_gather<__half>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
-        else hipLaunchKernelGGL(k_bucket_gather<float>, dim3(L.blocks), dim3(GB_THREADS), 0, s, sel[0], L.map);
+        const dim3 grid(L.blocks);
+        if (kind == 0 && half) hipLaunchKernelGGL((k_push_gather<128, __half>), grid, dim3(128), 0, s, sel[0], L.map);
+        else if (kind == 0) hipLaunchKernelGGL((k_push_gather<128, float>), grid, dim3(128), 0, s, sel[0], L.map);
+        else if (kind == 1 && half) hipLaunchKernelGGL((k_push_gather<256, __half>), grid, dim3(256), 0, s, sel[0], L.map);
+        else if (kind == 1) hipLaunchKernelGGL((k_push_gather<256, float>), grid, dim3(256), 0, s, sel[0], L.map);
+        else if (half) hipLaunchKernelGGL(k_bucket_gather<__half>, grid, dim3(GB_THREADS), 0, s, sel[0], L.map);
+        else hipLaunchKernelGGL(k_bucket_gather<float>, grid, dim3(GB_THREADS), 0, s, sel[0], L.map);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
