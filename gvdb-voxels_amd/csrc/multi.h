@@ -16,17 +16,32 @@
 
 namespace gvdb {
 
+// A few small records travel inside the kernel arguments (no staging copy,
+// no extra launch); larger sets go through a device table.
+static const int MULTI_INLINE = 8;
+static const int MULTI_INLINE_BYTES = 2048;
+
 struct MultiMap {
     int G;
-    const uint32_t* first;   // G + 1 entries
-    const void* table;       // G argument records
+    const uint32_t* first;   // G + 1 entries (device), or null: inline
+    const void* table;       // G argument records (device), or null: inline
+    uint32_t ifirst[MULTI_INLINE + 1];
+    alignas(16) unsigned char itab[MULTI_INLINE_BYTES];
 };
 
+__device__ inline const uint32_t* multi_first(const MultiMap& m) { return m.first ? m.first : m.ifirst; }
+
+template <class A>
+__device__ inline const A& multi_rec(const MultiMap& m, int g) {
+    return reinterpret_cast<const A*>(m.table ? m.table : static_cast<const void*>(m.itab))[g];
+}
+
 __device__ inline int multi_grid(const MultiMap& m, uint32_t b) {
+    const uint32_t* first = multi_first(m);
     int lo = 0, hi = m.G - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (m.first[mid] <= b) lo = mid;
+        if (first[mid] <= b) lo = mid;
         else hi = mid - 1;
     }
     return lo;
@@ -49,8 +64,8 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t n) {
             return;                                                                              \
         }                                                                                        \
         const int g = multi_grid(m, blockIdx.x);                                                 \
-        const A* tab = reinterpret_cast<const A*>(m.table);                                      \
-        NAME##_body(tab[g], blockIdx.x - m.first[g], m.first[g + 1] - m.first[g]);               \
+        const uint32_t* first = multi_first(m);                                                  \
+        NAME##_body(multi_rec<A>(m, g), blockIdx.x - first[g], first[g + 1] - first[g]);         \
     }
 
 // Same for a body templated on one parameter (TDECL e.g. `typename AT`).
@@ -62,8 +77,8 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t n) {
             return;                                                                              \
         }                                                                                        \
         const int g = multi_grid(m, blockIdx.x);                                                 \
-        const A* tab = reinterpret_cast<const A*>(m.table);                                      \
-        NAME##_body<TNAME>(tab[g], blockIdx.x - m.first[g], m.first[g + 1] - m.first[g]);        \
+        const uint32_t* first = multi_first(m);                                                  \
+        NAME##_body<TNAME>(multi_rec<A>(m, g), blockIdx.x - first[g], first[g + 1] - first[g]);  \
     }
 
 // Pinned staging for argument tables: bump-allocated, copied to the device
@@ -82,7 +97,7 @@ struct Arena {
 
 // Launch geometry for G grids needing nb[g] blocks each.
 struct MultiLaunch {
-    MultiMap map{1, nullptr, nullptr};
+    MultiMap map{1, nullptr, nullptr, {}, {}};
     uint32_t blocks = 0;
 };
 

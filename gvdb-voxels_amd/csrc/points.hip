@@ -1248,8 +1248,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         return;
     }
     const int g = multi_grid(m, blockIdx.x);
-    const BucketGatherArgs* tab = reinterpret_cast<const BucketGatherArgs*>(m.table);
-    k_push_gather_body<NT, AT>(tab[g], blockIdx.x - m.first[g], m.first[g + 1] - m.first[g]);
+    const uint32_t* first = multi_first(m);
+    k_push_gather_body<NT, AT>(multi_rec<BucketGatherArgs>(m, g), blockIdx.x - first[g], first[g + 1] - first[g]);
 }
 
 hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* ar, hipStream_t s) {

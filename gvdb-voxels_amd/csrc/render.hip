@@ -384,7 +384,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(render_waves
         return;
     }
     const int g = multi_grid(m, blockIdx.x);
-    k_render_body<MODE, AT, U>(reinterpret_cast<const RenderArgs*>(m.table)[g], blockIdx.x - m.first[g], S);
+    k_render_body<MODE, AT, U>(multi_rec<RenderArgs>(m, g), blockIdx.x - multi_first(m)[g], S);
 }
 
 // Samples fetched ahead per brick step (1, 2 or 4), per march mode: 4 for

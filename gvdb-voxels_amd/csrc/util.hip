@@ -42,14 +42,15 @@ void* Arena::push(const void* src, size_t n, hipStream_t s, hipError_t& e) {
 
 hipError_t multi_setup_raw(Arena* ar, const void* args, size_t rec, const uint32_t* nb, int G, hipStream_t s,
                            MultiLaunch& out) {
-    out.map = MultiMap{1, nullptr, nullptr};
+    out.map.G = 1;
+    out.map.first = nullptr;
+    out.map.table = nullptr;
     out.blocks = 0;
     if (G <= 0) return hipSuccess;
     if (G == 1) {
         out.blocks = nb[0];
         return hipSuccess;
     }
-    if (!ar) return hipErrorInvalidValue;
     std::vector<uint32_t> first(size_t(G) + 1);
     u64 acc = 0;
     for (int g = 0; g < G; g++) {
@@ -60,6 +61,13 @@ hipError_t multi_setup_raw(Arena* ar, const void* args, size_t rec, const uint32
     first[G] = uint32_t(acc);
     out.blocks = uint32_t(acc);
     if (acc == 0) return hipSuccess;
+    if (G <= MULTI_INLINE && rec * size_t(G) <= size_t(MULTI_INLINE_BYTES)) {
+        out.map.G = G;
+        memcpy(out.map.ifirst, first.data(), first.size() * 4);
+        memcpy(out.map.itab, args, rec * size_t(G));
+        return hipSuccess;
+    }
+    if (!ar) return hipErrorInvalidValue;
     hipError_t e = hipSuccess;
     void* t = ar->push(args, rec * size_t(G), s, e);
     if (!t) return e;
@@ -96,14 +104,6 @@ static uint32_t fill_grid(u64 n) {
 }
 
 hipError_t launch_fill_multi(const FillArgs* a, int G, Arena* ar, hipStream_t s) {
-    if (G > 1 && G <= 4) {
-        // a handful of fills without staging: one launch each
-        for (int g = 0; g < G; g++) {
-            hipError_t e = launch_fill_multi(a + g, 1, nullptr, s);
-            if (e != hipSuccess) return e;
-        }
-        return hipSuccess;
-    }
     std::vector<uint32_t> nb(G);
     for (int g = 0; g < G; g++) nb[g] = a[g].n ? fill_grid(a[g].n) : 0;
     MultiLaunch L;
