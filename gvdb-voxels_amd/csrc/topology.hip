@@ -40,11 +40,36 @@ static uint32_t grid_for(u64 items, uint32_t cap) {
 // keys6 must hold {~0 x3, 0 x3} before the launch.
 __device__ inline void k_bbox_body(const BboxArgs& a, uint32_t bid, uint32_t nblk) {
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int i = bid * 256 + threadIdx.x; i < a.pts.n; i += nblk * 256) {
-        const float* p = point_at(a.pts, i);
-        float w[3] = {p[0] + a.o[0], p[1] + a.o[1], p[2] + a.o[2]};
+    const PointSrc& P = a.pts;
+    if (P.stride == 12 && P.off == 0 && P.n < (1 << 30) && (reinterpret_cast<uintptr_t>(P.base) & 15) == 0) {
+        // packed float3: stream the array as float4 (four floats of axes
+        // j%3, j%3+1, j%3+2, j%3 for vector j), then the tail floats
+        const float4* v = reinterpret_cast<const float4*>(P.base);
+        const uint32_t nf = uint32_t(P.n) * 3u, nv = nf / 4u;
+        auto upd = [&](uint32_t ax, float f) {
+            const float w = f + (ax == 0 ? a.o[0] : (ax == 1 ? a.o[1] : a.o[2]));
+            if (ax == 0) { mn[0] = fminf(mn[0], w); mx[0] = fmaxf(mx[0], w); }
+            else if (ax == 1) { mn[1] = fminf(mn[1], w); mx[1] = fmaxf(mx[1], w); }
+            else { mn[2] = fminf(mn[2], w); mx[2] = fmaxf(mx[2], w); }
+        };
+        for (uint32_t j = bid * 256 + threadIdx.x; j < nv; j += nblk * 256) {
+            const float4 q = v[j];
+            const uint32_t ax = j % 3u;
+            upd(ax, q.x);
+            upd(ax == 2u ? 0u : ax + 1u, q.y);
+            upd(ax == 0u ? 2u : ax - 1u, q.z);
+            upd(ax, q.w);
+        }
+        const float* f = reinterpret_cast<const float*>(P.base);
+        if (bid == 0)
+            for (uint32_t k = nv * 4u + threadIdx.x; k < nf; k += 256) upd(k % 3u, f[k]);
+    } else {
+        for (int i = bid * 256 + threadIdx.x; i < P.n; i += nblk * 256) {
+            const float* p = point_at(P, i);
+            float w[3] = {p[0] + a.o[0], p[1] + a.o[1], p[2] + a.o[2]};
 #pragma unroll
-        for (int k = 0; k < 3; k++) { mn[k] = fminf(mn[k], w[k]); mx[k] = fmaxf(mx[k], w[k]); }
+            for (int k = 0; k < 3; k++) { mn[k] = fminf(mn[k], w[k]); mx[k] = fmaxf(mx[k], w[k]); }
+        }
     }
 #pragma unroll
     for (int k = 0; k < 3; k++) {
