@@ -42,27 +42,24 @@ __device__ inline void k_bbox_body(const BboxArgs& a, uint32_t bid, uint32_t nbl
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     const PointSrc& P = a.pts;
     if (P.stride == 12 && P.off == 0 && P.n < (1 << 30) && (reinterpret_cast<uintptr_t>(P.base) & 15) == 0) {
-        // packed float3: stream the array as float4 (four floats of axes
-        // j%3, j%3+1, j%3+2, j%3 for vector j), then the tail floats
+        // packed float3: four points = three float4 per thread (axes fixed:
+        // x y z x | y z x y | z x y z), then the last n % 4 points
         const float4* v = reinterpret_cast<const float4*>(P.base);
-        const uint32_t nf = uint32_t(P.n) * 3u, nv = nf / 4u;
-        auto upd = [&](uint32_t ax, float f) {
-            const float w = f + (ax == 0 ? a.o[0] : (ax == 1 ? a.o[1] : a.o[2]));
-            if (ax == 0) { mn[0] = fminf(mn[0], w); mx[0] = fmaxf(mx[0], w); }
-            else if (ax == 1) { mn[1] = fminf(mn[1], w); mx[1] = fmaxf(mx[1], w); }
-            else { mn[2] = fminf(mn[2], w); mx[2] = fmaxf(mx[2], w); }
-        };
-        for (uint32_t j = bid * 256 + threadIdx.x; j < nv; j += nblk * 256) {
-            const float4 q = v[j];
-            const uint32_t ax = j % 3u;
-            upd(ax, q.x);
-            upd(ax == 2u ? 0u : ax + 1u, q.y);
-            upd(ax == 0u ? 2u : ax - 1u, q.z);
-            upd(ax, q.w);
+        const uint32_t ng = uint32_t(P.n) / 4u;
+        const float ox = a.o[0], oy = a.o[1], oz = a.o[2];
+        auto ux = [&](float f) { f += ox; mn[0] = fminf(mn[0], f); mx[0] = fmaxf(mx[0], f); };
+        auto uy = [&](float f) { f += oy; mn[1] = fminf(mn[1], f); mx[1] = fmaxf(mx[1], f); };
+        auto uz = [&](float f) { f += oz; mn[2] = fminf(mn[2], f); mx[2] = fmaxf(mx[2], f); };
+        for (uint32_t g = bid * 256 + threadIdx.x; g < ng; g += nblk * 256) {
+            const float4 q0 = v[3 * u64(g)], q1 = v[3 * u64(g) + 1], q2 = v[3 * u64(g) + 2];
+            ux(q0.x); uy(q0.y); uz(q0.z); ux(q0.w);
+            uy(q1.x); uz(q1.y); ux(q1.z); uy(q1.w);
+            uz(q2.x); ux(q2.y); uy(q2.z); uz(q2.w);
         }
-        const float* f = reinterpret_cast<const float*>(P.base);
-        if (bid == 0)
-            for (uint32_t k = nv * 4u + threadIdx.x; k < nf; k += 256) upd(k % 3u, f[k]);
+        if (bid == 0 && threadIdx.x < uint32_t(P.n) - 4u * ng) {
+            const float* p = point_at(P, int(4u * ng + threadIdx.x));
+            ux(p[0]); uy(p[1]); uz(p[2]);
+        }
     } else {
         for (int i = bid * 256 + threadIdx.x; i < P.n; i += nblk * 256) {
             const float* p = point_at(P, i);
@@ -100,7 +97,7 @@ hipError_t launch_bbox_multi(const BboxArgs* a, int G, Arena* ar, hipStream_t s)
     for (int g = 0; g < G; g++) {
         f.push_back(FillArgs{a[g].keys6, 3, 0xFFFFFFFFull, 0});
         f.push_back(FillArgs{a[g].keys6 + 3, 3, 0ull, 0});
-        nb[g] = grid_for(u64(a[g].pts.n), 2048);
+        nb[g] = grid_for(u64(a[g].pts.n) / 4, 1024);   // each block ends in six same-address atomics
     }
     hipError_t e = launch_fill_multi(f.data(), int(f.size()), ar, s);
     if (e != hipSuccess) return e;
