@@ -104,7 +104,11 @@ int Batch::insert_points_subcell(Volume* const* v, int G, int scs, const int* n,
 }
 int Batch::gather_density(Volume* const* v, int G, int scs, const int* len, const float* radius, int chan, bool accum,
                           int* failed) {
-    BATCH_CALL(Volume::multi_gather(v, G, scs, len, radius, chan, accum, exec(), fp_));
+    BATCH_CALL(Volume::multi_gather(v, G, scs, len, radius, chan, accum, false, exec(), fp_));
+}
+int Batch::gather_level_set(Volume* const* v, int G, int scs, const int* len, const float* radius, int chan, bool accum,
+                            int* failed) {
+    BATCH_CALL(Volume::multi_gather(v, G, scs, len, radius, chan, accum, true, exec(), fp_));
 }
 int Batch::update_apron(Volume* const* v, int G, int chan, float boundval, int* failed) {
     BATCH_CALL(Volume::multi_apron(v, G, chan, boundval, exec(), fp_));

@@ -29,6 +29,8 @@ public:
     int clear_channel(Volume* const* v, int G, int chan, int* failed);
     int insert_points_subcell(Volume* const* v, int G, int scs, const int* n, const float* radius, const float* trans3,
                               int* len, int* failed);
+    int gather_level_set(Volume* const* v, int G, int scs, const int* len, const float* radius, int chan, bool accum,
+                         int* failed);
     int gather_density(Volume* const* v, int G, int scs, const int* len, const float* radius, int chan, bool accum,
                        int* failed);
     int update_apron(Volume* const* v, int G, int chan, float boundval, int* failed);

@@ -103,6 +103,13 @@ int gvdb_gather_density(gvdb_handle h, int subcell_size, int n, float radius, co
     const float z[3] = {0, 0, 0};
     return h->v.gather_density(subcell_size, n, radius, trans ? trans : z, sc_len, chan_density, chan_clr, accumulate != 0);
 }
+int gvdb_gather_level_set(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len,
+                          int chan_levelset, int chan_clr, int accumulate) {
+    H_OR_FAIL(h);
+    const float z[3] = {0, 0, 0};
+    return h->v.gather_level_set(subcell_size, n, radius, trans ? trans : z, sc_len, chan_levelset, chan_clr,
+                                 accumulate != 0);
+}
 int gvdb_update_apron(gvdb_handle h, int chan, float boundval) { H_OR_FAIL(h); return h->v.update_apron(chan, boundval); }
 
 int gvdb_set_camera_orbit(gvdb_handle h, const float angs[3], const float to[3], float dist, float dolly, float fov,
@@ -295,6 +302,12 @@ int gvdb_batch_gather_density(gvdb_batch b, int count, const gvdb_handle* hs, in
     VOLS(b, count, hs);
     if (!sc_len || !radius) return GVDB_ERR_INVALID;
     return b->b.gather_density(vs_.data(), count, subcell_size, sc_len, radius, chan_density, accumulate != 0, failed);
+}
+int gvdb_batch_gather_level_set(gvdb_batch b, int count, const gvdb_handle* hs, int subcell_size, const int* sc_len,
+                                const float* radius, int chan_levelset, int accumulate, int* failed) {
+    VOLS(b, count, hs);
+    if (!sc_len || !radius) return GVDB_ERR_INVALID;
+    return b->b.gather_level_set(vs_.data(), count, subcell_size, sc_len, radius, chan_levelset, accumulate != 0, failed);
 }
 int gvdb_batch_update_apron(gvdb_batch b, int count, const gvdb_handle* hs, int chan, float boundval, int* failed) {
     VOLS(b, count, hs);

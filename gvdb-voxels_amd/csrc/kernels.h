@@ -185,6 +185,7 @@ struct GatherArgs {
     int atlas_type;
     int accumulate;
     int nbricks;          // linked bricks (one workgroup each)
+    int levelset;         // 1: GatherLevelSet, min(3, min (|p - v| - r)) over every entry
 };
 hipError_t launch_gather_multi(const GatherArgs* a, int G, Arena* ar, hipStream_t s);
 
@@ -254,6 +255,7 @@ struct BucketGatherArgs {
     int atlas_type;
     int accumulate;
     void* faces;          // optional (scatter kernel): the six 8x8 boundary faces of each brick, channel type
+    int levelset;         // 1: GatherLevelSet (nearest-entry kernel)
 };
 hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* ar, hipStream_t s);
 // Face buffer layout: brick b's faces at b * FACE_VALS; face f = x0, x7, y0,

@@ -555,23 +555,25 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         v->last_insert_pull_ = pl.pull;
         // the pull path leaves the subcell tables to be built on demand
         // (materialize_entries); the gather reads the buckets directly
-        v->entries_stale_ = pl.pull && pl.total > 0;
+        v->entries_stale_ = pl.pull;
         v->bkt_scs_ = scs;
         v->bkt_R_ = pl.R;
         v->bkt_radius_ = radius[g];
         v->bkt_gen_ = pl.pull ? v->topo_gen_ : ~0u;
-        if (pl.total == 0 || pl.pull) continue;
-        XCHK(v, v->sc_pnt_.ensure(size_t(pl.total) * 12));
+        if (pl.pull) continue;
+        // an empty list still gets its subcell corners: GatherLevelSet writes
+        // every voxel of the linked bricks
         XCHK(v, v->sc_pos_.ensure(size_t(pl.numSC) * 12));
         XCHK(v, v->sc_nid_.ensure(size_t(pl.numSC) * 4));
-        {
+        if (pl.total > 0) {
+            XCHK(v, v->sc_pnt_.ensure(size_t(pl.total) * 12));
             XCHK(v, launch_fill_u32(v->sc_cnt_.as<uint32_t>(), 0u, pl.numSC, ex.s));
             XCHK(v, launch_insert_subcell(pl.t, pl.P, scs, pl.spb, trans3 + 3 * g, radius[g], v->sc_map_.as<uint32_t>(),
                                           v->sc_cnt_.as<uint32_t>(), v->sc_off_.as<uint32_t>(), v->sc_pnt_.as<float>(),
                                           ex.s));
-            // unordered entries: the gather considers all of them
-            XCHK(v, hipMemcpyAsync(v->sc_use_.p, v->sc_cnt_.p, size_t(pl.numSC) * 4, hipMemcpyDeviceToDevice, ex.s));
         }
+        // unordered entries: the gather considers all of them
+        XCHK(v, hipMemcpyAsync(v->sc_use_.p, v->sc_cnt_.p, size_t(pl.numSC) * 4, hipMemcpyDeviceToDevice, ex.s));
         sp.push_back(SubPosArgs{pl.t, scs, pl.spb, v->sc_map_.as<uint32_t>(), v->sc_pos_.as<int>(), v->sc_nid_.as<int>(),
                                 int(pl.numMap)});
     }
@@ -625,20 +627,23 @@ int Volume::multi_materialize(Volume* const* vs, int G, const Exec& ex, int* fai
 
 // --------------------------------------------------------- GatherDensity ----
 int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, const float* radius, int chanD, bool accum,
-                         const Exec& ex, int* failed) {
-    // GatherDensity (gvdb_volume_gvdb.cpp:5318-5344)
+                         bool levelset, const Exec& ex, int* failed) {
+    // GatherDensity (gvdb_volume_gvdb.cpp:5318-5344) or GatherLevelSet (:5348-5371,
+    // which does not return early on an empty entry list)
     std::vector<GatherArgs> ga;
     std::vector<BucketGatherArgs> bg;
     std::vector<Volume*> need;
+    std::vector<int> need_g, ents;   // grid index of each volume to materialize / to gather from entries
+    const char* what = levelset ? "GatherLevelSet" : "GatherDensity";
     for (int g = 0; g < G; g++) {
         Volume* v = vs[g];
-        if (len[g] == 0 || v->pool_[0][0].usedNum == 0) continue;
+        if ((len[g] == 0 && !levelset) || v->pool_[0][0].usedNum == 0) continue;
         if (chanD < 0 || chanD >= 8 || !v->chan_[chanD].exists)
-            FAILG(g, v->fail(GVDB_ERR_INVALID, "GatherDensity: no such channel"));
+            FAILG(g, v->fail(GVDB_ERR_INVALID, std::string(what) + ": no such channel"));
         if (len[g] != v->sc_len_ || scs <= 0 || v->res_[0] % scs)
-            FAILG(g, v->fail(GVDB_ERR_STATE, "GatherDensity: call InsertPointsSubcell first"));
+            FAILG(g, v->fail(GVDB_ERR_STATE, std::string(what) + ": call InsertPointsSubcell first"));
         if (v->atlas_gen_ != v->topo_gen_ || v->sc_gen_ != v->topo_gen_)
-            FAILG(g, v->fail(GVDB_ERR_STATE, "GatherDensity: topology changed since UpdateAtlas/InsertPointsSubcell"));
+            FAILG(g, v->fail(GVDB_ERR_STATE, std::string(what) + ": topology changed since UpdateAtlas/InsertPointsSubcell"));
         int rc = v->commit_pools();
         if (rc) FAILG(g, rc);
         const bool buckets = v->bkt_gen_ == v->topo_gen_ && scs == v->bkt_scs_ && radius[g] == v->bkt_radius_ &&
@@ -647,7 +652,7 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
         if (buckets) {
             // the scatter kernel (no accumulate) also leaves the bricks' faces for the apron
             void* faces = nullptr;
-            if (!accum) {
+            if (!accum && !levelset) {
                 const size_t need = size_t(v->pool_[0][0].lastEle) * FACE_VALS * v->chan_[chanD].elem();
                 if (v->faces_.ensure(need) == hipSuccess) {
                     faces = v->faces_.p;
@@ -659,20 +664,37 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
             bg.push_back(BucketGatherArgs{v->make_tree(), int(v->pool_[0][0].lastEle), v->bkt_R_, radius[g],
                                           v->nbr_.as<int>(), v->bkt_cnt_.as<uint32_t>(), v->bkt_off_.as<uint32_t>(),
                                           v->sorted_.as<float4>(), v->chan_[chanD].data.p, v->chan_[chanD].type,
-                                          accum ? 1 : 0, faces});
+                                          accum ? 1 : 0, faces, levelset ? 1 : 0});
             continue;
         }
-        if (v->entries_stale_) need.push_back(v);
-        ga.push_back(GatherArgs{v->make_tree(), scs, radius[g], v->sc_nid_.as<int>(), v->sc_cnt_.as<uint32_t>(),
-                                v->sc_use_.as<uint32_t>(), v->sc_off_.as<uint32_t>(), v->sc_pos_.as<int>(),
-                                v->sc_pnt_.as<float>(), v->chan_[chanD].data.p, v->chan_[chanD].type, accum ? 1 : 0,
-                                v->sc_bricks_});
+        if (v->entries_stale_) {
+            need.push_back(v);
+            need_g.push_back(g);
+        }
+        ents.push_back(g);
     }
-    if (ga.empty() && bg.empty()) return GVDB_OK;
+    if (ents.empty() && bg.empty()) return GVDB_OK;
     Volume* v0 = vs[0];
     if (!need.empty()) {
-        int rc = multi_materialize(need.data(), int(need.size()), ex, failed);
-        if (rc) return rc;
+        // allocates the entry tables, so the entry pointers are taken only after it
+        int fi = -1;
+        int rc = multi_materialize(need.data(), int(need.size()), ex, &fi);
+        if (rc) {
+            if (failed && fi >= 0) *failed = need_g[fi];
+            return rc;
+        }
+    }
+    for (int g : ents) {
+        Volume* v = vs[g];
+        if (v->sc_len_ > 0 && !v->sc_pnt_.p) FAILG(g, v->fail(GVDB_ERR_STATE, std::string(what) + ": no subcell entries"));
+        // a bucketed insert places first the entries that can raise a voxel
+        // at its own radius; a gather at a larger radius reads them all
+        const bool all = v->last_insert_pull_ && !(radius[g] <= v->bkt_radius_);
+        ga.push_back(GatherArgs{v->make_tree(), scs, radius[g], v->sc_nid_.as<int>(), v->sc_cnt_.as<uint32_t>(),
+                                all ? v->sc_cnt_.as<uint32_t>() : v->sc_use_.as<uint32_t>(), v->sc_off_.as<uint32_t>(),
+                                v->sc_pos_.as<int>(),
+                                v->sc_pnt_.as<float>(), v->chan_[chanD].data.p, v->chan_[chanD].type, accum ? 1 : 0,
+                                v->sc_bricks_, levelset ? 1 : 0});
     }
     if (G == 1) {
         v0->stage_begin(ST_GATHER);

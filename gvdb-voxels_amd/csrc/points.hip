@@ -273,7 +273,9 @@ __device__ inline void k_gather_body(const GatherArgs& a, uint32_t bid, uint32_t
         vy = sc_pos[3 * sc + 1] + (q / scs) % scs;
         vz = sc_pos[3 * sc + 2] + q / (scs * scs);
         beg = sc_off[sc];
-        cnt = accumulate ? sc_cnt[sc] : sc_use[sc];
+        // the level set and accumulation take every entry; the density only
+        // those that can raise a voxel (placed first)
+        cnt = (accumulate || a.levelset) ? sc_cnt[sc] : sc_use[sc];
     }
     const u64 sc0 = brick * spb;
     const uint32_t bbeg = sc_off[sc0];
@@ -303,11 +305,11 @@ __device__ inline void k_gather_body(const GatherArgs& a, uint32_t bid, uint32_t
     int ay = nd->mValue[1] + (vy - nd->mPos[1]);
     int az = nd->mValue[2] + (vz - nd->mPos[2]);
     u64 off = atlas_offset(t, ax, ay, az);
-    float val = accumulate ? load_atlas<AT>(atlas, off) : 0.0f;
+    float val = accumulate ? load_atlas<AT>(atlas, off) : (a.levelset ? 3.0f : 0.0f);
     if (arg != 0xFFFFFFFFu) {
         const float* p = sc_pnt + 3 * u64(arg);
         float f = falloff_accurate(p[0], p[1], p[2], vx, vy, vz, radius);
-        val = fmaxf(val, f);
+        val = a.levelset ? fminf(val, -f) : fmaxf(val, f);
     }
     store_atlas<AT>(atlas, off, val);
 }
@@ -399,9 +401,11 @@ hipError_t launch_gather_multi(const GatherArgs* a, int G, Arena* ar, hipStream_
         std::vector<GatherArgs> sel;
         std::vector<uint32_t> nb;
         for (int g = 0; g < G; g++) {
-            const bool r = !a[g].accumulate && a[g].t.res[0] == 8 && a[g].scs == 4;
+            const bool r = !a[g].accumulate && !a[g].levelset && a[g].t.res[0] == 8 && a[g].scs == 4;
             if (r != rows || (a[g].atlas_type == T_HALF) != half || a[g].nbricks <= 0) continue;
             if (a[g].t.res[0] * a[g].t.res[0] * a[g].t.res[0] > 512) return hipErrorInvalidValue;
+            if (!a[g].sc_nid || !a[g].sc_cnt || !a[g].sc_use || !a[g].sc_off || !a[g].sc_pos || !a[g].atlas)
+                return hipErrorInvalidValue;
             sel.push_back(a[g]);
             nb.push_back(uint32_t(a[g].nbricks));
         }
@@ -971,11 +975,12 @@ __device__ inline void k_bucket_gather_body(const BucketGatherArgs& a, uint32_t 
                     uint32_t am[3];
 #pragma unroll
                     for (int ax = 0; ax < 3; ax++) {
-                        // subcell i of this axis can be reached when some voxel of
-                        // [org + 4i, org + 4i + 3] lies within r (a superset test)
-                        const float lo = pp[ax] - r, hi = pp[ax] + r;
-                        am[ax] = (lo < float(org[ax] + 3) && hi > float(org[ax]) ? 1u : 0u) |
-                                 (lo < float(org[ax] + 7) && hi > float(org[ax] + 4) ? 2u : 0u);
+                        // the entry-list test of CountSubcell/InsertSubcell: subcell
+                        // origin s is listed when int(p - r)/4*4 <= s <= int(p + 1 + r)/4*4
+                        // (C truncations), p including trans as bucketed
+                        const int lo = child_cell(int(pp[ax] - r), 2) * 4, hi = child_cell(int(pp[ax] + 1 + r), 2) * 4;
+                        am[ax] = (lo <= org[ax] && org[ax] <= hi ? 1u : 0u) |
+                                 (lo <= org[ax] + 4 && org[ax] + 4 <= hi ? 2u : 0u);
                     }
 #pragma unroll
                     for (int k = 0; k < 8; k++)
@@ -1034,10 +1039,11 @@ __device__ inline void k_bucket_gather_body(const BucketGatherArgs& a, uint32_t 
                                   bn->mValue[2] + (vz - bn->mPos[2]));
 #pragma unroll
     for (int x = 0; x < 4; x++) {
-        float val = a.accumulate ? load_atlas<AT>(atlas, aoff + x) : 0.0f;
+        float val = a.accumulate ? load_atlas<AT>(atlas, aoff + x) : (a.levelset ? 3.0f : 0.0f);
         if (B[x].id != 0xFFFFFFFFu) {
             const float4 q = a.sorted[B[x].id];
-            val = fmaxf(val, falloff_accurate(q.x, q.y, q.z, sx + x, vy, vz, r));
+            const float f = falloff_accurate(q.x, q.y, q.z, sx + x, vy, vz, r);
+            val = a.levelset ? fminf(val, -f) : fmaxf(val, f);
         }
         store_atlas<AT>(atlas, aoff + x, val);
     }
@@ -1263,7 +1269,7 @@ hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* a
         std::vector<uint32_t> nb;
         for (int g = 0; g < G; g++) {
             if (a[g].t.res[0] != 8 || a[g].R < 1 || a[g].R > 2) return hipErrorInvalidValue;
-            const int k = a[g].accumulate ? 2 : (a[g].R == 1 ? 0 : 1);
+            const int k = (a[g].accumulate || a[g].levelset) ? 2 : (a[g].R == 1 ? 0 : 1);
             if ((a[g].atlas_type == T_HALF) != half || k != kind || a[g].nleaves <= 0) continue;
             sel.push_back(a[g]);
             nb.push_back(uint32_t(a[g].nleaves));

@@ -609,7 +609,14 @@ int Volume::insert_points_subcell(int scs, int n, float radius, const float tran
 int Volume::gather_density(int scs, int n, float radius, const float trans[3], int len, int chanD, int chanC, bool accum) {
     (void)n; (void)trans; (void)chanC;
     Volume* v = this;
-    return multi_gather(&v, 1, scs, &len, &radius, chanD, accum, self_exec(), nullptr);
+    return multi_gather(&v, 1, scs, &len, &radius, chanD, accum, false, self_exec(), nullptr);
+}
+
+// GatherLevelSet (gvdb_volume_gvdb.cpp:5348-5371)
+int Volume::gather_level_set(int scs, int n, float radius, const float trans[3], int len, int chanD, int chanC, bool accum) {
+    (void)n; (void)trans; (void)chanC;
+    Volume* v = this;
+    return multi_gather(&v, 1, scs, &len, &radius, chanD, accum, true, self_exec(), nullptr);
 }
 
 int Volume::update_apron(int c, float boundval) {

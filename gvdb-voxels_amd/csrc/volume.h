@@ -101,6 +101,7 @@ public:
     // points -> voxels
     int insert_points_subcell(int scs, int n, float radius, const float trans[3], int* len);
     int gather_density(int scs, int n, float radius, const float trans[3], int len, int chanD, int chanC, bool accum);
+    int gather_level_set(int scs, int n, float radius, const float trans[3], int len, int chanL, int chanC, bool accum);
     int update_apron(int chan, float boundval);
 
     // render
@@ -148,6 +149,7 @@ public:
     static int multi_insert(Volume* const* v, int G, int scs, const int* n, const float* radius, const float* trans3,
                             int* len, const Exec& ex, int* failed);
     static int multi_gather(Volume* const* v, int G, int scs, const int* len, const float* radius, int chanD, bool accum,
+                            bool levelset,
                             const Exec& ex, int* failed);
     static int multi_materialize(Volume* const* v, int G, const Exec& ex, int* failed);
     static int multi_apron(Volume* const* v, int G, int chan, float boundval, const Exec& ex, int* failed);

@@ -40,6 +40,7 @@ _SIGS = [
     ("gvdb_clear_channel", _i, [_vp, _i]),
     ("gvdb_insert_points_subcell", _i, [_vp, _i, _i, _f, _vp, ctypes.POINTER(_i)]),
     ("gvdb_gather_density", _i, [_vp, _i, _i, _f, _vp, _i, _i, _i, _i]), ("gvdb_update_apron", _i, [_vp, _i, _f]),
+    ("gvdb_gather_level_set", _i, [_vp, _i, _i, _f, _vp, _i, _i, _i, _i]),
     ("gvdb_set_camera_orbit", _i, [_vp, _vp, _vp, _f, _f, _f, _f, _f]), ("gvdb_set_light", _i, [_vp, _vp]),
     ("gvdb_set_background", _i, [_vp, _vp]), ("gvdb_set_extinct", _i, [_vp, _f, _f, _f]),
     ("gvdb_set_steps", _i, [_vp, _f, _f, _f]), ("gvdb_set_cutoff", _i, [_vp, _f, _f, _f]),
@@ -60,6 +61,7 @@ _SIGS = [
     ("gvdb_batch_update_atlas", _i, [_vp, _i, _vp, _vp]), ("gvdb_batch_clear_channel", _i, [_vp, _i, _vp, _i, _vp]),
     ("gvdb_batch_insert_points_subcell", _i, [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp]),
     ("gvdb_batch_gather_density", _i, [_vp, _i, _vp, _i, _vp, _vp, _i, _i, _vp]),
+    ("gvdb_batch_gather_level_set", _i, [_vp, _i, _vp, _i, _vp, _vp, _i, _i, _vp]),
     ("gvdb_batch_update_apron", _i, [_vp, _i, _vp, _i, _f, _vp]),
     ("gvdb_batch_render", _i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
     ("gvdb_batch_synchronize", _i, [_vp]), ("gvdb_batch_elapsed_ms", _i, [_vp, _vp]),
@@ -198,6 +200,10 @@ class VolumeGVDB:
     def GatherDensity(self, subcell_size, n, radius, trans, sc_len, chan_density=0, chan_clr=0, accumulate=False):
         return self._chk(load().gvdb_gather_density(self.h, subcell_size, int(n), float(radius), _p(_f3(trans)),
                                                     int(sc_len), chan_density, chan_clr, int(accumulate)))
+
+    def GatherLevelSet(self, subcell_size, n, radius, trans, sc_len, chan_levelset=0, chan_clr=0, accumulate=False):
+        return self._chk(load().gvdb_gather_level_set(self.h, subcell_size, int(n), float(radius), _p(_f3(trans)),
+                                                      int(sc_len), chan_levelset, chan_clr, int(accumulate)))
 
     def UpdateApron(self, chan=0, boundval=0.0):
         return self._chk(load().gvdb_update_apron(self.h, chan, float(boundval)))
@@ -398,6 +404,13 @@ class GridBatch:
         f = ctypes.c_int(-1)
         return self._chk(load().gvdb_batch_gather_density(self.h, G, self._hs(vols), subcell_size, _p(ln), _p(r),
                                                           chan_density, int(accumulate), ctypes.byref(f)), vols, f)
+
+    def GatherLevelSet(self, vols, subcell_size, sc_len, radius, chan_levelset=0, accumulate=False):
+        G = len(vols)
+        ln, r = self._arr(sc_len, G, np.int32), self._arr(radius, G, np.float32)
+        f = ctypes.c_int(-1)
+        return self._chk(load().gvdb_batch_gather_level_set(self.h, G, self._hs(vols), subcell_size, _p(ln), _p(r),
+                                                            chan_levelset, int(accumulate), ctypes.byref(f)), vols, f)
 
     def UpdateApron(self, vols, chan=0, boundval=0.0):
         f = ctypes.c_int(-1)

@@ -105,6 +105,11 @@ int gvdb_insert_points_subcell(gvdb_handle h, int subcell_size, int n, float rad
 /* GatherDensity(subcell_size, num_pnts, radius, trans, len, chanDensity, chanClr, bAccum): gvdb_volume_gvdb.cpp:5318-5344 */
 int gvdb_gather_density(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len,
                         int chan_density, int chan_clr, int accumulate);
+/* GatherLevelSet(subcell_size, num_pnts, radius, trans, len, chanLevelset, chanClr, bAccum): gvdb_volume_gvdb.cpp:5348-5371,
+ * kernel gvdbGatherLevelSet (kernels/cuda_gvdb_particles.cuh:736-774): min(3, min over the subcell's entries of
+ * |p - v| - r), from the atlas value when accumulating */
+int gvdb_gather_level_set(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len,
+                          int chan_levelset, int chan_clr, int accumulate);
 /* UpdateApron(chan, boundval): gvdb_volume_gvdb.cpp:4418-4457 */
 int gvdb_update_apron(gvdb_handle h, int chan, float boundval);
 
@@ -195,6 +200,9 @@ int gvdb_batch_insert_points_subcell(gvdb_batch b, int count, const gvdb_handle*
 /* GatherDensity (gvdb_volume_gvdb.cpp:5318-5344 per grid) */
 int gvdb_batch_gather_density(gvdb_batch b, int count, const gvdb_handle* hs, int subcell_size, const int* sc_len,
                               const float* radius, int chan_density, int accumulate, int* failed);
+/* GatherLevelSet (gvdb_volume_gvdb.cpp:5348-5371 per grid) */
+int gvdb_batch_gather_level_set(gvdb_batch b, int count, const gvdb_handle* hs, int subcell_size, const int* sc_len,
+                                const float* radius, int chan_levelset, int accumulate, int* failed);
 /* UpdateApron (gvdb_volume_gvdb.cpp:4418-4457 per grid) */
 int gvdb_batch_update_apron(gvdb_batch b, int count, const gvdb_handle* hs, int chan, float boundval, int* failed);
 /* Render (gvdb_volume_gvdb.cpp:4336-4381 per grid), each into its own buffer */

@@ -111,6 +111,12 @@ public:
         check(gvdb_gather_density(h_, subcell_size, num_pnts, radius, t, pSCPntsLength, chanDensity, chanClr,
                                   bAccum ? 1 : 0), "GatherDensity");
     }
+    void GatherLevelSet(int subcell_size, int num_pnts, float radius, Vector3DF trans, int& pSCPntsLength,
+                        int chanLevelset, int chanClr, bool bAccum = false) {
+        const float t[3] = {trans.x, trans.y, trans.z};
+        check(gvdb_gather_level_set(h_, subcell_size, num_pnts, radius, t, pSCPntsLength, chanLevelset, chanClr,
+                                    bAccum ? 1 : 0), "GatherLevelSet");
+    }
     void UpdateApron(unsigned char chan, float boundval = 0.0f) { check(gvdb_update_apron(h_, chan, boundval), "UpdateApron"); }
 
     /* Scene setters of Scene / Camera3D used by the point samples */
@@ -231,6 +237,13 @@ public:
         check(gvdb_batch_gather_density(b_, int(hs.size()), hs.data(), subcell_size, len.data(), radius.data(),
                                         chan_density, accumulate, &failed_),
               "GatherDensity");
+    }
+    void GatherLevelSet(const std::vector<VolumeGVDB*>& vols, int subcell_size, const std::vector<int>& len,
+                        const std::vector<float>& radius, int chan_levelset, bool accumulate = false) {
+        auto hs = handles(vols);
+        check(gvdb_batch_gather_level_set(b_, int(hs.size()), hs.data(), subcell_size, len.data(), radius.data(),
+                                          chan_levelset, accumulate, &failed_),
+              "GatherLevelSet");
     }
     void UpdateApron(const std::vector<VolumeGVDB*>& vols, int chan, float boundval) {
         auto hs = handles(vols);

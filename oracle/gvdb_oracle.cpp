@@ -764,22 +764,26 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
     int len = v.sc_off[numSC - 1] + v.sc_cnt[numSC - 1];
     *outLen = len;
     v.sc_len = len;
-    if (len == 0) return 0;
     // insert pass
-    std::vector<int> cnt2(numSC, 0);
     v.sc_pnt.assign(size_t(len) * 3, 0.f);
-    for (int i = 0; i < n; i++) {
-        const float* p = pts + 3 * i;
-        float w[3];
-        if (!binned(p, w)) continue;
-        forEachSubcell(w, [&](int idx) {
-            int slot = cnt2[idx]++;
-            size_t e = size_t(v.sc_off[idx]) + slot;
-            v.sc_pnt[3 * e] = w[0]; v.sc_pnt[3 * e + 1] = w[1]; v.sc_pnt[3 * e + 2] = w[2];
-        });
+    if (len > 0) {
+        std::vector<int> cnt2(numSC, 0);
+        for (int i = 0; i < n; i++) {
+            const float* p = pts + 3 * i;
+            float w[3];
+            if (!binned(p, w)) continue;
+            forEachSubcell(w, [&](int idx) {
+                int slot = cnt2[idx]++;
+                size_t e = size_t(v.sc_off[idx]) + slot;
+                v.sc_pnt[3 * e] = w[0]; v.sc_pnt[3 * e + 1] = w[1]; v.sc_pnt[3 * e + 2] = w[2];
+            });
+        }
+        v.sc_cnt = cnt2;
     }
-    v.sc_cnt = cnt2;
-    // CalcSubcellPos
+    // CalcSubcellPos.  The reference returns before it on an empty list
+    // (gvdb_volume_gvdb.cpp:5058), which leaves GatherLevelSet reading the
+    // corners of an earlier insert or unallocated tables; here an empty list
+    // still gets its corners, so the level set is 3 on every linked brick.
     v.sc_pos.assign(size_t(numSC) * 3, 0);
     v.sc_nid.assign(numSC, 0);
     for (int i = 0; i < numMap; i++) {
@@ -800,8 +804,12 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
 // src/gvdb_volume_gvdb.cpp:5318-5344; kernels/cuda_gvdb_particles.cuh:696-733
 // One subcell at a time, each voxel takes max(val, radius - |p - v|) over the
 // subcell's points, v being the integer voxel coordinate.
+// levelset = false: gvdbGatherDensity (kernels/cuda_gvdb_particles.cuh:696-733),
+//   val = max(val, r - c) from 0 (or the atlas value when accumulating);
+// levelset = true: gvdbGatherLevelSet (kernels/cuda_gvdb_particles.cuh:736-774),
+//   dist = min(dist, c - r) from 3 (or the atlas value when accumulating).
 template <typename T>
-static void gatherT(Vol& v, int scs, float radius, bool accum, std::vector<T>& atlas) {
+static void gatherT(Vol& v, int scs, float radius, bool accum, std::vector<T>& atlas, bool levelset = false) {
     int spb = (v.res[0] / scs) * (v.res[0] / scs) * (v.res[0] / scs);
     int numSC = spb * int(v.pool[0][0].usedNum);
     const T R = T(radius);
@@ -815,13 +823,13 @@ static void gatherT(Vol& v, int scs, float radius, bool accum, std::vector<T>& a
                     int vox[3] = {nd->mValue[0] + (w[0] - nd->mPos[0]), nd->mValue[1] + (w[1] - nd->mPos[1]),
                                   nd->mValue[2] + (w[2] - nd->mPos[2])};
                     size_t ai = aidx(v, vox[0], vox[1], vox[2]);
-                    T val = accum ? atlas[ai] : T(0);
+                    T val = accum ? atlas[ai] : (levelset ? T(3) : T(0));
                     size_t off = size_t(v.sc_off[s]);
                     for (int j = 0; j < v.sc_cnt[s]; j++) {
                         const float* p = &v.sc_pnt[3 * (off + j)];
                         T jx = T(p[0]) - T(w[0]), jy = T(p[1]) - T(w[1]), jz = T(p[2]) - T(w[2]);
                         T c = std::sqrt(jx * jx + jy * jy + jz * jz);
-                        val = std::max(val, R - c);
+                        val = levelset ? std::min(val, c - R) : std::max(val, R - c);
                     }
                     atlas[ai] = val;
                 }
@@ -1420,6 +1428,16 @@ int orc_gather_density(void* h, int scs, float radius, int precision, int accum)
     if (precision == 64) gatherT<double>(v, scs, radius, accum != 0, v.atlasD);
     else gatherT<float>(v, scs, radius, accum != 0, v.atlasF);
     v.stage_ms[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+// GatherLevelSet (src/gvdb_volume_gvdb.cpp:5348-5371): no early return on an
+// empty entry list, unlike GatherDensity.
+int orc_gather_level_set(void* h, int scs, float radius, int precision, int accum) {
+    Vol& v = *static_cast<Vol*>(h);
+    if (v.pool[0][0].usedNum == 0) return 0;
+    if (precision == 64) gatherT<double>(v, scs, radius, accum != 0, v.atlasD, true);
+    else gatherT<float>(v, scs, radius, accum != 0, v.atlasF, true);
     return 0;
 }
 

@@ -49,6 +49,7 @@ def lib():
             "orc_insert_points_subcell": (i, [vp, i, fp, i, f, fp, ip]), "orc_num_subcells": (i, [vp]),
             "orc_subcell_copy": (None, [vp, i, vp]),
             "orc_gather_density": (i, [vp, i, f, i, i]), "orc_update_apron": (i, [vp, f, i]),
+            "orc_gather_level_set": (i, [vp, i, f, i, i]),
             "orc_camera_scn": (None, [fp, i, i, fp]), "orc_scn_floats": (i, []),
             "orc_linear_tf": (None, [f, f, fp, fp, fp]),
             "orc_render": (i, [vp, fp, fp, i, i, fp, vp]), "orc_stage_ms": (ctypes.c_double, [vp, i]),
@@ -174,6 +175,10 @@ class Oracle:
 
     def gather_density(self, radius, scs=4, precision=32, accumulate=False):
         return lib().orc_gather_density(self.h, scs, float(radius), precision, int(accumulate))
+
+    def gather_level_set(self, radius, scs=4, precision=32, accumulate=False):
+        """GatherLevelSet: min(3, min over the subcell's entries of |p - v| - r)."""
+        return lib().orc_gather_level_set(self.h, scs, float(radius), precision, int(accumulate))
 
     def update_apron(self, boundval=0.0, precision=32):
         return lib().orc_update_apron(self.h, float(boundval), precision)

@@ -100,6 +100,17 @@ def used_brick_mask(atlas_shape, amap, cnt, br=10):
     return m
 
 
+def interior_mask(atlas_shape, amap, cnt, br=10, apron=1):
+    """Boolean mask of the interior (apron excluded) voxels of used slots."""
+    m = np.zeros(atlas_shape, bool)
+    used = np.nonzero(amap["mLeafNode"] >= 0)[0]
+    for s in used:
+        sx, sy, sz = s % cnt[0], (s // cnt[0]) % cnt[1], s // (cnt[0] * cnt[1])
+        m[sz * br + apron:(sz + 1) * br - apron, sy * br + apron:(sy + 1) * br - apron,
+          sx * br + apron:(sx + 1) * br - apron] = True
+    return m
+
+
 def density_errors(a, b):
     """max |a-b|, max relative error over |b| > 0, and mismatched exact zeros."""
     a = a.astype(np.float64)

@@ -223,6 +223,88 @@ def test_gather_fp32_close_to_fp64():
     assert np.array_equal(a == 0, b == 0)
 
 
+# ------------------------------------------------------------- level set
+def _levelset(pts, radius, precision=64, scs=4):
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, radius) == 0
+    o.finish_topology()
+    o.update_atlas()
+    o.clear_channel()
+    rc, n = o.insert_points_subcell(pts, radius, scs)
+    assert rc == 0
+    o.gather_level_set(radius, scs, precision)
+    return o
+
+
+@pytest.mark.parametrize("radius,scs", [(2.0, 4), (1.5, 2)])
+def test_level_set_equals_brute_force_over_binned_lists(radius, scs):
+    """min(3, min over the points binned into the voxel's subcell of |p-v| - r),
+    with the lists from the brute-force binning model."""
+    pts = _ball(600, 31)
+    o = _levelset(pts, radius, scs=scs)
+    leaves = o.pool(0, 0)
+    corners = [tuple(int(c) for c in p) for p in leaves["mPos"]]
+    sm = M.subcell_model(pts, radius, corners, scs=scs)
+    got = M.brick_interiors(o.atlas(64), leaves["mPos"], leaves["mValue"])
+    sd = 8 // scs
+    for s in range(len(sm["cnt"])):
+        b, loc = divmod(s, sd ** 3)
+        sx, sy, sz = loc % sd * scs, (loc // sd) % sd * scs, loc // (sd * sd) * scs
+        lst = sm["pnt"][sm["off"][s]:sm["off"][s] + sm["cnt"][s]].astype(np.float64)
+        for z in range(scs):
+            for y in range(scs):
+                for x in range(scs):
+                    v = np.array([corners[b][0] + sx + x, corners[b][1] + sy + y, corners[b][2] + sz + z], np.float64)
+                    want = 3.0
+                    if len(lst):
+                        want = min(3.0, float(np.min(np.sqrt(((lst - v) ** 2).sum(1)) - np.float64(np.float32(radius)))))
+                    assert abs(got[b, sx + x, sy + y, sz + z] - want) <= 1e-12, (s, x, y, z)
+
+
+def test_level_set_is_minus_density_inside_the_radius():
+    """Both gathers pick the nearest binned point: where the density r - c is
+    positive the level set c - r is exactly its negative; elsewhere the level
+    set is not negative."""
+    pts = _ball(1500, 32)
+    d = _gathered(pts, 2.0, precision=64)
+    ls = _levelset(pts, 2.0, precision=64)
+    leaves = d.pool(0, 0)
+    a = M.brick_interiors(d.atlas(64), leaves["mPos"], leaves["mValue"])
+    b = M.brick_interiors(ls.atlas(64), leaves["mPos"], leaves["mValue"])
+    inside = a > 0
+    assert inside.any() and np.array_equal(b[inside], -a[inside])
+    assert (b[~inside] >= 0).all() and (b <= 3.0).all()
+
+
+@pytest.mark.parametrize("scs", [4, 2])
+def test_level_set_of_an_empty_insert_is_three(scs):
+    """An insert that bins no point still has its subcell corners, and the
+    minimum over no entries is the starting value 3 on every brick voxel."""
+    pts = _ball(600, 34)
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, 2.0) == 0
+    o.finish_topology()
+    o.update_atlas()
+    o.clear_channel()
+    rc, n = o.insert_points_subcell(pts[:0], 2.0, scs)
+    assert rc == 0 and n == 0
+    sub = o.subcells(0)
+    leaves = o.pool(0, 0)
+    sd = 8 // scs
+    assert len(sub["pos"]) == len(leaves) * sd ** 3
+    assert np.array_equal(sub["pos"].reshape(-1, 3)[::sd ** 3], leaves["mPos"].astype(sub["pos"].dtype))
+    o.gather_level_set(2.0, scs, 64)
+    got = M.brick_interiors(o.atlas(64), leaves["mPos"], leaves["mValue"])
+    assert (got == 3.0).all()
+
+
+def test_level_set_fp32_close_to_fp64():
+    pts = _ball(1500, 33)
+    a = _levelset(pts, 2.0, precision=32)
+    b = _levelset(pts, 2.0, precision=64)
+    assert np.abs(a.atlas(32).astype(np.float64) - b.atlas(64)).max() < 4e-6
+
+
 @pytest.mark.parametrize("boundval", [0.0, -1.5])
 def test_apron_matches_neighbour_lookup(boundval):
     pts = _ball(1200, 18)
