@@ -103,6 +103,19 @@ int gvdb_gather_density(gvdb_handle h, int subcell_size, int n, float radius, co
     const float z[3] = {0, 0, 0};
     return h->v.gather_density(subcell_size, n, radius, trans ? trans : z, sc_len, chan_density, chan_clr, accumulate != 0);
 }
+int gvdb_insert_points_subcell_fp16(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3],
+                                    int* out_len) {
+    H_OR_FAIL(h);
+    if (!out_len) return GVDB_ERR_INVALID;
+    const float z[3] = {0, 0, 0};
+    return h->v.insert_points_subcell_fp16(subcell_size, n, radius, trans ? trans : z, out_len);
+}
+int gvdb_gather_level_set_fp16(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len,
+                               int chan_levelset, int chan_clr) {
+    H_OR_FAIL(h);
+    const float z[3] = {0, 0, 0};
+    return h->v.gather_level_set_fp16(subcell_size, n, radius, trans ? trans : z, sc_len, chan_levelset, chan_clr);
+}
 int gvdb_gather_level_set(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len,
                           int chan_levelset, int chan_clr, int accumulate) {
     H_OR_FAIL(h);

@@ -152,12 +152,26 @@ struct BoundsArgs {
 };
 hipError_t launch_leaf_bounds_multi(const BoundsArgs* a, int G, Arena* ar, hipStream_t s);
 
+// Which points a subcell insert bins (w = p + trans):
+//   BIN_FLOAT (InsertPointsSubcell): the reference's count pass skips
+//     p.z == NOHIT and w < 0, its insert pass p < 0; binned when both agree;
+//   BIN_FP16 (InsertPointsSubcell_FP16): the count pass as above, the insert
+//     pass skips w.z == NOHIT and w < 0; binned when both agree.
+enum { BIN_FLOAT = 0, BIN_FP16 = 1 };
+
 // push binning (the reference's two atomic passes), one grid per launch
 hipError_t launch_count_subcell(const Tree& t, PointSrc pts, int scs, int spb, const float trans[3], float radius,
-                                const uint32_t* mapping, uint32_t* sc_cnt, hipStream_t s);
+                                const uint32_t* mapping, uint32_t* sc_cnt, int rule, hipStream_t s);
 hipError_t launch_insert_subcell(const Tree& t, PointSrc pts, int scs, int spb, const float trans[3], float radius,
                                  const uint32_t* mapping, uint32_t* sc_cnt, const uint32_t* sc_off,
-                                 float* sc_pnt, hipStream_t s);
+                                 float* sc_pnt, int rule, hipStream_t s);
+
+// InsertPointsSubcell_FP16's entry format (gvdbInsertSubcell_fp16,
+// kernels/cuda_gvdb_particles.cuh:154-216): each coordinate as
+// u16((w - pos_min) / pos_range * 65535), here saturated to [0, 65535];
+// n3 = 3 x entries.
+hipError_t launch_quantize_entries(const float* src, uint16_t* dst, u64 n3, const float qmin[3], const float qrange[3],
+                                   hipStream_t s);
 
 struct SubPosArgs {
     Tree t;
@@ -186,6 +200,10 @@ struct GatherArgs {
     int accumulate;
     int nbricks;          // linked bricks (one workgroup each)
     int levelset;         // 1: GatherLevelSet, min(3, min (|p - v| - r)) over every entry
+    // GatherLevelSet_FP16: u16 entries, position = u / 65535 * qrange + qmin
+    // (sc_pnt unused)
+    const uint16_t* sc_pnt16;
+    float qmin[3], qrange[3];
 };
 hipError_t launch_gather_multi(const GatherArgs* a, int G, Arena* ar, hipStream_t s);
 
@@ -205,6 +223,7 @@ struct BucketArgs {
     const uint32_t* nmask;
     float radius;
     u64* len;             // 64 partial counters, summed by launch_sum_parts_multi
+    int rule;             // BIN_FLOAT or BIN_FP16
 };
 hipError_t launch_sum_parts_multi(const u64* const* parts, u64* const* out, int G, Arena* ar, hipStream_t s);
 hipError_t launch_bucket_points_multi(const BucketArgs* a, int G, Arena* ar, hipStream_t s);

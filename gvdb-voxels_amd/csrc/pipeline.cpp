@@ -383,8 +383,9 @@ int Volume::multi_clear_channel(Volume* const* vs, int G, int c, const Exec& ex,
 
 // --------------------------------------------------- InsertPointsSubcell ----
 int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const float* radius, const float* trans3,
-                         int* len, const Exec& ex, int* failed) {
-    // InsertPointsSubcell (gvdb_volume_gvdb.cpp:4971-5092)
+                         int* len, const Exec& ex, int* failed, int rule) {
+    // InsertPointsSubcell (gvdb_volume_gvdb.cpp:4971-5092); with rule BIN_FP16
+    // the binning of InsertPointsSubcell_FP16 (:5094-5189)
     struct Plan {
         int sd, spb, R;
         u64 numSC, numMap;
@@ -397,6 +398,7 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
     for (int g = 0; g < G; g++) {
         Volume* v = vs[g];
         Plan& pl = plan[g];
+        v->sc16_valid_ = false;
         u64 numBrick = v->pool_[0][0].usedNum;
         if (numBrick == 0) FAILG(g, v->fail(GVDB_ERR_NOBRICKS, "Warning: InsertPointsSubcell yet no bricks exist."));
         if (scs <= 0 || v->res_[0] % scs != 0)
@@ -487,7 +489,7 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
             if (!pl.pull) {
                 // push path: the reference's count pass with global atomics
                 XCHK(v, launch_count_subcell(pl.t, pl.P, scs, pl.spb, tr, radius[g], v->sc_map_.as<uint32_t>(),
-                                             v->sc_cnt_.as<uint32_t>(), ex.s));
+                                             v->sc_cnt_.as<uint32_t>(), rule, ex.s));
                 sc.push_back(ScanArgs{v->sc_cnt_.as<uint32_t>(), nullptr, 0, v->sc_off_.as<uint32_t>(), pl.numSC,
                                       tmp2(g), small(g, SM_TOTAL)});
                 continue;
@@ -495,7 +497,7 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
             u64 nb = pl.numMap * u64(pl.spb);
             ba.push_back(BucketArgs{pl.t, pl.P, scs, pl.spb, {tr[0], tr[1], tr[2]}, v->bkt_key_.as<uint32_t>(),
                                     v->bkt_slot_.as<uint32_t>(), v->bkt_cnt_.as<uint32_t>(), small(g, SM_ORPHANS),
-                                    v->nmask_.as<uint32_t>(), radius[g], v->lenparts_.as<u64>()});
+                                    v->nmask_.as<uint32_t>(), radius[g], v->lenparts_.as<u64>(), rule});
             lp.push_back(v->lenparts_.as<u64>());
             lo.push_back(reinterpret_cast<u64*>(small(g, SM_LEN)));
             sb.push_back(ScanArgs{v->bkt_cnt_.as<uint32_t>(), nullptr, 0, v->bkt_off_.as<uint32_t>(), nb, tmp2(g),
@@ -530,7 +532,7 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
             redo = true;
             XCHK(v, launch_fill_u32(v->sc_cnt_.as<uint32_t>(), 0u, pl.numSC, ex.s));
             XCHK(v, launch_count_subcell(pl.t, pl.P, scs, pl.spb, trans3 + 3 * g, radius[g], v->sc_map_.as<uint32_t>(),
-                                         v->sc_cnt_.as<uint32_t>(), ex.s));
+                                         v->sc_cnt_.as<uint32_t>(), rule, ex.s));
             XCHK(v, launch_scan_u32(v->sc_cnt_.as<uint32_t>(), nullptr, 0, v->sc_off_.as<uint32_t>(), pl.numSC,
                                     small(g, SM_TOTAL), tmp2(g), ex.s));
         }
@@ -552,6 +554,7 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         v->sc_bricks_ = int(pl.linked);
         v->sc_map_n_ = int(pl.numMap);
         v->sc_gen_ = v->topo_gen_;
+        v->sc16_valid_ = false;
         v->last_insert_pull_ = pl.pull;
         // the pull path leaves the subcell tables to be built on demand
         // (materialize_entries); the gather reads the buckets directly
@@ -570,7 +573,7 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
             XCHK(v, launch_fill_u32(v->sc_cnt_.as<uint32_t>(), 0u, pl.numSC, ex.s));
             XCHK(v, launch_insert_subcell(pl.t, pl.P, scs, pl.spb, trans3 + 3 * g, radius[g], v->sc_map_.as<uint32_t>(),
                                           v->sc_cnt_.as<uint32_t>(), v->sc_off_.as<uint32_t>(), v->sc_pnt_.as<float>(),
-                                          ex.s));
+                                          rule, ex.s));
         }
         // unordered entries: the gather considers all of them
         XCHK(v, hipMemcpyAsync(v->sc_use_.p, v->sc_cnt_.p, size_t(pl.numSC) * 4, hipMemcpyDeviceToDevice, ex.s));
@@ -627,14 +630,14 @@ int Volume::multi_materialize(Volume* const* vs, int G, const Exec& ex, int* fai
 
 // --------------------------------------------------------- GatherDensity ----
 int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, const float* radius, int chanD, bool accum,
-                         bool levelset, const Exec& ex, int* failed) {
+                         bool levelset, const Exec& ex, int* failed, bool fp16) {
     // GatherDensity (gvdb_volume_gvdb.cpp:5318-5344) or GatherLevelSet (:5348-5371,
     // which does not return early on an empty entry list)
     std::vector<GatherArgs> ga;
     std::vector<BucketGatherArgs> bg;
     std::vector<Volume*> need;
     std::vector<int> need_g, ents;   // grid index of each volume to materialize / to gather from entries
-    const char* what = levelset ? "GatherLevelSet" : "GatherDensity";
+    const char* what = fp16 ? "GatherLevelSet_FP16" : (levelset ? "GatherLevelSet" : "GatherDensity");
     for (int g = 0; g < G; g++) {
         Volume* v = vs[g];
         if ((len[g] == 0 && !levelset) || v->pool_[0][0].usedNum == 0) continue;
@@ -646,7 +649,9 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
             FAILG(g, v->fail(GVDB_ERR_STATE, std::string(what) + ": topology changed since UpdateAtlas/InsertPointsSubcell"));
         int rc = v->commit_pools();
         if (rc) FAILG(g, rc);
-        const bool buckets = v->bkt_gen_ == v->topo_gen_ && scs == v->bkt_scs_ && radius[g] == v->bkt_radius_ &&
+        if (fp16 && !v->sc16_valid_)
+            FAILG(g, v->fail(GVDB_ERR_STATE, std::string(what) + ": call InsertPointsSubcell_FP16 first"));
+        const bool buckets = !fp16 && v->bkt_gen_ == v->topo_gen_ && scs == v->bkt_scs_ && radius[g] == v->bkt_radius_ &&
                              v->res_[0] == 8 && scs == 4 && v->bkt_R_ >= 1 && v->bkt_R_ <= 2;
         v->atlas_epoch_++;
         if (buckets) {
@@ -686,7 +691,8 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
     }
     for (int g : ents) {
         Volume* v = vs[g];
-        if (v->sc_len_ > 0 && !v->sc_pnt_.p) FAILG(g, v->fail(GVDB_ERR_STATE, std::string(what) + ": no subcell entries"));
+        if (v->sc_len_ > 0 && !(fp16 ? v->sc_pnt16_.p : v->sc_pnt_.p))
+            FAILG(g, v->fail(GVDB_ERR_STATE, std::string(what) + ": no subcell entries"));
         // a bucketed insert places first the entries that can raise a voxel
         // at its own radius; a gather at a larger radius reads them all
         const bool all = v->last_insert_pull_ && !(radius[g] <= v->bkt_radius_);
@@ -695,6 +701,13 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
                                 v->sc_pos_.as<int>(),
                                 v->sc_pnt_.as<float>(), v->chan_[chanD].data.p, v->chan_[chanD].type, accum ? 1 : 0,
                                 v->sc_bricks_, levelset ? 1 : 0});
+        if (fp16) {
+            ga.back().sc_pnt16 = v->sc_pnt16_.as<uint16_t>();
+            for (int a = 0; a < 3; a++) {
+                ga.back().qmin[a] = v->q_min_[a];
+                ga.back().qrange[a] = v->q_range_[a];
+            }
+        }
     }
     if (G == 1) {
         v0->stage_begin(ST_GATHER);

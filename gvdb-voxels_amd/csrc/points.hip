@@ -75,26 +75,33 @@ hipError_t launch_leaf_flags_multi(const FlagArgs* a, int G, Arena* ar, hipStrea
 }
 
 // -------------------------------------------------------- subcell binning
+// w = p + trans.  BIN_FLOAT: count pass p.z != NOHIT, w >= 0; insert pass
+// p >= 0 (gvdbInsertSubcell, kernels/cuda_gvdb_particles.cuh:218-273).
+// BIN_FP16: insert pass w.z != NOHIT, w >= 0 (gvdbInsertSubcell_fp16,
+// :154-216).  Non-finite positions are never binned.
+__device__ inline bool binned_point(float p0, float p1, float p2, const float w[3], int rule) {
+    if (p2 == NOHIT || !isfinite(w[0]) || !isfinite(w[1]) || !isfinite(w[2])) return false;
+    if (w[0] < 0 || w[1] < 0 || w[2] < 0) return false;
+    return rule == BIN_FP16 ? w[2] != NOHIT : (p0 >= 0 && p1 >= 0 && p2 >= 0);
+}
+
 // Each point is binned into every subcell overlapping
 // [int(p-r)/sc*sc, int(p+1+r)/sc*sc] per axis whose brick exists
 // (gvdbCountSubcell / gvdbInsertSubcell, kernels/cuda_gvdb_particles.cuh:218-315).
+// The reference's count pass and its insert pass test different things (see
+// BIN_FLOAT / BIN_FP16); where they disagree it writes past the counted
+// entries, so a point is binned here only when both tests pass.
 // The last leaf found is reused while the subcell corner stays inside it.
 template <bool INSERT>
 __global__ __launch_bounds__(256) void k_bin_subcell(Tree t, PointSrc pts, int scs, int spb, float tx, float ty, float tz,
                                                      float radius, const uint32_t* mapping, uint32_t* sc_cnt,
-                                                     const uint32_t* sc_off, float* sc_pnt) {
+                                                     const uint32_t* sc_off, float* sc_pnt, int rule) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= pts.n) return;
     const float* p = point_at(pts, i);
-    float p0 = p[0], p1 = p[1], p2 = p[2];
-    if (p2 == NOHIT) return;
+    const float p0 = p[0], p1 = p[1], p2 = p[2];
     float w[3] = {p0 + tx, p1 + ty, p2 + tz};
-    if (!isfinite(w[0]) || !isfinite(w[1]) || !isfinite(w[2])) return;
-    if (w[0] < 0 || w[1] < 0 || w[2] < 0) return;
-    // The reference's insert pass tests p < 0 before adding trans while its
-    // count pass tests p + trans < 0 (it writes past the counted entries when
-    // they disagree); both tests apply to both passes here.
-    if (p0 < 0 || p1 < 0 || p2 < 0) return;
+    if (!binned_point(p0, p1, p2, w, rule)) return;
     int scmin[3], scmax[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -130,19 +137,19 @@ __global__ __launch_bounds__(256) void k_bin_subcell(Tree t, PointSrc pts, int s
 }
 
 hipError_t launch_count_subcell(const Tree& t, PointSrc pts, int scs, int spb, const float trans[3], float radius,
-                                const uint32_t* mapping, uint32_t* sc_cnt, hipStream_t s) {
+                                const uint32_t* mapping, uint32_t* sc_cnt, int rule, hipStream_t s) {
     if (pts.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_bin_subcell<false>, dim3((pts.n + 255) / 256), dim3(256), 0, s, t, pts, scs, spb, trans[0],
-                       trans[1], trans[2], radius, mapping, sc_cnt, (const uint32_t*)nullptr, (float*)nullptr);
+                       trans[1], trans[2], radius, mapping, sc_cnt, (const uint32_t*)nullptr, (float*)nullptr, rule);
     return hipGetLastError();
 }
 
 hipError_t launch_insert_subcell(const Tree& t, PointSrc pts, int scs, int spb, const float trans[3], float radius,
                                  const uint32_t* mapping, uint32_t* sc_cnt, const uint32_t* sc_off, float* sc_pnt,
-                                 hipStream_t s) {
+                                 int rule, hipStream_t s) {
     if (pts.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_bin_subcell<true>, dim3((pts.n + 255) / 256), dim3(256), 0, s, t, pts, scs, spb, trans[0],
-                       trans[1], trans[2], radius, mapping, sc_cnt, sc_off, sc_pnt);
+                       trans[1], trans[2], radius, mapping, sc_cnt, sc_off, sc_pnt, rule);
     return hipGetLastError();
 }
 
@@ -244,6 +251,13 @@ __device__ inline float load_atlas<float>(const float* a, u64 off) { return a[of
 template <>
 __device__ inline float load_atlas<__half>(const __half* a, u64 off) { return __half2float(a[off]); }
 
+// gvdbGatherLevelSet_fp16 (kernels/cuda_gvdb_particles.cuh:776-811) reads a
+// coordinate as u / 65535.0f * range + min; the product and the sum are
+// fused, as a contracting compiler emits them.
+__device__ inline float dequant16(uint16_t u, float mn, float rg) {
+    return __fmaf_rn(__fdiv_rn(float(u), 65535.0f), rg, mn);
+}
+
 template <typename AT>
 __device__ inline void k_gather_body(const GatherArgs& a, uint32_t bid, uint32_t) {
     const Tree& t = a.t;
@@ -253,6 +267,7 @@ __device__ inline void k_gather_body(const GatherArgs& a, uint32_t bid, uint32_t
     const uint32_t *sc_cnt = a.sc_cnt, *sc_use = a.sc_use, *sc_off = a.sc_off;
     const int* sc_pos = a.sc_pos;
     const float* sc_pnt = a.sc_pnt;
+    const uint16_t* sc_pnt16 = a.sc_pnt16;
     AT* atlas = reinterpret_cast<AT*>(a.atlas);
     const int accumulate = a.accumulate;
     __shared__ float lx[GATHER_CHUNK], ly[GATHER_CHUNK], lz[GATHER_CHUNK];
@@ -286,10 +301,18 @@ __device__ inline void k_gather_body(const GatherArgs& a, uint32_t bid, uint32_t
     for (uint32_t c0 = bbeg; c0 < bend; c0 += GATHER_CHUNK) {
         uint32_t c1 = min(bend, c0 + GATHER_CHUNK);
         __syncthreads();
-        for (uint32_t e = c0 + tid; e < c1; e += blockDim.x) {
-            lx[e - c0] = sc_pnt[3 * u64(e)];
-            ly[e - c0] = sc_pnt[3 * u64(e) + 1];
-            lz[e - c0] = sc_pnt[3 * u64(e) + 2];
+        if (sc_pnt16) {
+            for (uint32_t e = c0 + tid; e < c1; e += blockDim.x) {
+                lx[e - c0] = dequant16(sc_pnt16[3 * u64(e)], a.qmin[0], a.qrange[0]);
+                ly[e - c0] = dequant16(sc_pnt16[3 * u64(e) + 1], a.qmin[1], a.qrange[1]);
+                lz[e - c0] = dequant16(sc_pnt16[3 * u64(e) + 2], a.qmin[2], a.qrange[2]);
+            }
+        } else {
+            for (uint32_t e = c0 + tid; e < c1; e += blockDim.x) {
+                lx[e - c0] = sc_pnt[3 * u64(e)];
+                ly[e - c0] = sc_pnt[3 * u64(e) + 1];
+                lz[e - c0] = sc_pnt[3 * u64(e) + 2];
+            }
         }
         __syncthreads();
         uint32_t j0 = max(beg, c0), j1 = min(beg + cnt, c1);
@@ -307,7 +330,14 @@ __device__ inline void k_gather_body(const GatherArgs& a, uint32_t bid, uint32_t
     u64 off = atlas_offset(t, ax, ay, az);
     float val = accumulate ? load_atlas<AT>(atlas, off) : (a.levelset ? 3.0f : 0.0f);
     if (arg != 0xFFFFFFFFu) {
-        const float* p = sc_pnt + 3 * u64(arg);
+        float p[3];
+        if (sc_pnt16) {
+#pragma unroll
+            for (int ax3 = 0; ax3 < 3; ax3++) p[ax3] = dequant16(sc_pnt16[3 * u64(arg) + ax3], a.qmin[ax3], a.qrange[ax3]);
+        } else {
+#pragma unroll
+            for (int ax3 = 0; ax3 < 3; ax3++) p[ax3] = sc_pnt[3 * u64(arg) + ax3];
+        }
         float f = falloff_accurate(p[0], p[1], p[2], vx, vy, vz, radius);
         val = a.levelset ? fminf(val, -f) : fmaxf(val, f);
     }
@@ -391,6 +421,29 @@ __device__ inline void k_gather_rows_body(const GatherArgs& a, uint32_t bid, uin
 }
 
 GVDB_MULTI_T(k_gather, typename AT, AT, GatherArgs, 512)
+
+// gvdbInsertSubcell_fp16 (kernels/cuda_gvdb_particles.cuh:204-207):
+// u16((w - pos_min) / pos_range * 65535), truncated; values outside
+// [0, 65535] (points outside the bounds RebuildTopology measured) saturate.
+__global__ __launch_bounds__(256) void k_quantize(const float* src, uint16_t* dst, u64 n3, float m0, float m1, float m2,
+                                                  float r0, float r1, float r2) {
+    const u64 i = u64(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n3) return;
+    const int ax = int(i % 3);
+    const float mn = ax == 0 ? m0 : (ax == 1 ? m1 : m2);
+    const float rg = ax == 0 ? r0 : (ax == 1 ? r1 : r2);
+    const float q = __fmul_rn(__fdiv_rn(__fsub_rn(src[i], mn), rg), 65535.0f);
+    dst[i] = q >= 65535.0f ? uint16_t(65535) : (q > 0.0f ? uint16_t(int(q)) : uint16_t(0));
+}
+
+hipError_t launch_quantize_entries(const float* src, uint16_t* dst, u64 n3, const float qmin[3], const float qrange[3],
+                                   hipStream_t s) {
+    if (n3 == 0) return hipSuccess;
+    if (!src || !dst) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_quantize, dim3(unsigned((n3 + 255) / 256)), dim3(256), 0, s, src, dst, n3, qmin[0], qmin[1],
+                       qmin[2], qrange[0], qrange[1], qrange[2]);
+    return hipGetLastError();
+}
 GVDB_MULTI_T(k_gather_rows, typename AT, AT, GatherArgs, 512)
 
 // One launch per (kernel, channel type) group: 8^3 bricks with 4^3 subcells
@@ -401,10 +454,11 @@ hipError_t launch_gather_multi(const GatherArgs* a, int G, Arena* ar, hipStream_
         std::vector<GatherArgs> sel;
         std::vector<uint32_t> nb;
         for (int g = 0; g < G; g++) {
-            const bool r = !a[g].accumulate && !a[g].levelset && a[g].t.res[0] == 8 && a[g].scs == 4;
+            const bool r = !a[g].accumulate && !a[g].levelset && !a[g].sc_pnt16 && a[g].t.res[0] == 8 && a[g].scs == 4;
             if (r != rows || (a[g].atlas_type == T_HALF) != half || a[g].nbricks <= 0) continue;
             if (a[g].t.res[0] * a[g].t.res[0] * a[g].t.res[0] > 512) return hipErrorInvalidValue;
-            if (!a[g].sc_nid || !a[g].sc_cnt || !a[g].sc_use || !a[g].sc_off || !a[g].sc_pos || !a[g].atlas)
+            if (!a[g].sc_nid || !a[g].sc_cnt || !a[g].sc_use || !a[g].sc_off || !a[g].sc_pos || !a[g].atlas ||
+                (a[g].sc_pnt16 && !a[g].levelset))
                 return hipErrorInvalidValue;
             sel.push_back(a[g]);
             nb.push_back(uint32_t(a[g].nbricks));
@@ -512,8 +566,7 @@ __device__ inline void k_bucket_points_body(const BucketArgs& a, uint32_t bid, u
         const float* p = point_at(pts, i);
         float p0 = p[0], p1 = p[1], p2 = p[2];
         float w[3] = {p0 + tx, p1 + ty, p2 + tz};
-        bool valid = p2 != NOHIT && isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && w[0] >= 0 && w[1] >= 0 &&
-                     w[2] >= 0 && p0 >= 0 && p1 >= 0 && p2 >= 0;
+        const bool valid = binned_point(p0, p1, p2, w, a.rule);
         if (valid) {
             // the subcell size divides the (power-of-two) brick width, so it is
             // a power of two: divisions are shifts (towards zero for negatives)

@@ -619,6 +619,41 @@ int Volume::gather_level_set(int scs, int n, float radius, const float trans[3],
     return multi_gather(&v, 1, scs, &len, &radius, chanD, accum, true, self_exec(), nullptr);
 }
 
+// InsertPointsSubcell_FP16 (gvdb_volume_gvdb.cpp:5094-5189): the subcell
+// tables of InsertPointsSubcell with the FP16 binning rule, the entries
+// quantised to u16 against the point bounds of the last RebuildTopology
+// (mPosMin, mPosRange: :5452-5462).
+int Volume::insert_points_subcell_fp16(int scs, int n, float radius, const float trans[3], int* len) {
+    float mn[3], rg[3];
+    for (int a = 0; a < 3; a++) {
+        mn[a] = posMin_[a];
+        rg[a] = posMax_[a] - posMin_[a];
+        if (!(rg[a] > 0.0f))
+            return fail(GVDB_ERR_STATE, "InsertPointsSubcell_FP16: no point bounds (call RebuildTopology first)");
+    }
+    Volume* v = this;
+    int rc = multi_insert(&v, 1, scs, &n, &radius, trans, len, self_exec(), nullptr, BIN_FP16);
+    if (rc) return rc;
+    if (entries_stale_) {
+        rc = multi_materialize(&v, 1, self_exec(), nullptr);
+        if (rc) return rc;
+    }
+    const u64 n3 = u64(sc_len_) * 3;
+    HIPCHK(sc_pnt16_.ensure(size_t(n3) * 2));
+    HIPCHK(launch_quantize_entries(sc_pnt_.as<float>(), sc_pnt16_.as<uint16_t>(), n3, mn, rg, stream));
+    for (int a = 0; a < 3; a++) { q_min_[a] = mn[a]; q_range_[a] = rg[a]; }
+    sc16_valid_ = true;
+    return GVDB_OK;
+}
+
+// GatherLevelSet_FP16 (gvdb_volume_gvdb.cpp:5374-5401): no accumulation.
+int Volume::gather_level_set_fp16(int scs, int n, float radius, const float trans[3], int len, int chanD, int chanC) {
+    (void)n; (void)trans; (void)chanC;
+    if (!sc16_valid_) return fail(GVDB_ERR_STATE, "GatherLevelSet_FP16: call InsertPointsSubcell_FP16 first");
+    Volume* v = this;
+    return multi_gather(&v, 1, scs, &len, &radius, chanD, false, true, self_exec(), nullptr, true);
+}
+
 int Volume::update_apron(int c, float boundval) {
     Volume* v = this;
     return multi_apron(&v, 1, c, boundval, self_exec(), nullptr);
@@ -759,7 +794,10 @@ int Volume::read_subcells(int which, void* dst, u64* count) {
         case 3: b = &sc_nid_; n = num_sc_; break;
         case 4: b = &sc_pnt_; n = sc_len_; esz = 12; break;
         case 5: b = &sc_map_; n = sc_map_n_; break;
-        default: return fail(GVDB_ERR_INVALID, "read_subcells: which must be 0..5");
+        case 6:
+            if (!sc16_valid_) return fail(GVDB_ERR_STATE, "read_subcells: no InsertPointsSubcell_FP16 entries");
+            b = &sc_pnt16_; n = sc_len_; esz = 6; break;
+        default: return fail(GVDB_ERR_INVALID, "read_subcells: which must be 0..6");
     }
     if (count) *count = n;
     if (!dst || !n) return GVDB_OK;

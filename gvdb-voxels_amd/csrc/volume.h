@@ -102,6 +102,8 @@ public:
     int insert_points_subcell(int scs, int n, float radius, const float trans[3], int* len);
     int gather_density(int scs, int n, float radius, const float trans[3], int len, int chanD, int chanC, bool accum);
     int gather_level_set(int scs, int n, float radius, const float trans[3], int len, int chanL, int chanC, bool accum);
+    int insert_points_subcell_fp16(int scs, int n, float radius, const float trans[3], int* len);
+    int gather_level_set_fp16(int scs, int n, float radius, const float trans[3], int len, int chanL, int chanC);
     int update_apron(int chan, float boundval);
 
     // render
@@ -147,10 +149,11 @@ public:
     static int multi_update_atlas(Volume* const* v, int G, const Exec& ex, int* failed);
     static int multi_clear_channel(Volume* const* v, int G, int chan, const Exec& ex, int* failed);
     static int multi_insert(Volume* const* v, int G, int scs, const int* n, const float* radius, const float* trans3,
-                            int* len, const Exec& ex, int* failed);
+                            int* len, const Exec& ex, int* failed, int rule = 0);
+    // levelset: GatherLevelSet; fp16 (with levelset): GatherLevelSet_FP16 over
+    // the u16 entries of InsertPointsSubcell_FP16
     static int multi_gather(Volume* const* v, int G, int scs, const int* len, const float* radius, int chanD, bool accum,
-                            bool levelset,
-                            const Exec& ex, int* failed);
+                            bool levelset, const Exec& ex, int* failed, bool fp16 = false);
     static int multi_materialize(Volume* const* v, int G, const Exec& ex, int* failed);
     static int multi_apron(Volume* const* v, int G, int chan, float boundval, const Exec& ex, int* failed);
     static int multi_render(Volume* const* v, int G, const int* shading, const int* chan, const int* buf, const Exec& ex,
@@ -204,6 +207,11 @@ private:
     int sc_len_ = 0, num_sc_ = 0, sc_bricks_ = 0, sc_map_n_ = 0;
     bool last_insert_pull_ = false;
     bool entries_stale_ = false;   // pull insert done, subcell tables not built yet
+    // InsertPointsSubcell_FP16: u16 entries (3 per entry) and the bounds they
+    // are quantised against; valid until the next insert
+    DevBuf sc_pnt16_;
+    bool sc16_valid_ = false;
+    float q_min_[3] = {0, 0, 0}, q_range_[3] = {0, 0, 0};
     int bkt_scs_ = 0, bkt_R_ = 0;
     float bkt_radius_ = 0.f;
     unsigned bkt_gen_ = ~0u;       // topology generation of the current buckets

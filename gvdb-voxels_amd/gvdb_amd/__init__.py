@@ -41,6 +41,8 @@ _SIGS = [
     ("gvdb_insert_points_subcell", _i, [_vp, _i, _i, _f, _vp, ctypes.POINTER(_i)]),
     ("gvdb_gather_density", _i, [_vp, _i, _i, _f, _vp, _i, _i, _i, _i]), ("gvdb_update_apron", _i, [_vp, _i, _f]),
     ("gvdb_gather_level_set", _i, [_vp, _i, _i, _f, _vp, _i, _i, _i, _i]),
+    ("gvdb_insert_points_subcell_fp16", _i, [_vp, _i, _i, _f, _vp, _vp]),
+    ("gvdb_gather_level_set_fp16", _i, [_vp, _i, _i, _f, _vp, _i, _i, _i]),
     ("gvdb_set_camera_orbit", _i, [_vp, _vp, _vp, _f, _f, _f, _f, _f]), ("gvdb_set_light", _i, [_vp, _vp]),
     ("gvdb_set_background", _i, [_vp, _vp]), ("gvdb_set_extinct", _i, [_vp, _f, _f, _f]),
     ("gvdb_set_steps", _i, [_vp, _f, _f, _f]), ("gvdb_set_cutoff", _i, [_vp, _f, _f, _f]),
@@ -205,6 +207,16 @@ class VolumeGVDB:
         return self._chk(load().gvdb_gather_level_set(self.h, subcell_size, int(n), float(radius), _p(_f3(trans)),
                                                       int(sc_len), chan_levelset, chan_clr, int(accumulate)))
 
+    def InsertPointsSubcell_FP16(self, subcell_size, n, radius, trans=(0, 0, 0)):
+        ln = ctypes.c_int(0)
+        self._chk(load().gvdb_insert_points_subcell_fp16(self.h, subcell_size, int(n), float(radius), _p(_f3(trans)),
+                                                         ctypes.byref(ln)))
+        return ln.value
+
+    def GatherLevelSet_FP16(self, subcell_size, n, radius, trans, sc_len, chan_levelset=0, chan_clr=0):
+        return self._chk(load().gvdb_gather_level_set_fp16(self.h, subcell_size, int(n), float(radius), _p(_f3(trans)),
+                                                           int(sc_len), chan_levelset, chan_clr))
+
     def UpdateApron(self, chan=0, boundval=0.0):
         return self._chk(load().gvdb_update_apron(self.h, chan, float(boundval)))
 
@@ -305,6 +317,16 @@ class VolumeGVDB:
                 self._chk(L.gvdb_read_subcells(self.h, k, _p(a), ctypes.byref(c)))
             out[name] = a
         return out
+
+    def subcells_fp16(self):
+        """u16 entries of InsertPointsSubcell_FP16, one row per entry."""
+        L = load()
+        c = ctypes.c_uint64(0)
+        self._chk(L.gvdb_read_subcells(self.h, 6, None, ctypes.byref(c)))
+        a = np.zeros((int(c.value), 3), np.uint16)
+        if c.value:
+            self._chk(L.gvdb_read_subcells(self.h, 6, _p(a), ctypes.byref(c)))
+        return a
 
     def synchronize(self):
         return self._chk(load().gvdb_synchronize(self.h))

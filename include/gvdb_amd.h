@@ -110,6 +110,17 @@ int gvdb_gather_density(gvdb_handle h, int subcell_size, int n, float radius, co
  * |p - v| - r), from the atlas value when accumulating */
 int gvdb_gather_level_set(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len,
                           int chan_levelset, int chan_clr, int accumulate);
+/* InsertPointsSubcell_FP16(subcell_size, num_pnts, radius, trans, &len): gvdb_volume_gvdb.cpp:5094-5189,
+ * kernel gvdbInsertSubcell_fp16 (kernels/cuda_gvdb_particles.cuh:154-216): the subcell tables, each entry stored as
+ * u16((p + trans - pos_min) / pos_range * 65535) per axis against the bounds of the last RebuildTopology
+ * (saturated outside them).  Read the u16 entries with gvdb_read_subcells(h, 6, ...). */
+int gvdb_insert_points_subcell_fp16(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3],
+                                    int* out_len);
+/* GatherLevelSet_FP16(subcell_size, num_pnts, radius, trans, len, chanLevelset, chanClr): gvdb_volume_gvdb.cpp:5374-5401,
+ * kernel gvdbGatherLevelSet_fp16 (kernels/cuda_gvdb_particles.cuh:776-811): min(3, min (|q - v| - r)) over the
+ * subcell's entries q = u / 65535 * pos_range + pos_min; no accumulation */
+int gvdb_gather_level_set_fp16(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len,
+                               int chan_levelset, int chan_clr);
 /* UpdateApron(chan, boundval): gvdb_volume_gvdb.cpp:4418-4457 */
 int gvdb_update_apron(gvdb_handle h, int chan, float boundval);
 
@@ -157,7 +168,8 @@ int gvdb_get_atlas_info(gvdb_handle h, int chan, int* out7);
 int gvdb_read_atlas(gvdb_handle h, int chan, void* host_dst);
 /* Atlas map (AtlasNode {int3 pos; int leaf}) per slot */
 int gvdb_read_atlas_map(gvdb_handle h, void* host_dst);
-/* InsertPointsSubcell outputs: which = 0 cnt, 1 off, 2 pos (int3), 3 nid, 4 positions (float3), 5 mapping */
+/* InsertPointsSubcell outputs: which = 0 cnt, 1 off, 2 pos (int3), 3 nid, 4 positions (float3), 5 mapping,
+ * 6 InsertPointsSubcell_FP16 positions (ushort3) */
 int gvdb_read_subcells(gvdb_handle h, int which, void* host_dst, uint64_t* count);
 
 /* Device-side handles for callers that stay on the GPU */

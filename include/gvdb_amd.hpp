@@ -117,6 +117,17 @@ public:
         check(gvdb_gather_level_set(h_, subcell_size, num_pnts, radius, t, pSCPntsLength, chanLevelset, chanClr,
                                     bAccum ? 1 : 0), "GatherLevelSet");
     }
+    void InsertPointsSubcell_FP16(int subcell_size, int num_pnts, float pRadius, Vector3DF trans, int& pSCPntsLength) {
+        const float t[3] = {trans.x, trans.y, trans.z};
+        check(gvdb_insert_points_subcell_fp16(h_, subcell_size, num_pnts, pRadius, t, &pSCPntsLength),
+              "InsertPointsSubcell_FP16");
+    }
+    void GatherLevelSet_FP16(int subcell_size, int num_pnts, float radius, Vector3DF trans, int& pSCPntsLength,
+                             int chanLevelset, int chanClr) {
+        const float t[3] = {trans.x, trans.y, trans.z};
+        check(gvdb_gather_level_set_fp16(h_, subcell_size, num_pnts, radius, t, pSCPntsLength, chanLevelset, chanClr),
+              "GatherLevelSet_FP16");
+    }
     void UpdateApron(unsigned char chan, float boundval = 0.0f) { check(gvdb_update_apron(h_, chan, boundval), "UpdateApron"); }
 
     /* Scene setters of Scene / Camera3D used by the point samples */

@@ -99,6 +99,10 @@ struct Vol {
     std::vector<int> sc_cnt, sc_off, sc_nid, sc_mapping, sc_pos;
     std::vector<float> sc_pnt;
     int sc_len = 0;
+    // InsertPointsSubcell_FP16 entries and the bounds they are quantised against
+    std::vector<uint16_t> sc_pnt16;
+    bool sc16 = false;     // sc_pnt16 belongs to the current entry lists
+    float qmin[3] = {0, 0, 0}, qrange[3] = {0, 0, 0};
     int num_sc = 0;
     // timing of the last call of each stage (ms), for the CPU baseline
     double stage_ms[8] = {0};
@@ -700,9 +704,16 @@ static inline size_t aidx(const Vol& v, int x, int y, int z) {
 // (SetFlagSubcell), :275-315 (CountSubcell), :218-273 (InsertSubcell),
 // :677-694 (CalcSubcellPos).  Entries inside a subcell are kept in ascending
 // point order (the canonical order; the reference's atomics leave it unordered).
-static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float radius, const float trans[3], int* outLen) {
+// rule 0: InsertPointsSubcell's binning; rule 1: InsertPointsSubcell_FP16's
+// (gvdb_volume_gvdb.cpp:5094-5189; cuda_gvdb_particles.cuh:154-216), whose
+// insert pass tests p + trans for NOHIT and for < 0, where the float
+// variant's tests p < 0.  A point is binned when its count pass and its insert
+// pass both take it.
+static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float radius, const float trans[3], int* outLen,
+                               int rule = 0) {
     int numBrick = int(v.pool[0][0].usedNum);
     *outLen = 0;
+    v.sc16 = false;
     if (numBrick == 0) return -5;
     int res0 = v.res[0];
     int spb = (res0 / scs) * (res0 / scs) * (res0 / scs);
@@ -744,6 +755,7 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
         if (p[2] == NOHIT) return false;
         for (int a = 0; a < 3; a++) w[a] = p[a] + trans[a];
         bool wneg = w[0] < 0 || w[1] < 0 || w[2] < 0, pneg = p[0] < 0 || p[1] < 0 || p[2] < 0;
+        if (rule == 1) return !wneg && w[2] != NOHIT;
         return !wneg && !pneg;
     };
     for (int i = 0; i < n; i++) {
@@ -752,7 +764,8 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
         if (p[2] == NOHIT) continue;
         bool wneg = p[0] + trans[0] < 0 || p[1] + trans[1] < 0 || p[2] + trans[2] < 0;
         bool pneg = p[0] < 0 || p[1] < 0 || p[2] < 0;
-        if (wneg != pneg) v.undefined_events++;
+        if (rule == 0 && wneg != pneg) v.undefined_events++;
+        if (rule == 1 && !wneg && p[2] + trans[2] == NOHIT) v.undefined_events++;
         if (!binned(p, w)) continue;
         forEachSubcell(w, [&](int idx) { v.sc_cnt[idx]++; });
     }
@@ -808,8 +821,15 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
 //   val = max(val, r - c) from 0 (or the atlas value when accumulating);
 // levelset = true: gvdbGatherLevelSet (kernels/cuda_gvdb_particles.cuh:736-774),
 //   dist = min(dist, c - r) from 3 (or the atlas value when accumulating).
+// gvdbGatherLevelSet_fp16 (cuda_gvdb_particles.cuh:776-811) reads an entry
+// coordinate as u / 65535.0f * range + min; the product and the sum are
+// fused, as a contracting compiler emits them.
+static float dequant16(uint16_t u, float mn, float rg) { return std::fmaf(float(u) / 65535.0f, rg, mn); }
+
+// fp16: the entries are InsertPointsSubcell_FP16's u16 ones (level set only).
 template <typename T>
-static void gatherT(Vol& v, int scs, float radius, bool accum, std::vector<T>& atlas, bool levelset = false) {
+static void gatherT(Vol& v, int scs, float radius, bool accum, std::vector<T>& atlas, bool levelset = false,
+                    bool fp16 = false) {
     int spb = (v.res[0] / scs) * (v.res[0] / scs) * (v.res[0] / scs);
     int numSC = spb * int(v.pool[0][0].usedNum);
     const T R = T(radius);
@@ -826,7 +846,12 @@ static void gatherT(Vol& v, int scs, float radius, bool accum, std::vector<T>& a
                     T val = accum ? atlas[ai] : (levelset ? T(3) : T(0));
                     size_t off = size_t(v.sc_off[s]);
                     for (int j = 0; j < v.sc_cnt[s]; j++) {
+                        float pq[3];
                         const float* p = &v.sc_pnt[3 * (off + j)];
+                        if (fp16) {
+                            for (int a = 0; a < 3; a++) pq[a] = dequant16(v.sc_pnt16[3 * (off + j) + a], v.qmin[a], v.qrange[a]);
+                            p = pq;
+                        }
                         T jx = T(p[0]) - T(w[0]), jy = T(p[1]) - T(w[1]), jz = T(p[2]) - T(w[2]);
                         T c = std::sqrt(jx * jx + jy * jy + jz * jz);
                         val = levelset ? std::min(val, c - R) : std::max(val, R - c);
@@ -1408,6 +1433,35 @@ int orc_insert_points_subcell(void* h, int scs, const float* pts, int n, float r
     v.stage_ms[2] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
 }
+// InsertPointsSubcell_FP16 (gvdb_volume_gvdb.cpp:5094-5189): each entry
+// coordinate u16((w - mPosMin) / mPosRange * 65535) (cuda_gvdb_particles.cuh:
+// 204-207; float subtract, divide, multiply, truncation), against the bounds
+// GetBoundingBox set (:5452-5462).  Outside [0, 65535] the reference's
+// conversion is undefined; here it saturates.
+static uint16_t quantize16(float w, float mn, float rg) {
+    const float q = (w - mn) / rg * 65535.0f;
+    if (q >= 65535.0f) return 65535;
+    return q > 0.0f ? uint16_t(int(q)) : uint16_t(0);
+}
+int orc_insert_points_subcell_fp16(void* h, int scs, const float* pts, int n, float radius, const float* trans,
+                                   int* outLen) {
+    Vol& v = *static_cast<Vol*>(h);
+    for (int a = 0; a < 3; a++) {
+        v.qmin[a] = v.posMin[a];
+        v.qrange[a] = v.posMax[a] - v.posMin[a];
+        if (!(v.qrange[a] > 0.0f)) return -3;
+    }
+    int rc = insertPointsSubcell(v, scs, pts, n, radius, trans, outLen, 1);
+    if (rc) return rc;
+    v.sc_pnt16.resize(v.sc_pnt.size());
+    for (size_t i = 0; i < v.sc_pnt.size(); i++) v.sc_pnt16[i] = quantize16(v.sc_pnt[i], v.qmin[i % 3], v.qrange[i % 3]);
+    v.sc16 = true;
+    return 0;
+}
+void orc_subcell16_copy(void* h, void* dst) {
+    Vol& v = *static_cast<Vol*>(h);
+    memcpy(dst, v.sc_pnt16.data(), v.sc_pnt16.size() * 2);
+}
 int orc_num_subcells(void* h) { return static_cast<Vol*>(h)->num_sc; }
 void orc_subcell_copy(void* h, int which, void* dst) {
     Vol& v = *static_cast<Vol*>(h);
@@ -1438,6 +1492,16 @@ int orc_gather_level_set(void* h, int scs, float radius, int precision, int accu
     if (v.pool[0][0].usedNum == 0) return 0;
     if (precision == 64) gatherT<double>(v, scs, radius, accum != 0, v.atlasD, true);
     else gatherT<float>(v, scs, radius, accum != 0, v.atlasF, true);
+    return 0;
+}
+
+// GatherLevelSet_FP16 (src/gvdb_volume_gvdb.cpp:5374-5401): no accumulation.
+int orc_gather_level_set_fp16(void* h, int scs, float radius, int precision) {
+    Vol& v = *static_cast<Vol*>(h);
+    if (v.pool[0][0].usedNum == 0) return 0;
+    if (!v.sc16) return -3;
+    if (precision == 64) gatherT<double>(v, scs, radius, false, v.atlasD, true, true);
+    else gatherT<float>(v, scs, radius, false, v.atlasF, true, true);
     return 0;
 }
 

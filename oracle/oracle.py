@@ -50,6 +50,9 @@ def lib():
             "orc_subcell_copy": (None, [vp, i, vp]),
             "orc_gather_density": (i, [vp, i, f, i, i]), "orc_update_apron": (i, [vp, f, i]),
             "orc_gather_level_set": (i, [vp, i, f, i, i]),
+            "orc_insert_points_subcell_fp16": (i, [vp, i, fp, i, f, fp, ip]),
+            "orc_subcell16_copy": (None, [vp, vp]),
+            "orc_gather_level_set_fp16": (i, [vp, i, f, i]),
             "orc_camera_scn": (None, [fp, i, i, fp]), "orc_scn_floats": (i, []),
             "orc_linear_tf": (None, [f, f, fp, fp, fp]),
             "orc_render": (i, [vp, fp, fp, i, i, fp, vp]), "orc_stage_ms": (ctypes.c_double, [vp, i]),
@@ -179,6 +182,24 @@ class Oracle:
     def gather_level_set(self, radius, scs=4, precision=32, accumulate=False):
         """GatherLevelSet: min(3, min over the subcell's entries of |p - v| - r)."""
         return lib().orc_gather_level_set(self.h, scs, float(radius), precision, int(accumulate))
+
+    def insert_points_subcell_fp16(self, pts, radius, scs=4, trans=(0, 0, 0)):
+        """InsertPointsSubcell_FP16: the subcell lists under the FP16 binning
+        rule, entries quantised to u16 against the point bounds."""
+        pts = np.ascontiguousarray(pts, dtype=np.float32)
+        n = ctypes.c_int(0)
+        rc = lib().orc_insert_points_subcell_fp16(self.h, scs, _p(pts), len(pts), float(radius), _p(f3(trans)),
+                                                  ctypes.byref(n))
+        return rc, n.value
+
+    def subcells_fp16(self, length):
+        a = np.zeros((length, 3), np.uint16)
+        if a.size:
+            lib().orc_subcell16_copy(self.h, _p(a))
+        return a
+
+    def gather_level_set_fp16(self, radius, scs=4, precision=32):
+        return lib().orc_gather_level_set_fp16(self.h, scs, float(radius), precision)
 
     def update_apron(self, boundval=0.0, precision=32):
         return lib().orc_update_apron(self.h, float(boundval), precision)

@@ -198,3 +198,31 @@ def brick_interiors(atlas, leaves, slots, w=8):
         blk = atlas[s[2]:s[2] + w, s[1]:s[1] + w, s[0]:s[0] + w]
         out[i] = blk.transpose(2, 1, 0)
     return out
+
+
+def quantize16(w, mn, rg):
+    """u16((w - mn) / rg * 65535) with one float32 rounding per operation,
+    truncated; saturated to [0, 65535]."""
+    w = np.asarray(w, np.float32)
+    q = ((w - np.float32(mn)) / np.float32(rg)) * np.float32(65535)
+    q = np.where(q >= np.float32(65535), np.float32(65535), np.where(q > 0, q, np.float32(0)))
+    return np.trunc(q).astype(np.uint16)
+
+
+def fma32(a, b, c):
+    """Correctly rounded float32 a * b + c, from the exact rational value."""
+    from fractions import Fraction
+    exact = Fraction(float(np.float32(a))) * Fraction(float(np.float32(b))) + Fraction(float(np.float32(c)))
+    x = np.float32(float(exact))
+    best = None
+    for cand in (np.nextafter(x, np.float32(-np.inf)), x, np.nextafter(x, np.float32(np.inf))):
+        err = abs(Fraction(float(cand)) - exact)
+        if best is None or err < best[0] or (err == best[0] and int(cand.view(np.uint32)) % 2 == 0):
+            best = (err, cand)
+    return best[1]
+
+
+def dequantize16(u, mn, rg):
+    """u / 65535.0f * rg + mn, the quotient rounded to float32 and the
+    product and sum fused (one rounding)."""
+    return np.float32(fma32(np.float32(np.float32(u) / np.float32(65535)), rg, mn))

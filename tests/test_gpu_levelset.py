@@ -143,3 +143,97 @@ def test_level_set_empty_lists(gpu, scs, r):
     assert (b[mask] == 3.0).all()
     assert np.array_equal(a[mask], b[mask].astype(np.float32))
     v.close()
+
+
+# ------------------------------------------------ FP16 subcell variants
+def _translated():
+    """Points q with w = q + t; 50 have q.x < 0 <= w.x (binned by the FP16
+    rule only) and 10 have q.z = NOHIT; the topology is built from w."""
+    base = _pts("sphere")
+    t = np.array([5.0, 3.0, 0.0], np.float32)
+    q = (base - t).astype(np.float32)
+    q[:50, 0] = np.float32(-0.5)
+    q[50:60, 2] = np.float32(1.0e10)
+    w = (q + t).astype(np.float32)
+    topo = np.concatenate([base, w[:50]]).astype(np.float32)
+    return topo, q, t
+
+
+def _gpu16(gpu, topo, q, t, r, scs):
+    v, _ = H.gpu_volume(gpu, topo, r, scs=scs, stop_after="topology")
+    v.UpdateAtlas()
+    v.ClearChannel(0)
+    v.SetPoints(q)
+    n = v.InsertPointsSubcell_FP16(scs, len(q), r, t)
+    return v, n
+
+
+def _oracle16(topo, q, t, r, scs):
+    import oracle as O
+    o, _ = H.oracle_volume(O, topo, r, stop_after="topology")
+    assert o.update_atlas() == 0
+    o.clear_channel()
+    rc, m = o.insert_points_subcell_fp16(q, r, scs, t)
+    assert rc == 0
+    return o, m
+
+
+FP16_CASES = {"sphere_s4": 4, "sphere_s2": 2, "translated_s4": 4}
+
+
+def _case16(name):
+    if name.startswith("translated"):
+        return _translated()
+    p = _pts("sphere")
+    return p, p, np.zeros(3, np.float32)
+
+
+@pytest.mark.parametrize("case", sorted(FP16_CASES))
+def test_fp16_insert_bit_exact(gpu, case):
+    """Counts, offsets and the u16 entries (as a sorted set per subcell) of
+    InsertPointsSubcell_FP16 equal the oracle's."""
+    scs = FP16_CASES[case]
+    topo, q, t = _case16(case)
+    v, n = _gpu16(gpu, topo, q, t, 2.0, scs)
+    o, m = _oracle16(topo, q, t, 2.0, scs)
+    assert n == m
+    gs, os_ = v.subcells(), o.subcells(m)
+    assert np.array_equal(gs["cnt"].astype(np.int64), os_["cnt"].astype(np.int64))
+    assert np.array_equal(gs["off"].astype(np.int64), os_["off"].astype(np.int64))
+    la = H.subcell_lists(gs["cnt"], gs["off"], v.subcells_fp16())
+    lb = H.subcell_lists(os_["cnt"], os_["off"], o.subcells_fp16(m))
+    assert all(np.array_equal(x, y) for x, y in zip(la, lb))
+    v.close()
+
+
+@pytest.mark.parametrize("case", sorted(FP16_CASES))
+def test_fp16_level_set_vs_fp64(gpu, case):
+    """GatherLevelSet_FP16 within 1e-5 relative (2e-7 absolute at the zero
+    crossing) of the FP64 level set over the same dequantised entries."""
+    scs = FP16_CASES[case]
+    topo, q, t = _case16(case)
+    v, n = _gpu16(gpu, topo, q, t, 2.0, scs)
+    v.GatherLevelSet_FP16(scs, len(q), 2.0, t, n, 0, 0)
+    v.synchronize()
+    o, m = _oracle16(topo, q, t, 2.0, scs)
+    assert o.gather_level_set_fp16(2.0, scs, 64) == 0
+    _check(v, o)
+    v.close()
+
+
+def test_fp16_gather_needs_the_fp16_insert(gpu):
+    """The u16 entries belong to the insert that made them: a later plain
+    insert makes GatherLevelSet_FP16 an error until InsertPointsSubcell_FP16
+    runs again."""
+    from gvdb_amd import GVDBError
+    pts = _pts("sphere")
+    v, n = H.gpu_volume(gpu, pts, 2.0, stop_after="insert")
+    with pytest.raises(GVDBError):
+        v.GatherLevelSet_FP16(4, len(pts), 2.0, H.ZERO3, n, 0, 0)
+    n16 = v.InsertPointsSubcell_FP16(4, len(pts), 2.0, H.ZERO3)
+    assert n16 == n
+    v.GatherLevelSet_FP16(4, len(pts), 2.0, H.ZERO3, n16, 0, 0)
+    v.InsertPointsSubcell(4, len(pts), 2.0, H.ZERO3)
+    with pytest.raises(GVDBError):
+        v.GatherLevelSet_FP16(4, len(pts), 2.0, H.ZERO3, n, 0, 0)
+    v.close()

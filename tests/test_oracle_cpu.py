@@ -455,3 +455,112 @@ def test_isosurface_on_axis_through_one_point():
         vals.append(0.9 * l[2] + 0.1)
     assert min(vals) - 1e-4 <= img[0] <= max(vals) + 1e-4
     assert np.allclose(img[:3], img[0], atol=1e-12) and img[3] == 1.0
+
+
+# ------------------------------------------------ FP16 subcell variants
+def _fp16(pts, radius, scs=4, trans=(0.0, 0.0, 0.0), topo_pts=None):
+    o = O.Oracle()
+    assert o.rebuild_topology(pts if topo_pts is None else topo_pts, radius) == 0
+    o.finish_topology()
+    o.update_atlas()
+    o.clear_channel()
+    rc, n = o.insert_points_subcell_fp16(pts, radius, scs, trans)
+    assert rc == 0
+    return o, n
+
+
+def _bounds16(o):
+    b = o.bounds()
+    mn = np.asarray(b["pos_min"], np.float32)
+    return mn, np.asarray(b["pos_max"], np.float32) - mn
+
+
+@pytest.mark.parametrize("scs", [4, 2])
+def test_fp16_entries_are_the_quantised_lists(scs):
+    """Same lists as InsertPointsSubcell (no translation), each entry
+    u16((w - pos_min) / pos_range * 65535); compared as sorted sets."""
+    pts = _ball(800, 41)
+    o, n = _fp16(pts, 2.0, scs)
+    leaves = [tuple(int(c) for c in p) for p in o.pool(0, 0)["mPos"]]
+    sm = M.subcell_model(pts, 2.0, leaves, scs=scs)
+    assert n == int(sm["cnt"].sum())
+    sub = o.subcells(n)
+    assert np.array_equal(sub["cnt"], sm["cnt"]) and np.array_equal(sub["off"], sm["off"])
+    mn, rg = _bounds16(o)
+    got = o.subcells_fp16(n)
+    want = np.stack([M.quantize16(sm["pnt"][:, a], mn[a], rg[a]) for a in range(3)], 1)
+    for s in range(len(sm["cnt"])):
+        a0, a1 = int(sm["off"][s]), int(sm["off"][s] + sm["cnt"][s])
+        ga = got[a0:a1][np.lexsort(got[a0:a1].T[::-1])]
+        wa = want[a0:a1][np.lexsort(want[a0:a1].T[::-1])]
+        assert np.array_equal(ga, wa), s
+
+
+def test_fp16_level_set_equals_brute_force_over_dequantised_lists():
+    """min(3, min over the subcell's entries of |q - v| - r), q the
+    correctly rounded float32 u / 65535 * range + min (fused)."""
+    pts = _ball(500, 42)
+    o, n = _fp16(pts, 2.0)
+    o.gather_level_set_fp16(2.0, 4, 64)
+    mn, rg = _bounds16(o)
+    sub = o.subcells(n)
+    q16 = o.subcells_fp16(n)
+    deq = np.array([[M.dequantize16(q16[e, a], mn[a], rg[a]) for a in range(3)] for e in range(n)], np.float64)
+    leaves = o.pool(0, 0)
+    got = M.brick_interiors(o.atlas(64), leaves["mPos"], leaves["mValue"])
+    r = float(np.float32(2.0))
+    for s in range(len(sub["cnt"])):
+        b, loc = divmod(s, 8)
+        sx, sy, sz = loc % 2 * 4, (loc // 2) % 2 * 4, loc // 4 * 4
+        lst = deq[int(sub["off"][s]):int(sub["off"][s] + sub["cnt"][s])]
+        for z in range(4):
+            for y in range(4):
+                for x in range(4):
+                    v = np.array([leaves["mPos"][b][0] + sx + x, leaves["mPos"][b][1] + sy + y,
+                                  leaves["mPos"][b][2] + sz + z], np.float64)
+                    want = 3.0
+                    if len(lst):
+                        want = min(3.0, float(np.min(np.sqrt(((lst - v) ** 2).sum(1)) - r)))
+                    assert abs(got[b, sx + x, sy + y, sz + z] - want) <= 1e-12, (s, x, y, z)
+
+
+def test_fp16_level_set_is_within_a_quantisation_step_of_the_float_one():
+    pts = _ball(1500, 43)
+    o, n = _fp16(pts, 2.0)
+    o.gather_level_set_fp16(2.0, 4, 64)
+    f = _levelset(pts, 2.0, precision=64)
+    mn, rg = _bounds16(o)
+    step = float(np.linalg.norm(rg / 65535.0))
+    leaves = o.pool(0, 0)
+    a = M.brick_interiors(o.atlas(64), leaves["mPos"], leaves["mValue"])
+    b = M.brick_interiors(f.atlas(64), leaves["mPos"], leaves["mValue"])
+    d = np.abs(a - b)
+    assert d.max() <= step and d.max() > 0
+
+
+def test_fp16_binning_rule_differs_from_the_float_one_only_under_translation():
+    """InsertPointsSubcell_FP16 tests p + trans (for < 0 and NOHIT) in both
+    passes; the float variant's insert pass also drops p < 0.  Points with
+    p < 0 <= p + trans are binned by the first only."""
+    base = _ball(600, 44)
+    t = np.array([5.0, 3.0, 0.0], np.float32)
+    q = (base - t).astype(np.float32)
+    q[:50, 0] = np.float32(-0.5)                  # p.x < 0 <= w.x = 4.5
+    q[50:60, 2] = np.float32(1.0e10)              # NOHIT: skipped by both
+    w = (q + t).astype(np.float32)
+    o16, n16 = _fp16(q, 2.0, trans=t, topo_pts=np.concatenate([base, w[:50]]))
+    leaves = [tuple(int(c) for c in p) for p in o16.pool(0, 0)["mPos"]]
+    keep16 = (q[:, 2] != np.float32(1.0e10)) & np.all(w >= 0, 1)
+    keep32 = keep16 & np.all(q >= 0, 1)
+    sm16 = M.subcell_model(w[keep16], 2.0, leaves, scs=4)
+    sm32 = M.subcell_model(w[keep32], 2.0, leaves, scs=4)
+    assert np.array_equal(o16.subcells(n16)["cnt"], sm16["cnt"])
+    assert o16.undefined_events() == 0
+    o32 = O.Oracle()
+    assert o32.rebuild_topology(np.concatenate([base, w[:50]]), 2.0) == 0
+    o32.finish_topology()
+    o32.update_atlas()
+    o32.clear_channel()
+    rc, n32 = o32.insert_points_subcell(q, 2.0, 4, t)
+    assert rc == 0 and np.array_equal(o32.subcells(n32)["cnt"], sm32["cnt"])
+    assert n16 > n32
