@@ -253,6 +253,8 @@ def run_batched(args, grids):
 
     def batched():
         t0 = time.perf_counter()
+        for v in vols:
+            v.RequestFullRebuild(True)
         B.RebuildTopology(vols, ns, r)
         B.FinishTopology(vols, False, True)
         B.UpdateAtlas(vols)

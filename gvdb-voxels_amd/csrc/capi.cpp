@@ -74,6 +74,11 @@ int gvdb_rebuild_topology(gvdb_handle h, int n, float radius, const float orig[3
     const float z[3] = {0, 0, 0};
     return h->v.rebuild_topology(n, radius, orig ? orig : z);
 }
+int gvdb_accumulate_topology(gvdb_handle h, int n, float radius, const float orig[3], int min_depth) {
+    H_OR_FAIL(h);
+    const float z[3] = {0, 0, 0};
+    return h->v.accumulate_topology(n, radius, orig ? orig : z, min_depth);
+}
 int gvdb_rebuild_topology_cpu(gvdb_handle h, int n, const float orig[3], const float* host_xyz) {
     H_OR_FAIL(h);
     const float z[3] = {0, 0, 0};

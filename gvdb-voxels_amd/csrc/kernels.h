@@ -127,6 +127,39 @@ struct RootArgs {
 };
 hipError_t launch_write_root_multi(const RootArgs* a, int G, Arena* ar, hipStream_t s);
 
+// ---- incremental topology (ActivateIncreBricksGPU, topology.hip)
+// Per point w = p + o >= 0 and level L < D, the nodes covering
+// [int(w) - r, int(w) + r]: missing ones set their bit in bm[L] (domain
+// dom[L], node coordinates), found leaves get marker 1.
+struct IncreMarkArgs {
+    Tree t;
+    PointSrc pts;
+    float o[3];
+    float radius;
+    int D;
+    BitDomain dom[MAXLEV];
+    uint32_t* bm[MAXLEV];
+    uint32_t* marker;
+};
+hipError_t launch_incre_mark(const IncreMarkArgs& a, hipStream_t s);
+hipError_t launch_apply_markers(Node* leaves, const uint32_t* marker, u64 n, hipStream_t s);
+// new nodes of one level from its bitmap: node base + rank, child list cbase + rank
+struct IncreEmitArgs {
+    BitDomain dom;
+    const uint32_t* bm;
+    const uint32_t* rank;
+    Node* nodes;
+    int lev, range, has_clist;
+    u64 base, cbase;
+};
+hipError_t launch_incre_emit(const IncreEmitArgs& a, hipStream_t s);
+// gvdbLinkBricks / gvdbDelinkLeafBricks / gvdbDelinkBricks with tree walks
+hipError_t launch_link_walk(const Tree& t, int lev, hipStream_t s);
+hipError_t launch_delink_leaves(const Tree& t, hipStream_t s);
+hipError_t launch_delink_level(const Tree& t, int lev, hipStream_t s);
+// *out += leaves with mFlags == 0 (out must start at zero)
+hipError_t launch_count_unflagged(const Tree& t, uint32_t* out, hipStream_t s);
+
 // ---- atlas & points (points.hip)
 struct AtlasArgs {
     Tree t;

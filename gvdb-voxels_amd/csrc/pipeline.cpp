@@ -27,7 +27,8 @@ static inline float key_float(uint32_t k) {
 }
 
 // small per-grid result words (device dsmall / pinned hsmall, SMALL_WORDS each)
-enum { SM_BBOX = 0, SM_LINKED = 8, SM_TOTAL = 9, SM_ORPHANS = 10, SM_BUCKETS = 11, SM_LEN = 12, SM_COUNTS = 16, SM_BOUNDS = 40 };
+enum { SM_BBOX = 0, SM_LINKED = 8, SM_TOTAL = 9, SM_ORPHANS = 10, SM_BUCKETS = 11, SM_LEN = 12, SM_COUNTS = 16, SM_BOUNDS = 40,
+       SM_UNFLAGGED = 48 };
 
 static int drain(Volume* v, const Exec& ex, int G) {
     XCHK(v, hipMemcpyAsync(ex.hsmall, ex.dsmall, size_t(G) * SMALL_WORDS * 4, hipMemcpyDeviceToHost, ex.s));
@@ -65,6 +66,80 @@ int Volume::multi_rebuild_topology(Volume* const* vs, int G, const int* n, const
     int rc = drain(v0, ex, G);
     if (rc) return rc;
 
+    // per grid: bounds, depth and the reference's choice between a full and
+    // an incremental build (gvdb_volume_gvdb.cpp:1195-1217): full after
+    // RequestFullRebuild or Configure (mRebuildTopo), or when the depth is not
+    // mCurrDepth, which only that branch sets; incremental otherwise.  An
+    // incremental call whose depth is not the root's level (a requested full
+    // rebuild changed it since) rebuilds fully here, where the reference
+    // would walk a tree of another depth.
+    std::vector<float> mnmx(6 * size_t(G));
+    std::vector<int> depth(G), root(3 * size_t(G)), full_idx;
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        const uint32_t* hk = ex.hsmall + g * SMALL_WORDS + SM_BBOX;
+        for (int a = 0; a < 3; a++) {
+            mnmx[6 * g + a] = key_float(hk[a]);
+            mnmx[6 * g + 3 + a] = key_float(hk[3 + a]);
+            // GetBoundingBox padding (gvdb_volume_gvdb.cpp:5452-5460)
+            v->posMin_[a] = mnmx[6 * g + a] - 1;
+            v->posMax_[a] = mnmx[6 * g + 3 + a] + 2;
+            v->posMin_[a] = 0 > v->posMin_[a] ? 0 : v->posMin_[a];
+            v->posMax_[a] = 0 > v->posMax_[a] ? 0 : v->posMax_[a];
+        }
+        depth[g] = v->determine_depth(&root[3 * g]);
+        if (depth[g] == 0) FAILG(g, v->fail(GVDB_ERR_DEPTH, "Points extents smaller than 1 brick. Not yet supported."));
+        if (depth[g] < 0 || depth[g] > 5 || depth[g] >= v->levels_)
+            FAILG(g, v->fail(GVDB_ERR_DEPTH, "Point extents exceed 5 levels."));
+    }
+    for (int g = 0; g < G; g++) {
+        Volume* v = vs[g];
+        if (v->rebuild_topo_) {
+            v->clear();
+        } else if (depth[g] != v->curr_depth_ || v->root_ == ID_UNDEFL || int((v->root_ >> 8) & 0xFF) != depth[g]) {
+            v->clear();
+            v->rebuild_topo_ = true;
+            v->curr_depth_ = depth[g];
+        }
+        if (v->rebuild_topo_) {
+            full_idx.push_back(g);
+            continue;
+        }
+        const Exec eg{ex.s, ex.ar, ex.dsmall + g * SMALL_WORDS, ex.hsmall + g * SMALL_WORDS};
+        rc = v->incremental_topology(n[g], radius[g], orig3 + 3 * g, depth[g], false, &mnmx[6 * g], eg);
+        if (rc) FAILG(g, rc);
+    }
+    if (!full_idx.empty()) {
+        const int F = int(full_idx.size());
+        std::vector<Volume*> fv(F);
+        std::vector<int> fn(F), fd(F), fr(3 * size_t(F));
+        std::vector<float> frad(F), fo(3 * size_t(F)), fm(6 * size_t(F));
+        for (int k = 0; k < F; k++) {
+            const int g = full_idx[k];
+            fv[k] = vs[g];
+            fn[k] = n[g];
+            fd[k] = depth[g];
+            frad[k] = radius[g];
+            for (int a = 0; a < 3; a++) { fr[3 * k + a] = root[3 * g + a]; fo[3 * k + a] = orig3[3 * g + a]; }
+            for (int a = 0; a < 6; a++) fm[6 * k + a] = mnmx[6 * g + a];
+        }
+        int f = -1;
+        rc = multi_full_build(fv.data(), F, fn.data(), frad.data(), fo.data(), fd.data(), fr.data(), fm.data(), ex, &f);
+        if (rc) {
+            if (failed && f >= 0) *failed = full_idx[f];
+            return rc;
+        }
+    }
+    if (G == 1) v0->stage_end(ST_TOPO);
+    return GVDB_OK;
+}
+
+// The full build of RebuildTopology (ActivateBricksGPU + ActivateExtraBricksGPU)
+// for grids already cleared, with their depth, root corner and point bounds.
+int Volume::multi_full_build(Volume* const* vs, int G, const int* n, const float* radius, const float* orig3,
+                             const int* depth, const int* rootPos, const float* mnmx, const Exec& ex, int* failed) {
+    Volume* v0 = vs[0];
+    int rc = 0;
     // per grid: depth, root, domains, scratch
     struct Plan {
         int D = 0;
@@ -79,29 +154,12 @@ int Volume::multi_rebuild_topology(Volume* const* vs, int G, const int* n, const
     for (int g = 0; g < G; g++) {
         Volume* v = vs[g];
         Plan& pl = plan[g];
-        const uint32_t* hk = ex.hsmall + g * SMALL_WORDS + SM_BBOX;
-        float mn[3], mx[3];
-        for (int a = 0; a < 3; a++) {
-            mn[a] = key_float(hk[a]);
-            mx[a] = key_float(hk[3 + a]);
-            // GetBoundingBox padding (gvdb_volume_gvdb.cpp:5452-5460)
-            v->posMin_[a] = mn[a] - 1;
-            v->posMax_[a] = mx[a] + 2;
-            v->posMin_[a] = 0 > v->posMin_[a] ? 0 : v->posMin_[a];
-            v->posMax_[a] = 0 > v->posMax_[a] ? 0 : v->posMax_[a];
-        }
-        int depth = v->determine_depth(pl.rootPos);
-        if (depth == 0) FAILG(g, v->fail(GVDB_ERR_DEPTH, "Points extents smaller than 1 brick. Not yet supported."));
-        if (depth < 0 || depth > 5 || depth >= v->levels_) FAILG(g, v->fail(GVDB_ERR_DEPTH, "Point extents exceed 5 levels."));
-        // The reference rebuilds fully on the first call, after
-        // RequestFullRebuild and on depth changes, and walks an incremental
-        // path otherwise (gvdb_volume_gvdb.cpp:1195-1217); every call here is
-        // a full rebuild.
-        v->clear();
-        v->curr_depth_ = depth;
-        pl.D = depth;
+        const float* mn = mnmx + 6 * g;
+        const float* mx = mnmx + 6 * g + 3;
+        for (int a = 0; a < 3; a++) pl.rootPos[a] = rootPos[3 * g + a];
+        pl.D = depth[g];
         pl.dilate = radius[g] >= 1.0f;
-        const int D = depth;
+        const int D = pl.D;
         int rr = pl.dilate ? int(ceilf(radius[g])) : 0;
         long long lo_v[3], hi_v[3];
         for (int a = 0; a < 3; a++) {
@@ -260,7 +318,165 @@ int Volume::multi_rebuild_topology(Volume* const* vs, int G, const int* n, const
         v->dev_stale_ = false;
         v->topo_gen_++;
     }
-    if (G == 1) v0->stage_end(ST_TOPO);
+    return GVDB_OK;
+}
+
+// AccumulateTopology (gvdb_volume_gvdb.cpp:1158-1178): depth at least
+// min_depth; RebuildTopology while a full rebuild is pending; otherwise the
+// incremental build keeping the flags of the existing leaves.  A depth above
+// the root's level is refused (the reference would walk past its root).
+int Volume::accumulate_topology(int n, float radius, const float orig[3], int min_depth) {
+    if (levels_ == 0) return fail(GVDB_ERR_STATE, "AccumulateTopology before Configure");
+    if (n <= 0 || !pts_.base || n > pts_.n) return fail(GVDB_ERR_INVALID, "AccumulateTopology: point count exceeds SetPoints");
+    if (!(radius >= 0.0f) || radius > 32.0f) return fail(GVDB_ERR_UNSUPPORTED, "AccumulateTopology: radius must be in [0, 32]");
+    if (rebuild_topo_) return rebuild_topology(n, radius, orig);
+    const Exec ex = self_exec();
+    stage_begin(ST_TOPO);
+    PointSrc P = pts_;
+    P.n = n;
+    BboxArgs ba{P, {orig[0], orig[1], orig[2]}, ex.dsmall + SM_BBOX};
+    XCHK(this, launch_bbox_multi(&ba, 1, ex.ar, ex.s));
+    int rc = drain(this, ex, 1);
+    if (rc) return rc;
+    float mnmx[6];
+    for (int a = 0; a < 3; a++) {
+        mnmx[a] = key_float(ex.hsmall[SM_BBOX + a]);
+        mnmx[3 + a] = key_float(ex.hsmall[SM_BBOX + 3 + a]);
+        posMin_[a] = mnmx[a] - 1;
+        posMax_[a] = mnmx[3 + a] + 2;
+        posMin_[a] = 0 > posMin_[a] ? 0 : posMin_[a];
+        posMax_[a] = 0 > posMax_[a] ? 0 : posMax_[a];
+    }
+    int rootPos[3];
+    int depth = determine_depth(rootPos);
+    if (depth < min_depth) depth = min_depth;
+    if (depth == 0) return fail(GVDB_ERR_DEPTH, "Points extents smaller than 1 brick. Not yet supported.");
+    if (depth < 0 || depth > 5 || depth >= levels_) return fail(GVDB_ERR_DEPTH, "Point extents exceed 5 levels.");
+    if (root_ == ID_UNDEFL || depth > int((root_ >> 8) & 0xFF))
+        return fail(GVDB_ERR_DEPTH, "AccumulateTopology: the points need a deeper tree than the current one "
+                                    "(call RequestFullRebuild)");
+    rc = incremental_topology(n, radius, orig, depth, true, mnmx, ex);
+    if (rc) return rc;
+    stage_end(ST_TOPO);
+    return GVDB_OK;
+}
+
+// ActivateIncreBricksGPU (gvdb_volume_gvdb.cpp:1020-1156) for one grid:
+// markers, the bitmaps of missing nodes, and (only when some node is
+// missing) markers applied, nodes appended in key order, levels linked
+// top-down, unflagged leaves and childless nodes delinked bottom-up, the
+// leaf used count updated.  `mnmx`: min and max of p + orig over the points.
+int Volume::incremental_topology(int n, float radius, const float orig[3], int D, bool accum, const float* mnmx,
+                                 const Exec& ex) {
+    const u64 total = pool_[0][0].lastEle;
+    if (total == 0) return GVDB_OK;
+    int rc = commit_pools();
+    if (rc) return rc;
+    if (D <= 0 || D >= levels_) return fail(GVDB_ERR_DEPTH, "incremental topology: depth out of range");
+    BitDomain dom[MAXLEV];
+    u64 bm_off[MAXLEV], tmp_off[MAXLEV], bm_words = 0, tmp_words = 0;
+    for (int L = 0; L < D; L++) {
+        const int rr = int(range_[L]);
+        BitDomain& d = dom[L];
+        d.nbits = 1;
+        for (int a = 0; a < 3; a++) {
+            const float lo = std::isfinite(mnmx[a]) ? std::max(mnmx[a], 0.0f) : 0.0f;
+            const float hi = std::isfinite(mnmx[3 + a]) ? std::max(mnmx[3 + a], 0.0f) : 0.0f;
+            const int blo = int(float(int(lo)) - radius), bhi = int(float(int(hi)) + radius);
+            d.lo[a] = std::max(0, blo / rr);
+            d.dim[a] = std::max(1, bhi / rr - d.lo[a] + 1);
+            d.nbits *= u64(d.dim[a]);
+        }
+        if (d.nbits > (u64(1) << 34))
+            return fail(GVDB_ERR_RANGE, "incremental topology: node box of the points exceeds 2^34 nodes");
+        d.nwords = (d.nbits + 31) / 32;
+        bm_off[L] = bm_words;
+        bm_words += d.nwords;
+        tmp_off[L] = tmp_words;
+        tmp_words += scan_tmp_words(d.nwords);
+    }
+    XCHK(this, bitmaps_.ensure(bm_words * 4 + 256));
+    XCHK(this, ranks_.ensure(bm_words * 4 + 256));
+    XCHK(this, scan_tmp_.ensure(tmp_words * 4 + 256));
+    XCHK(this, incr_marker_.ensure(size_t(total) * 4));
+    Tree t = make_tree();
+    std::vector<FillArgs> fa{FillArgs{bitmaps_.p, bm_words, 0, 0}};
+    if (!accum) fa.push_back(FillArgs{incr_marker_.p, total, 0, 0});
+    XCHK(this, launch_fill_multi(fa.data(), int(fa.size()), ex.ar, ex.s));
+    if (accum) {
+        FlagArgs fl{t, incr_marker_.as<uint32_t>(), int(total), 0};
+        XCHK(this, launch_leaf_flags_multi(&fl, 1, ex.ar, ex.s));
+    }
+    IncreMarkArgs ma;
+    memset(&ma, 0, sizeof(ma));
+    ma.t = t;
+    ma.pts = pts_;
+    ma.pts.n = n;
+    for (int a = 0; a < 3; a++) ma.o[a] = orig[a];
+    ma.radius = radius;
+    ma.D = D;
+    for (int L = 0; L < D; L++) {
+        ma.dom[L] = dom[L];
+        ma.bm[L] = bitmaps_.as<uint32_t>() + bm_off[L];
+    }
+    ma.marker = incr_marker_.as<uint32_t>();
+    XCHK(this, launch_incre_mark(ma, ex.s));
+    std::vector<ScanArgs> sa;
+    for (int L = 0; L < D; L++)
+        sa.push_back(ScanArgs{bitmaps_.as<uint32_t>() + bm_off[L], nullptr, 1, ranks_.as<uint32_t>() + bm_off[L],
+                              dom[L].nwords, scan_tmp_.as<uint32_t>() + tmp_off[L], ex.dsmall + SM_COUNTS + 2 * L});
+    XCHK(this, launch_scan_multi(sa.data(), int(sa.size()), ex.ar, ex.s));
+    rc = drain(this, ex, 1);
+    if (rc) return rc;
+    u64 cnt[MAXLEV], added = 0;
+    for (int L = 0; L < D; L++) {
+        cnt[L] = ex.hsmall[SM_COUNTS + 2 * L];
+        added += cnt[L];
+    }
+    if (added == 0) return GVDB_OK;   // the reference returns before applying the markers
+    XCHK(this, launch_apply_markers(pool_[0][0].dev.as<Node>(), incr_marker_.as<uint32_t>(), total, ex.s));
+    std::vector<FillArgs> cf;
+    u64 base0[MAXLEV], base1[MAXLEV];
+    for (int L = 0; L < D; L++) {
+        base0[L] = pool_[0][L].lastEle;
+        base1[L] = L > 0 ? pool_[1][L].lastEle : 0;
+        rc = ensure_pool_capacity(0, L, std::max<u64>(base0[L] + cnt[L], 1), true, ex.s);
+        if (rc) return rc;
+        if (L > 0) {
+            rc = ensure_pool_capacity(1, L, std::max<u64>(base1[L] + cnt[L], 1), true, ex.s);
+            if (rc) return rc;
+        }
+    }
+    for (int L = 0; L < D; L++) {
+        if (cnt[L] == 0) continue;
+        IncreEmitArgs ea{dom[L], bitmaps_.as<uint32_t>() + bm_off[L], ranks_.as<uint32_t>() + bm_off[L],
+                         pool_[0][L].dev.as<Node>(), L, int(range_[L]), L > 0 ? 1 : 0, base0[L], base1[L]};
+        XCHK(this, launch_incre_emit(ea, ex.s));
+        if (L > 0)
+            cf.push_back(FillArgs{pool_[1][L].dev.as<u64>() + base1[L] * voxcnt(L), cnt[L] * voxcnt(L), ID_UNDEF64, 1});
+        pool_[0][L].lastEle += cnt[L];
+        pool_[0][L].usedNum += cnt[L];
+        if (L > 0) {
+            pool_[1][L].lastEle += cnt[L];
+            pool_[1][L].usedNum += cnt[L];
+        }
+    }
+    XCHK(this, launch_fill_multi(cf.data(), int(cf.size()), ex.ar, ex.s));
+    compute_top_lev();
+    t = make_tree();
+    for (int L = D - 1; L >= 0; L--) XCHK(this, launch_link_walk(t, L, ex.s));
+    XCHK(this, launch_delink_leaves(t, ex.s));
+    for (int L = 1; L < D; L++) XCHK(this, launch_delink_level(t, L, ex.s));
+    FillArgs z{ex.dsmall + SM_UNFLAGGED, 1, 0, 0};
+    XCHK(this, launch_fill_multi(&z, 1, ex.ar, ex.s));
+    XCHK(this, launch_count_unflagged(t, ex.dsmall + SM_UNFLAGGED, ex.s));
+    rc = drain(this, ex, 1);
+    if (rc) return rc;
+    pool_[0][0].usedNum = pool_[0][0].lastEle - ex.hsmall[SM_UNFLAGGED];
+    compute_top_lev();
+    host_valid_ = false;
+    dev_stale_ = false;
+    topo_gen_++;
     return GVDB_OK;
 }
 
@@ -356,7 +572,7 @@ int Volume::multi_update_atlas(Volume* const* vs, int G, const Exec& ex, int* fa
     XCHK(v0, launch_update_atlas_multi(aa.data(), int(aa.size()), ex.ar, ex.s));
     for (int g = 0; g < G; g++) {
         Volume* v = vs[g];
-        v->atlas_used_ = v->pool_[0][0].lastEle;
+        v->atlas_used_ = v->pool_[0][0].usedNum;     // the flagged leaves (AtlasAlloc calls)
         v->host_valid_ = false;
         v->compute_top_lev();
         v->atlas_gen_ = v->topo_gen_;

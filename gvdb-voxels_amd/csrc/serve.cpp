@@ -110,6 +110,7 @@ int Server::build(const std::vector<int>& which, Batch& b) {
     const int G = int(vs.size());
     std::vector<float> z(size_t(G) * 3, 0.f);
     int f = -1;
+    for (Volume* v : vs) v->request_full_rebuild(true);
     int rc = b.rebuild_topology(vs.data(), G, n.data(), r.data(), z.data(), &f);
     if (!rc) rc = b.finish_topology(vs.data(), G, false, true, &f);
     if (!rc) rc = b.update_atlas(vs.data(), G, &f);

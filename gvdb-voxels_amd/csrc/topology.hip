@@ -17,6 +17,7 @@
 // Higher levels are marked from the set bits of the level below, which is the
 // same set because power-of-two node ranges nest.
 #include "kernels.h"
+#include "tree_device.h"
 
 namespace gvdb {
 
@@ -401,6 +402,202 @@ hipError_t launch_write_root_multi(const RootArgs* a, int G, Arena* ar, hipStrea
     hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
     if (e != hipSuccess || L.blocks == 0) return e;
     hipLaunchKernelGGL(k_write_root, dim3(L.blocks), dim3(64), 0, s, a[0], L.map);
+    return hipGetLastError();
+}
+
+
+// ------------------------------------------------- incremental topology ----
+// ActivateIncreBricksGPU (src/gvdb_volume_gvdb.cpp:1020-1156).  The reference
+// appends the keys of missing nodes with an atomic counter and sorts them;
+// here they are bits of one x-major bitmap per level (bit order = key order),
+// so a scan gives each new node its place.
+
+// gvdbCalcIncreExtraBrickId (kernels/cuda_gvdb_particles.cuh:420-465): the
+// box is [int(int(w) - r), int(int(w) + r)] per axis, node corners from
+// (C-truncated) box / range * range.  Corners below zero (radius 8 or more)
+// are skipped: the reference would cast them into its u16 keys.
+__global__ __launch_bounds__(256) void k_incre_mark(IncreMarkArgs a) {
+    const Tree& t = a.t;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < a.pts.n; i += gridDim.x * 256) {
+        const float* p = point_at(a.pts, i);
+        const float w[3] = {p[0] + a.o[0], p[1] + a.o[1], p[2] + a.o[2]};
+        if (!isfinite(w[0]) || !isfinite(w[1]) || !isfinite(w[2])) continue;
+        if (w[0] < 0 || w[1] < 0 || w[2] < 0) continue;
+        int bmin[3], bmax[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int iw = int(w[k]);
+            bmin[k] = int(float(iw) - a.radius);
+            bmax[k] = int(float(iw) + a.radius);
+        }
+        for (int lev = 0; lev < a.D; lev++) {
+            const int rr = t.range[lev];
+            const int x0 = bmin[0] / rr * rr, x1 = bmax[0] / rr * rr;
+            const int y0 = bmin[1] / rr * rr, y1 = bmax[1] / rr * rr;
+            const int z0 = bmin[2] / rr * rr, z1 = bmax[2] / rr * rr;
+            for (int x = x0; x <= x1; x += rr)
+                for (int y = y0; y <= y1; y += rr)
+                    for (int z = z0; z <= z1; z += rr) {
+                        if (x < 0 || y < 0 || z < 0) continue;
+                        const int id = pos_parent(t, x, y, z, lev);
+                        if (id < 0) {
+                            u64 idx;
+                            if (dom_index(a.dom[lev], x / rr, y / rr, z / rr, idx)) set_bit(a.bm[lev], idx);
+                        } else if (lev == 0) {
+                            a.marker[id] = 1u;
+                        }
+                    }
+        }
+    }
+}
+
+hipError_t launch_incre_mark(const IncreMarkArgs& a, hipStream_t s) {
+    if (a.pts.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_incre_mark, dim3(grid_for(u64(a.pts.n), 16384)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// "Update markers": the leaves present before the call take their markers.
+__global__ __launch_bounds__(256) void k_apply_markers(Node* leaves, const uint32_t* marker, u64 n) {
+    const u64 i = u64(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) leaves[i].mFlags = uint8_t(marker[i]);
+}
+
+hipError_t launch_apply_markers(Node* leaves, const uint32_t* marker, u64 n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_apply_markers, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, leaves, marker, n);
+    return hipGetLastError();
+}
+
+// SetupNode + child-list allocation of the appended nodes
+// (src/gvdb_volume_gvdb.cpp:1087-1110), in key order.
+__global__ __launch_bounds__(256) void k_incre_emit(IncreEmitArgs a) {
+    for (u64 w = u64(blockIdx.x) * 256 + threadIdx.x; w < a.dom.nwords; w += u64(gridDim.x) * 256) {
+        uint32_t bits = a.bm[w];
+        u64 r = a.rank[w];
+        while (bits) {
+            const int b = __ffs(bits) - 1;
+            bits &= bits - 1;
+            int x, y, z;
+            dom_coords(a.dom, w * 32 + b, x, y, z);
+            write_node(a.nodes + a.base + r, a.lev, x, y, z, a.range,
+                       a.has_clist ? Elem(1, a.lev, a.cbase + r) : ID_UNDEFL);
+            r++;
+        }
+    }
+}
+
+hipError_t launch_incre_emit(const IncreEmitArgs& a, hipStream_t s) {
+    if (a.dom.nwords == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_incre_emit, dim3(grid_for(a.dom.nwords, 16384)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// Cell of `pos` inside the node `pn` of level plev: ((pos - corner) * res) /
+// range per axis, as the reference computes it.
+__device__ inline bool cell_in(const Tree& t, int plev, const Node* pn, const int pos[3], int& bit) {
+    const int res = t.res[plev];
+    const int sh = __builtin_ctz(unsigned(t.range[plev])) - t.logdim[plev];
+    int c[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) c[k] = child_cell((long long)pos[k] - pn->mPos[k], sh);
+    if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= res || c[1] >= res || c[2] >= res) return false;
+    bit = (c[2] * res + c[1]) * res + c[0];
+    return true;
+}
+
+// gvdbLinkBricks (kernels/cuda_gvdb_nodes.cuh:470-505) over every node of
+// one level: flagged nodes are put in the slot of the node one level up
+// that the walk from the top node reaches.  The walk reads child lists two
+// or more levels up, the writes go one level up, so one launch per level.
+__global__ __launch_bounds__(256) void k_link_walk(Tree t, int lev) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= t.nodecnt[lev]) return;
+    Node* n = t.nodes[lev] + i;
+    if (!n->mFlags) return;
+    const int pid = get_parent(t, lev + 1, n->mPos);
+    if (pid < 0) return;
+    const Node* pn = t.nodes[lev + 1] + pid;
+    int bit;
+    if (!cell_in(t, lev + 1, pn, n->mPos, bit)) return;
+    n->mParent = Elem(0, lev + 1, u64(pid));
+    const u64 cndx = pn->mChildList >> 16;
+    if (cndx >= u64(t.nodecnt[lev + 1])) return;
+    const u64 r = u64(t.res[lev + 1]);
+    t.clists[lev + 1][cndx * r * r * r + u64(bit)] = Elem(0, lev, u64(i));
+}
+
+hipError_t launch_link_walk(const Tree& t, int lev, hipStream_t s) {
+    const int n = t.nodecnt[lev];
+    if (n <= 0 || lev + 1 >= t.levels || !t.clists[lev + 1]) return hipSuccess;
+    hipLaunchKernelGGL(k_link_walk, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, t, lev);
+    return hipGetLastError();
+}
+
+// Empties the parent's slot of node n (level lev) and marks n parentless.
+__device__ inline void delink_from_parent(const Tree& t, int lev, Node* n) {
+    const int pid = get_parent(t, lev + 1, n->mPos);
+    if (pid < 0) return;
+    const Node* pn = t.nodes[lev + 1] + pid;
+    int bit;
+    if (!cell_in(t, lev + 1, pn, n->mPos, bit)) return;
+    n->mParent = ID_UNDEF64;
+    const u64 r = u64(t.res[lev + 1]);
+    t.clists[lev + 1][(pn->mChildList >> 16) * r * r * r + u64(bit)] = ID_UNDEF64;
+}
+
+// gvdbDelinkLeafBricks (kernels/cuda_gvdb_nodes.cuh:401-429)
+__global__ __launch_bounds__(256) void k_delink_leaves(Tree t) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= t.nodecnt[0]) return;
+    Node* n = t.nodes[0] + i;
+    if (n->mFlags || n->mParent == ID_UNDEF64) return;
+    delink_from_parent(t, 0, n);
+}
+
+hipError_t launch_delink_leaves(const Tree& t, hipStream_t s) {
+    const int n = t.nodecnt[0];
+    if (n <= 0 || t.levels < 2 || !t.clists[1]) return hipSuccess;
+    hipLaunchKernelGGL(k_delink_leaves, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, t);
+    return hipGetLastError();
+}
+
+// gvdbDelinkBricks (kernels/cuda_gvdb_nodes.cuh:431-467): a parentless node
+// is unflagged; a node with a child is flagged; a childless one is unflagged
+// and leaves its parent.  Its own list is read at its pool index.
+__global__ __launch_bounds__(256) void k_delink_level(Tree t, int lev) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= t.nodecnt[lev]) return;
+    Node* n = t.nodes[lev] + i;
+    if (n->mParent == ID_UNDEF64) { n->mFlags = 0; return; }
+    const u64 r = u64(t.res[lev]);
+    const u64 len = r * r * r;
+    const u64* cl = t.clists[lev] + u64(i) * len;
+    for (u64 c = 0; c < len; c++)
+        if (cl[c] != ID_UNDEF64) { n->mFlags = 1; return; }
+    n->mFlags = 0;
+    delink_from_parent(t, lev, n);
+}
+
+hipError_t launch_delink_level(const Tree& t, int lev, hipStream_t s) {
+    const int n = t.nodecnt[lev];
+    if (n <= 0 || lev + 1 >= t.levels || !t.clists[lev] || !t.clists[lev + 1]) return hipSuccess;
+    hipLaunchKernelGGL(k_delink_level, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, t, lev);
+    return hipGetLastError();
+}
+
+// UpdateUsed (src/gvdb_volume_gvdb.cpp:1145-1149): unflagged leaves.
+__global__ __launch_bounds__(256) void k_count_unflagged(Tree t, uint32_t* out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool off = i < t.nodecnt[0] && !t.nodes[0][i].mFlags;
+    const u64 b = __ballot(off);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(out, uint32_t(__popcll(b)));
+}
+
+hipError_t launch_count_unflagged(const Tree& t, uint32_t* out, hipStream_t s) {
+    const int n = t.nodecnt[0];
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_count_unflagged, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, t, out);
     return hipGetLastError();
 }
 

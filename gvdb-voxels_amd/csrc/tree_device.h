@@ -47,6 +47,54 @@ __device__ inline int pos_leaf(const Tree& t, int px, int py, int pz) {
     }
 }
 
+// getPosParent (kernels/cuda_gvdb_nodes.cuh:303-348): the node at stopLev
+// on the walk from the top node to `pos`, or -1.  A stopLev at or above the
+// top node's level is never reached, as in the reference.
+__device__ inline int pos_parent(const Tree& t, int px, int py, int pz, int stopLev) {
+    const Node* nd = node_at(t, t.top_lev, 0);
+    int l = nd->mLev;
+    if (l == 0) return 0;
+    while (l > 0) {
+        int res = t.res[l];
+        const int sh = __builtin_ctz(unsigned(t.range[l])) - t.logdim[l];
+        int a = child_cell((long long)px - nd->mPos[0], sh);
+        int b = child_cell((long long)py - nd->mPos[1], sh);
+        int c = child_cell((long long)pz - nd->mPos[2], sh);
+        if (a < 0 || b < 0 || c < 0 || a >= res || b >= res || c >= res) return -1;
+        int child = child_of(t, nd, (c * res + b) * res + a);
+        l--;
+        if (child < 0) return -1;
+        if (l == stopLev) return child;
+        nd = node_at(t, l, u64(child));
+    }
+    return -1;
+}
+
+// getParent (kernels/cuda_gvdb_nodes.cuh:350-397): the node at stopLev
+// holding `pos`; the top node itself (0) when stopLev is its level or it is
+// unflagged; -1 when the walk meets an empty slot (where the reference reads
+// past it) or leaves the node.
+__device__ inline int get_parent(const Tree& t, int stopLev, const int pos[3]) {
+    const Node* nd = node_at(t, t.top_lev, 0);
+    if (!nd->mFlags) return 0;
+    int l = nd->mLev;
+    if (l == stopLev) return 0;
+    while (l > stopLev) {
+        int res = t.res[l];
+        const int sh = __builtin_ctz(unsigned(t.range[l])) - t.logdim[l];
+        int a = child_cell((long long)pos[0] - nd->mPos[0], sh);
+        int b = child_cell((long long)pos[1] - nd->mPos[1], sh);
+        int c = child_cell((long long)pos[2] - nd->mPos[2], sh);
+        if (a < 0 || b < 0 || c < 0 || a >= res || b >= res || c >= res) return -1;
+        l--;
+        int child = child_of(t, nd, (c * res + b) * res + a);
+        if (child < 0) return -1;
+        if (l == stopLev) return child;
+        nd = node_at(t, l, u64(child));
+    }
+    return -1;
+}
+
 // getNodeAtPoint: float walk with per-level bounds, returns the leaf or -1.
 __device__ inline int point_leaf(const Tree& t, float px, float py, float pz) {
     int lev = t.top_lev;

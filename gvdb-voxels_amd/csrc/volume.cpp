@@ -96,7 +96,8 @@ Volume::~Volume() {
     for (auto& c : chan_) c.data.release();
     for (auto& r : rbuf_) r.data.release();
     DevBuf* bufs[] = {&amap_, &nbr_, &nmask_, &nbase_, &faces_, &lenparts_, &pts_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_, &sc_use_,
-                      &bkt_key_, &bkt_slot_, &bkt_cnt_, &bkt_off_, &sorted_, &sc_flags_, &small_, &masks_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev};
+                      &bkt_key_, &bkt_slot_, &bkt_cnt_, &bkt_off_, &sorted_, &sc_flags_, &small_, &masks_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev,
+                      &incr_marker_, &sc_pnt16_};
     for (DevBuf* b : bufs) b->release();
     for (int s = 0; s < ST_COUNT; s++)
         for (int k = 0; k < 2; k++)
@@ -285,10 +286,10 @@ u64 Volume::pool_alloc(int grp, int lev) {
     return Elem(grp, lev, p.lastEle - 1);
 }
 
-int Volume::ensure_pool_capacity(int grp, int lev, u64 need) {
+int Volume::ensure_pool_capacity(int grp, int lev, u64 need, bool preserve, hipStream_t s) {
     Pool& p = pool_[grp][lev];
     while (p.max < need) p.max *= 2;
-    HIPCHK(p.dev.ensure(size_t(p.max * p.stride)));
+    HIPCHK(p.dev.ensure(size_t(p.max * p.stride), preserve, s));
     return GVDB_OK;
 }
 
@@ -711,7 +712,8 @@ unsigned long long Volume::device_bytes() const {
     for (const auto& r : rbuf_) b += r.data.bytes;
     const DevBuf* bufs[] = {&amap_, &nbr_, &nmask_, &nbase_, &faces_, &lenparts_, &pts_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_,
                             &sc_use_, &bkt_key_, &bkt_slot_, &bkt_cnt_, &bkt_off_, &sorted_, &sc_flags_, &small_,
-                            &masks_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev};
+                            &masks_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev,
+                      &incr_marker_, &sc_pnt16_};
     for (const DevBuf* d : bufs) b += d->bytes;
     return b;
 }

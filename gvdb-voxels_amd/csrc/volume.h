@@ -89,8 +89,10 @@ public:
     int upload_points(const float* host, int n);
 
     // topology
-    int request_full_rebuild(bool on) { if (on) rebuild_topo_ = true; return 0; }
+    // RequestFullRebuild (gvdb_volume_gvdb.h:432): sets mRebuildTopo to `on`
+    int request_full_rebuild(bool on) { rebuild_topo_ = on; return 0; }
     int rebuild_topology(int n, float radius, const float orig[3]);
+    int accumulate_topology(int n, float radius, const float orig[3], int min_depth);
     int rebuild_topology_cpu(int n, const float orig[3], const float* host);
     int activate_space(const float pos[3]);
     int clear();
@@ -145,6 +147,10 @@ public:
     // above call these with G = 1.  On failure `failed` names the grid.
     static int multi_rebuild_topology(Volume* const* v, int G, const int* n, const float* radius, const float* orig3,
                                       const Exec& ex, int* failed);
+    static int multi_full_build(Volume* const* v, int G, const int* n, const float* radius, const float* orig3,
+                                const int* depth, const int* rootPos, const float* mnmx, const Exec& ex, int* failed);
+    int incremental_topology(int n, float radius, const float orig[3], int D, bool accum, const float* mnmx,
+                             const Exec& ex);
     static int multi_finish_topology(Volume* const* v, int G, bool commit, bool bounds, const Exec& ex, int* failed);
     static int multi_update_atlas(Volume* const* v, int G, const Exec& ex, int* failed);
     static int multi_clear_channel(Volume* const* v, int G, int chan, const Exec& ex, int* failed);
@@ -219,6 +225,7 @@ private:
     // topology scratch
     DevBuf small_;       // bbox keys, counts
     DevBuf bitmaps_, ranks_, masks_, scan_tmp_, flags_, slot_scan_;
+    DevBuf incr_marker_;   // per-leaf markers of the incremental build
     uint32_t* pinned_ = nullptr;   // host staging (256 words)
     Arena arena_;
     std::shared_ptr<SharedStream> own_stream_, cur_stream_;
@@ -236,7 +243,7 @@ private:
     void stage_begin(Stage s);
     void stage_end(Stage s);
     u64 pool_alloc(int grp, int lev);
-    int ensure_pool_capacity(int grp, int lev, u64 need);
+    int ensure_pool_capacity(int grp, int lev, u64 need, bool preserve = false, hipStream_t s = nullptr);
     int sync_host();
     int commit_pools();
     Tree make_tree();

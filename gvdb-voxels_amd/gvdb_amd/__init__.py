@@ -36,6 +36,7 @@ _SIGS = [
     ("gvdb_set_points_gpu", _i, [_vp, _vp, _i, _i, _i]), ("gvdb_upload_points", _i, [_vp, _vp, _i]),
     ("gvdb_request_full_rebuild", _i, [_vp, _i]), ("gvdb_rebuild_topology", _i, [_vp, _i, _f, _vp]),
     ("gvdb_rebuild_topology_cpu", _i, [_vp, _i, _vp, _vp]), ("gvdb_activate_space", _i, [_vp, _vp]),
+    ("gvdb_accumulate_topology", _i, [_vp, _i, _f, _vp, _i]),
     ("gvdb_clear", _i, [_vp]), ("gvdb_finish_topology", _i, [_vp, _i, _i]), ("gvdb_update_atlas", _i, [_vp]),
     ("gvdb_clear_channel", _i, [_vp, _i]),
     ("gvdb_insert_points_subcell", _i, [_vp, _i, _i, _f, _vp, ctypes.POINTER(_i)]),
@@ -172,6 +173,9 @@ class VolumeGVDB:
 
     def RebuildTopology(self, n, radius, orig=(0, 0, 0)):
         return self._chk(load().gvdb_rebuild_topology(self.h, int(n), float(radius), _p(_f3(orig))))
+
+    def AccumulateTopology(self, n, radius, orig=(0, 0, 0), depth=1):
+        return self._chk(load().gvdb_accumulate_topology(self.h, int(n), float(radius), _p(_f3(orig)), int(depth)))
 
     def RebuildTopologyCPU(self, pts, orig=(0, 0, 0)):
         pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)

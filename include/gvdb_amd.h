@@ -85,8 +85,13 @@ int gvdb_upload_points(gvdb_handle h, const float* host_xyz, int n);
 
 /* RequestFullRebuild: gvdb_volume_gvdb.h:432 */
 int gvdb_request_full_rebuild(gvdb_handle h, int on);
-/* RebuildTopology(num_pnts, radius, orig): gvdb_volume_gvdb.cpp:1181-1218 */
+/* RebuildTopology(num_pnts, radius, orig): gvdb_volume_gvdb.cpp:1181-1218.  A full rebuild after Configure or
+ * RequestFullRebuild, or when the depth changes (the second call after Configure is one); otherwise the incremental
+ * build of ActivateIncreBricksGPU (:1020-1156) without accumulation. */
 int gvdb_rebuild_topology(gvdb_handle h, int n, float radius, const float orig[3]);
+/* AccumulateTopology(num_pnts, radius, orig, iDepth): gvdb_volume_gvdb.cpp:1158-1178, the incremental build keeping
+ * the existing leaves; GVDB_ERR_DEPTH when the points need a deeper tree than the current one. */
+int gvdb_accumulate_topology(gvdb_handle h, int n, float radius, const float orig[3], int min_depth);
 /* RebuildTopologyCPU(num_pnts, orig, pos): gvdb_volume_gvdb.cpp:1220-1227 (host build) */
 int gvdb_rebuild_topology_cpu(gvdb_handle h, int n, const float orig[3], const float* host_xyz);
 /* ActivateSpace(pos): gvdb_volume_gvdb.cpp:2766-2774 (host build) */

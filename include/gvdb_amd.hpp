@@ -86,6 +86,10 @@ public:
         const float o[3] = {orig.x, orig.y, orig.z};
         check(gvdb_rebuild_topology(h_, num_pnts, radius, o), "RebuildTopology");
     }
+    void AccumulateTopology(int num_pnts, float radius, Vector3DF orig, int iDepth = 1) {
+        const float o[3] = {orig.x, orig.y, orig.z};
+        check(gvdb_accumulate_topology(h_, num_pnts, radius, o, iDepth), "AccumulateTopology");
+    }
     void RebuildTopologyCPU(int num_pnts, Vector3DF orig, const float* host_xyz) {
         const float o[3] = {orig.x, orig.y, orig.z};
         check(gvdb_rebuild_topology_cpu(h_, num_pnts, o, host_xyz), "RebuildTopologyCPU");

@@ -426,8 +426,16 @@ static int determineDepth(Vol& v, int rootPos[3]) {
     return -1;
 }
 
-// RebuildTopology (full-rebuild branch): src/gvdb_volume_gvdb.cpp:1181-1218,
-// ActivateBricksGPU :1229-1413, ActivateExtraBricksGPU :931-1016.
+static void activateIncre(Vol& v, const float* pts, int n, float radius, const float orig[3], int D, bool accum);
+
+// RebuildTopology: src/gvdb_volume_gvdb.cpp:1181-1218.  A full rebuild
+// (ActivateBricksGPU :1229-1413, ActivateExtraBricksGPU :931-1016) after
+// RequestFullRebuild or Configure, or when the depth differs from mCurrDepth
+// (which only that branch sets, so the second call after Configure rebuilds
+// fully too); otherwise the incremental ActivateIncreBricksGPU without
+// accumulation.  An incremental call whose depth is not the root's level
+// (a full rebuild at another depth since mCurrDepth was set) rebuilds fully
+// here; the reference would walk a tree of another depth.
 // Returns 0 on success, a negative code on the reference's error exits.
 static int rebuildTopology(Vol& v, const float* pts, int n, float radius, const float orig[3]) {
     boundingBox(v, pts, n, orig);
@@ -435,9 +443,18 @@ static int rebuildTopology(Vol& v, const float* pts, int n, float radius, const 
     int depth = determineDepth(v, rootPos);
     if (depth <= 0) return -4;                  // :1187-1190 (and "not found")
     if (depth > 5 || depth >= v.levels) return -4;  // :1191-1194, and no pool above the top level
-    Clear(v);
-    v.curr_depth = depth;
     v.undefined_events = 0;
+    if (v.rebuild_topo) {
+        Clear(v);
+    } else if (depth != v.curr_depth || v.root == ID_UNDEFL || ElemLev(v.root) != depth) {
+        Clear(v);
+        v.rebuild_topo = true;
+        v.curr_depth = depth;
+    }
+    if (!v.rebuild_topo) {
+        activateIncre(v, pts, n, radius, orig, depth, false);
+        return 0;
+    }
 
     // gvdbCalcBrickId: kernels/cuda_gvdb_particles.cuh:352-382
     std::vector<u64> keys((size_t)n * depth);
@@ -500,6 +517,113 @@ static int rebuildTopology(Vol& v, const float* pts, int n, float radius, const 
 }
 
 // ----------------------------------------------------------- host topology
+// ActivateIncreBricksGPU: src/gvdb_volume_gvdb.cpp:1020-1156.  Leaf markers
+// (the leaves' flags when accumulating, else 0); every point w = p + orig >= 0
+// looks up, at each level below D, the nodes covering [int(w) - r, int(w) + r]
+// (gvdbCalcIncreExtraBrickId, kernels/cuda_gvdb_particles.cuh:420-465):
+// missing ones become keys, found leaves get marker 1.  With no key nothing
+// changes (the markers are not applied either).  Otherwise the leaves take
+// their markers, the unique keys become nodes appended in key order, every
+// level is linked top-down (gvdbLinkBricks), unflagged leaves are delinked
+// (gvdbDelinkLeafBricks, kernels/cuda_gvdb_nodes.cuh:401-429) and then each
+// level 1..D-1 bottom-up (gvdbDelinkBricks, :431-467), and the leaf pool's
+// used count drops by the unflagged leaves.  Node positions below zero
+// (radius 8 or more) would go through the reference's signed-to-u16 key
+// cast; they are counted as undefined and skipped.
+static void activateIncre(Vol& v, const float* pts, int n, float radius, const float orig[3], int D, bool accum) {
+    const u64 total = v.pool[0][0].lastEle;
+    if (total == 0) return;
+    std::vector<int> marker(total, 0);
+    if (accum)
+        for (u64 i = 0; i < total; i++) marker[i] = nodeAt(v, 0, i)->mFlags;
+    std::vector<u64> ex;
+    for (int i = 0; i < n; i++) {
+        float w[3] = {pts[3 * i] + orig[0], pts[3 * i + 1] + orig[1], pts[3 * i + 2] + orig[2]};
+        if (w[0] < 0 || w[1] < 0 || w[2] < 0) continue;
+        int bmin[3], bmax[3];
+        for (int a = 0; a < 3; a++) {
+            bmin[a] = int(float(int(w[a])) - radius);
+            bmax[a] = int(float(int(w[a])) + radius);
+        }
+        for (int lev = 0; lev < D; lev++) {
+            int rr = int(v.range[lev]);
+            int nd[3];
+            for (nd[0] = bmin[0] / rr * rr; nd[0] <= bmax[0] / rr * rr; nd[0] += rr)
+                for (nd[1] = bmin[1] / rr * rr; nd[1] <= bmax[1] / rr * rr; nd[1] += rr)
+                    for (nd[2] = bmin[2] / rr * rr; nd[2] <= bmax[2] / rr * rr; nd[2] += rr) {
+                        if (nd[0] < 0 || nd[1] < 0 || nd[2] < 0) { v.undefined_events++; continue; }
+                        i64 id = getPosParentD(v, nd, lev);
+                        if (id < 0) ex.push_back(makeKey(lev, nd[0] / rr, nd[1] / rr, nd[2] / rr));
+                        else if (lev == 0) marker[size_t(id)] = 1;
+                    }
+        }
+    }
+    if (ex.empty()) return;
+    std::vector<u64> uniq;
+    std::vector<int> levCnt;
+    findUnique(ex, D, uniq, levCnt);
+    for (u64 i = 0; i < total; i++) nodeAt(v, 0, i)->mFlags = uint8_t(marker[i]);
+    createNodes(v, D, uniq, levCnt);
+    linkLevels(v, D);
+    // gvdbDelinkLeafBricks
+    auto clearSlot = [&](Node* node, int plev) {
+        i64 pid = getParentD(v, plev, node->mPos);
+        if (pid < 0) { v.undefined_events++; return; }
+        Node* pn = nodeAt(v, plev, u64(pid));
+        int res = v.res[plev];
+        i64 rg = v.range[plev];
+        int pin[3];
+        for (int a = 0; a < 3; a++) pin[a] = int(((i64)(node->mPos[a] - pn->mPos[a]) * res) / rg);
+        if (!inRes(v, pin, res)) return;
+        node->mParent = ID_UNDEF64;
+        u64* cl = PoolData64(v, pn->mChildList);
+        cl[(pin[2] * res + pin[1]) * res + pin[0]] = ID_UNDEF64;
+    };
+    for (u64 i = 0; i < v.pool[0][0].lastEle; i++) {
+        Node* node = nodeAt(v, 0, i);
+        if (node->mFlags || node->mParent == ID_UNDEF64) continue;
+        clearSlot(node, 1);
+    }
+    // gvdbDelinkBricks, levels 1 .. D-1
+    for (int lev = 1; lev < D; lev++) {
+        u64 cnt = v.pool[0][lev].lastEle;
+        for (u64 i = 0; i < cnt; i++) {
+            Node* node = nodeAt(v, lev, i);
+            if (node->mParent == ID_UNDEF64) { node->mFlags = 0; continue; }
+            u64* cl = reinterpret_cast<u64*>(&v.pool[1][lev].data[i * v.pool[1][lev].stride]);
+            bool any = false;
+            for (u64 c = 0; c < voxcnt(v, lev) && !any; c++) any = cl[c] != ID_UNDEF64;
+            if (any) { node->mFlags = 1; continue; }
+            node->mFlags = 0;
+            clearSlot(node, lev + 1);
+        }
+    }
+    // UpdateUsed: :1145-1149
+    u64 used = v.pool[0][0].lastEle;
+    for (u64 i = 0; i < v.pool[0][0].lastEle; i++)
+        if (!nodeAt(v, 0, i)->mFlags) used--;
+    v.pool[0][0].usedNum = used;
+    computeTopLev(v);
+}
+
+// AccumulateTopology: src/gvdb_volume_gvdb.cpp:1158-1178 (depth at least
+// iDepth; RebuildTopology while a full rebuild is pending; else the
+// incremental path with accumulation).  A depth above the root's level
+// returns -6: the reference would walk past its root.
+static int accumulateTopology(Vol& v, const float* pts, int n, float radius, const float orig[3], int iDepth) {
+    boundingBox(v, pts, n, orig);
+    int rootPos[3];
+    int depth = determineDepth(v, rootPos);
+    if (depth < iDepth) depth = iDepth;
+    if (depth == 0) return -4;
+    if (depth > 5 || depth >= v.levels) return -4;
+    if (v.rebuild_topo) return rebuildTopology(v, pts, n, radius, orig);
+    if (v.root == ID_UNDEFL || depth > ElemLev(v.root)) return -6;
+    v.undefined_events = 0;
+    activateIncre(v, pts, n, radius, orig, depth, true);
+    return 0;
+}
+
 static bool getPosInNode(Vol& v, u64 cur, const int pos[3], uint32_t& bit) {
     // src/gvdb_volume_gvdb.cpp:2858-2875
     Node* c = getNode(v, cur);
@@ -864,6 +988,10 @@ static void gatherT(Vol& v, int scs, float radius, bool accum, std::vector<T>& a
 // -------------------------------------------------------- UpdateApron ----
 // src/gvdb_volume_gvdb.cpp:4418-4457; kernels/cuda_gvdb_operators.cuh:72-131
 // with getAtlasToWorld (kernels/cuda_gvdb_nodes.cuh:145-153) and getNodeAtPoint.
+// Unflagged leaves (left by the incremental build) are skipped: the reference
+// runs them too, through an atlas position that is either stale, and then
+// rewrites the value its current owner writes, or (-1, -1, -1), which it turns
+// into an out-of-range atlas-map read.
 template <typename T>
 static void apronT(Vol& v, T boundval, std::vector<T>& atlas) {
     int bricks = int(v.pool[0][0].lastEle);
@@ -871,6 +999,7 @@ static void apronT(Vol& v, T boundval, std::vector<T>& atlas) {
 #pragma omp parallel for schedule(dynamic, 16)
     for (int b = 0; b < bricks; b++) {
         Node* node = nodeAt(v, 0, b);
+        if (!node->mFlags) continue;
         for (int side = 0; side < 6; side++)
             for (int sy = 0; sy < pbr; sy++)
                 for (int sx = 0; sx < pbr; sx++) {
@@ -1351,6 +1480,16 @@ int orc_rebuild_topology(void* h, const float* pts, int n, float radius, const f
     v.stage_ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
 }
+
+int orc_accumulate_topology(void* h, const float* pts, int n, float radius, const float* orig, int iDepth) {
+    Vol& v = *static_cast<Vol*>(h);
+    return accumulateTopology(v, pts, n, radius, orig, iDepth);
+}
+void orc_request_full_rebuild(void* h, int on) {
+    // RequestFullRebuild: src/gvdb_volume_gvdb.h:432
+    static_cast<Vol*>(h)->rebuild_topo = on != 0;
+}
+unsigned long long orc_pool_used(void* h, int grp, int lev) { return static_cast<Vol*>(h)->pool[grp][lev].usedNum; }
 
 int orc_rebuild_topology_cpu(void* h, const float* pts, int n, const float* orig) {
     // RebuildTopologyCPU: src/gvdb_volume_gvdb.cpp:1220-1227 (ActivateSpace per point, Vector3DI truncation)

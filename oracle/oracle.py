@@ -40,6 +40,8 @@ def lib():
             "orc_set_channel_default": (None, [vp, i, i, i]), "orc_add_channel": (None, [vp, i]),
             "orc_set_epsilon": (None, [vp, f]), "orc_undefined_events": (i, [vp]),
             "orc_rebuild_topology": (i, [vp, fp, i, f, fp]), "orc_rebuild_topology_cpu": (i, [vp, fp, i, fp]),
+            "orc_accumulate_topology": (i, [vp, fp, i, f, fp, i]), "orc_request_full_rebuild": (None, [vp, i]),
+            "orc_pool_used": (ctypes.c_ulonglong, [vp, i, i]),
             "orc_clear": (None, [vp]), "orc_finish_topology": (None, [vp]), "orc_update_atlas": (i, [vp]),
             "orc_clear_channel": (None, [vp]), "orc_fill_channel": (None, [vp, f]),
             "orc_pool_count": (ctypes.c_ulonglong, [vp, i, i]), "orc_pool_stride": (ctypes.c_ulonglong, [vp, i, i]),
@@ -95,6 +97,17 @@ class Oracle:
     def rebuild_topology(self, pts, radius, orig=(0, 0, 0)):
         pts = np.ascontiguousarray(pts, dtype=np.float32)
         return lib().orc_rebuild_topology(self.h, _p(pts), len(pts), float(radius), _p(f3(orig)))
+
+    def accumulate_topology(self, pts, radius, orig=(0, 0, 0), depth=1):
+        """AccumulateTopology: the incremental build keeping existing leaves."""
+        pts = np.ascontiguousarray(pts, dtype=np.float32)
+        return lib().orc_accumulate_topology(self.h, _p(pts), len(pts), float(radius), _p(f3(orig)), int(depth))
+
+    def request_full_rebuild(self, on=True):
+        lib().orc_request_full_rebuild(self.h, int(on))
+
+    def pool_used(self, grp, lev):
+        return int(lib().orc_pool_used(self.h, grp, lev))
 
     def rebuild_topology_cpu(self, pts, orig=(0, 0, 0)):
         pts = np.ascontiguousarray(pts, dtype=np.float32)

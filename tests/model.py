@@ -226,3 +226,165 @@ def dequantize16(u, mn, rg):
     """u / 65535.0f * rg + mn, the quotient rounded to float32 and the
     product and sum fused (one rounding)."""
     return np.float32(fma32(np.float32(np.float32(u) / np.float32(65535)), rg, mn))
+
+
+def incremental_model(pools, clists, pts, radius, D, accum, log2dims=(3, 3, 3, 4, 5)):
+    """Expected pools after ActivateIncreBricksGPU (src/gvdb_volume_gvdb.cpp:
+    1020-1156) from the pools before it, written with position dictionaries:
+
+      found(lev, pos)  = a node at `lev` with corner `pos` is reachable from the
+                         top node (the lowest level holding one node) through
+                         child slots;
+      keys             = every node corner of the per-axis box
+                         [int(w) - r, int(w) + r] at each level below D that is
+                         not found; found leaves of the boxes get marker 1;
+      no key           -> nothing changes;
+      otherwise        leaves take their markers (else 0, or their flags when
+                       accumulating), new nodes are appended per level in
+                       ascending (x, y, z) with flag 1 and empty child lists;
+                       top-down, every flagged node of levels D-1..0 is put in
+                       the slot of the reachable node one level up containing
+                       it; unflagged leaves and then, per level 1..D-1, nodes
+                       without children leave their parent's slot (parentless
+                       nodes of those levels get flag 0; nodes with a child
+                       get flag 1); the leaf used count is the flagged leaves.
+
+    pools[lev]: list of dicts (pos, flags, parent, clist); clists[lev]: list of
+    lists.  Returns (pools, clists, used0) as new objects."""
+    import copy
+    res, rng = tree_ranges(log2dims)
+    P = copy.deepcopy(pools)
+    C = copy.deepcopy(clists)
+    nlev = len(P)
+
+    def top_level():
+        t = 1
+        for n in range(nlev - 1, -1, -1):
+            if len(P[n]) == 1:
+                t = n
+        return t
+
+    def reachable():
+        t = top_level()
+        out = {t: {0}}
+        for lev in range(t, 0, -1):
+            out[lev - 1] = set()
+            for i in out[lev]:
+                cl = P[lev][i]["clist"]
+                if cl == ID_UNDEFL:
+                    continue
+                for v in C[lev][cl >> 16]:
+                    if v != ID_UNDEF64:
+                        out[lev - 1].add(v >> 16)
+        return out
+
+    def lookup(reach, lev, pos):
+        # getPosParent answers only below the top node's level
+        if lev >= top_level():
+            return None
+        for i in reach.get(lev, ()):
+            if tuple(P[lev][i]["pos"]) == tuple(pos):
+                return i
+        return None
+
+    def containing(reach, lev, pos):
+        """getParent's answer: the top node itself at the top level, else the
+        reachable node at `lev` whose box holds pos (None if none)."""
+        t = top_level()
+        if lev == t:
+            return 0
+        if lev > t:
+            return None
+        for i in reach.get(lev, ()):
+            c = P[lev][i]["pos"]
+            if all(c[a] <= pos[a] < c[a] + rng[lev] for a in range(3)):
+                return i
+        return None
+
+    reach = reachable()
+    nleaf = len(P[0])
+    if nleaf == 0:
+        return P, C, None
+    marker = [P[0][i]["flags"] if accum else 0 for i in range(nleaf)]
+    keys = set()
+    for p in np.asarray(pts, np.float32):
+        if np.any(p < 0):
+            continue
+        iw = [int(p[a]) for a in range(3)]
+        lo = [int(np.float32(np.float32(iw[a]) - np.float32(radius))) for a in range(3)]
+        hi = [int(np.float32(np.float32(iw[a]) + np.float32(radius))) for a in range(3)]
+        for lev in range(D):
+            rr = rng[lev]
+            ax = [range(int(ctrunc_div(lo[a], rr)) * rr, int(ctrunc_div(hi[a], rr)) * rr + 1, rr) for a in range(3)]
+            for x in ax[0]:
+                for y in ax[1]:
+                    for z in ax[2]:
+                        i = lookup(reach, lev, (x, y, z))
+                        if i is None:
+                            keys.add((lev, x // rr, y // rr, z // rr))
+                        elif lev == 0:
+                            marker[i] = 1
+    if not keys:
+        return P, C, None
+    for i in range(nleaf):
+        P[0][i]["flags"] = marker[i]
+    for (lev, x, y, z) in sorted(keys):
+        rr = rng[lev]
+        nd = dict(pos=(x * rr, y * rr, z * rr), flags=1, parent=ID_UNDEFL, clist=ID_UNDEFL, value=(-1, -1, -1))
+        if lev > 0:
+            nd["clist"] = elem(1, lev, len(C[lev]))
+            C[lev].append([ID_UNDEF64] * res[lev] ** 3)
+        P[lev].append(nd)
+
+    def slot(lev_p, pnode, pos):
+        c = [(pos[a] - pnode["pos"][a]) * res[lev_p] // rng[lev_p] for a in range(3)]
+        return c, (c[2] * res[lev_p] + c[1]) * res[lev_p] + c[0]
+
+    for lev in range(D - 1, -1, -1):
+        reach = reachable()
+        for i, nd in enumerate(P[lev]):
+            if not nd["flags"]:
+                continue
+            pid = containing(reach, lev + 1, nd["pos"])
+            if pid is None:
+                continue
+            c, b = slot(lev + 1, P[lev + 1][pid], nd["pos"])
+            if any(v < 0 or v >= res[lev + 1] for v in c):
+                continue
+            nd["parent"] = elem(0, lev + 1, pid)
+            C[lev + 1][P[lev + 1][pid]["clist"] >> 16][b] = elem(0, lev, i)
+
+    def delink(lev, nd):
+        pid = containing(reachable(), lev + 1, nd["pos"])
+        _, b = slot(lev + 1, P[lev + 1][pid], nd["pos"])
+        nd["parent"] = ID_UNDEF64
+        C[lev + 1][P[lev + 1][pid]["clist"] >> 16][b] = ID_UNDEF64
+
+    for nd in P[0]:
+        if not nd["flags"] and nd["parent"] != ID_UNDEF64:
+            delink(0, nd)
+    for lev in range(1, D):
+        for i, nd in enumerate(P[lev]):
+            if nd["parent"] == ID_UNDEF64:
+                nd["flags"] = 0
+                continue
+            if any(v != ID_UNDEF64 for v in C[lev][i]):
+                nd["flags"] = 1
+                continue
+            nd["flags"] = 0
+            delink(lev, nd)
+    used0 = sum(1 for nd in P[0] if nd["flags"])
+    return P, C, used0
+
+
+def pools_as_dicts(o, nlev=5):
+    """Node pools and child lists of an Oracle (or a GPU volume) as the
+    lists incremental_model takes."""
+    P, C = [], []
+    for lev in range(nlev):
+        a = o.pool(0, lev)
+        P.append([dict(pos=tuple(int(v) for v in n["mPos"]), flags=int(n["mFlags"]), parent=int(n["mParent"]),
+                       clist=int(n["mChildList"]), value=tuple(int(v) for v in n["mValue"])) for n in a])
+        cl = o.pool(1, lev) if lev > 0 else np.zeros((0, 0), np.uint64)
+        C.append([[int(v) for v in row] for row in cl])
+    return P, C

@@ -564,3 +564,94 @@ def test_fp16_binning_rule_differs_from_the_float_one_only_under_translation():
     rc, n32 = o32.insert_points_subcell(q, 2.0, 4, t)
     assert rc == 0 and np.array_equal(o32.subcells(n32)["cnt"], sm32["cnt"])
     assert n16 > n32
+
+
+# ------------------------------------------------------ incremental topology
+def _tree_consistent(o, nlev=5):
+    """Every filled child slot names a node whose parent names the slot's
+    owner, and the child's corner lies in the slot's cell."""
+    res, rng = M.tree_ranges((3, 3, 3, 4, 5))
+    for lev in range(1, nlev):
+        nodes = o.pool(0, lev)
+        if not len(nodes):
+            continue
+        cl = o.pool(1, lev)
+        kids = o.pool(0, lev - 1)
+        for i, nd in enumerate(nodes):
+            lst = cl[int(nd["mChildList"]) >> 16]
+            for b in np.nonzero(lst != M.ID_UNDEF64)[0]:
+                ci = int(lst[b]) >> 16
+                assert int(kids[ci]["mParent"]) == M.elem(0, lev, i)
+                c = (kids[ci]["mPos"] - nd["mPos"]) * res[lev] // rng[lev]
+                assert (int(c[2]) * res[lev] + int(c[1])) * res[lev] + int(c[0]) == b
+
+
+@pytest.mark.parametrize("shift,accum", [((15.0, 0.0, 0.0), False), ((70.0, 0.0, 0.0), False),
+                                         ((15.0, 0.0, 0.0), True), ((70.0, -40.0, 0.0), True)])
+def test_incremental_topology_matches_model(shift, accum):
+    """RebuildTopology on points A, then the incremental build on points B
+    (the third RebuildTopology call after Configure, or AccumulateTopology):
+    pools, child lists and the leaf used count equal the dictionary model."""
+    A = _ball(400, 51)
+    B = _ball(400, 52, centre=(200.0 + shift[0], 180.0 + shift[1], 220.0 + shift[2]))
+    o = O.Oracle()
+    assert o.rebuild_topology(A, 2.0) == 0
+    if not accum:
+        assert o.rebuild_topology(A, 2.0) == 0      # depth change from -1: full again, sets mCurrDepth
+    P, C = M.pools_as_dicts(o)
+    rc = o.accumulate_topology(B, 2.0) if accum else o.rebuild_topology(B, 2.0)
+    assert rc == 0 and o.undefined_events() == 0
+    P2, C2, used = M.incremental_model(P, C, B, 2.0, 2, accum)
+    Q, D = M.pools_as_dicts(o)
+    assert used is not None and len(P2[0]) > len(P[0])
+    assert Q == P2 and D == C2
+    assert o.pool_used(0, 0) == used
+    _tree_consistent(o)
+    leaves = o.pool(0, 0)
+    flagged = {tuple(p) for p in leaves["mPos"][leaves["mFlags"] == 1].tolist()}
+    assert flagged
+    if not accum:
+        # every flagged leaf is a brick of some point's box [int(w) - r, int(w) + r]
+        want = set()
+        for p in B:
+            lo = [int(np.float32(int(p[a]) - np.float32(2.0))) // 8 * 8 for a in range(3)]
+            hi = [int(np.float32(int(p[a]) + np.float32(2.0))) // 8 * 8 for a in range(3)]
+            for x in range(lo[0], hi[0] + 1, 8):
+                for y in range(lo[1], hi[1] + 1, 8):
+                    for z in range(lo[2], hi[2] + 1, 8):
+                        want.add((x, y, z))
+        assert flagged == want
+
+
+def _full(pts, r):
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, r) == 0
+    return o
+
+
+def test_incremental_topology_without_new_nodes_changes_nothing():
+    """Points inside the existing leaves only: the reference returns before
+    applying the markers, so even unused leaves stay flagged."""
+    A = _ball(400, 53)
+    o = O.Oracle()
+    assert o.rebuild_topology(A, 2.0) == 0
+    assert o.rebuild_topology(A, 2.0) == 0
+    before = [o.pool(0, l).copy() for l in range(3)]
+    assert o.rebuild_topology(A[:50], 2.0) == 0
+    after = [o.pool(0, l) for l in range(3)]
+    assert all(np.array_equal(a, b) for a, b in zip(before, after))
+
+
+def test_second_rebuild_after_configure_is_full():
+    """mCurrDepth starts at -1 and only the depth-change branch sets it: the
+    second RebuildTopology is a full rebuild, the third incremental."""
+    A = _ball(400, 54)
+    B = _ball(400, 55, centre=(215.0, 180.0, 220.0))
+    o = O.Oracle()
+    assert o.rebuild_topology(A, 2.0) == 0
+    assert o.rebuild_topology(B, 2.0) == 0
+    f = _full(B, 2.0)
+    assert all(np.array_equal(o.pool(0, l), f.pool(0, l)) for l in range(3))
+    o.request_full_rebuild()
+    assert o.rebuild_topology(A, 2.0) == 0
+    assert all(np.array_equal(o.pool(0, l), _full(A, 2.0).pool(0, l)) for l in range(3))
