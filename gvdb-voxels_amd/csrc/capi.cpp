@@ -108,6 +108,17 @@ int gvdb_gather_density(gvdb_handle h, int subcell_size, int n, float radius, co
     const float z[3] = {0, 0, 0};
     return h->v.gather_density(subcell_size, n, radius, trans ? trans : z, sc_len, chan_density, chan_clr, accumulate != 0);
 }
+int gvdb_save_vbx(gvdb_handle h, const char* path) { H_OR_FAIL(h); return h->v.save_vbx(path); }
+int gvdb_load_vbx(gvdb_handle h, const char* path, int force_major, int force_minor) {
+    H_OR_FAIL(h);
+    return h->v.load_vbx(path, force_major, force_minor);
+}
+int gvdb_get_transform(gvdb_handle h, float out12[12]) {
+    H_OR_FAIL(h);
+    if (!out12) return GVDB_ERR_INVALID;
+    h->v.get_transform(out12);
+    return GVDB_OK;
+}
 int gvdb_insert_points_subcell_fp16(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3],
                                     int* out_len) {
     H_OR_FAIL(h);

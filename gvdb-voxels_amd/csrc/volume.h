@@ -93,6 +93,12 @@ public:
     int request_full_rebuild(bool on) { rebuild_topo_ = on; return 0; }
     int rebuild_topology(int n, float radius, const float orig[3]);
     int accumulate_topology(int n, float radius, const float orig[3], int min_depth);
+    // VBX files (vbx.cpp)
+    int save_vbx(const char* path);
+    int load_vbx(const char* path, int force_major, int force_minor);
+    int write_atlas(int chan, const void* src);
+    void get_transform(float out12[12]) const;
+    bool identity_transform() const;
     int rebuild_topology_cpu(int n, const float orig[3], const float* host);
     int activate_space(const float pos[3]);
     int clear();
@@ -182,6 +188,8 @@ private:
     unsigned atlas_gen_ = ~0u; // topology generation UpdateAtlas last ran on
     unsigned sc_gen_ = ~0u;    // topology generation InsertPointsSubcell last ran on
     float posMin_[3] = {0, 0, 0}, posMax_[3] = {0, 0, 0};
+    // grid transform kept for VBX files (SetTransform's defaults)
+    float xf_pretrans_[3] = {0, 0, 0}, xf_angs_[3] = {0, 0, 0}, xf_scale_[3] = {1, 1, 1}, xf_trans_[3] = {0, 0, 0};
     float objMin_[3] = {0, 0, 0}, objMax_[3] = {0, 0, 0};
     float epsilon_ = 0.001f;
     int maxiter_ = 256;

@@ -37,6 +37,8 @@ _SIGS = [
     ("gvdb_request_full_rebuild", _i, [_vp, _i]), ("gvdb_rebuild_topology", _i, [_vp, _i, _f, _vp]),
     ("gvdb_rebuild_topology_cpu", _i, [_vp, _i, _vp, _vp]), ("gvdb_activate_space", _i, [_vp, _vp]),
     ("gvdb_accumulate_topology", _i, [_vp, _i, _f, _vp, _i]),
+    ("gvdb_save_vbx", _i, [_vp, ctypes.c_char_p]), ("gvdb_load_vbx", _i, [_vp, ctypes.c_char_p, _i, _i]),
+    ("gvdb_get_transform", _i, [_vp, _vp]),
     ("gvdb_clear", _i, [_vp]), ("gvdb_finish_topology", _i, [_vp, _i, _i]), ("gvdb_update_atlas", _i, [_vp]),
     ("gvdb_clear_channel", _i, [_vp, _i]),
     ("gvdb_insert_points_subcell", _i, [_vp, _i, _i, _f, _vp, ctypes.POINTER(_i)]),
@@ -210,6 +212,18 @@ class VolumeGVDB:
     def GatherLevelSet(self, subcell_size, n, radius, trans, sc_len, chan_levelset=0, chan_clr=0, accumulate=False):
         return self._chk(load().gvdb_gather_level_set(self.h, subcell_size, int(n), float(radius), _p(_f3(trans)),
                                                       int(sc_len), chan_levelset, chan_clr, int(accumulate)))
+
+    def SaveVBX(self, path):
+        return self._chk(load().gvdb_save_vbx(self.h, os.fsencode(path)))
+
+    def LoadVBX(self, path, force_major=0, force_minor=0):
+        return self._chk(load().gvdb_load_vbx(self.h, os.fsencode(path), int(force_major), int(force_minor)))
+
+    def transform(self):
+        """(pretrans, angs, scale, trans) of the grid, as a VBX file stores them."""
+        out = np.zeros(12, np.float32)
+        self._chk(load().gvdb_get_transform(self.h, _p(out)))
+        return out.reshape(4, 3)
 
     def InsertPointsSubcell_FP16(self, subcell_size, n, radius, trans=(0, 0, 0)):
         ln = ctypes.c_int(0)

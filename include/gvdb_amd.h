@@ -40,6 +40,7 @@ extern "C" {
 #define GVDB_ERR_RANGE (-6)       /* coordinates outside what the topology build supports */
 #define GVDB_ERR_UNSUPPORTED (-7) /* feature of the reference not provided */
 #define GVDB_ERR_OOM (-8)         /* device allocation failed */
+#define GVDB_ERR_IO (-9)          /* file missing, unreadable or malformed (VBX) */
 
 /* channel types (src/gvdb_types.h of the reference); T_HALF is an addition */
 #define GVDB_T_FLOAT 3
@@ -126,6 +127,16 @@ int gvdb_insert_points_subcell_fp16(gvdb_handle h, int subcell_size, int n, floa
  * subcell's entries q = u / 65535 * pos_range + pos_min; no accumulation */
 int gvdb_gather_level_set_fp16(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len,
                                int chan_levelset, int chan_clr);
+/* SaveVBX(fname): gvdb_volume_gvdb.cpp:1626-1768.  VBX 1.11 with one grid: header, GVDB topology pools, each
+ * channel's atlas in the reference's 3D layout (GVDB_FILESPEC.txt). */
+int gvdb_save_vbx(gvdb_handle h, const char* path);
+/* LoadVBX(fname, force_maj, force_min): gvdb_volume_gvdb.cpp:507-685.  Configure from the file, pools, FinishTopology,
+ * channels, UpdateAtlas.  Files with bitmask nodes (1.0, or 2.x with masks) and non-float grids return
+ * GVDB_ERR_UNSUPPORTED; unreadable or malformed files GVDB_ERR_IO. */
+int gvdb_load_vbx(gvdb_handle h, const char* path, int force_major, int force_minor);
+/* The grid transform a VBX file carries: pretrans, angs, scale, trans (SetTransform's fields).  Render supports only
+ * the identity transform and returns GVDB_ERR_UNSUPPORTED otherwise. */
+int gvdb_get_transform(gvdb_handle h, float out12[12]);
 /* UpdateApron(chan, boundval): gvdb_volume_gvdb.cpp:4418-4457 */
 int gvdb_update_apron(gvdb_handle h, int chan, float boundval);
 

@@ -133,6 +133,12 @@ public:
               "GatherLevelSet_FP16");
     }
     void UpdateApron(unsigned char chan, float boundval = 0.0f) { check(gvdb_update_apron(h_, chan, boundval), "UpdateApron"); }
+    void SaveVBX(const std::string& fname) { check(gvdb_save_vbx(h_, fname.c_str()), "SaveVBX"); }
+    bool LoadVBX(const std::string& fname, int force_maj = 0, int force_min = 0) {
+        const int rc = gvdb_load_vbx(h_, fname.c_str(), force_maj, force_min);
+        check(rc, "LoadVBX");
+        return rc == GVDB_OK;
+    }
 
     /* Scene setters of Scene / Camera3D used by the point samples */
     void SetCameraOrbit(Vector3DF angs, Vector3DF to, float dist, float dolly, float fov = 40.f, float znear = 0.1f,

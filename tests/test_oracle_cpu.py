@@ -655,3 +655,34 @@ def test_second_rebuild_after_configure_is_full():
     o.request_full_rebuild()
     assert o.rebuild_topology(A, 2.0) == 0
     assert all(np.array_equal(o.pool(0, l), _full(A, 2.0).pool(0, l)) for l in range(3))
+
+
+# ------------------------------------------------------------------ VBX file
+def test_vbx_field_sizes_follow_the_specification():
+    """Byte counts of GVDB_FILESPEC.txt: header 1+1+4*12+4, grid header
+    256+1+1+1+12+4+12+4+4+8+1+4+1+12+12 (SaveVBX's order: channel count
+    before the atlas size), topology 4+8, per level 4+4+12+4+4+4+4."""
+    import vbx
+    assert vbx.HEADER.size == 54 and vbx.GRID.size == 333 and vbx.TOPO.size == 12 and vbx.LEVEL.size == 36
+
+
+def _vbx_meta(o, bricks):
+    res, rng = M.tree_ranges((3, 3, 3, 4, 5))
+    return dict(levels=5, root=M.elem(0, 2, 0), log2dims=(3, 3, 3, 4, 5), res=res, ranges=rng, apron=1, brick=8,
+                axiscnt=o.atlas_info()["cnt"], bricks=bricks)
+
+
+def test_vbx_round_trip_of_oracle_volume():
+    import vbx
+    pts = _ball(800, 71)
+    o = _gathered(pts, 2.0)
+    p0 = [o.pool(0, l).tobytes() for l in range(5)]
+    p1 = [o.pool(1, l).tobytes() if l else b"" for l in range(5)]
+    a = o.atlas(32)
+    buf = vbx.write_vbx(_vbx_meta(o, len(o.pool(0, 0))), p0, p1, [(3, a)])
+    r = vbx.read_vbx(buf)
+    assert r["major"] == 1 and r["minor"] == 11 and r["offsets"] == (62,) and r["end"] == len(buf)
+    assert r["dtype"] == "f" and r["topology"] == 2 and r["layout"] == 0 and r["channels"] == 1
+    assert r["pools0"] == p0 and r["pools1"][1:] == p1[1:] and np.array_equal(r["atlases"][0][1], a)
+    assert r["atlas_bytes"] == a.nbytes and r["axisres"] == tuple(int(x) for x in o.atlas_info()["res"])
+    assert len(buf) == 62 + 333 + 12 + 5 * 36 + sum(map(len, p0)) + sum(map(len, p1)) + 8 + a.nbytes
