@@ -108,6 +108,12 @@ int gvdb_gather_density(gvdb_handle h, int subcell_size, int n, float radius, co
     const float z[3] = {0, 0, 0};
     return h->v.gather_density(subcell_size, n, radius, trans ? trans : z, sc_len, chan_density, chan_clr, accumulate != 0);
 }
+int gvdb_compute(gvdb_handle h, int effect, int chan, int iterations, const float params[3], int update_apron,
+                 int skip_aprons, float boundval) {
+    H_OR_FAIL(h);
+    const float z[3] = {0, 0, 0};
+    return h->v.compute(effect, chan, iterations, params ? params : z, update_apron != 0, skip_aprons != 0, boundval);
+}
 int gvdb_save_vbx(gvdb_handle h, const char* path) { H_OR_FAIL(h); return h->v.save_vbx(path); }
 int gvdb_load_vbx(gvdb_handle h, const char* path, int force_major, int force_minor) {
     H_OR_FAIL(h);

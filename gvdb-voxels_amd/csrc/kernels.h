@@ -160,6 +160,20 @@ hipError_t launch_delink_level(const Tree& t, int lev, hipStream_t s);
 // *out += leaves with mFlags == 0 (out must start at zero)
 hipError_t launch_count_unflagged(const Tree& t, uint32_t* out, hipStream_t s);
 
+// ---- Compute operators (compute.hip): effect ids of the reference
+// (src/gvdb_volume_gvdb.h:201-208) for float/half channels
+enum { OP_FILL = 110, OP_SMOOTH = 113, OP_NOISE = 114, OP_GROW = 115 };
+struct ComputeArgs {
+    Tree t;               // atlas geometry (brick_res, atlas_cnt)
+    void* atlas;
+    int atlas_type;
+    int op;
+    float p1, p2, p3;
+    int slots;            // smooth/noise/grow: atlas slots 0 .. slots-1
+    int lo[3], hi[3];     // fill: inclusive atlas-voxel range
+};
+hipError_t launch_compute(const ComputeArgs& a, hipStream_t s);
+
 // ---- atlas & points (points.hip)
 struct AtlasArgs {
     Tree t;

@@ -655,6 +655,46 @@ int Volume::gather_level_set_fp16(int scs, int n, float radius, const float tran
     return multi_gather(&v, 1, scs, &len, &radius, chanD, false, true, self_exec(), nullptr, true);
 }
 
+// Compute (gvdb_volume_gvdb.cpp:4527-4552): `iterations` launches of the
+// effect, each followed by UpdateApron(chan, boundval) when asked.
+int Volume::compute(int effect, int c, int iterations, const float p[3], bool update_apron_each, bool skip_aprons,
+                    float boundval) {
+    if (c < 0 || c >= 8 || !chan_[c].exists) return fail(GVDB_ERR_INVALID, "Compute: no such channel");
+    if (effect != OP_FILL && effect != OP_SMOOTH && effect != OP_NOISE && effect != OP_GROW)
+        return fail(GVDB_ERR_UNSUPPORTED, "Compute: effect not provided (float channels: fill, smooth, noise, grow)");
+    if (effect != OP_FILL && apron_ != 1)
+        return fail(GVDB_ERR_UNSUPPORTED, "Compute: smooth, noise and grow need an apron of 1");
+    if (iterations < 0) return fail(GVDB_ERR_INVALID, "Compute: negative iteration count");
+    int rc = commit_pools();
+    if (rc) return rc;
+    ComputeArgs a;
+    memset(&a, 0, sizeof(a));
+    a.t = make_tree();
+    a.atlas = chan_[c].data.p;
+    a.atlas_type = chan_[c].type;
+    a.op = effect;
+    a.p1 = p[0]; a.p2 = p[1]; a.p3 = p[2];
+    a.slots = int(std::min<u64>(atlas_used_, atlas_max_));
+    const int br = brickwid_ + 2 * apron_;
+    for (int k = 0; k < 3; k++) {
+        // the reference's unpacked index b * 8 + t + 1 over ceil(count / 8) blocks
+        const int res = atlas_cnt_[k] * br;
+        const int count = skip_aprons ? atlas_cnt_[k] * brickwid_ : res;
+        const int blocks = (count + 7) / 8;
+        a.lo[k] = 1;
+        a.hi[k] = std::min(8 * blocks, res - 1);
+    }
+    for (int it = 0; it < iterations; it++) {
+        HIPCHK(launch_compute(a, stream));
+        atlas_epoch_++;
+        if (update_apron_each) {
+            rc = update_apron(c, boundval);
+            if (rc) return rc;
+        }
+    }
+    return GVDB_OK;
+}
+
 int Volume::update_apron(int c, float boundval) {
     Volume* v = this;
     return multi_apron(&v, 1, c, boundval, self_exec(), nullptr);

@@ -93,6 +93,8 @@ public:
     int request_full_rebuild(bool on) { rebuild_topo_ = on; return 0; }
     int rebuild_topology(int n, float radius, const float orig[3]);
     int accumulate_topology(int n, float radius, const float orig[3], int min_depth);
+    int compute(int effect, int chan, int iterations, const float params[3], bool update_apron, bool skip_aprons,
+                float boundval);
     // VBX files (vbx.cpp)
     int save_vbx(const char* path);
     int load_vbx(const char* path, int force_major, int force_minor);

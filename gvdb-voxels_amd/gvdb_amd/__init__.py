@@ -18,6 +18,7 @@ OK = 0
 ERR_INVALID, ERR_HIP, ERR_STATE, ERR_DEPTH, ERR_NOBRICKS, ERR_RANGE, ERR_UNSUPPORTED, ERR_OOM = -1, -2, -3, -4, -5, -6, -7, -8
 T_FLOAT, T_HALF = 3, 100
 SHADE_TRILINEAR, SHADE_VOLUME = 4, 7
+FUNC_FILL_F, FUNC_SMOOTH, FUNC_NOISE, FUNC_GROW = 110, 113, 114, 115
 SCN_FLOATS = 68
 
 NODE_DTYPE = np.dtype([
@@ -39,6 +40,7 @@ _SIGS = [
     ("gvdb_accumulate_topology", _i, [_vp, _i, _f, _vp, _i]),
     ("gvdb_save_vbx", _i, [_vp, ctypes.c_char_p]), ("gvdb_load_vbx", _i, [_vp, ctypes.c_char_p, _i, _i]),
     ("gvdb_get_transform", _i, [_vp, _vp]),
+    ("gvdb_compute", _i, [_vp, _i, _i, _i, _vp, _i, _i, _f]),
     ("gvdb_clear", _i, [_vp]), ("gvdb_finish_topology", _i, [_vp, _i, _i]), ("gvdb_update_atlas", _i, [_vp]),
     ("gvdb_clear_channel", _i, [_vp, _i]),
     ("gvdb_insert_points_subcell", _i, [_vp, _i, _i, _f, _vp, ctypes.POINTER(_i)]),
@@ -212,6 +214,10 @@ class VolumeGVDB:
     def GatherLevelSet(self, subcell_size, n, radius, trans, sc_len, chan_levelset=0, chan_clr=0, accumulate=False):
         return self._chk(load().gvdb_gather_level_set(self.h, subcell_size, int(n), float(radius), _p(_f3(trans)),
                                                       int(sc_len), chan_levelset, chan_clr, int(accumulate)))
+
+    def Compute(self, effect, chan, iterations, params, update_apron, skip_aprons=True, boundval=0.0):
+        return self._chk(load().gvdb_compute(self.h, int(effect), int(chan), int(iterations), _p(_f3(params)),
+                                             int(update_apron), int(skip_aprons), float(boundval)))
 
     def SaveVBX(self, path):
         return self._chk(load().gvdb_save_vbx(self.h, os.fsencode(path)))

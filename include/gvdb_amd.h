@@ -127,6 +127,18 @@ int gvdb_insert_points_subcell_fp16(gvdb_handle h, int subcell_size, int n, floa
  * subcell's entries q = u / 65535 * pos_range + pos_min; no accumulation */
 int gvdb_gather_level_set_fp16(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len,
                                int chan_levelset, int chan_clr);
+/* Compute(effect, channel, num_iterations, parameters, bUpdateApron, skipOverAprons, boundval):
+ * gvdb_volume_gvdb.cpp:4527-4552, kernels cuda_gvdb_operators.cuh:148-455.  Float and half channels:
+ * GVDB_FUNC_FILL_F (p1, or a sine pattern when p3 < 0), GVDB_FUNC_SMOOTH ((p1 v + six face neighbours) / (p1 + 6)
+ * + p2), GVDB_FUNC_NOISE (v + random(voxel) p1 where v > 0.01), GVDB_FUNC_GROW (v + 10 p1 where v != 0, then 0 below
+ * 0.01); the uchar effects return GVDB_ERR_UNSUPPORTED.  UpdateApron(channel, boundval) after every iteration when
+ * update_apron is set. */
+#define GVDB_FUNC_FILL_F 110
+#define GVDB_FUNC_SMOOTH 113
+#define GVDB_FUNC_NOISE 114
+#define GVDB_FUNC_GROW 115
+int gvdb_compute(gvdb_handle h, int effect, int chan, int iterations, const float params[3], int update_apron,
+                 int skip_aprons, float boundval);
 /* SaveVBX(fname): gvdb_volume_gvdb.cpp:1626-1768.  VBX 1.11 with one grid: header, GVDB topology pools, each
  * channel's atlas in the reference's 3D layout (GVDB_FILESPEC.txt). */
 int gvdb_save_vbx(gvdb_handle h, const char* path);

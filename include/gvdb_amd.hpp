@@ -133,6 +133,12 @@ public:
               "GatherLevelSet_FP16");
     }
     void UpdateApron(unsigned char chan, float boundval = 0.0f) { check(gvdb_update_apron(h_, chan, boundval), "UpdateApron"); }
+    void Compute(int effect, unsigned char channel, int num_iterations, Vector3DF parameters, bool bUpdateApron,
+                 bool skipOverAprons, float boundval = 0.0f) {
+        const float p[3] = {parameters.x, parameters.y, parameters.z};
+        check(gvdb_compute(h_, effect, channel, num_iterations, p, bUpdateApron ? 1 : 0, skipOverAprons ? 1 : 0,
+                           boundval), "Compute");
+    }
     void SaveVBX(const std::string& fname) { check(gvdb_save_vbx(h_, fname.c_str()), "SaveVBX"); }
     bool LoadVBX(const std::string& fname, int force_maj = 0, int force_min = 0) {
         const int rc = gvdb_load_vbx(h_, fname.c_str(), force_maj, force_min);

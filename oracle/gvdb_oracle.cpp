@@ -1437,6 +1437,109 @@ static void renderT(Vol& v, const std::vector<T>& atlas, const ScnBlock& s, cons
 }
 
 // ============================================================ C ABI ====
+// ------------------------------------------------------------- Compute ----
+// Compute (src/gvdb_volume_gvdb.cpp:4527-4552) with the float operators of
+// kernels/cuda_gvdb_operators.cuh: gvdbOpFillF (:371-384, every atlas voxel
+// of the unpacked index range), gvdbOpSmooth (:400-417), gvdbOpNoise
+// (:442-453), gvdbOpGrow (:235-253) (each over the interior of every used
+// brick, reading the brick as it was before the launch).  The hash is
+// random(float3) of kernels/cuda_gvdb.cuh:135-164 (Bob Jenkins'
+// one-at-a-time final mix, compounded, low 23 bits as a float in [0, 1)).
+static uint32_t oaat(uint32_t x) {
+    x += (x << 10u);
+    x ^= (x >> 6u);
+    x += (x << 3u);
+    x ^= (x >> 11u);
+    x += (x << 15u);
+    return x;
+}
+static float random3(float a, float b, float c) {
+    uint32_t ia, ib, ic;
+    memcpy(&ia, &a, 4); memcpy(&ib, &b, 4); memcpy(&ic, &c, 4);
+    uint32_t m = (oaat(ia ^ oaat(ib) ^ oaat(ic)) & 0x007FFFFFu) | 0x3F800000u;
+    float f;
+    memcpy(&f, &m, 4);
+    return f - 1.0f;
+}
+
+template <typename T>
+static void computeT(Vol& v, int op, float p1, float p2, float p3, bool skip, std::vector<T>& atlas) {
+    const int br = brickres(v);
+    int res[3];
+    atlasRes(v, res);
+    if (op == 110) {
+        int lo[3], hi[3];
+        for (int k = 0; k < 3; k++) {
+            int count = skip ? v.atlas_cnt[k] * v.brickwid : res[k];
+            int blocks = (count + 7) / 8;
+            lo[k] = 1;
+            hi[k] = std::min(8 * blocks, res[k] - 1);
+        }
+        for (int z = lo[2]; z <= hi[2]; z++)
+            for (int y = lo[1]; y <= hi[1]; y++)
+                for (int x = lo[0]; x <= hi[0]; x++) {
+                    T val = T(p1);
+                    if (p3 < 0) {
+                        double ax = double(uint32_t(x) * 12u) / (3.141592 * 30.0);
+                        double ay = double(uint32_t(y) * 12u) / (3.141592 * 30.0);
+                        double az = double(uint32_t(z) * 12u) / (3.141592 * 30.0);
+                        if (sizeof(T) == 4) {
+                            float f = sinf(float(ax));
+                            f += sinf(float(ay));
+                            f += sinf(float(az));
+                            val = T(f);
+                        } else {
+                            val = T(std::sin(double(float(ax))) + std::sin(double(float(ay))) + std::sin(double(float(az))));
+                        }
+                    }
+                    atlas[aidx(v, x, y, z)] = val;
+                }
+        return;
+    }
+    const u64 slots = std::min<u64>(v.atlas_used, u64(v.atlas_cnt[0]) * v.atlas_cnt[1] * v.atlas_cnt[2]);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (i64 s = 0; s < i64(slots); s++) {
+        const int sx = int(s % v.atlas_cnt[0]), sy = int((s / v.atlas_cnt[0]) % v.atlas_cnt[1]);
+        const int sz = int(s / (i64(v.atlas_cnt[0]) * v.atlas_cnt[1]));
+        const int ox = sx * br, oy = sy * br, oz = sz * br;
+        std::vector<T> b(size_t(br) * br * br);
+        for (int z = 0; z < br; z++)
+            for (int y = 0; y < br; y++)
+                for (int x = 0; x < br; x++) b[(size_t(z) * br + y) * br + x] = atlas[aidx(v, ox + x, oy + y, oz + z)];
+        auto at = [&](int x, int y, int z) { return b[(size_t(z) * br + y) * br + x]; };
+        for (int z = 1; z < br - 1; z++)
+            for (int y = 1; y < br - 1; y++)
+                for (int x = 1; x < br - 1; x++) {
+                    const T c = at(x, y, z);
+                    T val = c;
+                    if (op == 113) {
+                        if (sizeof(T) == 4) {
+                            float f = p1 * float(c);
+                            f += float(at(x - 1, y, z));
+                            f += float(at(x + 1, y, z));
+                            f += float(at(x, y - 1, z));
+                            f += float(at(x, y + 1, z));
+                            f += float(at(x, y, z - 1));
+                            f += float(at(x, y, z + 1));
+                            val = T(float(double(f) / (double(p1) + 6.0) + double(p2)));
+                        } else {
+                            double d = double(p1) * double(c) + double(at(x - 1, y, z)) + double(at(x + 1, y, z)) +
+                                       double(at(x, y - 1, z)) + double(at(x, y + 1, z)) + double(at(x, y, z - 1)) +
+                                       double(at(x, y, z + 1));
+                            val = T(d / (double(p1) + 6.0) + double(p2));
+                        }
+                    } else if (op == 114) {
+                        if (double(c) > 0.01)
+                            val = c + T(random3(float(ox + x), float(oy + y), float(oz + z))) * T(p1);
+                    } else if (op == 115) {
+                        if (c != T(0)) val = T(double(c) + double(p1) * 10.0);
+                        if (double(val) < 0.01) val = T(0);
+                    }
+                    atlas[aidx(v, ox + x, oy + y, oz + z)] = val;
+                }
+    }
+}
+
 extern "C" {
 
 void* orc_new(int levels, const int* logdims) {
@@ -1560,6 +1663,12 @@ void orc_atlas_copy(void* h, int precision, void* dst) {
     if (precision == 64) memcpy(dst, v.atlasD.data(), v.atlasD.size() * sizeof(double));
     else memcpy(dst, v.atlasF.data(), v.atlasF.size() * sizeof(float));
 }
+void orc_atlas_set(void* h, int precision, const void* src) {
+    Vol& v = *static_cast<Vol*>(h);
+    if (precision == 64) memcpy(v.atlasD.data(), src, v.atlasD.size() * sizeof(double));
+    else memcpy(v.atlasF.data(), src, v.atlasF.size() * sizeof(float));
+}
+unsigned orc_hash(unsigned x) { return oaat(x); }
 void orc_atlas_map_copy(void* h, void* dst) {
     Vol& v = *static_cast<Vol*>(h);
     memcpy(dst, v.amap.data(), v.amap.size() * sizeof(AtlasNode));
@@ -1651,6 +1760,23 @@ int orc_update_apron(void* h, float boundval, int precision) {
     if (precision == 64) apronT<double>(v, double(boundval), v.atlasD);
     else apronT<float>(v, boundval, v.atlasF);
     v.stage_ms[4] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+float orc_random3(float a, float b, float c) { return random3(a, b, c); }
+
+int orc_compute(void* h, int op, int iters, float p1, float p2, float p3, int update_apron, int skip, float boundval,
+                int precision) {
+    Vol& v = *static_cast<Vol*>(h);
+    if (op != 110 && op != 113 && op != 114 && op != 115) return -7;
+    for (int it = 0; it < iters; it++) {
+        if (precision == 64) computeT<double>(v, op, p1, p2, p3, skip != 0, v.atlasD);
+        else computeT<float>(v, op, p1, p2, p3, skip != 0, v.atlasF);
+        if (update_apron && v.apron) {
+            if (precision == 64) apronT<double>(v, double(boundval), v.atlasD);
+            else apronT<float>(v, boundval, v.atlasF);
+        }
+    }
     return 0;
 }
 

@@ -55,6 +55,8 @@ def lib():
             "orc_insert_points_subcell_fp16": (i, [vp, i, fp, i, f, fp, ip]),
             "orc_subcell16_copy": (None, [vp, vp]),
             "orc_gather_level_set_fp16": (i, [vp, i, f, i]),
+            "orc_compute": (i, [vp, i, i, f, f, f, i, i, f, i]), "orc_random3": (f, [f, f, f]),
+            "orc_atlas_set": (None, [vp, i, vp]), "orc_hash": (ctypes.c_uint, [ctypes.c_uint]),
             "orc_camera_scn": (None, [fp, i, i, fp]), "orc_scn_floats": (i, []),
             "orc_linear_tf": (None, [f, f, fp, fp, fp]),
             "orc_render": (i, [vp, fp, fp, i, i, fp, vp]), "orc_stage_ms": (ctypes.c_double, [vp, i]),
@@ -195,6 +197,17 @@ class Oracle:
     def gather_level_set(self, radius, scs=4, precision=32, accumulate=False):
         """GatherLevelSet: min(3, min over the subcell's entries of |p - v| - r)."""
         return lib().orc_gather_level_set(self.h, scs, float(radius), precision, int(accumulate))
+
+    def compute(self, effect, iterations, params, update_apron, skip_aprons=True, boundval=0.0, precision=32):
+        """Compute(effect, ...): fill 110, smooth 113, noise 114, grow 115."""
+        p = [float(x) for x in params]
+        return lib().orc_compute(self.h, int(effect), int(iterations), p[0], p[1], p[2], int(update_apron),
+                                 int(skip_aprons), float(boundval), precision)
+
+    def set_atlas(self, a, precision=32):
+        a = np.ascontiguousarray(a, np.float64 if precision == 64 else np.float32)
+        assert a.size == np.prod(self.atlas_info()["res"])
+        lib().orc_atlas_set(self.h, precision, _p(a))
 
     def insert_points_subcell_fp16(self, pts, radius, scs=4, trans=(0, 0, 0)):
         """InsertPointsSubcell_FP16: the subcell lists under the FP16 binning

@@ -686,3 +686,74 @@ def test_vbx_round_trip_of_oracle_volume():
     assert r["pools0"] == p0 and r["pools1"][1:] == p1[1:] and np.array_equal(r["atlases"][0][1], a)
     assert r["atlas_bytes"] == a.nbytes and r["axisres"] == tuple(int(x) for x in o.atlas_info()["res"])
     assert len(buf) == 62 + 333 + 12 + 5 * 36 + sum(map(len, p0)) + sum(map(len, p1)) + 8 + a.nbytes
+
+
+# ---------------------------------------------------------- Compute operators
+def test_hash_is_the_one_at_a_time_hash():
+    """Applied to a one-byte key the reference's hash is Jenkins'
+    one-at-a-time hash, whose published value for "a" is 0xca2e9442."""
+    assert O.lib().orc_hash(0x61) == 0xCA2E9442
+    vals = np.array([O.lib().orc_random3(float(x), float(y), 3.0) for x in range(40) for y in range(40)])
+    assert vals.min() >= 0.0 and vals.max() < 1.0 and abs(vals.mean() - 0.5) < 0.02
+
+
+def _computed(field, effect, iters, params, precision=64, skip=True, update_apron=False):
+    pts = _ball(600, 91)
+    o = _gathered(pts, 2.0, precision=precision)
+    info = o.atlas_info()
+    rx, ry, rz = info["res"]
+    z, y, x = np.meshgrid(np.arange(rz), np.arange(ry), np.arange(rx), indexing="ij")
+    a = field(x.astype(np.float64), y.astype(np.float64), z.astype(np.float64))
+    o.set_atlas(a, precision)
+    assert o.compute(effect, iters, params, update_apron, skip, 0.0, precision) == 0
+    return o, a, o.atlas(precision)
+
+
+def _interiors(o, a):
+    import model as Mm
+    info = o.atlas_info()
+    used = info["used"]
+    cx, cy = info["cnt"][0], info["cnt"][1]
+    m = np.zeros(a.shape, bool)
+    for s in range(used):
+        sx, sy, sz = s % cx, (s // cx) % cy, s // (cx * cy)
+        m[sz * 10 + 1:sz * 10 + 9, sy * 10 + 1:sy * 10 + 9, sx * 10 + 1:sx * 10 + 9] = True
+    return m
+
+
+def test_smooth_keeps_linear_fields_and_adds_p2():
+    """The six-neighbour average of a linear field is the field, so one pass
+    of (p1 v + neighbours) / (p1 + 6) + p2 adds exactly p2 on every brick
+    interior and leaves the other voxels alone."""
+    f = lambda x, y, z: 0.5 * x - 0.25 * y + 0.125 * z + 3.0
+    o, a, b = _computed(f, 113, 1, (2.0, 0.75, 0.0))
+    m = _interiors(o, a)
+    assert np.abs(b[m] - (a[m] + 0.75)).max() < 1e-9
+    assert np.array_equal(b[~m], a[~m])
+
+
+def test_smooth_of_a_spike_spreads_by_the_weights():
+    """A unit spike at one interior voxel: the voxel keeps p1 / (p1 + 6), each
+    face neighbour 1 / (p1 + 6)."""
+    def f(x, y, z):
+        return ((x == 15) & (y == 15) & (z == 15)).astype(np.float64)
+    o, a, b = _computed(f, 113, 1, (3.0, 0.0, 0.0))
+    assert abs(b[15, 15, 15] - 3.0 / 9.0) < 1e-15
+    for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+        assert abs(b[15 + d[2], 15 + d[1], 15 + d[0]] - 1.0 / 9.0) < 1e-15
+
+
+def test_grow_and_fill_closed_forms():
+    f = lambda x, y, z: np.where(x % 3 == 0, 0.0, np.where(x % 3 == 1, 0.005, 1.0))
+    o, a, b = _computed(f, 115, 1, (0.5, 0.0, 0.0))
+    m = _interiors(o, a)
+    want = np.where(a == 0, 0.0, a + 5.0)
+    want = np.where(want < 0.01, 0.0, want)
+    assert np.array_equal(b[m], want[m]) and np.array_equal(b[~m], a[~m])
+    o, a, b = _computed(lambda x, y, z: x * 0.0, 110, 1, (2.5, 0.0, 0.0), skip=False)
+    rx = b.shape[2]
+    assert (b[1:, 1:, 1:] == 2.5).all() and (b[0] == 0).all() and (b[:, 0] == 0).all() and (b[:, :, 0] == 0).all()
+    o, a, b = _computed(lambda x, y, z: x * 0.0, 110, 1, (2.5, 0.0, 0.0), skip=True)
+    hi = [min(8 * ((c * 8 + 7) // 8), r - 1) for c, r in zip(o.atlas_info()["cnt"], o.atlas_info()["res"])]
+    assert (b[1:hi[2] + 1, 1:hi[1] + 1, 1:hi[0] + 1] == 2.5).all()
+    assert (b != 0).sum() == hi[0] * hi[1] * hi[2]
