@@ -16,7 +16,7 @@ LIB = os.path.join(LIBDIR, "libgvdb_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GVDB_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["util.hip", "topology.hip", "points.hip", "apron.hip", "render.hip", "compute.hip", "pipeline.cpp", "batch.cpp", "serve.cpp", "volume.cpp", "vbx.cpp", "camera.cpp", "capi.cpp"]
+SOURCES = ["util.hip", "topology.hip", "points.hip", "apron.hip", "render.hip", "compute.hip", "pipeline.cpp", "batch.cpp", "serve.cpp", "volume.cpp", "vbx.cpp", "plugin.cpp", "camera.cpp", "capi.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-I" + SRC, "-I" + os.path.join(HERE, "..", "include")]
 
@@ -50,6 +50,15 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("facade_demo build failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(demo + ".tmp", demo)
+    # sample user kernels as a code object for gvdb_compute_kernel / gvdb_render_kernel
+    hsaco = os.path.join(LIBDIR, "user_kernels.hsaco")
+    cmd = [HIPCC, "--genco", "-O3", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-I" + os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "examples", "user_kernels.hip"), "-o", hsaco + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("user kernel build failed: %s\n%s" % (r.stdout, r.stderr))
+    os.replace(hsaco + ".tmp", hsaco)
     if verbose:
         print("built", LIB)
     return LIB

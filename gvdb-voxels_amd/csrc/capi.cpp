@@ -114,6 +114,39 @@ int gvdb_compute(gvdb_handle h, int effect, int chan, int iterations, const floa
     const float z[3] = {0, 0, 0};
     return h->v.compute(effect, chan, iterations, params ? params : z, update_apron != 0, skip_aprons != 0, boundval);
 }
+int gvdb_compute_kernel(gvdb_handle h, void* fn, int chan, int update_apron, int skip_aprons) {
+    H_OR_FAIL(h);
+    return h->v.compute_kernel(fn, chan, update_apron != 0, skip_aprons != 0);
+}
+int gvdb_render_kernel(gvdb_handle h, void* fn, int chan, int rbuf) {
+    H_OR_FAIL(h);
+    return h->v.render_kernel(fn, chan, rbuf);
+}
+int gvdb_module_load(const char* path, void** mod) {
+    if (!path || !mod) return GVDB_ERR_INVALID;
+    hipModule_t m = nullptr;
+    const hipError_t e = hipModuleLoad(&m, path);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return e == hipErrorFileNotFound ? GVDB_ERR_IO : GVDB_ERR_HIP;
+    }
+    *mod = reinterpret_cast<void*>(m);
+    return GVDB_OK;
+}
+int gvdb_module_function(void* mod, const char* name, void** fn) {
+    if (!mod || !name || !fn) return GVDB_ERR_INVALID;
+    hipFunction_t f = nullptr;
+    if (hipModuleGetFunction(&f, reinterpret_cast<hipModule_t>(mod), name) != hipSuccess) {
+        (void)hipGetLastError();
+        return GVDB_ERR_INVALID;
+    }
+    *fn = reinterpret_cast<void*>(f);
+    return GVDB_OK;
+}
+int gvdb_module_unload(void* mod) {
+    if (!mod) return GVDB_ERR_INVALID;
+    return hipModuleUnload(reinterpret_cast<hipModule_t>(mod)) == hipSuccess ? GVDB_OK : GVDB_ERR_HIP;
+}
 int gvdb_save_vbx(gvdb_handle h, const char* path) { H_OR_FAIL(h); return h->v.save_vbx(path); }
 int gvdb_load_vbx(gvdb_handle h, const char* path, int force_major, int force_minor) {
     H_OR_FAIL(h);

@@ -99,6 +99,10 @@ public:
     int save_vbx(const char* path);
     int load_vbx(const char* path, int force_major, int force_minor);
     int write_atlas(int chan, const void* src);
+    // user kernels (plugin.cpp)
+    int compute_kernel(void* hip_function, int chan, bool update_apron, bool skip_aprons);
+    int render_kernel(void* hip_function, int chan, int rbuf);
+    int upload_view(int chan, int rbuf, void** dev);
     void get_transform(float out12[12]) const;
     bool identity_transform() const;
     int rebuild_topology_cpu(int n, const float orig[3], const float* host);
@@ -236,6 +240,7 @@ private:
     DevBuf small_;       // bbox keys, counts
     DevBuf bitmaps_, ranks_, masks_, scan_tmp_, flags_, slot_scan_;
     DevBuf incr_marker_;   // per-leaf markers of the incremental build
+    DevBuf view_dev_, view_scene_;   // the device view handed to user kernels
     uint32_t* pinned_ = nullptr;   // host staging (256 words)
     Arena arena_;
     std::shared_ptr<SharedStream> own_stream_, cur_stream_;

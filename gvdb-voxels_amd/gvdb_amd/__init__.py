@@ -41,6 +41,9 @@ _SIGS = [
     ("gvdb_save_vbx", _i, [_vp, ctypes.c_char_p]), ("gvdb_load_vbx", _i, [_vp, ctypes.c_char_p, _i, _i]),
     ("gvdb_get_transform", _i, [_vp, _vp]),
     ("gvdb_compute", _i, [_vp, _i, _i, _i, _vp, _i, _i, _f]),
+    ("gvdb_compute_kernel", _i, [_vp, _vp, _i, _i, _i]), ("gvdb_render_kernel", _i, [_vp, _vp, _i, _i]),
+    ("gvdb_module_load", _i, [ctypes.c_char_p, _vp]), ("gvdb_module_function", _i, [_vp, ctypes.c_char_p, _vp]),
+    ("gvdb_module_unload", _i, [_vp]),
     ("gvdb_clear", _i, [_vp]), ("gvdb_finish_topology", _i, [_vp, _i, _i]), ("gvdb_update_atlas", _i, [_vp]),
     ("gvdb_clear_channel", _i, [_vp, _i]),
     ("gvdb_insert_points_subcell", _i, [_vp, _i, _i, _f, _vp, ctypes.POINTER(_i)]),
@@ -116,6 +119,32 @@ def _p(a):
 
 def _f3(x):
     return np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(3))
+
+
+def module_load(path):
+    """hipModuleLoad of a code object; returns the module handle."""
+    m = ctypes.c_void_p()
+    rc = load().gvdb_module_load(os.fsencode(path), ctypes.byref(m))
+    if rc:
+        raise GVDBError(rc, "module_load: " + str(path))
+    return m.value
+
+
+def module_function(module, name):
+    f = ctypes.c_void_p()
+    rc = load().gvdb_module_function(ctypes.c_void_p(module), name.encode(), ctypes.byref(f))
+    if rc:
+        raise GVDBError(rc, "module_function: " + name)
+    return f.value
+
+
+def module_unload(module):
+    load().gvdb_module_unload(ctypes.c_void_p(module))
+
+
+def user_kernels_path():
+    """The sample user kernels built with the library."""
+    return os.path.join(os.path.dirname(LIB_PATH), "user_kernels.hsaco")
 
 
 class VolumeGVDB:
@@ -218,6 +247,14 @@ class VolumeGVDB:
     def Compute(self, effect, chan, iterations, params, update_apron, skip_aprons=True, boundval=0.0):
         return self._chk(load().gvdb_compute(self.h, int(effect), int(chan), int(iterations), _p(_f3(params)),
                                              int(update_apron), int(skip_aprons), float(boundval)))
+
+    def ComputeKernel(self, function, chan, update_apron, skip_aprons=True):
+        """Launches a user kernel (a hipFunction_t from module_function) over the atlas."""
+        return self._chk(load().gvdb_compute_kernel(self.h, ctypes.c_void_p(function), int(chan), int(update_apron),
+                                                    int(skip_aprons)))
+
+    def RenderKernel(self, function, chan=0, rbuf=0):
+        return self._chk(load().gvdb_render_kernel(self.h, ctypes.c_void_p(function), int(chan), int(rbuf)))
 
     def SaveVBX(self, path):
         return self._chk(load().gvdb_save_vbx(self.h, os.fsencode(path)))

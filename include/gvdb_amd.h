@@ -139,6 +139,17 @@ int gvdb_gather_level_set_fp16(gvdb_handle h, int subcell_size, int n, float rad
 #define GVDB_FUNC_GROW 115
 int gvdb_compute(gvdb_handle h, int effect, int chan, int iterations, const float params[3], int update_apron,
                  int skip_aprons, float boundval);
+/* ComputeKernel(user_module, user_kernel, channel, bUpdateApron, skipOverAprons): gvdb_volume_gvdb.cpp:4499-4525.
+ * hip_function is a hipFunction_t; the kernel's arguments and launch shape are in gvdb_amd_device.h.  With
+ * update_apron every channel's apron is refreshed afterwards (UpdateApron()). */
+int gvdb_compute_kernel(gvdb_handle h, void* hip_function, int chan, int update_apron, int skip_aprons);
+/* RenderKernel(user_kernel, in_channel, outbuf): gvdb_volume_gvdb.cpp:4309-4333; 8x8 blocks over the render buffer. */
+int gvdb_render_kernel(gvdb_handle h, void* hip_function, int chan, int rbuf);
+/* Loading a code object (no counterpart: the reference's applications call cuModuleLoad themselves):
+ * hipModuleLoad / hipModuleGetFunction / hipModuleUnload on the current device. */
+int gvdb_module_load(const char* code_object_path, void** hip_module);
+int gvdb_module_function(void* hip_module, const char* kernel_name, void** hip_function);
+int gvdb_module_unload(void* hip_module);
 /* SaveVBX(fname): gvdb_volume_gvdb.cpp:1626-1768.  VBX 1.11 with one grid: header, GVDB topology pools, each
  * channel's atlas in the reference's 3D layout (GVDB_FILESPEC.txt). */
 int gvdb_save_vbx(gvdb_handle h, const char* path);

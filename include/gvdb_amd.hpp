@@ -139,6 +139,15 @@ public:
         check(gvdb_compute(h_, effect, channel, num_iterations, p, bUpdateApron ? 1 : 0, skipOverAprons ? 1 : 0,
                            boundval), "Compute");
     }
+    void ComputeKernel(void* user_module, void* user_kernel, unsigned char channel, bool bUpdateApron,
+                       bool skipOverAprons) {
+        (void)user_module;   // the function handle carries its module
+        check(gvdb_compute_kernel(h_, user_kernel, channel, bUpdateApron ? 1 : 0, skipOverAprons ? 1 : 0),
+              "ComputeKernel");
+    }
+    void RenderKernel(void* user_kernel, unsigned char in_channel = 0, unsigned char outbuf = 0) {
+        check(gvdb_render_kernel(h_, user_kernel, in_channel, outbuf), "RenderKernel");
+    }
     void SaveVBX(const std::string& fname) { check(gvdb_save_vbx(h_, fname.c_str()), "SaveVBX"); }
     bool LoadVBX(const std::string& fname, int force_maj = 0, int force_min = 0) {
         const int rc = gvdb_load_vbx(h_, fname.c_str(), force_maj, force_min);
