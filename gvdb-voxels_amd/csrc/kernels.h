@@ -77,6 +77,7 @@ struct MarkArgs {
     uint32_t* bmA;
     uint32_t* bmB;
     uint32_t* masks;      // optional zeroed u32 per level-0 domain brick
+    uint8_t* bytes;       // optional zeroed byte per domain brick: A at 0, B at nwords * 32 (instead of masks)
 };
 hipError_t launch_mark_points_multi(const MarkArgs* a, int G, Arena* ar, hipStream_t s);
 

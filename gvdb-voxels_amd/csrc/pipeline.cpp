@@ -148,6 +148,7 @@ int Volume::multi_full_build(Volume* const* vs, int G, const int* n, const float
         BitDomain dom[MAXLEV];
         u64 bm_off[MAXLEV], rk_off[MAXLEV], tmp_off[2 * MAXLEV];
         uint32_t* masks = nullptr;
+        uint8_t* bytes = nullptr;
     };
     std::vector<Plan> plan(G);
     std::vector<FillArgs> fills;
@@ -198,9 +199,16 @@ int Volume::multi_full_build(Volume* const* vs, int G, const int* n, const float
         XCHK(v, v->ranks_.ensure(rk_words * 4 + 256));
         XCHK(v, v->scan_tmp_.ensure(tmp_words * 4 + 256));
         fills.push_back(FillArgs{v->bitmaps_.p, bm_words, 0, 0});
-        // direction masks cost 4 bytes per domain brick (filled, then scanned
-        // once): used while that stays comparable to the point stream itself
-        if (pl.dom[0].nbits <= std::max(u64(1) << 22, u64(n[g]) * 4)) {
+        // marks: a byte per domain brick (own-brick marks, then dilation
+        // marks, whole 32-byte groups) up to 2^27 bricks; else direction masks
+        // (4 bytes per domain brick) while that stays comparable to the point
+        // stream; else bits set directly
+        if (pl.dom[0].nbits <= (u64(1) << 27)) {
+            const u64 nbytes = pl.dom[0].nwords * 32 * (pl.dilate ? 2 : 1);
+            XCHK(v, v->masks_.ensure(size_t(nbytes)));
+            pl.bytes = reinterpret_cast<uint8_t*>(v->masks_.p);
+            fills.push_back(FillArgs{v->masks_.p, nbytes / 4, 0, 0});
+        } else if (pl.dom[0].nbits <= std::max(u64(1) << 22, u64(n[g]) * 4)) {
             XCHK(v, v->masks_.ensure(size_t(pl.dom[0].nbits) * 4));
             pl.masks = v->masks_.as<uint32_t>();
             fills.push_back(FillArgs{pl.masks, pl.dom[0].nbits, 0, 0});
@@ -218,7 +226,8 @@ int Volume::multi_full_build(Volume* const* vs, int G, const int* n, const float
         PointSrc P = v->pts_;
         P.n = n[g];
         ma[g] = MarkArgs{P, {orig3[3 * g], orig3[3 * g + 1], orig3[3 * g + 2]}, radius[g], plan[g].dilate ? 1 : 0,
-                         int(v->range_[0]), 1.0f / float(v->range_[0]), plan[g].dom[0], A(g, 0), B(g, 0), plan[g].masks};
+                         int(v->range_[0]), 1.0f / float(v->range_[0]), plan[g].dom[0], A(g, 0), B(g, 0), plan[g].masks,
+                         plan[g].bytes};
         maxD = std::max(maxD, plan[g].D);
     }
     XCHK(v0, launch_mark_points_multi(ma.data(), G, ex.ar, ex.s));

@@ -138,7 +138,9 @@ __device__ inline void set_bit(uint32_t* bm, u64 idx) {
 // gvdbCalcBrickId (kernels/cuda_gvdb_particles.cuh:352-382) for level 0 and
 // gvdbCalcExtraBrickId (:467-512) for the dilation.  Each point contributes
 // its brick and, for the dilation, the bricks covering [int(p)-r, int(p)+2r]
-// on every axis.  When those lie within one brick of the point's own (r up to
+// on every axis.  With `bytes` (domains up to 2^27 bricks) each mark is a
+// byte store of 1 after a plain read (no atomics: concurrent stores write the
+// same value), packed into the bitmaps by k_pack_bytes.  Otherwise:  When those lie within one brick of the point's own (r up to
 // one brick), the point ORs a 27-bit direction mask into its own brick's word
 // (`masks`, bit 13 = the brick itself), which k_expand_masks turns into bitmap
 // bits; this replaces up to 8 bitmap atomics per point by one rarely taken
@@ -166,6 +168,20 @@ __device__ inline void k_mark_points_body(const MarkArgs& a, uint32_t bid, uint3
                 lo[k] = bmin / range0;   // C truncation, as the reference's int division
                 hi[k] = bmax / range0;
             }
+        }
+        if (a.bytes) {
+            // one plain byte store per marked brick: concurrent stores of the
+            // same value need no atomics; k_pack_bytes turns bytes into bits
+            // a plain read skips the store where the byte is already set
+            if (inside && !a.bytes[idx]) a.bytes[idx] = 1;
+            if (a.dilate) {
+                uint8_t* bb = a.bytes + d0.nwords * 32;
+                for (int x = lo[0]; x <= hi[0]; x++)
+                    for (int y = lo[1]; y <= hi[1]; y++)
+                        for (int z = lo[2]; z <= hi[2]; z++)
+                            if (dom_index(d0, x, y, z, idx) && !bb[idx]) bb[idx] = 1;
+            }
+            continue;
         }
         const bool near = lo[0] >= c[0] - 1 && lo[1] >= c[1] - 1 && lo[2] >= c[2] - 1 && hi[0] <= c[0] + 1 &&
                           hi[1] <= c[1] + 1 && hi[2] <= c[2] + 1;
@@ -216,13 +232,41 @@ __device__ inline void k_expand_masks_body(const MarkArgs& a, uint32_t bid, uint
 }
 GVDB_MULTI(k_expand_masks, MarkArgs, 256)
 
+// Byte marks -> level-0 bitmaps: 32 bytes (each 0 or 1) per 32-bit word.
+__device__ inline uint32_t pack32(const uint8_t* b) {
+    const uint4* q = reinterpret_cast<const uint4*>(b);
+    uint32_t m = 0;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const uint4 v = q[h];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t x = w[k];
+            m |= ((x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u)) << (16 * h + 4 * k);
+        }
+    }
+    return m;
+}
+
+__device__ inline void k_pack_bytes_body(const MarkArgs& a, uint32_t bid, uint32_t nblk) {
+    const BitDomain d0 = a.d0;
+    for (u64 w = u64(bid) * 256 + threadIdx.x; w < d0.nwords; w += u64(nblk) * 256) {
+        a.bmA[w] = pack32(a.bytes + w * 32);
+        if (a.dilate) a.bmB[w] = pack32(a.bytes + d0.nwords * 32 + w * 32);
+    }
+}
+GVDB_MULTI(k_pack_bytes, MarkArgs, 256)
+
 hipError_t launch_mark_points_multi(const MarkArgs* a, int G, Arena* ar, hipStream_t s) {
-    std::vector<uint32_t> nb(G), nb2(G);
-    bool any_mask = false;
+    std::vector<uint32_t> nb(G), nb2(G), nb3(G);
+    bool any_mask = false, any_bytes = false;
     for (int g = 0; g < G; g++) {
         nb[g] = a[g].pts.n > 0 ? grid_for(u64(a[g].pts.n), 16384) : 0;
-        nb2[g] = a[g].masks ? grid_for(a[g].d0.nbits, 16384) : 0;
-        any_mask |= a[g].masks != nullptr;
+        nb2[g] = a[g].masks && !a[g].bytes ? grid_for(a[g].d0.nbits, 16384) : 0;
+        nb3[g] = a[g].bytes ? grid_for(a[g].d0.nwords, 16384) : 0;
+        any_mask |= a[g].masks != nullptr && !a[g].bytes;
+        any_bytes |= a[g].bytes != nullptr;
     }
     MultiLaunch L;
     hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
@@ -231,6 +275,16 @@ hipError_t launch_mark_points_multi(const MarkArgs* a, int G, Arena* ar, hipStre
         hipLaunchKernelGGL(k_mark_points, dim3(L.blocks), dim3(256), 0, s, a[0], L.map);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
+    }
+    if (any_bytes) {
+        MultiLaunch L3;
+        e = multi_setup(ar, a, nb3.data(), G, s, L3);
+        if (e != hipSuccess) return e;
+        if (L3.blocks) {
+            hipLaunchKernelGGL(k_pack_bytes, dim3(L3.blocks), dim3(256), 0, s, a[0], L3.map);
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
     }
     if (!any_mask) return hipSuccess;
     MultiLaunch L2;
