@@ -294,24 +294,49 @@ hipError_t launch_mark_points_multi(const MarkArgs* a, int G, Arena* ar, hipStre
     return hipGetLastError();
 }
 
-// Mark level L+1 from the set bits of level L (parent = child >> shift).
+// Node emission walks domains up to this many bits one thread per bit (dense
+// words would serialise a thread-per-word loop); larger, sparser ones per word.
+static const u64 PER_BIT_MAX = u64(1) << 24;
+
+// Mark level L+1 from the set bits of level L (parent = child >> shift):
+// one wave per parent bit reads its children's rows (lane = one (x, y) row of
+// 2^shift z bits, consecutive in x-major order) and one lane sets the bit,
+// so no two threads race on a parent word's atomics more than once per
+// parent.
+__device__ inline bool any_bits(const uint32_t* bm, u64 i0, u64 i1) {
+    // any bit set in [i0, i1] (inclusive)
+    for (u64 w = i0 >> 5; w <= (i1 >> 5); w++) {
+        uint32_t m = ~0u;
+        if (w == (i0 >> 5)) m &= ~0u << (i0 & 31);
+        if (w == (i1 >> 5)) m &= ~0u >> (31 - (i1 & 31));
+        if (bm[w] & m) return true;
+    }
+    return false;
+}
+
 __device__ inline void k_mark_parents_body(const ParentArgs& a, uint32_t bid, uint32_t nblk) {
-    for (u64 w = u64(bid) * 256 + threadIdx.x; w < a.dc.nwords; w += u64(nblk) * 256) {
-        for (int pass = 0; pass < 2; pass++) {
-            uint32_t bits = pass == 0 ? a.Ac[w] : (a.Bc ? a.Bc[w] : 0u);
-            uint32_t* out = pass == 0 ? a.Ap : a.Bp;
-            u64 last = ~u64(0);
-            while (bits) {
-                int b = __ffs(bits) - 1;
-                bits &= bits - 1;
-                int x, y, z;
-                dom_coords(a.dc, w * 32 + b, x, y, z);
-                u64 idx;
-                if (dom_index(a.dp, x >> a.shift, y >> a.shift, z >> a.shift, idx) && idx != last) {
-                    set_bit(out, idx);
-                    last = idx;
-                }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int s = a.shift, side = 1 << s;
+    const BitDomain& c = a.dc;
+    for (u64 p = u64(bid) * 4 + wave; p < a.dp.nbits; p += u64(nblk) * 4) {
+        int px, py, pz;
+        dom_coords(a.dp, p, px, py, pz);
+        const int z0 = max(pz << s, c.lo[2]), z1 = min(((pz + 1) << s) - 1, c.lo[2] + c.dim[2] - 1);
+        bool hasA = false, hasB = false;
+        if (z0 <= z1) {
+            for (int r = lane; r < side * side; r += 64) {
+                const int cx = (px << s) + (r & (side - 1)), cy = (py << s) + (r >> s);
+                if (cx < c.lo[0] || cy < c.lo[1] || cx >= c.lo[0] + c.dim[0] || cy >= c.lo[1] + c.dim[1]) continue;
+                const u64 i0 = (u64(cx - c.lo[0]) * c.dim[1] + u64(cy - c.lo[1])) * c.dim[2] + u64(z0 - c.lo[2]);
+                const u64 i1 = i0 + u64(z1 - z0);
+                hasA = hasA || any_bits(a.Ac, i0, i1);
+                if (a.Bc) hasB = hasB || any_bits(a.Bc, i0, i1);
             }
+        }
+        const u64 ba = __ballot(hasA), bb = __ballot(hasB);
+        if (lane == 0) {
+            if (ba) set_bit(a.Ap, p);
+            if (bb && a.Bp) set_bit(a.Bp, p);
         }
     }
 }
@@ -319,7 +344,7 @@ GVDB_MULTI(k_mark_parents, ParentArgs, 256)
 
 hipError_t launch_mark_parents_multi(const ParentArgs* a, int G, Arena* ar, hipStream_t s) {
     std::vector<uint32_t> nb(G);
-    for (int g = 0; g < G; g++) nb[g] = grid_for(a[g].dc.nwords, 16384);
+    for (int g = 0; g < G; g++) nb[g] = uint32_t(std::min<u64>((a[g].dp.nbits + 3) / 4, 65536));
     MultiLaunch L;
     hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
     if (e != hipSuccess || L.blocks == 0) return e;
@@ -346,6 +371,26 @@ __device__ inline void write_node(Node* n, int lev, int x, int y, int z, int ran
 // first pass, :965-988 for the appended dilation nodes).
 __device__ inline void k_emit_nodes_body(const EmitArgs& a, uint32_t bid, uint32_t nblk) {
     const LevelBuild& L = a.L;
+    if (L.dom.nbits <= PER_BIT_MAX) {
+        const bool dil = L.bmB != L.bmA && L.bmB;
+        for (u64 i = u64(bid) * 256 + threadIdx.x; i < L.dom.nbits; i += u64(nblk) * 256) {
+            const u64 w = i >> 5;
+            const uint32_t bit = 1u << (i & 31), below = bit - 1u;
+            const uint32_t av = L.bmA[w];
+            uint32_t idx;
+            if (av & bit) {
+                idx = L.rankA[w] + uint32_t(__popc(av & below));
+            } else {
+                const uint32_t e = dil ? (L.bmB[w] & ~av) : 0u;
+                if (!(e & bit)) continue;
+                idx = L.countA + L.rankE[w] + uint32_t(__popc(e & below));
+            }
+            int x, y, z;
+            dom_coords(L.dom, i, x, y, z);
+            write_node(a.nodes + idx, L.lev, x, y, z, L.range, a.has_clist ? Elem(1, L.lev, idx) : ID_UNDEFL);
+        }
+        return;
+    }
     for (u64 w = u64(bid) * 256 + threadIdx.x; w < L.dom.nwords; w += u64(nblk) * 256) {
         uint32_t av = L.bmA[w];
         uint32_t idx = L.rankA[w];
@@ -376,7 +421,8 @@ GVDB_MULTI(k_emit_nodes, EmitArgs, 256)
 
 hipError_t launch_emit_nodes_multi(const EmitArgs* a, int G, Arena* ar, hipStream_t s) {
     std::vector<uint32_t> nb(G);
-    for (int g = 0; g < G; g++) nb[g] = grid_for(a[g].L.dom.nwords, 16384);
+    for (int g = 0; g < G; g++)
+        nb[g] = grid_for(a[g].L.dom.nbits <= PER_BIT_MAX ? a[g].L.dom.nbits : a[g].L.dom.nwords, 16384);
     MultiLaunch L;
     hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
     if (e != hipSuccess || L.blocks == 0) return e;
