@@ -882,21 +882,43 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
         if (rule == 1) return !wneg && w[2] != NOHIT;
         return !wneg && !pneg;
     };
-    for (int i = 0; i < n; i++) {
-        const float* p = pts + 3 * i;
-        float w[3];
-        if (p[2] == NOHIT) continue;
-        bool wneg = p[0] + trans[0] < 0 || p[1] + trans[1] < 0 || p[2] + trans[2] < 0;
-        bool pneg = p[0] < 0 || p[1] < 0 || p[2] < 0;
-        if (rule == 0 && wneg != pneg) v.undefined_events++;
-        if (rule == 1 && !wneg && p[2] + trans[2] == NOHIT) v.undefined_events++;
-        if (!binned(p, w)) continue;
-        forEachSubcell(w, [&](int idx) { v.sc_cnt[idx]++; });
+    // Both passes run over the same static chunks of points, one per thread:
+    // per-thread counts per subcell, and per-thread starting slots from them,
+    // so each subcell's entries stay in ascending point order.
+    const int nt = std::max(1, std::min(omp_get_max_threads(), n / 4096 + 1));
+    std::vector<std::vector<int>> tc(nt, std::vector<int>(numSC, 0));
+    i64 undef = 0;
+#pragma omp parallel for num_threads(nt) schedule(static, 1) reduction(+ : undef)
+    for (int t = 0; t < nt; t++) {
+        const int i0 = int(i64(n) * t / nt), i1 = int(i64(n) * (t + 1) / nt);
+        std::vector<int>& c = tc[t];
+        for (int i = i0; i < i1; i++) {
+            const float* p = pts + 3 * i;
+            float w[3];
+            if (p[2] == NOHIT) continue;
+            bool wneg = p[0] + trans[0] < 0 || p[1] + trans[1] < 0 || p[2] + trans[2] < 0;
+            bool pneg = p[0] < 0 || p[1] < 0 || p[2] < 0;
+            if (rule == 0 && wneg != pneg) undef++;
+            if (rule == 1 && !wneg && p[2] + trans[2] == NOHIT) undef++;
+            if (!binned(p, w)) continue;
+            forEachSubcell(w, [&](int idx) { c[idx]++; });
+        }
     }
+    v.undefined_events += int(undef);
     v.sc_off.assign(numSC, 0);
     {
         i64 acc = 0;
-        for (int i = 0; i < numSC; i++) { v.sc_off[i] = int(acc); acc += v.sc_cnt[i]; }
+        for (int i = 0; i < numSC; i++) {
+            v.sc_off[i] = int(acc);
+            int tot = 0;
+            for (int t = 0; t < nt; t++) {
+                const int c = tc[t][i];
+                tc[t][i] = int(acc) + tot;      // thread t's first slot in subcell i
+                tot += c;
+            }
+            v.sc_cnt[i] = tot;
+            acc += tot;
+        }
     }
     int len = v.sc_off[numSC - 1] + v.sc_cnt[numSC - 1];
     *outLen = len;
@@ -904,18 +926,20 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
     // insert pass
     v.sc_pnt.assign(size_t(len) * 3, 0.f);
     if (len > 0) {
-        std::vector<int> cnt2(numSC, 0);
-        for (int i = 0; i < n; i++) {
-            const float* p = pts + 3 * i;
-            float w[3];
-            if (!binned(p, w)) continue;
-            forEachSubcell(w, [&](int idx) {
-                int slot = cnt2[idx]++;
-                size_t e = size_t(v.sc_off[idx]) + slot;
-                v.sc_pnt[3 * e] = w[0]; v.sc_pnt[3 * e + 1] = w[1]; v.sc_pnt[3 * e + 2] = w[2];
-            });
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+        for (int t = 0; t < nt; t++) {
+            const int i0 = int(i64(n) * t / nt), i1 = int(i64(n) * (t + 1) / nt);
+            std::vector<int>& slot = tc[t];
+            for (int i = i0; i < i1; i++) {
+                const float* p = pts + 3 * i;
+                float w[3];
+                if (!binned(p, w)) continue;
+                forEachSubcell(w, [&](int idx) {
+                    size_t e = size_t(slot[idx]++);
+                    v.sc_pnt[3 * e] = w[0]; v.sc_pnt[3 * e + 1] = w[1]; v.sc_pnt[3 * e + 2] = w[2];
+                });
+            }
         }
-        v.sc_cnt = cnt2;
     }
     // CalcSubcellPos.  The reference returns before it on an empty list
     // (gvdb_volume_gvdb.cpp:5058), which leaves GatherLevelSet reading the
