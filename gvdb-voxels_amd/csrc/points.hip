@@ -1,5 +1,6 @@
 // points.hip -- atlas slot assignment, subcell binning and the density gather.
 #include "kernels.h"
+#include <type_traits>
 #include "tree_device.h"
 #include <hip/hip_fp16.h>
 
@@ -1131,19 +1132,25 @@ GVDB_MULTI_T(k_bucket_gather, typename AT, AT, BucketGatherArgs, GB_THREADS)
 // slice are padded to 65 slots so the slices of one candidate fall in
 // different banks.
 static const int PG_ZSTRIDE = 65;
+// Radii whose box has at most PG_ZMAX z slices (2r + 1/16 < PG_ZMAX: r <= 2)
+// list only the slices inside the brick as work items; larger ones walk every
+// slice of every candidate.
+static const int PG_ZMAX = 5;
 // NT threads per brick: PG_SMALL = 128 (two waves; measured faster than 64
 // and 256 on the medium workload) when the (2 + 2R)^3 cells fit one per
 // thread (R = 1: 64), else 256 (R = 2: 216 cells).
 static const int PG_SMALL = 128;
-template <int NT>
+template <int NT, bool LISTED>
 struct PGShared {
-    uint32_t cell_off[NT];
-    uint32_t cell_start[NT];
+    // the listed variant runs with R = 1 only: (2 + 2R)^3 = 64 cells
+    uint32_t cell_off[LISTED ? 64 : NT];
+    uint32_t cell_start[LISTED ? 64 : NT];
     int nbr[27];
     uint32_t wsum[NT / 64];
-    uint32_t ncand;
+    uint32_t ncand, nitems;
     u64 key[8 * PG_ZSTRIDE];
-    float4 pt[2 * NT];
+    typename std::conditional<LISTED, float3, float4>::type pt[2 * NT];
+    uint16_t item[LISTED ? 2 * NT * PG_ZMAX : 2];   // candidate << 3 | z slice in the brick
 };
 
 // r - sqrt(D) 2^-s, accurate to a few ulps: D split into float hi + lo
@@ -1160,9 +1167,9 @@ __device__ inline float falloff_from_d2(u64 D, int s, float r) {
     return num / (r + __builtin_sqrtf(hi));
 }
 
-template <int NT, typename AT>
+template <int NT, typename AT, bool LISTED>
 __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bid, uint32_t) {
-    __shared__ PGShared<NT> S;
+    __shared__ PGShared<NT, LISTED> S;
     const Tree& t = a.t;
     const int brick = int(bid);
     const Node* bn = t.nodes[0] + brick;
@@ -1190,6 +1197,7 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
     // roundings at most (under 1/32 each for coordinates below 2^18)
     const uint32_t nz = uint32_t(2.0f * r + 0.0625f) + 1u;
     const uint32_t nzinv = uint32_t(0xFFFFFFFFu / nz) + 1u;
+    constexpr bool listed = LISTED;   // the host picks it when nz <= PG_ZMAX
     const u64 lower = lane ? ((~0ull) >> (64 - lane)) : 0ull;
     for (uint32_t c0 = 0; c0 < T; c0 += (2 * NT)) {
         float4 p[2];
@@ -1206,32 +1214,74 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
                 p[h] = a.sorted[S.cell_off[lo] + (f - S.cell_start[lo])];
             }
         }
-        if (tid == 0) S.ncand = 0;
-        __syncthreads();                           // ncand cleared; previous chunk done with pt
+        if (tid == 0) S.ncand = S.nitems = 0;
+        __syncthreads();                           // counters cleared; previous chunk done with pt
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const uint32_t f = c0 + uint32_t(h * NT + tid);
             bool keep = false;
+            int zlo = 0, nzv = 0;
             if (f < T) {
                 const float4 q = p[h];
                 keep = q.x - r <= bx1 && q.x + r >= bx0 && q.y - r <= by1 && q.y + r >= by0 &&
                        q.z - r <= bz1 && q.z + r >= bz0;
+                if (keep && listed) {
+                    // the slices z in [fl(q.z - r), fl(q.z + r)] inside the brick, at
+                    // most nz of them from the first (so nzv <= nz <= PG_ZMAX)
+                    const int zc = int(ceilf(q.z - r));
+                    zlo = max(zc, oz);
+                    nzv = max(min(min(int(floorf(q.z + r)), zc + int(min(nz, uint32_t(PG_ZMAX))) - 1), oz + 7) - zlo + 1, 0);
+                    keep = nzv > 0;
+                }
             }
             const u64 bal = __ballot(keep);
             uint32_t base = 0;
-            if (lane == 0 && bal) base = atomicAdd(&S.ncand, uint32_t(__popcll(bal)));
-            base = __shfl(base, 0, 64);
-            if (keep) S.pt[base + uint32_t(__popcll(bal & lower))] = p[h];
+            if constexpr (LISTED) {
+                // lanes' slice counts (0..5) summed below each lane, from the bits
+                const u64 b0 = __ballot(nzv & 1), b1 = __ballot(nzv & 2), b2 = __ballot(nzv & 4);
+                const uint32_t ipre = uint32_t(__popcll(b0 & lower) + 2 * __popcll(b1 & lower) + 4 * __popcll(b2 & lower));
+                const uint32_t itot = uint32_t(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
+                uint32_t ibase = 0;
+                if (lane == 0 && bal) {
+                    base = atomicAdd(&S.ncand, uint32_t(__popcll(bal)));
+                    ibase = atomicAdd(&S.nitems, itot);
+                }
+                base = __shfl(base, 0, 64);
+                ibase = __shfl(ibase, 0, 64);
+                if (keep) {
+                    const uint32_t ci = base + uint32_t(__popcll(bal & lower));
+                    S.pt[ci] = make_float3(p[h].x, p[h].y, p[h].z);
+                    uint16_t* dst = S.item + ibase + ipre;
+                    const uint32_t w0 = ci << 3 | uint32_t(zlo - oz);
+#pragma nounroll
+                    for (int k = 0; k < nzv; k++) dst[k] = uint16_t(w0 + uint32_t(k));
+                }
+            } else {
+                if (lane == 0 && bal) base = atomicAdd(&S.ncand, uint32_t(__popcll(bal)));
+                base = __shfl(base, 0, 64);
+                if (keep) S.pt[base + uint32_t(__popcll(bal & lower))] = p[h];
+            }
         }
         __syncthreads();
-        const uint32_t items = S.ncand * nz;
+        const uint32_t items = listed ? S.nitems : S.ncand * nz;
         for (uint32_t it = uint32_t(tid); it < items; it += NT) {
-            uint32_t c = __umulhi(it, nzinv);
-            if (c * nz > it) c--;
-            const int zi = int(it - c * nz);
-            const float4 q = S.pt[c];
-            const int z = int(ceilf(q.z - r)) + zi;
-            if (z < oz || z > oz + 7 || float(z) > q.z + r) continue;
+            uint32_t c;
+            int z;
+            float4 q;
+            if constexpr (LISTED) {
+                const uint32_t w = S.item[it];
+                c = w >> 3;
+                z = oz + int(w & 7u);
+                const auto q3 = S.pt[c];
+                q = make_float4(q3.x, q3.y, q3.z, 0.0f);
+            } else {
+                c = __umulhi(it, nzinv);
+                if (c * nz > it) c--;
+                const int zi = int(it - c * nz);
+                q = S.pt[c];
+                z = int(ceilf(q.z - r)) + zi;
+                if (z < oz || z > oz + 7 || float(z) > q.z + r) continue;
+            }
             const int y0 = max(int(ceilf(q.y - r)), oy), y1 = min(int(floorf(q.y + r)), oy + 7);
             const int x0 = max(int(ceilf(q.x - r)), ox), x1 = min(int(floorf(q.x + r)), ox + 7);
             if (y0 > y1 || x0 > x1) continue;
@@ -1299,30 +1349,36 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
 
 // Written out (not GVDB_MULTI_T) to ask for eight waves per SIMD (at most 64
 // VGPRs).
-template <int NT, typename AT>
+template <int NT, typename AT, bool LISTED>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_push_gather(BucketGatherArgs a0,
                                                                                               MultiMap m) {
     if (m.G <= 1) {
-        k_push_gather_body<NT, AT>(a0, blockIdx.x, gridDim.x);
+        k_push_gather_body<NT, AT, LISTED>(a0, blockIdx.x, gridDim.x);
         return;
     }
     const int g = multi_grid(m, blockIdx.x);
     const uint32_t* first = multi_first(m);
-    k_push_gather_body<NT, AT>(multi_rec<BucketGatherArgs>(m, g), blockIdx.x - first[g], first[g + 1] - first[g]);
+    k_push_gather_body<NT, AT, LISTED>(multi_rec<BucketGatherArgs>(m, g), blockIdx.x - first[g], first[g + 1] - first[g]);
 }
+
+// The push gather's z slices per candidate, as the kernel computes it.
+static inline uint32_t pg_slices(float r) { return uint32_t(2.0f * r + 0.0625f) + 1u; }
 
 hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* ar, hipStream_t s) {
     // variants: scatter kernel (no accumulate) with 128 or 256 threads, or the
     // nearest-entry kernel (accumulate: there the reference's max also takes
     // the negative r - |p - v| of entries beyond the radius); FP32 or FP16
-    for (int variant = 0; variant < 6; variant++) {
+    for (int variant = 0; variant < 8; variant++) {
         const bool half = variant & 1;
-        const int kind = variant >> 1;             // 0: scatter, 128; 1: scatter, 256; 2: nearest entry
+        // 0: scatter, 128 threads, slices listed; 1: scatter, 128; 2: scatter, 256; 3: nearest entry
+        const int kind = variant >> 1;
         std::vector<BucketGatherArgs> sel;
         std::vector<uint32_t> nb;
         for (int g = 0; g < G; g++) {
             if (a[g].t.res[0] != 8 || a[g].R < 1 || a[g].R > 2) return hipErrorInvalidValue;
-            const int k = (a[g].accumulate || a[g].levelset) ? 2 : (a[g].R == 1 ? 0 : 1);
+            const int k = (a[g].accumulate || a[g].levelset) ? 3
+                          : a[g].R == 2                      ? 2
+                          : pg_slices(a[g].radius) <= uint32_t(PG_ZMAX) ? 0 : 1;
             if ((a[g].atlas_type == T_HALF) != half || k != kind || a[g].nleaves <= 0) continue;
             sel.push_back(a[g]);
             nb.push_back(uint32_t(a[g].nleaves));
@@ -1333,10 +1389,12 @@ hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* a
         if (e != hipSuccess) return e;
         if (L.blocks == 0) continue;
         const dim3 grid(L.blocks);
-        if (kind == 0 && half) hipLaunchKernelGGL((k_push_gather<PG_SMALL, __half>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map);
-        else if (kind == 0) hipLaunchKernelGGL((k_push_gather<PG_SMALL, float>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map);
-        else if (kind == 1 && half) hipLaunchKernelGGL((k_push_gather<256, __half>), grid, dim3(256), 0, s, sel[0], L.map);
-        else if (kind == 1) hipLaunchKernelGGL((k_push_gather<256, float>), grid, dim3(256), 0, s, sel[0], L.map);
+        if (kind == 0 && half) hipLaunchKernelGGL((k_push_gather<PG_SMALL, __half, true>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map);
+        else if (kind == 0) hipLaunchKernelGGL((k_push_gather<PG_SMALL, float, true>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map);
+        else if (kind == 1 && half) hipLaunchKernelGGL((k_push_gather<PG_SMALL, __half, false>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map);
+        else if (kind == 1) hipLaunchKernelGGL((k_push_gather<PG_SMALL, float, false>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map);
+        else if (kind == 2 && half) hipLaunchKernelGGL((k_push_gather<256, __half, false>), grid, dim3(256), 0, s, sel[0], L.map);
+        else if (kind == 2) hipLaunchKernelGGL((k_push_gather<256, float, false>), grid, dim3(256), 0, s, sel[0], L.map);
         else if (half) hipLaunchKernelGGL(k_bucket_gather<__half>, grid, dim3(GB_THREADS), 0, s, sel[0], L.map);
         else hipLaunchKernelGGL(k_bucket_gather<float>, grid, dim3(GB_THREADS), 0, s, sel[0], L.map);
         e = hipGetLastError();
