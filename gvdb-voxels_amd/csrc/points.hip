@@ -1296,14 +1296,32 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
                 const int X0 = int(hx * scale) + int(ex * scale);
                 int Y = int(hy * scale) + int(ey * scale);
                 const long long Z2 = (long long)Z * Z;
-                for (int y = y0; y <= y1; y++, Y -= step) {
-                    const u64 YZ = u64((long long)Y * Y + Z2);
-                    if (YZ >= dcut) continue;
-                    u64* kr = kz + (y - oy) * 8;
-                    int X = X0;
-                    for (int x = x0; x <= x1; x++, X -= step) {
-                        const u64 D = u64((long long)X * X) + YZ;
-                        if (D < dcut) atomicMin(kr + x, D);
+                if constexpr (LISTED) {
+                    // at most PG_ZMAX rows and columns (the box bound of the slices
+                    // holds per axis for coordinates below 2^18): fixed, unrolled
+                    u64* k0 = kz + (y0 - oy) * 8 + x0;
+#pragma unroll
+                    for (int j = 0; j < PG_ZMAX; j++) {
+                        const int Yj = Y - j * step;
+                        const u64 YZ = u64((long long)Yj * Yj + Z2);
+                        if (y0 + j > y1 || YZ >= dcut) continue;
+#pragma unroll
+                        for (int i = 0; i < PG_ZMAX; i++) {
+                            const int Xi = X0 - i * step;
+                            const u64 D = u64((long long)Xi * Xi) + YZ;
+                            if (x0 + i <= x1 && D < dcut) atomicMin(k0 + j * 8 + i, D);
+                        }
+                    }
+                } else {
+                    for (int y = y0; y <= y1; y++, Y -= step) {
+                        const u64 YZ = u64((long long)Y * Y + Z2);
+                        if (YZ >= dcut) continue;
+                        u64* kr = kz + (y - oy) * 8;
+                        int X = X0;
+                        for (int x = x0; x <= x1; x++, X -= step) {
+                            const u64 D = u64((long long)X * X) + YZ;
+                            if (D < dcut) atomicMin(kr + x, D);
+                        }
                     }
                 }
             } else {
@@ -1376,9 +1394,10 @@ hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* a
         std::vector<uint32_t> nb;
         for (int g = 0; g < G; g++) {
             if (a[g].t.res[0] != 8 || a[g].R < 1 || a[g].R > 2) return hipErrorInvalidValue;
-            const int k = (a[g].accumulate || a[g].levelset) ? 3
-                          : a[g].R == 2                      ? 2
-                          : pg_slices(a[g].radius) <= uint32_t(PG_ZMAX) ? 0 : 1;
+            // listed: at most PG_ZMAX slices per axis, coordinates below 2^18
+            const bool listed = pg_slices(a[g].radius) <= uint32_t(PG_ZMAX) &&
+                                a[g].t.range[a[g].t.levels - 1] <= (1 << 18);
+            const int k = (a[g].accumulate || a[g].levelset) ? 3 : a[g].R == 2 ? 2 : listed ? 0 : 1;
             if ((a[g].atlas_type == T_HALF) != half || k != kind || a[g].nleaves <= 0) continue;
             sel.push_back(a[g]);
             nb.push_back(uint32_t(a[g].nleaves));
