@@ -198,6 +198,17 @@ __device__ inline float two_diff(float a, float b, float& e) {
     return s;
 }
 
+// The least integer above the exact p - r: p - r = h + e exactly (TwoSum); an
+// integer cannot lie strictly between h and h + e (it would be nearer), so
+// floor(p - r) = floor(h), less one when h is an integer and e < 0.
+__device__ inline int first_above(float p, float r) {
+    float e;
+    const float h = two_diff(p, r, e);
+    int f = int(floorf(h));
+    if (float(f) == h && e < 0.0f) f--;
+    return f + 1;
+}
+
 __device__ inline void exact_d2(float px, float py, float pz, float vx, float vy, float vz, float& hi, float& lo) {
 #pragma clang fp contract(off)
     float ex, ey, ez;
@@ -1140,17 +1151,17 @@ static const int PG_ZMAX = 5;
 // and 256 on the medium workload) when the (2 + 2R)^3 cells fit one per
 // thread (R = 1: 64), else 256 (R = 2: 216 cells).
 static const int PG_SMALL = 128;
-template <int NT, bool LISTED>
+template <int NT, int LS>
 struct PGShared {
     // the listed variant runs with R = 1 only: (2 + 2R)^3 = 64 cells
-    uint32_t cell_off[LISTED ? 64 : NT];
-    uint32_t cell_start[LISTED ? 64 : NT];
+    uint32_t cell_off[LS ? 64 : NT];
+    uint32_t cell_start[LS ? 64 : NT];
     int nbr[27];
     uint32_t wsum[NT / 64];
     uint32_t ncand, nitems;
     u64 key[8 * PG_ZSTRIDE];
-    typename std::conditional<LISTED, float3, float4>::type pt[2 * NT];
-    uint16_t item[LISTED ? 2 * NT * PG_ZMAX : 2];   // candidate << 3 | z slice in the brick
+    typename std::conditional<LS != 0, float3, float4>::type pt[2 * NT];
+    uint16_t item[LS ? 2 * NT * PG_ZMAX : 2];   // candidate << 3 | z slice in the brick
 };
 
 // r - sqrt(D) 2^-s, accurate to a few ulps: D split into float hi + lo
@@ -1167,9 +1178,15 @@ __device__ inline float falloff_from_d2(u64 D, int s, float r) {
     return num / (r + __builtin_sqrtf(hi));
 }
 
-template <int NT, typename AT, bool LISTED>
+// LS = 0: every slice of every candidate, variable loops.  LS = 5: the slices
+// inside the brick listed, rows and columns in fixed unrolled loops of 5
+// (2r + 1/16 < 5).  LS = 4 (r <= 2): on bricks with exact integer distances
+// each axis starts at the first integer above p - r (found exactly), so four
+// slots cover every voxel with d < r; the voxels this leaves out have
+// D >= r^2 2^2s, whose value r - d is <= 0 and stores 0 either way.
+template <int NT, typename AT, int LS>
 __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bid, uint32_t) {
-    __shared__ PGShared<NT, LISTED> S;
+    __shared__ PGShared<NT, LS> S;
     const Tree& t = a.t;
     const int brick = int(bid);
     const Node* bn = t.nodes[0] + brick;
@@ -1197,7 +1214,8 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
     // roundings at most (under 1/32 each for coordinates below 2^18)
     const uint32_t nz = uint32_t(2.0f * r + 0.0625f) + 1u;
     const uint32_t nzinv = uint32_t(0xFFFFFFFFu / nz) + 1u;
-    constexpr bool listed = LISTED;   // the host picks it when nz <= PG_ZMAX
+    constexpr bool listed = LS != 0;   // the host picks LS from the radius
+    const bool tight = LS == 4 && exact;
     const u64 lower = lane ? ((~0ull) >> (64 - lane)) : 0ull;
     for (uint32_t c0 = 0; c0 < T; c0 += (2 * NT)) {
         float4 p[2];
@@ -1227,16 +1245,18 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
                        q.z - r <= bz1 && q.z + r >= bz0;
                 if (keep && listed) {
                     // the slices z in [fl(q.z - r), fl(q.z + r)] inside the brick, at
-                    // most nz of them from the first (so nzv <= nz <= PG_ZMAX)
-                    const int zc = int(ceilf(q.z - r));
+                    // most nz of them from the first (so nzv <= nz <= PG_ZMAX); tight:
+                    // from the first integer above q.z - r, at most four
+                    const int zc = tight ? first_above(q.z, r) : int(ceilf(q.z - r));
+                    const int cap = tight ? 4 : int(min(nz, uint32_t(PG_ZMAX)));
                     zlo = max(zc, oz);
-                    nzv = max(min(min(int(floorf(q.z + r)), zc + int(min(nz, uint32_t(PG_ZMAX))) - 1), oz + 7) - zlo + 1, 0);
+                    nzv = max(min(min(int(floorf(q.z + r)), zc + cap - 1), oz + 7) - zlo + 1, 0);
                     keep = nzv > 0;
                 }
             }
             const u64 bal = __ballot(keep);
             uint32_t base = 0;
-            if constexpr (LISTED) {
+            if constexpr (LS != 0) {
                 // lanes' slice counts (0..5) summed below each lane, from the bits
                 const u64 b0 = __ballot(nzv & 1), b1 = __ballot(nzv & 2), b2 = __ballot(nzv & 4);
                 const uint32_t ipre = uint32_t(__popcll(b0 & lower) + 2 * __popcll(b1 & lower) + 4 * __popcll(b2 & lower));
@@ -1268,7 +1288,7 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
             uint32_t c;
             int z;
             float4 q;
-            if constexpr (LISTED) {
+            if constexpr (LS != 0) {
                 const uint32_t w = S.item[it];
                 c = w >> 3;
                 z = oz + int(w & 7u);
@@ -1282,8 +1302,8 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
                 z = int(ceilf(q.z - r)) + zi;
                 if (z < oz || z > oz + 7 || float(z) > q.z + r) continue;
             }
-            const int y0 = max(int(ceilf(q.y - r)), oy), y1 = min(int(floorf(q.y + r)), oy + 7);
-            const int x0 = max(int(ceilf(q.x - r)), ox), x1 = min(int(floorf(q.x + r)), ox + 7);
+            const int y0 = max(tight ? first_above(q.y, r) : int(ceilf(q.y - r)), oy), y1 = min(int(floorf(q.y + r)), oy + 7);
+            const int x0 = max(tight ? first_above(q.x, r) : int(ceilf(q.x - r)), ox), x1 = min(int(floorf(q.x + r)), ox + 7);
             if (y0 > y1 || x0 > x1) continue;
             u64* kz = S.key + (z - oz) * PG_ZSTRIDE - ox;
             const float dz = q.z - float(z);
@@ -1296,17 +1316,17 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
                 const int X0 = int(hx * scale) + int(ex * scale);
                 int Y = int(hy * scale) + int(ey * scale);
                 const long long Z2 = (long long)Z * Z;
-                if constexpr (LISTED) {
-                    // at most PG_ZMAX rows and columns (the box bound of the slices
+                if constexpr (LS != 0) {
+                    // at most LS rows and columns (the box bound of the slices
                     // holds per axis for coordinates below 2^18): fixed, unrolled
                     u64* k0 = kz + (y0 - oy) * 8 + x0;
 #pragma unroll
-                    for (int j = 0; j < PG_ZMAX; j++) {
+                    for (int j = 0; j < LS; j++) {
                         const int Yj = Y - j * step;
                         const u64 YZ = u64((long long)Yj * Yj + Z2);
                         if (y0 + j > y1 || YZ >= dcut) continue;
 #pragma unroll
-                        for (int i = 0; i < PG_ZMAX; i++) {
+                        for (int i = 0; i < LS; i++) {
                             const int Xi = X0 - i * step;
                             const u64 D = u64((long long)Xi * Xi) + YZ;
                             if (x0 + i <= x1 && D < dcut) atomicMin(k0 + j * 8 + i, D);
@@ -1367,37 +1387,42 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
 
 // Written out (not GVDB_MULTI_T) to ask for eight waves per SIMD (at most 64
 // VGPRs).
-template <int NT, typename AT, bool LISTED>
+template <int NT, typename AT, int LS>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_push_gather(BucketGatherArgs a0,
                                                                                               MultiMap m) {
     if (m.G <= 1) {
-        k_push_gather_body<NT, AT, LISTED>(a0, blockIdx.x, gridDim.x);
+        k_push_gather_body<NT, AT, LS>(a0, blockIdx.x, gridDim.x);
         return;
     }
     const int g = multi_grid(m, blockIdx.x);
     const uint32_t* first = multi_first(m);
-    k_push_gather_body<NT, AT, LISTED>(multi_rec<BucketGatherArgs>(m, g), blockIdx.x - first[g], first[g + 1] - first[g]);
+    k_push_gather_body<NT, AT, LS>(multi_rec<BucketGatherArgs>(m, g), blockIdx.x - first[g], first[g + 1] - first[g]);
 }
 
 // The push gather's z slices per candidate, as the kernel computes it.
 static inline uint32_t pg_slices(float r) { return uint32_t(2.0f * r + 0.0625f) + 1u; }
 
 hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* ar, hipStream_t s) {
-    // variants: scatter kernel (no accumulate) with 128 or 256 threads, or the
-    // nearest-entry kernel (accumulate: there the reference's max also takes
-    // the negative r - |p - v| of entries beyond the radius); FP32 or FP16
-    for (int variant = 0; variant < 8; variant++) {
+    // variants: scatter kernel (no accumulate) with 128 threads and 4, 5 or any
+    // voxels per axis, or 256 threads, or the nearest-entry kernel (accumulate:
+    // there the reference's max also takes the negative r - |p - v| of entries
+    // beyond the radius); FP32 or FP16
+    for (int variant = 0; variant < 10; variant++) {
         const bool half = variant & 1;
-        // 0: scatter, 128 threads, slices listed; 1: scatter, 128; 2: scatter, 256; 3: nearest entry
+        // 0: scatter 128, LS 4; 1: scatter 128, LS 5; 2: scatter 128; 3: scatter 256; 4: nearest entry
         const int kind = variant >> 1;
         std::vector<BucketGatherArgs> sel;
         std::vector<uint32_t> nb;
         for (int g = 0; g < G; g++) {
             if (a[g].t.res[0] != 8 || a[g].R < 1 || a[g].R > 2) return hipErrorInvalidValue;
-            // listed: at most PG_ZMAX slices per axis, coordinates below 2^18
+            // listed slices: at most PG_ZMAX per axis, coordinates below 2^18
             const bool listed = pg_slices(a[g].radius) <= uint32_t(PG_ZMAX) &&
                                 a[g].t.range[a[g].t.levels - 1] <= (1 << 18);
-            const int k = (a[g].accumulate || a[g].levelset) ? 3 : a[g].R == 2 ? 2 : listed ? 0 : 1;
+            const int k = (a[g].accumulate || a[g].levelset) ? 4
+                          : a[g].R == 2                      ? 3
+                          : !listed                          ? 2
+                          : a[g].radius <= 2.0f              ? 0
+                                                             : 1;
             if ((a[g].atlas_type == T_HALF) != half || k != kind || a[g].nleaves <= 0) continue;
             sel.push_back(a[g]);
             nb.push_back(uint32_t(a[g].nleaves));
@@ -1408,14 +1433,18 @@ hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* a
         if (e != hipSuccess) return e;
         if (L.blocks == 0) continue;
         const dim3 grid(L.blocks);
-        if (kind == 0 && half) hipLaunchKernelGGL((k_push_gather<PG_SMALL, __half, true>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map);
-        else if (kind == 0) hipLaunchKernelGGL((k_push_gather<PG_SMALL, float, true>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map);
-        else if (kind == 1 && half) hipLaunchKernelGGL((k_push_gather<PG_SMALL, __half, false>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map);
-        else if (kind == 1) hipLaunchKernelGGL((k_push_gather<PG_SMALL, float, false>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map);
-        else if (kind == 2 && half) hipLaunchKernelGGL((k_push_gather<256, __half, false>), grid, dim3(256), 0, s, sel[0], L.map);
-        else if (kind == 2) hipLaunchKernelGGL((k_push_gather<256, float, false>), grid, dim3(256), 0, s, sel[0], L.map);
-        else if (half) hipLaunchKernelGGL(k_bucket_gather<__half>, grid, dim3(GB_THREADS), 0, s, sel[0], L.map);
-        else hipLaunchKernelGGL(k_bucket_gather<float>, grid, dim3(GB_THREADS), 0, s, sel[0], L.map);
+        switch (variant) {
+        case 0: hipLaunchKernelGGL((k_push_gather<PG_SMALL, float, 4>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map); break;
+        case 1: hipLaunchKernelGGL((k_push_gather<PG_SMALL, __half, 4>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map); break;
+        case 2: hipLaunchKernelGGL((k_push_gather<PG_SMALL, float, 5>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map); break;
+        case 3: hipLaunchKernelGGL((k_push_gather<PG_SMALL, __half, 5>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map); break;
+        case 4: hipLaunchKernelGGL((k_push_gather<PG_SMALL, float, 0>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map); break;
+        case 5: hipLaunchKernelGGL((k_push_gather<PG_SMALL, __half, 0>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map); break;
+        case 6: hipLaunchKernelGGL((k_push_gather<256, float, 0>), grid, dim3(256), 0, s, sel[0], L.map); break;
+        case 7: hipLaunchKernelGGL((k_push_gather<256, __half, 0>), grid, dim3(256), 0, s, sel[0], L.map); break;
+        case 8: hipLaunchKernelGGL(k_bucket_gather<float>, grid, dim3(GB_THREADS), 0, s, sel[0], L.map); break;
+        default: hipLaunchKernelGGL(k_bucket_gather<__half>, grid, dim3(GB_THREADS), 0, s, sel[0], L.map); break;
+        }
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

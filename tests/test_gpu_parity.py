@@ -213,12 +213,14 @@ def test_host_topology_path(gpu):
     assert zeros == 0 and rel <= REL_TOL
 
 
-@pytest.mark.parametrize("variant", ["large_radius", "orphans", "subcell2", "radius3"])
+@pytest.mark.parametrize("variant", ["large_radius", "orphans", "subcell2", "radius3", "radius2.25"])
 def test_insert_paths(gpu, variant):
     """Both binning paths against the oracle: the pull path (own-subcell
     buckets) and the push path it falls back to for points outside every brick
     (topology built from a subset), neighbourhoods wider than one brick, and
-    subcell sizes with more than 32 subcells per brick."""
+    subcell sizes with more than 32 subcells per brick; radii 2, 2.25 and 3
+    take the scatter gather's three variants (four or five voxels per axis
+    of a point's box, or any number)."""
     import oracle as O
     pts = _sphere(6000, 31)
     n_topo, r_topo, r_ins, scs = len(pts), 2.0, 2.0, 4
@@ -230,6 +232,8 @@ def test_insert_paths(gpu, variant):
         scs = 2
     elif variant == "radius3":
         r_topo = r_ins = 3.0
+    elif variant == "radius2.25":             # five voxels per axis of a point's box
+        r_topo = r_ins = 2.25
     v = gpu.VolumeGVDB(0)
     v.Configure(*H.TREE)
     v.SetChannelDefault(16, 16, 1)
