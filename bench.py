@@ -297,11 +297,11 @@ def run_batched(args, grids):
                 speedup=tp / tb, same_images=bool(same), render_tuning=dict(best=tuned[0], ms=tuned[1]))
 
 
-def run_cpu_baseline(sample_stride=4, render_w=512):
+def run_cpu_baseline(sample_stride=1, render_w=1024):
     """The oracle's host path (host ActivateSpace topology, UpdateAtlas, subcell
-    insert, FP32 gather, apron; FP32 march) on a bounded sample: every
-    `sample_stride`-th point of the same workload, and the same isosurface
-    scene at render_w x render_w."""
+    insert, FP32 gather, apron; FP32 march) on every `sample_stride`-th point
+    of the same workload (1: all of it, a few seconds on 16 cores) and the
+    same isosurface scene at render_w x render_w."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from gvdb_amd import workloads as W
@@ -324,8 +324,10 @@ def run_cpu_baseline(sample_stride=4, render_w=512):
     o.render(blk, tf, 4, 32)
     t3 = time.perf_counter()
     return dict(points=len(pts), t_splat=t1 - t0, t_render=t3 - t2, rays=render_w * render_w,
-                threads=O.max_threads(), sample="every %dth point of the workload (%d points), %dx%d render" % (
-                    sample_stride, len(pts), render_w, render_w))
+                threads=O.max_threads(),
+                sample=("the whole workload (%d points), %dx%d render" % (len(pts), render_w, render_w)
+                        if sample_stride == 1 else "every %dth point of the workload (%d points), %dx%d render" % (
+                            sample_stride, len(pts), render_w, render_w)))
 
 
 def copy_calibration(mbytes):
