@@ -36,6 +36,24 @@ __device__ inline const A& multi_rec(const MultiMap& m, int g) {
     return reinterpret_cast<const A*>(m.table ? m.table : static_cast<const void*>(m.itab))[g];
 }
 
+// Launches set up with inline_ok = false keep first[] and the records in
+// device memory; kernels that index them only through these pointers never
+// address the inline arrays of the by-value map (which a runtime index can
+// make the compiler copy to scratch, per lane).
+template <class A>
+__device__ inline const A& multi_rec_dev(const MultiMap& m, int g) {
+    return static_cast<const A*>(m.table)[g];
+}
+__device__ inline int multi_grid_dev(const MultiMap& m, uint32_t b) {
+    int lo = 0, hi = m.G - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (m.first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ inline int multi_grid(const MultiMap& m, uint32_t b) {
     const uint32_t* first = multi_first(m);
     int lo = 0, hi = m.G - 1;
@@ -102,11 +120,13 @@ struct MultiLaunch {
 };
 
 hipError_t multi_setup_raw(Arena* ar, const void* args, size_t rec, const uint32_t* nb, int G, hipStream_t s,
-                           MultiLaunch& out);
+                           MultiLaunch& out,
+                           bool inline_ok = true);
 
 template <typename A>
-inline hipError_t multi_setup(Arena* ar, const A* args, const uint32_t* nb, int G, hipStream_t s, MultiLaunch& out) {
-    return multi_setup_raw(ar, args, sizeof(A), nb, G, s, out);
+inline hipError_t multi_setup(Arena* ar, const A* args, const uint32_t* nb, int G, hipStream_t s, MultiLaunch& out,
+                              bool inline_ok = true) {
+    return multi_setup_raw(ar, args, sizeof(A), nb, G, s, out, inline_ok);
 }
 
 }  // namespace gvdb

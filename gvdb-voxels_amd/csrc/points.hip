@@ -1387,16 +1387,28 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
 
 // Written out (not GVDB_MULTI_T) to ask for eight waves per SIMD (at most 64
 // VGPRs).
-template <int NT, typename AT, int LS>
+// MULTI = false: one grid, its record by value (launched when the launch
+// covers one grid), so the single-grid kernel does not carry the registers of
+// the multi-grid record lookup (63 VGPRs without spills instead of 64 with).
+template <int NT, typename AT, int LS, bool MULTI>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_push_gather(BucketGatherArgs a0,
                                                                                               MultiMap m) {
-    if (m.G <= 1) {
+    if (!MULTI) {
         k_push_gather_body<NT, AT, LS>(a0, blockIdx.x, gridDim.x);
         return;
     }
-    const int g = multi_grid(m, blockIdx.x);
-    const uint32_t* first = multi_first(m);
-    k_push_gather_body<NT, AT, LS>(multi_rec<BucketGatherArgs>(m, g), blockIdx.x - first[g], first[g + 1] - first[g]);
+    // several grids: first[] and the records in device memory (set up with
+    // inline_ok = false); the record copied once, from a wave-uniform index
+    const int g = __builtin_amdgcn_readfirstlane(multi_grid_dev(m, blockIdx.x));
+    const BucketGatherArgs a = multi_rec_dev<BucketGatherArgs>(m, g);
+    const uint32_t f0 = m.first[g], f1 = m.first[g + 1];
+    k_push_gather_body<NT, AT, LS>(a, blockIdx.x - f0, f1 - f0);
+}
+
+template <int NT, typename AT, int LS>
+static void launch_push_gather(dim3 grid, hipStream_t s, const BucketGatherArgs& a0, const MultiMap& m) {
+    if (m.G > 1) hipLaunchKernelGGL((k_push_gather<NT, AT, LS, true>), grid, dim3(NT), 0, s, a0, m);
+    else hipLaunchKernelGGL((k_push_gather<NT, AT, LS, false>), grid, dim3(NT), 0, s, a0, m);
 }
 
 // The push gather's z slices per candidate, as the kernel computes it.
@@ -1429,19 +1441,20 @@ hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* a
         }
         if (sel.empty()) continue;
         MultiLaunch L;
-        hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
+        // the push gather's multi-grid kernels read device tables only
+        hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L, kind == 4);
         if (e != hipSuccess) return e;
         if (L.blocks == 0) continue;
         const dim3 grid(L.blocks);
         switch (variant) {
-        case 0: hipLaunchKernelGGL((k_push_gather<PG_SMALL, float, 4>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map); break;
-        case 1: hipLaunchKernelGGL((k_push_gather<PG_SMALL, __half, 4>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map); break;
-        case 2: hipLaunchKernelGGL((k_push_gather<PG_SMALL, float, 5>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map); break;
-        case 3: hipLaunchKernelGGL((k_push_gather<PG_SMALL, __half, 5>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map); break;
-        case 4: hipLaunchKernelGGL((k_push_gather<PG_SMALL, float, 0>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map); break;
-        case 5: hipLaunchKernelGGL((k_push_gather<PG_SMALL, __half, 0>), grid, dim3(PG_SMALL), 0, s, sel[0], L.map); break;
-        case 6: hipLaunchKernelGGL((k_push_gather<256, float, 0>), grid, dim3(256), 0, s, sel[0], L.map); break;
-        case 7: hipLaunchKernelGGL((k_push_gather<256, __half, 0>), grid, dim3(256), 0, s, sel[0], L.map); break;
+        case 0: launch_push_gather<PG_SMALL, float, 4>(grid, s, sel[0], L.map); break;
+        case 1: launch_push_gather<PG_SMALL, __half, 4>(grid, s, sel[0], L.map); break;
+        case 2: launch_push_gather<PG_SMALL, float, 5>(grid, s, sel[0], L.map); break;
+        case 3: launch_push_gather<PG_SMALL, __half, 5>(grid, s, sel[0], L.map); break;
+        case 4: launch_push_gather<PG_SMALL, float, 0>(grid, s, sel[0], L.map); break;
+        case 5: launch_push_gather<PG_SMALL, __half, 0>(grid, s, sel[0], L.map); break;
+        case 6: launch_push_gather<256, float, 0>(grid, s, sel[0], L.map); break;
+        case 7: launch_push_gather<256, __half, 0>(grid, s, sel[0], L.map); break;
         case 8: hipLaunchKernelGGL(k_bucket_gather<float>, grid, dim3(GB_THREADS), 0, s, sel[0], L.map); break;
         default: hipLaunchKernelGGL(k_bucket_gather<__half>, grid, dim3(GB_THREADS), 0, s, sel[0], L.map); break;
         }

@@ -41,7 +41,7 @@ void* Arena::push(const void* src, size_t n, hipStream_t s, hipError_t& e) {
 }
 
 hipError_t multi_setup_raw(Arena* ar, const void* args, size_t rec, const uint32_t* nb, int G, hipStream_t s,
-                           MultiLaunch& out) {
+                           MultiLaunch& out, bool inline_ok) {
     out.map.G = 1;
     out.map.first = nullptr;
     out.map.table = nullptr;
@@ -61,7 +61,7 @@ hipError_t multi_setup_raw(Arena* ar, const void* args, size_t rec, const uint32
     first[G] = uint32_t(acc);
     out.blocks = uint32_t(acc);
     if (acc == 0) return hipSuccess;
-    if (G <= MULTI_INLINE && rec * size_t(G) <= size_t(MULTI_INLINE_BYTES)) {
+    if (inline_ok && G <= MULTI_INLINE && rec * size_t(G) <= size_t(MULTI_INLINE_BYTES)) {
         out.map.G = G;
         memcpy(out.map.ifirst, first.data(), first.size() * 4);
         memcpy(out.map.itab, args, rec * size_t(G));
