@@ -388,7 +388,7 @@ def main():
         render_tuning=g["render_tuning"],
         kernel_ms=dict(gather=g["gather_ms"], apron=g["apron_ms"], render=g["render_ms"]),
         roofline=dict(bound="hbm", kernel="gather", achieved=gather_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=gather_gbs / HBM_PEAK_GBS, traffic=profiled_traffic("k_push_gather<128, float, 4>"),
+                      frac=gather_gbs / HBM_PEAK_GBS, traffic=profiled_traffic("k_push_gather<128, float, 4, false>"),
                       algorithmic_bytes=ab["gather"]),
         roofline_apron=dict(bound="hbm", achieved=apron_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
                             frac=apron_gbs / HBM_PEAK_GBS, traffic=profiled_traffic("k_apron<float>"),
