@@ -149,3 +149,52 @@ def test_batch_reports_failing_grid(gpu):
     with pytest.raises(gpu.GVDBError) as e:
         B.RebuildTopology([good, bad], [500, 10], 2.0)
     assert "grid 1" in str(e.value)
+
+
+def test_batch_gather_variants_match_single_grid_calls(gpu):
+    """Radii 2, 2.25, 3 and 5 take the scatter gather's variants (four or five
+    voxels per axis of a point's box, any number, 256 threads); two grids of
+    each in one batch must leave every atlas exactly as the one-grid calls
+    (whose kernels are compiled separately from the multi-grid ones) do."""
+    from gvdb_amd import workloads as W
+    radii = [2.0, 2.25, 3.0, 5.0, 2.0, 2.25, 3.0, 5.0]
+    clouds = [W.sphere_shell(2500 + 500 * k, seed=90 + k) for k in range(len(radii))]
+    B = gpu.GridBatch(0, len(radii))
+    vols = []
+    for pts in clouds:
+        v = gpu.VolumeGVDB(0)
+        v.Configure(*H.TREE)
+        v.SetChannelDefault(16, 16, 1)
+        v.AddChannel(0, gpu.T_FLOAT, 1)
+        v.SetPoints(pts)
+        vols.append(v)
+    n_all = [len(p) for p in clouds]
+    B.RebuildTopology(vols, n_all, radii)
+    B.FinishTopology(vols, False, True)
+    B.UpdateAtlas(vols)
+    B.ClearChannel(vols, 0)
+    lens = B.InsertPointsSubcell(vols, 4, n_all, radii)
+    B.GatherDensity(vols, 4, lens, radii, 0)
+    B.UpdateApron(vols, 0, 0.0)
+    B.synchronize()
+    for k, (pts, r) in enumerate(zip(clouds, radii)):
+        s = gpu.VolumeGVDB(0)
+        s.Configure(*H.TREE)
+        s.SetChannelDefault(16, 16, 1)
+        s.AddChannel(0, gpu.T_FLOAT, 1)
+        s.SetPoints(pts)
+        s.RebuildTopology(len(pts), r, H.ZERO3)
+        s.FinishTopology(False, True)
+        s.UpdateAtlas()
+        s.ClearChannel(0)
+        n = s.InsertPointsSubcell(4, len(pts), r, H.ZERO3)
+        s.GatherDensity(4, len(pts), r, H.ZERO3, n, 0, 0, False)
+        s.UpdateApron(0, 0.0)
+        assert lens[k] == n, r
+        H.assert_pools_equal(vols[k], s)
+        a, b = vols[k].atlas(), s.atlas()
+        assert a.shape == b.shape and np.array_equal(a, b), "radius %g: batched atlas differs" % r
+        s.close()
+    for v in vols:
+        v.close()
+    B.close()
