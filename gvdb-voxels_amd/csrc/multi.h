@@ -44,26 +44,25 @@ template <class A>
 __device__ inline const A& multi_rec_dev(const MultiMap& m, int g) {
     return static_cast<const A*>(m.table)[g];
 }
-__device__ inline int multi_grid_dev(const MultiMap& m, uint32_t b) {
-    int lo = 0, hi = m.G - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (m.first[mid] <= b) lo = mid;
-        else hi = mid - 1;
+// The grid of workgroup b: the last g with first[g] <= b (first[] ascending,
+// first[0] = 0), counted 64 entries at a time, one load per lane and a
+// ballot, instead of a binary search's chain of dependent loads (which a
+// workgroup with little work pays in full).  Called by whole wave64s, every
+// lane active.
+__device__ inline int multi_grid_in(const uint32_t* first, int G, uint32_t b) {
+    const int lane = int(threadIdx.x & 63u);
+    int g = 0;
+    for (int base = 1; base < G; base += 64) {
+        const int i = base + lane;
+        const uint64_t bal = __ballot(i < G && first[i] <= b);
+        g += __popcll(bal);
+        if (bal != ~0ull) break;
     }
-    return lo;
+    return g;
 }
+__device__ inline int multi_grid_dev(const MultiMap& m, uint32_t b) { return multi_grid_in(m.first, m.G, b); }
 
-__device__ inline int multi_grid(const MultiMap& m, uint32_t b) {
-    const uint32_t* first = multi_first(m);
-    int lo = 0, hi = m.G - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (first[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
+__device__ inline int multi_grid(const MultiMap& m, uint32_t b) { return multi_grid_in(multi_first(m), m.G, b); }
 
 // Workgroups are handed to the 8 XCDs of the device in turn; this maps the
 // b-th workgroup of a launch to the brick order so that each XCD (with its
