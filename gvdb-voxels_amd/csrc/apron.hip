@@ -142,6 +142,23 @@ __device__ inline void k_apron_body(const ApronArgs& ap, uint32_t bid, uint32_t)
 
 GVDB_MULTI_T(k_apron, typename AT, AT, ApronArgs, 512)
 
+// Several grids: APRON_RUN bricks of one grid per workgroup, one after the
+// other, so the grid lookup and the record loads before the first brick are
+// paid once per run (a brick alone is one voxel per thread).
+static const int APRON_RUN = 4;
+template <typename AT>
+__global__ __launch_bounds__(512) void k_apron_runs(ApronArgs, MultiMap m) {
+    const int g = multi_grid(m, blockIdx.x);
+    const uint32_t* first = multi_first(m);
+    const ApronArgs& a = multi_rec<ApronArgs>(m, g);
+    const uint32_t b0 = (blockIdx.x - first[g]) * uint32_t(APRON_RUN);
+    for (int k = 0; k < APRON_RUN; k++) {
+        if (b0 + uint32_t(k) >= uint32_t(a.n)) break;
+        if (k) __syncthreads();            // the previous brick is done with its offsets
+        k_apron_body<AT>(a, b0 + uint32_t(k), 0);
+    }
+}
+
 hipError_t launch_apron_multi(const ApronArgs* a, int G, Arena* ar, hipStream_t s) {
     for (int half = 0; half < 2; half++) {
         std::vector<ApronArgs> sel;
@@ -150,14 +167,18 @@ hipError_t launch_apron_multi(const ApronArgs* a, int G, Arena* ar, hipStream_t 
             if (a[g].t.apron != 1) return hipErrorInvalidValue;
             if ((a[g].atlas_type == T_HALF) != bool(half) || a[g].n <= 0) continue;
             sel.push_back(a[g]);
-            nb.push_back(uint32_t(a[g].n));
         }
         if (sel.empty()) continue;
+        // one grid: a workgroup per brick; several: runs of APRON_RUN bricks
+        const bool runs = sel.size() > 1;
+        for (const ApronArgs& x : sel) nb.push_back(runs ? uint32_t((x.n + APRON_RUN - 1) / APRON_RUN) : uint32_t(x.n));
         MultiLaunch L;
         hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
         if (e != hipSuccess) return e;
         if (L.blocks == 0) continue;
-        if (half) hipLaunchKernelGGL(k_apron<__half>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
+        if (runs && half) hipLaunchKernelGGL(k_apron_runs<__half>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
+        else if (runs) hipLaunchKernelGGL(k_apron_runs<float>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
+        else if (half) hipLaunchKernelGGL(k_apron<__half>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
         else hipLaunchKernelGGL(k_apron<float>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
