@@ -28,13 +28,20 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "points splatted/s + Mrays/s on 1 MI355X; batched p50/p99 latency vs plain form"
 
 
-def algorithmic_bytes(v, n_points):
-    """Per-launch algorithmic HBM bytes of the gather and apron kernels.
-    gather: N*16 (bucketed points, read once) + B*8*8 (bucket counts and
-            offsets) + B*27*4 (neighbour rows) + B*512*4 (interior writes)
-    apron:  B*27*8 (neighbour brick offsets) + B*488*4 read + B*488*4 written."""
+def algorithmic_bytes(v, n_points, entries):
+    """Per-launch algorithmic HBM bytes of the gather and apron kernels, as
+    the path's measurement plan states them (E subcell entries, S = 8 B
+    subcells, B bricks, N points):
+      gather: E*12 (entries read) + S*8 (subcell counts/offsets) + B*512*4
+              (interior writes);
+      apron:  B*488*4 read + B*488*4 written.
+    `gather_design` is this build's own minimum for its gather, which reads
+    each bucketed point once instead of one entry per (point, subcell):
+    N*16 + B*(64 + 108 + 2048)."""
     B = len(v.pool(0, 0))
-    return dict(B=B, gather=n_points * 16 + B * (64 + 108 + 2048), apron=B * (216 + 488 * 8))
+    S = 8 * B
+    return dict(B=B, S=S, E=int(entries), gather=int(entries) * 12 + S * 8 + B * 512 * 4, apron=B * 488 * 8,
+                gather_design=n_points * 16 + B * (64 + 108 + 2048))
 
 
 def profiled_traffic(kernel):
@@ -110,12 +117,12 @@ def run_gpu(args):
         v.SetPoints(pts)
         v.synchronize()
         t_up.append(time.perf_counter() - t0)
-    ab = algorithmic_bytes(v, n)
-    gather_ms = float(np.mean(kg))
-    apron_ms = float(np.mean(ka))
-    render_ms = float(np.mean(kr))
-    stages = {k: float(np.mean([s[k] for s in st])) for k in st[0]}
-    out = dict(n=n, entries=entries, t_splat=float(np.mean(t_splat)), t_render=float(np.mean(t_render)), t_all=t_all,
+    ab = algorithmic_bytes(v, n, entries)
+    gather_ms = float(np.median(kg))
+    apron_ms = float(np.median(ka))
+    render_ms = float(np.median(kr))
+    stages = {k: float(np.median([s[k] for s in st])) for k in st[0]}
+    out = dict(n=n, entries=entries, t_splat=float(np.median(t_splat)), t_render=float(np.median(t_render)), t_all=t_all,
                t_upload=float(np.median(t_up)), render_tuning=dict(best=tuned[0], ms=tuned[1]),
                ab=ab,
                gather_ms=gather_ms, apron_ms=apron_ms, render_ms=render_ms, stages=stages,
@@ -134,8 +141,8 @@ def _grid_scene(pts, w):
 
 
 def run_serving(args, grids):
-    """Serving: the 64 grids of the batched workload stay resident; a stream
-    of requests (70% renders of 256-1024 pixels a side from a random camera
+    """Serving: the serving workload's grids (up to 200, initial points
+    log-uniform in [50k, 4M]) stay resident; a stream of requests (70% renders of 256-1024 pixels a side from a random camera
     on a sphere of 1.5x the grid's extent, 30% inserts of 20k-1M points
     near the grid's existing points, each followed by the grid's rebuild)
     is served plainly (one request at a time, one-grid calls) and with
@@ -146,7 +153,7 @@ def run_serving(args, grids):
     from gvdb_amd import workloads as W
     g = np.random.default_rng(5)
     n_req = args.serve_requests
-    S = G.Server(0, 200 << 30, 64)
+    S = G.Server(0, 250 << 30, 64)
     vols = []
     for kind, pts in grids:
         v = G.VolumeGVDB(0)
@@ -192,8 +199,9 @@ def run_serving(args, grids):
     arrivals -= arrivals[0]
     lp, wp = serve(S.PLAIN, arrivals)
     lc, wc = serve(S.CONTINUOUS, arrivals)
-    out = dict(workload="64 resident grids (batched workload), %d requests: 70%% EA renders 256-1024 px a side, "
-                        "30%% inserts of 20k-1M points + rebuild" % n_req,
+    out = dict(workload="%d resident grids, initial points log-uniform in [50k, 4M] (%d total), %d requests: 70%% EA "
+                        "renders 256-1024 px a side, 30%% inserts of 20k-1M points + rebuild"
+                        % (len(grids), sum(len(p) for _, p in grids), n_req),
                requests=n_req, renders=sum(1 for p in plan if p[0] == "render"),
                saturation=dict(plain_requests_per_s=n_req / wp0, continuous_requests_per_s=n_req / wc0,
                                speedup=wp0 / wc0),
@@ -297,37 +305,98 @@ def run_batched(args, grids):
                 speedup=tp / tb, same_images=bool(same), render_tuning=dict(best=tuned[0], ms=tuned[1]))
 
 
-def run_cpu_baseline(sample_stride=1, render_w=1024):
-    """The oracle's host path (host ActivateSpace topology, UpdateAtlas, subcell
-    insert, FP32 gather, apron; FP32 march) on every `sample_stride`-th point
-    of the same workload (1: all of it, a few seconds on 16 cores) and the
-    same isosurface scene at render_w x render_w."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_cpu_baseline(runs=5, render_w=1024):
+    """The oracle's host path, timed on this host's cores: host ActivateSpace
+    topology (single-threaded by construction), FinishTopology, UpdateAtlas,
+    ClearChannel, subcell insert, FP32 gather and apron (OpenMP over
+    subcells / bricks), and the FP32 isosurface march (OpenMP over pixels).
+    All cores (OMP_NUM_THREADS, or the OpenMP default): the whole workload.
+    One thread: the slab's first eighth in x (x < 106, with its plane strip),
+    so every stage scales down together.  One warm-up, then `runs` runs;
+    per-stage medians."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from gvdb_amd import workloads as W
-    pts = np.ascontiguousarray(W.slab_plane()[::sample_stride])
-    O.set_threads(os.cpu_count() or 1)
-    o = O.Oracle()
-    t0 = time.perf_counter()
-    o.rebuild_topology_cpu(pts)
-    o.finish_topology()
-    o.update_atlas()
-    o.clear_channel()
-    rc, ln = o.insert_points_subcell(pts, 2.0)
-    o.gather_density(2.0, precision=32)
-    o.update_apron(0.0, 32)
-    t1 = time.perf_counter()
+    full = np.ascontiguousarray(W.slab_plane())
+    crop = np.ascontiguousarray(full[full[:, 0] < 106.0])
     sc = W.iso_scene(render_w, render_w)
     blk = O.scene_block(sc)
     tf = O.linear_tf(*sc["tf"])
-    t2 = time.perf_counter()
-    o.render(blk, tf, 4, 32)
-    t3 = time.perf_counter()
-    return dict(points=len(pts), t_splat=t1 - t0, t_render=t3 - t2, rays=render_w * render_w,
-                threads=O.max_threads(),
-                sample=("the whole workload (%d points), %dx%d render" % (len(pts), render_w, render_w)
-                        if sample_stride == 1 else "every %dth point of the workload (%d points), %dx%d render" % (
-                            sample_stride, len(pts), render_w, render_w)))
+    names = ("topology", "atlas", "insert", "gather", "apron")
+
+    def leg(pts, threads):
+        O.set_threads(threads)
+        splat, stages, render = [], [], []
+        for k in range(runs + 1):
+            o = O.Oracle(fp64=False)
+            t0 = time.perf_counter()
+            o.rebuild_topology_cpu(pts)
+            o.finish_topology()
+            o.update_atlas()
+            o.clear_channel()
+            o.insert_points_subcell(pts, 2.0)
+            o.gather_density(2.0, 4, 32)
+            o.update_apron(0.0, 32)
+            t1 = time.perf_counter()
+            o.render(blk, tf, 4, 32)
+            t2 = time.perf_counter()
+            if k:
+                splat.append(t1 - t0)
+                render.append(t2 - t1)
+                stages.append([o.stage_ms(i) for i in range(5)])
+            del o
+        st = np.median(np.asarray(stages), axis=0)
+        ts, tr = float(np.median(splat)), float(np.median(render))
+        return dict(points=len(pts), threads=O.max_threads(), splat_s=ts, render_s=tr, points_per_s=len(pts) / ts,
+                    mrays_per_s=render_w * render_w / tr / 1e6, stage_ms={n: float(x) for n, x in zip(names, st)})
+
+    allc = leg(full, int(os.environ.get("OMP_NUM_THREADS", "0")) or O.max_threads())
+    one = leg(crop, 1)
+    return dict(all_cores=allc, one_thread=one, runs=runs, cpu_model=cpu_model(), nproc=os.cpu_count(),
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                sample=("all cores: the whole workload (%d points) and the %dx%d isosurface, median of %d runs after "
+                        "one warm-up; one thread: the points with x < 106 (%d points) and the same render"
+                        % (len(full), render_w, render_w, runs, len(crop))))
+
+
+def stream_copy_gbs(mib=2048, reps=10):
+    """Achievable HBM bandwidth: a device-to-device copy (the runtime's copy
+    kernel) of `mib` MiB timed with HIP events; bytes moved = 2 x size."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = int(mib) << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    assert hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+    best = []
+    for k in range(reps + 2):
+        assert hip.hipEventRecord(e0, None) == 0
+        assert hip.hipMemcpyAsync(b, a, ctypes.c_size_t(n), 3, None) == 0     # hipMemcpyDeviceToDevice
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float(0)
+        assert hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        if k >= 2:
+            best.append(ms.value)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return 2.0 * n / (float(np.median(best)) * 1e-3) / 1e9
 
 
 def copy_calibration(mbytes):
@@ -360,6 +429,8 @@ def main():
     ap.add_argument("--batch-width", type=int, default=512)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving comparison")
     ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-grids", type=int, default=200, help="resident grids of the serving workload")
+    ap.add_argument("--no-stream", action="store_true", help="skip the copy-bandwidth measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -389,24 +460,33 @@ def main():
         kernel_ms=dict(gather=g["gather_ms"], apron=g["apron_ms"], render=g["render_ms"]),
         roofline=dict(bound="hbm", kernel="gather", achieved=gather_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=gather_gbs / HBM_PEAK_GBS, traffic=profiled_traffic("k_push_gather<128, float, 4, false>"),
-                      algorithmic_bytes=ab["gather"]),
-        roofline_apron=dict(bound="hbm", achieved=apron_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
+                      algorithmic_bytes=ab["gather"], bytes_formula="E*12 + S*8 + B*512*4",
+                      design_bytes=ab["gather_design"], design_frac=ab["gather_design"] / (g["gather_ms"] * 1e-3) / 1e9 /
+                      HBM_PEAK_GBS),
+        roofline_apron=dict(bound="hbm", kernel="apron", achieved=apron_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
                             frac=apron_gbs / HBM_PEAK_GBS, traffic=profiled_traffic("k_apron<float>"),
-                            algorithmic_bytes=ab["apron"]),
+                            algorithmic_bytes=ab["apron"], bytes_formula="B*488*8"),
         batched=None,
     )
-    if not (args.no_batched and args.no_serve):
+    if not args.no_stream:
+        cp = stream_copy_gbs()
+        line["copy_bandwidth"] = dict(achieved=cp, peak=HBM_PEAK_GBS, unit="GB/s", frac=cp / HBM_PEAK_GBS,
+                                      kernel="device-to-device hipMemcpyAsync, 2 GiB, median of 10",
+                                      gather_frac_of_copy=gather_gbs / cp, apron_frac_of_copy=apron_gbs / cp)
+    if not args.no_batched:
         from gvdb_amd import workloads as W
-        grids = W.batch_grids()
-        if not args.no_batched:
-            line["batched"] = run_batched(args, grids)
-        if not args.no_serve:
-            line["serving"] = run_serving(args, grids)
+        line["batched"] = run_batched(args, W.batch_grids())
+    if not args.no_serve:
+        from gvdb_amd import workloads as W
+        line["serving"] = run_serving(args, W.serve_grids(count=args.serve_grids))
     if not args.no_cpu:
         c = run_cpu_baseline()
-        line["cpu_baseline"] = dict(value=c["points"] / c["t_splat"], unit="points/s", cores=c["threads"], kind="port",
-                                    sample=c["sample"], mrays_per_s=c["rays"] / c["t_render"] / 1e6,
-                                    splat_s=c["t_splat"], render_s=c["t_render"])
+        a = c["all_cores"]
+        line["cpu_baseline"] = dict(value=a["points_per_s"], unit="points/s", cores=a["threads"], kind="port",
+                                    sample=c["sample"], mrays_per_s=a["mrays_per_s"], splat_s=a["splat_s"],
+                                    render_s=a["render_s"], stage_ms=a["stage_ms"], one_thread=c["one_thread"],
+                                    cpu_model=c["cpu_model"], nproc=c["nproc"], omp_num_threads=c["omp_num_threads"],
+                                    runs=c["runs"])
     print(json.dumps(line), flush=True)
 
 

@@ -336,6 +336,7 @@ __device__ inline void k_gather_body(const GatherArgs& a, uint32_t bid, uint32_t
     }
     if (!active) return;
     const Node* nd = t.nodes[0] + sc_nid[sc];
+    if (!nd->mFlags || nd->mValue[0] < 0) return;   // no atlas slot to write
     int ax = nd->mValue[0] + (vx - nd->mPos[0]);
     int ay = nd->mValue[1] + (vy - nd->mPos[1]);
     int az = nd->mValue[2] + (vz - nd->mPos[2]);
@@ -418,6 +419,7 @@ __device__ inline void k_gather_rows_body(const GatherArgs& a, uint32_t bid, uin
     }
     if (g != 0) return;
     const Node* nd = t.nodes[0] + sc_nid[sc];
+    if (!nd->mFlags || nd->mValue[0] < 0) return;   // no atlas slot to write
     const int vy = sy + ry, vz = sz + rz;
     const u64 off = atlas_offset(t, nd->mValue[0] + (sx - nd->mPos[0]), nd->mValue[1] + (vy - nd->mPos[1]),
                                  nd->mValue[2] + (vz - nd->mPos[2]));
@@ -1001,6 +1003,7 @@ __device__ inline void k_bucket_gather_body(const BucketGatherArgs& a, uint32_t 
     const int brick = int(bid);
     const Node* bn = t.nodes[0] + brick;
     if (bn->mParent == ID_UNDEF64) return;        // unlinked leaves have no subcells
+    if (!bn->mFlags || bn->mValue[0] < 0) return;  // no atlas slot to write (uniform per brick)
     const float r = a.radius;
     const int R = a.R, span = 2 + 2 * R, ncell = span * span * span;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1191,6 +1194,7 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
     const int brick = int(bid);
     const Node* bn = t.nodes[0] + brick;
     if (bn->mParent == ID_UNDEF64) return;        // unlinked leaves have no subcells
+    if (!bn->mFlags || bn->mValue[0] < 0) return;  // no atlas slot to write (uniform per brick)
     const int tid = threadIdx.x, lane = tid & 63;
     // voxel v = tid + 256 j (j = 0, 1) of the brick: x = v & 7, y = (v >> 3) & 7, z = v >> 6
     for (int v = tid; v < 512; v += NT) S.key[(v >> 6) * PG_ZSTRIDE + (v & 63)] = ~0ull;

@@ -201,6 +201,42 @@ int Volume::load_vbx(const char* path, int force_major, int force_minor) {
                 p.lastEle = p.usedNum = u64(cnt);
             }
         if (!fp.ok) return fail(GVDB_ERR_IO, "LoadVBX: truncated pools");
+        // The kernels walk the pools without bounds checks, so every reference
+        // a file holds must name a node or child list that exists: child list
+        // ids, the child slots of each list, and parents.
+        for (int n = 0; n < levels; n++) {
+            const Pool& p0 = pool_[0][n];
+            for (u64 i = 0; i < p0.lastEle; i++) {
+                const Node* nd = reinterpret_cast<const Node*>(p0.host.data() + i * p0.stride);
+                const u64 par = nd->mParent, cl = nd->mChildList;
+                if (int(nd->mLev) != n) return fail(GVDB_ERR_IO, "LoadVBX: node level differs from its pool");
+                // UpdateAtlas rewrites the corners of used leaves; an unused
+                // one keeps its own, which must stay inside the atlas
+                if (n == 0 && !nd->mFlags)
+                    for (int a = 0; a < 3; a++)
+                        if (nd->mValue[a] < -1 || nd->mValue[a] >= axisres[a])
+                            return fail(GVDB_ERR_IO, "LoadVBX: unused brick's atlas corner outside the atlas");
+                if (par != ID_UNDEF64 && par != ID_UNDEFL &&
+                    (n + 1 >= levels || (par & 0xFF) != 0 || int((par >> 8) & 0xFF) != n + 1 ||
+                     (par >> 16) >= pool_[0][n + 1].lastEle))
+                    return fail(GVDB_ERR_IO, "LoadVBX: parent reference outside the pools");
+                if (n == 0 || cl == ID_UNDEFL) continue;
+                if ((cl & 0xFF) != 1 || int((cl >> 8) & 0xFF) != n || (cl >> 16) >= pool_[1][n].lastEle)
+                    return fail(GVDB_ERR_IO, "LoadVBX: child list reference outside the pools");
+            }
+            if (n == 0) continue;
+            const Pool& p1 = pool_[1][n];
+            const u64 slots = voxcnt(n);
+            for (u64 i = 0; i < p1.lastEle * slots; i++) {
+                const u64 c = reinterpret_cast<const u64*>(p1.host.data())[i];
+                if (c != ID_UNDEF64 && ((c & 0xFF) != 0 || int((c >> 8) & 0xFF) != n - 1 ||
+                                        (c >> 16) >= pool_[0][n - 1].lastEle))
+                    return fail(GVDB_ERR_IO, "LoadVBX: child slot outside the pools");
+            }
+        }
+        if (root != ID_UNDEFL && ((root & 0xFF) != 0 || int((root >> 8) & 0xFF) >= levels ||
+                                  (root >> 16) >= pool_[0][(root >> 8) & 0xFF].lastEle))
+            return fail(GVDB_ERR_IO, "LoadVBX: root reference outside the pools");
         root_ = root;
         host_valid_ = true;
         dev_stale_ = true;

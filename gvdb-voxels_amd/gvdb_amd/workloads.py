@@ -12,6 +12,10 @@ Workloads (tree log2 dims {3,3,3,4,5} leaf first, 8^3 bricks, apron 1):
                  where 3-octave value noise exceeds a threshold
   clusters       spherical Gaussian clusters (sigma 20-80) with centres in a
                  2048^3 index space; points outside [0, 2048) are dropped
+  batch_grids    64 grids, point counts log-uniform in [10k, 2M], one of the
+                 three shapes above each (the batched workload)
+  serve_grids    up to 200 grids, point counts log-uniform in [50k, 4M]
+                 (the serving workload's resident grids)
 """
 import numpy as np
 
@@ -111,6 +115,13 @@ def batch_grids(count=64, seed=3, n_lo=10000, n_hi=2000000):
         else:
             out.append(("cloud", noise_cloud(n, seed=s)))
     return out
+
+
+def serve_grids(count=200, seed=5, n_lo=50000, n_hi=4000000):
+    """The serving workload's resident grids: `count` grids with initial
+    point counts log-uniform in [n_lo, n_hi], one of the three shapes each
+    (the batched workload's generator with the serving sizes)."""
+    return batch_grids(count=count, seed=seed, n_lo=n_lo, n_hi=n_hi)
 
 
 # ----------------------------------------------------------------- scenes --

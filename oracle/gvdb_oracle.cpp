@@ -28,7 +28,9 @@
 #include <vector>
 #include <algorithm>
 #include <numeric>
+#include <atomic>
 #include <chrono>
+#include <parallel/algorithm>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -83,7 +85,7 @@ struct Vol {
     float objMin[3] = {0, 0, 0}, objMax[3] = {0, 0, 0};
     int top_lev = 1;
     float epsilon = 0.001f;
-    int undefined_events = 0;
+    std::atomic<int> undefined_events{0};
     // atlas (one float channel; the FP64 shadow copy serves the FP64 oracle)
     int default_axiscnt[3] = {16, 16, 1};
     bool has_atlas = false;
@@ -93,6 +95,7 @@ struct Vol {
     u64 atlas_max = 0, atlas_last = 0, atlas_used = 0;
     std::vector<float> atlasF;
     std::vector<double> atlasD;
+    bool fp64 = true;      // keep the FP64 shadow atlas (off for the timed CPU baseline)
     std::vector<AtlasNode> amap;
     u64 amap_max = 0;
     // InsertPointsSubcell outputs
@@ -331,7 +334,9 @@ static u64 makeKey(int lev, int x, int y, int z) {
 
 // FindUniqueBrick: src/gvdb_volume_gvdb.cpp:843-873, kernels/cuda_gvdb_particles.cuh:336-350,934-943
 static void findUnique(std::vector<u64>& keys, int depth, std::vector<u64>& uniq, std::vector<int>& levCnt) {
-    std::sort(keys.begin(), keys.end());   // gvdbDeviceRadixSort: ascending u64 (src/gvdb_cutils.cu:11-16)
+    // gvdbDeviceRadixSort: ascending u64 (src/gvdb_cutils.cu:11-16); any
+    // correct sort gives the same sequence, so the oracle sorts in parallel
+    __gnu_parallel::sort(keys.begin(), keys.end());
     uniq.clear();
     levCnt.assign(depth, 0);
     for (size_t i = 0; i < keys.size(); i++) {
@@ -458,6 +463,7 @@ static int rebuildTopology(Vol& v, const float* pts, int n, float radius, const 
 
     // gvdbCalcBrickId: kernels/cuda_gvdb_particles.cuh:352-382
     std::vector<u64> keys((size_t)n * depth);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++) {
         float w[3] = {pts[3 * i] + orig[0], pts[3 * i + 1] + orig[1], pts[3 * i + 2] + orig[2]};
         bool neg = w[0] < 0 || w[1] < 0 || w[2] < 0;
@@ -485,7 +491,13 @@ static int rebuildTopology(Vol& v, const float* pts, int n, float radius, const 
 
     if (radius >= 1) {
         // gvdbCalcExtraBrickId: kernels/cuda_gvdb_particles.cuh:467-512
+        // the lookups only read the tree, and the keys are sorted and made
+        // unique afterwards, so points are split over threads
         std::vector<u64> ex;
+#pragma omp parallel
+        {
+        std::vector<u64> mine;
+#pragma omp for schedule(static) nowait
         for (int i = 0; i < n; i++) {
             float w[3] = {pts[3 * i] + orig[0], pts[3 * i + 1] + orig[1], pts[3 * i + 2] + orig[2]};
             if (w[0] < 0 || w[1] < 0 || w[2] < 0) continue;
@@ -501,9 +513,15 @@ static int rebuildTopology(Vol& v, const float* pts, int n, float radius, const 
                     for (nd[1] = bmin[1] / rr * rr; nd[1] <= bmax[1] / rr * rr; nd[1] += rr)
                         for (nd[2] = bmin[2] / rr * rr; nd[2] <= bmax[2] / rr * rr; nd[2] += rr) {
                             if (getPosParentD(v, nd, lev) < 0)
-                                ex.push_back(makeKey(lev, nd[0] / rr, nd[1] / rr, nd[2] / rr));
+                                mine.push_back(makeKey(lev, nd[0] / rr, nd[1] / rr, nd[2] / rr));
                         }
             }
+        }
+        // drop repeats before merging (the set is what counts)
+        std::sort(mine.begin(), mine.end());
+        mine.erase(std::unique(mine.begin(), mine.end()), mine.end());
+#pragma omp critical
+        ex.insert(ex.end(), mine.begin(), mine.end());
         }
         std::vector<u64> xu;
         std::vector<int> xc;
@@ -772,7 +790,7 @@ static void atlasResizeTo(Vol& v, int cz) {
     atlasRes(v, r);
     size_t sz = size_t(r[0]) * r[1] * r[2];
     v.atlasF.resize(sz, 0.f);
-    v.atlasD.resize(sz, 0.0);
+    if (v.fp64) v.atlasD.resize(sz, 0.0);
 }
 
 static void updateAtlas(Vol& v) {
@@ -1597,6 +1615,9 @@ void orc_add_channel(void* h, int apron) {
     v.atlasD.assign(size_t(r[0]) * r[1] * r[2], 0.0);
 }
 
+// Keeps (1, the default) or drops (0) the FP64 shadow atlas; set before the
+// first UpdateAtlas.  Without it only precision 32 runs.
+void orc_set_fp64(void* h, int on) { static_cast<Vol*>(h)->fp64 = on != 0; }
 void orc_set_epsilon(void* h, float eps) { static_cast<Vol*>(h)->epsilon = eps; }
 int orc_undefined_events(void* h) { return static_cast<Vol*>(h)->undefined_events; }
 
@@ -1749,6 +1770,7 @@ void orc_subcell_copy(void* h, int which, void* dst) {
 
 int orc_gather_density(void* h, int scs, float radius, int precision, int accum) {
     Vol& v = *static_cast<Vol*>(h);
+    if (precision == 64 && !v.fp64) return -1;
     if (v.sc_len == 0) return 0;
     auto t0 = std::chrono::steady_clock::now();
     if (precision == 64) gatherT<double>(v, scs, radius, accum != 0, v.atlasD);
@@ -1761,6 +1783,7 @@ int orc_gather_density(void* h, int scs, float radius, int precision, int accum)
 // empty entry list, unlike GatherDensity.
 int orc_gather_level_set(void* h, int scs, float radius, int precision, int accum) {
     Vol& v = *static_cast<Vol*>(h);
+    if (precision == 64 && !v.fp64) return -1;
     if (v.pool[0][0].usedNum == 0) return 0;
     if (precision == 64) gatherT<double>(v, scs, radius, accum != 0, v.atlasD, true);
     else gatherT<float>(v, scs, radius, accum != 0, v.atlasF, true);
@@ -1770,6 +1793,7 @@ int orc_gather_level_set(void* h, int scs, float radius, int precision, int accu
 // GatherLevelSet_FP16 (src/gvdb_volume_gvdb.cpp:5374-5401): no accumulation.
 int orc_gather_level_set_fp16(void* h, int scs, float radius, int precision) {
     Vol& v = *static_cast<Vol*>(h);
+    if (precision == 64 && !v.fp64) return -1;
     if (v.pool[0][0].usedNum == 0) return 0;
     if (!v.sc16) return -3;
     if (precision == 64) gatherT<double>(v, scs, radius, false, v.atlasD, true, true);
@@ -1779,6 +1803,7 @@ int orc_gather_level_set_fp16(void* h, int scs, float radius, int precision) {
 
 int orc_update_apron(void* h, float boundval, int precision) {
     Vol& v = *static_cast<Vol*>(h);
+    if (precision == 64 && !v.fp64) return -1;
     if (v.apron == 0) return 0;
     auto t0 = std::chrono::steady_clock::now();
     if (precision == 64) apronT<double>(v, double(boundval), v.atlasD);
@@ -1792,6 +1817,7 @@ float orc_random3(float a, float b, float c) { return random3(a, b, c); }
 int orc_compute(void* h, int op, int iters, float p1, float p2, float p3, int update_apron, int skip, float boundval,
                 int precision) {
     Vol& v = *static_cast<Vol*>(h);
+    if (precision == 64 && !v.fp64) return -1;
     if (op != 110 && op != 113 && op != 114 && op != 115) return -7;
     for (int it = 0; it < iters; it++) {
         if (precision == 64) computeT<double>(v, op, p1, p2, p3, skip != 0, v.atlasD);
@@ -1837,6 +1863,7 @@ void orc_linear_tf(float t0, float t1, const float* a, const float* b, float* lu
 // capped (may be null): one byte per pixel, 1 where a ray stopped at MAX_ITER
 int orc_render(void* h, const float* scn, const float* tf, int shading, int precision, float* out, uint8_t* capped) {
     Vol& v = *static_cast<Vol*>(h);
+    if (precision == 64 && !v.fp64) return -1;
     const ScnBlock& s = *reinterpret_cast<const ScnBlock*>(scn);
     auto t0 = std::chrono::steady_clock::now();
     if (precision == 64) renderT<double>(v, v.atlasD, s, tf, shading, out, capped);

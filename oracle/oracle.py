@@ -38,7 +38,7 @@ def lib():
         sig = {
             "orc_new": (vp, [i, ip]), "orc_free": (None, [vp]),
             "orc_set_channel_default": (None, [vp, i, i, i]), "orc_add_channel": (None, [vp, i]),
-            "orc_set_epsilon": (None, [vp, f]), "orc_undefined_events": (i, [vp]),
+            "orc_set_epsilon": (None, [vp, f]), "orc_set_fp64": (None, [vp, i]), "orc_undefined_events": (i, [vp]),
             "orc_rebuild_topology": (i, [vp, fp, i, f, fp]), "orc_rebuild_topology_cpu": (i, [vp, fp, i, fp]),
             "orc_accumulate_topology": (i, [vp, fp, i, f, fp, i]), "orc_request_full_rebuild": (None, [vp, i]),
             "orc_pool_used": (ctypes.c_ulonglong, [vp, i, i]),
@@ -82,11 +82,12 @@ def f3(x):
 class Oracle:
     """One GVDB volume restated on the CPU (Configure + AddChannel done)."""
 
-    def __init__(self, log2dims=(3, 3, 3, 4, 5), apron=1, axiscnt=(16, 16, 1)):
+    def __init__(self, log2dims=(3, 3, 3, 4, 5), apron=1, axiscnt=(16, 16, 1), fp64=True):
         L = lib()
         dims = np.asarray(log2dims, dtype=np.int32)
         self.levels = len(dims)
         self.h = L.orc_new(self.levels, _p(dims))
+        L.orc_set_fp64(self.h, int(fp64))
         L.orc_set_channel_default(self.h, *axiscnt)
         L.orc_add_channel(self.h, apron)
 

@@ -122,3 +122,41 @@ def density_errors(a, b):
 
 def rmse(a, b):
     return float(np.sqrt(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2)))
+
+
+def used_bricks(atlas, amap, cnt, br=10):
+    """Yields (slot ids, padded bricks (k, br, br, br)) of the used slots, one
+    layer of atlas slots (fixed slot z) at a time, so full-size atlases are
+    compared without a whole-atlas mask."""
+    c0, c1 = int(cnt[0]), int(cnt[1])
+    per = c0 * c1
+    leaf = amap["mLeafNode"]
+    for sz in range(int(cnt[2])):
+        used = np.nonzero(leaf[sz * per:(sz + 1) * per] >= 0)[0]
+        if used.size == 0:
+            continue
+        layer = atlas[sz * br:(sz + 1) * br].reshape(br, c1, br, c0, br).transpose(1, 3, 0, 2, 4)
+        yield sz * per + used, layer.reshape(per, br, br, br)[used]
+
+
+def brick_density_errors(a, b, amap, cnt):
+    """density_errors over the used bricks of two full atlases (3D layout)."""
+    dmax, rel, zeros = 0.0, 0.0, 0
+    for (_, x), (_, y) in zip(used_bricks(a, amap, cnt), used_bricks(b, amap, cnt)):
+        d, r, z = density_errors(x, y.astype(np.float64))
+        dmax, rel, zeros = max(dmax, d), max(rel, r), zeros + z
+    return dmax, rel, zeros
+
+
+def half_errors(a, b, amap, cnt):
+    """FP16 storage against the FP64 field over the used bricks: the largest
+    error in units of the half rounding bound (2^-11 |b| + 2^-25: relative
+    half-ulp of a normal half, half the subnormal spacing below), and the
+    voxels where b is exactly zero and a is not."""
+    worst, zeros = 0.0, 0
+    for (_, x), (_, y) in zip(used_bricks(a, amap, cnt), used_bricks(b, amap, cnt)):
+        x = x.astype(np.float64)
+        bound = np.abs(y) * 2.0 ** -11 + 2.0 ** -25
+        worst = max(worst, float((np.abs(x - y) / bound).max()))
+        zeros += int(np.count_nonzero((y == 0) & (x != 0)))
+    return worst, zeros

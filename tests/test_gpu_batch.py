@@ -198,3 +198,63 @@ def test_batch_gather_variants_match_single_grid_calls(gpu):
     for v in vols:
         v.close()
     B.close()
+
+
+def test_batch_of_130_grids_fp32_and_fp16(gpu):
+    """130 tiny grids in one batch (the multi-grid lookup's ballot passes over
+    64 grids at a time, so grids 64-129 take its later passes), every third
+    one with an FP16 channel, the apron with a nonzero boundary value: each
+    grid's pools, atlas and image equal the one-grid calls' bit for bit."""
+    g = np.random.default_rng(130)
+    G = 130
+    grids = []
+    for k in range(G):
+        n = int(g.integers(40, 400))
+        c = g.random(3) * 200 + 40
+        grids.append((c + g.normal(0, 6.0, (n, 3))).astype(np.float32))
+    types = [gpu.T_HALF if k % 3 == 2 else gpu.T_FLOAT for k in range(G)]
+    size = (24, 16)
+
+    def make(pts, t):
+        v = gpu.VolumeGVDB(0)
+        v.Configure(*H.TREE)
+        v.SetChannelDefault(16, 16, 1)
+        v.AddChannel(0, t, 1)
+        v.SetPoints(pts)
+        sc = _scene(0, *size)
+        v.apply_scene(sc)
+        v.AddRenderBuf(0, size[0], size[1], 16)
+        return v
+
+    vols = [make(p, t) for p, t in zip(grids, types)]
+    ns = [len(p) for p in grids]
+    B = gpu.GridBatch(0, G)
+    B.RebuildTopology(vols, ns, 2.0)
+    B.FinishTopology(vols, False, True)
+    B.UpdateAtlas(vols)
+    B.ClearChannel(vols, 0)
+    lens = B.InsertPointsSubcell(vols, 4, ns, 2.0)
+    B.GatherDensity(vols, 4, lens, 2.0, 0)
+    B.UpdateApron(vols, 0, 0.25)
+    B.Render(vols, gpu.SHADE_VOLUME, 0, 0)
+    B.synchronize()
+    for k, (pts, t) in enumerate(zip(grids, types)):
+        s = make(pts, t)
+        s.RebuildTopology(len(pts), 2.0, H.ZERO3)
+        s.FinishTopology(False, True)
+        s.UpdateAtlas()
+        s.ClearChannel(0)
+        n = s.InsertPointsSubcell(4, len(pts), 2.0, H.ZERO3)
+        s.GatherDensity(4, len(pts), 2.0, H.ZERO3, n, 0, 0, False)
+        s.UpdateApron(0, 0.25)
+        s.Render(gpu.SHADE_VOLUME, 0, 0)
+        assert lens[k] == n, k
+        H.assert_pools_equal(vols[k], s)
+        dt = np.float16 if t == gpu.T_HALF else np.float32
+        a, b = vols[k].atlas(0, dt), s.atlas(0, dt)
+        assert a.shape == b.shape and np.array_equal(a, b), "grid %d: batched atlas differs" % k
+        assert np.array_equal(vols[k].ReadRenderBuf(0, size[0], size[1], 16), s.ReadRenderBuf(0, size[0], size[1], 16)), k
+        s.close()
+    for v in vols:
+        v.close()
+    B.close()

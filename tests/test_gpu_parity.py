@@ -91,6 +91,22 @@ def test_density_and_apron_vs_fp64(gpu, case):
     assert rel <= REL_TOL, "max relative density error %.3g (abs %.3g)" % (rel, dmax)
 
 
+@pytest.mark.parametrize("case", ["sphere_r2", "slab_r2"])
+def test_apron_boundary_value(gpu, case):
+    """UpdateApron(chan, boundval) with a nonzero boundary value: apron voxels
+    with no neighbour brick take it, the others their neighbour's value."""
+    import oracle as O
+    pts, r = CASES[case][0](), CASES[case][1]
+    v, n = H.gpu_volume(gpu, pts, r, boundval=0.375)
+    o, m = H.oracle_volume(O, pts, r, precision=64, boundval=0.375)
+    assert n == m
+    a, b = v.atlas(), o.atlas(64)
+    assert np.count_nonzero(b == 0.375) > 0
+    _, rel, zeros = H.brick_density_errors(a, b, o.atlas_map(), o.atlas_info()["cnt"])
+    assert zeros == 0 and rel <= REL_TOL, rel
+    v.close()
+
+
 @pytest.mark.parametrize("shading", [7, 4])
 def test_render_vs_fp64(gpu, shading):
     import oracle as O

@@ -150,5 +150,40 @@ def test_load_vbx_errors(gpu, tmp_path):
         _render(w)
     assert e.value.code == -7
     assert vbx.read_vbx(good)["end"] == len(good)
+    # references outside the pools are refused before any kernel walks them:
+    # a child slot naming a leaf past the leaf pool, a child list id past the
+    # list pool, an unused leaf's atlas corner outside the atlas
+    p0 = [v.pool(0, l).copy() for l in range(5)]
+    p1 = [v.pool(1, l).copy() if l else None for l in range(5)]
+    nleaf = len(p0[0])
+
+    def write(name, q0, q1):
+        data = vbx.write_vbx(_meta(v), [x.tobytes() for x in q0], [x.tobytes() if x is not None else b"" for x in q1],
+                             [(3, v.atlas())])
+        (tmp_path / name).write_bytes(data)
+        return str(tmp_path / name)
+
+    c = p1[1].reshape(-1).copy()
+    c[np.nonzero(c != np.uint64(2 ** 64 - 1))[0][0]] = np.uint64((nleaf + 5) << 16)
+    q1 = list(p1)
+    q1[1] = c
+    with pytest.raises(GVDBError) as e:
+        w.LoadVBX(write("badslot.vbx", p0, q1))
+    assert e.value.code == -9
+    q0 = list(p0)
+    q0[1] = p0[1].copy()
+    q0[1]["mChildList"][0] = np.uint64(1 | (1 << 8) | ((len(p1[1]) + 3) << 16))
+    with pytest.raises(GVDBError) as e:
+        w.LoadVBX(write("badlist.vbx", q0, p1))
+    assert e.value.code == -9
+    q0 = list(p0)
+    q0[0] = p0[0].copy()
+    q0[0]["mFlags"][0] = 0
+    q0[0]["mValue"][0] = (10 ** 6, 0, 0)
+    with pytest.raises(GVDBError) as e:
+        w.LoadVBX(write("badcorner.vbx", q0, p1))
+    assert e.value.code == -9
+    # the unmodified pools still load
+    w.LoadVBX(write("same.vbx", p0, p1))
     v.close()
     w.close()
