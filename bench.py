@@ -86,8 +86,15 @@ def run_gpu(args):
     def render():
         v.Render(G.SHADE_TRILINEAR, 0, 0)
 
-    splat()
-    tuned = v.TuneRender(G.SHADE_TRILINEAR)
+    ln0 = splat()
+    # the autotuner on this workload: gather threads per brick, then the
+    # march's ray-packet shape x samples fetched ahead
+    tg = v.TuneGather(4, n, r, z, ln0)
+    v.UpdateApron(0, 0.0)
+    tr = v.TuneRenderShape(G.SHADE_TRILINEAR)
+    tuned = dict(gather_threads=tg[0], gather_ms={str(k): x for k, x in tg[1].items()},
+                 render_unroll=tr[0][0], render_tile="%dx%d" % (tr[0][1], 64 // tr[0][1]),
+                 render_ms={"%dx%d/u%d" % (tw, 64 // tw, u): x for (tw, u), x in tr[1].items()})
     for _ in range(args.warmup):
         splat()
         render()
@@ -123,7 +130,7 @@ def run_gpu(args):
     render_ms = float(np.median(kr))
     stages = {k: float(np.median([s[k] for s in st])) for k in st[0]}
     out = dict(n=n, entries=entries, t_splat=float(np.median(t_splat)), t_render=float(np.median(t_render)), t_all=t_all,
-               t_upload=float(np.median(t_up)), render_tuning=dict(best=tuned[0], ms=tuned[1]),
+               t_upload=float(np.median(t_up)), autotune=tuned,
                ab=ab,
                gather_ms=gather_ms, apron_ms=apron_ms, render_ms=render_ms, stages=stages,
                rays=sc["width"] * sc["height"], topo=[len(v.pool(0, l)) for l in range(5)])
@@ -276,7 +283,17 @@ def run_batched(args, grids):
         return [t] * len(vols)
 
     plain()
-    tuned = vols[int(np.argmax(ns))].TuneRender(G.SHADE_VOLUME)      # on the largest grid
+    batched()
+    # the autotuner over the whole batch: gather threads per brick, apron
+    # bricks per workgroup, ray-packet shape x samples fetched ahead
+    lens = B.InsertPointsSubcell(vols, 4, ns, r)
+    tg = B.Tune(vols, "gather", 4, lens, r)
+    ta = B.Tune(vols, "apron")
+    tr = B.Tune(vols, "render", shading=G.SHADE_VOLUME)
+    tuned = dict(gather_threads=tg[0], gather_ms={str(k): x for k, x in tg[1].items()},
+                 apron_run=ta[0], apron_ms={str(k): x for k, x in ta[1].items()},
+                 render_unroll=tr[0][0], render_tile="%dx%d" % (tr[0][1], 64 // tr[0][1]),
+                 render_ms={"%dx%d/u%d" % (tw, 64 // tw, u): x for (tw, u), x in tr[1].items()})
     for _ in range(max(1, args.warmup)):
         plain()
         batched()
@@ -302,7 +319,7 @@ def run_batched(args, grids):
                            p99_ms=1e3 * float(np.percentile(allp, 99))),
                 batched=dict(ms=1e3 * tb, requests_per_s=len(ns) / tb, p50_ms=1e3 * float(np.percentile(allb, 50)),
                              p99_ms=1e3 * float(np.percentile(allb, 99))),
-                speedup=tp / tb, same_images=bool(same), render_tuning=dict(best=tuned[0], ms=tuned[1]))
+                speedup=tp / tb, same_images=bool(same), autotune=tuned)
 
 
 def cpu_model():
@@ -456,7 +473,7 @@ def main():
         mrays_per_s=g["rays"] / g["t_render"] / 1e6,
         splat_ms=1e3 * g["t_splat"], render_ms=1e3 * g["t_render"], stage_ms=g["stages"],
         upload_ms=1e3 * g["t_upload"], pcie_inclusive_points_per_s=g["n"] / (g["t_splat"] + g["t_upload"]),
-        render_tuning=g["render_tuning"],
+        autotune=g["autotune"],
         kernel_ms=dict(gather=g["gather_ms"], apron=g["apron_ms"], render=g["render_ms"]),
         roofline=dict(bound="hbm", kernel="gather", achieved=gather_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=gather_gbs / HBM_PEAK_GBS, traffic=profiled_traffic("k_push_gather<128, float, 4, false>"),

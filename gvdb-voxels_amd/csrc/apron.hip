@@ -67,6 +67,7 @@ hipError_t launch_neighbors_multi(const NbrArgs* a, int G, Arena* ar, hipStream_
     MultiLaunch L;
     hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
     if (e != hipSuccess || L.blocks == 0) return e;
+    L.map.xcd = knob(KNOB_XCD_ORDER);
     hipLaunchKernelGGL(k_neighbors, dim3(L.blocks), dim3(256), 0, s, a[0], L.map);
     return hipGetLastError();
 }
@@ -142,20 +143,31 @@ __device__ inline void k_apron_body(const ApronArgs& ap, uint32_t bid, uint32_t)
 
 GVDB_MULTI_T(k_apron, typename AT, AT, ApronArgs, 512)
 
-// Several grids: APRON_RUN bricks of one grid per workgroup, one after the
-// other, so the grid lookup and the record loads before the first brick are
-// paid once per run (a brick alone is one voxel per thread).
-static const int APRON_RUN = 4;
-template <typename AT>
+// Several grids: RUN bricks of one grid per workgroup (the apron-run knob:
+// 1, 2, 4 or 8), one after the other, so the grid lookup and the record
+// loads before the first brick are paid once per run (a brick alone is one
+// voxel per thread).
+template <typename AT, int RUN>
 __global__ __launch_bounds__(512) void k_apron_runs(ApronArgs, MultiMap m) {
-    const int g = multi_grid(m, blockIdx.x);
+    const uint32_t bi = m.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int g = multi_grid(m, bi);
     const uint32_t* first = multi_first(m);
     const ApronArgs& a = multi_rec<ApronArgs>(m, g);
-    const uint32_t b0 = (blockIdx.x - first[g]) * uint32_t(APRON_RUN);
-    for (int k = 0; k < APRON_RUN; k++) {
+    const uint32_t b0 = (bi - first[g]) * uint32_t(RUN);
+    for (int k = 0; k < RUN; k++) {
         if (b0 + uint32_t(k) >= uint32_t(a.n)) break;
         if (k) __syncthreads();            // the previous brick is done with its offsets
         k_apron_body<AT>(a, b0 + uint32_t(k), 0);
+    }
+}
+
+template <typename AT>
+static void launch_apron_runs(int run, dim3 grid, hipStream_t s, const ApronArgs& a0, const MultiMap& m) {
+    switch (run) {
+    case 1: hipLaunchKernelGGL((k_apron_runs<AT, 1>), grid, dim3(512), 0, s, a0, m); break;
+    case 2: hipLaunchKernelGGL((k_apron_runs<AT, 2>), grid, dim3(512), 0, s, a0, m); break;
+    case 8: hipLaunchKernelGGL((k_apron_runs<AT, 8>), grid, dim3(512), 0, s, a0, m); break;
+    default: hipLaunchKernelGGL((k_apron_runs<AT, 4>), grid, dim3(512), 0, s, a0, m); break;
     }
 }
 
@@ -169,15 +181,17 @@ hipError_t launch_apron_multi(const ApronArgs* a, int G, Arena* ar, hipStream_t 
             sel.push_back(a[g]);
         }
         if (sel.empty()) continue;
-        // one grid: a workgroup per brick; several: runs of APRON_RUN bricks
+        // one grid: a workgroup per brick; several: runs of `run` bricks
         const bool runs = sel.size() > 1;
-        for (const ApronArgs& x : sel) nb.push_back(runs ? uint32_t((x.n + APRON_RUN - 1) / APRON_RUN) : uint32_t(x.n));
+        const int run = knob(KNOB_APRON_RUN);
+        for (const ApronArgs& x : sel) nb.push_back(runs ? uint32_t((x.n + run - 1) / run) : uint32_t(x.n));
         MultiLaunch L;
         hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
         if (e != hipSuccess) return e;
         if (L.blocks == 0) continue;
-        if (runs && half) hipLaunchKernelGGL(k_apron_runs<__half>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
-        else if (runs) hipLaunchKernelGGL(k_apron_runs<float>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
+        L.map.xcd = knob(KNOB_XCD_ORDER);
+        if (runs && half) launch_apron_runs<__half>(run, dim3(L.blocks), s, sel[0], L.map);
+        else if (runs) launch_apron_runs<float>(run, dim3(L.blocks), s, sel[0], L.map);
         else if (half) hipLaunchKernelGGL(k_apron<__half>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
         else hipLaunchKernelGGL(k_apron<float>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
         e = hipGetLastError();

@@ -221,39 +221,34 @@ int gvdb_render(gvdb_handle h, int shading, int chan, int buf) { H_OR_FAIL(h); r
 int gvdb_read_render_buf(gvdb_handle h, int buf, void* dst) { H_OR_FAIL(h); return h->v.read_render_buf(buf, dst); }
 int gvdb_set_render_unroll(int shading, int unroll) {
     if (shading != GVDB_SHADE_VOLUME && shading != GVDB_SHADE_TRILINEAR) return GVDB_ERR_INVALID;
-    if (unroll != 1 && unroll != 2 && unroll != 4) return GVDB_ERR_INVALID;
-    gvdb::set_render_unroll(shading == GVDB_SHADE_VOLUME ? 0 : 1, unroll);
-    return GVDB_OK;
+    return gvdb::set_knob(shading == GVDB_SHADE_VOLUME ? gvdb::KNOB_RENDER_UNROLL_EA : gvdb::KNOB_RENDER_UNROLL_ISO,
+                          unroll)
+               ? GVDB_OK
+               : GVDB_ERR_INVALID;
 }
 int gvdb_tune_render(gvdb_handle h, int shading, int chan, int buf, int* best, float* ms3) {
     H_OR_FAIL(h);
-    if (shading != GVDB_SHADE_VOLUME && shading != GVDB_SHADE_TRILINEAR) return GVDB_ERR_INVALID;
-    const int mode = shading == GVDB_SHADE_VOLUME ? 0 : 1;
-    const int keep = gvdb::render_unroll(mode);
-    const int us[3] = {1, 2, 4};
-    int pick = keep;
-    float pick_ms = 0.f;
-    for (int i = 0; i < 3; i++) {
-        gvdb::set_render_unroll(mode, us[i]);
-        float ms = 0.f;
-        for (int rep = 0; rep < 3; rep++) {          // first call warms up, the next two are timed
-            int rc = h->v.render(shading, chan, buf);
-            float k[4];
-            if (rc == GVDB_OK) rc = h->v.kernel_ms(k);
-            if (rc != GVDB_OK) {
-                gvdb::set_render_unroll(mode, keep);
-                return rc;
-            }
-            if (rep) ms += 0.5f * k[2];
-        }
-        if (ms3) ms3[i] = ms;
-        if (i == 0 || ms < pick_ms) {
-            pick = us[i];
-            pick_ms = ms;
-        }
-    }
-    gvdb::set_render_unroll(mode, pick);
-    if (best) *best = pick;
+    return gvdb::tune_render_unroll(h->v, shading, chan, buf, best, ms3);
+}
+int gvdb_tune_render_shape(gvdb_handle h, int shading, int chan, int buf, int* best2, float* ms12) {
+    H_OR_FAIL(h);
+    return gvdb::tune_render(h->v, shading, chan, buf, best2, ms12);
+}
+int gvdb_tune_gather(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len, int chan,
+                     int* best_threads, float* ms2) {
+    H_OR_FAIL(h);
+    const float z[3] = {0, 0, 0};
+    return gvdb::tune_gather(h->v, subcell_size, n, radius, trans ? trans : z, sc_len, chan, best_threads, ms2);
+}
+int gvdb_set_tuning(int knob, int value) { return gvdb::set_knob(knob, value) ? GVDB_OK : GVDB_ERR_INVALID; }
+int gvdb_get_tuning(int knob, int* value, int* choices4, int* nchoices) {
+    if (knob < 0 || knob >= gvdb::KNOB_COUNT) return GVDB_ERR_INVALID;
+    if (value) *value = gvdb::knob(knob);
+    int c[4];
+    const int n = gvdb::knob_values(knob, c);
+    if (choices4)
+        for (int i = 0; i < n; i++) choices4[i] = c[i];
+    if (nchoices) *nchoices = n;
     return GVDB_OK;
 }
 int gvdb_get_scene_block(gvdb_handle h, int w, int hgt, float* out68) {
@@ -386,6 +381,15 @@ int gvdb_batch_render(gvdb_batch b, int count, const gvdb_handle* hs, const int*
     VOLS(b, count, hs);
     if (!shading || !chan || !buf) return GVDB_ERR_INVALID;
     return b->b.render(vs_.data(), count, shading, chan, buf, failed);
+}
+int gvdb_batch_tune(gvdb_batch b, int count, const gvdb_handle* hs, int stage, int subcell_size, const int* sc_len,
+                    const float* radius, int chan, float boundval, const int* shading, const int* chans,
+                    const int* bufs, int* best, float* ms, int* failed) {
+    VOLS(b, count, hs);
+    if (stage == 0 && (!sc_len || !radius)) return GVDB_ERR_INVALID;
+    if (stage == 2 && (!shading || !chans || !bufs)) return GVDB_ERR_INVALID;
+    return gvdb::tune_batch(b->b, vs_.data(), count, stage, subcell_size, sc_len, radius, chan, boundval, shading,
+                            chans, bufs, best, ms, failed);
 }
 int gvdb_batch_synchronize(gvdb_batch b) {
     if (!b) return GVDB_ERR_INVALID;

@@ -366,11 +366,36 @@ struct RenderArgs {
     int out_float;
     int shading;
     int atlas_type;
+    int tile_w;          // pixel tile of one 64-lane workgroup: tile_w x (64 / tile_w); set by the launcher
 };
 hipError_t launch_render_multi(const RenderArgs* a, int G, Arena* ar, hipStream_t s);
-// samples fetched ahead in each brick (1, 2 or 4) for mode 0 (emission-
-// absorption) and 1 (isosurface); process-wide
-void set_render_unroll(int mode, int u);
-int render_unroll(int mode);
+
+// ---- launch-shape knobs (tune.cpp), process-wide, picked by the autotuner
+enum Knob {
+    KNOB_RENDER_UNROLL_EA = 0,   // samples fetched ahead per brick step, emission-absorption: 1, 2, 4
+    KNOB_RENDER_UNROLL_ISO,      // the same for the isosurface march
+    KNOB_RENDER_TILE_EA,         // ray-packet width of a 64-lane workgroup (8x8, 16x4, 32x2, 64x1): 8, 16, 32, 64
+    KNOB_RENDER_TILE_ISO,
+    KNOB_GATHER_THREADS,         // threads per brick of the scatter gather at radius <= 3: 128, 256
+    KNOB_APRON_RUN,              // bricks per workgroup of the multi-grid apron: 1, 2, 4, 8
+    KNOB_XCD_ORDER,              // per-brick kernels: 0 launch order, 1 each XCD a contiguous range of bricks
+    KNOB_COUNT
+};
+int knob(int k);
+bool set_knob(int k, int v);          // false (and unchanged) when v is not one of the knob's values
+int knob_values(int k, int out[4]);   // the knob's values; returns how many
+class Volume;
+class Batch;
+// The autotuner (tune.cpp): times the caller's own workload per knob value
+// and keeps the fastest.  Render: tile x unroll (ms12 in that order, best2 =
+// {unroll, tile width}); unroll alone; gather: threads per brick; batched
+// stages 0 gather, 1 apron, 2 render.
+int tune_render(Volume& v, int shading, int chan, int buf, int* best2, float* ms12);
+int tune_render_unroll(Volume& v, int shading, int chan, int buf, int* best, float* ms3);
+int tune_gather(Volume& v, int scs, int n, float radius, const float trans[3], int len, int chan, int* best,
+                float* ms2);
+int tune_batch(Batch& B, Volume* const* vs, int G, int stage, int scs, const int* len, const float* radius, int chan,
+               float boundval, const int* shading, const int* chans, const int* bufs, int* best, float* ms,
+               int* failed);
 
 }  // namespace gvdb

@@ -25,6 +25,7 @@ struct MultiMap {
     int G;
     const uint32_t* first;   // G + 1 entries (device), or null: inline
     const void* table;       // G argument records (device), or null: inline
+    int xcd;                 // 1: workgroups take bricks in XCD-contiguous order (xcd_remap)
     uint32_t ifirst[MULTI_INLINE + 1];
     alignas(16) unsigned char itab[MULTI_INLINE_BYTES];
 };
@@ -76,26 +77,28 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t n) {
 
 #define GVDB_MULTI(NAME, A, THREADS)                                                             \
     __global__ __launch_bounds__(THREADS) void NAME(A a0, MultiMap m) {                          \
+        const uint32_t bi = m.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;               \
         if (m.G <= 1) {                                                                          \
-            NAME##_body(a0, blockIdx.x, gridDim.x);                                              \
+            NAME##_body(a0, bi, gridDim.x);                                                      \
             return;                                                                              \
         }                                                                                        \
-        const int g = multi_grid(m, blockIdx.x);                                                 \
+        const int g = multi_grid(m, bi);                                                         \
         const uint32_t* first = multi_first(m);                                                  \
-        NAME##_body(multi_rec<A>(m, g), blockIdx.x - first[g], first[g + 1] - first[g]);         \
+        NAME##_body(multi_rec<A>(m, g), bi - first[g], first[g + 1] - first[g]);                 \
     }
 
 // Same for a body templated on one parameter (TDECL e.g. `typename AT`).
 #define GVDB_MULTI_T(NAME, TDECL, TNAME, A, THREADS)                                             \
     template <TDECL>                                                                             \
     __global__ __launch_bounds__(THREADS) void NAME(A a0, MultiMap m) {                          \
+        const uint32_t bi = m.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;               \
         if (m.G <= 1) {                                                                          \
-            NAME##_body<TNAME>(a0, blockIdx.x, gridDim.x);                                       \
+            NAME##_body<TNAME>(a0, bi, gridDim.x);                                               \
             return;                                                                              \
         }                                                                                        \
-        const int g = multi_grid(m, blockIdx.x);                                                 \
+        const int g = multi_grid(m, bi);                                                         \
         const uint32_t* first = multi_first(m);                                                  \
-        NAME##_body<TNAME>(multi_rec<A>(m, g), blockIdx.x - first[g], first[g + 1] - first[g]);  \
+        NAME##_body<TNAME>(multi_rec<A>(m, g), bi - first[g], first[g + 1] - first[g]);          \
     }
 
 // Pinned staging for argument tables: bump-allocated, copied to the device
@@ -114,7 +117,7 @@ struct Arena {
 
 // Launch geometry for G grids needing nb[g] blocks each.
 struct MultiLaunch {
-    MultiMap map{1, nullptr, nullptr, {}, {}};
+    MultiMap map{1, nullptr, nullptr, 0, {}, {}};
     uint32_t blocks = 0;
 };
 

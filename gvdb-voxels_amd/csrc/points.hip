@@ -1397,16 +1397,17 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
 template <int NT, typename AT, int LS, bool MULTI>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_push_gather(BucketGatherArgs a0,
                                                                                               MultiMap m) {
+    const uint32_t bi = m.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     if (!MULTI) {
-        k_push_gather_body<NT, AT, LS>(a0, blockIdx.x, gridDim.x);
+        k_push_gather_body<NT, AT, LS>(a0, bi, gridDim.x);
         return;
     }
     // several grids: first[] and the records in device memory (set up with
     // inline_ok = false); the record copied once, from a wave-uniform index
-    const int g = __builtin_amdgcn_readfirstlane(multi_grid_dev(m, blockIdx.x));
+    const int g = __builtin_amdgcn_readfirstlane(multi_grid_dev(m, bi));
     const BucketGatherArgs a = multi_rec_dev<BucketGatherArgs>(m, g);
     const uint32_t f0 = m.first[g], f1 = m.first[g + 1];
-    k_push_gather_body<NT, AT, LS>(a, blockIdx.x - f0, f1 - f0);
+    k_push_gather_body<NT, AT, LS>(a, bi - f0, f1 - f0);
 }
 
 template <int NT, typename AT, int LS>
@@ -1450,11 +1451,26 @@ hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* a
         if (e != hipSuccess) return e;
         if (L.blocks == 0) continue;
         const dim3 grid(L.blocks);
+        if (kind < 4) L.map.xcd = knob(KNOB_XCD_ORDER);
+        // the listed-slice variants run with the tuned threads per brick
+        const bool wide = knob(KNOB_GATHER_THREADS) == 256;
         switch (variant) {
-        case 0: launch_push_gather<PG_SMALL, float, 4>(grid, s, sel[0], L.map); break;
-        case 1: launch_push_gather<PG_SMALL, __half, 4>(grid, s, sel[0], L.map); break;
-        case 2: launch_push_gather<PG_SMALL, float, 5>(grid, s, sel[0], L.map); break;
-        case 3: launch_push_gather<PG_SMALL, __half, 5>(grid, s, sel[0], L.map); break;
+        case 0:
+            if (wide) launch_push_gather<256, float, 4>(grid, s, sel[0], L.map);
+            else launch_push_gather<PG_SMALL, float, 4>(grid, s, sel[0], L.map);
+            break;
+        case 1:
+            if (wide) launch_push_gather<256, __half, 4>(grid, s, sel[0], L.map);
+            else launch_push_gather<PG_SMALL, __half, 4>(grid, s, sel[0], L.map);
+            break;
+        case 2:
+            if (wide) launch_push_gather<256, float, 5>(grid, s, sel[0], L.map);
+            else launch_push_gather<PG_SMALL, float, 5>(grid, s, sel[0], L.map);
+            break;
+        case 3:
+            if (wide) launch_push_gather<256, __half, 5>(grid, s, sel[0], L.map);
+            else launch_push_gather<PG_SMALL, __half, 5>(grid, s, sel[0], L.map);
+            break;
         case 4: launch_push_gather<PG_SMALL, float, 0>(grid, s, sel[0], L.map); break;
         case 5: launch_push_gather<PG_SMALL, __half, 0>(grid, s, sel[0], L.map); break;
         case 6: launch_push_gather<256, float, 0>(grid, s, sel[0], L.map); break;

@@ -7,7 +7,8 @@
 // (:132-141) and Phong + shadow ray (kernels/cuda_gvdb_module.cu:38-57).
 // Samples are exact software trilinear reads of the brick-major atlas; the
 // reference's texture unit is replaced because its filter weights are
-// quantised.  One 64-thread workgroup renders an 8x8 pixel tile.
+// quantised.  One 64-thread workgroup renders one pixel tile (8x8 by
+// default; 16x4, 32x2 or 64x1 when the autotuner finds them faster).
 #include "kernels.h"
 #include "tree_device.h"
 #include <hip/hip_fp16.h>
@@ -318,10 +319,12 @@ __device__ inline f3 mmult(const float* m, f3 v) {
 
 template <int MODE, typename AT, int U>
 __device__ inline void k_render_body(const RenderArgs& A, uint32_t bid, LevelStack& S) {
-    const int tiles_x = (A.s.width + 7) / 8;
+    // the workgroup's ray packet: a tile_w x (64 / tile_w) pixel tile
+    const int tw = A.tile_w, sh = __builtin_ctz(unsigned(tw));
+    const int tiles_x = (A.s.width + tw - 1) >> sh;
     const int tile = int(bid);
-    const int x = (tile % tiles_x) * 8 + (threadIdx.x & 7);
-    const int y = (tile / tiles_x) * 8 + (threadIdx.x >> 3);
+    const int x = (tile % tiles_x) * tw + int(threadIdx.x & unsigned(tw - 1));
+    const int y = (tile / tiles_x) * (64 >> sh) + int(threadIdx.x >> sh);
     if (x >= A.s.width || y >= A.s.height) return;
     const AT* atlas = reinterpret_cast<const AT*>(A.atlas);
     f3 rpos = mmult(A.s.invxform, mk3(A.s.campos[0], A.s.campos[1], A.s.campos[2]));
@@ -371,7 +374,7 @@ __device__ inline void k_render_body(const RenderArgs& A, uint32_t bid, LevelSta
     }
 }
 
-// One 64-thread workgroup per 8x8 tile; several grids' images in one launch.
+// One 64-thread workgroup per pixel tile; several grids' images in one launch.
 // Occupancy asked of the compiler per unroll (waves per SIMD: 8, 6, 4 for
 // U = 1, 2, 4): the most each variant reaches with no register spills.
 constexpr int render_waves(int u) { return u == 1 ? 8 : (u == 2 ? 6 : 4); }
@@ -387,15 +390,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(render_waves
     k_render_body<MODE, AT, U>(multi_rec<RenderArgs>(m, g), blockIdx.x - multi_first(m)[g], S);
 }
 
-// Samples fetched ahead per brick step (1, 2 or 4), per march mode: 4 for
-// emission-absorption and 2 for the isosurface measured fastest on the
-// medium and batched workloads; gvdb_tune_render re-picks per scene.
-static int g_unroll[2] = {4, 2};
-void set_render_unroll(int mode, int u) {
-    if ((mode == 0 || mode == 1) && (u == 1 || u == 2 || u == 4)) g_unroll[mode] = u;
-}
-int render_unroll(int mode) { return g_unroll[mode & 1]; }
-
+// Samples fetched ahead per brick step and the ray-packet shape come from
+// the launch-shape knobs (tune.cpp): defaults are the measured winners, the
+// autotuner re-picks them for the caller's scene.
 template <int MODE, typename AT>
 static void launch_u(int u, dim3 grid, hipStream_t s, const RenderArgs& a0, const MultiMap& m) {
     if (u == 4) hipLaunchKernelGGL((k_render<MODE, AT, 4>), grid, dim3(64), 0, s, a0, m);
@@ -413,15 +410,17 @@ hipError_t launch_render_multi(const RenderArgs* a, int G, Arena* ar, hipStream_
             if (a[g].shading != SHADE_VOLUME && a[g].shading != SHADE_TRILINEAR) return hipErrorInvalidValue;
             const int am = a[g].shading == SHADE_VOLUME ? 0 : 1;
             if (am != mode || (a[g].atlas_type == T_HALF) != half) continue;
+            const int tw = knob(mode ? KNOB_RENDER_TILE_ISO : KNOB_RENDER_TILE_EA), th = 64 / tw;
             sel.push_back(a[g]);
-            nb.push_back(uint32_t(((a[g].s.width + 7) / 8) * ((a[g].s.height + 7) / 8)));
+            sel.back().tile_w = tw;
+            nb.push_back(uint32_t(((a[g].s.width + tw - 1) / tw) * ((a[g].s.height + th - 1) / th)));
         }
         if (sel.empty()) continue;
         MultiLaunch L;
         hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
         if (e != hipSuccess) return e;
         if (L.blocks == 0) continue;
-        const int u = g_unroll[mode];
+        const int u = knob(mode ? KNOB_RENDER_UNROLL_ISO : KNOB_RENDER_UNROLL_EA);
         if (mode == 0 && half) launch_u<0, __half>(u, dim3(L.blocks), s, sel[0], L.map);
         else if (mode == 0) launch_u<0, float>(u, dim3(L.blocks), s, sel[0], L.map);
         else if (half) launch_u<1, __half>(u, dim3(L.blocks), s, sel[0], L.map);

@@ -20,6 +20,9 @@ T_FLOAT, T_HALF = 3, 100
 SHADE_TRILINEAR, SHADE_VOLUME = 4, 7
 FUNC_FILL_F, FUNC_SMOOTH, FUNC_NOISE, FUNC_GROW = 110, 113, 114, 115
 SCN_FLOATS = 68
+# launch-shape knobs of the autotuner (gvdb_set_tuning / gvdb_get_tuning)
+KNOB_RENDER_UNROLL_EA, KNOB_RENDER_UNROLL_ISO, KNOB_RENDER_TILE_EA, KNOB_RENDER_TILE_ISO = 0, 1, 2, 3
+KNOB_GATHER_THREADS, KNOB_APRON_RUN, KNOB_XCD_ORDER = 4, 5, 6
 
 NODE_DTYPE = np.dtype([
     ("mLev", "u1"), ("mFlags", "u1"), ("mPriority", "u1"), ("pad", "u1"),
@@ -59,6 +62,10 @@ _SIGS = [
     ("gvdb_add_render_buf", _i, [_vp, _i, _i, _i, _i]), ("gvdb_render", _i, [_vp, _i, _i, _i]),
     ("gvdb_read_render_buf", _i, [_vp, _i, _vp]), ("gvdb_get_scene_block", _i, [_vp, _i, _i, _vp]),
     ("gvdb_tune_render", _i, [_vp, _i, _i, _i, _vp, _vp]), ("gvdb_set_render_unroll", _i, [_i, _i]),
+    ("gvdb_tune_render_shape", _i, [_vp, _i, _i, _i, _vp, _vp]),
+    ("gvdb_tune_gather", _i, [_vp, _i, _i, _f, _vp, _i, _i, _vp, _vp]),
+    ("gvdb_set_tuning", _i, [_i, _i]), ("gvdb_get_tuning", _i, [_i, _vp, _vp, _vp]),
+    ("gvdb_batch_tune", _i, [_vp, _i, _vp, _i, _i, _vp, _vp, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("gvdb_get_pool", _i, [_vp, _i, _i, _vp, _u64p, _u64p]), ("gvdb_get_info", _i, [_vp, _vp, _vp]),
     ("gvdb_get_atlas_info", _i, [_vp, _i, _vp]), ("gvdb_read_atlas", _i, [_vp, _i, _vp]),
     ("gvdb_read_atlas_map", _i, [_vp, _vp]), ("gvdb_read_subcells", _i, [_vp, _i, _vp, _u64p]),
@@ -119,6 +126,22 @@ def _p(a):
 
 def _f3(x):
     return np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(3))
+
+
+def get_tuning(knob):
+    """(current value, values it can take) of a launch-shape knob."""
+    v, n = ctypes.c_int(0), ctypes.c_int(0)
+    ch = np.zeros(4, np.int32)
+    rc = load().gvdb_get_tuning(int(knob), ctypes.byref(v), _p(ch), ctypes.byref(n))
+    if rc != OK:
+        raise GVDBError(rc, "gvdb_get_tuning(%d)" % knob)
+    return v.value, [int(x) for x in ch[:n.value]]
+
+
+def set_tuning(knob, value):
+    rc = load().gvdb_set_tuning(int(knob), int(value))
+    if rc != OK:
+        raise GVDBError(rc, "gvdb_set_tuning(%d, %d)" % (knob, value))
 
 
 def module_load(path):
@@ -320,6 +343,25 @@ class VolumeGVDB:
         self._chk(load().gvdb_tune_render(self.h, int(shading), int(chan), int(buf), ctypes.byref(best), _p(ms)))
         return best.value, {1: float(ms[0]), 2: float(ms[1]), 4: float(ms[2])}
 
+    def TuneRenderShape(self, shading, chan=0, buf=0):
+        """Autotunes the ray-packet shape and the samples fetched ahead for
+        this shading mode on the current scene; returns ((unroll, tile
+        width), {(tile width, unroll): ms})."""
+        best = np.zeros(2, np.int32)
+        ms = np.zeros(12, np.float32)
+        self._chk(load().gvdb_tune_render_shape(self.h, int(shading), int(chan), int(buf), _p(best), _p(ms)))
+        return (int(best[0]), int(best[1])), {(tw, u): float(ms[i * 3 + j]) for i, tw in enumerate((8, 16, 32, 64))
+                                              for j, u in enumerate((1, 2, 4))}
+
+    def TuneGather(self, subcell_size, n, radius, trans, sc_len, chan=0):
+        """Autotunes the gather's threads per brick (GatherDensity without
+        accumulate, run on the current buckets); returns (best, {threads: ms})."""
+        best = ctypes.c_int(0)
+        ms = np.zeros(2, np.float32)
+        self._chk(load().gvdb_tune_gather(self.h, int(subcell_size), int(n), float(radius), _p(_f3(trans)),
+                                          int(sc_len), int(chan), ctypes.byref(best), _p(ms)))
+        return best.value, {128: float(ms[0]), 256: float(ms[1])}
+
     def ReadRenderBuf(self, buf, w, h, bytes_per_pixel=4):
         out = np.zeros((h, w, 4), np.float32 if bytes_per_pixel == 16 else np.uint8)
         self._chk(load().gvdb_read_render_buf(self.h, buf, _p(out)))
@@ -506,6 +548,29 @@ class GridBatch:
         f = ctypes.c_int(-1)
         return self._chk(load().gvdb_batch_render(self.h, G, self._hs(vols), _p(sh), _p(c), _p(b), ctypes.byref(f)),
                          vols, f)
+
+    def Tune(self, vols, stage, subcell_size=4, sc_len=None, radius=None, chan=0, boundval=0.0, shading=SHADE_VOLUME,
+             buf=0):
+        """Batched autotuner: stage "gather" (threads per brick), "apron"
+        (bricks per workgroup) or "render" (ray-packet shape x samples
+        ahead), timed over the whole batch; returns (best, {value: ms})."""
+        G = len(vols)
+        k = {"gather": 0, "apron": 1, "render": 2}[stage]
+        ln = self._arr(sc_len if sc_len is not None else 0, G, np.int32)
+        r = self._arr(radius if radius is not None else 0.0, G, np.float32)
+        sh, c, b = self._arr(shading, G, np.int32), self._arr(chan, G, np.int32), self._arr(buf, G, np.int32)
+        best = np.zeros(2, np.int32)
+        ms = np.zeros(12, np.float32)
+        f = ctypes.c_int(-1)
+        self._chk(load().gvdb_batch_tune(self.h, G, self._hs(vols), k, subcell_size, _p(ln), _p(r), chan,
+                                         float(boundval), _p(sh), _p(c), _p(b), _p(best), _p(ms), ctypes.byref(f)),
+                  vols, f)
+        if k == 0:
+            return int(best[0]), {128: float(ms[0]), 256: float(ms[1])}
+        if k == 1:
+            return int(best[0]), {v: float(ms[i]) for i, v in enumerate((1, 2, 4, 8))}
+        return (int(best[0]), int(best[1])), {(tw, u): float(ms[i * 3 + j]) for i, tw in enumerate((8, 16, 32, 64))
+                                              for j, u in enumerate((1, 2, 4))}
 
     def synchronize(self):
         return self._chk(load().gvdb_batch_synchronize(self.h))

@@ -184,13 +184,28 @@ int gvdb_add_render_buf(gvdb_handle h, int buf, int w, int hgt, int bytes_per_pi
 int gvdb_render(gvdb_handle h, int shading, int chan, int buf);
 /* ReadRenderBuf(buf, dst): gvdb_volume_gvdb.cpp:4241-4251 */
 int gvdb_read_render_buf(gvdb_handle h, int buf, void* host_dst);
-/* Render autotuner (no counterpart in the reference; for callers that render
- * one scene size many times): renders with 1, 2 and 4 samples fetched ahead
- * per brick step, times each on the handle's stream (ms3[i], may be NULL) and
- * keeps the fastest for that shading mode, process-wide.  Images do not
- * depend on the choice. */
+/* Autotuner (no counterpart in the reference; for callers that run one workload shape many times).  The launch-shape
+ * knobs below change no result, only time; each tuner runs the caller's own workload once per value (a warm-up, then
+ * the median of three runs timed with HIP events on the handle's or batch's stream) and keeps the fastest,
+ * process-wide.  Times go to ms[] in the order of the knob's values (may be NULL). */
+#define GVDB_KNOB_RENDER_UNROLL_EA 0  /* samples fetched ahead per brick step, SHADE_VOLUME: 1, 2, 4 */
+#define GVDB_KNOB_RENDER_UNROLL_ISO 1 /* the same, SHADE_TRILINEAR */
+#define GVDB_KNOB_RENDER_TILE_EA 2    /* ray-packet width of a 64-lane workgroup (8x8, 16x4, 32x2, 64x1): 8..64 */
+#define GVDB_KNOB_RENDER_TILE_ISO 3
+#define GVDB_KNOB_GATHER_THREADS 4    /* threads per brick of the scatter gather, radius <= 3: 128, 256 */
+#define GVDB_KNOB_APRON_RUN 5         /* bricks per workgroup of the multi-grid apron: 1, 2, 4, 8 */
+#define GVDB_KNOB_XCD_ORDER 6         /* per-brick kernels: 0 launch order, 1 each XCD a contiguous range of bricks */
+int gvdb_set_tuning(int knob, int value);
+/* current value, and the values it can take (choices4, may be NULL; nchoices of them) */
+int gvdb_get_tuning(int knob, int* value, int* choices4, int* nchoices);
+/* Render: samples fetched ahead (1, 2, 4) at the current ray-packet shape; ms3 per value */
 int gvdb_tune_render(gvdb_handle h, int shading, int chan, int buf, int* best_unroll, float* ms3);
-/* Sets that choice directly (1, 2 or 4), e.g. from an earlier tuning run. */
+/* Render: ray-packet shape x samples fetched ahead; best2 = {unroll, tile width}; ms12[tile][unroll] */
+int gvdb_tune_render_shape(gvdb_handle h, int shading, int chan, int buf, int* best2, float* ms12);
+/* GatherDensity (arguments as gvdb_gather_density, no accumulate): threads per brick; ms2 per value */
+int gvdb_tune_gather(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len, int chan,
+                     int* best_threads, float* ms2);
+/* Sets the samples-ahead knob of a shading mode directly (1, 2 or 4). */
 int gvdb_set_render_unroll(int shading, int unroll);
 /* The 68-word scene block PrepareRender sends to the kernels (gvdb_volume_gvdb.cpp:4254-4306) */
 int gvdb_get_scene_block(gvdb_handle h, int w, int hgt, float* out68);
@@ -259,6 +274,12 @@ int gvdb_batch_update_apron(gvdb_batch b, int count, const gvdb_handle* hs, int 
 /* Render (gvdb_volume_gvdb.cpp:4336-4381 per grid), each into its own buffer */
 int gvdb_batch_render(gvdb_batch b, int count, const gvdb_handle* hs, const int* shading, const int* chan,
                       const int* buf, int* failed);
+/* Batched autotuner (no counterpart in the reference): stage 0 = gvdb_batch_gather_density (sc_len, radius, chan;
+ * threads per brick, ms[2]), 1 = gvdb_batch_update_apron (chan, boundval; bricks per workgroup, ms[4]), 2 =
+ * gvdb_batch_render (shading, chans, bufs; best = {unroll, tile width}, ms[12]), each timed over the whole batch. */
+int gvdb_batch_tune(gvdb_batch b, int count, const gvdb_handle* hs, int stage, int subcell_size, const int* sc_len,
+                    const float* radius, int chan, float boundval, const int* shading, const int* chans,
+                    const int* bufs, int* best, float* ms, int* failed);
 int gvdb_batch_synchronize(gvdb_batch b);
 /* Device time from the first stage call after the last synchronize/elapsed to the last one (ms); synchronises. */
 int gvdb_batch_elapsed_ms(gvdb_batch b, float* ms);
