@@ -11,6 +11,7 @@
 #include "kernels.h"
 #include "tree_device.h"
 #include <hip/hip_fp16.h>
+#include <mutex>
 
 namespace gvdb {
 
@@ -198,6 +199,58 @@ hipError_t launch_apron_multi(const ApronArgs* a, int G, Arena* ar, hipStream_t 
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// ------------------------------------------- UpdateApron without allocation
+// The neighbour table of gvdb_update_apron_noalloc.  Every device that loads
+// this code object holds its own copy until the process ends.  In host code
+// the name stands for no device address: noalloc_scratch() looks up the
+// current device's copy with hipGetSymbolAddress, and only that address is
+// handed to kernels.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+static const int NOALLOC_DEVICES = 64;
+
+hipError_t noalloc_scratch(int device, void** ptr) {
+    static std::mutex mu;
+    static void* addr[NOALLOC_DEVICES] = {};
+    *ptr = nullptr;
+    if (device < 0 || device >= NOALLOC_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = p;
+    }
+    *ptr = addr[device];
+    return hipSuccess;
+}
+
+hipError_t launch_apron_noalloc(const Tree& t, void* sp, const void* faces, void* atlas, int atlas_type,
+                                float boundval, int bricks, hipStream_t s) {
+    if (bricks <= 0) return hipSuccess;
+    if (!sp || bricks > NOALLOC_MAX_BRICKS || t.apron != 1) return hipErrorInvalidValue;
+    // offsets first (8-byte entries), then the leaf indices
+    long long* base = static_cast<long long*>(sp);
+    int* nbr = reinterpret_cast<int*>(static_cast<char*>(sp) + size_t(bricks) * 27 * 8);
+    const MultiMap one{1, nullptr, nullptr, knob(KNOB_XCD_ORDER), {}, {}};
+    // every entry of both tables is written by the neighbour pass before the
+    // copy reads it, so nothing from an earlier call is read
+    const NbrArgs na{t, nbr, bricks, nullptr, base};
+    hipLaunchKernelGGL(k_neighbors, dim3(uint32_t((u64(bricks) * 32 + 255) / 256)), dim3(256), 0, s, na, one);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const ApronArgs aa{t, base, nbr, faces, atlas, atlas_type, boundval, bricks};
+    if (atlas_type == T_HALF) hipLaunchKernelGGL(k_apron<__half>, dim3(uint32_t(bricks)), dim3(512), 0, s, aa, one);
+    else hipLaunchKernelGGL(k_apron<float>, dim3(uint32_t(bricks)), dim3(512), 0, s, aa, one);
+    return hipGetLastError();
 }
 
 }  // namespace gvdb

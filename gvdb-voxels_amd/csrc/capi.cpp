@@ -179,6 +179,10 @@ int gvdb_gather_level_set(gvdb_handle h, int subcell_size, int n, float radius, 
                                  accumulate != 0);
 }
 int gvdb_update_apron(gvdb_handle h, int chan, float boundval) { H_OR_FAIL(h); return h->v.update_apron(chan, boundval); }
+int gvdb_update_apron_noalloc(gvdb_handle h, int chan, float boundval) {
+    H_OR_FAIL(h);
+    return h->v.update_apron_noalloc(chan, boundval);
+}
 
 int gvdb_set_camera_orbit(gvdb_handle h, const float angs[3], const float to[3], float dist, float dolly, float fov,
                           float znear, float zfar) {

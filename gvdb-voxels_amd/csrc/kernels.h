@@ -356,6 +356,23 @@ struct ApronArgs {
 };
 hipError_t launch_apron_multi(const ApronArgs* a, int G, Arena* ar, hipStream_t s);
 
+// UpdateApron without allocation: the neighbour table (27 atlas offsets of
+// 8 bytes, then 27 leaf indices of 4 bytes, per brick) lives in a fixed
+// device array of NOALLOC_SCRATCH_BYTES per device instead of memory the
+// library allocates, so at most NOALLOC_MAX_BRICKS bricks fit.
+static const size_t NOALLOC_SCRATCH_BYTES = size_t(32) << 20;
+static const int NOALLOC_BYTES_PER_BRICK = 27 * (8 + 4);
+static const int NOALLOC_MAX_BRICKS = int(NOALLOC_SCRATCH_BYTES / NOALLOC_BYTES_PER_BRICK);
+// The device address of `device`'s copy of the array, looked up once per
+// device (with that device current) and kept; later calls return the kept
+// address.
+hipError_t noalloc_scratch(int device, void** ptr);
+// The neighbour pass and the apron copy for one grid of `bricks` bricks,
+// with the neighbour table in `scratch` (the address noalloc_scratch gave);
+// `faces` as in ApronArgs.
+hipError_t launch_apron_noalloc(const Tree& t, void* scratch, const void* faces, void* atlas, int atlas_type,
+                                float boundval, int bricks, hipStream_t s);
+
 // ---- render (render.hip)
 struct RenderArgs {
     Tree t;

@@ -992,6 +992,37 @@ int Volume::multi_apron(Volume* const* vs, int G, int c, float boundval, const E
     return GVDB_OK;
 }
 
+// UpdateApron for callers that cannot allocate: the same kernels, with the
+// neighbour table in the fixed device array of apron.hip.  Fails instead of
+// doing anything that would allocate: pools changed on the host and not yet
+// on the device, or more bricks than the array holds.  Every call on one
+// device shares that array, so calls that overlap on one device race.
+int Volume::update_apron_noalloc(int c, float boundval) {
+    if (apron_ == 0) return GVDB_OK;
+    if (c < 0 || c >= 8 || !chan_[c].exists) return fail(GVDB_ERR_INVALID, "UpdateApron: no such channel");
+    if (dev_stale_)
+        return fail(GVDB_ERR_STATE, "UpdateApron (no allocation): the pools changed on the host and are not on the "
+                                    "device yet; call UpdateAtlas first");
+    const u64 bricks = pool_[0][0].lastEle;
+    if (bricks == 0) return GVDB_OK;
+    if (atlas_gen_ != topo_gen_) return fail(GVDB_ERR_STATE, "UpdateApron: call UpdateAtlas after the topology changes");
+    if (bricks > u64(NOALLOC_MAX_BRICKS))
+        return fail(GVDB_ERR_OOM, "UpdateApron (no allocation): " + std::to_string(bricks) + " bricks; the fixed "
+                                  "scratch array holds the neighbour table of at most " +
+                                  std::to_string(NOALLOC_MAX_BRICKS));
+    if (!noalloc_) return fail(GVDB_ERR_HIP, "UpdateApron (no allocation): no scratch array on this device");
+    const bool faces = faces_chan_ == c && faces_epoch_ == atlas_epoch_ && faces_topo_ == topo_gen_;
+    stage_begin(ST_APRON);
+    (void)hipEventRecord(kev_[1][0], stream);
+    hipError_t e = launch_apron_noalloc(make_tree(), noalloc_, faces ? faces_.p : nullptr, chan_[c].data.p,
+                                        chan_[c].type, boundval, int(bricks), stream);
+    if (e != hipSuccess) return hip_fail(e, "UpdateApron (no allocation)");
+    (void)hipEventRecord(kev_[1][1], stream);
+    kev_used_[1] = true;
+    stage_end(ST_APRON);
+    return GVDB_OK;
+}
+
 // ---------------------------------------------------------------- Render ----
 int Volume::multi_render(Volume* const* vs, int G, const int* shading, const int* chan, const int* buf, const Exec& ex,
                          int* failed) {

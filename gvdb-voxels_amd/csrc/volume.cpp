@@ -129,6 +129,9 @@ int Volume::init() {
     HIPCHK(small_.ensure(SMALL_WORDS * 4 * 4));
     HIPCHK(arena_.init(1 << 16));
     HIPCHK(hipEventCreateWithFlags(&handoff_, hipEventDisableTiming));
+    // the device's copy of the allocation-free apron's scratch array, found
+    // once per device here so that entry point never has to look it up
+    HIPCHK(noalloc_scratch(device, &noalloc_));
     // Initialize(): default transfer function and range (gvdb_volume_gvdb.cpp:2341-2348)
     tf_host.assign(size_t(TF_SIZE) * 4, 0.f);
     const float a[4] = {0, 0, 0, 0}, b[4] = {1, 1, 1, 0.1f};

@@ -119,6 +119,9 @@ public:
     int insert_points_subcell_fp16(int scs, int n, float radius, const float trans[3], int* len);
     int gather_level_set_fp16(int scs, int n, float radius, const float trans[3], int len, int chanL, int chanC);
     int update_apron(int chan, float boundval);
+    // UpdateApron with the neighbour table in the library's fixed device
+    // array (no allocation, at most NOALLOC_MAX_BRICKS bricks)
+    int update_apron_noalloc(int chan, float boundval);
 
     // render
     CameraParams cam;
@@ -216,6 +219,7 @@ private:
     unsigned atlas_epoch_ = 0, faces_epoch_ = ~0u, faces_topo_ = ~0u;
     int faces_chan_ = -1;
     unsigned nbr_gen_ = ~0u;
+    void* noalloc_ = nullptr;   // this device's copy of the no-allocation scratch array
 
     // points
     PointSrc pts_{nullptr, 0, 0, 12};

@@ -23,6 +23,9 @@ SCN_FLOATS = 68
 # launch-shape knobs of the autotuner (gvdb_set_tuning / gvdb_get_tuning)
 KNOB_RENDER_UNROLL_EA, KNOB_RENDER_UNROLL_ISO, KNOB_RENDER_TILE_EA, KNOB_RENDER_TILE_ISO = 0, 1, 2, 3
 KNOB_GATHER_THREADS, KNOB_APRON_RUN, KNOB_XCD_ORDER = 4, 5, 6
+# the allocation-free UpdateApron: size of its fixed device array and the bricks it holds
+NOALLOC_SCRATCH_BYTES = 32 << 20
+NOALLOC_MAX_BRICKS = NOALLOC_SCRATCH_BYTES // (27 * 12)
 
 NODE_DTYPE = np.dtype([
     ("mLev", "u1"), ("mFlags", "u1"), ("mPriority", "u1"), ("pad", "u1"),
@@ -51,6 +54,7 @@ _SIGS = [
     ("gvdb_clear_channel", _i, [_vp, _i]),
     ("gvdb_insert_points_subcell", _i, [_vp, _i, _i, _f, _vp, ctypes.POINTER(_i)]),
     ("gvdb_gather_density", _i, [_vp, _i, _i, _f, _vp, _i, _i, _i, _i]), ("gvdb_update_apron", _i, [_vp, _i, _f]),
+    ("gvdb_update_apron_noalloc", _i, [_vp, _i, _f]),
     ("gvdb_gather_level_set", _i, [_vp, _i, _i, _f, _vp, _i, _i, _i, _i]),
     ("gvdb_insert_points_subcell_fp16", _i, [_vp, _i, _i, _f, _vp, _vp]),
     ("gvdb_gather_level_set_fp16", _i, [_vp, _i, _i, _f, _vp, _i, _i, _i]),
@@ -303,6 +307,11 @@ class VolumeGVDB:
 
     def UpdateApron(self, chan=0, boundval=0.0):
         return self._chk(load().gvdb_update_apron(self.h, chan, float(boundval)))
+
+    def UpdateApronNoAlloc(self, chan=0, boundval=0.0):
+        """UpdateApron with the neighbour table in the library's fixed device
+        array: allocates nothing; at most NOALLOC_MAX_BRICKS bricks."""
+        return self._chk(load().gvdb_update_apron_noalloc(self.h, chan, float(boundval)))
 
     # scene / render --------------------------------------------------------
     def apply_scene(self, sc):

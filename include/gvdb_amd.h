@@ -162,6 +162,16 @@ int gvdb_load_vbx(gvdb_handle h, const char* path, int force_major, int force_mi
 int gvdb_get_transform(gvdb_handle h, float out12[12]);
 /* UpdateApron(chan, boundval): gvdb_volume_gvdb.cpp:4418-4457 */
 int gvdb_update_apron(gvdb_handle h, int chan, float boundval);
+/* UpdateApron without allocation.  No counterpart in the reference: an addition for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same result as
+ * gvdb_update_apron; the neighbour table it builds lives in a fixed device array of GVDB_NOALLOC_SCRATCH_BYTES that
+ * each device holds from the first gvdb_create on it until the process ends, so grids of more than
+ * GVDB_NOALLOC_MAX_BRICKS bricks are refused (GVDB_ERR_OOM), as are pools built on the host and not yet committed
+ * (GVDB_ERR_STATE: call gvdb_update_atlas first).  Calls no hipMalloc/hipFree.  Every call on one device shares that
+ * array: calls that overlap on one device (other handles' streams included) race; order them on the host. */
+#define GVDB_NOALLOC_SCRATCH_BYTES (32u << 20)
+#define GVDB_NOALLOC_MAX_BRICKS 103563 /* GVDB_NOALLOC_SCRATCH_BYTES / (27 * 12) */
+int gvdb_update_apron_noalloc(gvdb_handle h, int chan, float boundval);
 
 /* Scene: Camera3D::setOrbit + fov/near/far (gvdb_camera.cpp:90-105, :175-228),
  * Light position, Scene setters SetBackgroundClr/SetExtinct/SetSteps/SetCutoff/
