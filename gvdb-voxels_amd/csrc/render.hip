@@ -37,12 +37,17 @@ __device__ inline float ldv<float>(const float* a, u64 o) { return a[o]; }
 template <>
 __device__ inline float ldv<__half>(const __half* a, u64 o) { return __half2float(a[o]); }
 
-// Trilinear sample at brick-local position p (interior coordinates) from the
-// padded brick starting at `b`; equals an unnormalised linear texture fetch
-// at atlas coordinate p + mValue.  Split into the fetch (four x-adjacent
-// pairs, one 8-byte load each for FP32) and the blend, so the loads of
-// several samples can be in flight together.  Indices are clamped into the
-// brick, so a fetch at any position stays inside it.
+// Trilinear sample at atlas coordinate (x, y, z) = p + mValue, the sum formed
+// in FP32 as the reference's tex3D call forms it (far from the atlas origin
+// the position carries that sum's rounding), read from the padded brick
+// whose first voxel is atlas voxel (cx, cy, cz) and whose data starts at
+// `b`: an unnormalised linear fetch (texel centres at integer + 0.5) with
+// exact weights, blended as a + f (b - a) along x, then y, then z.  Split
+// into the fetch (four x-adjacent pairs, one 8-byte load each for FP32) and
+// the blend, so the loads of several samples can be in flight together.
+// A march sample (p in [0, res)) has its lower taps at padded indices 0..8;
+// indices are clamped into the brick, so a fetch at any position stays
+// inside it.
 struct Taps {
     float c[8];   // c000 c100 c010 c110 c001 c101 c011 c111
     float fx, fy, fz;
@@ -65,14 +70,20 @@ __device__ inline void ld_pair<__half>(const __half* p, float& a, float& b) {
     b = __high2float(v);
 }
 
+struct Corner {
+    int c[3];      // atlas voxel of the padded brick's first voxel (mValue - apron)
+    float o[3];    // mValue as floats: atlas position = p + o
+};
+
 template <typename AT>
-__device__ inline void tri_fetch(const AT* b, int br, float px, float py, float pz, Taps& T) {
-    float qx = px + 0.5f, qy = py + 0.5f, qz = pz + 0.5f;
-    float fx0 = floorf(qx), fy0 = floorf(qy), fz0 = floorf(qz);
-    int ix = min(max(int(fx0), 0), br - 2), iy = min(max(int(fy0), 0), br - 2), iz = min(max(int(fz0), 0), br - 2);
-    T.fx = qx - fx0;
-    T.fy = qy - fy0;
-    T.fz = qz - fz0;
+__device__ inline void tri_fetch(const AT* b, int br, const Corner& K, float x, float y, float z, Taps& T) {
+    const float ux = x - 0.5f, uy = y - 0.5f, uz = z - 0.5f;
+    const float fx0 = floorf(ux), fy0 = floorf(uy), fz0 = floorf(uz);
+    const int ix = min(max(int(fx0) - K.c[0], 0), br - 2), iy = min(max(int(fy0) - K.c[1], 0), br - 2),
+              iz = min(max(int(fz0) - K.c[2], 0), br - 2);
+    T.fx = ux - fx0;
+    T.fy = uy - fy0;
+    T.fz = uz - fz0;
     const AT* o = b + ((iz * br + iy) * br + ix);
     const int sy = br, sz = br * br;
     ld_pair<AT>(o, T.c[0], T.c[1]);
@@ -89,13 +100,34 @@ __device__ inline float tri_blend(const Taps& T) {
     return c0 + T.fz * (c1 - c0);
 }
 
+// The same sample where the lower tap may be the brick's last padded voxel
+// (the gradient's tap half a voxel past a march sample): that tap's weight
+// is 1 and its upper neighbour (outside the brick) is weighted 0, so the
+// fetch takes the pair below and moves the upper value down.
 template <typename AT>
-__device__ inline float tri(const AT* b, int br, float px, float py, float pz) {
+__device__ inline float tri_edge(const AT* b, int br, const Corner& K, float x, float y, float z) {
     Taps T;
-    tri_fetch<AT>(b, br, px, py, pz, T);
+    tri_fetch<AT>(b, br, K, x, y, z, T);
+    float* c = T.c;
+    if (int(floorf(x - 0.5f)) - K.c[0] == br - 1)
+        for (int k = 0; k < 8; k += 2) c[k] = c[k + 1];
+    if (int(floorf(y - 0.5f)) - K.c[1] == br - 1) {
+        c[0] = c[2]; c[1] = c[3]; c[4] = c[6]; c[5] = c[7];
+    }
+    if (int(floorf(z - 0.5f)) - K.c[2] == br - 1)
+        for (int k = 0; k < 4; k++) c[k] = c[k + 4];
     return tri_blend(T);
 }
 
+template <typename AT>
+__device__ inline Corner brick_corner(const Tree& t, const Node* n) {
+    Corner K;
+    for (int a = 0; a < 3; a++) {
+        K.c[a] = n->mValue[a] - t.apron;
+        K.o[a] = float(n->mValue[a]);
+    }
+    return K;
+}
 
 struct Hdda {
     f3 pos, dir, tDel, t, tSide;
@@ -147,6 +179,7 @@ __device__ void deep_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 
     const Node* n = t.nodes[0] + nodeid;
     const AT* b = brick_base(t, atlas, n);
     const int br = t.brick_res;
+    const Corner K = brick_corner<AT>(t, n);
     const float dstep = A.s.steps[0];
     f3 vmin = mk3(float(n->mPos[0]), float(n->mPos[1]), float(n->mPos[2]));
     float tx = dstep * ceilf(tt.x / dstep);
@@ -163,7 +196,7 @@ __device__ void deep_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 
 #pragma unroll
         for (int u = 0; u < U; u++) {
             ok[u] = it + u < MAX_ITER && inside(q, r0);
-            tri_fetch<AT>(b, br, q.x, q.y, q.z, T[u]);
+            tri_fetch<AT>(b, br, K, q.x + K.o[0], q.y + K.o[1], q.z + K.o[2], T[u]);
             q = q + wpt;
         }
         float raw[U];
@@ -201,6 +234,7 @@ __device__ void surf_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 
     const Tree& t = A.t;
     const Node* n = t.nodes[0] + nodeid;
     const AT* b = brick_base(t, atlas, n);
+    const Corner K = brick_corner<AT>(t, n);
     const float dstep = A.s.steps[0];
     f3 vmin = mk3(float(n->mPos[0]), float(n->mPos[1]), float(n->mPos[2]));
     float tx = dstep * ceilf(tt.x / dstep);
@@ -215,7 +249,7 @@ __device__ void surf_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 
 #pragma unroll
         for (int u = 0; u < U; u++) {
             ok[u] = it + u < MAX_ITER && inside(q, r0);
-            tri_fetch<AT>(b, br, q.x, q.y, q.z, T[u]);
+            tri_fetch<AT>(b, br, K, q.x + K.o[0], q.y + K.o[1], q.z + K.o[2], T[u]);
             q = q + dir * dstep;
         }
 #pragma unroll
@@ -223,10 +257,12 @@ __device__ void surf_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 
             if (!ok[u]) return;
             if (tri_blend(T[u]) >= th) {
                 hit = p + vmin;
+                // getGradient at the atlas position p + o
+                const f3 P = mk3(p.x + K.o[0], p.y + K.o[1], p.z + K.o[2]);
                 f3 g;
-                g.x = tri<AT>(b, br, p.x - 0.5f, p.y, p.z) - tri<AT>(b, br, p.x + 0.5f, p.y, p.z);
-                g.y = tri<AT>(b, br, p.x, p.y - 0.5f, p.z) - tri<AT>(b, br, p.x, p.y + 0.5f, p.z);
-                g.z = tri<AT>(b, br, p.x, p.y, p.z - 0.5f) - tri<AT>(b, br, p.x, p.y, p.z + 0.5f);
+                g.x = tri_edge<AT>(b, br, K, P.x - 0.5f, P.y, P.z) - tri_edge<AT>(b, br, K, P.x + 0.5f, P.y, P.z);
+                g.y = tri_edge<AT>(b, br, K, P.x, P.y - 0.5f, P.z) - tri_edge<AT>(b, br, K, P.x, P.y + 0.5f, P.z);
+                g.z = tri_edge<AT>(b, br, K, P.x, P.y, P.z - 0.5f) - tri_edge<AT>(b, br, K, P.x, P.y, P.z + 0.5f);
                 nrm = norm3(g);
                 return;
             }

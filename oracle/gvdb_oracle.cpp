@@ -1209,7 +1209,11 @@ template <typename T> struct RenderCtx {
     T back[4];
 };
 
-// Unnormalised linear texture fetch with clamp addressing (exact weights).
+// Unnormalised linear texture fetch with clamp addressing (exact weights),
+// blended as a + f (b - a) along x, then y, then z.  The reference leaves the
+// weights to the texture unit (8-bit fixed-point fractions), so the blend
+// form is not the reference's; this one is fixed so that an FP32 march can be
+// compared operation for operation.
 template <typename T> static T tex3(const RenderCtx<T>& c, T x, T y, T z) {
     T u[3] = {x - T(0.5), y - T(0.5), z - T(0.5)};
     int i0[3], i1[3];
@@ -1222,13 +1226,14 @@ template <typename T> static T tex3(const RenderCtx<T>& c, T x, T y, T z) {
         i1[a] = std::min(std::max(i + 1, 0), c.ares[a] - 1);
     }
     auto at = [&](int X, int Y, int Z) { return (*c.atlas)[(size_t(Z) * c.ares[1] + Y) * c.ares[0] + X]; };
-    T c00 = at(i0[0], i0[1], i0[2]) * (1 - f[0]) + at(i1[0], i0[1], i0[2]) * f[0];
-    T c10 = at(i0[0], i1[1], i0[2]) * (1 - f[0]) + at(i1[0], i1[1], i0[2]) * f[0];
-    T c01 = at(i0[0], i0[1], i1[2]) * (1 - f[0]) + at(i1[0], i0[1], i1[2]) * f[0];
-    T c11 = at(i0[0], i1[1], i1[2]) * (1 - f[0]) + at(i1[0], i1[1], i1[2]) * f[0];
-    T c0 = c00 * (1 - f[1]) + c10 * f[1];
-    T c1 = c01 * (1 - f[1]) + c11 * f[1];
-    return c0 * (1 - f[2]) + c1 * f[2];
+    auto lerp = [](T a, T b, T w) { return a + w * (b - a); };
+    T c00 = lerp(at(i0[0], i0[1], i0[2]), at(i1[0], i0[1], i0[2]), f[0]);
+    T c10 = lerp(at(i0[0], i1[1], i0[2]), at(i1[0], i1[1], i0[2]), f[0]);
+    T c01 = lerp(at(i0[0], i0[1], i1[2]), at(i1[0], i0[1], i1[2]), f[0]);
+    T c11 = lerp(at(i0[0], i1[1], i1[2]), at(i1[0], i1[1], i1[2]), f[0]);
+    T c0 = lerp(c00, c10, f[1]);
+    T c1 = lerp(c01, c11, f[1]);
+    return lerp(c0, c1, f[2]);
 }
 
 template <typename T> struct HDDA {
