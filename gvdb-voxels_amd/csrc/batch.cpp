@@ -1,8 +1,14 @@
 // batch.cpp -- see batch.h.
 #include "batch.h"
+#include "graph.h"
 #include "../../include/gvdb_amd.h"
 
 namespace gvdb {
+
+Exec Batch::exec() {
+    if (cap_) return Exec{stream, &cap_->arena, dsmall_, hsmall_, cap_};
+    return Exec{stream, &arena_, dsmall_, hsmall_};
+}
 
 Batch::Batch(int dev, int mg) : device(dev), max_grids(mg) {}
 
@@ -56,6 +62,21 @@ int Batch::enter(Volume* const* v, int G, int* failed) {
                 err = "batch: a volume appears twice in one call";
                 return GVDB_ERR_INVALID;
             }
+        if (cap_) {
+            // a recording cannot wait on another stream: the grid must already
+            // run on the batch's stream (any earlier batched call moves it)
+            if (v[g]->stream != stream || (v[g]->capturing() && v[g]->capture() != cap_)) {
+                if (failed) *failed = g;
+                err = "batch capture: every grid must take part in a batched call before the recording";
+                if (!cap_->failed) {
+                    cap_->failed = true;
+                    cap_->err = err;
+                }
+                return GVDB_ERR_STATE;
+            }
+            v[g]->set_capture(cap_);
+            continue;
+        }
         if (v[g]->stream != stream) {
             int rc = v[g]->use_stream(shared);
             if (rc) {
@@ -65,7 +86,7 @@ int Batch::enter(Volume* const* v, int G, int* failed) {
             }
         }
     }
-    if (!started_) {
+    if (!started_ && !cap_) {
         (void)hipEventRecord(ev0_, stream);
         started_ = true;
     }
@@ -73,7 +94,11 @@ int Batch::enter(Volume* const* v, int G, int* failed) {
 }
 
 int Batch::leave(int rc, Volume* const* v, int G, int* failed) {
-    (void)hipEventRecord(ev1_, stream);
+    if (!cap_) (void)hipEventRecord(ev1_, stream);
+    else if (rc && !cap_->failed) {
+        cap_->failed = true;
+        cap_->err = (failed && *failed >= 0 && *failed < G) ? v[*failed]->err : err;
+    }
     if (rc && failed && *failed >= 0 && *failed < G) err = v[*failed]->err;
     return rc;
 }
@@ -118,6 +143,10 @@ int Batch::render(Volume* const* v, int G, const int* shading, const int* chan, 
 }
 
 int Batch::synchronize() {
+    if (cap_) {
+        err = "batch: synchronize while recording a graph";
+        return GVDB_ERR_STATE;
+    }
     hipError_t e = hipStreamSynchronize(stream);
     arena_.reset();
     if (e != hipSuccess) {
@@ -129,7 +158,7 @@ int Batch::synchronize() {
 
 int Batch::elapsed_ms(float* ms) {
     *ms = 0.f;
-    if (!started_) return GVDB_OK;
+    if (!started_ || cap_) return GVDB_OK;
     hipError_t e = hipEventSynchronize(ev1_);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, ev0_, ev1_);
     started_ = false;

@@ -35,6 +35,10 @@ public:
                        int* failed);
     int update_apron(Volume* const* v, int G, int chan, float boundval, int* failed);
     int render(Volume* const* v, int G, const int* shading, const int* chan, const int* buf, int* failed);
+    // HIP graph capture of the batched chain (graph.cpp): between these the
+    // stage calls are recorded, not run
+    int capture_begin(Graph*& out);
+    int capture_end(Graph* g);
     int synchronize();
     // milliseconds between the start of the first and the end of the last
     // stage call since the previous synchronize (HIP events on the stream)
@@ -46,9 +50,10 @@ private:
     uint32_t* hsmall_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     bool started_ = false;
+    Graph* cap_ = nullptr;   // the graph being recorded, if any
     int enter(Volume* const* v, int G, int* failed);
     int leave(int rc, Volume* const* v, int G, int* failed);
-    Exec exec() { return Exec{stream, &arena_, dsmall_, hsmall_}; }
+    Exec exec();
 };
 
 }  // namespace gvdb

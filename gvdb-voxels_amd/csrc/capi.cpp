@@ -3,6 +3,7 @@
 #include "kernels.h"
 #include "batch.h"
 #include "serve.h"
+#include "graph.h"
 #include "../../include/gvdb_amd.h"
 #include <new>
 #include <cstring>
@@ -23,11 +24,21 @@ struct gvdb_server_s {
     gvdb_server_s(int d, unsigned long long b, int m) : s(d, b, m) {}
 };
 
+struct gvdb_graph_s {
+    std::unique_ptr<gvdb::Graph> g;
+    std::string err;
+};
+
 static_assert(sizeof(gvdb_node) == sizeof(gvdb::Node), "gvdb_node mirrors the 64-byte node");
 
 #define H_OR_FAIL(h)                     \
     if (!(h)) return GVDB_ERR_INVALID;   \
     (void)hipSetDevice((h)->v.device)
+
+// calls outside the per-frame chain fail (and fail the recording) while the
+// grid records a graph
+#define NOT_RECORDING(h, what) \
+    if ((h)->v.capturing()) return (h)->v.not_recordable(what)
 
 extern "C" {
 
@@ -53,48 +64,48 @@ int gvdb_destroy(gvdb_handle h) {
 const char* gvdb_last_error(gvdb_handle h) { return h ? h->v.err.c_str() : "null handle"; }
 
 int gvdb_configure(gvdb_handle h, int levels, const int* log2dims, const int* initcnt) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "Configure");
     if (!log2dims) return GVDB_ERR_INVALID;
     return h->v.configure(levels, log2dims, initcnt);
 }
-int gvdb_set_channel_default(gvdb_handle h, int cx, int cy, int cz) { H_OR_FAIL(h); return h->v.set_channel_default(cx, cy, cz); }
+int gvdb_set_channel_default(gvdb_handle h, int cx, int cy, int cz) { H_OR_FAIL(h); NOT_RECORDING(h, "SetChannelDefault"); return h->v.set_channel_default(cx, cy, cz); }
 int gvdb_add_channel(gvdb_handle h, int chan, int dtype, int apron, int filter, int border, int ax, int ay, int az) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "AddChannel");
     return h->v.add_channel(chan, dtype, apron, filter, border, ax, ay, az);
 }
 int gvdb_set_epsilon(gvdb_handle h, float eps, int maxiter) { H_OR_FAIL(h); return h->v.set_epsilon(eps, maxiter); }
 int gvdb_set_points_gpu(gvdb_handle h, const void* dptr, int n, int offset, int stride) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "SetPoints");
     return h->v.set_points_gpu(dptr, n, offset, stride);
 }
-int gvdb_upload_points(gvdb_handle h, const float* host_xyz, int n) { H_OR_FAIL(h); return h->v.upload_points(host_xyz, n); }
+int gvdb_upload_points(gvdb_handle h, const float* host_xyz, int n) { H_OR_FAIL(h); NOT_RECORDING(h, "SetPoints"); return h->v.upload_points(host_xyz, n); }
 int gvdb_request_full_rebuild(gvdb_handle h, int on) { H_OR_FAIL(h); return h->v.request_full_rebuild(on != 0); }
 int gvdb_rebuild_topology(gvdb_handle h, int n, float radius, const float orig[3]) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "RebuildTopology");
     const float z[3] = {0, 0, 0};
     return h->v.rebuild_topology(n, radius, orig ? orig : z);
 }
 int gvdb_accumulate_topology(gvdb_handle h, int n, float radius, const float orig[3], int min_depth) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "AccumulateTopology");
     const float z[3] = {0, 0, 0};
     return h->v.accumulate_topology(n, radius, orig ? orig : z, min_depth);
 }
 int gvdb_rebuild_topology_cpu(gvdb_handle h, int n, const float orig[3], const float* host_xyz) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "RebuildTopologyCPU");
     const float z[3] = {0, 0, 0};
     return h->v.rebuild_topology_cpu(n, orig ? orig : z, host_xyz);
 }
 int gvdb_activate_space(gvdb_handle h, const float pos[3]) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "ActivateSpace");
     if (!pos) return GVDB_ERR_INVALID;
     return h->v.activate_space(pos);
 }
-int gvdb_clear(gvdb_handle h) { H_OR_FAIL(h); return h->v.clear(); }
+int gvdb_clear(gvdb_handle h) { H_OR_FAIL(h); NOT_RECORDING(h, "Clear"); return h->v.clear(); }
 int gvdb_finish_topology(gvdb_handle h, int commit_pool, int compute_bounds) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "FinishTopology");
     return h->v.finish_topology(commit_pool != 0, compute_bounds != 0);
 }
-int gvdb_update_atlas(gvdb_handle h) { H_OR_FAIL(h); return h->v.update_atlas(); }
+int gvdb_update_atlas(gvdb_handle h) { H_OR_FAIL(h); NOT_RECORDING(h, "UpdateAtlas"); return h->v.update_atlas(); }
 int gvdb_clear_channel(gvdb_handle h, int chan) { H_OR_FAIL(h); return h->v.clear_channel(chan); }
 int gvdb_insert_points_subcell(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int* out_len) {
     H_OR_FAIL(h);
@@ -110,16 +121,16 @@ int gvdb_gather_density(gvdb_handle h, int subcell_size, int n, float radius, co
 }
 int gvdb_compute(gvdb_handle h, int effect, int chan, int iterations, const float params[3], int update_apron,
                  int skip_aprons, float boundval) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "Compute");
     const float z[3] = {0, 0, 0};
     return h->v.compute(effect, chan, iterations, params ? params : z, update_apron != 0, skip_aprons != 0, boundval);
 }
 int gvdb_compute_kernel(gvdb_handle h, void* fn, int chan, int update_apron, int skip_aprons) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "ComputeKernel");
     return h->v.compute_kernel(fn, chan, update_apron != 0, skip_aprons != 0);
 }
 int gvdb_render_kernel(gvdb_handle h, void* fn, int chan, int rbuf) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "RenderKernel");
     return h->v.render_kernel(fn, chan, rbuf);
 }
 int gvdb_module_load(const char* path, void** mod) {
@@ -147,9 +158,9 @@ int gvdb_module_unload(void* mod) {
     if (!mod) return GVDB_ERR_INVALID;
     return hipModuleUnload(reinterpret_cast<hipModule_t>(mod)) == hipSuccess ? GVDB_OK : GVDB_ERR_HIP;
 }
-int gvdb_save_vbx(gvdb_handle h, const char* path) { H_OR_FAIL(h); return h->v.save_vbx(path); }
+int gvdb_save_vbx(gvdb_handle h, const char* path) { H_OR_FAIL(h); NOT_RECORDING(h, "SaveVBX"); return h->v.save_vbx(path); }
 int gvdb_load_vbx(gvdb_handle h, const char* path, int force_major, int force_minor) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "LoadVBX");
     return h->v.load_vbx(path, force_major, force_minor);
 }
 int gvdb_get_transform(gvdb_handle h, float out12[12]) {
@@ -160,14 +171,14 @@ int gvdb_get_transform(gvdb_handle h, float out12[12]) {
 }
 int gvdb_insert_points_subcell_fp16(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3],
                                     int* out_len) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "InsertPointsSubcell_FP16");
     if (!out_len) return GVDB_ERR_INVALID;
     const float z[3] = {0, 0, 0};
     return h->v.insert_points_subcell_fp16(subcell_size, n, radius, trans ? trans : z, out_len);
 }
 int gvdb_gather_level_set_fp16(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len,
                                int chan_levelset, int chan_clr) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "GatherLevelSet_FP16");
     const float z[3] = {0, 0, 0};
     return h->v.gather_level_set_fp16(subcell_size, n, radius, trans ? trans : z, sc_len, chan_levelset, chan_clr);
 }
@@ -219,10 +230,10 @@ int gvdb_linear_transfer_func(gvdb_handle h, float t0, float t1, const float a[4
     if (!a || !b || !(t1 > t0)) return GVDB_ERR_INVALID;
     return h->v.linear_tf(t0, t1, a, b);
 }
-int gvdb_commit_transfer_func(gvdb_handle h) { H_OR_FAIL(h); return h->v.commit_tf(); }
-int gvdb_add_render_buf(gvdb_handle h, int buf, int w, int hgt, int bpp) { H_OR_FAIL(h); return h->v.add_render_buf(buf, w, hgt, bpp); }
+int gvdb_commit_transfer_func(gvdb_handle h) { H_OR_FAIL(h); NOT_RECORDING(h, "CommitTransferFunc"); return h->v.commit_tf(); }
+int gvdb_add_render_buf(gvdb_handle h, int buf, int w, int hgt, int bpp) { H_OR_FAIL(h); NOT_RECORDING(h, "AddRenderBuf"); return h->v.add_render_buf(buf, w, hgt, bpp); }
 int gvdb_render(gvdb_handle h, int shading, int chan, int buf) { H_OR_FAIL(h); return h->v.render(shading, chan, buf); }
-int gvdb_read_render_buf(gvdb_handle h, int buf, void* dst) { H_OR_FAIL(h); return h->v.read_render_buf(buf, dst); }
+int gvdb_read_render_buf(gvdb_handle h, int buf, void* dst) { H_OR_FAIL(h); NOT_RECORDING(h, "ReadRenderBuf"); return h->v.read_render_buf(buf, dst); }
 int gvdb_set_render_unroll(int shading, int unroll) {
     if (shading != GVDB_SHADE_VOLUME && shading != GVDB_SHADE_TRILINEAR) return GVDB_ERR_INVALID;
     return gvdb::set_knob(shading == GVDB_SHADE_VOLUME ? gvdb::KNOB_RENDER_UNROLL_EA : gvdb::KNOB_RENDER_UNROLL_ISO,
@@ -231,16 +242,16 @@ int gvdb_set_render_unroll(int shading, int unroll) {
                : GVDB_ERR_INVALID;
 }
 int gvdb_tune_render(gvdb_handle h, int shading, int chan, int buf, int* best, float* ms3) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "tuning");
     return gvdb::tune_render_unroll(h->v, shading, chan, buf, best, ms3);
 }
 int gvdb_tune_render_shape(gvdb_handle h, int shading, int chan, int buf, int* best2, float* ms12) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "tuning");
     return gvdb::tune_render(h->v, shading, chan, buf, best2, ms12);
 }
 int gvdb_tune_gather(gvdb_handle h, int subcell_size, int n, float radius, const float trans[3], int sc_len, int chan,
                      int* best_threads, float* ms2) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "tuning");
     const float z[3] = {0, 0, 0};
     return gvdb::tune_gather(h->v, subcell_size, n, radius, trans ? trans : z, sc_len, chan, best_threads, ms2);
 }
@@ -264,23 +275,23 @@ int gvdb_get_scene_block(gvdb_handle h, int w, int hgt, float* out68) {
     return GVDB_OK;
 }
 int gvdb_get_pool(gvdb_handle h, int grp, int lev, void* dst, uint64_t* count, uint64_t* stride) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "getNode pools");
     gvdb::u64 c = 0, s = 0;
     int rc = h->v.get_pool(grp, lev, dst, &c, &s);
     if (count) *count = c;
     if (stride) *stride = s;
     return rc;
 }
-int gvdb_get_info(gvdb_handle h, int* out2, float* outf12) { H_OR_FAIL(h); return h->v.get_info(out2, outf12); }
+int gvdb_get_info(gvdb_handle h, int* out2, float* outf12) { H_OR_FAIL(h); NOT_RECORDING(h, "reading the tree descriptor"); return h->v.get_info(out2, outf12); }
 int gvdb_get_atlas_info(gvdb_handle h, int chan, int* out7) {
     H_OR_FAIL(h);
     if (!out7) return GVDB_ERR_INVALID;
     return h->v.get_atlas_info(chan, out7);
 }
-int gvdb_read_atlas(gvdb_handle h, int chan, void* dst) { H_OR_FAIL(h); return h->v.read_atlas(chan, dst); }
-int gvdb_read_atlas_map(gvdb_handle h, void* dst) { H_OR_FAIL(h); return h->v.read_atlas_map(dst); }
+int gvdb_read_atlas(gvdb_handle h, int chan, void* dst) { H_OR_FAIL(h); NOT_RECORDING(h, "reading the atlas"); return h->v.read_atlas(chan, dst); }
+int gvdb_read_atlas_map(gvdb_handle h, void* dst) { H_OR_FAIL(h); NOT_RECORDING(h, "reading the atlas map"); return h->v.read_atlas_map(dst); }
 int gvdb_read_subcells(gvdb_handle h, int which, void* dst, uint64_t* count) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "reading subcells");
     gvdb::u64 c = 0;
     int rc = h->v.read_subcells(which, dst, &c);
     if (count) *count = c;
@@ -293,16 +304,16 @@ int gvdb_get_stream(gvdb_handle h, void** stream) {
     return GVDB_OK;
 }
 int gvdb_synchronize(gvdb_handle h) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "synchronize");
     return hipStreamSynchronize(h->v.stream) == hipSuccess ? GVDB_OK : GVDB_ERR_HIP;
 }
 int gvdb_stage_ms(gvdb_handle h, float* out8) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "stage timing");
     if (!out8) return GVDB_ERR_INVALID;
     return h->v.stage_ms(out8);
 }
 int gvdb_kernel_ms(gvdb_handle h, float* out4) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "kernel timing");
     if (!out4) return GVDB_ERR_INVALID;
     return h->v.kernel_ms(out4);
 }
@@ -404,7 +415,7 @@ int gvdb_batch_elapsed_ms(gvdb_batch b, float* ms) {
     return b->b.elapsed_ms(ms);
 }
 int gvdb_release_stream(gvdb_handle h) {
-    H_OR_FAIL(h);
+    H_OR_FAIL(h); NOT_RECORDING(h, "releasing the stream");
     return h->v.use_stream(std::shared_ptr<gvdb::SharedStream>());
 }
 
@@ -454,5 +465,59 @@ int gvdb_server_run(gvdb_server s, int mode, double* latency, int* status, doubl
     return s->s.run(mode, latency, status, wall);
 }
 unsigned long long gvdb_server_device_bytes(gvdb_server s) { return s ? s->s.device_bytes() : 0ull; }
+
+// ------------------------------------------------------------- graphs
+int gvdb_capture_begin(gvdb_handle h, gvdb_graph* out) {
+    if (!out) return GVDB_ERR_INVALID;
+    *out = nullptr;
+    H_OR_FAIL(h);
+    gvdb_graph_s* g = new (std::nothrow) gvdb_graph_s();
+    if (!g) return GVDB_ERR_OOM;
+    gvdb::Graph* raw = nullptr;
+    int rc = gvdb::capture_begin(h->v, raw, h->v.err);
+    if (rc) { delete g; return rc; }
+    g->g.reset(raw);
+    *out = g;
+    return GVDB_OK;
+}
+int gvdb_capture_end(gvdb_handle h, gvdb_graph g) {
+    H_OR_FAIL(h);
+    if (!g || !g->g) return GVDB_ERR_INVALID;
+    int rc = gvdb::capture_end(h->v, g->g.get(), g->err);
+    if (rc) h->v.err = g->err;
+    return rc;
+}
+int gvdb_batch_capture_begin(gvdb_batch b, gvdb_graph* out) {
+    if (!b || !out) return GVDB_ERR_INVALID;
+    *out = nullptr;
+    gvdb_graph_s* g = new (std::nothrow) gvdb_graph_s();
+    if (!g) return GVDB_ERR_OOM;
+    gvdb::Graph* raw = nullptr;
+    int rc = b->b.capture_begin(raw);
+    if (rc) { delete g; return rc; }
+    g->g.reset(raw);
+    *out = g;
+    return GVDB_OK;
+}
+int gvdb_batch_capture_end(gvdb_batch b, gvdb_graph g) {
+    if (!b || !g || !g->g) return GVDB_ERR_INVALID;
+    int rc = b->b.capture_end(g->g.get());
+    if (rc) g->err = b->b.err;
+    return rc;
+}
+int gvdb_graph_launch(gvdb_graph g) {
+    if (!g || !g->g) return GVDB_ERR_INVALID;
+    return gvdb::graph_launch(g->g.get(), g->err);
+}
+int gvdb_graph_synchronize(gvdb_graph g) {
+    if (!g || !g->g) return GVDB_ERR_INVALID;
+    return gvdb::graph_synchronize(g->g.get(), g->err);
+}
+int gvdb_graph_destroy(gvdb_graph g) {
+    if (!g) return GVDB_ERR_INVALID;
+    delete g;
+    return GVDB_OK;
+}
+const char* gvdb_graph_last_error(gvdb_graph g) { return g ? g->err.c_str() : "null graph"; }
 
 }  // extern "C"

@@ -108,6 +108,9 @@ struct Arena {
     uint8_t* host = nullptr;
     uint8_t* dev = nullptr;
     size_t cap = 0, used = 0;
+    // a captured graph's arena: never wraps (wrapping waits for the stream),
+    // so its tables stay as the graph's copies read them at every replay
+    bool fixed = false;
     hipError_t init(size_t bytes);
     void release();
     // Returns the device address of a copy of src[0..n), or null on failure.

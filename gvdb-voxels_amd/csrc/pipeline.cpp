@@ -6,6 +6,7 @@
 // whole set, not once per grid.  A single VolumeGVDB call is the same code
 // with G = 1.  Grids of one call share the stream and staging in `Exec`.
 #include "volume.h"
+#include "graph.h"
 #include "../../include/gvdb_amd.h"
 #include <algorithm>
 #include <cmath>
@@ -26,12 +27,10 @@ static inline float key_float(uint32_t k) {
     return f;
 }
 
-// small per-grid result words (device dsmall / pinned hsmall, SMALL_WORDS each)
-enum { SM_BBOX = 0, SM_LINKED = 8, SM_TOTAL = 9, SM_ORPHANS = 10, SM_BUCKETS = 11, SM_LEN = 12, SM_COUNTS = 16, SM_BOUNDS = 40,
-       SM_UNFLAGGED = 48 };
-
 static int drain(Volume* v, const Exec& ex, int G) {
     XCHK(v, hipMemcpyAsync(ex.hsmall, ex.dsmall, size_t(G) * SMALL_WORDS * 4, hipMemcpyDeviceToHost, ex.s));
+    // recording: the copy is part of the graph; its words are read after a replay
+    if (ex.cap) return GVDB_OK;
     XCHK(v, hipStreamSynchronize(ex.s));
     if (ex.ar) ex.ar->reset();
     return GVDB_OK;
@@ -46,6 +45,7 @@ static int drain(Volume* v, const Exec& ex, int G) {
 // ------------------------------------------------------- RebuildTopology ----
 int Volume::multi_rebuild_topology(Volume* const* vs, int G, const int* n, const float* radius, const float* orig3,
                                    const Exec& ex, int* failed) {
+    if (ex.cap) FAILG(0, vs[0]->not_recordable("RebuildTopology"));
     for (int g = 0; g < G; g++) {
         Volume* v = vs[g];
         if (v->levels_ == 0) FAILG(g, v->fail(GVDB_ERR_STATE, "RebuildTopology before Configure"));
@@ -335,6 +335,7 @@ int Volume::multi_full_build(Volume* const* vs, int G, const int* n, const float
 // incremental build keeping the flags of the existing leaves.  A depth above
 // the root's level is refused (the reference would walk past its root).
 int Volume::accumulate_topology(int n, float radius, const float orig[3], int min_depth) {
+    if (capture_) return not_recordable("AccumulateTopology");
     if (levels_ == 0) return fail(GVDB_ERR_STATE, "AccumulateTopology before Configure");
     if (n <= 0 || !pts_.base || n > pts_.n) return fail(GVDB_ERR_INVALID, "AccumulateTopology: point count exceeds SetPoints");
     if (!(radius >= 0.0f) || radius > 32.0f) return fail(GVDB_ERR_UNSUPPORTED, "AccumulateTopology: radius must be in [0, 32]");
@@ -491,6 +492,7 @@ int Volume::incremental_topology(int n, float radius, const float orig[3], int D
 
 // -------------------------------------------------------- FinishTopology ----
 int Volume::multi_finish_topology(Volume* const* vs, int G, bool commit, bool bounds, const Exec& ex, int* failed) {
+    if (ex.cap) FAILG(0, vs[0]->not_recordable("FinishTopology"));
     for (int g = 0; g < G; g++)
         if (vs[g]->levels_ == 0) FAILG(g, vs[g]->fail(GVDB_ERR_STATE, "FinishTopology before Configure"));
     if (bounds) {
@@ -542,6 +544,7 @@ int Volume::multi_finish_topology(Volume* const* vs, int G, bool commit, bool bo
 
 // ----------------------------------------------------------- UpdateAtlas ----
 int Volume::multi_update_atlas(Volume* const* vs, int G, const Exec& ex, int* failed) {
+    if (ex.cap) FAILG(0, vs[0]->not_recordable("UpdateAtlas"));
     std::vector<FillArgs> fa;
     std::vector<FlagArgs> fl;
     std::vector<ScanArgs> sa;
@@ -599,8 +602,9 @@ int Volume::multi_clear_channel(Volume* const* vs, int G, int c, const Exec& ex,
         if (c < 0 || c >= 8 || !v->chan_[c].exists) FAILG(g, v->fail(GVDB_ERR_INVALID, "ClearChannel: no such channel"));
         fa.push_back(FillArgs{v->chan_[c].data.p, v->chan_[c].data.bytes / 4, 0, 0});
         v->atlas_epoch_++;
+        if (ex.cap) ex.cap->note(v);
     }
-    if (G == 1) vs[0]->stage_begin(ST_CLEAR);
+    if (G == 1) vs[0]->stage_begin(ST_CLEAR);   // (no events while recording)
     XCHK(vs[0], launch_fill_multi(fa.data(), G, ex.ar, ex.s));
     if (G == 1) vs[0]->stage_end(ST_CLEAR);
     return GVDB_OK;
@@ -644,6 +648,11 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         // pull path when the neighbourhood of a brick stays inside its 27 neighbours
         pl.R = int(ceilf((radius[g] + 1.0f) / float(scs)));
         pl.pull = pl.R <= pl.sd && pull_cells(pl.sd, pl.R) <= pull_max_cells() && pl.spb <= pull_max_spb();
+        if (ex.cap) {
+            if (!pl.pull || rule != BIN_FLOAT)
+                FAILG(g, v->not_recordable("InsertPointsSubcell outside the bucketed path"));
+            ex.cap->note(v);
+        }
         XCHK(v, v->sc_flags_.ensure(size_t(pl.numMap) * 4));
         XCHK(v, v->sc_map_.ensure(size_t(pl.numMap) * 4));
         XCHK(v, v->sc_cnt_.ensure(size_t(pl.numSC) * 4));
@@ -738,6 +747,28 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
     }
     int rc = drain(v0, ex, G);
     if (rc) return rc;
+    if (ex.cap) {
+        // recording: the entry counts arrive with the replay (graph_synchronize
+        // reads them from the recorded copy); the calls between see the
+        // previous count, which a bucketed gather does not read
+        for (int g = 0; g < G; g++) {
+            Volume* v = vs[g];
+            Plan& pl = plan[g];
+            len[g] = v->sc_len_;
+            v->num_sc_ = int(pl.numSC);
+            v->sc_map_n_ = int(pl.numMap);
+            v->sc_gen_ = v->topo_gen_;
+            v->sc16_valid_ = false;
+            v->last_insert_pull_ = true;
+            v->entries_stale_ = true;
+            v->bkt_scs_ = scs;
+            v->bkt_R_ = pl.R;
+            v->bkt_radius_ = radius[g];
+            v->bkt_gen_ = v->topo_gen_;
+            ex.cap->inserts.push_back(Graph::Insert{v, ex.hsmall + size_t(g) * SMALL_WORDS});
+        }
+        return GVDB_OK;
+    }
     // grids with points outside every brick fall back to the push path
     bool redo = false;
     for (int g = 0; g < G; g++) {
@@ -814,6 +845,7 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
 // per-brick count pass, the offsets, the entries (useful ones first) and the
 // subcell corners.
 int Volume::multi_materialize(Volume* const* vs, int G, const Exec& ex, int* failed) {
+    if (ex.cap) FAILG(0, vs[0]->not_recordable("building the subcell entry tables"));
     std::vector<FillArgs> fa;
     std::vector<PullArgs> pc, pw;
     std::vector<ScanArgs> sc;
@@ -878,6 +910,10 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
             FAILG(g, v->fail(GVDB_ERR_STATE, std::string(what) + ": call InsertPointsSubcell_FP16 first"));
         const bool buckets = !fp16 && v->bkt_gen_ == v->topo_gen_ && scs == v->bkt_scs_ && radius[g] == v->bkt_radius_ &&
                              v->res_[0] == 8 && scs == 4 && v->bkt_R_ >= 1 && v->bkt_R_ <= 2;
+        if (ex.cap) {
+            if (!buckets) FAILG(g, v->not_recordable(what));
+            ex.cap->note(v);
+        }
         v->atlas_epoch_++;
         if (buckets) {
             // the scatter kernel (no accumulate) also leaves the bricks' faces for the apron
@@ -934,13 +970,14 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
             }
         }
     }
-    if (G == 1) {
+    const bool timed = G == 1 && !ex.cap;   // no timing events inside a recording
+    if (timed) {
         v0->stage_begin(ST_GATHER);
         (void)hipEventRecord(v0->kev_[0][0], ex.s);
     }
     XCHK(v0, launch_gather_multi(ga.data(), int(ga.size()), ex.ar, ex.s));
     XCHK(v0, launch_bucket_gather_multi(bg.data(), int(bg.size()), ex.ar, ex.s));
-    if (G == 1) {
+    if (timed) {
         (void)hipEventRecord(v0->kev_[0][1], ex.s);
         v0->kev_used_[0] = true;
         v0->stage_end(ST_GATHER);
@@ -962,6 +999,7 @@ int Volume::multi_apron(Volume* const* vs, int G, int c, float boundval, const E
         if (rc) FAILG(g, rc);
         int bricks = int(v->pool_[0][0].lastEle);
         if (bricks == 0) continue;
+        if (ex.cap) ex.cap->note(v);
         if (v->atlas_gen_ != v->topo_gen_)
             FAILG(g, v->fail(GVDB_ERR_STATE, "UpdateApron: call UpdateAtlas after the topology changes"));
         Tree t = v->make_tree();
@@ -979,12 +1017,13 @@ int Volume::multi_apron(Volume* const* vs, int G, int c, float boundval, const E
     }
     if (aa.empty()) return GVDB_OK;
     Volume* v0 = vs[0];
-    if (G == 1) v0->stage_begin(ST_APRON);
+    const bool timed = G == 1 && !ex.cap;
+    if (timed) v0->stage_begin(ST_APRON);
     XCHK(v0, launch_fill_multi(nf.data(), int(nf.size()), ex.ar, ex.s));
     XCHK(v0, launch_neighbors_multi(na.data(), int(na.size()), ex.ar, ex.s));
-    if (G == 1) (void)hipEventRecord(v0->kev_[1][0], ex.s);
+    if (timed) (void)hipEventRecord(v0->kev_[1][0], ex.s);
     XCHK(v0, launch_apron_multi(aa.data(), int(aa.size()), ex.ar, ex.s));
-    if (G == 1) {
+    if (timed) {
         (void)hipEventRecord(v0->kev_[1][1], ex.s);
         v0->kev_used_[1] = true;
         v0->stage_end(ST_APRON);
@@ -1012,13 +1051,16 @@ int Volume::update_apron_noalloc(int c, float boundval) {
                                   std::to_string(NOALLOC_MAX_BRICKS));
     if (!noalloc_) return fail(GVDB_ERR_HIP, "UpdateApron (no allocation): no scratch array on this device");
     const bool faces = faces_chan_ == c && faces_epoch_ == atlas_epoch_ && faces_topo_ == topo_gen_;
+    if (capture_) capture_->note(this);
     stage_begin(ST_APRON);
-    (void)hipEventRecord(kev_[1][0], stream);
+    if (!capture_) (void)hipEventRecord(kev_[1][0], stream);
     hipError_t e = launch_apron_noalloc(make_tree(), noalloc_, faces ? faces_.p : nullptr, chan_[c].data.p,
                                         chan_[c].type, boundval, int(bricks), stream);
     if (e != hipSuccess) return hip_fail(e, "UpdateApron (no allocation)");
-    (void)hipEventRecord(kev_[1][1], stream);
-    kev_used_[1] = true;
+    if (!capture_) {
+        (void)hipEventRecord(kev_[1][1], stream);
+        kev_used_[1] = true;
+    }
     stage_end(ST_APRON);
     return GVDB_OK;
 }
@@ -1038,6 +1080,7 @@ int Volume::multi_render(Volume* const* vs, int G, const int* shading, const int
         if (shading[g] != SHADE_VOLUME && shading[g] != SHADE_TRILINEAR)
             FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, "Render: shading must be SHADE_VOLUME or SHADE_TRILINEAR"));
         if (v->pool_[0][0].lastEle == 0 || v->amap_max_ == 0) FAILG(g, v->fail(GVDB_ERR_STATE, "Render: no topology or atlas"));
+        if (ex.cap) ex.cap->note(v);
         if (v->atlas_gen_ != v->topo_gen_)
             FAILG(g, v->fail(GVDB_ERR_STATE, "Render: call UpdateAtlas after the topology changes"));
         int rc = v->commit_pools();
@@ -1059,12 +1102,13 @@ int Volume::multi_render(Volume* const* vs, int G, const int* shading, const int
         ra.push_back(a);
     }
     Volume* v0 = vs[0];
-    if (G == 1) {
+    const bool timed = G == 1 && !ex.cap;
+    if (timed) {
         v0->stage_begin(ST_RENDER);
         (void)hipEventRecord(v0->kev_[2][0], ex.s);
     }
     XCHK(v0, launch_render_multi(ra.data(), int(ra.size()), ex.ar, ex.s));
-    if (G == 1) {
+    if (timed) {
         (void)hipEventRecord(v0->kev_[2][1], ex.s);
         v0->kev_used_[2] = true;
         v0->stage_end(ST_RENDER);

@@ -27,6 +27,7 @@ void Arena::release() {
 void* Arena::push(const void* src, size_t n, hipStream_t s, hipError_t& e) {
     size_t at = (used + 255) & ~size_t(255);
     if (at + n > cap) {
+        if (fixed) { e = hipErrorOutOfMemory; return nullptr; }
         // the stream drains before the staging is reused
         e = hipStreamSynchronize(s);
         if (e != hipSuccess) return nullptr;

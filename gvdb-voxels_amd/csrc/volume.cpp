@@ -8,6 +8,7 @@
 // own stream; the only host round trips are the ones whose results the API
 // returns to the caller (bounding box/depth, node counts, subcell entries).
 #include "volume.h"
+#include "graph.h"
 #include "../../include/gvdb_amd.h"
 #include <hip/hip_runtime.h>
 #include <cmath>
@@ -19,6 +20,7 @@ namespace gvdb {
 // ------------------------------------------------------------- buffers ----
 hipError_t DevBuf::ensure(size_t n, bool preserve, hipStream_t s) {
     if (n <= bytes) return hipSuccess;
+    bump_alloc_generation();
     size_t want = std::max(n, bytes + bytes / 4);
     want = (want + 255) & ~size_t(255);
     void* q = nullptr;
@@ -41,6 +43,7 @@ hipError_t DevBuf::ensure(size_t n, bool preserve, hipStream_t s) {
 }
 
 void DevBuf::release() {
+    if (p) bump_alloc_generation();
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
@@ -53,7 +56,37 @@ void DevBuf::release() {
     } while (0)
 
 Exec Volume::self_exec() {
+    if (capture_) return Exec{stream, &capture_->arena, small_.as<uint32_t>(), pinned_, capture_};
     return Exec{stream, &arena_, small_.as<uint32_t>(), pinned_};
+}
+
+int Volume::finish_recorded_insert(const uint32_t* slot) {
+    // the words the insert's drain copies: linked bricks, points outside
+    // every brick, the 64-bit entry total
+    if (slot[SM_ORPHANS] != 0)
+        return fail(GVDB_ERR_STATE, "graph replay: " + std::to_string(slot[SM_ORPHANS]) +
+                                        " points fell outside the topology, which the recorded insert leaves out; "
+                                        "rebuild the topology (or insert without the graph)");
+    u64 total = 0;
+    memcpy(&total, slot + SM_LEN, 8);
+    if (total > 0x7FFFFFFFull) return fail(GVDB_ERR_RANGE, "graph replay: more than 2^31 subcell entries");
+    sc_len_ = int(total);
+    sc_bricks_ = int(slot[SM_LINKED]);
+    return GVDB_OK;
+}
+
+int Volume::not_recordable(const char* what) {
+    err = std::string(what) + ": cannot be recorded into a graph (the per-frame chain is ClearChannel, "
+                              "InsertPointsSubcell, GatherDensity/GatherLevelSet, UpdateApron and Render)";
+    if (capture_ && !capture_->failed) {
+        capture_->failed = true;
+        capture_->err = err;
+    }
+    return GVDB_ERR_STATE;
+}
+
+void Volume::forget_graph(Graph* g) {
+    graphs_.erase(std::remove(graphs_.begin(), graphs_.end(), g), graphs_.end());
 }
 
 SharedStream::~SharedStream() {
@@ -89,6 +122,19 @@ int Volume::hip_fail(hipError_t e, const char* what) {
 Volume::Volume(int dev) : device(dev) {}
 
 Volume::~Volume() {
+    // graphs that replay calls on this grid can no longer run
+    for (Graph* g : graphs_) {
+        g->failed = true;
+        g->err = "a grid it replays calls on was destroyed";
+        for (size_t k = g->vols.size(); k-- > 0;)
+            if (g->vols[k] == this) {
+                g->vols.erase(g->vols.begin() + long(k));
+                g->topo.erase(g->topo.begin() + long(k));
+            }
+        g->inserts.erase(std::remove_if(g->inserts.begin(), g->inserts.end(),
+                                        [this](const Graph::Insert& in) { return in.v == this; }),
+                         g->inserts.end());
+    }
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     for (int g = 0; g < 2; g++)
@@ -140,9 +186,11 @@ int Volume::init() {
 }
 
 void Volume::stage_begin(Stage s) {
+    if (capture_) return;   // no timing inside a recording
     (void)hipEventRecord(ev_[s][0], stream);
 }
 void Volume::stage_end(Stage s) {
+    if (capture_) return;
     (void)hipEventRecord(ev_[s][1], stream);
     ev_used_[s] = true;
 }
@@ -323,6 +371,7 @@ int Volume::sync_host() {
 int Volume::commit_pools() {
     // PoolCommitAll (gvdb_allocator.cpp:93-116)
     if (!dev_stale_) return GVDB_OK;
+    if (capture_) return not_recordable("pools built on the host and not yet on the device");
     for (int g = 0; g < 2; g++)
         for (int l = 0; l < levels_; l++) {
             Pool& p = pool_[g][l];
@@ -718,6 +767,7 @@ int Volume::linear_tf(float t0, float t1, const float a[4], const float b[4]) {
 }
 
 int Volume::commit_tf() {
+    if (capture_) return not_recordable("committing the transfer function (render once before recording)");
     HIPCHK(tf_dev.ensure(size_t(TF_SIZE) * 16));
     HIPCHK(hipMemcpyAsync(tf_dev.p, tf_host.data(), size_t(TF_SIZE) * 16, hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));

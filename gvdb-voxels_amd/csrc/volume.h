@@ -62,11 +62,16 @@ struct SharedStream {
 // SMALL_WORDS-word slot per grid (device and pinned host) for the few values
 // a stage reads back (bounding box, node counts, subcell totals).
 static const int SMALL_WORDS = 64;
+// small per-grid result words (device dsmall / pinned hsmall, SMALL_WORDS each)
+enum { SM_BBOX = 0, SM_LINKED = 8, SM_TOTAL = 9, SM_ORPHANS = 10, SM_BUCKETS = 11, SM_LEN = 12, SM_COUNTS = 16, SM_BOUNDS = 40,
+       SM_UNFLAGGED = 48 };
+struct Graph;
 struct Exec {
     hipStream_t s;
     Arena* ar;
     uint32_t* dsmall;
     uint32_t* hsmall;
+    Graph* cap = nullptr;   // recording into this graph (graph.h): no host round trips
 };
 
 class Volume {
@@ -154,6 +159,21 @@ public:
     hipStream_t stream = nullptr;
     int device = 0;
     Exec self_exec();
+
+    // HIP graph capture (graph.cpp): while capture() is set, the calls of the
+    // per-frame chain are recorded into it instead of run
+    bool capturing() const { return capture_ != nullptr; }
+    Graph* capture() const { return capture_; }
+    void set_capture(Graph* g) { capture_ = g; }
+    // fails the recording with `code` (a call that cannot be recorded)
+    int not_recordable(const char* what);
+    std::shared_ptr<SharedStream> stream_owner() const { return cur_stream_; }
+    unsigned topo_generation() const { return topo_gen_; }
+    void remember_graph(Graph* g) { graphs_.push_back(g); }
+    void forget_graph(Graph* g);
+    // after a replay: the recorded insert's readback (entry count, points
+    // it could not place) into this grid's state
+    int finish_recorded_insert(const uint32_t* slot);
     // moves this grid's work to another stream (null: its own), ordered
     // after what is already queued, without waiting on the host
     int use_stream(const std::shared_ptr<SharedStream>& s);
@@ -220,6 +240,8 @@ private:
     int faces_chan_ = -1;
     unsigned nbr_gen_ = ~0u;
     void* noalloc_ = nullptr;   // this device's copy of the no-allocation scratch array
+    Graph* capture_ = nullptr;      // the graph being recorded, if any
+    std::vector<Graph*> graphs_;    // graphs that replay calls on this grid
 
     // points
     PointSrc pts_{nullptr, 0, 0, 12};

@@ -69,6 +69,10 @@ _SIGS = [
     ("gvdb_tune_render_shape", _i, [_vp, _i, _i, _i, _vp, _vp]),
     ("gvdb_tune_gather", _i, [_vp, _i, _i, _f, _vp, _i, _i, _vp, _vp]),
     ("gvdb_set_tuning", _i, [_i, _i]), ("gvdb_get_tuning", _i, [_i, _vp, _vp, _vp]),
+    ("gvdb_capture_begin", _i, [_vp, ctypes.POINTER(_vp)]), ("gvdb_capture_end", _i, [_vp, _vp]),
+    ("gvdb_batch_capture_begin", _i, [_vp, ctypes.POINTER(_vp)]), ("gvdb_batch_capture_end", _i, [_vp, _vp]),
+    ("gvdb_graph_launch", _i, [_vp]), ("gvdb_graph_synchronize", _i, [_vp]), ("gvdb_graph_destroy", _i, [_vp]),
+    ("gvdb_graph_last_error", ctypes.c_char_p, [_vp]),
     ("gvdb_batch_tune", _i, [_vp, _i, _vp, _i, _i, _vp, _vp, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("gvdb_get_pool", _i, [_vp, _i, _i, _vp, _u64p, _u64p]), ("gvdb_get_info", _i, [_vp, _vp, _vp]),
     ("gvdb_get_atlas_info", _i, [_vp, _i, _vp]), ("gvdb_read_atlas", _i, [_vp, _i, _vp]),
@@ -352,6 +356,22 @@ class VolumeGVDB:
         self._chk(load().gvdb_tune_render(self.h, int(shading), int(chan), int(buf), ctypes.byref(best), _p(ms)))
         return best.value, {1: float(ms[0]), 2: float(ms[1]), 4: float(ms[2])}
 
+    def CaptureBegin(self):
+        """Starts recording the per-frame chain of this grid into a graph."""
+        g = ctypes.c_void_p()
+        self._chk(load().gvdb_capture_begin(self.h, ctypes.byref(g)))
+        self._graph = Graph(g)
+
+    def CaptureEnd(self):
+        """Ends the recording; returns the Graph."""
+        g, self._graph = self._graph, None
+        rc = load().gvdb_capture_end(self.h, g.h)
+        if rc != OK:
+            msg = load().gvdb_graph_last_error(g.h).decode()
+            g.close()
+            raise GVDBError(rc, "capture: " + msg)
+        return g
+
     def TuneRenderShape(self, shading, chan=0, buf=0):
         """Autotunes the ray-packet shape and the samples fetched ahead for
         this shading mode on the current scene; returns ((unroll, tile
@@ -452,6 +472,31 @@ class VolumeGVDB:
         o = np.zeros(4, np.float32)
         self._chk(load().gvdb_kernel_ms(self.h, _p(o)))
         return dict(gather=float(o[0]), apron=float(o[1]), render=float(o[2]))
+
+
+class Graph:
+    """A recorded per-frame chain (gvdb_capture_* / gvdb_batch_capture_*):
+    launch() replays it, synchronize() waits and brings the insert's entry
+    count back."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise GVDBError(rc, "graph: " + load().gvdb_graph_last_error(self.h).decode())
+        return rc
+
+    def launch(self):
+        return self._chk(load().gvdb_graph_launch(self.h))
+
+    def synchronize(self):
+        return self._chk(load().gvdb_graph_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            load().gvdb_graph_destroy(self.h)
+            self.h = None
 
 
 class GridBatch:
@@ -580,6 +625,22 @@ class GridBatch:
             return int(best[0]), {v: float(ms[i]) for i, v in enumerate((1, 2, 4, 8))}
         return (int(best[0]), int(best[1])), {(tw, u): float(ms[i * 3 + j]) for i, tw in enumerate((8, 16, 32, 64))
                                               for j, u in enumerate((1, 2, 4))}
+
+    def CaptureBegin(self):
+        """Starts recording the batched per-frame chain into a graph (every
+        grid must have taken part in a batched call before)."""
+        g = ctypes.c_void_p()
+        self._chk(load().gvdb_batch_capture_begin(self.h, ctypes.byref(g)))
+        self._graph = Graph(g)
+
+    def CaptureEnd(self):
+        g, self._graph = self._graph, None
+        rc = load().gvdb_batch_capture_end(self.h, g.h)
+        if rc != OK:
+            msg = load().gvdb_graph_last_error(g.h).decode()
+            g.close()
+            raise GVDBError(rc, "batch capture: " + msg)
+        return g
 
     def synchronize(self):
         return self._chk(load().gvdb_batch_synchronize(self.h))

@@ -295,6 +295,26 @@ int gvdb_batch_synchronize(gvdb_batch b);
 int gvdb_batch_elapsed_ms(gvdb_batch b, float* ms);
 int gvdb_release_stream(gvdb_handle h);
 
+/* HIP graphs of the per-frame chain (no counterpart in the reference; for callers that repeat ClearChannel ->
+ * InsertPointsSubcell -> GatherDensity/GatherLevelSet -> UpdateApron -> Render over a fixed topology, e.g. new
+ * point positions written into the same device buffer every frame).  Between capture_begin and capture_end the calls
+ * on that grid (or batch) are recorded, not run; any other call on it fails with GVDB_ERR_STATE and fails the
+ * recording.  Run the chain once without capture first: buffers that grow while recording fail the capture.
+ * gvdb_graph_launch replays the recorded launches, with their recorded arguments (point count, radius, camera), over
+ * the current device buffers, with no host round trip; gvdb_graph_synchronize waits for it and brings the insert's
+ * entry count back to the grid.  A launch is refused once a grid's topology changed or a library buffer was
+ * reallocated since the capture; a replay whose insert met points outside every brick reports GVDB_ERR_STATE at
+ * synchronize.  Destroy graphs before their grids or batch. */
+typedef struct gvdb_graph_s* gvdb_graph;
+int gvdb_capture_begin(gvdb_handle h, gvdb_graph* out);
+int gvdb_capture_end(gvdb_handle h, gvdb_graph g);
+int gvdb_batch_capture_begin(gvdb_batch b, gvdb_graph* out);
+int gvdb_batch_capture_end(gvdb_batch b, gvdb_graph g);
+int gvdb_graph_launch(gvdb_graph g);
+int gvdb_graph_synchronize(gvdb_graph g);
+int gvdb_graph_destroy(gvdb_graph g);
+const char* gvdb_graph_last_error(gvdb_graph g);
+
 /* ---------------------------------------------------------------- serving
  * A serving loop over resident grids (no counterpart in the reference).
  * Grids are added with their initial points and splat radius; the server
