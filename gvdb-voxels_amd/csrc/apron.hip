@@ -12,6 +12,8 @@
 #include "tree_device.h"
 #include <hip/hip_fp16.h>
 #include <mutex>
+#include <cstring>
+#include <algorithm>
 
 namespace gvdb {
 
@@ -85,6 +87,25 @@ template <>
 __device__ inline void st<float>(float* a, u64 o, float v) { a[o] = v; }
 template <>
 __device__ inline void st<__half>(__half* a, u64 o, float v) { a[o] = __float2half(v); }
+// uchar4 channels move as 32-bit words; the value travels through the body as
+// a float holding those bits (moved, never computed with), and the boundary
+// value arrives already packed (apron_bound)
+template <>
+__device__ inline float ld<uint32_t>(const uint32_t* a, u64 o) { return __uint_as_float(a[o]); }
+template <>
+__device__ inline void st<uint32_t>(uint32_t* a, u64 o, float v) { a[o] = __float_as_uint(v); }
+
+// UpdateApron<uchar4>'s boundary value, make_uchar4(boundval, ...) (kernels/
+// cuda_gvdb_operators.cuh:143-146): each byte is boundval converted to an
+// unsigned char, truncated and clamped to [0, 255] (NaN: 0), packed into the
+// bits of a float
+static float apron_bound_u8x4(float b) {
+    const float c = b > 0.f ? (b < 255.f ? b : 255.f) : 0.f;   // NaN fails b > 0
+    const uint32_t u = uint32_t(c) * 0x01010101u;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
 
 // One 512-thread workgroup per brick; threads 0..487 each own one apron voxel
 // of the padded 10^3 brick (apron 1, 8^3 interior).  Enumeration: the z = 0
@@ -162,6 +183,41 @@ __global__ __launch_bounds__(512) void k_apron_runs(ApronArgs, MultiMap m) {
     }
 }
 
+// Several grids, persistent: a fixed number of workgroups (about as many as
+// the device holds at once) take runs of QRUN bricks from one device counter
+// until the queue is empty.  A run's grid is looked up only when it falls
+// outside the workgroup's previous grid; every workgroup leaves when the
+// counter passes the last run.
+static const int QRUN = 8;
+static const uint32_t APRON_QUEUE_WORKGROUPS = 256 * 4;   // 4 per CU of 256
+template <typename AT>
+__global__ __launch_bounds__(512) void k_apron_queue(ApronArgs, MultiMap m, uint32_t* counter, uint32_t nruns) {
+    __shared__ uint32_t s_run;
+    const uint32_t* first = multi_first(m);
+    int g = -1;
+    uint32_t lo = 0, hi = 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_run = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t run = s_run;
+        __syncthreads();                   // read before thread 0 takes the next run
+        if (run >= nruns) break;
+        if (g < 0 || run < lo || run >= hi) {
+            g = multi_grid(m, run);
+            lo = first[g];
+            hi = first[g + 1];
+        }
+        const ApronArgs& a = multi_rec<ApronArgs>(m, g);
+        const uint32_t b0 = (run - lo) * uint32_t(QRUN);
+        for (int k = 0; k < QRUN; k++) {
+            if (b0 + uint32_t(k) >= uint32_t(a.n)) break;
+            __syncthreads();               // the previous brick is done with its offsets
+            k_apron_body<AT>(a, b0 + uint32_t(k), 0);
+        }
+        __syncthreads();
+    }
+}
+
 template <typename AT>
 static void launch_apron_runs(int run, dim3 grid, hipStream_t s, const ApronArgs& a0, const MultiMap& m) {
     switch (run) {
@@ -172,29 +228,54 @@ static void launch_apron_runs(int run, dim3 grid, hipStream_t s, const ApronArgs
     }
 }
 
+template <typename AT>
+static void launch_apron_as(bool queue, bool runs, int run, uint32_t nwg, uint32_t* ctr, const MultiLaunch& L,
+                            hipStream_t s, const ApronArgs& a0) {
+    if (queue)
+        hipLaunchKernelGGL(k_apron_queue<AT>, dim3(nwg), dim3(512), 0, s, a0, L.map, ctr, L.blocks);
+    else if (runs) launch_apron_runs<AT>(run, dim3(L.blocks), s, a0, L.map);
+    else hipLaunchKernelGGL(k_apron<AT>, dim3(L.blocks), dim3(512), 0, s, a0, L.map);
+}
+
 hipError_t launch_apron_multi(const ApronArgs* a, int G, Arena* ar, hipStream_t s) {
-    for (int half = 0; half < 2; half++) {
+    // one launch per channel type: 0 float, 1 half, 2 uchar4
+    for (int kind = 0; kind < 3; kind++) {
         std::vector<ApronArgs> sel;
         std::vector<uint32_t> nb;
         for (int g = 0; g < G; g++) {
             if (a[g].t.apron != 1) return hipErrorInvalidValue;
-            if ((a[g].atlas_type == T_HALF) != bool(half) || a[g].n <= 0) continue;
+            const int k = a[g].atlas_type == T_HALF ? 1 : (a[g].atlas_type == T_UCHAR4 ? 2 : 0);
+            if (k != kind || a[g].n <= 0) continue;
             sel.push_back(a[g]);
+            if (kind == 2) {
+                sel.back().boundval = apron_bound_u8x4(a[g].boundval);
+                sel.back().faces = nullptr;
+            }
         }
         if (sel.empty()) continue;
-        // one grid: a workgroup per brick; several: runs of `run` bricks
+        // one grid: a workgroup per brick; several: runs of `run` bricks, or
+        // the persistent queue over runs of QRUN bricks
         const bool runs = sel.size() > 1;
-        const int run = knob(KNOB_APRON_RUN);
+        const bool queue = runs && knob(KNOB_APRON_QUEUE) == 1;
+        const int run = queue ? QRUN : knob(KNOB_APRON_RUN);
         for (const ApronArgs& x : sel) nb.push_back(runs ? uint32_t((x.n + run - 1) / run) : uint32_t(x.n));
         MultiLaunch L;
         hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
         if (e != hipSuccess) return e;
         if (L.blocks == 0) continue;
         L.map.xcd = knob(KNOB_XCD_ORDER);
-        if (runs && half) launch_apron_runs<__half>(run, dim3(L.blocks), s, sel[0], L.map);
-        else if (runs) launch_apron_runs<float>(run, dim3(L.blocks), s, sel[0], L.map);
-        else if (half) hipLaunchKernelGGL(k_apron<__half>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
-        else hipLaunchKernelGGL(k_apron<float>, dim3(L.blocks), dim3(512), 0, s, sel[0], L.map);
+        uint32_t* ctr = nullptr;
+        const uint32_t nwg = std::min<uint32_t>(L.blocks, APRON_QUEUE_WORKGROUPS);
+        if (queue) {
+            // the counter starts at zero: a copy recorded on the stream (also
+            // inside a graph), from the argument staging
+            const uint32_t zero = 0;
+            ctr = ar ? static_cast<uint32_t*>(ar->push(&zero, sizeof(zero), s, e)) : nullptr;
+            if (!ctr) return e != hipSuccess ? e : hipErrorInvalidValue;
+        }
+        if (kind == 1) launch_apron_as<__half>(queue, runs, run, nwg, ctr, L, s, sel[0]);
+        else if (kind == 2) launch_apron_as<uint32_t>(queue, runs, run, nwg, ctr, L, s, sel[0]);
+        else launch_apron_as<float>(queue, runs, run, nwg, ctr, L, s, sel[0]);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -247,6 +328,11 @@ hipError_t launch_apron_noalloc(const Tree& t, void* sp, const void* faces, void
     hipLaunchKernelGGL(k_neighbors, dim3(uint32_t((u64(bricks) * 32 + 255) / 256)), dim3(256), 0, s, na, one);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (atlas_type == T_UCHAR4) {
+        const ApronArgs aa{t, base, nbr, nullptr, atlas, atlas_type, apron_bound_u8x4(boundval), bricks};
+        hipLaunchKernelGGL(k_apron<uint32_t>, dim3(uint32_t(bricks)), dim3(512), 0, s, aa, one);
+        return hipGetLastError();
+    }
     const ApronArgs aa{t, base, nbr, faces, atlas, atlas_type, boundval, bricks};
     if (atlas_type == T_HALF) hipLaunchKernelGGL(k_apron<__half>, dim3(uint32_t(bricks)), dim3(512), 0, s, aa, one);
     else hipLaunchKernelGGL(k_apron<float>, dim3(uint32_t(bricks)), dim3(512), 0, s, aa, one);

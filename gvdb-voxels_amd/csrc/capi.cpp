@@ -79,6 +79,18 @@ int gvdb_set_points_gpu(gvdb_handle h, const void* dptr, int n, int offset, int 
     return h->v.set_points_gpu(dptr, n, offset, stride);
 }
 int gvdb_upload_points(gvdb_handle h, const float* host_xyz, int n) { H_OR_FAIL(h); NOT_RECORDING(h, "SetPoints"); return h->v.upload_points(host_xyz, n); }
+int gvdb_set_colors_gpu(gvdb_handle h, const void* dptr, int n, int offset, int stride) {
+    H_OR_FAIL(h); NOT_RECORDING(h, "SetPoints");
+    return h->v.set_colors_gpu(dptr, n, offset, stride);
+}
+int gvdb_upload_colors(gvdb_handle h, const uint32_t* host_rgba, int n) {
+    H_OR_FAIL(h); NOT_RECORDING(h, "SetPoints");
+    return h->v.upload_colors(host_rgba, n);
+}
+int gvdb_set_color_channel(gvdb_handle h, int chan) {
+    H_OR_FAIL(h);
+    return h->v.set_color_channel(chan);
+}
 int gvdb_request_full_rebuild(gvdb_handle h, int on) { H_OR_FAIL(h); return h->v.request_full_rebuild(on != 0); }
 int gvdb_rebuild_topology(gvdb_handle h, int n, float radius, const float orig[3]) {
     H_OR_FAIL(h); NOT_RECORDING(h, "RebuildTopology");

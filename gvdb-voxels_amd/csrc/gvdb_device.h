@@ -30,7 +30,8 @@ static const float NOHIT = 1.0e10f;
 // shading modes (kernels/cuda_gvdb_scene.cuh:25-32 of the reference)
 enum { SHADE_VOXEL = 0, SHADE_TRILINEAR = 4, SHADE_VOLUME = 7 };
 // channel types (src/gvdb_types.h of the reference)
-enum { T_UCHAR = 0, T_FLOAT = 3, T_HALF = 100 };
+enum { T_UCHAR = 0, T_UCHAR4 = 2, T_FLOAT = 3, T_HALF = 100 };
+static const int CHAN_UNDEF = 255;   // no colour channel (kernels/cuda_gvdb_nodes.cuh:21)
 
 struct Node {
     uint8_t mLev, mFlags, mPriority, pad;

@@ -278,6 +278,7 @@ hipError_t launch_bucket_points_multi(const BucketArgs* a, int G, Arena* ar, hip
 
 struct ScatterArgs {
     PointSrc pts;
+    PointSrc clr;            // optional (base null): a uint32 colour per point, kept in the fourth word
     float tr[3];
     const uint32_t* key;
     const uint32_t* slot;
@@ -325,6 +326,24 @@ struct BucketGatherArgs {
     int levelset;         // 1: GatherLevelSet (nearest-entry kernel)
 };
 hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* ar, hipStream_t s);
+
+// GatherDensity / GatherLevelSet colour channel (chanClr) from the colours
+// the bucketed insert carried in each point's fourth word: every voxel of a
+// subcell gets the mean of its entry list's colours (uchar4 per voxel).
+struct ColorArgs {
+    Tree t;
+    int nleaves;
+    int R;
+    float radius;
+    int scs;
+    const int* nbr;
+    const uint32_t* bcnt;
+    const uint32_t* boff;
+    const float4* sorted;
+    void* atlas;          // the uchar4 channel
+    int accumulate;
+};
+hipError_t launch_gather_color_multi(const ColorArgs* a, int G, Arena* ar, hipStream_t s);
 // Face buffer layout: brick b's faces at b * FACE_VALS; face f = x0, x7, y0,
 // y7, z0, z7 (local interior coordinate 0 or 7) at f * 64, each face row-major
 // in its other two coordinates, the higher one first (x faces: z*8 + y; y
@@ -384,6 +403,7 @@ struct RenderArgs {
     int shading;
     int atlas_type;
     int tile_w;          // pixel tile of one 64-lane workgroup: tile_w x (64 / tile_w); set by the launcher
+    const uint32_t* clr; // the colour channel (uchar4 per voxel, SetColorChannel), or null
 };
 hipError_t launch_render_multi(const RenderArgs* a, int G, Arena* ar, hipStream_t s);
 
@@ -396,6 +416,8 @@ enum Knob {
     KNOB_GATHER_THREADS,         // threads per brick of the scatter gather at radius <= 3: 128, 256
     KNOB_APRON_RUN,              // bricks per workgroup of the multi-grid apron: 1, 2, 4, 8
     KNOB_XCD_ORDER,              // per-brick kernels: 0 launch order, 1 each XCD a contiguous range of bricks
+    KNOB_APRON_QUEUE,            // multi-grid apron: 0 static runs, 1 persistent workgroups over a brick queue
+    KNOB_MARK_PATH,              // topology marks: 0 by domain size, 1 a byte per brick, 2 direction masks, 3 bits
     KNOB_COUNT
 };
 int knob(int k);

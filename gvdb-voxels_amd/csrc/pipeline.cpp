@@ -203,12 +203,20 @@ int Volume::multi_full_build(Volume* const* vs, int G, const int* n, const float
         // marks, whole 32-byte groups) up to 2^27 bricks; else direction masks
         // (4 bytes per domain brick) while that stays comparable to the point
         // stream; else bits set directly
-        if (pl.dom[0].nbits <= (u64(1) << 27)) {
+        // (the mark-path knob forces one: every path marks the same bits)
+        const int force = knob(KNOB_MARK_PATH);
+        const bool bytes_ok = pl.dom[0].nbits <= (u64(1) << 27);
+        const bool masks_ok = pl.dom[0].nbits <= (u64(1) << 30);
+        int path = bytes_ok ? 1 : (pl.dom[0].nbits <= std::max(u64(1) << 22, u64(n[g]) * 4) ? 2 : 3);
+        if (force == 1 && bytes_ok) path = 1;
+        if (force == 2 && masks_ok) path = 2;
+        if (force == 3) path = 3;
+        if (path == 1) {
             const u64 nbytes = pl.dom[0].nwords * 32 * (pl.dilate ? 2 : 1);
             XCHK(v, v->masks_.ensure(size_t(nbytes)));
             pl.bytes = reinterpret_cast<uint8_t*>(v->masks_.p);
             fills.push_back(FillArgs{v->masks_.p, nbytes / 4, 0, 0});
-        } else if (pl.dom[0].nbits <= std::max(u64(1) << 22, u64(n[g]) * 4)) {
+        } else if (path == 2) {
             XCHK(v, v->masks_.ensure(size_t(pl.dom[0].nbits) * 4));
             pl.masks = v->masks_.as<uint32_t>();
             fills.push_back(FillArgs{pl.masks, pl.dom[0].nbits, 0, 0});
@@ -634,6 +642,9 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
             FAILG(g, v->fail(GVDB_ERR_INVALID, "InsertPointsSubcell: subcell size must divide the brick"));
         if (n[g] < 0 || n[g] > v->pts_.n || (n[g] > 0 && !v->pts_.base))
             FAILG(g, v->fail(GVDB_ERR_INVALID, "InsertPointsSubcell: point count exceeds SetPoints"));
+        if (v->clr_.base && n[g] > v->clr_.n)
+            FAILG(g, v->fail(GVDB_ERR_INVALID, "InsertPointsSubcell: point count exceeds SetPoints' colours"));
+        v->bkt_clr_ = false;
         if (!(radius[g] >= 0.0f) || radius[g] > 32.0f)
             FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, "InsertPointsSubcell: radius must be in [0, 32]"));
         if (v->atlas_gen_ != v->topo_gen_)
@@ -736,8 +747,12 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
             lo.push_back(reinterpret_cast<u64*>(small(g, SM_LEN)));
             sb.push_back(ScanArgs{v->bkt_cnt_.as<uint32_t>(), nullptr, 0, v->bkt_off_.as<uint32_t>(), nb, tmp2(g),
                                   small(g, SM_BUCKETS)});
-            sca.push_back(ScatterArgs{pl.P, {tr[0], tr[1], tr[2]}, v->bkt_key_.as<uint32_t>(),
-                                      v->bkt_slot_.as<uint32_t>(), v->bkt_off_.as<uint32_t>(), v->sorted_.as<float4>()});
+            // SetPoints' colours travel with the bucketed points (fourth word)
+            const bool clr = v->clr_.base && rule == BIN_FLOAT;
+            sca.push_back(ScatterArgs{pl.P, clr ? v->clr_ : PointSrc{nullptr, 0, 0, 4}, {tr[0], tr[1], tr[2]},
+                                      v->bkt_key_.as<uint32_t>(), v->bkt_slot_.as<uint32_t>(),
+                                      v->bkt_off_.as<uint32_t>(), v->sorted_.as<float4>()});
+            v->bkt_clr_ = clr;
         }
         XCHK(v0, launch_bucket_points_multi(ba.data(), int(ba.size()), ex.ar, ex.s));
         XCHK(v0, launch_sum_parts_multi(lp.data(), lo.data(), int(lp.size()), ex.ar, ex.s));
@@ -887,19 +902,20 @@ int Volume::multi_materialize(Volume* const* vs, int G, const Exec& ex, int* fai
 
 // --------------------------------------------------------- GatherDensity ----
 int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, const float* radius, int chanD, bool accum,
-                         bool levelset, const Exec& ex, int* failed, bool fp16) {
+                         bool levelset, const Exec& ex, int* failed, bool fp16, int chanC) {
     // GatherDensity (gvdb_volume_gvdb.cpp:5318-5344) or GatherLevelSet (:5348-5371,
     // which does not return early on an empty entry list)
     std::vector<GatherArgs> ga;
     std::vector<BucketGatherArgs> bg;
+    std::vector<ColorArgs> ca;
     std::vector<Volume*> need;
     std::vector<int> need_g, ents;   // grid index of each volume to materialize / to gather from entries
     const char* what = fp16 ? "GatherLevelSet_FP16" : (levelset ? "GatherLevelSet" : "GatherDensity");
     for (int g = 0; g < G; g++) {
         Volume* v = vs[g];
         if ((len[g] == 0 && !levelset) || v->pool_[0][0].usedNum == 0) continue;
-        if (chanD < 0 || chanD >= 8 || !v->chan_[chanD].exists)
-            FAILG(g, v->fail(GVDB_ERR_INVALID, std::string(what) + ": no such channel"));
+        if (chanD < 0 || chanD >= 8 || !v->chan_[chanD].exists || v->chan_[chanD].type == T_UCHAR4)
+            FAILG(g, v->fail(GVDB_ERR_INVALID, std::string(what) + ": no such float or half channel"));
         if (len[g] != v->sc_len_ || scs <= 0 || v->res_[0] % scs)
             FAILG(g, v->fail(GVDB_ERR_STATE, std::string(what) + ": call InsertPointsSubcell first"));
         if (v->atlas_gen_ != v->topo_gen_ || v->sc_gen_ != v->topo_gen_)
@@ -913,6 +929,20 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
         if (ex.cap) {
             if (!buckets) FAILG(g, v->not_recordable(what));
             ex.cap->note(v);
+        }
+        // the colour channel (gvdbGatherDensity's chanClr): only when the
+        // insert carried SetPoints' colours, as the reference writes it only
+        // when its per-entry colour table exists
+        const bool colour = chanC >= 0 && v->bkt_clr_ && !fp16;
+        if (colour) {
+            if (chanC >= 8 || !v->chan_[chanC].exists || v->chan_[chanC].type != T_UCHAR4 || chanC == chanD)
+                FAILG(g, v->fail(GVDB_ERR_INVALID, std::string(what) + ": the colour channel must be a uchar4 channel"));
+            if (!buckets)
+                FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, std::string(what) + ": colours need the bucketed insert at the "
+                                                       "gather's radius (8^3 bricks, subcell 4)"));
+            ca.push_back(ColorArgs{v->make_tree(), int(v->pool_[0][0].lastEle), v->bkt_R_, radius[g], scs,
+                                   v->nbr_.as<int>(), v->bkt_cnt_.as<uint32_t>(), v->bkt_off_.as<uint32_t>(),
+                                   v->sorted_.as<float4>(), v->chan_[chanC].data.p, accum ? 1 : 0});
         }
         v->atlas_epoch_++;
         if (buckets) {
@@ -939,7 +969,7 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
         }
         ents.push_back(g);
     }
-    if (ents.empty() && bg.empty()) return GVDB_OK;
+    if (ents.empty() && bg.empty() && ca.empty()) return GVDB_OK;
     Volume* v0 = vs[0];
     if (!need.empty()) {
         // allocates the entry tables, so the entry pointers are taken only after it
@@ -977,6 +1007,7 @@ int Volume::multi_gather(Volume* const* vs, int G, int scs, const int* len, cons
     }
     XCHK(v0, launch_gather_multi(ga.data(), int(ga.size()), ex.ar, ex.s));
     XCHK(v0, launch_bucket_gather_multi(bg.data(), int(bg.size()), ex.ar, ex.s));
+    XCHK(v0, launch_gather_color_multi(ca.data(), int(ca.size()), ex.ar, ex.s));
     if (timed) {
         (void)hipEventRecord(v0->kev_[0][1], ex.s);
         v0->kev_used_[0] = true;
@@ -1076,7 +1107,8 @@ int Volume::multi_render(Volume* const* vs, int G, const int* shading, const int
         if (b < 0 || b >= 4 || v->rbuf_[b].w == 0) FAILG(g, v->fail(GVDB_ERR_INVALID, "Render: no such render buffer"));
         if (!v->identity_transform())
             FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, "Render: only the identity grid transform is supported"));
-        if (c < 0 || c >= 8 || !v->chan_[c].exists) FAILG(g, v->fail(GVDB_ERR_INVALID, "Render: no such channel"));
+        if (c < 0 || c >= 8 || !v->chan_[c].exists || v->chan_[c].type == T_UCHAR4)
+            FAILG(g, v->fail(GVDB_ERR_INVALID, "Render: no such float or half channel"));
         if (shading[g] != SHADE_VOLUME && shading[g] != SHADE_TRILINEAR)
             FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, "Render: shading must be SHADE_VOLUME or SHADE_TRILINEAR"));
         if (v->pool_[0][0].lastEle == 0 || v->amap_max_ == 0) FAILG(g, v->fail(GVDB_ERR_STATE, "Render: no topology or atlas"));
@@ -1099,6 +1131,15 @@ int Volume::multi_render(Volume* const* vs, int G, const int* shading, const int
         a.out_float = r.bpp == 16;
         a.shading = shading[g];
         a.atlas_type = v->chan_[c].type;
+        a.tile_w = 8;
+        a.clr = nullptr;
+        if (v->clr_chan_ != CHAN_UNDEF) {
+            // SetColorChannel: the march reads that uchar4 channel's colours
+            const int cc = v->clr_chan_;
+            if (!v->chan_[cc].exists || v->chan_[cc].type != T_UCHAR4)
+                FAILG(g, v->fail(GVDB_ERR_INVALID, "Render: the colour channel must be a uchar4 channel"));
+            a.clr = v->chan_[cc].data.as<uint32_t>();
+        }
         ra.push_back(a);
     }
     Volume* v0 = vs[0];

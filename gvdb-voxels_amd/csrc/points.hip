@@ -666,7 +666,8 @@ __device__ inline void k_bucket_scatter_body(const ScatterArgs& a, uint32_t bid,
     if (k == 0xFFFFFFFFu) return;
     const float* p = point_at(pts, i);
     u64 e = u64(boff[k]) + slot[i];
-    sorted[e] = make_float4(p[0] + tx, p[1] + ty, p[2] + tz, 0.0f);
+    const uint32_t c = a.clr.base ? *reinterpret_cast<const uint32_t*>(a.clr.base + u64(i) * a.clr.stride + a.clr.off) : 0u;
+    sorted[e] = make_float4(p[0] + tx, p[1] + ty, p[2] + tz, __uint_as_float(c));
 }
 
 static const int PULL_THREADS = 256;
@@ -1482,6 +1483,114 @@ hipError_t launch_bucket_gather_multi(const BucketGatherArgs* a, int G, Arena* a
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// ------------------------------------------------------- colour gather
+// gvdbGatherDensity's colour (kernels/cuda_gvdb_particles.cuh:717-731; the
+// level set's :757-771 is the same): starting from (0, 0, 0, 1), or from the
+// voxel's colour with bAccumulate, it adds every entry's colour / 255 over the
+// subcell's list and divides by the entry count (+1 with bAccumulate), then
+// stores x * 255 truncated to uchar (clamped: an empty list divides by zero,
+// storing 0 for NaN and 255 for the alpha's +inf).  Here the sums are exact
+// integers, so the stored byte is (255 [alpha only] + sum) / count, or
+// (old + sum) / (count + 1), in integer division: the mean of the list's
+// colours, independent of the order the entries were inserted in (the
+// reference adds floats in atomic-insertion order).  One workgroup per brick
+// reads the candidates of the brick's (2 + 2R)^3 cells, as the density
+// gather does, and counts a candidate for each of the brick's subcells whose
+// list the insert rule puts it in: int(w - r)/sc*sc <= s <= int(w + 1 + r)/sc*sc
+// on every axis (kernels/cuda_gvdb_particles.cuh:237-242).
+static const int CG_THREADS = 256;
+struct CGShared {
+    uint32_t cell_off[CG_THREADS];
+    uint32_t cell_start[CG_THREADS];
+    int nbr[27];
+    uint32_t wsum[CG_THREADS / 64];
+    uint32_t sum[8][5];   // r, g, b, a, entries per subcell of the brick
+};
+
+__device__ inline void k_gather_color_body(const ColorArgs& a, uint32_t bid, uint32_t) {
+    __shared__ CGShared S;
+    const Tree& t = a.t;
+    const int brick = int(bid);
+    const Node* bn = t.nodes[0] + brick;
+    if (bn->mParent == ID_UNDEF64) return;        // unlinked leaves have no subcells
+    if (!bn->mFlags || bn->mValue[0] < 0) return;  // no atlas slot to write (uniform per brick)
+    const int tid = threadIdx.x;
+    if (tid < 40) (&S.sum[0][0])[tid] = 0;
+    const uint32_t T = gb_setup<CG_THREADS>(brick, a.R, a.nbr, a.bcnt, a.boff, S);
+    __syncthreads();
+    const int span = 2 + 2 * a.R, ncell = span * span * span;
+    const int sc = a.scs, sd = t.brickwid / sc;   // subcells per axis (2 for 8^3 bricks, sc 4)
+    const float r = a.radius;
+    for (uint32_t f = uint32_t(tid); f < T; f += CG_THREADS) {
+        int lo = 0, hi = ncell - 1;                // last cell starting at or before f
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (S.cell_start[mid] <= f) lo = mid;
+            else hi = mid - 1;
+        }
+        const float4 q = a.sorted[S.cell_off[lo] + (f - S.cell_start[lo])];
+        const float w[3] = {q.x, q.y, q.z};
+        uint32_t m[3];                              // subcells of the brick along each axis whose list takes it
+        for (int ax = 0; ax < 3; ax++) {
+            const int s0 = int(w[ax] - r) / sc * sc, s1 = int(w[ax] + 1.0f + r) / sc * sc;
+            m[ax] = 0;
+            for (int k = 0; k < sd; k++) {
+                const int st = bn->mPos[ax] + k * sc;
+                if (s0 <= st && st <= s1) m[ax] |= 1u << k;
+            }
+        }
+        if (!(m[0] && m[1] && m[2])) continue;
+        const uint32_t c = __float_as_uint(q.w);
+        for (int kz = 0; kz < sd; kz++)
+            for (int ky = 0; ky < sd; ky++)
+                for (int kx = 0; kx < sd; kx++) {
+                    if (!((m[0] >> kx) & (m[1] >> ky) & (m[2] >> kz) & 1u)) continue;
+                    uint32_t* sm = S.sum[(kz * sd + ky) * sd + kx];
+                    atomicAdd(sm + 0, c & 0xFFu);
+                    atomicAdd(sm + 1, (c >> 8) & 0xFFu);
+                    atomicAdd(sm + 2, (c >> 16) & 0xFFu);
+                    atomicAdd(sm + 3, c >> 24);
+                    atomicAdd(sm + 4, 1u);
+                }
+    }
+    __syncthreads();
+    uint32_t* atlas = reinterpret_cast<uint32_t*>(a.atlas);
+    const int w = t.brickwid;
+    for (int v = tid; v < w * w * w; v += CG_THREADS) {
+        const int lx = v % w, ly = (v / w) % w, lz = v / (w * w);
+        const uint32_t* sm = S.sum[((lz / sc) * sd + ly / sc) * sd + lx / sc];
+        const u64 off = atlas_offset(t, bn->mValue[0] + lx, bn->mValue[1] + ly, bn->mValue[2] + lz);
+        const uint32_t n = sm[4];
+        uint32_t out = 0;
+        if (a.accumulate) {
+            const uint32_t old = atlas[off];
+            for (int ch = 0; ch < 4; ch++) out |= (((old >> (8 * ch)) & 0xFFu) + sm[ch]) / (n + 1u) << (8 * ch);
+        } else if (n == 0) {
+            out = 0xFF000000u;
+        } else {
+            for (int ch = 0; ch < 4; ch++) out |= min((sm[ch] + (ch == 3 ? 255u : 0u)) / n, 255u) << (8 * ch);
+        }
+        atlas[off] = out;
+    }
+}
+
+GVDB_MULTI(k_gather_color, ColorArgs, CG_THREADS)
+
+hipError_t launch_gather_color_multi(const ColorArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<uint32_t> nb(G);
+    for (int g = 0; g < G; g++) {
+        if (a[g].t.res[0] != 8 || a[g].R < 1 || a[g].R > 2 || a[g].scs <= 0 || a[g].t.brickwid % a[g].scs ||
+            a[g].t.brickwid / a[g].scs > 2 || (2 + 2 * a[g].R) * (2 + 2 * a[g].R) * (2 + 2 * a[g].R) > CG_THREADS)
+            return hipErrorInvalidValue;
+        nb[g] = uint32_t(a[g].nleaves > 0 ? a[g].nleaves : 0);
+    }
+    MultiLaunch L;
+    hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
+    if (e != hipSuccess || L.blocks == 0) return e;
+    hipLaunchKernelGGL(k_gather_color, dim3(L.blocks), dim3(CG_THREADS), 0, s, a[0], L.map);
+    return hipGetLastError();
 }
 
 GVDB_MULTI(k_bucket_points, BucketArgs, 256)

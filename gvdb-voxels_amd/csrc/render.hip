@@ -119,6 +119,16 @@ __device__ inline float tri_edge(const AT* b, int br, const Corner& K, float x, 
     return tri_blend(T);
 }
 
+// getColorF (kernels/cuda_gvdb_raycast.cuh:206-209): the colour channel's
+// voxel at the integer part of the atlas position, bytes / 255
+__device__ inline float4 color_at(const uint32_t* cb, int br, const Corner& K, float x, float y, float z) {
+    const int ix = min(max(int(x) - K.c[0], 0), br - 1), iy = min(max(int(y) - K.c[1], 0), br - 1),
+              iz = min(max(int(z) - K.c[2], 0), br - 1);
+    const uint32_t c = cb[(iz * br + iy) * br + ix];
+    return make_float4(float(c & 0xFFu) / 255.0f, float((c >> 8) & 0xFFu) / 255.0f, float((c >> 16) & 0xFFu) / 255.0f,
+                       float(c >> 24) / 255.0f);
+}
+
 template <typename AT>
 __device__ inline Corner brick_corner(const Tree& t, const Node* n) {
     Corner K;
@@ -180,6 +190,7 @@ __device__ void deep_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 
     const AT* b = brick_base(t, atlas, n);
     const int br = t.brick_res;
     const Corner K = brick_corner<AT>(t, n);
+    const uint32_t* cb = A.clr ? brick_base(t, A.clr, n) : nullptr;
     const float dstep = A.s.steps[0];
     f3 vmin = mk3(float(n->mPos[0]), float(n->mPos[1]), float(n->mPos[2]));
     float tx = dstep * ceilf(tt.x / dstep);
@@ -211,11 +222,20 @@ __device__ void deep_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 
         for (int u = 0; u < U; u++) {
             if (!(clr[3] > alphacut) || !ok[u]) goto done;
             if (raw[u] >= minval) {
+                // the reference's products, left to right; the colour channel's
+                // factor only when there is one (a factor of 1 changes nothing)
                 float w = expf(extinct * val[u].w * dstep);
-                float k = clr[3] * (1.0f - w) * albedo;
-                clr[0] += val[u].x * k;
-                clr[1] += val[u].y * k;
-                clr[2] += val[u].z * k;
+                float kx = val[u].x * clr[3] * (1.0f - w) * albedo, ky = val[u].y * clr[3] * (1.0f - w) * albedo,
+                      kz = val[u].z * clr[3] * (1.0f - w) * albedo;
+                if (cb) {
+                    const float4 h = color_at(cb, br, K, p.x + K.o[0], p.y + K.o[1], p.z + K.o[2]);
+                    kx = kx * h.x;
+                    ky = ky * h.y;
+                    kz = kz * h.z;
+                }
+                clr[0] += kx;
+                clr[1] += ky;
+                clr[2] += kz;
                 clr[3] *= w;
             }
             p = p + wpt;
@@ -230,7 +250,8 @@ done:
 // raySurfaceTrilinearBrick: first sample >= threshold, gradient normal; U
 // samples fetched ahead as in deep_brick.
 template <typename AT, int U>
-__device__ void surf_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 tt, f3 pos, f3 dir, f3& hit, f3& nrm) {
+__device__ void surf_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 tt, f3 pos, f3 dir, f3& hit, f3& nrm,
+                           float hclr[4]) {
     const Tree& t = A.t;
     const Node* n = t.nodes[0] + nodeid;
     const AT* b = brick_base(t, atlas, n);
@@ -264,6 +285,10 @@ __device__ void surf_brick(const RenderArgs& A, const AT* atlas, int nodeid, f3 
                 g.y = tri_edge<AT>(b, br, K, P.x, P.y - 0.5f, P.z) - tri_edge<AT>(b, br, K, P.x, P.y + 0.5f, P.z);
                 g.z = tri_edge<AT>(b, br, K, P.x, P.y, P.z - 0.5f) - tri_edge<AT>(b, br, K, P.x, P.y, P.z + 0.5f);
                 nrm = norm3(g);
+                if (A.clr) {
+                    const float4 h = color_at(brick_base(t, A.clr, n), br, K, P.x, P.y, P.z);
+                    hclr[0] = h.x; hclr[1] = h.y; hclr[2] = h.z; hclr[3] = h.w;
+                }
                 return;
             }
             p = p + dir * dstep;
@@ -321,7 +346,7 @@ __device__ void ray_cast(const RenderArgs& A, const AT* atlas, f3 pos, f3 dir, f
                     deep_brick<AT, U>(A, atlas, child, tt, pos, dir, clr);
                     if (clr[3] <= 0) { clr[3] = 0; return; }
                 } else {
-                    surf_brick<AT, U>(A, atlas, child, tt, pos, dir, hit, nrm);
+                    surf_brick<AT, U>(A, atlas, child, tt, pos, dir, hit, nrm, clr);
                     if (hit.z != NOHIT) return;
                 }
                 d.step();

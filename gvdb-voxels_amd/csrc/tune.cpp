@@ -18,10 +18,10 @@
 namespace gvdb {
 
 static const int kValues[KNOB_COUNT][4] = {
-    {1, 2, 4}, {1, 2, 4}, {8, 16, 32, 64}, {8, 16, 32, 64}, {128, 256}, {1, 2, 4, 8}, {0, 1}};
-static const int kCount[KNOB_COUNT] = {3, 3, 4, 4, 2, 4, 2};
+    {1, 2, 4}, {1, 2, 4}, {8, 16, 32, 64}, {8, 16, 32, 64}, {128, 256}, {1, 2, 4, 8}, {0, 1}, {0, 1}, {0, 1, 2, 3}};
+static const int kCount[KNOB_COUNT] = {3, 3, 4, 4, 2, 4, 2, 2, 4};
 // defaults: the values measured fastest on the medium and 64-grid workloads
-static int g_knob[KNOB_COUNT] = {4, 2, 8, 8, 128, 4, 0};
+static int g_knob[KNOB_COUNT] = {4, 2, 8, 8, 128, 4, 0, 1, 0};
 
 int knob(int k) { return (k >= 0 && k < KNOB_COUNT) ? g_knob[k] : 0; }
 
@@ -157,8 +157,35 @@ int tune_batch(Batch& B, Volume* const* vs, int G, int stage, int scs, const int
                           best, ms);
     }
     if (stage == 1) {
+        // static runs of 1, 2, 4, 8 bricks, then the persistent queue
         const int ks[1] = {KNOB_APRON_RUN};
-        return tune_knobs(ks, 1, timed([&] { return B.update_apron(vs, G, chan, boundval, failed); }), best, ms);
+        const int keepq = g_knob[KNOB_APRON_QUEUE];
+        g_knob[KNOB_APRON_QUEUE] = 0;
+        int b0 = 0;
+        int rc = tune_knobs(ks, 1, timed([&] { return B.update_apron(vs, G, chan, boundval, failed); }), &b0, ms);
+        if (rc != GVDB_OK) {
+            g_knob[KNOB_APRON_QUEUE] = keepq;
+            return rc;
+        }
+        g_knob[KNOB_APRON_QUEUE] = 1;
+        float tq = 0.f;
+        rc = time_runs(timed([&] { return B.update_apron(vs, G, chan, boundval, failed); }), tq);
+        if (rc != GVDB_OK) {
+            g_knob[KNOB_APRON_QUEUE] = keepq;
+            return rc;
+        }
+        if (ms) ms[4] = tq;
+        int vals[4];
+        const int n = knob_values(KNOB_APRON_RUN, vals);
+        float tb = FLT_MAX;
+        for (int i = 0; i < n && ms; i++)
+            if (vals[i] == b0) tb = ms[i];
+        g_knob[KNOB_APRON_QUEUE] = (ms && tq < tb) ? 1 : 0;
+        if (best) {
+            best[0] = b0;
+            best[1] = g_knob[KNOB_APRON_QUEUE];
+        }
+        return GVDB_OK;
     }
     if (stage == 2) {
         if (G <= 0 || (shading[0] != GVDB_SHADE_VOLUME && shading[0] != GVDB_SHADE_TRILINEAR)) return GVDB_ERR_INVALID;

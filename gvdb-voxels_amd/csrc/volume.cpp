@@ -141,7 +141,7 @@ Volume::~Volume() {
         for (int l = 0; l < MAXLEV; l++) pool_[g][l].dev.release();
     for (auto& c : chan_) c.data.release();
     for (auto& r : rbuf_) r.data.release();
-    DevBuf* bufs[] = {&amap_, &nbr_, &nmask_, &nbase_, &faces_, &lenparts_, &pts_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_, &sc_use_,
+    DevBuf* bufs[] = {&amap_, &nbr_, &nmask_, &nbase_, &faces_, &lenparts_, &pts_owned_, &clr_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_, &sc_use_,
                       &bkt_key_, &bkt_slot_, &bkt_cnt_, &bkt_off_, &sorted_, &sc_flags_, &small_, &masks_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev,
                       &incr_marker_, &sc_pnt16_, &view_dev_, &view_scene_};
     for (DevBuf* b : bufs) b->release();
@@ -272,7 +272,8 @@ int Volume::add_channel(int c, int dtype, int apron, int filter, int border, int
     // AddChannel + AtlasCreate (gvdb_volume_gvdb.cpp:2437-2453, gvdb_allocator.cpp:528-560)
     if (levels_ == 0) return fail(GVDB_ERR_STATE, "AddChannel before Configure");
     if (c < 0 || c >= 8) return fail(GVDB_ERR_INVALID, "AddChannel: channel must be 0..7");
-    if (dtype != T_FLOAT && dtype != T_HALF) return fail(GVDB_ERR_UNSUPPORTED, "AddChannel: only float and half channels");
+    if (dtype != T_FLOAT && dtype != T_HALF && dtype != T_UCHAR4)
+        return fail(GVDB_ERR_UNSUPPORTED, "AddChannel: float, half and uchar4 channels");
     if (apron < 0 || apron > 1) return fail(GVDB_ERR_UNSUPPORTED, "AddChannel: apron must be 0 or 1");
     if (c > 0 && !chan_[0].exists) return fail(GVDB_ERR_STATE, "AddChannel: channel 0 first");
     if (ax == 0 && ay == 0 && az == 0) {
@@ -310,6 +311,38 @@ int Volume::set_points_gpu(const void* p, int n, int off, int stride) {
     if (n < 0 || (n > 0 && !p)) return fail(GVDB_ERR_INVALID, "SetPoints: null positions");
     if (stride < 12 || off < 0) return fail(GVDB_ERR_INVALID, "SetPoints: stride must hold a float3");
     pts_ = PointSrc{reinterpret_cast<const char*>(p), n, off, stride};
+    return GVDB_OK;
+}
+
+int Volume::set_colors_gpu(const void* p, int n, int off, int stride) {
+    // SetPoints' colour DataPtr (gvdb_volume_gvdb.cpp:4900-4908): a uint32
+    // (uchar4) per point; null clears it
+    if (!p || n <= 0) {
+        clr_ = PointSrc{nullptr, 0, 0, 4};
+        return GVDB_OK;
+    }
+    if (stride < 4 || off < 0) return fail(GVDB_ERR_INVALID, "SetPoints: colour stride must hold a uint32");
+    clr_ = PointSrc{reinterpret_cast<const char*>(p), n, off, stride};
+    return GVDB_OK;
+}
+
+int Volume::upload_colors(const uint32_t* host, int n) {
+    if (!host || n <= 0) return set_colors_gpu(nullptr, 0, 0, 4);
+    HIPCHK(clr_owned_.ensure(size_t(n) * 4));
+    HIPCHK(hipMemcpyAsync(clr_owned_.p, host, size_t(n) * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    clr_ = PointSrc{reinterpret_cast<const char*>(clr_owned_.p), n, 0, 4};
+    return GVDB_OK;
+}
+
+int Volume::set_color_channel(int c) {
+    // SetColorChannel (gvdb_volume_gvdb.cpp:686-689); 255 (CHAN_UNDEF) or -1: none
+    if (c < 0 || c == CHAN_UNDEF) {
+        clr_chan_ = CHAN_UNDEF;
+        return GVDB_OK;
+    }
+    if (c >= 8) return fail(GVDB_ERR_INVALID, "SetColorChannel: channel 0..7");
+    clr_chan_ = c;
     return GVDB_OK;
 }
 
@@ -660,16 +693,16 @@ int Volume::insert_points_subcell(int scs, int n, float radius, const float tran
 }
 
 int Volume::gather_density(int scs, int n, float radius, const float trans[3], int len, int chanD, int chanC, bool accum) {
-    (void)n; (void)trans; (void)chanC;
+    (void)n; (void)trans;
     Volume* v = this;
-    return multi_gather(&v, 1, scs, &len, &radius, chanD, accum, false, self_exec(), nullptr);
+    return multi_gather(&v, 1, scs, &len, &radius, chanD, accum, false, self_exec(), nullptr, false, chanC);
 }
 
 // GatherLevelSet (gvdb_volume_gvdb.cpp:5348-5371)
 int Volume::gather_level_set(int scs, int n, float radius, const float trans[3], int len, int chanD, int chanC, bool accum) {
-    (void)n; (void)trans; (void)chanC;
+    (void)n; (void)trans;
     Volume* v = this;
-    return multi_gather(&v, 1, scs, &len, &radius, chanD, accum, true, self_exec(), nullptr);
+    return multi_gather(&v, 1, scs, &len, &radius, chanD, accum, true, self_exec(), nullptr, false, chanC);
 }
 
 // InsertPointsSubcell_FP16 (gvdb_volume_gvdb.cpp:5094-5189): the subcell
@@ -712,6 +745,8 @@ int Volume::gather_level_set_fp16(int scs, int n, float radius, const float tran
 int Volume::compute(int effect, int c, int iterations, const float p[3], bool update_apron_each, bool skip_aprons,
                     float boundval) {
     if (c < 0 || c >= 8 || !chan_[c].exists) return fail(GVDB_ERR_INVALID, "Compute: no such channel");
+    if (chan_[c].type == T_UCHAR4)
+        return fail(GVDB_ERR_UNSUPPORTED, "Compute: the uchar effects are not provided (float and half channels)");
     if (effect != OP_FILL && effect != OP_SMOOTH && effect != OP_NOISE && effect != OP_GROW)
         return fail(GVDB_ERR_UNSUPPORTED, "Compute: effect not provided (float channels: fill, smooth, noise, grow)");
     if (effect != OP_FILL && apron_ != 1)

@@ -92,6 +92,11 @@ public:
     // points
     int set_points_gpu(const void* p, int n, int off, int stride);
     int upload_points(const float* host, int n);
+    // point colours (SetPoints' colour buffer) and the render's colour channel
+    int set_colors_gpu(const void* p, int n, int off, int stride);
+    int upload_colors(const uint32_t* host, int n);
+    int set_color_channel(int chan);
+    int color_channel() const { return clr_chan_; }
 
     // topology
     // RequestFullRebuild (gvdb_volume_gvdb.h:432): sets mRebuildTopo to `on`
@@ -193,8 +198,9 @@ public:
                             int* len, const Exec& ex, int* failed, int rule = 0);
     // levelset: GatherLevelSet; fp16 (with levelset): GatherLevelSet_FP16 over
     // the u16 entries of InsertPointsSubcell_FP16
+    // chanC >= 0: also the colour channel, from the colours the insert carried
     static int multi_gather(Volume* const* v, int G, int scs, const int* len, const float* radius, int chanD, bool accum,
-                            bool levelset, const Exec& ex, int* failed, bool fp16 = false);
+                            bool levelset, const Exec& ex, int* failed, bool fp16 = false, int chanC = -1);
     static int multi_materialize(Volume* const* v, int G, const Exec& ex, int* failed);
     static int multi_apron(Volume* const* v, int G, int chan, float boundval, const Exec& ex, int* failed);
     static int multi_render(Volume* const* v, int G, const int* shading, const int* chan, const int* buf, const Exec& ex,
@@ -246,6 +252,10 @@ private:
     // points
     PointSrc pts_{nullptr, 0, 0, 12};
     DevBuf pts_owned_;
+    PointSrc clr_{nullptr, 0, 0, 4};   // colours, or base null
+    DevBuf clr_owned_;
+    int clr_chan_ = CHAN_UNDEF;
+    bool bkt_clr_ = false;             // the current buckets carry colours
 
     // subcells
     DevBuf sc_cnt_, sc_use_, sc_off_, sc_pos_, sc_nid_, sc_map_, sc_pnt_, sc_flags_;

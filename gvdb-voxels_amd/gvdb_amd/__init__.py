@@ -16,13 +16,14 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libgvdb_amd.so")
 
 OK = 0
 ERR_INVALID, ERR_HIP, ERR_STATE, ERR_DEPTH, ERR_NOBRICKS, ERR_RANGE, ERR_UNSUPPORTED, ERR_OOM = -1, -2, -3, -4, -5, -6, -7, -8
-T_FLOAT, T_HALF = 3, 100
+T_UCHAR4, T_FLOAT, T_HALF = 2, 3, 100
+CHAN_UNDEF = 255
 SHADE_TRILINEAR, SHADE_VOLUME = 4, 7
 FUNC_FILL_F, FUNC_SMOOTH, FUNC_NOISE, FUNC_GROW = 110, 113, 114, 115
 SCN_FLOATS = 68
 # launch-shape knobs of the autotuner (gvdb_set_tuning / gvdb_get_tuning)
 KNOB_RENDER_UNROLL_EA, KNOB_RENDER_UNROLL_ISO, KNOB_RENDER_TILE_EA, KNOB_RENDER_TILE_ISO = 0, 1, 2, 3
-KNOB_GATHER_THREADS, KNOB_APRON_RUN, KNOB_XCD_ORDER = 4, 5, 6
+KNOB_GATHER_THREADS, KNOB_APRON_RUN, KNOB_XCD_ORDER, KNOB_APRON_QUEUE, KNOB_MARK_PATH = 4, 5, 6, 7, 8
 # the allocation-free UpdateApron: size of its fixed device array and the bricks it holds
 NOALLOC_SCRATCH_BYTES = 32 << 20
 NOALLOC_MAX_BRICKS = NOALLOC_SCRATCH_BYTES // (27 * 12)
@@ -41,6 +42,8 @@ _SIGS = [
     ("gvdb_configure", _i, [_vp, _i, _vp, _vp]), ("gvdb_set_channel_default", _i, [_vp, _i, _i, _i]),
     ("gvdb_add_channel", _i, [_vp, _i, _i, _i, _i, _i, _i, _i, _i]), ("gvdb_set_epsilon", _i, [_vp, _f, _i]),
     ("gvdb_set_points_gpu", _i, [_vp, _vp, _i, _i, _i]), ("gvdb_upload_points", _i, [_vp, _vp, _i]),
+    ("gvdb_set_colors_gpu", _i, [_vp, _vp, _i, _i, _i]), ("gvdb_upload_colors", _i, [_vp, _vp, _i]),
+    ("gvdb_set_color_channel", _i, [_vp, _i]),
     ("gvdb_request_full_rebuild", _i, [_vp, _i]), ("gvdb_rebuild_topology", _i, [_vp, _i, _f, _vp]),
     ("gvdb_rebuild_topology_cpu", _i, [_vp, _i, _vp, _vp]), ("gvdb_activate_space", _i, [_vp, _vp]),
     ("gvdb_accumulate_topology", _i, [_vp, _i, _f, _vp, _i]),
@@ -223,10 +226,20 @@ class VolumeGVDB:
         return self._chk(load().gvdb_set_epsilon(self.h, float(eps), int(maxiter)))
 
     # points ----------------------------------------------------------------
-    def SetPoints(self, pts):
-        """Uploads host float3 positions (AllocData + CommitData + SetPoints)."""
+    def SetPoints(self, pts, colors=None):
+        """Uploads host float3 positions (AllocData + CommitData + SetPoints),
+        and with `colors` a uint32 (uchar4 r, g, b, a) per point; without,
+        the colours are cleared."""
         pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
-        return self._chk(load().gvdb_upload_points(self.h, _p(pts), len(pts)))
+        self._chk(load().gvdb_upload_points(self.h, _p(pts), len(pts)))
+        if colors is None:
+            return self._chk(load().gvdb_set_colors_gpu(self.h, None, 0, 0, 4))
+        c = np.ascontiguousarray(colors, dtype=np.uint32).reshape(-1)
+        return self._chk(load().gvdb_upload_colors(self.h, _p(c), len(c)))
+
+    def SetColorChannel(self, chan):
+        """SetColorChannel(chan): the march reads that uchar4 channel's colours (255 or -1: none)."""
+        return self._chk(load().gvdb_set_color_channel(self.h, int(chan)))
 
     def SetDataGPU(self, dev_ptr, n, offset=0, stride=12):
         return self._chk(load().gvdb_set_points_gpu(self.h, ctypes.c_void_p(dev_ptr), n, offset, stride))
@@ -622,7 +635,9 @@ class GridBatch:
         if k == 0:
             return int(best[0]), {128: float(ms[0]), 256: float(ms[1])}
         if k == 1:
-            return int(best[0]), {v: float(ms[i]) for i, v in enumerate((1, 2, 4, 8))}
+            d = {v: float(ms[i]) for i, v in enumerate((1, 2, 4, 8))}
+            d["queue"] = float(ms[4])
+            return ("queue" if best[1] else int(best[0])), d
         return (int(best[0]), int(best[1])), {(tw, u): float(ms[i * 3 + j]) for i, tw in enumerate((8, 16, 32, 64))
                                               for j, u in enumerate((1, 2, 4))}
 

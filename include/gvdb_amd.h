@@ -43,8 +43,10 @@ extern "C" {
 #define GVDB_ERR_IO (-9)          /* file missing, unreadable or malformed (VBX) */
 
 /* channel types (src/gvdb_types.h of the reference); T_HALF is an addition */
+#define GVDB_T_UCHAR4 2
 #define GVDB_T_FLOAT 3
 #define GVDB_T_HALF 100
+#define GVDB_CHAN_UNDEF 255 /* no colour channel */
 
 /* shading modes (kernels/cuda_gvdb_scene.cuh:25-32 of the reference) */
 #define GVDB_SHADE_TRILINEAR 4
@@ -83,6 +85,15 @@ int gvdb_set_epsilon(gvdb_handle h, float eps, int maxiter);
 int gvdb_set_points_gpu(gvdb_handle h, const void* dptr, int n, int offset, int stride);
 /* AllocData + CommitData: gvdb_volume_gvdb.cpp:4821-4862 (owning copy of host float3 positions) */
 int gvdb_upload_points(gvdb_handle h, const float* host_xyz, int n);
+/* SetPoints' colour DataPtr (gvdb_volume_gvdb.cpp:4900-4908): a uint32 (uchar4 r, g, b, a) per point at
+ * dptr + i*stride + offset, not owned; null (or n = 0) clears it.  InsertPointsSubcell carries the colours with the
+ * points; GatherDensity / GatherLevelSet then write chanClr (cuda_gvdb_particles.cuh:717-731). */
+int gvdb_set_colors_gpu(gvdb_handle h, const void* dptr, int n, int offset, int stride);
+/* the same from host memory (copied into a buffer the handle owns) */
+int gvdb_upload_colors(gvdb_handle h, const uint32_t* host_rgba, int n);
+/* SetColorChannel(chan): gvdb_volume_gvdb.cpp:686-689; the march reads that uchar4 channel's colours
+ * (getColorF, cuda_gvdb_raycast.cuh:206-209); GVDB_CHAN_UNDEF or -1: none */
+int gvdb_set_color_channel(gvdb_handle h, int chan);
 
 /* RequestFullRebuild: gvdb_volume_gvdb.h:432 */
 int gvdb_request_full_rebuild(gvdb_handle h, int on);
@@ -205,6 +216,8 @@ int gvdb_read_render_buf(gvdb_handle h, int buf, void* host_dst);
 #define GVDB_KNOB_GATHER_THREADS 4    /* threads per brick of the scatter gather, radius <= 3: 128, 256 */
 #define GVDB_KNOB_APRON_RUN 5         /* bricks per workgroup of the multi-grid apron: 1, 2, 4, 8 */
 #define GVDB_KNOB_XCD_ORDER 6         /* per-brick kernels: 0 launch order, 1 each XCD a contiguous range of bricks */
+#define GVDB_KNOB_APRON_QUEUE 7       /* multi-grid apron: 0 static runs, 1 persistent workgroups over a brick queue */
+#define GVDB_KNOB_MARK_PATH 8         /* topology marks: 0 by domain size, 1 byte per brick, 2 direction masks, 3 bits */
 int gvdb_set_tuning(int knob, int value);
 /* current value, and the values it can take (choices4, may be NULL; nchoices of them) */
 int gvdb_get_tuning(int knob, int* value, int* choices4, int* nchoices);
@@ -285,7 +298,8 @@ int gvdb_batch_update_apron(gvdb_batch b, int count, const gvdb_handle* hs, int 
 int gvdb_batch_render(gvdb_batch b, int count, const gvdb_handle* hs, const int* shading, const int* chan,
                       const int* buf, int* failed);
 /* Batched autotuner (no counterpart in the reference): stage 0 = gvdb_batch_gather_density (sc_len, radius, chan;
- * threads per brick, ms[2]), 1 = gvdb_batch_update_apron (chan, boundval; bricks per workgroup, ms[4]), 2 =
+ * threads per brick, ms[2]), 1 = gvdb_batch_update_apron (chan, boundval; best = {bricks per static run, queue 0/1}, ms[5]: runs of 1, 2, 4, 8,
+ * then the queue), 2 =
  * gvdb_batch_render (shading, chans, bufs; best = {unroll, tile width}, ms[12]), each timed over the whole batch. */
 int gvdb_batch_tune(gvdb_batch b, int count, const gvdb_handle* hs, int stage, int subcell_size, const int* sc_len,
                     const float* radius, int chan, float boundval, const int* shading, const int* chans,

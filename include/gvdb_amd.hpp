@@ -39,7 +39,7 @@ struct DataPtr {
     int subdim_offset = 0;
 };
 
-enum { T_FLOAT = GVDB_T_FLOAT, T_HALF = GVDB_T_HALF };
+enum { T_UCHAR4 = GVDB_T_UCHAR4, T_FLOAT = GVDB_T_FLOAT, T_HALF = GVDB_T_HALF };
 enum { SHADE_TRILINEAR = GVDB_SHADE_TRILINEAR, SHADE_VOLUME = GVDB_SHADE_VOLUME };
 
 struct Error : std::runtime_error {
@@ -75,11 +75,15 @@ public:
     }
     void SetEpsilon(float eps, int maxiter) { check(gvdb_set_epsilon(h_, eps, maxiter), "SetEpsilon"); }
 
-    /* SetPoints(pos, vel, clr): only the positions are read on this path */
-    void SetPoints(const DataPtr& pos, const DataPtr& = DataPtr(), const DataPtr& = DataPtr()) {
+    /* SetPoints(pos, vel, clr): positions, and the colours (uint32 per point) when clr.gpu is set; the velocities
+     * are not read on this path */
+    void SetPoints(const DataPtr& pos, const DataPtr& = DataPtr(), const DataPtr& clr = DataPtr()) {
         check(gvdb_set_points_gpu(h_, pos.gpu, pos.num, pos.subdim_offset, pos.stride), "SetPoints");
+        check(gvdb_set_colors_gpu(h_, clr.gpu, clr.gpu ? clr.num : 0, clr.subdim_offset, clr.stride ? clr.stride : 4),
+              "SetPoints");
     }
     void SetPointsHost(const float* xyz, int n) { check(gvdb_upload_points(h_, xyz, n), "SetPoints"); }
+    void SetColorChannel(unsigned char chan) { check(gvdb_set_color_channel(h_, chan), "SetColorChannel"); }
 
     void RequestFullRebuild(bool on) { check(gvdb_request_full_rebuild(h_, on ? 1 : 0), "RequestFullRebuild"); }
     void RebuildTopology(int num_pnts, float radius, Vector3DF orig) {

@@ -95,12 +95,18 @@ struct Vol {
     u64 atlas_max = 0, atlas_last = 0, atlas_used = 0;
     std::vector<float> atlasF;
     std::vector<double> atlasD;
+    // one uchar4 channel (the colour channel of GatherDensity's chanClr and
+    // of SetColorChannel), same slot layout, a uint32 per voxel
+    bool has_clr = false;
+    std::vector<uint32_t> atlasC;
+    bool render_clr = false;   // SetColorChannel on that channel
     bool fp64 = true;      // keep the FP64 shadow atlas (off for the timed CPU baseline)
     std::vector<AtlasNode> amap;
     u64 amap_max = 0;
     // InsertPointsSubcell outputs
     std::vector<int> sc_cnt, sc_off, sc_nid, sc_mapping, sc_pos;
     std::vector<float> sc_pnt;
+    std::vector<uint32_t> sc_clr;   // per-entry colours (SetPoints' colour buffer), or empty
     int sc_len = 0;
     // InsertPointsSubcell_FP16 entries and the bounds they are quantised against
     std::vector<uint16_t> sc_pnt16;
@@ -791,6 +797,7 @@ static void atlasResizeTo(Vol& v, int cz) {
     size_t sz = size_t(r[0]) * r[1] * r[2];
     v.atlasF.resize(sz, 0.f);
     if (v.fp64) v.atlasD.resize(sz, 0.0);
+    if (v.has_clr) v.atlasC.resize(sz, 0u);
 }
 
 static void updateAtlas(Vol& v) {
@@ -852,7 +859,7 @@ static inline size_t aidx(const Vol& v, int x, int y, int z) {
 // variant's tests p < 0.  A point is binned when its count pass and its insert
 // pass both take it.
 static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float radius, const float trans[3], int* outLen,
-                               int rule = 0) {
+                               int rule = 0, const uint32_t* clr = nullptr) {
     int numBrick = int(v.pool[0][0].usedNum);
     *outLen = 0;
     v.sc16 = false;
@@ -943,6 +950,9 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
     v.sc_len = len;
     // insert pass
     v.sc_pnt.assign(size_t(len) * 3, 0.f);
+    // InsertSubcell's sc_pnt_clr (kernels/cuda_gvdb_particles.cuh:244,269)
+    if (clr) v.sc_clr.assign(size_t(len), 0u);
+    else v.sc_clr.clear();
     if (len > 0) {
 #pragma omp parallel for num_threads(nt) schedule(static, 1)
         for (int t = 0; t < nt; t++) {
@@ -955,6 +965,7 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
                 forEachSubcell(w, [&](int idx) {
                     size_t e = size_t(slot[idx]++);
                     v.sc_pnt[3 * e] = w[0]; v.sc_pnt[3 * e + 1] = w[1]; v.sc_pnt[3 * e + 2] = w[2];
+                    if (clr) v.sc_clr[e] = clr[i];
                 });
             }
         }
@@ -1027,6 +1038,67 @@ static void gatherT(Vol& v, int scs, float radius, bool accum, std::vector<T>& a
     }
 }
 
+// -------------------------------------------------- colour of the gather ----
+// gvdbGatherDensity / gvdbGatherLevelSet with a colour table
+// (kernels/cuda_gvdb_particles.cuh:717-731, :757-771): per voxel of a subcell,
+// clr = (0, 0, 0, 1), or CHAR2CLR(the voxel's colour) with bAccumulate; add
+// CHAR2CLR(entry colour) = byte / 255.0f per component for every entry of the
+// subcell's list; divide by the entry count (+1 with bAccumulate); store
+// CLR2CHAR = uchar(x * 255.0f) per component.  The float-to-uchar conversion
+// truncates and saturates (NaN: 0), so an empty list stores (0, 0, 0, 255).
+// exact = false: that float arithmetic in list order (which the reference's
+// atomics leave unordered); exact = true: the same mean taken exactly,
+// (255 [alpha only] + sum of bytes) / count, or (old + sum) / (count + 1), in
+// integer division.
+static uint8_t toU8(float x) {
+    if (!(x > 0.f)) return 0;                  // NaN and non-positive
+    if (x >= 255.f) return 255;
+    return uint8_t(x);
+}
+static void gatherColor(Vol& v, int scs, bool accum, bool exact) {
+    int spb = (v.res[0] / scs) * (v.res[0] / scs) * (v.res[0] / scs);
+    int numSC = spb * int(v.pool[0][0].usedNum);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int s = 0; s < numSC; s++) {
+        Node* nd = nodeAt(v, 0, u64(v.sc_nid[s]));
+        const int cnt = v.sc_cnt[s];
+        const size_t off = size_t(v.sc_off[s]);
+        uint64_t sum[4] = {0, 0, 0, 0};
+        for (int j = 0; j < cnt; j++) {
+            const uint32_t c = v.sc_clr[off + j];
+            for (int k = 0; k < 4; k++) {
+                sum[k] += (c >> (8 * k)) & 0xFFu;
+            }
+        }
+        for (int tz = 0; tz < scs; tz++)
+            for (int ty = 0; ty < scs; ty++)
+                for (int tx = 0; tx < scs; tx++) {
+                    int w[3] = {v.sc_pos[3 * s] + tx, v.sc_pos[3 * s + 1] + ty, v.sc_pos[3 * s + 2] + tz};
+                    int vox[3] = {nd->mValue[0] + (w[0] - nd->mPos[0]), nd->mValue[1] + (w[1] - nd->mPos[1]),
+                                  nd->mValue[2] + (w[2] - nd->mPos[2])};
+                    size_t ai = aidx(v, vox[0], vox[1], vox[2]);
+                    const uint32_t old = v.atlasC[ai];
+                    uint32_t out = 0;
+                    for (int k = 0; k < 4; k++) {
+                        const uint32_t ob = (old >> (8 * k)) & 0xFFu;
+                        uint32_t b;
+                        if (exact) {
+                            if (accum) b = uint32_t((uint64_t(ob) + sum[k]) / uint64_t(cnt + 1));
+                            else if (cnt == 0) b = k == 3 ? 255u : 0u;
+                            else b = uint32_t(std::min<uint64_t>((sum[k] + (k == 3 ? 255u : 0u)) / uint64_t(cnt), 255u));
+                        } else {
+                            float f = accum ? float(ob) / 255.0f : (k == 3 ? 1.0f : 0.0f);
+                            for (int j = 0; j < cnt; j++) f += float((v.sc_clr[off + j] >> (8 * k)) & 0xFFu) / 255.0f;
+                            f /= float(cnt + (accum ? 1 : 0));
+                            b = toU8(f * 255.0f);
+                        }
+                        out |= b << (8 * k);
+                    }
+                    v.atlasC[ai] = out;
+                }
+    }
+}
+
 // -------------------------------------------------------- UpdateApron ----
 // src/gvdb_volume_gvdb.cpp:4418-4457; kernels/cuda_gvdb_operators.cuh:72-131
 // with getAtlasToWorld (kernels/cuda_gvdb_nodes.cuh:145-153) and getNodeAtPoint.
@@ -1034,7 +1106,9 @@ static void gatherT(Vol& v, int scs, float radius, bool accum, std::vector<T>& a
 // runs them too, through an atlas position that is either stale, and then
 // rewrites the value its current owner writes, or (-1, -1, -1), which it turns
 // into an out-of-range atlas-map read.
-template <typename T>
+// T: the voxel type; C: the coordinate arithmetic (UpdateApron<uchar4>,
+// kernels/cuda_gvdb_operators.cuh:143-146, walks in float like the float one).
+template <typename T, typename C = T>
 static void apronT(Vol& v, T boundval, std::vector<T>& atlas) {
     int bricks = int(v.pool[0][0].lastEle);
     int pbr = brickres(v);
@@ -1059,15 +1133,15 @@ static void apronT(Vol& v, T boundval, std::vector<T>& atlas) {
                     u64 bid = (u64(bi[2]) * v.atlas_cnt[1] + bi[1]) * v.atlas_cnt[0] + bi[0];
                     const AtlasNode& an = v.amap[bid];
                     if (an.mLeafNode == -1) continue;
-                    T wp[3];
-                    for (int a = 0; a < 3; a++) wp[a] = T(an.mPos[a]) + T(av[a] % pbr - v.apron) + T(0.5);
-                    i64 id = getNodeAtPointT<T>(v, wp);
+                    C wp[3];
+                    for (int a = 0; a < 3; a++) wp[a] = C(an.mPos[a]) + C(av[a] % pbr - v.apron) + C(0.5);
+                    i64 id = getNodeAtPointT<C>(v, wp);
                     T value;
                     if (id < 0) value = boundval;
                     else {
                         Node* nn = nodeAt(v, 0, u64(id));
                         int o[3];
-                        for (int a = 0; a < 3; a++) o[a] = int(unsigned(T(nn->mValue[a]) + (wp[a] - T(nn->mPos[a]))));
+                        for (int a = 0; a < 3; a++) o[a] = int(unsigned(C(nn->mValue[a]) + (wp[a] - C(nn->mPos[a]))));
                         value = atlas[aidx(v, o[0], o[1], o[2])];
                     }
                     atlas[aidx(v, av[0], av[1], av[2])] = value;
@@ -1204,6 +1278,7 @@ template <typename T> struct RenderCtx {
     const std::vector<T>* atlas;
     int ares[3];
     const float* tf;   // 16384 x float4
+    const std::vector<uint32_t>* clr = nullptr;   // SetColorChannel's uchar4 channel, or none
     V3<T> light;
     T extinct, albedo, dstep, minval, alphacut, thresh, vmin, vmax, shadowamt, shadowbias, eps;
     T back[4];
@@ -1271,6 +1346,19 @@ template <typename T> struct HDDA {
     }
 };
 
+// getColorF (kernels/cuda_gvdb_raycast.cuh:206-209): the colour channel's
+// voxel at the integer part of atlas position p, bytes / 255; (1, 1, 1, 1)
+// without a colour channel
+template <typename T> static void colorAt(const RenderCtx<T>& c, V3<T> p, T out[4]) {
+    if (!c.clr) {
+        for (int k = 0; k < 4; k++) out[k] = T(1);
+        return;
+    }
+    auto cl = [&](int i, int a) { return std::min(std::max(i, 0), c.ares[a] - 1); };
+    const uint32_t u = (*c.clr)[(size_t(cl(int(p.z), 2)) * c.ares[1] + cl(int(p.y), 1)) * c.ares[0] + cl(int(p.x), 0)];
+    for (int k = 0; k < 4; k++) out[k] = T(float((u >> (8 * k)) & 0xFFu) / 255.0f);
+}
+
 template <typename T> static void tfLookup(const RenderCtx<T>& c, T v, T out[4]) {
     // transfer(): kernels/cuda_gvdb_dda.cuh:20-23
     T x = (v - c.thresh) / (c.vmax - c.vmin);
@@ -1299,9 +1387,11 @@ static void rayDeepBrick(const RenderCtx<T>& c, int nodeid, V3<T> t, V3<T> pos, 
             T val[4];
             tfLookup(c, raw, val);
             val[3] = std::exp(c.extinct * val[3] * c.dstep);
-            clr[0] += val[0] * clr[3] * (1 - val[3]) * c.albedo;
-            clr[1] += val[1] * clr[3] * (1 - val[3]) * c.albedo;
-            clr[2] += val[2] * clr[3] * (1 - val[3]) * c.albedo;
+            T h[4];
+            colorAt(c, mk(p.x + o.x, p.y + o.y, p.z + o.z), h);
+            clr[0] += val[0] * clr[3] * (1 - val[3]) * c.albedo * h[0];
+            clr[1] += val[1] * clr[3] * (1 - val[3]) * c.albedo * h[1];
+            clr[2] += val[2] * clr[3] * (1 - val[3]) * c.albedo * h[2];
             clr[3] *= val[3];
         }
         p = p + wpt;
@@ -1323,7 +1413,8 @@ template <typename T> static V3<T> gradient(const RenderCtx<T>& c, V3<T> p) {
 }
 
 template <typename T>
-static void raySurfTriBrick(const RenderCtx<T>& c, int nodeid, V3<T> t, V3<T> pos, V3<T> dir, V3<T>& hit, V3<T>& norm) {
+static void raySurfTriBrick(const RenderCtx<T>& c, int nodeid, V3<T> t, V3<T> pos, V3<T> dir, V3<T>& hit, V3<T>& norm,
+                            T hclr[4]) {
     // kernels/cuda_gvdb_raycast.cuh:281-300
     Vol& v = *c.v;
     Node* nd = nodeAt(v, 0, u64(nodeid));
@@ -1336,6 +1427,7 @@ static void raySurfTriBrick(const RenderCtx<T>& c, int nodeid, V3<T> t, V3<T> po
         if (tex3(c, p.x + o.x, p.y + o.y, p.z + o.z) >= c.thresh) {
             hit = p + vmin;
             norm = gradient(c, p + o);
+            if (c.clr) colorAt(c, p + o, hclr);
             return;
         }
         p = p + dir * c.dstep;
@@ -1391,7 +1483,7 @@ static bool rayCast(const RenderCtx<T>& c, V3<T> pos, V3<T> dir, V3<T>& hit, V3<
             if (lev == 1) {
                 dda.t.x += c.eps;
                 if (mode == 0) rayDeepBrick(c, child, dda.t, pos, dir, hit, clr);
-                else raySurfTriBrick(c, child, dda.t, pos, dir, hit, norm);
+                else raySurfTriBrick(c, child, dda.t, pos, dir, hit, norm, clr);
                 if (clr[3] <= 0) { clr[3] = 0; return true; }
                 if (hit.z != T(NOHIT)) return true;
                 dda.Step();
@@ -1428,6 +1520,7 @@ static void renderT(Vol& v, const std::vector<T>& atlas, const ScnBlock& s, cons
     c.atlas = &atlas;
     atlasRes(v, c.ares);
     c.tf = tf;
+    if (v.render_clr && v.has_clr) c.clr = &v.atlasC;
     c.light = mk(T(s.light_pos[0]), T(s.light_pos[1]), T(s.light_pos[2]));
     c.extinct = T(s.extinct[0]); c.albedo = T(s.extinct[1]);
     c.dstep = T(s.steps[0]);
@@ -1722,6 +1815,52 @@ unsigned orc_hash(unsigned x) { return oaat(x); }
 void orc_atlas_map_copy(void* h, void* dst) {
     Vol& v = *static_cast<Vol*>(h);
     memcpy(dst, v.amap.data(), v.amap.size() * sizeof(AtlasNode));
+}
+
+// the colour channel: AddChannel(chan, T_UCHAR4, apron) on the same atlas
+void orc_add_color_channel(void* h) {
+    Vol& v = *static_cast<Vol*>(h);
+    v.has_clr = true;
+    int r[3];
+    atlasRes(v, r);
+    v.atlasC.assign(size_t(r[0]) * r[1] * r[2], 0u);
+}
+void orc_set_color_channel(void* h, int on) { static_cast<Vol*>(h)->render_clr = on != 0; }
+void orc_clear_color(void* h) {
+    Vol& v = *static_cast<Vol*>(h);
+    std::fill(v.atlasC.begin(), v.atlasC.end(), 0u);
+}
+void orc_color_copy(void* h, void* dst) {
+    Vol& v = *static_cast<Vol*>(h);
+    memcpy(dst, v.atlasC.data(), v.atlasC.size() * 4);
+}
+void orc_color_set(void* h, const void* src) {
+    Vol& v = *static_cast<Vol*>(h);
+    memcpy(v.atlasC.data(), src, v.atlasC.size() * 4);
+}
+// InsertPointsSubcell with SetPoints' colour buffer (a uint32 per point)
+int orc_insert_points_subcell_clr(void* h, int scs, const float* pts, const uint32_t* clr, int n, float radius,
+                                  const float* trans, int* outLen) {
+    return insertPointsSubcell(*static_cast<Vol*>(h), scs, pts, n, radius, trans, outLen, 0, clr);
+}
+// GatherDensity's colour channel (the density part is orc_gather_density);
+// exact = 0: the reference's float arithmetic in list order
+int orc_gather_color(void* h, int scs, int accum, int exact) {
+    Vol& v = *static_cast<Vol*>(h);
+    if (!v.has_clr || v.sc_clr.size() != size_t(v.sc_len)) return -3;
+    if (v.sc_len == 0) return 0;
+    gatherColor(v, scs, accum != 0, exact != 0);
+    return 0;
+}
+// UpdateApron<uchar4> (kernels/cuda_gvdb_operators.cuh:143-146): boundval
+// converted to uchar per component
+int orc_update_apron_color(void* h, float boundval) {
+    Vol& v = *static_cast<Vol*>(h);
+    if (!v.has_clr) return -3;
+    if (v.apron == 0) return 0;
+    const uint32_t b = uint32_t(toU8(boundval)) * 0x01010101u;
+    apronT<uint32_t, float>(v, b, v.atlasC);
+    return 0;
 }
 
 int orc_insert_points_subcell(void* h, int scs, const float* pts, int n, float radius, const float* trans, int* outLen) {

@@ -61,6 +61,10 @@ def lib():
             "orc_linear_tf": (None, [f, f, fp, fp, fp]),
             "orc_render": (i, [vp, fp, fp, i, i, fp, vp]), "orc_stage_ms": (ctypes.c_double, [vp, i]),
             "orc_max_threads": (i, []), "orc_set_threads": (None, [i]),
+            "orc_add_color_channel": (None, [vp]), "orc_set_color_channel": (None, [vp, i]),
+            "orc_clear_color": (None, [vp]), "orc_color_copy": (None, [vp, vp]), "orc_color_set": (None, [vp, vp]),
+            "orc_insert_points_subcell_clr": (i, [vp, i, fp, vp, i, f, fp, ip]),
+            "orc_gather_color": (i, [vp, i, i, i]), "orc_update_apron_color": (i, [vp, f]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -174,11 +178,51 @@ class Oracle:
         return a
 
     # points ---------------------------------------------------------------
-    def insert_points_subcell(self, pts, radius, scs=4, trans=(0, 0, 0)):
+    def insert_points_subcell(self, pts, radius, scs=4, trans=(0, 0, 0), colors=None):
+        """InsertPointsSubcell; with `colors` (a uint32 per point) the entries
+        carry them, as with SetPoints' colour buffer."""
         pts = np.ascontiguousarray(pts, dtype=np.float32)
         n = ctypes.c_int(0)
-        rc = lib().orc_insert_points_subcell(self.h, scs, _p(pts), len(pts), float(radius), _p(f3(trans)), ctypes.byref(n))
+        if colors is None:
+            rc = lib().orc_insert_points_subcell(self.h, scs, _p(pts), len(pts), float(radius), _p(f3(trans)),
+                                                 ctypes.byref(n))
+        else:
+            c = np.ascontiguousarray(colors, dtype=np.uint32).reshape(-1)
+            assert len(c) >= len(pts)
+            rc = lib().orc_insert_points_subcell_clr(self.h, scs, _p(pts), _p(c), len(pts), float(radius),
+                                                     _p(f3(trans)), ctypes.byref(n))
         return rc, n.value
+
+    # the colour channel (one uchar4 channel on the same atlas) -----------
+    def add_color_channel(self):
+        lib().orc_add_color_channel(self.h)
+
+    def set_color_channel(self, on=True):
+        """SetColorChannel: the march reads the colour channel."""
+        lib().orc_set_color_channel(self.h, int(on))
+
+    def clear_color(self):
+        lib().orc_clear_color(self.h)
+
+    def color_atlas(self):
+        """The colour channel in the atlas layout, a uint32 (uchar4) per voxel."""
+        rx, ry, rz = self.atlas_info()["res"]
+        a = np.zeros((rz, ry, rx), np.uint32)
+        lib().orc_color_copy(self.h, _p(a))
+        return a
+
+    def set_color_atlas(self, a):
+        a = np.ascontiguousarray(a, np.uint32)
+        assert a.size == np.prod(self.atlas_info()["res"])
+        lib().orc_color_set(self.h, _p(a))
+
+    def gather_color(self, scs=4, accumulate=False, exact=True):
+        """GatherDensity's colour channel from the entries' colours; exact:
+        the mean in integers, else the reference's float sum in list order."""
+        return lib().orc_gather_color(self.h, scs, int(accumulate), int(exact))
+
+    def update_apron_color(self, boundval=0.0):
+        return lib().orc_update_apron_color(self.h, float(boundval))
 
     def subcells(self, length):
         L = lib()

@@ -136,14 +136,16 @@ def atlas_slots(nbricks, cnt_xy=(16, 16), br=10, apron=1):
 
 def subcell_model(pts, radius, leaves, scs=4, res0=8):
     """Expected subcell tables: leaves = list of brick corners in pool order,
-    all linked.  Returns cnt, off, pos, nid and the entry list in point order."""
+    all linked.  Returns cnt, off, pos, nid, the entry list in point order and
+    `idx`, each subcell's list as point indices."""
     w = np.asarray(pts, np.float32)
     sd = res0 // scs
     spb = sd ** 3
     index = {tuple(p): i for i, p in enumerate(leaves)}
     lists = [[] for _ in range(spb * len(leaves))]
+    idx = [[] for _ in range(spb * len(leaves))]
     r = np.float32(radius)
-    for p in w:
+    for k, p in enumerate(w):
         if p[2] == np.float32(1.0e10) or np.any(p < 0):
             continue
         lo = [int(ctrunc_div(int(np.float32(p[a] - r)), scs)) * scs for a in range(3)]
@@ -156,6 +158,7 @@ def subcell_model(pts, radius, leaves, scs=4, res0=8):
                         continue
                     lx, ly, lz = (x - b[0]) // scs, (y - b[1]) // scs, (z - b[2]) // scs
                     lists[index[b] * spb + (lz * sd + ly) * sd + lx].append(p)
+                    idx[index[b] * spb + (lz * sd + ly) * sd + lx].append(k)
     cnt = np.array([len(l) for l in lists], np.int64)
     off = np.concatenate([[0], np.cumsum(cnt)[:-1]]) if len(cnt) else cnt
     pos = np.zeros((len(lists), 3), np.int64)
@@ -165,7 +168,7 @@ def subcell_model(pts, radius, leaves, scs=4, res0=8):
             pos[i * spb + s] = (b[0] + s % sd * scs, b[1] + (s // sd) % sd * scs, b[2] + (s // sd // sd) * scs)
             nid[i * spb + s] = i
     pnt = np.asarray([p for l in lists for p in l], np.float32).reshape(-1, 3)
-    return dict(cnt=cnt, off=off, pos=pos, nid=nid, pnt=pnt)
+    return dict(cnt=cnt, off=off, pos=pos, nid=nid, pnt=pnt, idx=idx)
 
 
 def density_grid(pts, radius, lo, hi):

@@ -1,0 +1,56 @@
+"""GPU topology over wide, sparse point sets: the mark paths the build picks
+by the size of the points' brick box (a byte per brick up to 2^27 bricks,
+direction masks, bits set directly) give the oracle's pools bit for bit.
+
+The natural case: a few thousand clusters spread over a 6000^3-voxel box
+(a 750^3-brick domain, above 2^27, so bits are set directly).  Each path
+forced on a smaller box (the mark-path knob changes only how the bits are
+marked), at radius 0 and with dilation."""
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _clusters(n_clusters, per, span, seed, sigma=3.0):
+    g = np.random.default_rng(seed)
+    c = g.uniform(16.0, span - 16.0, (n_clusters, 3))
+    p = c[:, None, :] + g.normal(0.0, sigma, (n_clusters, per, 3))
+    return np.clip(p.reshape(-1, 3), 0.5, span - 0.5).astype(np.float32)
+
+
+def _gpu_topology(gpu, pts, radius):
+    v, _ = H.gpu_volume(gpu, pts, radius, stop_after="topology")
+    return v
+
+
+@pytest.mark.parametrize("radius", [0.0, 2.0])
+def test_sparse_domain_above_2_27_bricks(gpu, radius):
+    import oracle as O
+    pts = _clusters(3000, 10, 6000.0, 5)
+    lo, hi = pts.min(0), pts.max(0)
+    bricks = np.prod(np.floor(hi / 8) - np.floor(lo / 8) + 3)
+    assert bricks > 2 ** 27
+    v = _gpu_topology(gpu, pts, radius)
+    o, _ = H.oracle_volume(O, pts, radius, stop_after="topology")
+    H.assert_pools_equal(v, o)
+    assert v.info()["top_lev"] == o.top_lev()
+    v.close()
+
+
+@pytest.mark.parametrize("radius", [0.0, 2.0, 3.0])
+def test_every_mark_path_gives_the_same_pools(gpu, radius):
+    import oracle as O
+    pts = _clusters(400, 60, 1000.0, 6)
+    o, _ = H.oracle_volume(O, pts, radius, stop_after="topology")
+    keep = gpu.get_tuning(gpu.KNOB_MARK_PATH)[0]
+    try:
+        for path in (1, 2, 3, 0):
+            gpu.set_tuning(gpu.KNOB_MARK_PATH, path)
+            v = _gpu_topology(gpu, pts, radius)
+            H.assert_pools_equal(v, o)
+            v.close()
+    finally:
+        gpu.set_tuning(gpu.KNOB_MARK_PATH, keep)

@@ -4,8 +4,9 @@ one of its values, the fastest it timed.
 
 Knobs: samples fetched ahead (1, 2, 4) and ray-packet shape (8x8, 16x4,
 32x2, 64x1) of both marches, threads per brick of the scatter gather (128,
-256), bricks per workgroup of the multi-grid apron (1, 2, 4, 8), brick order
-of the per-brick kernels (launch order, XCD-contiguous)."""
+256), bricks per workgroup of the multi-grid apron (1, 2, 4, 8, or the
+persistent queue), brick order of the per-brick kernels (launch order,
+XCD-contiguous)."""
 import numpy as np
 import pytest
 
@@ -16,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture
 def restore_knobs(gpu):
-    keep = [gpu.get_tuning(k)[0] for k in range(7)]
+    keep = [gpu.get_tuning(k)[0] for k in range(9)]
     yield
     for k, v in enumerate(keep):
         gpu.set_tuning(k, v)
@@ -120,11 +121,12 @@ def test_batched_knobs_same_results(gpu, restore_knobs):
     B.UpdateAtlas(vols)
     B.ClearChannel(vols, 0)
     lens = B.InsertPointsSubcell(vols, 4, ns, 2.0)
-    for run in (1, 2, 4, 8):
+    for run in (1, 2, 4, 8, "queue"):
         for nt in (128, 256):
-            gpu.set_tuning(gpu.KNOB_APRON_RUN, run)
+            gpu.set_tuning(gpu.KNOB_APRON_QUEUE, 1 if run == "queue" else 0)
+            gpu.set_tuning(gpu.KNOB_APRON_RUN, 4 if run == "queue" else run)
             gpu.set_tuning(gpu.KNOB_GATHER_THREADS, nt)
-            gpu.set_tuning(gpu.KNOB_XCD_ORDER, (run // 2 + nt // 256) & 1)
+            gpu.set_tuning(gpu.KNOB_XCD_ORDER, ((8 if run == "queue" else run) // 2 + nt // 256) & 1)
             B.ClearChannel(vols, 0)
             B.GatherDensity(vols, 4, lens, 2.0, 0)
             B.UpdateApron(vols, 0, 0.0)
@@ -140,8 +142,9 @@ def test_batched_knobs_same_results(gpu, restore_knobs):
     bg, msg = B.Tune(vols, "gather", 4, lens, 2.0)
     ba, msa = B.Tune(vols, "apron")
     br, msr = B.Tune(vols, "render", shading=gpu.SHADE_VOLUME)
-    assert bg in (128, 256) and ba in (1, 2, 4, 8) and br[0] in (1, 2, 4) and br[1] in (8, 16, 32, 64)
-    assert gpu.get_tuning(gpu.KNOB_APRON_RUN)[0] == ba and gpu.get_tuning(gpu.KNOB_GATHER_THREADS)[0] == bg
+    assert bg in (128, 256) and ba in (1, 2, 4, 8, "queue") and br[0] in (1, 2, 4) and br[1] in (8, 16, 32, 64)
+    assert gpu.get_tuning(gpu.KNOB_GATHER_THREADS)[0] == bg
+    assert gpu.get_tuning(gpu.KNOB_APRON_QUEUE)[0] == (1 if ba == "queue" else 0)
     assert msa[ba] == min(msa.values()) and msg[bg] == min(msg.values())
     B.close()
     for v in vols:

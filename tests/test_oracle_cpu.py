@@ -757,3 +757,143 @@ def test_grow_and_fill_closed_forms():
     hi = [min(8 * ((c * 8 + 7) // 8), r - 1) for c, r in zip(o.atlas_info()["cnt"], o.atlas_info()["res"])]
     assert (b[1:hi[2] + 1, 1:hi[1] + 1, 1:hi[0] + 1] == 2.5).all()
     assert (b != 0).sum() == hi[0] * hi[1] * hi[2]
+
+
+# ------------------------------------------------------------ colour channel
+def _colored(pts, radius, clr, scs=4):
+    o = O.Oracle()
+    assert o.rebuild_topology(pts, radius) == 0
+    o.finish_topology()
+    o.update_atlas()
+    o.add_color_channel()
+    o.clear_channel()
+    rc, n = o.insert_points_subcell(pts, radius, scs, colors=clr)
+    assert rc == 0
+    return o, n
+
+
+def _subcell_voxels(o, scs=4):
+    """(subcell index, atlas indices (z, y, x) of its voxels) for every subcell."""
+    leaves = o.pool(0, 0)
+    sd = 8 // scs
+    for b, lf in enumerate(leaves):
+        for s in range(sd ** 3):
+            lx, ly, lz = s % sd * scs, s // sd % sd * scs, s // sd // sd * scs
+            v = lf["mValue"]
+            zz, yy, xx = np.meshgrid(np.arange(scs) + v[2] + lz, np.arange(scs) + v[1] + ly, np.arange(scs) + v[0] + lx,
+                                     indexing="ij")
+            yield b * sd ** 3 + s, (zz.ravel(), yy.ravel(), xx.ravel())
+
+
+@pytest.mark.parametrize("accumulate", [False, True])
+def test_color_gather_is_the_exact_mean_of_each_list(accumulate):
+    """gvdbGatherDensity's colour (particles.cuh:717-731) taken exactly: every
+    voxel of a subcell gets (255 [alpha] + sum of its list's bytes) / count,
+    or (old + sum) / (count + 1) when accumulating; an empty list stores
+    (0, 0, 0, 255).  Lists from the per-point binning model."""
+    pts = _ball(700, 61)
+    g = np.random.default_rng(7)
+    clr = g.integers(0, 1 << 32, len(pts), dtype=np.uint64).astype(np.uint32)
+    o, n = _colored(pts, 2.0, clr)
+    leaves = [tuple(p) for p in o.pool(0, 0)["mPos"].tolist()]
+    m = M.subcell_model(pts, 2.0, leaves, 4)
+    old = None
+    if accumulate:
+        old = g.integers(0, 1 << 32, o.color_atlas().shape, dtype=np.uint64).astype(np.uint32)
+        o.set_color_atlas(old)
+    assert o.gather_color(4, accumulate, exact=True) == 0
+    a = o.color_atlas()
+    byte = (clr[:, None] >> np.arange(0, 32, 8, dtype=np.uint32)) & 0xFF
+    for s, vox in _subcell_voxels(o):
+        ids = m["idx"][s]
+        tot = byte[ids].astype(np.int64).sum(0) if ids else np.zeros(4, np.int64)
+        got = (a[vox][:, None] >> np.arange(0, 32, 8, dtype=np.uint32)) & 0xFF
+        if accumulate:
+            ob = (old[vox][:, None] >> np.arange(0, 32, 8, dtype=np.uint32)) & 0xFF
+            want = (ob.astype(np.int64) + tot[None, :]) // (len(ids) + 1)
+        elif not ids:
+            want = np.tile(np.array([0, 0, 0, 255]), (len(vox[0]), 1))
+        else:
+            want = np.tile(np.minimum((tot + np.array([0, 0, 0, 255])) // len(ids), 255), (len(vox[0]), 1))
+        assert np.array_equal(got.astype(np.int64), want), s
+
+
+def test_color_gather_closed_forms_and_the_float_formula():
+    """One colour for every point: each byte of a non-empty subcell is that
+    byte (alpha: min(255, (255 + n a) / n)); the reference's float sum
+    (list order) differs from the exact mean by at most one unit."""
+    pts = _ball(900, 62)
+    c = np.uint32(0x50C81E0A)      # r 10, g 30, b 200, a 80
+    o, n = _colored(pts, 2.0, np.full(len(pts), c, np.uint32))
+    assert o.gather_color(4, False, exact=True) == 0
+    a = o.color_atlas()
+    s = o.subcells(n)
+    for sc, vox in _subcell_voxels(o):
+        k = int(s["cnt"][sc])
+        want = 0xFF000000 if k == 0 else (0x00C81E0A | (min(255, (255 + k * 80) // k) << 24))
+        assert np.all(a[vox] == want), sc
+    g = np.random.default_rng(8)
+    clr = g.integers(0, 1 << 32, len(pts), dtype=np.uint64).astype(np.uint32)
+    for acc in (False, True):
+        o, n = _colored(pts, 2.0, clr)
+        o.gather_color(4, acc, exact=True)
+        e = o.color_atlas().view(np.uint8).astype(np.int64)
+        o.clear_color()
+        o.gather_color(4, acc, exact=False)
+        f = o.color_atlas().view(np.uint8).astype(np.int64)
+        assert np.abs(e - f).max() <= 1 and np.count_nonzero(e != f) < 0.05 * np.count_nonzero(e)
+
+
+def test_color_apron_moves_the_same_voxels_as_the_float_apron():
+    """UpdateApron<uchar4> walks as UpdateApron<float>: with the float
+    channel's bits copied into the colour channel, the two aprons give the
+    same bits; the boundary value becomes a truncated, clamped byte."""
+    pts = _ball(1000, 63)
+    o, n = _colored(pts, 2.0, np.zeros(len(pts), np.uint32))
+    o.gather_density(2.0, 4, 32)
+    o.set_color_atlas(o.atlas(32).view(np.uint32))
+    o.update_apron(0.0, 32)
+    o.update_apron_color(0.0)
+    assert np.array_equal(o.color_atlas(), o.atlas(32).view(np.uint32))
+    for bv, byte in ((3.7, 3), (-2.0, 0), (300.0, 255), (255.0, 255)):
+        o.update_apron(bv, 32)
+        o.update_apron_color(bv)
+        f, c = o.atlas(32), o.color_atlas()
+        m = f == np.float32(bv)
+        assert m.any()
+        assert np.all(c[m] == np.uint32(byte * 0x01010101))
+
+
+def test_render_color_channel_scales_by_a_uniform_color():
+    """SetColorChannel with every voxel one colour h: the emission-absorption
+    colour and the isosurface colour are h times the ones without it, per
+    channel (the reference multiplies each sample's term, and the surface
+    colour, by getColorF)."""
+    pts = _ball(1500, 64)
+    o = _gathered(pts, 2.0, precision=32)
+    o.add_color_channel()
+    rx, ry, rz = o.atlas_info()["res"]
+    h = np.array([0.2, 0.6, 1.0])
+    o.set_color_atlas(np.full((rz, ry, rx), np.uint32(0xFF000000 | (255 << 16) | (153 << 8) | 51), np.uint32))
+    lo, hi = pts.min(0), pts.max(0)
+    for shading, tf, vr in ((7, O.linear_tf(0.0, 1.0, (1, 1, 1, 0), (1, 0.9, 0.7, 1)), (0.0, 0.0, 2.0)),
+                            (4, O.linear_tf(0.0, 1.0, (0, 0, 0, 0), (1, 1, 1, 0.1)), (0.5, 0.0, 1.0))):
+        sc = _scene(96, 80, tuple((lo + hi) / 2), (30.0, 20.0, 0.0), 300.0, vrange=vr, light=(400.0, 500.0, 300.0))
+        blk = O.scene_block(sc)
+        o.set_color_channel(False)
+        a = o.render(blk, tf, shading, 32).astype(np.float64)
+        o.set_color_channel(True)
+        b = o.render(blk, tf, shading, 32).astype(np.float64)
+        back = np.asarray(sc["backclr"], np.float64)
+        if shading == 7:
+            # res = back + alpha (clr - back): clr scales by h
+            al = a[..., 3:4]
+            clr_a = np.where(al > 0, back[:3] + (a[..., :3] - back[:3]) / np.where(al > 0, al, 1), 0)
+            want = back[:3] + al * (h * clr_a - back[:3])
+            hitm = al[..., 0] > 1e-3
+        else:
+            want = a[..., :3] * h
+            hitm = np.any(np.abs(a[..., :3] - back[:3]) > 1e-6, axis=-1)
+        assert hitm.sum() > 100
+        assert np.abs(b[..., :3][hitm] - want[hitm]).max() < 2e-5, shading
+        assert np.array_equal(b[..., 3], a[..., 3])
