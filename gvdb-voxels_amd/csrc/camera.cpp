@@ -8,6 +8,7 @@
 // (updateFrustum / inverseRayProj, src/gvdb_camera.cpp:292-346, :382-393).
 // The aspect ratio is the render buffer's (Scene::SetRes).
 #include "volume.h"
+#include "../../include/gvdb_amd.h"
 #include <cmath>
 #include <cstring>
 
@@ -124,7 +125,63 @@ void Volume::scene_block(int w, int h, ScnBlock& s) const {
         s.shadow[a] = shadow_[a];
     }
     for (int k = 0; k < 4; k++) s.backclr[k] = back_[k];
-    for (int k = 0; k < 16; k++) s.invxform[k] = s.invxrot[k] = (k % 5 == 0) ? 1.0f : 0.0f;
+    for (int k = 0; k < 16; k++) {
+        s.invxform[k] = xf_inv_[k];
+        s.invxrot[k] = xf_invrot_[k];
+    }
+    // the light from application space into index space, as a position
+    // (PrepareRender, gvdb_volume_gvdb.cpp:4271-4275)
+    const float* m = xf_inv_;
+    for (int a = 0; a < 3; a++)
+        s.light_pos[a] = light_[0] * m[a] + light_[1] * m[4 + a] + light_[2] * m[8 + a] + m[12 + a];
+}
+
+// SetTransform: xform = T R S PT with R = Rz Ry Rx (angles in degrees about
+// x, then y, then z, right-handed), S = diag(scale), PT a translation by
+// pretrans applied first.  Its inverse PT^-1 S^-1 R^T T^-1 and the inverse of
+// the linear part, S^-1 R^T, are formed in double and rounded to float (the
+// reference inverts the float matrix with a general 4x4 inverse in double).
+int Volume::set_transform(const float pretrans[3], const float scale[3], const float angs[3], const float trans[3]) {
+    for (int a = 0; a < 3; a++)
+        if (!(scale[a] != 0.0f) || !std::isfinite(scale[a]) || !std::isfinite(angs[a]) || !std::isfinite(trans[a]) ||
+            !std::isfinite(pretrans[a]))
+            return fail(GVDB_ERR_INVALID, "SetTransform: finite values and nonzero scales");
+    for (int a = 0; a < 3; a++) {
+        xf_pretrans_[a] = pretrans[a];
+        xf_scale_[a] = scale[a];
+        xf_angs_[a] = angs[a];
+        xf_trans_[a] = trans[a];
+    }
+    const double d2r = 3.14159265358979323846 / 180.0;
+    const double cx = std::cos(angs[0] * d2r), sx = std::sin(angs[0] * d2r), cy = std::cos(angs[1] * d2r),
+                 sy = std::sin(angs[1] * d2r), cz = std::cos(angs[2] * d2r), sz = std::sin(angs[2] * d2r);
+    // R = Rz Ry Rx, row-major here: R[r][c]
+    const double Rx[3][3] = {{1, 0, 0}, {0, cx, -sx}, {0, sx, cx}};
+    const double Ry[3][3] = {{cy, 0, sy}, {0, 1, 0}, {-sy, 0, cy}};
+    const double Rz[3][3] = {{cz, -sz, 0}, {sz, cz, 0}, {0, 0, 1}};
+    double RyRx[3][3], R[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            RyRx[r][c] = 0;
+            for (int k = 0; k < 3; k++) RyRx[r][c] += Ry[r][k] * Rx[k][c];
+        }
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            R[r][c] = 0;
+            for (int k = 0; k < 3; k++) R[r][c] += Rz[r][k] * RyRx[k][c];
+        }
+    // L = S^-1 R^T (the inverse of R S), then inv = L (x - t) - pt
+    double L[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) L[r][c] = R[c][r] / double(scale[r]);
+    double lt[3];
+    for (int r = 0; r < 3; r++) lt[r] = L[r][0] * trans[0] + L[r][1] * trans[1] + L[r][2] * trans[2];
+    for (int k = 0; k < 16; k++) xf_inv_[k] = xf_invrot_[k] = (k % 5 == 0) ? 1.0f : 0.0f;
+    for (int r = 0; r < 3; r++) {
+        for (int c = 0; c < 3; c++) xf_inv_[c * 4 + r] = xf_invrot_[c * 4 + r] = float(L[r][c]);
+        xf_inv_[12 + r] = float(-lt[r] - double(pretrans[r]));
+    }
+    return GVDB_OK;
 }
 
 }  // namespace gvdb

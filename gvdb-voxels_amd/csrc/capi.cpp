@@ -175,6 +175,12 @@ int gvdb_load_vbx(gvdb_handle h, const char* path, int force_major, int force_mi
     H_OR_FAIL(h); NOT_RECORDING(h, "LoadVBX");
     return h->v.load_vbx(path, force_major, force_minor);
 }
+int gvdb_set_transform(gvdb_handle h, const float pretrans[3], const float scale[3], const float angs[3],
+                       const float trans[3]) {
+    H_OR_FAIL(h);
+    if (!pretrans || !scale || !angs || !trans) return GVDB_ERR_INVALID;
+    return h->v.set_transform(pretrans, scale, angs, trans);
+}
 int gvdb_get_transform(gvdb_handle h, float out12[12]) {
     H_OR_FAIL(h);
     if (!out12) return GVDB_ERR_INVALID;

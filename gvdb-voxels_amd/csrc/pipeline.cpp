@@ -1105,8 +1105,6 @@ int Volume::multi_render(Volume* const* vs, int G, const int* shading, const int
         Volume* v = vs[g];
         const int b = buf[g], c = chan[g];
         if (b < 0 || b >= 4 || v->rbuf_[b].w == 0) FAILG(g, v->fail(GVDB_ERR_INVALID, "Render: no such render buffer"));
-        if (!v->identity_transform())
-            FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, "Render: only the identity grid transform is supported"));
         if (c < 0 || c >= 8 || !v->chan_[c].exists || v->chan_[c].type == T_UCHAR4)
             FAILG(g, v->fail(GVDB_ERR_INVALID, "Render: no such float or half channel"));
         if (shading[g] != SHADE_VOLUME && shading[g] != SHADE_TRILINEAR)

@@ -241,11 +241,12 @@ int Volume::load_vbx(const char* path, int force_major, int force_minor) {
         host_valid_ = true;
         dev_stale_ = true;
         compute_top_lev();
-        for (int a = 0; a < 3; a++) {
-            xf_pretrans_[a] = xf[a];
-            xf_angs_[a] = xf[3 + a];
-            xf_scale_[a] = xf[6 + a];
-            xf_trans_[a] = xf[9 + a];
+        {
+            // the file's grid transform, as SetTransform sets it
+            const float pt[3] = {xf[0], xf[1], xf[2]}, an[3] = {xf[3], xf[4], xf[5]}, sc[3] = {xf[6], xf[7], xf[8]},
+                        tr[3] = {xf[9], xf[10], xf[11]};
+            rc = set_transform(pt, sc, an, tr);
+            if (rc) return rc;
         }
         rc = finish_topology(true, true);
         if (rc) return rc;

@@ -115,6 +115,10 @@ public:
     int upload_view(int chan, int rbuf, void** dev);
     void get_transform(float out12[12]) const;
     bool identity_transform() const;
+    // SetTransform (gvdb_volume_gvdb.cpp:5770-5793): the grid's placement in
+    // the application's space, xform = T R S PT; the march takes rays and the
+    // light back into index space through its inverse
+    int set_transform(const float pretrans[3], const float scale[3], const float angs[3], const float trans[3]);
     int rebuild_topology_cpu(int n, const float orig[3], const float* host);
     int activate_space(const float pos[3]);
     int clear();
@@ -225,6 +229,9 @@ private:
     float posMin_[3] = {0, 0, 0}, posMax_[3] = {0, 0, 0};
     // grid transform kept for VBX files (SetTransform's defaults)
     float xf_pretrans_[3] = {0, 0, 0}, xf_angs_[3] = {0, 0, 0}, xf_scale_[3] = {1, 1, 1}, xf_trans_[3] = {0, 0, 0};
+    // column-major 4x4: inverse of xform, and inverse of its linear part (R S)
+    float xf_inv_[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    float xf_invrot_[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
     float objMin_[3] = {0, 0, 0}, objMax_[3] = {0, 0, 0};
     float epsilon_ = 0.001f;
     int maxiter_ = 256;

@@ -48,7 +48,7 @@ _SIGS = [
     ("gvdb_rebuild_topology_cpu", _i, [_vp, _i, _vp, _vp]), ("gvdb_activate_space", _i, [_vp, _vp]),
     ("gvdb_accumulate_topology", _i, [_vp, _i, _f, _vp, _i]),
     ("gvdb_save_vbx", _i, [_vp, ctypes.c_char_p]), ("gvdb_load_vbx", _i, [_vp, ctypes.c_char_p, _i, _i]),
-    ("gvdb_get_transform", _i, [_vp, _vp]),
+    ("gvdb_get_transform", _i, [_vp, _vp]), ("gvdb_set_transform", _i, [_vp, _vp, _vp, _vp, _vp]),
     ("gvdb_compute", _i, [_vp, _i, _i, _i, _vp, _i, _i, _f]),
     ("gvdb_compute_kernel", _i, [_vp, _vp, _i, _i, _i]), ("gvdb_render_kernel", _i, [_vp, _vp, _i, _i]),
     ("gvdb_module_load", _i, [ctypes.c_char_p, _vp]), ("gvdb_module_function", _i, [_vp, ctypes.c_char_p, _vp]),
@@ -305,6 +305,11 @@ class VolumeGVDB:
 
     def LoadVBX(self, path, force_major=0, force_minor=0):
         return self._chk(load().gvdb_load_vbx(self.h, os.fsencode(path), int(force_major), int(force_minor)))
+
+    def SetTransform(self, pretrans, scale, angs, trans):
+        """SetTransform: the grid placed in application space as T R S PT."""
+        return self._chk(load().gvdb_set_transform(self.h, _p(_f3(pretrans)), _p(_f3(scale)), _p(_f3(angs)),
+                                                   _p(_f3(trans))))
 
     def transform(self):
         """(pretrans, angs, scale, trans) of the grid, as a VBX file stores them."""

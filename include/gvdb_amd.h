@@ -168,9 +168,13 @@ int gvdb_save_vbx(gvdb_handle h, const char* path);
  * channels, UpdateAtlas.  Files with bitmask nodes (1.0, or 2.x with masks) and non-float grids return
  * GVDB_ERR_UNSUPPORTED; unreadable or malformed files GVDB_ERR_IO. */
 int gvdb_load_vbx(gvdb_handle h, const char* path, int force_major, int force_minor);
-/* The grid transform a VBX file carries: pretrans, angs, scale, trans (SetTransform's fields).  Render supports only
- * the identity transform and returns GVDB_ERR_UNSUPPORTED otherwise. */
+/* The grid transform (SetTransform's fields, as a VBX file carries them): pretrans, angs, scale, trans. */
 int gvdb_get_transform(gvdb_handle h, float out12[12]);
+/* SetTransform(pretrans, scale, angs, trans): gvdb_volume_gvdb.cpp:5770-5793; places the grid in application space as
+ * T R S PT (R = Rz Ry Rx, angles in degrees); Render then takes the camera rays and the light into index space
+ * through its inverse. */
+int gvdb_set_transform(gvdb_handle h, const float pretrans[3], const float scale[3], const float angs[3],
+                       const float trans[3]);
 /* UpdateApron(chan, boundval): gvdb_volume_gvdb.cpp:4418-4457 */
 int gvdb_update_apron(gvdb_handle h, int chan, float boundval);
 /* UpdateApron without allocation.  No counterpart in the reference: an addition for callers that cannot allocate

@@ -84,6 +84,11 @@ public:
     }
     void SetPointsHost(const float* xyz, int n) { check(gvdb_upload_points(h_, xyz, n), "SetPoints"); }
     void SetColorChannel(unsigned char chan) { check(gvdb_set_color_channel(h_, chan), "SetColorChannel"); }
+    void SetTransform(Vector3DF pretrans, Vector3DF scal, Vector3DF angs, Vector3DF trans) {
+        const float p[3] = {pretrans.x, pretrans.y, pretrans.z}, s[3] = {scal.x, scal.y, scal.z},
+                    a[3] = {angs.x, angs.y, angs.z}, t[3] = {trans.x, trans.y, trans.z};
+        check(gvdb_set_transform(h_, p, s, a, t), "SetTransform");
+    }
 
     void RequestFullRebuild(bool on) { check(gvdb_request_full_rebuild(h_, on ? 1 : 0), "RequestFullRebuild"); }
     void RebuildTopology(int num_pnts, float radius, Vector3DF orig) {

@@ -3,7 +3,8 @@
 the file the format restatement (oracle/vbx.py) writes from the same pools
 and atlas; LoadVBX restores pools, atlas and rendered pixels exactly, also for
 a tree with leaves the incremental build deactivated; malformed files fail
-with GVDB_ERR_IO and bitmask files with GVDB_ERR_UNSUPPORTED."""
+with GVDB_ERR_IO and bitmask files with GVDB_ERR_UNSUPPORTED; a file's
+grid transform renders as SetTransform's."""
 import struct
 
 import numpy as np
@@ -140,15 +141,17 @@ def test_load_vbx_errors(gpu, tmp_path):
     with pytest.raises(GVDBError) as e:
         w.LoadVBX(str(tmp_path / "masked.vbx"))
     assert e.value.code == -7
-    # a non-identity transform loads, and Render refuses it
+    # a file's non-identity transform is the grid's: it renders as the same
+    # grid given that transform with SetTransform
     xf = bytearray(good)
     struct.pack_into("<3f", xf, 2 + 24, 2.0, 2.0, 2.0)
     (tmp_path / "scaled.vbx").write_bytes(bytes(xf))
     w.LoadVBX(str(tmp_path / "scaled.vbx"))
     assert np.array_equal(w.transform()[2], [2, 2, 2])
-    with pytest.raises(GVDBError) as e:
-        _render(w)
-    assert e.value.code == -7
+    v.SetTransform((0, 0, 0), (2, 2, 2), (0, 0, 0), (0, 0, 0))
+    a, b = _render(w), _render(v)
+    assert np.array_equal(a, b)
+    v.SetTransform((0, 0, 0), (1, 1, 1), (0, 0, 0), (0, 0, 0))
     assert vbx.read_vbx(good)["end"] == len(good)
     # references outside the pools are refused before any kernel walks them:
     # a child slot naming a leaf past the leaf pool, a child list id past the
