@@ -4,8 +4,9 @@ The reference ships no tests, golden vectors or fixtures for this path, so the
 oracle is pinned by what the mathematics fixes: set-based topology models on
 small inputs, the closed-form atlas layout, per-point subcell binning, an FP64
 brute-force density field, brute-force apron lookups, and closed-form renders
-(missed rays, a uniform-density emission-absorption slab, an isosurface on an
-axis through one point, the camera's projection).
+(missed rays, a uniform-density emission-absorption slab, runs of bricks
+separated by empty nodes, a row longer than the walk's iteration cap, an
+isosurface on an axis through one point, the camera's projection).
 """
 import math
 
@@ -431,6 +432,110 @@ def test_emission_absorption_uniform_slab():
         want = np.concatenate([back + (1 - T) * (clr - back), [1 - T]])
         ok.append(np.abs(img - want).max())
     assert min(ok) < 1e-6, "pixel %s, closest closed form off by %.3g" % (img, min(ok))   # float RGBA output
+
+
+def _brick_row(bricks):
+    """One point at the centre of each listed brick of the row y, z in [0, 8):
+    radius 0.5 activates exactly those bricks."""
+    return np.array([[8.0 * k + 4.0, 4.0, 4.0] for k in bricks], np.float32)
+
+
+def _ea_along_x(o, c, extinct, step):
+    """A 1x1 render of the uniform value c down +x along the row y = z ~ 4
+    (the ray is 0.01 degree off the axis, see the slab test), entering the
+    volume at x = 0.  Returns the pixel and the closed-form pixel of n samples."""
+    o.fill_channel(c)
+    sc = _scene(1, 1, (95.0, 4.2, 4.2), (-89.99, 0.01, 0.0), 100.17, extinct=(extinct, 0.9, 0.0),
+                steps=(step, 16.0, 0.1), vrange=(0.0, 0.0, 2.0))
+    col = np.array([1.0, 0.5, 0.25, 1.0], np.float32)
+    tf = O.linear_tf(0.0, 1.0, (0, 0, 0, 0), col)
+    img = o.render(O.scene_block(sc), tf, 7, 64)[0, 0].astype(np.float64)
+    tcol = tf[int(np.float64(c / 2.0) * 16300)].astype(np.float64)
+    e = math.exp(extinct * tcol[3] * step)
+    back = np.array(sc["backclr"][:3])
+
+    def want(n):
+        T = e ** n
+        clr = np.minimum(tcol[:3] * 0.9 * (1 - T), 1.0)
+        return np.concatenate([back + (1 - T) * (clr - back), [1 - T]])
+    return img, want
+
+
+def _closest_n(img, want, lo, hi):
+    errs = {n: np.abs(img - want(n)).max() for n in range(lo, hi + 1)}
+    n = min(errs, key=errs.get)
+    return n, errs[n]
+
+
+def test_emission_absorption_skips_empty_nodes():
+    """Three 8-brick runs of the row: level-1 nodes 0 and 3 (nodes 1, 2
+    empty) and level-1 node 16, in the third level-2 node (the second is
+    empty).  The walk steps over the empty nodes at levels 1 and 2 and
+    samples each run as the uniform slab: 3 * 64 / step samples, up to one
+    at either end of each run (rayCast, kernels/cuda_gvdb_raycast.cuh:
+    567-611).  A walk that loses a run after an empty node, or enters a run
+    twice, is off by 128 samples or more."""
+    runs = [range(0, 8), range(24, 32), range(128, 136)]
+    o = O.Oracle()
+    assert o.rebuild_topology(_brick_row([k for r in runs for k in r]), 0.5) == 0
+    o.finish_topology()
+    o.update_atlas()
+    assert len(o.pool(0, 0)) == 24 and len(o.pool(0, 1)) == 3
+    step = 0.5
+    img, want = _ea_along_x(o, 0.8, -0.05, step)
+    n0 = int(3 * 64 / step)
+    n, err = _closest_n(img, want, n0 - 200, n0 + 200)
+    assert abs(n - n0) <= 3 and err < 1e-6, (n, err)
+
+
+def _bricks_before_cap(top, per_entry):
+    """Bricks a walk down the row samples before its 256th iteration when it
+    spends top - 1 iterations descending from the root to the first brick,
+    per_entry more on every level-1 or level-2 node it enters after that,
+    and one per brick."""
+    it, K = top - 1, 0
+    while True:
+        if K and K % 8 == 0:
+            it += per_entry             # a new level-1 node
+            if K % 64 == 0 and top > 2:
+                it += per_entry         # a new level-2 node
+        if it >= 256:
+            return K
+        it += 1
+        K += 1
+
+
+def test_emission_absorption_stops_at_max_iter():
+    """A row of 300 bricks (2400 voxels) is longer than the walk's cap of
+    256 iterations (kernels/cuda_gvdb_raycast.cuh:24, :567).  An iteration
+    either descends into an occupied node or samples one brick; entering a
+    node takes one descent, and may take a second where the walk, back at
+    the parent level on the boundary, finds the node it just left.  So the
+    bricks sampled before the cap lie between the counts with one and with
+    two iterations per node entered, and the pixel is the uniform slab of a
+    whole number of them (16 samples per brick at step 0.5, up to one at
+    either end).  A row of 200 bricks is sampled whole.  No cap, a cap on
+    samples or a cap on bricks alone falls outside these bounds."""
+    step = 0.5
+    for L, capped in ((200, False), (300, True)):
+        o = O.Oracle()
+        assert o.rebuild_topology(_brick_row(range(L)), 0.5) == 0
+        o.finish_topology()
+        o.update_atlas()
+        assert len(o.pool(0, 0)) == L
+        top = o.top_lev()
+        hi, lo = _bricks_before_cap(top, 1), _bricks_before_cap(top, 2)
+        img, want = _ea_along_x(o, 0.8, -0.0008, step)
+        n, err = _closest_n(img, want, 0, int(8 * L / step) + 10)
+        K = round(n * step / 8)
+        assert abs(n - 8 * K / step) <= 1 and err < 1e-6, (n, err)
+        if capped:
+            assert hi < L and lo <= K <= hi, (lo, K, hi)
+        else:
+            assert K == L
+        _, cap = o.render(O.scene_block(_scene(1, 1, (95.0, 4.2, 4.2), (-89.99, 0.01, 0.0), 100.17)),
+                          O.linear_tf(0.0, 1.0, (0, 0, 0, 0), (1, 1, 1, 1)), 7, 64, capped=True)
+        assert bool(cap[0, 0]) == capped
 
 
 def test_isosurface_on_axis_through_one_point():
