@@ -39,6 +39,9 @@ struct FillArgs {
 hipError_t launch_fill_multi(const FillArgs* a, int G, Arena* ar, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, u64 n, hipStream_t s);
 hipError_t launch_fill_u64(u64* p, u64 v, u64 n, hipStream_t s);
+// *out = the sum of in[0, n) in 64 bits (the exact total behind a u32 scan,
+// whose own total wraps past 2^32)
+hipError_t launch_sum_u32(const uint32_t* in, u64 n, u64* out, hipStream_t s);
 
 // Exclusive scan of u32 values; `mode` selects what is scanned:
 //   0: in0[i];  1: popcount(in0[i]);  2: popcount(in1[i] & ~in0[i]);  3: in0[i] != 0

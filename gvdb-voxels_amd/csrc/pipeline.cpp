@@ -20,6 +20,14 @@ namespace gvdb {
         if (e_ != hipSuccess) return (v)->hip_fail(e_, #x);          \
     } while (0)
 
+// Point radius: the dilation marks every brick of [int(p) - r, int(p) + 2r]
+// and the insert every subcell of [int(p) - r, int(p) + 1 + r] per point, one
+// thread per point, so the radius bounds one thread's loop: at 1024 voxels
+// that is 5.7e7 bricks or, with 4-voxel subcells, 1.3e8 subcells.  Totals
+// beyond the tables' ranges (2^34 domain bricks, 2^31 entries) are refused
+// where they are counted.
+static inline bool radius_ok(float r) { return r >= 0.0f && r <= 1024.0f; }
+
 static inline float key_float(uint32_t k) {
     uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
     float f;
@@ -51,8 +59,8 @@ int Volume::multi_rebuild_topology(Volume* const* vs, int G, const int* n, const
         if (v->levels_ == 0) FAILG(g, v->fail(GVDB_ERR_STATE, "RebuildTopology before Configure"));
         if (n[g] <= 0 || !v->pts_.base || n[g] > v->pts_.n)
             FAILG(g, v->fail(GVDB_ERR_INVALID, "RebuildTopology: point count exceeds SetPoints"));
-        if (!(radius[g] >= 0.0f) || radius[g] > 32.0f)
-            FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, "RebuildTopology: radius must be in [0, 32]"));
+        if (!radius_ok(radius[g]))
+            FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, "RebuildTopology: radius must be in [0, 1024]"));
     }
     Volume* v0 = vs[0];
     if (G == 1) v0->stage_begin(ST_TOPO);
@@ -346,7 +354,7 @@ int Volume::accumulate_topology(int n, float radius, const float orig[3], int mi
     if (capture_) return not_recordable("AccumulateTopology");
     if (levels_ == 0) return fail(GVDB_ERR_STATE, "AccumulateTopology before Configure");
     if (n <= 0 || !pts_.base || n > pts_.n) return fail(GVDB_ERR_INVALID, "AccumulateTopology: point count exceeds SetPoints");
-    if (!(radius >= 0.0f) || radius > 32.0f) return fail(GVDB_ERR_UNSUPPORTED, "AccumulateTopology: radius must be in [0, 32]");
+    if (!radius_ok(radius)) return fail(GVDB_ERR_UNSUPPORTED, "AccumulateTopology: radius must be in [0, 1024]");
     if (rebuild_topo_) return rebuild_topology(n, radius, orig);
     const Exec ex = self_exec();
     stage_begin(ST_TOPO);
@@ -645,8 +653,8 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         if (v->clr_.base && n[g] > v->clr_.n)
             FAILG(g, v->fail(GVDB_ERR_INVALID, "InsertPointsSubcell: point count exceeds SetPoints' colours"));
         v->bkt_clr_ = false;
-        if (!(radius[g] >= 0.0f) || radius[g] > 32.0f)
-            FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, "InsertPointsSubcell: radius must be in [0, 32]"));
+        if (!radius_ok(radius[g]))
+            FAILG(g, v->fail(GVDB_ERR_UNSUPPORTED, "InsertPointsSubcell: radius must be in [0, 1024]"));
         if (v->atlas_gen_ != v->topo_gen_)
             FAILG(g, v->fail(GVDB_ERR_STATE, "InsertPointsSubcell: call UpdateAtlas after the topology changes"));
         int rc = v->commit_pools();
@@ -735,6 +743,8 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
                 // push path: the reference's count pass with global atomics
                 XCHK(v, launch_count_subcell(pl.t, pl.P, scs, pl.spb, tr, radius[g], v->sc_map_.as<uint32_t>(),
                                              v->sc_cnt_.as<uint32_t>(), rule, ex.s));
+                XCHK(v, launch_sum_u32(v->sc_cnt_.as<uint32_t>(), pl.numSC, reinterpret_cast<u64*>(small(g, SM_LEN)),
+                                       ex.s));
                 sc.push_back(ScanArgs{v->sc_cnt_.as<uint32_t>(), nullptr, 0, v->sc_off_.as<uint32_t>(), pl.numSC,
                                       tmp2(g), small(g, SM_TOTAL)});
                 continue;
@@ -790,12 +800,11 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         Plan& pl = plan[g];
         pl.linked = ex.hsmall[g * SMALL_WORDS + SM_LINKED];
         pl.orphans = pl.pull ? ex.hsmall[g * SMALL_WORDS + SM_ORPHANS] : 0;
-        if (pl.pull) {
+        {
+            // the exact entry total in 64 bits (the u32 scan's wraps)
             u64 l = 0;
             memcpy(&l, ex.hsmall + g * SMALL_WORDS + SM_LEN, 8);
             pl.total = l > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(l);
-        } else {
-            pl.total = ex.hsmall[g * SMALL_WORDS + SM_TOTAL];
         }
         if (pl.pull && pl.orphans) {
             Volume* v = vs[g];
@@ -804,6 +813,8 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
             XCHK(v, launch_fill_u32(v->sc_cnt_.as<uint32_t>(), 0u, pl.numSC, ex.s));
             XCHK(v, launch_count_subcell(pl.t, pl.P, scs, pl.spb, trans3 + 3 * g, radius[g], v->sc_map_.as<uint32_t>(),
                                          v->sc_cnt_.as<uint32_t>(), rule, ex.s));
+            XCHK(v, launch_sum_u32(v->sc_cnt_.as<uint32_t>(), pl.numSC, reinterpret_cast<u64*>(small(g, SM_LEN)),
+                                   ex.s));
             XCHK(v, launch_scan_u32(v->sc_cnt_.as<uint32_t>(), nullptr, 0, v->sc_off_.as<uint32_t>(), pl.numSC,
                                     small(g, SM_TOTAL), tmp2(g), ex.s));
         }
@@ -812,7 +823,11 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         rc = drain(v0, ex, G);
         if (rc) return rc;
         for (int g = 0; g < G; g++)
-            if (!plan[g].pull) plan[g].total = ex.hsmall[g * SMALL_WORDS + SM_TOTAL];
+            if (!plan[g].pull) {
+                u64 l = 0;
+                memcpy(&l, ex.hsmall + g * SMALL_WORDS + SM_LEN, 8);
+                plan[g].total = l > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(l);
+            }
     }
     std::vector<SubPosArgs> sp;
     for (int g = 0; g < G; g++) {

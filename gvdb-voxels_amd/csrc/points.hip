@@ -125,8 +125,15 @@ __global__ __launch_bounds__(256) void k_bin_subcell(Tree t, PointSrc pts, int s
                     cache_leaf = leaf;
                     cx = ln->mPos[0]; cy = ln->mPos[1]; cz = ln->mPos[2];
                 }
+                // A subcell corner less than one cell below a node's corner
+                // truncates into the node's first cell (the reference's walk),
+                // and the local offset goes negative: the entry lands in a
+                // neighbouring subcell, or before the first linked brick's
+                // subcells, which no table holds (skipped here).
                 int lx = (sx - cx) / scs, ly = (sy - cy) / scs, lz = (sz - cz) / scs;
-                u64 idx = u64(mapping[leaf]) * spb + u64((lz * scdim + ly) * scdim + lx);
+                const long long sidx = (long long)mapping[leaf] * spb + (lz * scdim + ly) * scdim + lx;
+                if (sidx < 0) continue;
+                const u64 idx = u64(sidx);
                 uint32_t slot = atomicAdd(&sc_cnt[idx], 1u);
                 if (INSERT) {
                     u64 e = u64(sc_off[idx]) + slot;
@@ -587,10 +594,14 @@ __device__ inline void k_bucket_points_body(const BucketArgs& a, uint32_t bid, u
             const int ssh = __builtin_ctz(unsigned(scs));
             int ox = (int(w[0]) >> ssh) << ssh, oy = (int(w[1]) >> ssh) << ssh, oz = (int(w[2]) >> ssh) << ssh;
             int leaf = pos_leaf(t, ox, oy, oz);
-            if (leaf < 0) {
+            const Node* n = leaf < 0 ? nullptr : t.nodes[0] + leaf;
+            // a subcell below the found brick's corner (the walk truncates
+            // positions less than a cell below a node into its first cell) has
+            // no bucket: such points go to the push path with the orphans
+            if (n && (ox < n->mPos[0] || oy < n->mPos[1] || oz < n->mPos[2])) n = nullptr;
+            if (!n) {
                 atomicAdd(orphans, 1u);
             } else {
-                const Node* n = t.nodes[0] + leaf;
                 const int res0 = t.res[0];
                 int sd = res0 >> ssh;
                 int lx = (ox - n->mPos[0]) >> ssh, ly = (oy - n->mPos[1]) >> ssh, lz = (oz - n->mPos[2]) >> ssh;
@@ -1369,7 +1380,10 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
         __syncthreads();
     }
     __syncthreads();
-    AT* atlas = reinterpret_cast<AT*>(a.atlas);
+    // the brick's atlas slot found once (its divisions by the padded side are
+    // per brick, not per voxel); a voxel adds its padded-local index
+    AT* atlas = reinterpret_cast<AT*>(a.atlas) + slot_of(t, bn->mValue) * u64(t.brick_vox);
+    const int pbr = t.brick_res, ap = t.apron;
     AT* F = a.faces ? reinterpret_cast<AT*>(a.faces) + u64(brick) * FACE_VALS : nullptr;
     for (int v = tid; v < 512; v += NT) {
         const int lx = v & 7, ly = (v >> 3) & 7, lz = v >> 6;
@@ -1377,7 +1391,7 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
         float val = 0.0f;
         if (k != ~0ull)
             val = exact ? fmaxf(0.0f, falloff_from_d2(k, sh, r)) : __uint_as_float(0x7F800000u - uint32_t(k));
-        store_atlas<AT>(atlas, atlas_offset(t, bn->mValue[0] + lx, bn->mValue[1] + ly, bn->mValue[2] + lz), val);
+        store_atlas<AT>(atlas, u64(((lz + ap) * pbr + ly + ap) * pbr + lx + ap), val);
         if (F) {
             // the brick's boundary layers, compact, for the apron pass
             if (lx == 0) store_atlas<AT>(F, 0 * 64 + lz * 8 + ly, val);
@@ -1556,12 +1570,12 @@ __device__ inline void k_gather_color_body(const ColorArgs& a, uint32_t bid, uin
                 }
     }
     __syncthreads();
-    uint32_t* atlas = reinterpret_cast<uint32_t*>(a.atlas);
-    const int w = t.brickwid;
+    uint32_t* atlas = reinterpret_cast<uint32_t*>(a.atlas) + slot_of(t, bn->mValue) * u64(t.brick_vox);
+    const int w = t.brickwid, pbr = t.brick_res, ap = t.apron;
     for (int v = tid; v < w * w * w; v += CG_THREADS) {
         const int lx = v % w, ly = (v / w) % w, lz = v / (w * w);
         const uint32_t* sm = S.sum[((lz / sc) * sd + ly / sc) * sd + lx / sc];
-        const u64 off = atlas_offset(t, bn->mValue[0] + lx, bn->mValue[1] + ly, bn->mValue[2] + lz);
+        const u64 off = u64(((lz + ap) * pbr + ly + ap) * pbr + lx + ap);
         const uint32_t n = sm[4];
         uint32_t out = 0;
         if (a.accumulate) {

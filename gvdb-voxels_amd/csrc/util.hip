@@ -2,6 +2,7 @@
 // prefixSum/prefixFixup kernels, kernels/cuda_gvdb_particles.cuh:856-902),
 // and the staging of argument tables for launches over many grids.
 #include "kernels.h"
+#include <algorithm>
 #include <cstring>
 
 namespace gvdb {
@@ -121,6 +122,26 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, u64 n, hipStream_t s) {
 hipError_t launch_fill_u64(u64* p, u64 v, u64 n, hipStream_t s) {
     FillArgs a{p, n, v, 1};
     return n ? launch_fill_multi(&a, 1, nullptr, s) : hipSuccess;
+}
+
+// ------------------------------------------------------------ 64-bit sum
+__global__ __launch_bounds__(256) void k_sum_u32(const uint32_t* in, u64 n, u64* out) {
+    __shared__ u64 part[4];
+    u64 acc = 0;
+    for (u64 i = u64(blockIdx.x) * 256 + threadIdx.x; i < n; i += u64(gridDim.x) * 256) acc += in[i];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_down(acc, d, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long*>(out), part[0] + part[1] + part[2] + part[3]);
+}
+
+hipError_t launch_sum_u32(const uint32_t* in, u64 n, u64* out, hipStream_t s) {
+    hipError_t e = launch_fill_u64(out, 0, 1, s);
+    if (e != hipSuccess || n == 0) return e;
+    const u64 blocks = std::min<u64>((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_sum_u32, dim3(unsigned(blocks)), dim3(256), 0, s, in, n, out);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------- scan

@@ -894,7 +894,12 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
                     Node* pn = nodeAt(v, 0, u64(pid));
                     int pin[3] = {(sp[0] - pn->mPos[0]) / scs, (sp[1] - pn->mPos[1]) / scs, (sp[2] - pn->mPos[2]) / scs};
                     int lo = (pin[2] * scdim + pin[1]) * scdim + pin[0];
-                    fn(v.sc_mapping[pid] * spb + lo);
+                    // a corner less than a cell below the brick's (truncated
+                    // into its first cell) gives a negative offset: before the
+                    // first linked brick that indexes outside sc_cnt
+                    const i64 idx = i64(v.sc_mapping[pid]) * spb + lo;
+                    if (idx < 0) { v.undefined_events++; continue; }
+                    fn(int(idx));
                 }
     };
     // The reference's count pass tests the position after adding trans, its
