@@ -21,6 +21,8 @@ import time
 
 import numpy as np
 
+T_START = time.time()
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gvdb-voxels_amd"))
 
@@ -416,6 +418,11 @@ def stream_copy_gbs(mib=2048, reps=10):
     return 2.0 * n / (float(np.median(best)) * 1e-3) / 1e9
 
 
+def progress(msg):
+    """One line per finished part on stderr (the JSON result goes to stdout)."""
+    print("[bench %.0fs] %s" % (time.time() - T_START, msg), file=sys.stderr, flush=True)
+
+
 def copy_calibration(mbytes):
     """One device-to-device hipMemcpy of `mbytes` MiB (a blit kernel moving a
     known number of bytes), for scaling the FETCH_SIZE / WRITE_SIZE counters
@@ -459,6 +466,7 @@ def main():
               "no multi-GPU form (--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     g = run_gpu(args)
+    progress("medium workload timed")
     ab = g["ab"]
     gather_gbs = ab["gather"] / (g["gather_ms"] * 1e-3) / 1e9
     apron_gbs = ab["apron"] / (g["apron_ms"] * 1e-3) / 1e9
@@ -487,17 +495,21 @@ def main():
     )
     if not args.no_stream:
         cp = stream_copy_gbs()
+        progress("copy bandwidth measured")
         line["copy_bandwidth"] = dict(achieved=cp, peak=HBM_PEAK_GBS, unit="GB/s", frac=cp / HBM_PEAK_GBS,
                                       kernel="device-to-device hipMemcpyAsync, 2 GiB, median of 10",
                                       gather_frac_of_copy=gather_gbs / cp, apron_frac_of_copy=apron_gbs / cp)
     if not args.no_batched:
         from gvdb_amd import workloads as W
         line["batched"] = run_batched(args, W.batch_grids())
+        progress("batched workload timed")
     if not args.no_serve:
         from gvdb_amd import workloads as W
         line["serving"] = run_serving(args, W.serve_grids(count=args.serve_grids))
+        progress("serving workload timed")
     if not args.no_cpu:
         c = run_cpu_baseline()
+        progress("CPU baseline timed")
         a = c["all_cores"]
         line["cpu_baseline"] = dict(value=a["points_per_s"], unit="points/s", cores=a["threads"], kind="port",
                                     sample=c["sample"], mrays_per_s=a["mrays_per_s"], splat_s=a["splat_s"],

@@ -16,7 +16,7 @@ LIB = os.path.join(LIBDIR, "libgvdb_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GVDB_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["util.hip", "topology.hip", "points.hip", "apron.hip", "render.hip", "compute.hip", "pipeline.cpp", "batch.cpp", "serve.cpp", "volume.cpp", "vbx.cpp", "plugin.cpp", "camera.cpp", "capi.cpp", "tune.cpp", "graph.cpp"]
+SOURCES = ["util.hip", "topology.hip", "points.hip", "apron.hip", "render.hip", "compute.hip", "pipeline.cpp", "batch.cpp", "serve.cpp", "volume.cpp", "vbx.cpp", "plugin.cpp", "camera.cpp", "capi.cpp", "tune.cpp", "graph.cpp", "keysort.hip", "keybuild.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-I" + SRC, "-I" + os.path.join(HERE, "..", "include")]
 

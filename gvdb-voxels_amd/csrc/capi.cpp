@@ -274,13 +274,13 @@ int gvdb_tune_gather(gvdb_handle h, int subcell_size, int n, float radius, const
     return gvdb::tune_gather(h->v, subcell_size, n, radius, trans ? trans : z, sc_len, chan, best_threads, ms2);
 }
 int gvdb_set_tuning(int knob, int value) { return gvdb::set_knob(knob, value) ? GVDB_OK : GVDB_ERR_INVALID; }
-int gvdb_get_tuning(int knob, int* value, int* choices4, int* nchoices) {
+int gvdb_get_tuning(int knob, int* value, int* choices8, int* nchoices) {
     if (knob < 0 || knob >= gvdb::KNOB_COUNT) return GVDB_ERR_INVALID;
     if (value) *value = gvdb::knob(knob);
-    int c[4];
+    int c[gvdb::KNOB_MAX_VALUES];
     const int n = gvdb::knob_values(knob, c);
-    if (choices4)
-        for (int i = 0; i < n; i++) choices4[i] = c[i];
+    if (choices8)
+        for (int i = 0; i < n; i++) choices8[i] = c[i];
     if (nchoices) *nchoices = n;
     return GVDB_OK;
 }

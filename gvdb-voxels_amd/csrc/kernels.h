@@ -124,6 +124,45 @@ struct LinkArgs {
 };
 hipError_t launch_link_level_multi(const LinkArgs* a, int G, Arena* ar, hipStream_t s);
 
+// ---- keyed topology build (keysort.hip)
+struct KeyGenArgs {
+    PointSrc pts;
+    float o[3];
+    float radius;
+    int dilate;
+    int range0;
+    float inv_range0;
+    BitDomain d0;
+    u64 sentinel;           // own key of a point outside the box (sorts last)
+    u64* keysA;             // per point: own brick's key or the sentinel
+    uint32_t* cntB;         // per point: bricks of its dilation box in the box
+    const uint32_t* offB;   // exclusive scan of cntB (emit pass)
+    u64* keysB;
+};
+hipError_t launch_key_gen(const KeyGenArgs& a, bool emit, hipStream_t s);
+u64 radix_hist_words(u64 n);
+// sorts n keys of `bits` significant bits (LSD, 8 bits a pass); *result is
+// keys or tmp, whichever holds the sorted keys
+hipError_t radix_sort_u64(u64* keys, u64* tmp, u64 n, int bits, uint32_t* hist, uint32_t* scan_tmp, hipStream_t s,
+                          u64** result);
+hipError_t launch_flag_unique(const u64* k, u64 n, u64 sentinel, uint32_t* flags, hipStream_t s);
+hipError_t launch_flag_absent(const u64* b, u64 nb, const u64* a, uint32_t na, uint32_t* flags, hipStream_t s);
+hipError_t launch_compact(const u64* k, u64 n, const uint32_t* flags, const uint32_t* pos, u64* out, hipStream_t s);
+hipError_t launch_parent_keys(const u64* c, u64 n, const BitDomain& dc, int ld, const BitDomain& dp, u64* out,
+                              hipStream_t s);
+struct KeyLevel {
+    BitDomain dom;
+    const u64* A;    // sorted keys of the own-brick cells
+    uint32_t na;
+    const u64* E;    // sorted keys of the dilation-only cells
+    uint32_t ne;
+    int lev;
+    int range;
+};
+hipError_t launch_emit_list(const KeyLevel& L, Node* nodes, int has_clist, hipStream_t s);
+hipError_t launch_link_list(const KeyLevel& C, const KeyLevel& P, int parent_is_root, int ld, Node* cnodes,
+                            u64* pclists, u64 clist_len, hipStream_t s);
+
 struct RootArgs {
     Node* root;
     int pos[3];
@@ -420,12 +459,14 @@ enum Knob {
     KNOB_APRON_RUN,              // bricks per workgroup of the multi-grid apron: 1, 2, 4, 8
     KNOB_XCD_ORDER,              // per-brick kernels: 0 launch order, 1 each XCD a contiguous range of bricks
     KNOB_APRON_QUEUE,            // multi-grid apron: 0 static runs, 1 persistent workgroups over a brick queue
-    KNOB_MARK_PATH,              // topology marks: 0 by domain size, 1 a byte per brick, 2 direction masks, 3 bits
+    KNOB_MARK_PATH,              // topology marks: 0 by domain size, 1 a byte per brick, 2 direction masks, 3 bits,
+                                 // 4 sorted brick keys (the build taken above 2^34 bricks)
     KNOB_COUNT
 };
 int knob(int k);
 bool set_knob(int k, int v);          // false (and unchanged) when v is not one of the knob's values
-int knob_values(int k, int out[4]);   // the knob's values; returns how many
+static const int KNOB_MAX_VALUES = 8;
+int knob_values(int k, int out[KNOB_MAX_VALUES]);   // the knob's values; returns how many
 class Volume;
 class Batch;
 // The autotuner (tune.cpp): times the caller's own workload per knob value

@@ -50,6 +50,8 @@ static int drain(Volume* v, const Exec& ex, int G) {
         return (code);               \
     } while (0)
 
+static bool keyed_build_wanted(const BitDomain& d0);
+
 // ------------------------------------------------------- RebuildTopology ----
 int Volume::multi_rebuild_topology(Volume* const* vs, int G, const int* n, const float* radius, const float* orig3,
                                    const Exec& ex, int* failed) {
@@ -117,6 +119,25 @@ int Volume::multi_rebuild_topology(Volume* const* vs, int G, const int* n, const
         rc = v->incremental_topology(n[g], radius[g], orig3 + 3 * g, depth[g], false, &mnmx[6 * g], eg);
         if (rc) FAILG(g, rc);
     }
+    // level-0 boxes too large for bitmaps (or the keyed path forced): the
+    // keyed build, one grid at a time
+    {
+        std::vector<int> bitmap_idx;
+        for (int g : full_idx) {
+            Volume* v = vs[g];
+            BitDomain dom[MAXLEV];
+            if (!plan_domains(v, depth[g], &root[3 * g], &mnmx[6 * g], &mnmx[6 * g + 3], radius[g], dom))
+                FAILG(g, v->fail(GVDB_ERR_RANGE, "RebuildTopology: brick coordinates exceed the 16-bit keys of the reference"));
+            if (!keyed_build_wanted(dom[0])) {
+                bitmap_idx.push_back(g);
+                continue;
+            }
+            const Exec eg{ex.s, ex.ar, ex.dsmall + g * SMALL_WORDS, ex.hsmall + g * SMALL_WORDS};
+            rc = v->keyed_build(n[g], radius[g], orig3 + 3 * g, depth[g], &root[3 * g], dom, eg);
+            if (rc) FAILG(g, rc);
+        }
+        full_idx.swap(bitmap_idx);
+    }
     if (!full_idx.empty()) {
         const int F = int(full_idx.size());
         std::vector<Volume*> fv(F);
@@ -141,6 +162,43 @@ int Volume::multi_rebuild_topology(Volume* const* vs, int G, const int* n, const
     if (G == 1) v0->stage_end(ST_TOPO);
     return GVDB_OK;
 }
+
+// The brick box of the points at each level below the root: the points'
+// bounds widened by the dilation ([int(p) - r, int(p) + 2r]) and one voxel,
+// clipped to the root node; level L holds the level-L cells of that box.
+// False when level-0 coordinates exceed the reference's 16-bit keys.
+bool Volume::plan_domains(const Volume* v, int D, const int rootPos[3], const float* mn, const float* mx, float radius,
+                  BitDomain* dom) {
+    const int rr = radius >= 1.0f ? int(ceilf(radius)) : 0;
+    long long lo_v[3], hi_v[3];
+    for (int a = 0; a < 3; a++) {
+        long long re = (long long)rootPos[a] + v->range_[D] - 1;
+        long long l = std::isfinite(mn[a]) ? (long long)floorf(std::max(mn[a], 0.0f)) - rr - 1 : 0;
+        long long h = std::isfinite(mx[a]) ? (long long)floorf(std::max(mx[a], 0.0f)) + 2 * rr + 1 : 0;
+        lo_v[a] = std::max<long long>(rootPos[a], l);
+        hi_v[a] = std::min<long long>(re, std::max<long long>(h, lo_v[a]));
+    }
+    if (hi_v[0] / v->range_[0] > 65535 || hi_v[1] / v->range_[0] > 65535 || hi_v[2] / v->range_[0] > 65535) return false;
+    for (int L = 0; L < D; L++) {
+        int shift = 0;
+        for (int k = 0; k <= L; k++) shift += v->logdim_[k];
+        BitDomain& d = dom[L];
+        d.nbits = 1;
+        for (int a = 0; a < 3; a++) {
+            d.lo[a] = int(lo_v[a] >> shift);
+            d.dim[a] = int((hi_v[a] >> shift) - d.lo[a] + 1);
+            d.nbits *= u64(d.dim[a]);
+        }
+        d.nwords = (d.nbits + 31) / 32;
+    }
+    return true;
+}
+
+// Level-0 boxes above this many bricks take the keyed build (bitmaps of
+// 2^34 bits are 2 GiB per level and set).
+static const u64 BITMAP_MAX_BITS = u64(1) << 34;
+
+static bool keyed_build_wanted(const BitDomain& d0) { return d0.nbits > BITMAP_MAX_BITS || knob(KNOB_MARK_PATH) == 4; }
 
 // The full build of RebuildTopology (ActivateBricksGPU + ActivateExtraBricksGPU)
 // for grids already cleared, with their depth, root corner and point bounds.
@@ -169,29 +227,11 @@ int Volume::multi_full_build(Volume* const* vs, int G, const int* n, const float
         pl.D = depth[g];
         pl.dilate = radius[g] >= 1.0f;
         const int D = pl.D;
-        int rr = pl.dilate ? int(ceilf(radius[g])) : 0;
-        long long lo_v[3], hi_v[3];
-        for (int a = 0; a < 3; a++) {
-            long long re = (long long)pl.rootPos[a] + v->range_[D] - 1;
-            long long l = std::isfinite(mn[a]) ? (long long)floorf(std::max(mn[a], 0.0f)) - rr - 1 : 0;
-            long long h = std::isfinite(mx[a]) ? (long long)floorf(std::max(mx[a], 0.0f)) + 2 * rr + 1 : 0;
-            lo_v[a] = std::max<long long>(pl.rootPos[a], l);
-            hi_v[a] = std::min<long long>(re, std::max<long long>(h, lo_v[a]));
-        }
-        if (hi_v[0] / v->range_[0] > 65535 || hi_v[1] / v->range_[0] > 65535 || hi_v[2] / v->range_[0] > 65535)
+        if (!plan_domains(v, D, pl.rootPos, mn, mx, radius[g], pl.dom))
             FAILG(g, v->fail(GVDB_ERR_RANGE, "RebuildTopology: brick coordinates exceed the 16-bit keys of the reference"));
         u64 bm_words = 0, rk_words = 0, tmp_words = 0;
         for (int L = 0; L < D; L++) {
-            int shift = 0;
-            for (int k = 0; k <= L; k++) shift += v->logdim_[k];
-            BitDomain& d = pl.dom[L];
-            d.nbits = 1;
-            for (int a = 0; a < 3; a++) {
-                d.lo[a] = int(lo_v[a] >> shift);
-                d.dim[a] = int((hi_v[a] >> shift) - d.lo[a] + 1);
-                d.nbits *= u64(d.dim[a]);
-            }
-            d.nwords = (d.nbits + 31) / 32;
+            const BitDomain& d = pl.dom[L];
             pl.bm_off[L] = bm_words;
             bm_words += d.nwords * (pl.dilate ? 2 : 1);
             pl.rk_off[L] = rk_words;
@@ -201,7 +241,7 @@ int Volume::multi_full_build(Volume* const* vs, int G, const int* n, const float
                 tmp_words += scan_tmp_words(d.nwords);
             }
         }
-        if (pl.dom[0].nbits > (u64(1) << 34))
+        if (pl.dom[0].nbits > BITMAP_MAX_BITS)
             FAILG(g, v->fail(GVDB_ERR_RANGE, "RebuildTopology: brick box of the points exceeds 2^34 bricks"));
         XCHK(v, v->bitmaps_.ensure(bm_words * 4 + 256));
         XCHK(v, v->ranks_.ensure(rk_words * 4 + 256));

@@ -17,15 +17,15 @@
 
 namespace gvdb {
 
-static const int kValues[KNOB_COUNT][4] = {
-    {1, 2, 4}, {1, 2, 4}, {8, 16, 32, 64}, {8, 16, 32, 64}, {128, 256}, {1, 2, 4, 8}, {0, 1}, {0, 1}, {0, 1, 2, 3}};
-static const int kCount[KNOB_COUNT] = {3, 3, 4, 4, 2, 4, 2, 2, 4};
+static const int kValues[KNOB_COUNT][KNOB_MAX_VALUES] = {
+    {1, 2, 4}, {1, 2, 4}, {8, 16, 32, 64}, {8, 16, 32, 64}, {128, 256}, {1, 2, 4, 8}, {0, 1}, {0, 1}, {0, 1, 2, 3, 4}};
+static const int kCount[KNOB_COUNT] = {3, 3, 4, 4, 2, 4, 2, 2, 5};
 // defaults: the values measured fastest on the medium and 64-grid workloads
 static int g_knob[KNOB_COUNT] = {4, 2, 8, 8, 128, 4, 0, 1, 0};
 
 int knob(int k) { return (k >= 0 && k < KNOB_COUNT) ? g_knob[k] : 0; }
 
-int knob_values(int k, int out[4]) {
+int knob_values(int k, int out[KNOB_MAX_VALUES]) {
     int n = 0;
     if (k < 0 || k >= KNOB_COUNT) return 0;
     for (int i = 0; i < kCount[k]; i++) out[n++] = kValues[k][i];
@@ -33,7 +33,7 @@ int knob_values(int k, int out[4]) {
 }
 
 bool set_knob(int k, int v) {
-    int vals[4];
+    int vals[KNOB_MAX_VALUES];
     const int n = knob_values(k, vals);
     for (int i = 0; i < n; i++)
         if (vals[i] == v) {
@@ -63,7 +63,7 @@ static int time_runs(F&& run, float& out) {
 // slowest.  On failure the knobs are restored.
 template <class F>
 static int tune_knobs(const int* ks, int nk, F&& run, int* best, float* ms) {
-    int vals[2][4], cnt[2] = {1, 1}, keep[2] = {0, 0};
+    int vals[2][KNOB_MAX_VALUES], cnt[2] = {1, 1}, keep[2] = {0, 0};
     for (int i = 0; i < nk; i++) {
         cnt[i] = knob_values(ks[i], vals[i]);
         keep[i] = g_knob[ks[i]];
@@ -175,7 +175,7 @@ int tune_batch(Batch& B, Volume* const* vs, int G, int stage, int scs, const int
             return rc;
         }
         if (ms) ms[4] = tq;
-        int vals[4];
+        int vals[KNOB_MAX_VALUES];
         const int n = knob_values(KNOB_APRON_RUN, vals);
         float tb = FLT_MAX;
         for (int i = 0; i < n && ms; i++)

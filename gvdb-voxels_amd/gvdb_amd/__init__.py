@@ -142,7 +142,7 @@ def _f3(x):
 def get_tuning(knob):
     """(current value, values it can take) of a launch-shape knob."""
     v, n = ctypes.c_int(0), ctypes.c_int(0)
-    ch = np.zeros(4, np.int32)
+    ch = np.zeros(8, np.int32)
     rc = load().gvdb_get_tuning(int(knob), ctypes.byref(v), _p(ch), ctypes.byref(n))
     if rc != OK:
         raise GVDBError(rc, "gvdb_get_tuning(%d)" % knob)

@@ -221,10 +221,11 @@ int gvdb_read_render_buf(gvdb_handle h, int buf, void* host_dst);
 #define GVDB_KNOB_APRON_RUN 5         /* bricks per workgroup of the multi-grid apron: 1, 2, 4, 8 */
 #define GVDB_KNOB_XCD_ORDER 6         /* per-brick kernels: 0 launch order, 1 each XCD a contiguous range of bricks */
 #define GVDB_KNOB_APRON_QUEUE 7       /* multi-grid apron: 0 static runs, 1 persistent workgroups over a brick queue */
-#define GVDB_KNOB_MARK_PATH 8         /* topology marks: 0 by domain size, 1 byte per brick, 2 direction masks, 3 bits */
+#define GVDB_KNOB_MARK_PATH 8         /* topology marks: 0 by domain size, 1 byte per brick, 2 direction masks, 3 bits,
+                                         4 sorted brick keys (taken above 2^34 level-0 bricks) */
 int gvdb_set_tuning(int knob, int value);
-/* current value, and the values it can take (choices4, may be NULL; nchoices of them) */
-int gvdb_get_tuning(int knob, int* value, int* choices4, int* nchoices);
+/* current value, and the values it can take (choices8: room for 8, may be NULL; nchoices of them) */
+int gvdb_get_tuning(int knob, int* value, int* choices8, int* nchoices);
 /* Render: samples fetched ahead (1, 2, 4) at the current ray-packet shape; ms3 per value */
 int gvdb_tune_render(gvdb_handle h, int shading, int chan, int buf, int* best_unroll, float* ms3);
 /* Render: ray-packet shape x samples fetched ahead; best2 = {unroll, tile width}; ms12[tile][unroll] */
