@@ -196,6 +196,13 @@ struct IncreEmitArgs {
     u64 base, cbase;
 };
 hipError_t launch_incre_emit(const IncreEmitArgs& a, hipStream_t s);
+// the incremental build over keys (keysort.hip): missing cells as
+// (level << 56 | cell) keys, counted (and leaf markers set) then written
+hipError_t launch_incre_keys(const IncreMarkArgs& a, uint32_t* cnt, const uint32_t* off, u64* keys, bool emit,
+                             hipStream_t s);
+hipError_t launch_level_counts(const u64* k, u64 n, uint32_t* counts, hipStream_t s);
+hipError_t launch_incre_emit_list(const u64* k, uint32_t n, const BitDomain& d, Node* nodes, int lev, int range,
+                                  int has_clist, u64 base, u64 cbase, hipStream_t s);
 // gvdbLinkBricks / gvdbDelinkLeafBricks / gvdbDelinkBricks with tree walks
 hipError_t launch_link_walk(const Tree& t, int lev, hipStream_t s);
 hipError_t launch_delink_leaves(const Tree& t, hipStream_t s);

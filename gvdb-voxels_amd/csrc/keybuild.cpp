@@ -43,6 +43,38 @@ struct Lists {
 };
 }  // namespace
 
+// Sorts keys[0, cnt) (`bits` significant) and keeps the distinct ones other
+// than `sentinel`, in order, in `out`; *count gets their number.  keys, kb_tmp_
+// and the flag buffers are scratch; the call waits for the count.
+int Volume::kb_sort_unique(u64* keys, u64 cnt, int bits, u64 sentinel, DevBuf& out, uint32_t* count, const Exec& ex) {
+    const hipStream_t s = ex.s;
+    *count = 0;
+    if (cnt == 0) return GVDB_OK;
+    KCHK(kb_tmp_.ensure(size_t(cnt) * 8));
+    const u64 hw = radix_hist_words(cnt);
+    KCHK(kb_hist_.ensure(size_t(hw + scan_tmp_words(hw) + 64) * 4));
+    uint32_t* hist = kb_hist_.as<uint32_t>();
+    u64* sorted = nullptr;
+    KCHK(radix_sort_u64(keys, kb_tmp_.as<u64>(), cnt, bits, hist, hist + hw + 32, s, &sorted));
+    KCHK(kb_flags_.ensure(size_t(cnt) * 4));
+    KCHK(kb_pos_.ensure(size_t(cnt) * 4));
+    KCHK(scan_tmp_.ensure(scan_tmp_words(cnt) * 4 + 256));
+    KCHK(launch_flag_unique(sorted, cnt, sentinel, kb_flags_.as<uint32_t>(), s));
+    KCHK(launch_scan_u32(kb_flags_.as<uint32_t>(), nullptr, 0, kb_pos_.as<uint32_t>(), cnt, ex.dsmall + SM_TOTAL,
+                         scan_tmp_.as<uint32_t>(), s));
+    KCHK(hipMemcpyAsync(ex.hsmall, ex.dsmall, SMALL_WORDS * 4, hipMemcpyDeviceToHost, s));
+    KCHK(hipStreamSynchronize(s));
+    *count = ex.hsmall[SM_TOTAL];
+    KCHK(out.ensure(size_t(std::max<uint32_t>(*count, 1)) * 8));
+    KCHK(launch_compact(sorted, cnt, kb_flags_.as<uint32_t>(), kb_pos_.as<uint32_t>(), out.as<u64>(), s));
+    return GVDB_OK;
+}
+
+void Volume::kb_release() {
+    for (DevBuf* b : {&kb_keys_, &kb_keysB_, &kb_cnt_, &kb_off_, &kb_tmp_, &kb_hist_, &kb_flags_, &kb_pos_, &kb_list_})
+        b->release();
+}
+
 int Volume::keyed_build(int n, float radius, const float orig[3], int D, const int rootPos[3], const BitDomain* dom,
                         const Exec& ex) {
     const hipStream_t s = ex.s;
@@ -58,30 +90,6 @@ int Volume::keyed_build(int n, float radius, const float orig[3], int D, const i
         u64 v = 0;
         memcpy(&v, hsmall + w, 8);
         return v;
-    };
-    // sorts keys[0, cnt) (bits significant), keeps the distinct ones other
-    // than `sentinel` in `out`; returns their number in *count
-    auto sort_unique = [&](u64* keys, u64 cnt, int bits, u64 sentinel, DevBuf& out, uint32_t* count) -> int {
-        *count = 0;
-        if (cnt == 0) return GVDB_OK;
-        KCHK(kb_tmp_.ensure(size_t(cnt) * 8));
-        const u64 hw = radix_hist_words(cnt);
-        KCHK(kb_hist_.ensure(size_t(hw + scan_tmp_words(hw) + 64) * 4));
-        uint32_t* hist = kb_hist_.as<uint32_t>();
-        u64* sorted = nullptr;
-        KCHK(radix_sort_u64(keys, kb_tmp_.as<u64>(), cnt, bits, hist, hist + hw + 32, s, &sorted));
-        KCHK(kb_flags_.ensure(size_t(cnt) * 4));
-        KCHK(kb_pos_.ensure(size_t(cnt) * 4));
-        KCHK(scan_tmp_.ensure(scan_tmp_words(cnt) * 4 + 256));
-        KCHK(launch_flag_unique(sorted, cnt, sentinel, kb_flags_.as<uint32_t>(), s));
-        KCHK(launch_scan_u32(kb_flags_.as<uint32_t>(), nullptr, 0, kb_pos_.as<uint32_t>(), cnt, small + SM_TOTAL,
-                             scan_tmp_.as<uint32_t>(), s));
-        int rc = sync();
-        if (rc) return rc;
-        *count = hsmall[SM_TOTAL];
-        KCHK(out.ensure(size_t(std::max<uint32_t>(*count, 1)) * 8));
-        KCHK(launch_compact(sorted, cnt, kb_flags_.as<uint32_t>(), kb_pos_.as<uint32_t>(), out.as<u64>(), s));
-        return GVDB_OK;
     };
     // out = the keys of b (sorted, distinct) absent from a (sorted), in order
     auto difference = [&](const DevBuf& b, uint32_t nb, const DevBuf& a, uint32_t na, DevBuf& out,
@@ -140,10 +148,10 @@ int Volume::keyed_build(int n, float radius, const float orig[3], int D, const i
         LB[L] = &lists.add();
         LE[L] = &lists.add();
     }
-    rc = sort_unique(kb_keys_.as<u64>(), u64(n), key_bits(d0.nbits), d0.nbits, *LA[0], &na[0]);
+    rc = kb_sort_unique(kb_keys_.as<u64>(), u64(n), key_bits(d0.nbits), d0.nbits, *LA[0], &na[0], ex);
     if (rc) return rc;
     if (dilate) {
-        rc = sort_unique(kb_keysB_.as<u64>(), totalB, key_bits(d0.nbits), d0.nbits, *LB[0], &nb[0]);
+        rc = kb_sort_unique(kb_keysB_.as<u64>(), totalB, key_bits(d0.nbits), d0.nbits, *LB[0], &nb[0], ex);
         if (rc) return rc;
     }
     for (int L = 1; L < D; L++) {
@@ -151,12 +159,12 @@ int Volume::keyed_build(int n, float radius, const float orig[3], int D, const i
         // parents of the level below, into the reusable key buffers, then sorted
         KCHK(kb_keys_.ensure(size_t(std::max<uint32_t>(na[L - 1], 1)) * 8));
         KCHK(launch_parent_keys(LA[L - 1]->as<u64>(), na[L - 1], dom[L - 1], ld, dom[L], kb_keys_.as<u64>(), s));
-        rc = sort_unique(kb_keys_.as<u64>(), na[L - 1], key_bits(dom[L].nbits), dom[L].nbits, *LA[L], &na[L]);
+        rc = kb_sort_unique(kb_keys_.as<u64>(), na[L - 1], key_bits(dom[L].nbits), dom[L].nbits, *LA[L], &na[L], ex);
         if (rc) return rc;
         if (dilate) {
             KCHK(kb_keysB_.ensure(size_t(std::max<uint32_t>(nb[L - 1], 1)) * 8));
             KCHK(launch_parent_keys(LB[L - 1]->as<u64>(), nb[L - 1], dom[L - 1], ld, dom[L], kb_keysB_.as<u64>(), s));
-            rc = sort_unique(kb_keysB_.as<u64>(), nb[L - 1], key_bits(dom[L].nbits), dom[L].nbits, *LB[L], &nb[L]);
+            rc = kb_sort_unique(kb_keysB_.as<u64>(), nb[L - 1], key_bits(dom[L].nbits), dom[L].nbits, *LB[L], &nb[L], ex);
             if (rc) return rc;
         }
     }
@@ -205,7 +213,7 @@ int Volume::keyed_build(int n, float radius, const float orig[3], int D, const i
     }
     // the lists and the key buffers are freed here: the launches above must be done
     KCHK(hipStreamSynchronize(s));
-    for (DevBuf* b : {&kb_keys_, &kb_keysB_, &kb_cnt_, &kb_off_, &kb_tmp_, &kb_hist_, &kb_flags_, &kb_pos_}) b->release();
+    kb_release();
     root_ = Elem(0, D, 0);
     rebuild_topo_ = false;
     host_valid_ = false;

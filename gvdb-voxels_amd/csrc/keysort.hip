@@ -12,6 +12,7 @@
 // its keys among equal digits with ballots (eight ballots give the lanes
 // sharing a lane's digit).
 #include "kernels.h"
+#include "tree_device.h"
 #include <algorithm>
 
 namespace gvdb {
@@ -330,6 +331,97 @@ hipError_t launch_link_list(const KeyLevel& C, const KeyLevel& P, int parent_is_
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_link_list, dim3(grid_of(n, 256, ~0ull)), dim3(256), 0, s, C, P, parent_is_root, ld, cnodes,
                        pclists, clist_len);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------- incremental build ----
+// gvdbCalcIncreExtraBrickId (kernels/cuda_gvdb_particles.cuh:420-465) with
+// keys instead of bitmap bits: the cells of every level in the point's box
+// that the walk from the top node does not find, as (level << 56 | cell key);
+// the count pass also sets the markers of the leaves it finds (the bitmap
+// form's k_incre_mark).
+__global__ __launch_bounds__(256) void k_incre_keys(IncreMarkArgs a, uint32_t* cnt, const uint32_t* off, u64* keys,
+                                                    int emit) {
+    const Tree& t = a.t;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.pts.n) return;
+    const float* p = reinterpret_cast<const float*>(a.pts.base + u64(i) * a.pts.stride + a.pts.off);
+    const float w[3] = {p[0] + a.o[0], p[1] + a.o[1], p[2] + a.o[2]};
+    uint32_t c = 0;
+    u64* out = emit ? keys + off[i] : nullptr;
+    if (isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]) && w[0] >= 0 && w[1] >= 0 && w[2] >= 0) {
+        int bmin[3], bmax[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int iw = int(w[k]);
+            bmin[k] = int(float(iw) - a.radius);
+            bmax[k] = int(float(iw) + a.radius);
+        }
+        for (int lev = 0; lev < a.D; lev++) {
+            const int rr = t.range[lev];
+            const int x0 = bmin[0] / rr * rr, x1 = bmax[0] / rr * rr;
+            const int y0 = bmin[1] / rr * rr, y1 = bmax[1] / rr * rr;
+            const int z0 = bmin[2] / rr * rr, z1 = bmax[2] / rr * rr;
+            for (int x = x0; x <= x1; x += rr)
+                for (int y = y0; y <= y1; y += rr)
+                    for (int z = z0; z <= z1; z += rr) {
+                        if (x < 0 || y < 0 || z < 0) continue;
+                        const int id = pos_parent(t, x, y, z, lev);
+                        if (id < 0) {
+                            u64 idx;
+                            if (in_box(a.dom[lev], x / rr, y / rr, z / rr, idx)) {
+                                if (emit) *out++ = (u64(lev) << 56) | idx;
+                                else c++;
+                            }
+                        } else if (lev == 0 && !emit) {
+                            a.marker[id] = 1u;
+                        }
+                    }
+        }
+    }
+    if (!emit) cnt[i] = c;
+}
+
+hipError_t launch_incre_keys(const IncreMarkArgs& a, uint32_t* cnt, const uint32_t* off, u64* keys, bool emit,
+                             hipStream_t s) {
+    if (a.pts.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_incre_keys, dim3(grid_of(u64(a.pts.n), 256, 1u << 31)), dim3(256), 0, s, a, cnt, off, keys,
+                       emit ? 1 : 0);
+    return hipGetLastError();
+}
+
+// Each level's range of a sorted (level << 56 | cell) list: range[2 * level]
+// = first, range[2 * level + 1] = one past the last (both zero beforehand)
+__global__ __launch_bounds__(256) void k_level_counts(const u64* k, u64 n, uint32_t* range) {
+    const u64 i = u64(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t lev = uint32_t(k[i] >> 56);
+    if (i == 0 || uint32_t(k[i - 1] >> 56) != lev) range[2 * lev] = uint32_t(i);
+    if (i + 1 == n || uint32_t(k[i + 1] >> 56) != lev) range[2 * lev + 1] = uint32_t(i + 1);
+}
+
+hipError_t launch_level_counts(const u64* k, u64 n, uint32_t* counts, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_level_counts, dim3(grid_of(n, 256, ~0ull)), dim3(256), 0, s, k, n, counts);
+    return hipGetLastError();
+}
+
+// New nodes of one level from its slice of the sorted list: node base + i,
+// child list cbase + i (the bitmap form's k_incre_emit)
+__global__ __launch_bounds__(256) void k_incre_emit_list(const u64* k, uint32_t n, BitDomain d, Node* nodes, int lev,
+                                                         int range, int has_clist, u64 base, u64 cbase) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    int x, y, z;
+    box_coords(d, k[i] & ((u64(1) << 56) - 1), x, y, z);
+    list_node(nodes + base + i, lev, x, y, z, range, has_clist ? Elem(1, lev, cbase + i) : ID_UNDEFL);
+}
+
+hipError_t launch_incre_emit_list(const u64* k, uint32_t n, const BitDomain& d, Node* nodes, int lev, int range,
+                                  int has_clist, u64 base, u64 cbase, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_incre_emit_list, dim3(grid_of(n, 256, ~0ull)), dim3(256), 0, s, k, n, d, nodes, lev, range,
+                       has_clist, base, cbase);
     return hipGetLastError();
 }
 

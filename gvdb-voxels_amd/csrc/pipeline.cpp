@@ -441,6 +441,9 @@ int Volume::incremental_topology(int n, float radius, const float orig[3], int D
     if (D <= 0 || D >= levels_) return fail(GVDB_ERR_DEPTH, "incremental topology: depth out of range");
     BitDomain dom[MAXLEV];
     u64 bm_off[MAXLEV], tmp_off[MAXLEV], bm_words = 0, tmp_words = 0;
+    // boxes above 2^34 cells (or the keyed path forced): missing cells as
+    // sorted keys instead of bitmaps
+    bool keyed = false;
     for (int L = 0; L < D; L++) {
         const int rr = int(range_[L]);
         BitDomain& d = dom[L];
@@ -453,20 +456,22 @@ int Volume::incremental_topology(int n, float radius, const float orig[3], int D
             d.dim[a] = std::max(1, bhi / rr - d.lo[a] + 1);
             d.nbits *= u64(d.dim[a]);
         }
-        if (d.nbits > (u64(1) << 34))
-            return fail(GVDB_ERR_RANGE, "incremental topology: node box of the points exceeds 2^34 nodes");
+        if (keyed_build_wanted(d)) keyed = true;
         d.nwords = (d.nbits + 31) / 32;
         bm_off[L] = bm_words;
         bm_words += d.nwords;
         tmp_off[L] = tmp_words;
         tmp_words += scan_tmp_words(d.nwords);
     }
-    XCHK(this, bitmaps_.ensure(bm_words * 4 + 256));
-    XCHK(this, ranks_.ensure(bm_words * 4 + 256));
-    XCHK(this, scan_tmp_.ensure(tmp_words * 4 + 256));
+    if (!keyed) {
+        XCHK(this, bitmaps_.ensure(bm_words * 4 + 256));
+        XCHK(this, ranks_.ensure(bm_words * 4 + 256));
+        XCHK(this, scan_tmp_.ensure(tmp_words * 4 + 256));
+    }
     XCHK(this, incr_marker_.ensure(size_t(total) * 4));
     Tree t = make_tree();
-    std::vector<FillArgs> fa{FillArgs{bitmaps_.p, bm_words, 0, 0}};
+    std::vector<FillArgs> fa;
+    if (!keyed) fa.push_back(FillArgs{bitmaps_.p, bm_words, 0, 0});
     if (!accum) fa.push_back(FillArgs{incr_marker_.p, total, 0, 0});
     XCHK(this, launch_fill_multi(fa.data(), int(fa.size()), ex.ar, ex.s));
     if (accum) {
@@ -486,18 +491,55 @@ int Volume::incremental_topology(int n, float radius, const float orig[3], int D
         ma.bm[L] = bitmaps_.as<uint32_t>() + bm_off[L];
     }
     ma.marker = incr_marker_.as<uint32_t>();
-    XCHK(this, launch_incre_mark(ma, ex.s));
-    std::vector<ScanArgs> sa;
-    for (int L = 0; L < D; L++)
-        sa.push_back(ScanArgs{bitmaps_.as<uint32_t>() + bm_off[L], nullptr, 1, ranks_.as<uint32_t>() + bm_off[L],
-                              dom[L].nwords, scan_tmp_.as<uint32_t>() + tmp_off[L], ex.dsmall + SM_COUNTS + 2 * L});
-    XCHK(this, launch_scan_multi(sa.data(), int(sa.size()), ex.ar, ex.s));
-    rc = drain(this, ex, 1);
-    if (rc) return rc;
-    u64 cnt[MAXLEV], added = 0;
-    for (int L = 0; L < D; L++) {
-        cnt[L] = ex.hsmall[SM_COUNTS + 2 * L];
-        added += cnt[L];
+    u64 cnt[MAXLEV], first[MAXLEV], added = 0;
+    if (!keyed) {
+        XCHK(this, launch_incre_mark(ma, ex.s));
+        std::vector<ScanArgs> sa;
+        for (int L = 0; L < D; L++)
+            sa.push_back(ScanArgs{bitmaps_.as<uint32_t>() + bm_off[L], nullptr, 1, ranks_.as<uint32_t>() + bm_off[L],
+                                  dom[L].nwords, scan_tmp_.as<uint32_t>() + tmp_off[L], ex.dsmall + SM_COUNTS + 2 * L});
+        XCHK(this, launch_scan_multi(sa.data(), int(sa.size()), ex.ar, ex.s));
+        rc = drain(this, ex, 1);
+        if (rc) return rc;
+        for (int L = 0; L < D; L++) {
+            cnt[L] = ex.hsmall[SM_COUNTS + 2 * L];
+            first[L] = 0;
+            added += cnt[L];
+        }
+    } else {
+        // count (and markers), 64-bit total, offsets; then the keys, sorted
+        // and made unique; each level's slice of the list
+        XCHK(this, kb_cnt_.ensure(size_t(std::max(n, 1)) * 4));
+        XCHK(this, kb_off_.ensure(size_t(std::max(n, 1)) * 4));
+        XCHK(this, launch_incre_keys(ma, kb_cnt_.as<uint32_t>(), nullptr, nullptr, false, ex.s));
+        XCHK(this, launch_sum_u32(kb_cnt_.as<uint32_t>(), u64(n), reinterpret_cast<u64*>(ex.dsmall + SM_LEN), ex.s));
+        XCHK(this, scan_tmp_.ensure(scan_tmp_words(u64(n)) * 4 + 256));
+        XCHK(this, launch_scan_u32(kb_cnt_.as<uint32_t>(), nullptr, 0, kb_off_.as<uint32_t>(), u64(n), nullptr,
+                                   scan_tmp_.as<uint32_t>(), ex.s));
+        rc = drain(this, ex, 1);
+        if (rc) return rc;
+        u64 nk = 0;
+        memcpy(&nk, ex.hsmall + SM_LEN, 8);
+        if (nk > (u64(1) << 31)) {
+            kb_release();
+            return fail(GVDB_ERR_RANGE, "incremental topology: more than 2^31 missing cells over the points");
+        }
+        XCHK(this, kb_keys_.ensure(size_t(std::max<u64>(nk, 1)) * 8));
+        XCHK(this, launch_incre_keys(ma, kb_cnt_.as<uint32_t>(), kb_off_.as<uint32_t>(), kb_keys_.as<u64>(), true, ex.s));
+        uint32_t nu = 0;
+        rc = kb_sort_unique(kb_keys_.as<u64>(), nk, 59, ~0ull, kb_list_, &nu, ex);
+        if (rc) return rc;
+        FillArgs z{ex.dsmall + SM_COUNTS, 2 * MAXLEV, 0, 0};
+        XCHK(this, launch_fill_multi(&z, 1, ex.ar, ex.s));
+        XCHK(this, launch_level_counts(kb_list_.as<u64>(), nu, ex.dsmall + SM_COUNTS, ex.s));
+        rc = drain(this, ex, 1);
+        if (rc) return rc;
+        for (int L = 0; L < D; L++) {
+            first[L] = ex.hsmall[SM_COUNTS + 2 * L];
+            cnt[L] = ex.hsmall[SM_COUNTS + 2 * L + 1] - first[L];
+            added += cnt[L];
+        }
+        if (added == 0) kb_release();
     }
     if (added == 0) return GVDB_OK;   // the reference returns before applying the markers
     XCHK(this, launch_apply_markers(pool_[0][0].dev.as<Node>(), incr_marker_.as<uint32_t>(), total, ex.s));
@@ -515,9 +557,15 @@ int Volume::incremental_topology(int n, float radius, const float orig[3], int D
     }
     for (int L = 0; L < D; L++) {
         if (cnt[L] == 0) continue;
-        IncreEmitArgs ea{dom[L], bitmaps_.as<uint32_t>() + bm_off[L], ranks_.as<uint32_t>() + bm_off[L],
-                         pool_[0][L].dev.as<Node>(), L, int(range_[L]), L > 0 ? 1 : 0, base0[L], base1[L]};
-        XCHK(this, launch_incre_emit(ea, ex.s));
+        if (keyed) {
+            XCHK(this, launch_incre_emit_list(kb_list_.as<u64>() + first[L], uint32_t(cnt[L]), dom[L],
+                                              pool_[0][L].dev.as<Node>(), L, int(range_[L]), L > 0 ? 1 : 0, base0[L],
+                                              base1[L], ex.s));
+        } else {
+            IncreEmitArgs ea{dom[L], bitmaps_.as<uint32_t>() + bm_off[L], ranks_.as<uint32_t>() + bm_off[L],
+                             pool_[0][L].dev.as<Node>(), L, int(range_[L]), L > 0 ? 1 : 0, base0[L], base1[L]};
+            XCHK(this, launch_incre_emit(ea, ex.s));
+        }
         if (L > 0)
             cf.push_back(FillArgs{pool_[1][L].dev.as<u64>() + base1[L] * voxcnt(L), cnt[L] * voxcnt(L), ID_UNDEF64, 1});
         pool_[0][L].lastEle += cnt[L];
@@ -538,6 +586,7 @@ int Volume::incremental_topology(int n, float radius, const float orig[3], int D
     XCHK(this, launch_count_unflagged(t, ex.dsmall + SM_UNFLAGGED, ex.s));
     rc = drain(this, ex, 1);
     if (rc) return rc;
+    if (keyed) kb_release();
     pool_[0][0].usedNum = pool_[0][0].lastEle - ex.hsmall[SM_UNFLAGGED];
     compute_top_lev();
     host_valid_ = false;

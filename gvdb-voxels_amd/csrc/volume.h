@@ -202,6 +202,8 @@ public:
                              float radius, BitDomain* dom);
     int keyed_build(int n, float radius, const float orig[3], int D, const int rootPos[3], const BitDomain* dom,
                     const Exec& ex);
+    int kb_sort_unique(u64* keys, u64 cnt, int bits, u64 sentinel, DevBuf& out, uint32_t* count, const Exec& ex);
+    void kb_release();
     static int multi_finish_topology(Volume* const* v, int G, bool commit, bool bounds, const Exec& ex, int* failed);
     static int multi_update_atlas(Volume* const* v, int G, const Exec& ex, int* failed);
     static int multi_clear_channel(Volume* const* v, int G, int chan, const Exec& ex, int* failed);
@@ -289,7 +291,7 @@ private:
     // topology scratch
     DevBuf small_;       // bbox keys, counts
     DevBuf bitmaps_, ranks_, masks_, scan_tmp_, flags_, slot_scan_;
-    DevBuf kb_keys_, kb_keysB_, kb_cnt_, kb_off_, kb_tmp_, kb_hist_, kb_flags_, kb_pos_;   // keyed build
+    DevBuf kb_keys_, kb_keysB_, kb_cnt_, kb_off_, kb_tmp_, kb_hist_, kb_flags_, kb_pos_, kb_list_;   // keyed builds
     DevBuf incr_marker_;   // per-leaf markers of the incremental build
     DevBuf view_dev_, view_scene_;   // the device view handed to user kernels
     uint32_t* pinned_ = nullptr;   // host staging (256 words)

@@ -137,3 +137,45 @@ def test_accumulate_refuses_a_deeper_tree(gpu):
     with pytest.raises(GVDBError):
         v.AccumulateTopology(len(far), 2.0, H.ZERO3)
     v.close()
+
+
+@pytest.mark.parametrize("shift,accum", [((70.0, 0.0, 0.0), False), ((70.0, -40.0, 0.0), True)])
+def test_incremental_keyed_path_same_pools(gpu, shift, accum):
+    """The keyed form of the incremental build (missing cells as sorted keys,
+    taken above 2^34 cells per level; forced here with the mark-path knob)
+    gives the oracle's pools."""
+    A, B = _pair(shift)
+    keep = gpu.get_tuning(gpu.KNOB_MARK_PATH)[0]
+    try:
+        gpu.set_tuning(gpu.KNOB_MARK_PATH, 4)
+        v = _gpu_steps(gpu, A, B, 2.0, accum)
+    finally:
+        gpu.set_tuning(gpu.KNOB_MARK_PATH, keep)
+    o = _oracle_steps(A, B, 2.0, accum)
+    assert o.undefined_events() == 0
+    H.assert_pools_equal(v, o)
+    assert v.info()["top_lev"] == o.top_lev()
+    v.close()
+
+
+def _clusters(n_clusters, per, span, seed, sigma=3.0):
+    g = np.random.default_rng(seed)
+    c = g.uniform(16.0, span - 16.0, (n_clusters, 3))
+    p = c[:, None, :] + g.normal(0.0, sigma, (n_clusters, per, 3))
+    return np.clip(p.reshape(-1, 3), 0.5, span - 0.5).astype(np.float32)
+
+
+@pytest.mark.parametrize("accum", [False, True])
+def test_incremental_above_2_34_cells(gpu, accum):
+    """Clusters over a 200000^3-voxel box (1.6e13 bricks), then more clusters
+    in the same box: the incremental branch's boxes exceed 2^34 cells, so the
+    keyed form runs; pools equal the oracle's."""
+    A = _clusters(2000, 10, 200000.0, 9)
+    B = _clusters(1500, 10, 200000.0, 19)
+    v = _gpu_steps(gpu, A, B, 2.0, accum)
+    o = _oracle_steps(A, B, 2.0, accum)
+    assert o.undefined_events() == 0
+    assert len(o.pool(0, 0)) > len(_oracle_full(A, 2.0).pool(0, 0))
+    H.assert_pools_equal(v, o)
+    assert v.info()["top_lev"] == o.top_lev()
+    v.close()
