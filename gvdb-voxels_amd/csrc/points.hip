@@ -1166,6 +1166,10 @@ static const int PG_ZMAX = 5;
 // and 256 on the medium workload) when the (2 + 2R)^3 cells fit one per
 // thread (R = 1: 64), else 256 (R = 2: 216 cells).
 static const int PG_SMALL = 128;
+// Listed variants: the cell of each of the first PG_OWN candidates comes from
+// a table filled at set-up (one LDS byte read instead of a binary search over
+// the 64 cell starts); later candidates search.
+static const int PG_OWN = 512;
 template <int NT, int LS>
 struct PGShared {
     // the listed variant runs with R = 1 only: (2 + 2R)^3 = 64 cells
@@ -1177,6 +1181,7 @@ struct PGShared {
     u64 key[8 * PG_ZSTRIDE];
     typename std::conditional<LS != 0, float3, float4>::type pt[2 * NT];
     uint16_t item[LS ? 2 * NT * PG_ZMAX : 2];   // candidate << 3 | z slice in the brick
+    uint8_t owner[LS ? PG_OWN : 4];             // cell of candidate f < PG_OWN
 };
 
 // r - sqrt(D) 2^-s, accurate to a few ulps: D split into float hi + lo
@@ -1215,6 +1220,13 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
     const float r = a.radius;
     const float r2cut = r * r * (1.0f + 0x1p-20f);
     const int span = 2 + 2 * a.R, ncell = span * span * span;
+    if constexpr (LS != 0) {
+        if (tid < ncell) {
+            const uint32_t b = S.cell_start[tid], e = tid + 1 < ncell ? S.cell_start[tid + 1] : T;
+            for (uint32_t f = b; f < min(e, uint32_t(PG_OWN)); f++) S.owner[f] = uint8_t(tid);
+        }
+        __syncthreads();
+    }
     const int ox = bn->mPos[0], oy = bn->mPos[1], oz = bn->mPos[2];
     const float bx0 = float(ox), by0 = float(oy), bz0 = float(oz);
     const float bx1 = float(ox + 7), by1 = float(oy + 7), bz1 = float(oz + 7);
@@ -1239,11 +1251,16 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
         for (int h = 0; h < 2; h++) {
             const uint32_t f = c0 + uint32_t(h * NT + tid);
             if (f < T) {
-                int lo = 0, hi = ncell - 1;        // last cell starting at or before f
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (S.cell_start[mid] <= f) lo = mid;
-                    else hi = mid - 1;
+                int lo = 0;
+                if (LS != 0 && f < uint32_t(PG_OWN)) {
+                    lo = S.owner[f];
+                } else {
+                    int hi = ncell - 1;            // last cell starting at or before f
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (S.cell_start[mid] <= f) lo = mid;
+                        else hi = mid - 1;
+                    }
                 }
                 p[h] = a.sorted[S.cell_off[lo] + (f - S.cell_start[lo])];
             }
