@@ -969,12 +969,17 @@ struct GBShared {
     uint16_t lst[GB_THREADS / 64][GB_CHUNK];
 };
 
+// The brick's 27 neighbour ids, one per thread below 27 (loaded by the
+// caller ahead of its own checks, so the load overlaps them).
+__device__ inline int gb_nbr(const int* nbr27, int brick) {
+    return threadIdx.x < 27 ? nbr27[u64(brick) * 27 + threadIdx.x] : 0;
+}
+
 template <int NT = GB_THREADS, class SH>
-__device__ inline uint32_t gb_setup(int brick, int R, const int* nbr27, const uint32_t* bcnt, const uint32_t* boff,
-                                    SH& S) {
+__device__ inline uint32_t gb_setup(int R, int nb_mine, const uint32_t* bcnt, const uint32_t* boff, SH& S) {
     const int sd = 2, span = sd + 2 * R, ncell = span * span * span;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (threadIdx.x < 27) S.nbr[threadIdx.x] = nbr27[u64(brick) * 27 + threadIdx.x];
+    if (threadIdx.x < 27) S.nbr[threadIdx.x] = nb_mine;
     __syncthreads();
     const bool ok = S.nbr[13] >= 0;
     const int c = threadIdx.x;
@@ -1029,7 +1034,7 @@ __device__ inline void k_bucket_gather_body(const BucketGatherArgs& a, uint32_t 
     Best B[4];
 #pragma unroll
     for (int x = 0; x < 4; x++) B[x] = Best{INFINITY, 0xFFFFFFFFu};
-    const uint32_t T = gb_setup(brick, R, a.nbr, a.bcnt, a.boff, S);
+    const uint32_t T = gb_setup(R, gb_nbr(a.nbr, brick), a.bcnt, a.boff, S);
     __syncthreads();
     const int org[3] = {bn->mPos[0], bn->mPos[1], bn->mPos[2]};
     const u64 lower = lane ? ((~0ull) >> (64 - lane)) : 0ull;
@@ -1210,12 +1215,13 @@ __device__ inline void k_push_gather_body(const BucketGatherArgs& a, uint32_t bi
     const Tree& t = a.t;
     const int brick = int(bid);
     const Node* bn = t.nodes[0] + brick;
+    const int nb_mine = gb_nbr(a.nbr, brick);     // in flight during the checks
     if (bn->mParent == ID_UNDEF64) return;        // unlinked leaves have no subcells
     if (!bn->mFlags || bn->mValue[0] < 0) return;  // no atlas slot to write (uniform per brick)
     const int tid = threadIdx.x, lane = tid & 63;
     // voxel v = tid + 256 j (j = 0, 1) of the brick: x = v & 7, y = (v >> 3) & 7, z = v >> 6
     for (int v = tid; v < 512; v += NT) S.key[(v >> 6) * PG_ZSTRIDE + (v & 63)] = ~0ull;
-    const uint32_t T = gb_setup<NT>(brick, a.R, a.nbr, a.bcnt, a.boff, S);
+    const uint32_t T = gb_setup<NT>(a.R, nb_mine, a.bcnt, a.boff, S);
     __syncthreads();                               // cell table and brick copy ready
     const float r = a.radius;
     const float r2cut = r * r * (1.0f + 0x1p-20f);
@@ -1545,11 +1551,12 @@ __device__ inline void k_gather_color_body(const ColorArgs& a, uint32_t bid, uin
     const Tree& t = a.t;
     const int brick = int(bid);
     const Node* bn = t.nodes[0] + brick;
+    const int nb_mine = gb_nbr(a.nbr, brick);
     if (bn->mParent == ID_UNDEF64) return;        // unlinked leaves have no subcells
     if (!bn->mFlags || bn->mValue[0] < 0) return;  // no atlas slot to write (uniform per brick)
     const int tid = threadIdx.x;
     if (tid < 40) (&S.sum[0][0])[tid] = 0;
-    const uint32_t T = gb_setup<CG_THREADS>(brick, a.R, a.nbr, a.bcnt, a.boff, S);
+    const uint32_t T = gb_setup<CG_THREADS>(a.R, nb_mine, a.bcnt, a.boff, S);
     __syncthreads();
     const int span = 2 + 2 * a.R, ncell = span * span * span;
     const int sc = a.scs, sd = t.brickwid / sc;   // subcells per axis (2 for 8^3 bricks, sc 4)
