@@ -228,13 +228,37 @@ static void launch_apron_runs(int run, dim3 grid, hipStream_t s, const ApronArgs
     }
 }
 
+// One grid: RUN consecutive bricks per workgroup, the record by value (a
+// workgroup per brick launches 8 waves for 488 one-voxel threads; a run
+// spreads that launch cost over RUN bricks).
+template <typename AT, int RUN>
+__global__ __launch_bounds__(512) void k_apron_seq(ApronArgs a0, MultiMap m) {
+    const uint32_t bi = m.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t b0 = bi * uint32_t(RUN);
+    for (int k = 0; k < RUN; k++) {
+        if (b0 + uint32_t(k) >= uint32_t(a0.n)) break;
+        if (k) __syncthreads();            // the previous brick is done with its offsets
+        k_apron_body<AT>(a0, b0 + uint32_t(k), 0);
+    }
+}
+
+template <typename AT>
+static void launch_apron_seq(int run, dim3 grid, hipStream_t s, const ApronArgs& a0, const MultiMap& m) {
+    switch (run) {
+    case 2: hipLaunchKernelGGL((k_apron_seq<AT, 2>), grid, dim3(512), 0, s, a0, m); break;
+    case 4: hipLaunchKernelGGL((k_apron_seq<AT, 4>), grid, dim3(512), 0, s, a0, m); break;
+    case 8: hipLaunchKernelGGL((k_apron_seq<AT, 8>), grid, dim3(512), 0, s, a0, m); break;
+    default: hipLaunchKernelGGL(k_apron<AT>, grid, dim3(512), 0, s, a0, m); break;
+    }
+}
+
 template <typename AT>
 static void launch_apron_as(bool queue, bool runs, int run, uint32_t nwg, uint32_t* ctr, const MultiLaunch& L,
                             hipStream_t s, const ApronArgs& a0) {
     if (queue)
         hipLaunchKernelGGL(k_apron_queue<AT>, dim3(nwg), dim3(512), 0, s, a0, L.map, ctr, L.blocks);
     else if (runs) launch_apron_runs<AT>(run, dim3(L.blocks), s, a0, L.map);
-    else hipLaunchKernelGGL(k_apron<AT>, dim3(L.blocks), dim3(512), 0, s, a0, L.map);
+    else launch_apron_seq<AT>(run, dim3(L.blocks), s, a0, L.map);
 }
 
 hipError_t launch_apron_multi(const ApronArgs* a, int G, Arena* ar, hipStream_t s) {
@@ -253,12 +277,13 @@ hipError_t launch_apron_multi(const ApronArgs* a, int G, Arena* ar, hipStream_t 
             }
         }
         if (sel.empty()) continue;
-        // one grid: a workgroup per brick; several: runs of `run` bricks, or
-        // the persistent queue over runs of QRUN bricks
+        // one grid: runs of `run` consecutive bricks per workgroup (1: a
+        // workgroup per brick); several: runs of one grid's bricks, or the
+        // persistent queue over runs of QRUN bricks
         const bool runs = sel.size() > 1;
         const bool queue = runs && knob(KNOB_APRON_QUEUE) == 1;
-        const int run = queue ? QRUN : knob(KNOB_APRON_RUN);
-        for (const ApronArgs& x : sel) nb.push_back(runs ? uint32_t((x.n + run - 1) / run) : uint32_t(x.n));
+        const int run = queue ? QRUN : (runs ? knob(KNOB_APRON_RUN) : knob(KNOB_APRON_RUN_ONE));
+        for (const ApronArgs& x : sel) nb.push_back(uint32_t((x.n + run - 1) / run));
         MultiLaunch L;
         hipError_t e = multi_setup(ar, sel.data(), nb.data(), int(sel.size()), s, L);
         if (e != hipSuccess) return e;

@@ -30,6 +30,7 @@ struct gvdb_graph_s {
 };
 
 static_assert(sizeof(gvdb_node) == sizeof(gvdb::Node), "gvdb_node mirrors the 64-byte node");
+static_assert(GVDB_KNOB_COUNT == gvdb::KNOB_COUNT, "the header names every launch-shape knob");
 
 #define H_OR_FAIL(h)                     \
     if (!(h)) return GVDB_ERR_INVALID;   \

@@ -468,6 +468,7 @@ enum Knob {
     KNOB_APRON_QUEUE,            // multi-grid apron: 0 static runs, 1 persistent workgroups over a brick queue
     KNOB_MARK_PATH,              // topology marks: 0 by domain size, 1 a byte per brick, 2 direction masks, 3 bits,
                                  // 4 sorted brick keys (the build taken above 2^34 bricks)
+    KNOB_APRON_RUN_ONE,          // bricks per workgroup of the one-grid apron: 1, 2, 4, 8
     KNOB_COUNT
 };
 int knob(int k);

@@ -24,6 +24,7 @@ SCN_FLOATS = 68
 # launch-shape knobs of the autotuner (gvdb_set_tuning / gvdb_get_tuning)
 KNOB_RENDER_UNROLL_EA, KNOB_RENDER_UNROLL_ISO, KNOB_RENDER_TILE_EA, KNOB_RENDER_TILE_ISO = 0, 1, 2, 3
 KNOB_GATHER_THREADS, KNOB_APRON_RUN, KNOB_XCD_ORDER, KNOB_APRON_QUEUE, KNOB_MARK_PATH = 4, 5, 6, 7, 8
+KNOB_APRON_RUN_ONE, KNOB_COUNT = 9, 10
 # the allocation-free UpdateApron: size of its fixed device array and the bricks it holds
 NOALLOC_SCRATCH_BYTES = 32 << 20
 NOALLOC_MAX_BRICKS = NOALLOC_SCRATCH_BYTES // (27 * 12)

@@ -223,6 +223,8 @@ int gvdb_read_render_buf(gvdb_handle h, int buf, void* host_dst);
 #define GVDB_KNOB_APRON_QUEUE 7       /* multi-grid apron: 0 static runs, 1 persistent workgroups over a brick queue */
 #define GVDB_KNOB_MARK_PATH 8         /* topology marks: 0 by domain size, 1 byte per brick, 2 direction masks, 3 bits,
                                          4 sorted brick keys (taken above 2^34 level-0 bricks) */
+#define GVDB_KNOB_APRON_RUN_ONE 9     /* bricks per workgroup of the one-grid apron: 1, 2, 4, 8 */
+#define GVDB_KNOB_COUNT 10
 int gvdb_set_tuning(int knob, int value);
 /* current value, and the values it can take (choices8: room for 8, may be NULL; nchoices of them) */
 int gvdb_get_tuning(int knob, int* value, int* choices8, int* nchoices);

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture
 def restore_knobs(gpu):
-    keep = [gpu.get_tuning(k)[0] for k in range(9)]
+    keep = [gpu.get_tuning(k)[0] for k in range(gpu.KNOB_COUNT)]
     yield
     for k, v in enumerate(keep):
         gpu.set_tuning(k, v)
@@ -83,6 +83,51 @@ def test_gather_threads_same_atlas(gpu, restore_knobs, radius):
                 v.UpdateApron(0, 0.0)
                 assert np.array_equal(v.atlas(0, out[(t, 128)].dtype).view(np.uint8), out[(t, 128)].view(np.uint8))
             v.close()
+
+
+def test_one_grid_apron_runs_same_atlas(gpu, restore_knobs):
+    """UpdateApron on one grid with runs of 1, 2, 4 and 8 bricks per
+    workgroup, launch order and XCD order, float, half and uchar4 channels,
+    nonzero boundary value: the same atlas bit for bit."""
+    from gvdb_amd import workloads as W
+    pts = W.sphere_shell(30000, seed=23, radius=45.0, centre=(90.0, 90.0, 90.0))
+    clr = np.random.default_rng(5).integers(0, 1 << 32, len(pts), dtype=np.uint64).astype(np.uint32)
+    for t in (gpu.T_FLOAT, gpu.T_HALF):
+        ref = None
+        for xcd in (0, 1):
+            for run in (1, 2, 4, 8):
+                gpu.set_tuning(gpu.KNOB_XCD_ORDER, xcd)
+                gpu.set_tuning(gpu.KNOB_APRON_RUN_ONE, run)
+                v, _ = H.gpu_volume(gpu, pts, 2.0, atlas_type=t, boundval=0.25)
+                a = v.atlas(0, np.float32 if t == gpu.T_FLOAT else np.float16).view(np.uint8).copy()
+                if ref is None:
+                    ref = a
+                assert np.array_equal(a, ref), (t, xcd, run)
+                v.close()
+    # the colour channel's apron (uchar4 words)
+    refc = None
+    for run in (1, 2, 4, 8):
+        gpu.set_tuning(gpu.KNOB_APRON_RUN_ONE, run)
+        v = gpu.VolumeGVDB(0)
+        v.Configure(*H.TREE)
+        v.SetChannelDefault(16, 16, 1)
+        v.AddChannel(0, gpu.T_FLOAT, 1)
+        v.AddChannel(1, gpu.T_UCHAR4, 1)
+        v.SetPoints(pts, clr)
+        v.RequestFullRebuild(True)
+        v.RebuildTopology(len(pts), 2.0, H.ZERO3)
+        v.FinishTopology(False, True)
+        v.UpdateAtlas()
+        v.ClearChannel(0)
+        v.ClearChannel(1)
+        n = v.InsertPointsSubcell(4, len(pts), 2.0, H.ZERO3)
+        v.GatherDensity(4, len(pts), 2.0, H.ZERO3, n, 0, 1, False)
+        v.UpdateApron(1, 7.0)
+        c = v.atlas(1, np.uint32).copy()
+        if refc is None:
+            refc = c
+        assert np.array_equal(c, refc), run
+        v.close()
 
 
 def test_batched_knobs_same_results(gpu, restore_knobs):
