@@ -107,25 +107,17 @@ static float apron_bound_u8x4(float b) {
     return f;
 }
 
-// One 512-thread workgroup per brick; threads 0..487 each own one apron voxel
-// of the padded 10^3 brick (apron 1, 8^3 interior).  Enumeration: the z = 0
-// and z = 9 faces (100 each), then the 36-voxel rings of z = 1..8.
+// Apron voxel `a` (0..487) of brick b, given the atlas offsets of the brick
+// and its 26 neighbours (-1: none).  Enumeration: the z = 0 and z = 9 faces
+// (100 each), then the 36-voxel rings of z = 1..8.
 template <typename AT>
-__device__ inline void k_apron_body(const ApronArgs& ap, uint32_t bid, uint32_t) {
-    __shared__ long long base[27];
+__device__ inline void apron_voxel(const ApronArgs& ap, int b, int a, const long long* base) {
     const Tree& t = ap.t;
     AT* atlas = reinterpret_cast<AT*>(ap.atlas);
-    const float boundval = ap.boundval;
-    const int b = int(bid);
-    const int a = threadIdx.x;
     const int br = t.brick_res;        // 10
     const int w = t.brickwid;          // 8
     const int face = br * br;
     const int ring = br * br - w * w;
-    // atlas offsets of the brick and its 26 neighbours (-1: none), one load
-    if (a < 27) base[a] = ap.base[u64(b) * 27 + a];
-    __syncthreads();
-    if (a >= 2 * face + w * ring || base[13] < 0) return;
     int x, y, z;
     if (a < 2 * face) {
         z = a < face ? 0 : br - 1;
@@ -146,7 +138,7 @@ __device__ inline void k_apron_body(const ApronArgs& ap, uint32_t bid, uint32_t)
     const int d = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
     const long long nbase = base[d];
     const u64 own = u64(base[13]) + u64((z * br + y) * br + x);
-    float v = boundval;
+    float v = ap.boundval;
     if (nbase >= 0) {
         // interior coordinate inside the neighbour
         const int sx = (x - 1) - dx * w, sy = (y - 1) - dy * w, sz = (z - 1) - dz * w;
@@ -163,7 +155,43 @@ __device__ inline void k_apron_body(const ApronArgs& ap, uint32_t bid, uint32_t)
     st<AT>(atlas, own, v);
 }
 
+// One 512-thread workgroup per brick; threads 0..487 each own one apron voxel
+// of the padded 10^3 brick (apron 1, 8^3 interior).
+template <typename AT>
+__device__ inline void k_apron_body(const ApronArgs& ap, uint32_t bid, uint32_t) {
+    __shared__ long long base[27];
+    const Tree& t = ap.t;
+    const int b = int(bid);
+    const int a = threadIdx.x;
+    const int br = t.brick_res, w = t.brickwid;
+    // atlas offsets of the brick and its 26 neighbours (-1: none), one load
+    if (a < 27) base[a] = ap.base[u64(b) * 27 + a];
+    __syncthreads();
+    if (a >= 2 * br * br + w * (br * br - w * w) || base[13] < 0) return;
+    apron_voxel<AT>(ap, b, a, base);
+}
+
 GVDB_MULTI_T(k_apron, typename AT, AT, ApronArgs, 512)
+
+// A run of up to RUN consecutive bricks b0.. of one grid, its voxels spread
+// over all 512 threads: the atlas offsets of the run's bricks in one load and
+// one barrier, then RUN * 488 voxels in turn (a workgroup per brick launched
+// eight waves for 488 one-voxel threads).  Ends with a barrier, so the next
+// run may reuse the offsets' LDS.
+template <typename AT, int RUN>
+__device__ inline void apron_run(const ApronArgs& a0, uint32_t b0, long long* base) {
+    const int nb = min(RUN, int(uint32_t(a0.n) - b0));
+    for (int i = threadIdx.x; i < nb * 27; i += 512) base[i] = a0.base[u64(b0) * 27 + u64(i)];
+    __syncthreads();
+    const int br = a0.t.brick_res, w = a0.t.brickwid;
+    const int per = 2 * br * br + w * (br * br - w * w);
+    for (int v = threadIdx.x; v < nb * per; v += 512) {
+        const int k = v / per, a = v - k * per;
+        if (base[k * 27 + 13] < 0) continue;
+        apron_voxel<AT>(a0, int(b0) + k, a, base + k * 27);
+    }
+    __syncthreads();
+}
 
 // Several grids: RUN bricks of one grid per workgroup (the apron-run knob:
 // 1, 2, 4 or 8), one after the other, so the grid lookup and the record
@@ -171,16 +199,13 @@ GVDB_MULTI_T(k_apron, typename AT, AT, ApronArgs, 512)
 // voxel per thread).
 template <typename AT, int RUN>
 __global__ __launch_bounds__(512) void k_apron_runs(ApronArgs, MultiMap m) {
+    __shared__ long long base[RUN * 27];
     const uint32_t bi = m.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const int g = multi_grid(m, bi);
     const uint32_t* first = multi_first(m);
     const ApronArgs& a = multi_rec<ApronArgs>(m, g);
     const uint32_t b0 = (bi - first[g]) * uint32_t(RUN);
-    for (int k = 0; k < RUN; k++) {
-        if (b0 + uint32_t(k) >= uint32_t(a.n)) break;
-        if (k) __syncthreads();            // the previous brick is done with its offsets
-        k_apron_body<AT>(a, b0 + uint32_t(k), 0);
-    }
+    if (b0 < uint32_t(a.n)) apron_run<AT, RUN>(a, b0, base);
 }
 
 // Several grids, persistent: a fixed number of workgroups (about as many as
@@ -193,6 +218,7 @@ static const uint32_t APRON_QUEUE_WORKGROUPS = 256 * 4;   // 4 per CU of 256
 template <typename AT>
 __global__ __launch_bounds__(512) void k_apron_queue(ApronArgs, MultiMap m, uint32_t* counter, uint32_t nruns) {
     __shared__ uint32_t s_run;
+    __shared__ long long base[QRUN * 27];
     const uint32_t* first = multi_first(m);
     int g = -1;
     uint32_t lo = 0, hi = 0;
@@ -209,12 +235,7 @@ __global__ __launch_bounds__(512) void k_apron_queue(ApronArgs, MultiMap m, uint
         }
         const ApronArgs& a = multi_rec<ApronArgs>(m, g);
         const uint32_t b0 = (run - lo) * uint32_t(QRUN);
-        for (int k = 0; k < QRUN; k++) {
-            if (b0 + uint32_t(k) >= uint32_t(a.n)) break;
-            __syncthreads();               // the previous brick is done with its offsets
-            k_apron_body<AT>(a, b0 + uint32_t(k), 0);
-        }
-        __syncthreads();
+        if (b0 < uint32_t(a.n)) apron_run<AT, QRUN>(a, b0, base);
     }
 }
 
@@ -228,26 +249,23 @@ static void launch_apron_runs(int run, dim3 grid, hipStream_t s, const ApronArgs
     }
 }
 
-// One grid: RUN consecutive bricks per workgroup, the record by value (a
-// workgroup per brick launches 8 waves for 488 one-voxel threads; a run
-// spreads that launch cost over RUN bricks).
+// One grid: runs of RUN consecutive bricks per workgroup, the record by value.
 template <typename AT, int RUN>
-__global__ __launch_bounds__(512) void k_apron_seq(ApronArgs a0, MultiMap m) {
+__global__ __launch_bounds__(512) void k_apron_flat(ApronArgs a0, MultiMap m) {
+    __shared__ long long base[RUN * 27];
     const uint32_t bi = m.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t b0 = bi * uint32_t(RUN);
-    for (int k = 0; k < RUN; k++) {
-        if (b0 + uint32_t(k) >= uint32_t(a0.n)) break;
-        if (k) __syncthreads();            // the previous brick is done with its offsets
-        k_apron_body<AT>(a0, b0 + uint32_t(k), 0);
-    }
+    if (b0 < uint32_t(a0.n)) apron_run<AT, RUN>(a0, b0, base);
 }
 
 template <typename AT>
 static void launch_apron_seq(int run, dim3 grid, hipStream_t s, const ApronArgs& a0, const MultiMap& m) {
     switch (run) {
-    case 2: hipLaunchKernelGGL((k_apron_seq<AT, 2>), grid, dim3(512), 0, s, a0, m); break;
-    case 4: hipLaunchKernelGGL((k_apron_seq<AT, 4>), grid, dim3(512), 0, s, a0, m); break;
-    case 8: hipLaunchKernelGGL((k_apron_seq<AT, 8>), grid, dim3(512), 0, s, a0, m); break;
+    case 2: hipLaunchKernelGGL((k_apron_flat<AT, 2>), grid, dim3(512), 0, s, a0, m); break;
+    case 4: hipLaunchKernelGGL((k_apron_flat<AT, 4>), grid, dim3(512), 0, s, a0, m); break;
+    case 8: hipLaunchKernelGGL((k_apron_flat<AT, 8>), grid, dim3(512), 0, s, a0, m); break;
+    case 16: hipLaunchKernelGGL((k_apron_flat<AT, 16>), grid, dim3(512), 0, s, a0, m); break;
+    case 32: hipLaunchKernelGGL((k_apron_flat<AT, 32>), grid, dim3(512), 0, s, a0, m); break;
     default: hipLaunchKernelGGL(k_apron<AT>, grid, dim3(512), 0, s, a0, m); break;
     }
 }

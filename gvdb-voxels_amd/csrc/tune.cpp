@@ -18,10 +18,10 @@
 namespace gvdb {
 
 static const int kValues[KNOB_COUNT][KNOB_MAX_VALUES] = {
-    {1, 2, 4}, {1, 2, 4}, {8, 16, 32, 64}, {8, 16, 32, 64}, {128, 256}, {1, 2, 4, 8}, {0, 1}, {0, 1}, {0, 1, 2, 3, 4}, {1, 2, 4, 8}};
-static const int kCount[KNOB_COUNT] = {3, 3, 4, 4, 2, 4, 2, 2, 5, 4};
+    {1, 2, 4}, {1, 2, 4}, {8, 16, 32, 64}, {8, 16, 32, 64}, {128, 256}, {1, 2, 4, 8}, {0, 1}, {0, 1}, {0, 1, 2, 3, 4}, {1, 2, 4, 8, 16, 32}};
+static const int kCount[KNOB_COUNT] = {3, 3, 4, 4, 2, 4, 2, 2, 5, 6};
 // defaults: the values measured fastest on the medium and 64-grid workloads
-static int g_knob[KNOB_COUNT] = {4, 2, 8, 8, 128, 4, 0, 1, 0, 4};
+static int g_knob[KNOB_COUNT] = {4, 2, 8, 8, 128, 4, 0, 1, 0, 8};
 
 int knob(int k) { return (k >= 0 && k < KNOB_COUNT) ? g_knob[k] : 0; }
 
