@@ -489,7 +489,7 @@ def main():
                       design_bytes=ab["gather_design"], design_frac=ab["gather_design"] / (g["gather_ms"] * 1e-3) / 1e9 /
                       HBM_PEAK_GBS),
         roofline_apron=dict(bound="hbm", kernel="apron", achieved=apron_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
-                            frac=apron_gbs / HBM_PEAK_GBS, traffic=profiled_traffic("k_apron<float>"),
+                            frac=apron_gbs / HBM_PEAK_GBS, traffic=profiled_traffic("k_apron_flat<float, 8>"),
                             algorithmic_bytes=ab["apron"], bytes_formula="B*488*8"),
         batched=None,
     )
