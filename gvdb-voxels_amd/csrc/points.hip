@@ -1200,7 +1200,9 @@ __device__ inline float falloff_from_d2(u64 D, int s, float r) {
     const float r2h = r * r;
     const float r2l = __builtin_fmaf(r, r, -r2h);
     const float num = (r2h - hi) + (r2l - lo);
-    return num / (r + __builtin_sqrtf(hi));
+    // the hardware square root and reciprocal (1 ulp each): the quotient
+    // keeps a relative error of a few ulps, as num does, down to the edge
+    return num * __builtin_amdgcn_rcpf(r + __builtin_amdgcn_sqrtf(hi));
 }
 
 // LS = 0: every slice of every candidate, variable loops.  LS = 5: the slices
