@@ -35,12 +35,18 @@ __device__ inline void k_neighbors_body(const NbrArgs& a, uint32_t bid, uint32_t
     int v = -1;
     if (valid) {
         const Node* n = t.nodes[0] + b;
-        if (d == 13) {
+        int dx, dy, dz;
+        dir_of(d, dx, dy, dz);
+        if (a.bt.tab) {
+            // the brick table at the brick's coordinates plus the direction
+            // (d = 13: the brick itself, -1 when the table does not hold it)
+            const int sh = __builtin_ctz(unsigned(t.brickwid));
+            v = tab_leaf(a.bt, (n->mPos[0] >> sh) + dx, (n->mPos[1] >> sh) + dy, (n->mPos[2] >> sh) + dz);
+            if (d == 13 && v != b) v = -1;
+        } else if (d == 13) {
             // the brick itself, or -1 when the walk from the top node does not reach it
             v = pos_leaf(t, n->mPos[0], n->mPos[1], n->mPos[2]) == b ? b : -1;
         } else {
-            int dx, dy, dz;
-            dir_of(d, dx, dy, dz);
             int w = t.brickwid;
             // a voxel centre inside the neighbour box: local coordinate -1, 0 or w
             float px = float(n->mPos[0]) + float(dx < 0 ? -1 : (dx > 0 ? w : 0)) + 0.5f;

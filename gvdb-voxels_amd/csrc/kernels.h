@@ -110,6 +110,7 @@ struct EmitArgs {
     LevelBuild L;
     Node* nodes;
     int has_clist;
+    uint32_t* btab;       // optional (level 0, box up to BRICK_TAB_MAX bits): the brick table
 };
 hipError_t launch_emit_nodes_multi(const EmitArgs* a, int G, Arena* ar, hipStream_t s);
 
@@ -321,6 +322,7 @@ struct BucketArgs {
     float radius;
     u64* len;             // 64 partial counters, summed by launch_sum_parts_multi
     int rule;             // BIN_FLOAT or BIN_FP16
+    BrickTab bt;          // leaf lookup (tab == nullptr: the walk)
 };
 hipError_t launch_sum_parts_multi(const u64* const* parts, u64* const* out, int G, Arena* ar, hipStream_t s);
 hipError_t launch_bucket_points_multi(const BucketArgs* a, int G, Arena* ar, hipStream_t s);
@@ -409,6 +411,7 @@ struct NbrArgs {
     int n;
     uint32_t* mask;       // optional: bit d set when nbr[b*27 + d] >= 0 (written whole)
     long long* base;      // optional: atlas offset of each neighbour's brick (d = 13: own), -1 if none
+    BrickTab bt;          // leaf lookup (tab == nullptr: the walks)
 };
 hipError_t launch_neighbors_multi(const NbrArgs* a, int G, Arena* ar, hipStream_t s);
 

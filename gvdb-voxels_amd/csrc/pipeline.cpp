@@ -215,6 +215,7 @@ int Volume::multi_full_build(Volume* const* vs, int G, const int* n, const float
         u64 bm_off[MAXLEV], rk_off[MAXLEV], tmp_off[2 * MAXLEV];
         uint32_t* masks = nullptr;
         uint8_t* bytes = nullptr;
+        bool btab = false;
     };
     std::vector<Plan> plan(G);
     std::vector<FillArgs> fills;
@@ -359,8 +360,16 @@ int Volume::multi_full_build(Volume* const* vs, int G, const int* n, const float
         r.lev = D;
         ra.push_back(r);
         cf.push_back(FillArgs{v->pool_[1][D].dev.p, v->voxcnt(D), ID_UNDEF64, 1});
+        // the brick table comes with the level-0 emission (one entry per box bit)
+        uint32_t* btab = nullptr;
+        if (pl.dom[0].nbits <= BRICK_TAB_MAX) {
+            XCHK(v, v->btab_.ensure(size_t(pl.dom[0].nbits) * 4));
+            btab = v->btab_.as<uint32_t>();
+            v->btab_dom_ = pl.dom[0];
+        }
+        pl.btab = btab != nullptr;
         for (int L = 0; L < D; L++) {
-            ea.push_back(EmitArgs{LB[L], v->pool_[0][L].dev.as<Node>(), L > 0 ? 1 : 0});
+            ea.push_back(EmitArgs{LB[L], v->pool_[0][L].dev.as<Node>(), L > 0 ? 1 : 0, L == 0 ? btab : nullptr});
             if (L > 0) cf.push_back(FillArgs{v->pool_[1][L].dev.p, v->pool_[1][L].lastEle * v->voxcnt(L), ID_UNDEF64, 1});
         }
         // gvdbLinkBricks walks from the top node; levels above it stay unlinked
@@ -382,6 +391,7 @@ int Volume::multi_full_build(Volume* const* vs, int G, const int* n, const float
         v->host_valid_ = false;
         v->dev_stale_ = false;
         v->topo_gen_++;
+        v->btab_gen_ = plan[g].btab ? v->topo_gen_ : ~0u;
     }
     return GVDB_OK;
 }
@@ -803,7 +813,7 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
                     nf.push_back(FillArgs{v->nmask_.p, pl.numMap, 0, 0});
                     XCHK(v, v->nbase_.ensure(size_t(pl.numMap) * 27 * 8));
                     na.push_back(NbrArgs{pl.t, v->nbr_.as<int>(), int(pl.numMap), v->nmask_.as<uint32_t>(),
-                                         v->nbase_.as<long long>()});
+                                         v->nbase_.as<long long>(), v->brick_tab()});
                     v->nbr_gen_ = v->topo_gen_;
                 }
                 fa.push_back(FillArgs{v->bkt_cnt_.p, pl.numMap * u64(pl.spb), 0, 0});
@@ -841,7 +851,7 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
             u64 nb = pl.numMap * u64(pl.spb);
             ba.push_back(BucketArgs{pl.t, pl.P, scs, pl.spb, {tr[0], tr[1], tr[2]}, v->bkt_key_.as<uint32_t>(),
                                     v->bkt_slot_.as<uint32_t>(), v->bkt_cnt_.as<uint32_t>(), small(g, SM_ORPHANS),
-                                    v->nmask_.as<uint32_t>(), radius[g], v->lenparts_.as<u64>(), rule});
+                                    v->nmask_.as<uint32_t>(), radius[g], v->lenparts_.as<u64>(), rule, v->brick_tab()});
             lp.push_back(v->lenparts_.as<u64>());
             lo.push_back(reinterpret_cast<u64*>(small(g, SM_LEN)));
             sb.push_back(ScanArgs{v->bkt_cnt_.as<uint32_t>(), nullptr, 0, v->bkt_off_.as<uint32_t>(), nb, tmp2(g),
@@ -1143,7 +1153,8 @@ int Volume::multi_apron(Volume* const* vs, int G, int c, float boundval, const E
             XCHK(v, v->nmask_.ensure(size_t(bricks) * 4));
             nf.push_back(FillArgs{v->nmask_.p, u64(bricks), 0, 0});
             XCHK(v, v->nbase_.ensure(size_t(bricks) * 27 * 8));
-            na.push_back(NbrArgs{t, v->nbr_.as<int>(), bricks, v->nmask_.as<uint32_t>(), v->nbase_.as<long long>()});
+            na.push_back(NbrArgs{t, v->nbr_.as<int>(), bricks, v->nmask_.as<uint32_t>(), v->nbase_.as<long long>(),
+                                 v->brick_tab()});
             v->nbr_gen_ = v->topo_gen_;
         }
         const bool faces = v->faces_chan_ == c && v->faces_epoch_ == v->atlas_epoch_ && v->faces_topo_ == v->topo_gen_;

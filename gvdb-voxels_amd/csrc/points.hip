@@ -130,7 +130,11 @@ __global__ __launch_bounds__(256) void k_bin_subcell(Tree t, PointSrc pts, int s
                 // and the local offset goes negative: the entry lands in a
                 // neighbouring subcell, or before the first linked brick's
                 // subcells, which no table holds (skipped here).
+                // A top node that is itself a leaf is returned for every
+                // position, inside it or not: corners past its far side have
+                // no subcell either.
                 int lx = (sx - cx) / scs, ly = (sy - cy) / scs, lz = (sz - cz) / scs;
+                if (lx >= scdim || ly >= scdim || lz >= scdim) continue;
                 const long long sidx = (long long)mapping[leaf] * spb + (lz * scdim + ly) * scdim + lx;
                 if (sidx < 0) continue;
                 const u64 idx = u64(sidx);
@@ -593,25 +597,39 @@ __device__ inline void k_bucket_points_body(const BucketArgs& a, uint32_t bid, u
             // a power of two: divisions are shifts (towards zero for negatives)
             const int ssh = __builtin_ctz(unsigned(scs));
             int ox = (int(w[0]) >> ssh) << ssh, oy = (int(w[1]) >> ssh) << ssh, oz = (int(w[2]) >> ssh) << ssh;
-            int leaf = pos_leaf(t, ox, oy, oz);
-            const Node* n = leaf < 0 ? nullptr : t.nodes[0] + leaf;
-            // a subcell below the found brick's corner (the walk truncates
-            // positions less than a cell below a node into its first cell) has
-            // no bucket: such points go to the push path with the orphans
-            if (n && (ox < n->mPos[0] || oy < n->mPos[1] || oz < n->mPos[2])) n = nullptr;
-            if (!n) {
+            int leaf, org[3];
+            if (a.bt.tab) {
+                // the brick table: the leaf whose box holds the subcell corner
+                const int bsh = __builtin_ctz(unsigned(t.res[0]));
+                leaf = tab_leaf(a.bt, ox >> bsh, oy >> bsh, oz >> bsh);
+                org[0] = (ox >> bsh) << bsh, org[1] = (oy >> bsh) << bsh, org[2] = (oz >> bsh) << bsh;
+            } else {
+                leaf = pos_leaf(t, ox, oy, oz);
+                if (leaf >= 0) {
+                    const Node* n = t.nodes[0] + leaf;
+                    org[0] = n->mPos[0], org[1] = n->mPos[1], org[2] = n->mPos[2];
+                    // a subcell below the found brick's corner (the walk truncates
+                    // positions less than a cell below a node into its first cell),
+                    // or past a top node that is a leaf, has no bucket: such points
+                    // go to the push path with the orphans
+                    const int res0 = t.res[0];
+                    if (ox < org[0] || oy < org[1] || oz < org[2] || ox >= org[0] + res0 || oy >= org[1] + res0 ||
+                        oz >= org[2] + res0)
+                        leaf = -1;
+                }
+            }
+            if (leaf < 0) {
                 atomicAdd(orphans, 1u);
             } else {
                 const int res0 = t.res[0];
                 int sd = res0 >> ssh;
-                int lx = (ox - n->mPos[0]) >> ssh, ly = (oy - n->mPos[1]) >> ssh, lz = (oz - n->mPos[2]) >> ssh;
+                int lx = (ox - org[0]) >> ssh, ly = (oy - org[1]) >> ssh, lz = (oz - org[2]) >> ssh;
                 k = uint32_t(leaf) * uint32_t(spb) + uint32_t((lz * sd + ly) * sd + lx);
                 slot[i] = atomicAdd(&bcnt[k], 1u);
                 if (a.len) {
                     // subcells the point is binned into (CountSubcell's per-axis
                     // range), split per axis into those below, inside and above
                     // this brick; all lie in it or a face/edge/corner neighbour
-                    const int org[3] = {n->mPos[0], n->mPos[1], n->mPos[2]};
                     uint32_t c[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
                     for (int ax = 0; ax < 3; ax++) {
                         const int lo = child_cell(int(w[ax] - a.radius), ssh) * scs,

@@ -296,7 +296,7 @@ hipError_t launch_mark_points_multi(const MarkArgs* a, int G, Arena* ar, hipStre
 
 // Node emission walks domains up to this many bits one thread per bit (dense
 // words would serialise a thread-per-word loop); larger, sparser ones per word.
-static const u64 PER_BIT_MAX = u64(1) << 24;
+static const u64 PER_BIT_MAX = BRICK_TAB_MAX;   // one thread per bit (and the brick table) up to here
 
 // Mark level L+1 from the set bits of level L (parent = child >> shift):
 // one wave per parent bit reads its children's rows (lane = one (x, y) row of
@@ -377,14 +377,15 @@ __device__ inline void k_emit_nodes_body(const EmitArgs& a, uint32_t bid, uint32
             const u64 w = i >> 5;
             const uint32_t bit = 1u << (i & 31), below = bit - 1u;
             const uint32_t av = L.bmA[w];
-            uint32_t idx;
+            uint32_t idx = ~0u;
             if (av & bit) {
                 idx = L.rankA[w] + uint32_t(__popc(av & below));
             } else {
                 const uint32_t e = dil ? (L.bmB[w] & ~av) : 0u;
-                if (!(e & bit)) continue;
-                idx = L.countA + L.rankE[w] + uint32_t(__popc(e & below));
+                if (e & bit) idx = L.countA + L.rankE[w] + uint32_t(__popc(e & below));
             }
+            if (a.btab) a.btab[i] = idx;   // every bit of the box, so no clear is needed
+            if (idx == ~0u) continue;
             int x, y, z;
             dom_coords(L.dom, i, x, y, z);
             write_node(a.nodes + idx, L.lev, x, y, z, L.range, a.has_clist ? Elem(1, L.lev, idx) : ID_UNDEFL);

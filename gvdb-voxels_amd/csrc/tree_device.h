@@ -47,6 +47,17 @@ __device__ inline int pos_leaf(const Tree& t, int px, int py, int pz) {
     }
 }
 
+// The leaf at brick coordinates (bx, by, bz) from a full rebuild's brick
+// table, or -1 outside its box or where no leaf is.  Within that topology it
+// equals pos_leaf at the brick's corner and point_leaf inside it: every leaf
+// is reachable, and the box holds every leaf.
+__device__ inline int tab_leaf(const BrickTab& b, int bx, int by, int bz) {
+    const unsigned x = unsigned(bx - b.lo[0]), y = unsigned(by - b.lo[1]), z = unsigned(bz - b.lo[2]);
+    if (x >= unsigned(b.dim[0]) || y >= unsigned(b.dim[1]) || z >= unsigned(b.dim[2])) return -1;
+    const uint32_t v = b.tab[(u64(x) * unsigned(b.dim[1]) + y) * unsigned(b.dim[2]) + z];
+    return v == ~0u ? -1 : int(v);
+}
+
 // getPosParent (kernels/cuda_gvdb_nodes.cuh:303-348): the node at stopLev
 // on the walk from the top node to `pos`, or -1.  A stopLev at or above the
 // top node's level is never reached, as in the reference.

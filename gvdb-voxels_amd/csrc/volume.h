@@ -293,6 +293,16 @@ private:
     DevBuf bitmaps_, ranks_, masks_, scan_tmp_, flags_, slot_scan_;
     DevBuf kb_keys_, kb_keysB_, kb_cnt_, kb_off_, kb_tmp_, kb_hist_, kb_flags_, kb_pos_, kb_list_;   // keyed builds
     DevBuf incr_marker_;   // per-leaf markers of the incremental build
+    // the brick table of the last full rebuild (BrickTab), valid while
+    // btab_gen_ equals topo_gen_
+    DevBuf btab_;
+    BitDomain btab_dom_{};
+    unsigned btab_gen_ = ~0u;
+    BrickTab brick_tab() const {
+        if (btab_gen_ != topo_gen_ || !btab_.p) return BrickTab{nullptr, {0, 0, 0}, {0, 0, 0}};
+        const BitDomain& d = btab_dom_;
+        return BrickTab{btab_.as<uint32_t>(), {d.lo[0], d.lo[1], d.lo[2]}, {d.dim[0], d.dim[1], d.dim[2]}};
+    }
     DevBuf view_dev_, view_scene_;   // the device view handed to user kernels
     uint32_t* pinned_ = nullptr;   // host staging (256 words)
     Arena arena_;

@@ -893,6 +893,9 @@ static int insertPointsSubcell(Vol& v, int scs, const float* pts, int n, float r
                     if (pid < 0) continue;
                     Node* pn = nodeAt(v, 0, u64(pid));
                     int pin[3] = {(sp[0] - pn->mPos[0]) / scs, (sp[1] - pn->mPos[1]) / scs, (sp[2] - pn->mPos[2]) / scs};
+                    // a top node that is a leaf is found for corners past it too,
+                    // whose offsets index past every subcell table
+                    if (pin[0] >= scdim || pin[1] >= scdim || pin[2] >= scdim) { v.undefined_events++; continue; }
                     int lo = (pin[2] * scdim + pin[1]) * scdim + pin[0];
                     // a corner less than a cell below the brick's (truncated
                     // into its first cell) gives a negative offset: before the
