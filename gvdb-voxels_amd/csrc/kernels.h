@@ -66,7 +66,11 @@ struct BboxArgs {
     PointSrc pts;
     float o[3];
     uint32_t* keys6;      // orderable float keys: min x,y,z then max x,y,z
+    uint32_t* part;       // BBOX_PART_WORDS: six keys per block
+    uint32_t nparts;      // set by the launcher: blocks of the grid
 };
+static const int BBOX_BLOCKS = 2048;                         // blocks per grid at most
+static const int BBOX_PART_WORDS = BBOX_BLOCKS * 6;
 hipError_t launch_bbox_multi(const BboxArgs* a, int G, Arena* ar, hipStream_t s);
 
 struct MarkArgs {

@@ -70,7 +70,9 @@ int Volume::multi_rebuild_topology(Volume* const* vs, int G, const int* n, const
     for (int g = 0; g < G; g++) {
         PointSrc P = vs[g]->pts_;
         P.n = n[g];
-        ba[g] = BboxArgs{P, {orig3[3 * g], orig3[3 * g + 1], orig3[3 * g + 2]}, ex.dsmall + g * SMALL_WORDS + SM_BBOX};
+        XCHK(vs[g], vs[g]->bbox_part_.ensure(size_t(BBOX_PART_WORDS) * 4));
+        ba[g] = BboxArgs{P, {orig3[3 * g], orig3[3 * g + 1], orig3[3 * g + 2]}, ex.dsmall + g * SMALL_WORDS + SM_BBOX,
+                         vs[g]->bbox_part_.as<uint32_t>()};
     }
     XCHK(v0, launch_bbox_multi(ba.data(), G, ex.ar, ex.s));
     int rc = drain(v0, ex, G);
@@ -410,7 +412,8 @@ int Volume::accumulate_topology(int n, float radius, const float orig[3], int mi
     stage_begin(ST_TOPO);
     PointSrc P = pts_;
     P.n = n;
-    BboxArgs ba{P, {orig[0], orig[1], orig[2]}, ex.dsmall + SM_BBOX};
+    XCHK(this, bbox_part_.ensure(size_t(BBOX_PART_WORDS) * 4));
+    BboxArgs ba{P, {orig[0], orig[1], orig[2]}, ex.dsmall + SM_BBOX, bbox_part_.as<uint32_t>()};
     XCHK(this, launch_bbox_multi(&ba, 1, ex.ar, ex.s));
     int rc = drain(this, ex, 1);
     if (rc) return rc;

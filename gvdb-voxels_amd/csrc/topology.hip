@@ -38,7 +38,6 @@ static uint32_t grid_for(u64 items, uint32_t cap) {
 }
 
 // GetBoundingBox's split + Thrust min/max (src/gvdb_volume_gvdb.cpp:5404-5467).
-// keys6 must hold {~0 x3, 0 x3} before the launch.
 __device__ inline void k_bbox_body(const BboxArgs& a, uint32_t bid, uint32_t nblk) {
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     const PointSrc& P = a.pts;
@@ -82,30 +81,66 @@ __device__ inline void k_bbox_body(const BboxArgs& a, uint32_t bid, uint32_t nbl
     if ((threadIdx.x & 63) == 0)
         for (int k = 0; k < 3; k++) { smn[wave][k] = mn[k]; smx[wave][k] = mx[k]; }
     __syncthreads();
+    // each block stores its six keys; k_bbox_reduce combines them (a kernel
+    // boundary orders the stores: no same-address atomics, no fences)
     if (threadIdx.x < 3) {
         int k = threadIdx.x;
         float m0 = smn[0][k], m1 = smx[0][k];
         for (int w = 1; w < 4; w++) { m0 = fminf(m0, smn[w][k]); m1 = fmaxf(m1, smx[w][k]); }
-        atomicMin(&a.keys6[k], float_key(m0));
-        atomicMax(&a.keys6[3 + k], float_key(m1));
+        a.part[bid * 6 + k] = float_key(m0);
+        a.part[bid * 6 + 3 + k] = float_key(m1);
     }
 }
 GVDB_MULTI(k_bbox, BboxArgs, 256)
 
-hipError_t launch_bbox_multi(const BboxArgs* a, int G, Arena* ar, hipStream_t s) {
-    std::vector<FillArgs> f;
-    std::vector<uint32_t> nb(G);
-    for (int g = 0; g < G; g++) {
-        f.push_back(FillArgs{a[g].keys6, 3, 0xFFFFFFFFull, 0});
-        f.push_back(FillArgs{a[g].keys6 + 3, 3, 0ull, 0});
-        nb[g] = grid_for(u64(a[g].pts.n) / 4, 1024);   // each block ends in six same-address atomics
+// One block per grid: the minimum and maximum over the blocks' keys.
+__device__ inline void k_bbox_reduce_body(const BboxArgs& a, uint32_t, uint32_t) {
+    uint32_t k6[6] = {~0u, ~0u, ~0u, 0u, 0u, 0u};
+    for (uint32_t j = threadIdx.x; j < a.nparts; j += 256) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            k6[k] = min(k6[k], a.part[j * 6 + k]);
+            k6[3 + k] = max(k6[3 + k], a.part[j * 6 + 3 + k]);
+        }
     }
-    hipError_t e = launch_fill_multi(f.data(), int(f.size()), ar, s);
-    if (e != hipSuccess) return e;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            k6[k] = min(k6[k], uint32_t(__shfl_xor(int(k6[k]), d, 64)));
+            k6[3 + k] = max(k6[3 + k], uint32_t(__shfl_xor(int(k6[3 + k]), d, 64)));
+        }
+    }
+    __shared__ uint32_t sk[4][6];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; k++) sk[wave][k] = k6[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        uint32_t m = sk[0][k];
+        for (int w = 1; w < 4; w++) m = k < 3 ? min(m, sk[w][k]) : max(m, sk[w][k]);
+        a.keys6[k] = m;
+    }
+}
+GVDB_MULTI(k_bbox_reduce, BboxArgs, 256)
+
+
+hipError_t launch_bbox_multi(const BboxArgs* a, int G, Arena* ar, hipStream_t s) {
+    std::vector<BboxArgs> r(a, a + G);
+    std::vector<uint32_t> nb(G), one(G, 1u);
+    for (int g = 0; g < G; g++) {
+        nb[g] = grid_for(u64(a[g].pts.n) / 4, BBOX_BLOCKS);
+        r[g].nparts = nb[g];
+    }
     MultiLaunch L;
-    e = multi_setup(ar, a, nb.data(), G, s, L);
+    hipError_t e = multi_setup(ar, a, nb.data(), G, s, L);
     if (e != hipSuccess || L.blocks == 0) return e;
     hipLaunchKernelGGL(k_bbox, dim3(L.blocks), dim3(256), 0, s, a[0], L.map);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    e = multi_setup(ar, r.data(), one.data(), G, s, L);
+    if (e != hipSuccess || L.blocks == 0) return e;
+    hipLaunchKernelGGL(k_bbox_reduce, dim3(L.blocks), dim3(256), 0, s, r[0], L.map);
     return hipGetLastError();
 }
 

@@ -144,7 +144,7 @@ Volume::~Volume() {
     DevBuf* bufs[] = {&amap_, &nbr_, &nmask_, &nbase_, &faces_, &lenparts_, &pts_owned_, &clr_owned_, &sc_cnt_, &sc_off_, &sc_pos_, &sc_nid_, &sc_map_, &sc_pnt_, &sc_use_,
                       &bkt_key_, &bkt_slot_, &bkt_cnt_, &bkt_off_, &sorted_, &sc_flags_, &small_, &masks_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev,
                       &incr_marker_, &sc_pnt16_, &view_dev_, &view_scene_, &kb_keys_, &kb_keysB_, &kb_cnt_, &kb_off_,
-                      &kb_tmp_, &kb_hist_, &kb_flags_, &kb_pos_, &kb_list_, &btab_};
+                      &kb_tmp_, &kb_hist_, &kb_flags_, &kb_pos_, &kb_list_, &btab_, &bbox_part_};
     for (DevBuf* b : bufs) b->release();
     for (int s = 0; s < ST_COUNT; s++)
         for (int k = 0; k < 2; k++)
@@ -843,7 +843,7 @@ unsigned long long Volume::device_bytes() const {
                             &sc_use_, &bkt_key_, &bkt_slot_, &bkt_cnt_, &bkt_off_, &sorted_, &sc_flags_, &small_,
                             &masks_, &bitmaps_, &ranks_, &scan_tmp_, &flags_, &slot_scan_, &tf_dev,
                       &incr_marker_, &sc_pnt16_, &view_dev_, &view_scene_, &kb_keys_, &kb_keysB_, &kb_cnt_, &kb_off_,
-                      &kb_tmp_, &kb_hist_, &kb_flags_, &kb_pos_, &kb_list_, &btab_};
+                      &kb_tmp_, &kb_hist_, &kb_flags_, &kb_pos_, &kb_list_, &btab_, &bbox_part_};
     for (const DevBuf* d : bufs) b += d->bytes;
     return b;
 }

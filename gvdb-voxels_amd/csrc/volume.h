@@ -290,6 +290,7 @@ private:
 
     // topology scratch
     DevBuf small_;       // bbox keys, counts
+    DevBuf bbox_part_;   // the bounding box's per-block keys
     DevBuf bitmaps_, ranks_, masks_, scan_tmp_, flags_, slot_scan_;
     DevBuf kb_keys_, kb_keysB_, kb_cnt_, kb_off_, kb_tmp_, kb_hist_, kb_flags_, kb_pos_, kb_list_;   // keyed builds
     DevBuf incr_marker_;   // per-leaf markers of the incremental build
