@@ -639,14 +639,18 @@ int Volume::multi_finish_topology(Volume* const* vs, int G, bool commit, bool bo
                 who.push_back(g);
             }
             for (int a = 0; a < 3; a++) { v->objMin_[a] = mnv[a]; v->objMax_[a] = mxv[a]; }
+            v->bounds_pending_ = false;
         }
         if (!bd.empty()) {
+            // no host wait: each grid's bounds come back behind an event and
+            // are taken when read (the render, the user-kernel view, GetInfo)
             XCHK(vs[0], launch_leaf_bounds_multi(bd.data(), int(bd.size()), ex.ar, ex.s));
-            int rc = drain(vs[0], ex, G);
-            if (rc) return rc;
             for (int g : who) {
-                const int* r = reinterpret_cast<const int*>(ex.hsmall + g * SMALL_WORDS + SM_BOUNDS);
-                for (int a = 0; a < 3; a++) { vs[g]->objMin_[a] = float(r[a]); vs[g]->objMax_[a] = float(r[3 + a]); }
+                Volume* v = vs[g];
+                XCHK(v, hipMemcpyAsync(v->pinned_ + PIN_BOUNDS, ex.dsmall + g * SMALL_WORDS + SM_BOUNDS, 6 * 4,
+                                       hipMemcpyDeviceToHost, ex.s));
+                XCHK(v, hipEventRecord(v->bounds_ev_, ex.s));
+                v->bounds_pending_ = true;
             }
         }
     }
@@ -1237,6 +1241,8 @@ int Volume::multi_render(Volume* const* vs, int G, const int* shading, const int
             rc = v->commit_tf();
             if (rc) FAILG(g, rc);
         }
+        rc = v->resolve_bounds();   // the march starts at the object bounds
+        if (rc) FAILG(g, rc);
         RenderBuf& r = v->rbuf_[b];
         RenderArgs a;
         a.t = v->make_tree();

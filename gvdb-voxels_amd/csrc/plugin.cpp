@@ -25,6 +25,8 @@ static_assert(MAXLEV == GVDB_VIEW_MAXLEV, "device view levels");
 int Volume::upload_view(int c, int rbuf, void** dev) {
     gvdb_device_view hv;
     memset(&hv, 0, sizeof(hv));
+    int rb = resolve_bounds();
+    if (rb) return rb;
     const Tree t = make_tree();
     const Channel& ch = chan_[c];
     hv.atlas = ch.data.p;

@@ -152,6 +152,7 @@ Volume::~Volume() {
     for (int s = 0; s < 4; s++)
         for (int k = 0; k < 2; k++)
             if (kev_[s][k]) (void)hipEventDestroy(kev_[s][k]);
+    if (bounds_ev_) (void)hipEventDestroy(bounds_ev_);
     if (pinned_) (void)hipHostFree(pinned_);
     arena_.release();
     if (handoff_) (void)hipEventDestroy(handoff_);
@@ -176,6 +177,7 @@ int Volume::init() {
     HIPCHK(small_.ensure(SMALL_WORDS * 4 * 4));
     HIPCHK(arena_.init(1 << 16));
     HIPCHK(hipEventCreateWithFlags(&handoff_, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&bounds_ev_, hipEventDisableTiming));
     // the device's copy of the allocation-free apron's scratch array, found
     // once per device here so that entry point never has to look it up
     HIPCHK(noalloc_scratch(device, &noalloc_));
@@ -861,7 +863,18 @@ int Volume::get_pool(int grp, int lev, void* dst, u64* count, u64* stride) {
     return GVDB_OK;
 }
 
+int Volume::resolve_bounds() {
+    if (!bounds_pending_) return GVDB_OK;
+    HIPCHK(hipEventSynchronize(bounds_ev_));
+    const int* r = reinterpret_cast<const int*>(pinned_ + PIN_BOUNDS);
+    for (int a = 0; a < 3; a++) { objMin_[a] = float(r[a]); objMax_[a] = float(r[3 + a]); }
+    bounds_pending_ = false;
+    return GVDB_OK;
+}
+
 int Volume::get_info(int* out2, float* outf) {
+    int rc = resolve_bounds();
+    if (rc) return rc;
     if (out2) { out2[0] = top_lev_; out2[1] = curr_depth_; }
     if (outf)
         for (int a = 0; a < 3; a++) {

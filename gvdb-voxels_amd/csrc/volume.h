@@ -242,6 +242,13 @@ private:
     float xf_inv_[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
     float xf_invrot_[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
     float objMin_[3] = {0, 0, 0}, objMax_[3] = {0, 0, 0};
+    // FinishTopology's device bounds arrive in pinned_[PIN_BOUNDS..+6) after
+    // bounds_ev_; objMin_/objMax_ take them when something reads them
+    // (resolve_bounds: the render, the user-kernel view, GetInfo)
+    static const int PIN_BOUNDS = 192;
+    bool bounds_pending_ = false;
+    hipEvent_t bounds_ev_ = nullptr;
+    int resolve_bounds();
     float epsilon_ = 0.001f;
     int maxiter_ = 256;
 
