@@ -800,6 +800,9 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
     }
     Volume* v0 = vs[0];
     if (G == 1) v0->stage_begin(ST_INSERT);
+    // one grid, not recording: the small words come back behind an event
+    // recorded before the last kernels, not after a wait for the stream
+    const bool early = G == 1 && !ex.cap;
     auto small = [&](int g, int k) { return ex.dsmall + g * SMALL_WORDS + k; };
     auto tmp2 = [&](int g) { return vs[g]->scan_tmp_.as<uint32_t>() + scan_tmp_words(plan[g].numMap) + 64; };
     {
@@ -872,11 +875,22 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         }
         XCHK(v0, launch_bucket_points_multi(ba.data(), int(ba.size()), ex.ar, ex.s));
         XCHK(v0, launch_sum_parts_multi(lp.data(), lo.data(), int(lp.size()), ex.ar, ex.s));
+        if (early) {
+            // every word the host reads below is final here: it comes back
+            // while the bucket scan and scatter still run, and the gather the
+            // caller launches next queues behind them
+            XCHK(v0, hipMemcpyAsync(ex.hsmall, ex.dsmall, SMALL_WORDS * 4, hipMemcpyDeviceToHost, ex.s));
+            XCHK(v0, hipEventRecord(v0->small_ev_, ex.s));
+        }
         XCHK(v0, launch_scan_multi(sb.data(), int(sb.size()), ex.ar, ex.s));
         XCHK(v0, launch_bucket_scatter_multi(sca.data(), int(sca.size()), ex.ar, ex.s));
         XCHK(v0, launch_scan_multi(sc.data(), int(sc.size()), ex.ar, ex.s));
     }
-    int rc = drain(v0, ex, G);
+    int rc = GVDB_OK;
+    if (early)
+        XCHK(v0, hipEventSynchronize(v0->small_ev_));
+    else
+        rc = drain(v0, ex, G);
     if (rc) return rc;
     if (ex.cap) {
         // recording: the entry counts arrive with the replay (graph_synchronize

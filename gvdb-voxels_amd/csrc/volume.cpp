@@ -153,6 +153,7 @@ Volume::~Volume() {
         for (int k = 0; k < 2; k++)
             if (kev_[s][k]) (void)hipEventDestroy(kev_[s][k]);
     if (bounds_ev_) (void)hipEventDestroy(bounds_ev_);
+    if (small_ev_) (void)hipEventDestroy(small_ev_);
     if (pinned_) (void)hipHostFree(pinned_);
     arena_.release();
     if (handoff_) (void)hipEventDestroy(handoff_);
@@ -178,6 +179,7 @@ int Volume::init() {
     HIPCHK(arena_.init(1 << 16));
     HIPCHK(hipEventCreateWithFlags(&handoff_, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&bounds_ev_, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&small_ev_, hipEventDisableTiming));
     // the device's copy of the allocation-free apron's scratch array, found
     // once per device here so that entry point never has to look it up
     HIPCHK(noalloc_scratch(device, &noalloc_));

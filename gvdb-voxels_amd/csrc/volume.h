@@ -248,6 +248,7 @@ private:
     static const int PIN_BOUNDS = 192;
     bool bounds_pending_ = false;
     hipEvent_t bounds_ev_ = nullptr;
+    hipEvent_t small_ev_ = nullptr;   // InsertPointsSubcell's early readback of its small words
     int resolve_bounds();
     float epsilon_ = 0.001f;
     int maxiter_ = 256;
