@@ -809,7 +809,7 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         std::vector<FlagArgs> fl;
         std::vector<ScanArgs> sa;
         std::vector<NbrArgs> na;
-        std::vector<FillArgs> fa, nf;
+        std::vector<FillArgs> fa;
         for (int g = 0; g < G; g++) {
             Volume* v = vs[g];
             Plan& pl = plan[g];
@@ -819,8 +819,8 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
             fa.push_back(FillArgs{v->sc_cnt_.p, pl.numSC, 0, 0});
             if (pl.pull) {
                 if (v->nbr_gen_ != v->topo_gen_) {
+                    // k_neighbors writes every brick's mask whole
                     XCHK(v, v->nmask_.ensure(size_t(pl.numMap) * 4));
-                    nf.push_back(FillArgs{v->nmask_.p, pl.numMap, 0, 0});
                     XCHK(v, v->nbase_.ensure(size_t(pl.numMap) * 27 * 8));
                     na.push_back(NbrArgs{pl.t, v->nbr_.as<int>(), int(pl.numMap), v->nmask_.as<uint32_t>(),
                                          v->nbase_.as<long long>(), v->brick_tab()});
@@ -834,7 +834,6 @@ int Volume::multi_insert(Volume* const* vs, int G, int scs, const int* n, const 
         }
         XCHK(v0, launch_leaf_flags_multi(fl.data(), int(fl.size()), ex.ar, ex.s));
         XCHK(v0, launch_scan_multi(sa.data(), int(sa.size()), ex.ar, ex.s));
-        XCHK(v0, launch_fill_multi(nf.data(), int(nf.size()), ex.ar, ex.s));
         XCHK(v0, launch_neighbors_multi(na.data(), int(na.size()), ex.ar, ex.s));
         XCHK(v0, launch_fill_multi(fa.data(), int(fa.size()), ex.ar, ex.s));
     }
