@@ -113,6 +113,29 @@ static float apron_bound_u8x4(float b) {
     return f;
 }
 
+// Apron voxel a of a padded brick of side br = w + 2: the two z faces
+// (br^2 each), then per interior z layer its ring (br^2 - w^2: the y = 0 and
+// y = br - 1 rows, then x = 0 / x = br - 1 of each row between).  W > 0: the
+// brick width known when compiled (its divisions become multiplies).
+template <int W>
+__device__ inline void apron_decode(int a, int w_rt, int& x, int& y, int& z) {
+    const unsigned w = W > 0 ? unsigned(W) : unsigned(w_rt);
+    const unsigned br = w + 2, face = br * br, ring = br * br - w * w, u = unsigned(a);
+    if (u < 2 * face) {
+        z = u < face ? 0 : int(br) - 1;
+        const unsigned r = u < face ? u : u - face;
+        y = int(r / br);
+        x = int(r - unsigned(y) * br);
+    } else {
+        const unsigned s = u - 2 * face;
+        z = int(s / ring) + 1;
+        const int r = int(s - unsigned(z - 1) * ring);
+        if (r < int(br)) { x = r; y = 0; }
+        else if (r < 2 * int(br)) { x = r - int(br); y = int(br) - 1; }
+        else { const int q = r - 2 * int(br); y = (q >> 1) + 1; x = (q & 1) ? int(br) - 1 : 0; }
+    }
+}
+
 // Apron voxel `a` (0..487) of brick b, given the atlas offsets of the brick
 // and its 26 neighbours (-1: none).  Enumeration: the z = 0 and z = 9 faces
 // (100 each), then the 36-voxel rings of z = 1..8.
@@ -122,22 +145,9 @@ __device__ inline void apron_voxel(const ApronArgs& ap, int b, int a, const long
     AT* atlas = reinterpret_cast<AT*>(ap.atlas);
     const int br = t.brick_res;        // 10
     const int w = t.brickwid;          // 8
-    const int face = br * br;
-    const int ring = br * br - w * w;
     int x, y, z;
-    if (a < 2 * face) {
-        z = a < face ? 0 : br - 1;
-        int r = a % face;
-        y = r / br;
-        x = r - y * br;
-    } else {
-        int s = a - 2 * face;
-        z = s / ring + 1;
-        int r = s - (z - 1) * ring;
-        if (r < br) { x = r; y = 0; }
-        else if (r < 2 * br) { x = r - br; y = br - 1; }
-        else { int q = r - 2 * br; y = q / 2 + 1; x = (q & 1) ? br - 1 : 0; }
-    }
+    if (w == 8) apron_decode<8>(a, w, x, y, z);   // uniform: the usual 8^3 bricks
+    else apron_decode<0>(a, w, x, y, z);
     int dx = x == 0 ? -1 : (x == br - 1 ? 1 : 0);
     int dy = y == 0 ? -1 : (y == br - 1 ? 1 : 0);
     int dz = z == 0 ? -1 : (z == br - 1 ? 1 : 0);
@@ -192,7 +202,7 @@ __device__ inline void apron_run(const ApronArgs& a0, uint32_t b0, long long* ba
     const int br = a0.t.brick_res, w = a0.t.brickwid;
     const int per = 2 * br * br + w * (br * br - w * w);
     for (int v = threadIdx.x; v < nb * per; v += 512) {
-        const int k = v / per, a = v - k * per;
+        const int k = w == 8 ? int(unsigned(v) / 488u) : v / per, a = v - k * per;   // 488 = per for 8^3 bricks
         if (base[k * 27 + 13] < 0) continue;
         apron_voxel<AT>(a0, int(b0) + k, a, base + k * 27);
     }
