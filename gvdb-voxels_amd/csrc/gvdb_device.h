@@ -51,13 +51,12 @@ struct AtlasNode {
 
 __host__ __device__ inline u64 Elem(int grp, int lev, u64 ndx) { return u64(grp) | (u64(lev) << 8) | (ndx << 16); }
 
-// Device view of one grid (the role of the reference's VDBInfo).  Passed to
-// kernels by value.
 // The leaf index of every brick of the level-0 box of a full rebuild, in the
-// box's bit order (index ((x - lo) * dim[1] + y - lo) * dim[2] + z - lo, brick
-// coordinates), ~0u where no leaf is: written by the node emission, so a lookup
-// replaces the walk from the top node while the topology is that build's
-// (every leaf of it is reachable from the top node).  tab == nullptr: walk.
+// box's bit order (index ((x - lo[0]) * dim[1] + (y - lo[1])) * dim[2] +
+// (z - lo[2]), brick coordinates), ~0u where no leaf is: written by the node
+// emission, so a lookup replaces the walk from the top node while the
+// topology is that build's (every leaf of it is reachable from the top
+// node).  tab == nullptr: walk.
 struct BrickTab {
     const uint32_t* tab;
     int lo[3];
@@ -65,6 +64,8 @@ struct BrickTab {
 };
 static const u64 BRICK_TAB_MAX = u64(1) << 24;   // box bricks with a table (the per-bit emission)
 
+// Device view of one grid (the role of the reference's VDBInfo).  Passed to
+// kernels by value.
 struct Tree {
     int levels;
     int top_lev;
