@@ -65,7 +65,20 @@ __device__ inline void k_neighbors_body(const NbrArgs& a, uint32_t bid, uint32_t
         }
     }
     const u64 bal = __ballot(valid && v >= 0);
-    if (a.mask && valid && d == 0) a.mask[b] = uint32_t(bal >> (threadIdx.x & 32)) & 0x7FFFFFFu;
+    if (a.mask && valid && d == 0) {
+        const uint32_t m = uint32_t(bal >> (threadIdx.x & 32)) & 0x7FFFFFFu;
+        a.mask[b] = m;
+        if (a.bt.tab) {
+            // beside the brick's index in the table (its own entry: v is b here
+            // when the table holds the brick), for the insert's one-read lookup
+            const Node* n = t.nodes[0] + b;
+            const int sh = __builtin_ctz(unsigned(t.brickwid));
+            const unsigned x = unsigned((n->mPos[0] >> sh) - a.bt.lo[0]), y = unsigned((n->mPos[1] >> sh) - a.bt.lo[1]),
+                           z = unsigned((n->mPos[2] >> sh) - a.bt.lo[2]);
+            if (x < unsigned(a.bt.dim[0]) && y < unsigned(a.bt.dim[1]) && z < unsigned(a.bt.dim[2]))
+                reinterpret_cast<uint32_t*>(const_cast<u64*>(a.bt.tab))[2 * tab_index(a.bt, x, y, z) + 1] = m;
+        }
+    }
 }
 
 GVDB_MULTI(k_neighbors, NbrArgs, 256)

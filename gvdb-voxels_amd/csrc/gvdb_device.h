@@ -56,9 +56,11 @@ __host__ __device__ inline u64 Elem(int grp, int lev, u64 ndx) { return u64(grp)
 // (z - lo[2]), brick coordinates), ~0u where no leaf is: written by the node
 // emission, so a lookup replaces the walk from the top node while the
 // topology is that build's (every leaf of it is reachable from the top
-// node).  tab == nullptr: walk.
+// node).  The high word of an entry holds the leaf's 27-bit neighbour mask
+// once the neighbour pass has run on that topology (0 before).  tab ==
+// nullptr: walk.
 struct BrickTab {
-    const uint32_t* tab;
+    const u64* tab;
     int lo[3];
     int dim[3];
 };

@@ -114,7 +114,7 @@ struct EmitArgs {
     LevelBuild L;
     Node* nodes;
     int has_clist;
-    uint32_t* btab;       // optional (level 0, box up to BRICK_TAB_MAX bits): the brick table
+    u64* btab;            // optional (level 0, box up to BRICK_TAB_MAX bits): the brick table
 };
 hipError_t launch_emit_nodes_multi(const EmitArgs* a, int G, Arena* ar, hipStream_t s);
 

@@ -363,10 +363,10 @@ int Volume::multi_full_build(Volume* const* vs, int G, const int* n, const float
         ra.push_back(r);
         cf.push_back(FillArgs{v->pool_[1][D].dev.p, v->voxcnt(D), ID_UNDEF64, 1});
         // the brick table comes with the level-0 emission (one entry per box bit)
-        uint32_t* btab = nullptr;
+        u64* btab = nullptr;
         if (pl.dom[0].nbits <= BRICK_TAB_MAX) {
-            XCHK(v, v->btab_.ensure(size_t(pl.dom[0].nbits) * 4));
-            btab = v->btab_.as<uint32_t>();
+            XCHK(v, v->btab_.ensure(size_t(pl.dom[0].nbits) * 8));
+            btab = v->btab_.as<u64>();
             v->btab_dom_ = pl.dom[0];
         }
         pl.btab = btab != nullptr;

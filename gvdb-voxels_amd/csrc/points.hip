@@ -598,10 +598,19 @@ __device__ inline void k_bucket_points_body(const BucketArgs& a, uint32_t bid, u
             const int ssh = __builtin_ctz(unsigned(scs));
             int ox = (int(w[0]) >> ssh) << ssh, oy = (int(w[1]) >> ssh) << ssh, oz = (int(w[2]) >> ssh) << ssh;
             int leaf, org[3];
+            uint32_t tmask = 0;
             if (a.bt.tab) {
-                // the brick table: the leaf whose box holds the subcell corner
+                // the brick table: the leaf whose box holds the subcell corner,
+                // and its neighbour mask, in one read
                 const int bsh = __builtin_ctz(unsigned(t.res[0]));
-                leaf = tab_leaf(a.bt, ox >> bsh, oy >> bsh, oz >> bsh);
+                const unsigned x = unsigned((ox >> bsh) - a.bt.lo[0]), y = unsigned((oy >> bsh) - a.bt.lo[1]),
+                               z = unsigned((oz >> bsh) - a.bt.lo[2]);
+                leaf = -1;
+                if (x < unsigned(a.bt.dim[0]) && y < unsigned(a.bt.dim[1]) && z < unsigned(a.bt.dim[2])) {
+                    const u64 e = a.bt.tab[tab_index(a.bt, x, y, z)];
+                    if (uint32_t(e) != ~0u) leaf = int(uint32_t(e));
+                    tmask = uint32_t(e >> 32);
+                }
                 org[0] = (ox >> bsh) << bsh, org[1] = (oy >> bsh) << bsh, org[2] = (oz >> bsh) << bsh;
             } else {
                 leaf = pos_leaf(t, ox, oy, oz);
@@ -636,7 +645,7 @@ __device__ inline void k_bucket_points_body(const BucketArgs& a, uint32_t bid, u
                                   hi = child_cell(int(w[ax] + 1 + a.radius), ssh) * scs;
                         for (int sv = lo; sv <= hi; sv += scs) c[ax][sv < org[ax] ? 0 : (sv >= org[ax] + res0 ? 2 : 1)]++;
                     }
-                    const uint32_t m = a.nmask[leaf];
+                    const uint32_t m = a.bt.tab ? tmask : a.nmask[leaf];
 #pragma unroll
                     for (int d = 0; d < 27; d++)
                         if ((m >> d) & 1u) binned += c[0][d % 3] * c[1][(d / 3) % 3] * c[2][d / 9];

@@ -419,7 +419,7 @@ __device__ inline void k_emit_nodes_body(const EmitArgs& a, uint32_t bid, uint32
                 const uint32_t e = dil ? (L.bmB[w] & ~av) : 0u;
                 if (e & bit) idx = L.countA + L.rankE[w] + uint32_t(__popc(e & below));
             }
-            if (a.btab) a.btab[i] = idx;   // every bit of the box, so no clear is needed
+            if (a.btab) a.btab[i] = idx;   // every bit of the box (mask word 0), so no clear is needed
             if (idx == ~0u) continue;
             int x, y, z;
             dom_coords(L.dom, i, x, y, z);

@@ -51,10 +51,13 @@ __device__ inline int pos_leaf(const Tree& t, int px, int py, int pz) {
 // table, or -1 outside its box or where no leaf is.  Within that topology it
 // equals pos_leaf at the brick's corner and point_leaf inside it: every leaf
 // is reachable, and the box holds every leaf.
+__device__ inline u64 tab_index(const BrickTab& b, unsigned x, unsigned y, unsigned z) {
+    return (u64(x) * unsigned(b.dim[1]) + y) * unsigned(b.dim[2]) + z;
+}
 __device__ inline int tab_leaf(const BrickTab& b, int bx, int by, int bz) {
     const unsigned x = unsigned(bx - b.lo[0]), y = unsigned(by - b.lo[1]), z = unsigned(bz - b.lo[2]);
     if (x >= unsigned(b.dim[0]) || y >= unsigned(b.dim[1]) || z >= unsigned(b.dim[2])) return -1;
-    const uint32_t v = b.tab[(u64(x) * unsigned(b.dim[1]) + y) * unsigned(b.dim[2]) + z];
+    const uint32_t v = uint32_t(b.tab[tab_index(b, x, y, z)]);
     return v == ~0u ? -1 : int(v);
 }
 

@@ -310,7 +310,7 @@ private:
     BrickTab brick_tab() const {
         if (btab_gen_ != topo_gen_ || !btab_.p) return BrickTab{nullptr, {0, 0, 0}, {0, 0, 0}};
         const BitDomain& d = btab_dom_;
-        return BrickTab{btab_.as<uint32_t>(), {d.lo[0], d.lo[1], d.lo[2]}, {d.dim[0], d.dim[1], d.dim[2]}};
+        return BrickTab{btab_.as<u64>(), {d.lo[0], d.lo[1], d.lo[2]}, {d.dim[0], d.dim[1], d.dim[2]}};
     }
     DevBuf view_dev_, view_scene_;   // the device view handed to user kernels
     uint32_t* pinned_ = nullptr;   // host staging (256 words)
