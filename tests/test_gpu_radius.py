@@ -103,3 +103,43 @@ def test_inserts_reaching_below_the_root_corner(gpu, radius):
     lb = H.subcell_lists(os_["cnt"], os_["off"], os_["pnt"])
     assert all(np.array_equal(x, y) for x, y in zip(la, lb))
     v.close()
+
+
+def test_insert_past_a_single_leaf_top_node(gpu):
+    """A host-built topology of one leaf (every topology point in the brick at
+    (200, 200, 200)): the top node is that leaf, and the walk from it returns
+    it for every position.  Inserted points around it reach subcells below and
+    past the brick; those past its far side have no subcell table entry
+    (skipped; the oracle counts them as undefined), and the tables equal the
+    oracle's."""
+    import oracle as O
+    g = np.random.default_rng(12)
+    topo = (200.5 + 7.0 * g.random((50, 3))).astype(np.float32)
+    extra = (190.0 + 30.0 * g.random((400, 3))).astype(np.float32)
+    pts = np.concatenate([topo, extra])
+    v = gpu.VolumeGVDB(0)
+    v.Configure(*H.TREE)
+    v.SetChannelDefault(16, 16, 1)
+    v.AddChannel(0, gpu.T_FLOAT, 1)
+    v.SetPoints(topo)
+    v.RebuildTopologyCPU(topo)
+    v.FinishTopology(True, True)
+    assert v.info()["top_lev"] == 0
+    v.UpdateAtlas()
+    v.SetPoints(pts)
+    n = v.InsertPointsSubcell(4, len(pts), 1.0, H.ZERO3)
+    o = O.Oracle()
+    assert o.rebuild_topology_cpu(topo) == 0
+    o.finish_topology()
+    assert o.top_lev() == 0
+    o.update_atlas()
+    rc, m = o.insert_points_subcell(pts, 1.0, 4)
+    assert rc == 0 and n == m
+    assert o.undefined_events() > 0
+    gs, os_ = v.subcells(), o.subcells(m)
+    assert np.array_equal(gs["cnt"].astype(np.int64), os_["cnt"].astype(np.int64))
+    assert np.array_equal(gs["off"].astype(np.int64), os_["off"].astype(np.int64))
+    la = H.subcell_lists(gs["cnt"], gs["off"], gs["pnt"])
+    lb = H.subcell_lists(os_["cnt"], os_["off"], os_["pnt"])
+    assert all(np.array_equal(x, y) for x, y in zip(la, lb))
+    v.close()
