@@ -76,7 +76,7 @@ __device__ inline void k_neighbors_body(const NbrArgs& a, uint32_t bid, uint32_t
             const unsigned x = unsigned((n->mPos[0] >> sh) - a.bt.lo[0]), y = unsigned((n->mPos[1] >> sh) - a.bt.lo[1]),
                            z = unsigned((n->mPos[2] >> sh) - a.bt.lo[2]);
             if (x < unsigned(a.bt.dim[0]) && y < unsigned(a.bt.dim[1]) && z < unsigned(a.bt.dim[2]))
-                reinterpret_cast<uint32_t*>(const_cast<u64*>(a.bt.tab))[2 * tab_index(a.bt, x, y, z) + 1] = m;
+                reinterpret_cast<uint32_t*>(a.bt.tab)[2 * tab_index(a.bt, x, y, z) + 1] = m;
         }
     }
 }

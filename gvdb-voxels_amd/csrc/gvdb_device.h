@@ -60,7 +60,7 @@ __host__ __device__ inline u64 Elem(int grp, int lev, u64 ndx) { return u64(grp)
 // once the neighbour pass has run on that topology (0 before).  tab ==
 // nullptr: walk.
 struct BrickTab {
-    const u64* tab;
+    u64* tab;   // written by the emission and the neighbour pass, read by lookups
     int lo[3];
     int dim[3];
 };
