@@ -54,3 +54,48 @@ def test_random_scene_matches_oracle(gpu, seed):
     _, rel, zeros = H.density_errors(a[mask], b[mask])
     assert zeros == 0 and rel <= REL_TOL, (seed, rel, zeros)
     v.close()
+
+
+@pytest.mark.parametrize("radius,scs", [(1.5, 2), (0.5, 4)])
+def test_4cubed_bricks_match_oracle(gpu, radius, scs):
+    """Leaf bricks of 4^3 voxels (log2 dims 2, 3, 3, 4, 5): the bucketed
+    insert with two subcells per axis or one, the general gather over the
+    subcell tables, and the apron's enumeration for a brick width known only
+    at run time (6^3 padded bricks); everything against the oracle."""
+    import oracle as O
+    pts, _, _ = _scene(0)
+    dims = (2, 3, 3, 4, 5)
+    v = gpu.VolumeGVDB(0)
+    v.Configure(*reversed(dims))
+    v.SetChannelDefault(16, 16, 1)
+    v.AddChannel(0, gpu.T_FLOAT, 1)
+    v.SetPoints(pts)
+    v.RequestFullRebuild(True)
+    v.RebuildTopology(len(pts), radius, H.ZERO3)
+    v.FinishTopology(False, True)
+    v.UpdateAtlas()
+    v.ClearChannel(0)
+    n = v.InsertPointsSubcell(scs, len(pts), radius, H.ZERO3)
+    v.GatherDensity(scs, len(pts), radius, H.ZERO3, n, 0, 0, False)
+    v.UpdateApron(0, 0.0)
+    v.synchronize()
+    o = O.Oracle(log2dims=dims)
+    assert o.rebuild_topology(pts, radius) == 0
+    o.finish_topology()
+    assert o.update_atlas() == 0
+    o.clear_channel()
+    rc, m = o.insert_points_subcell(pts, radius, scs)
+    assert rc == 0 and n == m
+    o.gather_density(radius, scs, 64)
+    o.update_apron(0.0, 64)
+    H.assert_pools_equal(v, o)
+    gs, os_ = v.subcells(), o.subcells(m)
+    assert np.array_equal(gs["cnt"].astype(np.int64), os_["cnt"].astype(np.int64))
+    la = H.subcell_lists(gs["cnt"], gs["off"], gs["pnt"])
+    lb = H.subcell_lists(os_["cnt"], os_["off"], os_["pnt"])
+    assert all(np.array_equal(x, y) for x, y in zip(la, lb))
+    a, b = v.atlas(), o.atlas(64)
+    mask = H.used_brick_mask(b.shape, o.atlas_map(), o.atlas_info()["cnt"], br=6)
+    _, rel, zeros = H.density_errors(a[mask], b[mask])
+    assert zeros == 0 and rel <= REL_TOL, (rel, zeros)
+    v.close()
